@@ -1,0 +1,152 @@
+"""GraphCut: tiled push-relabel max-flow / min-cut on a 4-connected grid.
+
+Drop-in for the reference's GraphCut (GraphCut.pyx:29-437) with int32
+capacities instead of an RGBA image.  Reference -> here:
+
+  GraphCut(context, devices, dImg, lamda)  GraphCut(context, devices, dim, tile=(32,32))
+      dImg / lamda built the pairwise       upload(source, sink, right, down) or
+      weights on the device                 upload_pairwise(right, down)
+  cut(dExcess, bfs=5) -> None               cut(dExcess=None, bfs=None) -> flow value
+      dExcess = source - sink, mutated      dExcess (int32 [H][W], source - sink) is
+                                            optional and is not mutated
+  dHeight (device buffer)                   dHeight: heights as numpy [PH][PW]
+  (no labels)                               labels(): uint8 [H][W], 1 = source side
+                                            (the minimal source set)
+
+Any width/height is accepted (the reference requires multiples of 32,
+GraphCut.pyx:85-86); the grid is padded to whole tiles internally.
+"""
+import ctypes
+
+import numpy as np
+
+from gcmi_lib import DeviceBuffer, check, host_ptr, lib, ptr
+
+TILE_DEFAULT = (32, 32)
+BFS_DEFAULT = 1       # push rounds per global relabel
+SWEEPS_DEFAULT = 16   # in-tile sweeps per tile residency
+
+STAT_NAMES = ("rounds", "residencies", "global_relabels", "bfs_passes",
+              "flood_passes", "host_syncs", "alg_bytes", "tiles")
+
+
+def _i32(a, shape):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    if a.shape != shape:
+        raise ValueError("expected shape %s, got %s" % (shape, a.shape))
+    return a
+
+
+class GraphCut:
+    lWorksize = (16, 16)
+
+    def __init__(self, context=None, devices=None, dim=None, tile=TILE_DEFAULT,
+                 bfs=BFS_DEFAULT, sweeps=SWEEPS_DEFAULT):
+        if dim is None:
+            raise ValueError("dim=(width, height) is required")
+        if hasattr(dim, "dim"):  # a Buffer2D, as the reference's dImg
+            dim = dim.dim
+        self.dim = (int(dim[0]), int(dim[1]))
+        if isinstance(tile, int):
+            tile = (tile, tile)
+        if tile[0] != tile[1]:
+            raise ValueError("tiles are square (16, 32 or 64)")
+        self.tile = (int(tile[0]), int(tile[1]))
+        self.bfs = int(bfs)
+        self.sweeps = int(sweeps)
+        h = ctypes.c_void_p()
+        check(lib().gcmi_cut_create(self.dim[0], self.dim[1], self.tile[0], ctypes.byref(h)))
+        self._h = h.value
+        self._right = None
+        self._down = None
+        self.flow = None
+
+    @property
+    def shape(self):
+        return (self.dim[1], self.dim[0])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gcmi_cut_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- input
+    def upload(self, source, sink, right, down):
+        """int32 [H][W] capacities: source (s->p), sink (p->t), right
+        (w(p, p+1)), down (w(p, p+W)); copied to the device."""
+        s = self.shape
+        a = [_i32(x, s) for x in (source, sink, right, down)]
+        check(lib().gcmi_cut_upload(self._h, *[host_ptr(x) for x in a]))
+        self._right, self._down = a[2], a[3]
+
+    def upload_pairwise(self, right, down):
+        """Keep pairwise weights for later cut(dExcess) calls."""
+        s = self.shape
+        self._right, self._down = _i32(right, s), _i32(down, s)
+
+    def upload_device(self, d_source, d_sink, d_right, d_down):
+        check(lib().gcmi_cut_upload_device(self._h, ptr(d_source), ptr(d_sink), ptr(d_right),
+                                           ptr(d_down)))
+
+    # ------------------------------------------------------------------ cut
+    def cut(self, dExcess=None, bfs=None, sweeps=None, max_rounds=0):
+        """Run push-relabel to termination; returns the max-flow value.
+
+        dExcess (optional): int32 [H][W] signed excess, source - sink as in
+        test_GraphCut.py:44-47; it sets source = max(e, 0), sink = max(-e, 0)
+        with the pairwise weights of the last upload / upload_pairwise."""
+        if dExcess is not None:
+            if self._right is None:
+                raise ValueError("upload_pairwise() before cut(dExcess)")
+            e = np.asarray(dExcess)
+            if hasattr(dExcess, "get"):
+                e = dExcess.get()
+            e = _i32(e, self.shape)
+            self.upload(np.maximum(e, 0), np.maximum(-e.astype(np.int64), 0).astype(np.int32),
+                        self._right, self._down)
+        bfs = self.bfs if bfs is None else int(bfs)
+        sweeps = self.sweeps if sweeps is None else int(sweeps)
+        flow = ctypes.c_int64()
+        check(lib().gcmi_cut_run(self._h, sweeps, bfs, int(max_rounds), ctypes.byref(flow)))
+        self.flow = int(flow.value)
+        return self.flow
+
+    def labels(self):
+        """uint8 [H][W]: 1 for the minimal source (foreground) set."""
+        out = np.empty(self.shape, np.uint8)
+        check(lib().gcmi_cut_labels(self._h, host_ptr(out)))
+        return out
+
+    def labels_device(self, d_out):
+        check(lib().gcmi_cut_labels_device(self._h, ptr(d_out)))
+
+    def stats(self):
+        a = (ctypes.c_int64 * len(STAT_NAMES))()
+        check(lib().gcmi_cut_stats(self._h, a, len(STAT_NAMES)))
+        return dict(zip(STAT_NAMES, [int(v) for v in a]))
+
+    @property
+    def dHeight(self):
+        """Heights (BFS distances) of the padded grid, [PH][PW] int32."""
+        g = (ctypes.c_int * 3)()
+        check(lib().gcmi_cut_geometry(self._h, g))
+        out = np.empty((g[1], g[0]), np.int32)
+        check(lib().gcmi_cut_heights(self._h, host_ptr(out)))
+        return out
+
+    @property
+    def maxHeight(self):
+        g = (ctypes.c_int * 3)()
+        check(lib().gcmi_cut_geometry(self._h, g))
+        return g[0] * g[1]
+
+    def reset(self):
+        """Kept for API parity (GraphCut.pyx:420-421): every cut() starts
+        from the uploaded capacities, so there is nothing to reset."""
+        return None

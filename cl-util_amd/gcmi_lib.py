@@ -1,0 +1,183 @@
+"""ctypes binding of libgcmi.so and device-buffer plumbing.
+
+The library is the only compute path: there is no CPU fallback.  If the
+shared object is missing, importing the drop-in classes raises
+GcmiUnavailable; if no GPU is visible, every compute call raises
+GcmiError(ENODEVICE).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgcmi.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "gcmi.h")
+
+OK = 0
+EINVAL = -1
+ENOMEM = -2
+ENODEVICE = -3
+ENOTCONVERGED = -4
+EHIP = -5
+EBUDGET = -6
+EBUSY = -7
+
+_NAMES = {EINVAL: "EINVAL", ENOMEM: "ENOMEM", ENODEVICE: "ENODEVICE",
+          ENOTCONVERGED: "ENOTCONVERGED", EHIP: "EHIP", EBUDGET: "EBUDGET",
+          EBUSY: "EBUSY"}
+
+
+class GcmiUnavailable(ImportError):
+    """libgcmi.so is not built."""
+
+
+class GcmiError(RuntimeError):
+    def __init__(self, code, message):
+        self.code = code
+        super().__init__("%s (%d): %s" % (_NAMES.get(code, "?"), code, message))
+
+
+class NotConverged(GcmiError):
+    pass
+
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_vp = ctypes.c_void_p
+_int = ctypes.c_int
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_sz = ctypes.c_size_t
+
+# name -> argtypes; every function returns int unless listed in _RESTYPE
+SIGNATURES = {
+    "gcmi_last_error": [],
+    "gcmi_version": [],
+    "gcmi_device_count": [ctypes.POINTER(_int)],
+    "gcmi_malloc": [ctypes.POINTER(_vp), _sz],
+    "gcmi_free": [_vp],
+    "gcmi_memcpy_h2d": [_vp, _vp, _sz],
+    "gcmi_memcpy_d2h": [_vp, _vp, _sz],
+    "gcmi_memcpy_d2d": [_vp, _vp, _sz],
+    "gcmi_memset_i32": [_vp, _i32, _sz, _vp],
+    "gcmi_stream_sync": [_vp],
+    "gcmi_prefix_sum": [_vp, _vp, _int, _vp],
+    "gcmi_stream_compact": [_vp, _vp, _vp, _int, _vp],
+    "gcmi_tiles_fill": [_vp, _int, _i32, _vp],
+    "gcmi_tiles_flag": [_vp, _vp, _int, _int, _i32, _vp],
+    "gcmi_tiles_flag_logical": [_vp, _vp, _vp, _int, _int, _int, _i32, _i32, _int, _vp],
+    "gcmi_tiles_increment_logical": [_vp, _vp, _int, _int, _int, _i32, _i32, _int, _i32, _vp],
+    "gcmi_tiles_build": [_vp, _int, _int, _i32, _vp, _vp, _vp],
+    "gcmi_tiles_build_logical": [_vp, _vp, _int, _int, _int, _i32, _i32, _int, _vp, _vp, _vp],
+    "gcmi_cut_create": [_int, _int, _int, ctypes.POINTER(_vp)],
+    "gcmi_cut_destroy": [_vp],
+    "gcmi_cut_upload": [_vp, _vp, _vp, _vp, _vp],
+    "gcmi_cut_upload_device": [_vp, _vp, _vp, _vp, _vp],
+    "gcmi_cut_run": [_vp, _int, _int, _i64, _i64p],
+    "gcmi_cut_labels": [_vp, _vp],
+    "gcmi_cut_labels_device": [_vp, _vp],
+    "gcmi_cut_stats": [_vp, _i64p, _int],
+    "gcmi_cut_heights": [_vp, _vp],
+    "gcmi_cut_geometry": [_vp, ctypes.POINTER(_int)],
+}
+_RESTYPE = {"gcmi_last_error": ctypes.c_char_p, "gcmi_version": ctypes.c_char_p}
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises GcmiUnavailable if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GcmiUnavailable(
+                "libgcmi.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.argtypes = args
+            fn.restype = _RESTYPE.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def last_error():
+    msg = lib().gcmi_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc):
+    if rc != OK:
+        cls = NotConverged if rc == ENOTCONVERGED else GcmiError
+        raise cls(rc, last_error())
+    return rc
+
+
+def device_count():
+    n = ctypes.c_int()
+    check(lib().gcmi_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def ptr(obj):
+    """Device address of a DeviceBuffer, a torch tensor or an int."""
+    if obj is None:
+        return None
+    if isinstance(obj, int):
+        return obj
+    if hasattr(obj, "ptr"):
+        return obj.ptr
+    if hasattr(obj, "data_ptr"):
+        return obj.data_ptr()
+    raise TypeError("not a device buffer: %r" % type(obj))
+
+
+def host_ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class DeviceBuffer:
+    """Owned device allocation (the role cl.Buffer plays in the reference)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(lib().gcmi_malloc(ctypes.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    @property
+    def size(self):
+        return self.nbytes
+
+    def upload(self, array):
+        a = np.ascontiguousarray(array)
+        if a.nbytes > self.nbytes:
+            raise ValueError("host array larger than buffer")
+        check(lib().gcmi_memcpy_h2d(self.ptr, host_ptr(a), a.nbytes))
+        return self
+
+    def download(self, dtype, count=None):
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = self.nbytes // dtype.itemsize
+        out = np.empty(count, dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("readback larger than buffer")
+        check(lib().gcmi_memcpy_d2h(host_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            if _lib is not None:
+                _lib.gcmi_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

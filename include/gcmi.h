@@ -1,0 +1,172 @@
+/*
+ * gcmi.h -- C ABI of libgcmi.so: the tiled push-relabel grid cut and its
+ * tile-list primitives for AMD Instinct MI355X (gfx950).
+ *
+ * Conventions
+ *   - Every function returns an int status: GCMI_OK (0) or a negative code;
+ *     gcmi_last_error() gives a thread-local message for the last failure.
+ *   - "d_" pointers are device pointers (from gcmi_malloc or any HIP
+ *     allocation on the current device); "h_" pointers are host pointers,
+ *     borrowed for the duration of the call only.
+ *   - `stream` arguments are hipStream_t passed as void*; NULL means the
+ *     library's own stream for the current device.  Calls that return a
+ *     length or a value to the host synchronise that stream.
+ *   - Handles (gcmi_cut, gcmi_batch, gcmi_server) own their device memory and
+ *     one HIP stream each.  A handle is not thread-safe; distinct handles are
+ *     independent.
+ *
+ * Reference interfaces replaced (file:line in rheiland/cl-util):
+ *   PrefixSum.scan              PrefixSum.pyx:83-129, prefixsum.cl:81-212
+ *   StreamCompact.compact       StreamCompact.pyx:66-82, streamcompact.cl:1-14
+ *   IncrementalTileList kernels incremental_tile_list.cl:13-154,
+ *                               host logic IncrementalTileList.pyx:100-189
+ *   GraphCut                    GraphCut.pyx:80-418, graphcut.cl:20-1016
+ */
+#ifndef GCMI_H
+#define GCMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define GCMI_OK 0
+#define GCMI_EINVAL (-1)        /* bad argument                              */
+#define GCMI_ENOMEM (-2)        /* device or host allocation failed          */
+#define GCMI_ENODEVICE (-3)     /* no usable HIP device                      */
+#define GCMI_ENOTCONVERGED (-4) /* cut hit its round cap before terminating  */
+#define GCMI_EHIP (-5)          /* a HIP runtime call failed                 */
+#define GCMI_EBUDGET (-6)       /* request exceeds the configured HBM budget */
+#define GCMI_EBUSY (-7)         /* no free slot / request still in flight    */
+
+/* Operators and logicals of IncrementalTileList (IncrementalTileList.pyx:26-35). */
+#define GCMI_OP_EQUAL 0
+#define GCMI_OP_GT 1
+#define GCMI_OP_LT 2
+#define GCMI_OP_GTE 3
+#define GCMI_OP_LTE 4
+#define GCMI_LOGICAL_AND 0
+#define GCMI_LOGICAL_OR 1
+
+/* Thread-local description of the last error (empty string if none). */
+const char *gcmi_last_error(void);
+/* Library version string. */
+const char *gcmi_version(void);
+/* Number of HIP devices visible (0 without a GPU; returns GCMI_OK). */
+int gcmi_device_count(int *out);
+
+/* ------------------------------------------------- device memory plumbing
+ * The reference allocates through pyopencl (cl.Buffer, Buffer2D.py:8-34);
+ * these are the equivalents the Python layer uses. */
+int gcmi_malloc(void **d_out, size_t bytes);
+int gcmi_free(void *d_ptr);
+int gcmi_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes);
+int gcmi_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes);
+int gcmi_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes);
+int gcmi_memset_i32(int32_t *d_dst, int32_t value, size_t count, void *stream);
+int gcmi_stream_sync(void *stream);
+
+/* ------------------------------------------------------------- primitives */
+
+/* PrefixSum.scan(dArray, dTotal, length) -- PrefixSum.pyx:83-129.
+ * In-place exclusive scan of d_array[0..n); the sum of all n inputs is
+ * written to d_total[0] (int32 arithmetic, wrapping as the reference's
+ * OpenCL ints do).  n may be 0 (total = 0). */
+int gcmi_prefix_sum(int32_t *d_array, int32_t *d_total, int n, void *stream);
+
+/* StreamCompact.compact(dFlags, dList, dLength, length) --
+ * StreamCompact.pyx:66-82.  d_list[0..k) = ascending indices i < n with
+ * d_flags[i] != 0; d_length[0] = k.  d_list needs room for n entries.
+ * d_flags is not modified (the reference's in-place scatter race,
+ * streamcompact.cl:13, does not exist here). */
+int gcmi_stream_compact(const int32_t *d_flags, int32_t *d_list,
+                        int32_t *d_length, int n, void *stream);
+
+/* init_incremental (incremental_tile_list.cl:27-38): d_tiles[i] = value. */
+int gcmi_tiles_fill(int32_t *d_tiles, int n, int32_t value, void *stream);
+
+/* flag (incremental_tile_list.cl:40-64): d_flags[i] = (d_tiles[i] OP operand). */
+int gcmi_tiles_flag(const int32_t *d_tiles, int32_t *d_flags, int n, int op,
+                    int32_t operand, void *stream);
+
+/* flag_logical (incremental_tile_list.cl:66-108):
+ * d_flags[i] = (t1[i] OP1 a) LOGICAL (t2[i] OP2 b). */
+int gcmi_tiles_flag_logical(const int32_t *d_t1, const int32_t *d_t2,
+                            int32_t *d_flags, int n, int op1, int op2,
+                            int32_t a, int32_t b, int logical, void *stream);
+
+/* increment_logical (incremental_tile_list.cl:110-154), used by
+ * IncrementalTileList.incorporate (IncrementalTileList.pyx:138-155):
+ * where (t1[i] OP1 a) LOGICAL (t2[i] OP2 b), t1[i] = iteration. */
+int gcmi_tiles_increment_logical(int32_t *d_t1, const int32_t *d_t2, int n,
+                                 int op1, int op2, int32_t a, int32_t b,
+                                 int logical, int32_t iteration, void *stream);
+
+/* IncrementalTileList.build(op, operand) (IncrementalTileList.pyx:119-136):
+ * flag + compact fused: d_list = ascending i with d_tiles[i] OP operand,
+ * d_length[0] = count. */
+int gcmi_tiles_build(const int32_t *d_tiles, int n, int op, int32_t operand,
+                     int32_t *d_list, int32_t *d_length, void *stream);
+
+/* IncrementalTileList.buildLogical (IncrementalTileList.pyx:157-176), fused. */
+int gcmi_tiles_build_logical(const int32_t *d_t1, const int32_t *d_t2, int n,
+                             int op1, int op2, int32_t a, int32_t b,
+                             int logical, int32_t *d_list, int32_t *d_length,
+                             void *stream);
+
+/* ------------------------------------------------------------- grid cut
+ * GraphCut (GraphCut.pyx:80-418).  The reference builds its pairwise weights
+ * from an RGBA image on the device and takes one float "excess" image; this
+ * interface takes int32 capacities instead (all arrays [height][width],
+ * row-major, C-contiguous):
+ *   source[p] >= 0 (s->p), sink[p] >= 0 (p->t),
+ *   right[p] = w(p, p+1) >= 0 (last column ignored),
+ *   down[p]  = w(p, p+width) >= 0 (last row ignored);
+ *   max(source, sink) + 4*max(right, down) must be < 2^31.
+ * Any width/height >= 1 is accepted; the grid is padded internally to a
+ * multiple of the tile with zero-capacity pixels. */
+typedef struct gcmi_cut gcmi_cut;
+
+/* tile: 16, 32 or 64 (edge length of a square tile). */
+int gcmi_cut_create(int width, int height, int tile, gcmi_cut **out);
+int gcmi_cut_destroy(gcmi_cut *cut);
+/* Upload capacities from host memory (copied; pointers not retained). */
+int gcmi_cut_upload(gcmi_cut *cut, const int32_t *h_source,
+                    const int32_t *h_sink, const int32_t *h_right,
+                    const int32_t *h_down);
+/* Same from device memory (row-major int32 [height][width] each). */
+int gcmi_cut_upload_device(gcmi_cut *cut, const int32_t *d_source,
+                           const int32_t *d_sink, const int32_t *d_right,
+                           const int32_t *d_down);
+/* GraphCut.cut (GraphCut.pyx:348-418): run push-relabel to termination.
+ *   sweeps      in-tile push/relabel sweeps per tile residency (<=0: default)
+ *   bfs         push rounds between global relabels (<=0: default)
+ *   max_rounds  cap on push rounds (<=0: no cap); GCMI_ENOTCONVERGED if hit
+ *   flow_out    max-flow value (int64), may be NULL
+ * After it returns GCMI_OK, gcmi_cut_labels gives the minimal source set. */
+int gcmi_cut_run(gcmi_cut *cut, int sweeps, int bfs, int64_t max_rounds,
+                 int64_t *flow_out);
+/* Labels of the last cut: h_out[y*width+x] = 1 for the minimal source
+ * (foreground) set -- pixels reachable from the source in the final residual
+ * graph -- else 0.  (The reference exposes only dHeight, GraphCut.pyx:62.) */
+int gcmi_cut_labels(gcmi_cut *cut, uint8_t *h_out);
+/* Same into device memory (width*height bytes). */
+int gcmi_cut_labels_device(gcmi_cut *cut, uint8_t *d_out);
+/* Counters of the last run:
+ *   [0] push rounds  [1] tile residencies  [2] global relabels
+ *   [3] BFS tile passes  [4] flood tile passes  [5] host syncs
+ *   [6] algorithmic bytes (see DESIGN)  [7] tiles  */
+int gcmi_cut_stats(gcmi_cut *cut, int64_t *out, int n);
+/* Device-resident tile-blocked height image (int32, padded) for inspection:
+ * copies padded_w*padded_h ints in row-major order into h_out. */
+int gcmi_cut_heights(gcmi_cut *cut, int32_t *h_out);
+/* Padded geometry: out[0]=padded width, out[1]=padded height, out[2]=tile. */
+int gcmi_cut_geometry(gcmi_cut *cut, int *out3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GCMI_H */
