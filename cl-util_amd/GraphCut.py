@@ -23,8 +23,8 @@ import numpy as np
 from gcmi_lib import DeviceBuffer, check, host_ptr, lib, ptr
 
 TILE_DEFAULT = (32, 32)
-BFS_DEFAULT = 1       # push rounds per global relabel
-SWEEPS_DEFAULT = 16   # in-tile sweeps per tile residency
+BFS_DEFAULT = 0       # push rounds per global relabel (0: library default, 1)
+SWEEPS_DEFAULT = 0    # in-tile sweeps per tile residency (0: library default, 4)
 
 STAT_NAMES = ("rounds", "residencies", "global_relabels", "bfs_passes",
               "flood_passes", "host_syncs", "alg_bytes", "tiles")
@@ -130,6 +130,19 @@ class GraphCut:
         a = (ctypes.c_int64 * len(STAT_NAMES))()
         check(lib().gcmi_cut_stats(self._h, a, len(STAT_NAMES)))
         return dict(zip(STAT_NAMES, [int(v) for v in a]))
+
+    def profile(self, enable=True):
+        """Time every later run's kernels with HIP events on the cut's stream."""
+        check(lib().gcmi_cut_profile(self._h, 1 if enable else 0))
+
+    def kernel_times(self):
+        """{class: (total ms, launches)} of the last profiled run."""
+        from gcmi_lib import KERNEL_CLASSES
+        n = len(KERNEL_CLASSES)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int64 * n)()
+        check(lib().gcmi_cut_kernel_times(self._h, ms, cnt, n))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_CLASSES)}
 
     @property
     def dHeight(self):

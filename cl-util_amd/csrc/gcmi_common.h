@@ -113,10 +113,15 @@ struct Pred {
   int logical;
 
   __device__ __forceinline__ bool eval(int i) const {
-    if (kind == NONZERO) return t1[i] != 0;
-    bool f1 = tile_cmp(t1[i], op1, a);
+    return eval_values(t1[i], kind == LOGICAL ? t2[i] : 0);
+  }
+
+  // The predicate on already-loaded values (v2 is ignored unless LOGICAL).
+  __device__ __forceinline__ bool eval_values(int32_t v1, int32_t v2) const {
+    if (kind == NONZERO) return v1 != 0;
+    bool f1 = tile_cmp(v1, op1, a);
     if (kind == CMP) return f1;
-    bool f2 = tile_cmp(t2[i], op2, b);
+    bool f2 = tile_cmp(v2, op2, b);
     if (logical == GCMI_LOGICAL_AND) return f1 && f2;
     if (logical == GCMI_LOGICAL_OR) return f1 || f2;
     return f1;

@@ -6,8 +6,9 @@
 
 namespace gcmi {
 
-constexpr int kDefaultSweeps = 16;  // in-tile sweeps per residency
+constexpr int kDefaultSweeps = 4;   // in-tile sweeps per residency
 constexpr int kDefaultBfs = 1;      // push rounds per global relabel
+constexpr int kRoundBatch = 2;      // BFS / flood rounds launched per host check
 
 enum { CNT_ACT = 0, CNT_BFS = 1, CNT_POS = 2, CNT_FLOOD = 3, CNT_N = 8 };
 enum {
@@ -24,15 +25,43 @@ enum {
 
 struct CutState;
 
+// Kernel classes timed by the optional HIP-event profiler.
+enum {
+  K_PUSH_FIRST = 0,
+  K_PUSH = 1,
+  K_FOLD = 2,
+  K_BFS_INIT = 3,
+  K_BFS_ROUND = 4,
+  K_FLOOD = 5,
+  K_LABELS = 6,
+  K_COMPACT = 7,
+  K_N = 8
+};
+
+// Per-launch HIP events on the cut's stream, summed per kernel class after
+// the run's final synchronisation.
+struct Prof {
+  bool on = false;
+  int used = 0;
+  int cap = 0;
+  hipEvent_t* ev = nullptr;  // 2 per launch
+  int* cls = nullptr;
+  double ms[K_N];
+  int64_t launches[K_N];
+  void reset();
+  void begin(hipStream_t s, int k);
+  void end(hipStream_t s);
+  void collect();
+  void release();
+};
+
 struct CutOps {
   int tile;
   int threads;
-  void (*init)(CutState*, const int32_t* const in[4]);
-  void (*push)(CutState*, int grid, int sweeps);
-  void (*apply_pending)(CutState*);
-  void (*bfs_init)(CutState*);
+  void (*push)(CutState*, int grid, int sweeps, bool first);
+  void (*fold)(CutState*, int grid);
+  void (*bfs_init)(CutState*, int grid);
   void (*bfs_round)(CutState*, int grid);
-  void (*check)(CutState*, int grid);
   void (*flood)(CutState*, int grid, bool first);
   void (*labels)(CutState*, uint8_t* out);
   void (*untile)(CutState*, const int32_t* src, int32_t* out);
@@ -45,21 +74,26 @@ struct CutState {
   int32_t *e, *h, *cu, *cd, *cl, *cr;  // tile-blocked state
   uint8_t *mask, *reach;               // tile-blocked residual bits / reach
   int32_t* pend;                       // pending edge inflow, 4*T per tile
+  int32_t* pflag;                      // tile has pending inflow
+  long long* tpos;                     // per tile: sum of max(e, 0)
   int32_t *act, *bflag, *fflag, *haspos;
-  int32_t *list, *plist;
+  int32_t *list, *plist, *alist;       // BFS/flood list, excess tiles, active tiles
   int32_t* counts;                     // device list lengths
   unsigned long long* sums;            // [0] sum min, [1] sum e0+, [2] sum e+
   int32_t* scratch;
   int32_t* in[4];                      // row-major source, sink, right, down
   uint8_t* labels;                     // row-major result
-  void* host;                          // pinned readback buffer
+  void* host;                          // pinned readback of the counts
+  unsigned long long* host_sums;       // pinned readback of the sums
   size_t device_bytes;
   bool uploaded, solved;
   int64_t flow;
   int64_t stats[STAT_N];
+  Prof* prof;
 };
 
 const CutOps* cut_ops(int tile);
+CutDev cut_dev(CutState* c);
 size_t cut_state_bytes(int W, int H, int tile);
 int cut_state_init(CutState* c, int W, int H, int tile, hipStream_t stream);
 void cut_state_free(CutState* c);

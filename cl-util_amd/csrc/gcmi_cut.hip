@@ -19,40 +19,107 @@
 
 namespace gcmi {
 
+void Prof::reset() {
+  used = 0;
+  for (int k = 0; k < K_N; ++k) {
+    ms[k] = 0;
+    launches[k] = 0;
+  }
+}
+
+void Prof::begin(hipStream_t s, int k) {
+  if (used == cap) {
+    int ncap = cap ? cap * 2 : 256;
+    hipEvent_t* nev = (hipEvent_t*)realloc(ev, sizeof(hipEvent_t) * 2 * ncap);
+    int* ncls = (int*)realloc(cls, sizeof(int) * ncap);
+    if (!nev || !ncls) {
+      if (nev) ev = nev;
+      if (ncls) cls = ncls;
+      on = false;
+      return;
+    }
+    ev = nev;
+    cls = ncls;
+    for (int i = 2 * cap; i < 2 * ncap; ++i) (void)hipEventCreate(&ev[i]);
+    cap = ncap;
+  }
+  cls[used] = k;
+  (void)hipEventRecord(ev[2 * used], s);
+}
+
+void Prof::end(hipStream_t s) {
+  (void)hipEventRecord(ev[2 * used + 1], s);
+  ++used;
+}
+
+void Prof::collect() {
+  for (int i = 0; i < used; ++i) {
+    float t = 0;
+    if (hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]) == hipSuccess) {
+      ms[cls[i]] += t;
+      launches[cls[i]]++;
+    }
+  }
+  used = 0;
+}
+
+void Prof::release() {
+  for (int i = 0; i < 2 * cap; ++i) (void)hipEventDestroy(ev[i]);
+  free(ev);
+  free(cls);
+  ev = nullptr;
+  cls = nullptr;
+  cap = used = 0;
+}
+
+CutDev cut_dev(CutState* c) {
+  CutDev d;
+  d.e = c->e;
+  d.h = c->h;
+  d.cu = c->cu;
+  d.cd = c->cd;
+  d.cl = c->cl;
+  d.cr = c->cr;
+  d.mask = c->mask;
+  d.pend = c->pend;
+  d.pflag = c->pflag;
+  d.act = c->act;
+  d.tpos = c->tpos;
+  d.sums = c->sums;
+  d.cs = c->in[0];
+  d.ct = c->in[1];
+  d.right = c->in[2];
+  d.down = c->in[3];
+  return d;
+}
+
 template <int T, int NT>
 struct CutKernels {
-  static void init(CutState* c, const int32_t* const in[4]) {
-    hipLaunchKernelGGL((k_init<T, NT>), dim3(c->g.ntiles), dim3(NT), 0, c->stream, c->g, in[0],
-                       in[1], in[2], in[3], c->e, c->h, c->cu, c->cd, c->cl, c->cr, c->act,
-                       c->sums);
+  static void push(CutState* c, int grid, int sweeps, bool first) {
+    if (first)
+      hipLaunchKernelGGL((k_push<T, NT, true>), dim3(grid), dim3(NT), 0, c->stream, c->g,
+                         cut_dev(c), (const int32_t*)nullptr, (const int32_t*)nullptr, sweeps);
+    else
+      hipLaunchKernelGGL((k_push<T, NT, false>), dim3(grid), dim3(NT), 0, c->stream, c->g,
+                         cut_dev(c), c->alist, c->counts + CNT_ACT, sweeps);
   }
-  static void push(CutState* c, int grid, int sweeps) {
-    hipLaunchKernelGGL((k_push<T, NT>), dim3(grid), dim3(NT), 0, c->stream, c->g, c->list,
-                       c->counts + CNT_ACT, c->e, c->h, c->cu, c->cd, c->cl, c->cr, c->pend,
-                       c->act, sweeps);
+  static void fold(CutState* c, int grid) {
+    hipLaunchKernelGGL((k_fold<T>), dim3(grid), dim3(kFoldNT), 0, c->stream, c->g, cut_dev(c));
   }
-  static void apply_pending(CutState* c) {
-    const size_t slots = (size_t)c->g.ntiles * 4 * T;
-    hipLaunchKernelGGL((k_apply_pending<T>), dim3((unsigned)((slots + 255) / 256)), dim3(256), 0,
-                       c->stream, c->g, c->pend, c->e, c->cu, c->cd, c->cl, c->cr);
-  }
-  static void bfs_init(CutState* c) {
-    hipLaunchKernelGGL((k_bfs_init<T, NT>), dim3(c->g.ntiles), dim3(NT), 0, c->stream, c->g, c->e,
-                       c->cu, c->cd, c->cl, c->cr, c->h, c->mask, c->bflag, c->haspos);
+  static void bfs_init(CutState* c, int grid) {
+    hipLaunchKernelGGL((k_bfs_init<T, NT>), dim3(grid), dim3(NT), 0, c->stream, c->g, c->mask,
+                       c->h, c->bflag, c->haspos, c->act);
   }
   static void bfs_round(CutState* c, int grid) {
     hipLaunchKernelGGL((k_bfs_round<T, NT>), dim3(grid), dim3(NT), 0, c->stream, c->g, c->list,
-                       c->counts + CNT_BFS, c->h, c->mask, c->bflag);
-  }
-  static void check(CutState* c, int grid) {
-    hipLaunchKernelGGL((k_check<T, NT>), dim3(grid), dim3(NT), 0, c->stream, c->g, c->plist,
-                       c->counts + CNT_POS, c->e, c->h, c->act, c->sums);
+                       c->counts + CNT_BFS, c->h, c->mask, c->bflag, c->act, c->sums);
   }
   static void flood(CutState* c, int grid, bool first) {
     const int32_t* lst = first ? c->plist : c->list;
     const int32_t* cnt = c->counts + (first ? CNT_POS : CNT_FLOOD);
     hipLaunchKernelGGL((k_flood<T, NT>), dim3(grid), dim3(NT), 0, c->stream, c->g, lst, cnt,
-                       c->mask, c->reach, c->fflag, first ? 1 : 0);
+                       c->mask, c->reach, c->fflag, first ? 1 : 0, c->sums,
+                       (const long long*)c->tpos);
   }
   static void labels(CutState* c, uint8_t* out) {
     hipLaunchKernelGGL((k_labels<T>), dim3((c->g.W + 255) / 256, c->g.H), dim3(256), 0,
@@ -69,12 +136,10 @@ template <int T, int NT>
 static const CutOps* ops_for() {
   static const CutOps ops = {T,
                              NT,
-                             &CutKernels<T, NT>::init,
                              &CutKernels<T, NT>::push,
-                             &CutKernels<T, NT>::apply_pending,
+                             &CutKernels<T, NT>::fold,
                              &CutKernels<T, NT>::bfs_init,
                              &CutKernels<T, NT>::bfs_round,
-                             &CutKernels<T, NT>::check,
                              &CutKernels<T, NT>::flood,
                              &CutKernels<T, NT>::labels,
                              &CutKernels<T, NT>::untile};
@@ -145,19 +210,24 @@ int cut_state_init(CutState* c, int W, int H, int tile, hipStream_t stream) {
   ALLOC(c->mask, np);
   ALLOC(c->reach, np);
   ALLOC(c->pend, nt * 4 * tile * 4);
+  ALLOC(c->pflag, nt * 4);
+  ALLOC(c->tpos, nt * 8);
   ALLOC(c->act, nt * 4);
   ALLOC(c->bflag, nt * 4);
   ALLOC(c->fflag, nt * 4);
   ALLOC(c->haspos, nt * 4);
   ALLOC(c->list, nt * 4);
   ALLOC(c->plist, nt * 4);
+  ALLOC(c->alist, nt * 4);
   ALLOC(c->counts, CNT_N * 4);
-  ALLOC(c->sums, 4 * sizeof(unsigned long long));
+  ALLOC(c->sums, kSumBytes);
   ALLOC(c->scratch, (size_t)compact_scratch_ints(c->g.ntiles) * 4);
   for (int k = 0; k < 4; ++k) ALLOC(c->in[k], n * 4);
   ALLOC(c->labels, n);
 #undef ALLOC
-  if (rc == GCMI_OK && hipHostMalloc((void**)&c->host, 64, hipHostMallocDefault) != hipSuccess) {
+  if (rc == GCMI_OK && (hipHostMalloc((void**)&c->host, 256, hipHostMallocDefault) != hipSuccess ||
+                        hipHostMalloc((void**)&c->host_sums, kSumBytes, hipHostMallocDefault) !=
+                            hipSuccess)) {
     (void)hipGetLastError();
     set_error("pinned host allocation failed");
     rc = GCMI_ENOMEM;
@@ -176,23 +246,28 @@ int cut_state_init(CutState* c, int W, int H, int tile, hipStream_t stream) {
 }
 
 void cut_state_free(CutState* c) {
-  void* ptrs[] = {c->e,     c->h,     c->cu,      c->cd,    c->cl,    c->cr,    c->mask,
+  void* ptrs[] = {c->pflag, c->tpos, c->e,     c->h,     c->cu,      c->cd,    c->cl,    c->cr,    c->mask,
                   c->reach, c->pend,  c->act,     c->bflag, c->fflag, c->haspos, c->list,
-                  c->plist, c->counts, c->sums,   c->scratch, c->in[0], c->in[1], c->in[2],
+                  c->plist, c->alist, c->counts, c->sums,   c->scratch, c->in[0], c->in[1], c->in[2],
                   c->in[3], c->labels};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->host) (void)hipHostFree(c->host);
+  if (c->host_sums) (void)hipHostFree(c->host_sums);
   const CutOps* ops = c->ops;
+  Prof* prof = c->prof;
   memset(c, 0, sizeof *c);
   c->ops = ops;
+  c->prof = prof;
 }
 
-// Reads counts[idx] to the host (synchronises the stream).
-static int read_count(CutState* c, int idx, int32_t* out) {
-  GCMI_HIP(hipMemcpyAsync(c->host, c->counts + idx, 4, hipMemcpyDeviceToHost, c->stream));
+// Reads the active count (and the BFS count) to the host; one sync.
+static int read_counts(CutState* c, int32_t* nact, int32_t* nbfs) {
+  int32_t* hc = (int32_t*)c->host;
+  GCMI_HIP(hipMemcpyAsync(hc, c->counts, CNT_N * 4, hipMemcpyDeviceToHost, c->stream));
   GCMI_HIP(hipStreamSynchronize(c->stream));
-  *out = ((int32_t*)c->host)[0];
+  *nact = hc[CNT_ACT];
+  if (nbfs) *nbfs = hc[CNT_BFS];
   c->stats[STAT_SYNCS]++;
   return GCMI_OK;
 }
@@ -201,8 +276,12 @@ static int compact_flags(CutState* c, int32_t* flags, int32_t* list, int cnt, bo
   Pred p{};
   p.t1 = flags;
   p.kind = Pred::NONZERO;
-  return compact_async(p, c->g.ntiles, list, c->counts + cnt, clear ? flags : nullptr,
-                       c->scratch, c->stream);
+  Prof* P = (c->prof && c->prof->on) ? c->prof : nullptr;
+  if (P) P->begin(c->stream, K_COMPACT);
+  int rc = compact_async(p, c->g.ntiles, list, c->counts + cnt, clear ? flags : nullptr,
+                         c->scratch, c->stream);
+  if (P) P->end(c->stream);
+  return rc;
 }
 
 int cut_validate_inputs(const int32_t* cs, const int32_t* ct, const int32_t* right,
@@ -235,6 +314,14 @@ int cut_run(CutState* c, int sweeps, int bfs, int64_t max_rounds, int64_t* flow_
   if (bfs <= 0) bfs = kDefaultBfs;
   const CutOps* ops = c->ops;
   const int T = ops->tile;
+  Prof* P = (c->prof && c->prof->on) ? c->prof : nullptr;
+  auto B = [&](int k) {
+    if (P) P->begin(c->stream, k);
+  };
+  auto E = [&]() {
+    if (P) P->end(c->stream);
+  };
+  if (P) P->reset();
   for (int k = 0; k < STAT_N; ++k) c->stats[k] = 0;
   c->stats[STAT_TILES] = c->g.ntiles;
   c->solved = false;
@@ -243,91 +330,130 @@ int cut_run(CutState* c, int sweeps, int bfs, int64_t max_rounds, int64_t* flow_
   GCMI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   const int grid_cap = ncu > 0 ? ncu * 8 : 2048;
   auto grid_for = [&](int n) { return n < grid_cap ? (n > 0 ? n : 1) : grid_cap; };
+  const int nt = c->g.ntiles;
+  int32_t* hc = (int32_t*)c->host;  // pinned count readback
 
-  GCMI_HIP(hipMemsetAsync(c->sums, 0, 4 * sizeof(unsigned long long), c->stream));
-  GCMI_HIP(hipMemsetAsync(c->pend, 0, (size_t)c->g.ntiles * 4 * T * 4, c->stream));
-  const int32_t* in[4] = {c->in[0], c->in[1], c->in[2], c->in[3]};
-  ops->init(c, in);
+  GCMI_HIP(hipMemsetAsync(c->sums, 0, kSumBytes, c->stream));
+  GCMI_HIP(hipMemsetAsync(c->pend, 0, (size_t)nt * 4 * T * 4, c->stream));
+  GCMI_HIP(hipMemsetAsync(c->pflag, 0, (size_t)nt * 4, c->stream));
+  GCMI_HIP(hipMemsetAsync(c->act, 0, (size_t)nt * 4, c->stream));
+
+  // round 1 over every tile, building the state from the row-major inputs
+  B(K_PUSH_FIRST);
+  ops->push(c, nt, sweeps, true);
+  E();
   GCMI_HIP(hipGetLastError());
-
-  int64_t rounds = 0;
-  bool have_list = false;
+  int64_t rounds = 1;
+  c->stats[STAT_RESIDENCIES] += nt;
   int32_t nact = 0;
   for (;;) {
-    for (int r = 0; r < bfs; ++r) {
-      if (!have_list) {
-        GCMI_TRY(compact_flags(c, c->act, c->list, CNT_ACT, true));
-        GCMI_TRY(read_count(c, CNT_ACT, &nact));
-      }
-      have_list = false;
+    // further push rounds before the global relabel
+    for (int r = 1; r < bfs; ++r) {
+      GCMI_TRY(compact_flags(c, c->act, c->alist, CNT_ACT, true));
+      GCMI_TRY(read_counts(c, &nact, nullptr));
       if (nact == 0) break;
       if (max_rounds > 0 && rounds >= max_rounds) {
         set_error("cut did not terminate within %lld push rounds", (long long)max_rounds);
         return GCMI_ENOTCONVERGED;
       }
-      ops->push(c, grid_for(nact), sweeps);
+      B(K_PUSH);
+      ops->push(c, grid_for(nact), sweeps, false);
+      E();
       GCMI_HIP(hipGetLastError());
       ++rounds;
       c->stats[STAT_RESIDENCIES] += nact;
     }
-    // global relabel
-    ops->apply_pending(c);
-    ops->bfs_init(c);
+    // fold pending flow, then global relabel + completion test
+    B(K_FOLD);
+    ops->fold(c, nt);
+    E();
+    B(K_BFS_INIT);
+    ops->bfs_init(c, nt);
+    E();
     GCMI_HIP(hipGetLastError());
     c->stats[STAT_GLOBAL_RELABELS]++;
-    c->stats[STAT_BFS_PASSES] += c->g.ntiles;
+    // later BFS rounds, kRoundBatch per host check; a round whose list is
+    // empty exits at once
+    bool list_ready = false;
     for (;;) {
-      int32_t nb = 0;
+      for (int b = 0; b < kRoundBatch; ++b) {
+        if (!list_ready) GCMI_TRY(compact_flags(c, c->bflag, c->list, CNT_BFS, true));
+        list_ready = false;
+        B(K_BFS_ROUND);
+        ops->bfs_round(c, grid_cap);
+        E();
+        GCMI_HIP(hipGetLastError());
+      }
       GCMI_TRY(compact_flags(c, c->bflag, c->list, CNT_BFS, true));
-      GCMI_TRY(read_count(c, CNT_BFS, &nb));
+      GCMI_TRY(compact_flags(c, c->act, c->alist, CNT_ACT, false));
+      int32_t nb = 0;
+      GCMI_TRY(read_counts(c, &nact, &nb));
       if (nb == 0) break;
-      ops->bfs_round(c, grid_for(nb));
-      GCMI_HIP(hipGetLastError());
-      c->stats[STAT_BFS_PASSES] += nb;
+      list_ready = true;
     }
-    // completion check over tiles holding excess
-    GCMI_TRY(compact_flags(c, c->haspos, c->plist, CNT_POS, false));
-    GCMI_HIP(hipMemsetAsync(c->sums + 2, 0, sizeof(unsigned long long), c->stream));
-    ops->check(c, grid_cap);
-    GCMI_HIP(hipGetLastError());
-    GCMI_TRY(compact_flags(c, c->act, c->list, CNT_ACT, true));
-    GCMI_TRY(read_count(c, CNT_ACT, &nact));
     if (nact == 0) break;
-    have_list = true;
+    // the active list was built without clearing the flags (more BFS rounds
+    // could still have added tiles); clear them for the push kernel
+    GCMI_HIP(hipMemsetAsync(c->act, 0, (size_t)nt * 4, c->stream));
+    if (max_rounds > 0 && rounds >= max_rounds) {
+      set_error("cut did not terminate within %lld push rounds", (long long)max_rounds);
+      return GCMI_ENOTCONVERGED;
+    }
+    B(K_PUSH);
+    ops->push(c, grid_for(nact), sweeps, false);
+    E();
+    GCMI_HIP(hipGetLastError());
+    ++rounds;
+    c->stats[STAT_RESIDENCIES] += nact;
   }
+  GCMI_TRY(compact_flags(c, c->haspos, c->plist, CNT_POS, false));
   c->stats[STAT_ROUNDS] = rounds;
 
   // minimal source set
-  GCMI_HIP(hipMemsetAsync(c->reach, 0, (size_t)c->g.ntiles * T * T, c->stream));
+  GCMI_HIP(hipMemsetAsync(c->reach, 0, (size_t)nt * T * T, c->stream));
+  B(K_FLOOD);
   ops->flood(c, grid_cap, true);
+  E();
   GCMI_HIP(hipGetLastError());
   for (;;) {
-    int32_t nf = 0;
-    GCMI_TRY(compact_flags(c, c->fflag, c->list, CNT_FLOOD, true));
-    GCMI_TRY(read_count(c, CNT_FLOOD, &nf));
-    if (nf == 0) break;
-    ops->flood(c, grid_for(nf), false);
+    for (int b = 0; b < kRoundBatch; ++b) {
+      GCMI_TRY(compact_flags(c, c->fflag, c->list, CNT_FLOOD, true));
+      B(K_FLOOD);
+      ops->flood(c, grid_cap, false);
+      E();
+      GCMI_HIP(hipGetLastError());
+    }
+    // anything flagged by the last round?
+    GCMI_TRY(compact_flags(c, c->fflag, c->list, CNT_FLOOD, false));
+    GCMI_HIP(hipMemcpyAsync(hc, c->counts + CNT_FLOOD, 4, hipMemcpyDeviceToHost, c->stream));
+    B(K_LABELS);
+    ops->labels(c, c->labels);
+    E();
     GCMI_HIP(hipGetLastError());
-    c->stats[STAT_FLOOD_PASSES] += nf;
+    GCMI_HIP(hipMemcpyAsync(c->host_sums, c->sums, kSumBytes, hipMemcpyDeviceToHost,
+                            c->stream));
+    GCMI_HIP(hipStreamSynchronize(c->stream));
+    c->stats[STAT_SYNCS]++;
+    if (hc[0] == 0) break;
   }
-  ops->labels(c, c->labels);
-  GCMI_HIP(hipGetLastError());
-  unsigned long long* hs = (unsigned long long*)c->host;
-  GCMI_HIP(hipMemcpyAsync(hs, c->sums, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                          c->stream));
-  GCMI_HIP(hipStreamSynchronize(c->stream));
-  c->flow = (int64_t)hs[0] + (int64_t)hs[1] - (int64_t)hs[2];
-  // algorithmic bytes: 48 per pixel per residency, 24 per pixel for the
-  // relabel's first pass (read e + 4 caps, write h), 9 per pixel per later
-  // BFS pass (mask + h read, h write), 16 per pixel of input read + 24 of
-  // state written at init, 2 per pixel per flood pass, 1 per label byte.
+  const unsigned long long* hs = c->host_sums;
+  int64_t row[SUM_ROWS_ALL] = {0};
+  for (int r = 0; r < SUM_ROWS_ALL; ++r)
+    for (int k = 0; k < kSumSlots; ++k) row[r] += (int64_t)hs[r * kSumSlots + k];
+  c->flow = row[SUM_MIN] + row[SUM_POS0] - row[SUM_POSF];
+  c->stats[STAT_BFS_PASSES] = c->stats[STAT_GLOBAL_RELABELS] * nt + row[TALLY_BFS];
+  c->stats[STAT_FLOOD_PASSES] = row[TALLY_FLOOD];
+  if (P) P->collect();
+  // algorithmic bytes: first round 16 per image pixel read + 25 per pixel
+  // written (state + mask); later residencies 49 (24 read, 25 written);
+  // relabel first pass 5 (mask read, h written); later BFS passes 9 (mask +
+  // h read, h written); flood passes 2 (mask read, reach written); labels 2.
   const int64_t tp = (int64_t)T * T;
-  c->stats[STAT_ALG_BYTES] = c->stats[STAT_RESIDENCIES] * tp * 48 +
-                             c->stats[STAT_GLOBAL_RELABELS] * c->g.ntiles * tp * 24 +
-                             (c->stats[STAT_BFS_PASSES] -
-                              c->stats[STAT_GLOBAL_RELABELS] * c->g.ntiles) * tp * 9 +
-                             (int64_t)c->g.ntiles * tp * 40 + c->stats[STAT_FLOOD_PASSES] * tp * 2 +
-                             (int64_t)c->g.W * c->g.H;
+  c->stats[STAT_ALG_BYTES] = (int64_t)c->g.W * c->g.H * 16 + (int64_t)nt * tp * 25 +
+                             (c->stats[STAT_RESIDENCIES] - nt) * tp * 49 +
+                             c->stats[STAT_GLOBAL_RELABELS] * nt * tp * 5 +
+                             row[TALLY_BFS] * tp * 9 + row[TALLY_FLOOD] * tp * 2 +
+                             (int64_t)c->g.W * c->g.H * 2;
   c->solved = true;
   if (flow_out) *flow_out = c->flow;
   return GCMI_OK;
@@ -339,6 +465,7 @@ using namespace gcmi;
 
 struct gcmi_cut {
   CutState s;
+  Prof prof;
 };
 
 extern "C" {
@@ -370,6 +497,7 @@ int gcmi_cut_create(int width, int height, int tile, gcmi_cut** out) {
     free(c);
     return rc;
   }
+  c->s.prof = &c->prof;
   *out = c;
   return GCMI_OK;
 }
@@ -379,8 +507,25 @@ int gcmi_cut_destroy(gcmi_cut* c) {
   hipStream_t st = c->s.stream;
   if (st) (void)hipStreamSynchronize(st);
   cut_state_free(&c->s);
+  c->prof.release();
   if (st) (void)hipStreamDestroy(st);
   free(c);
+  return GCMI_OK;
+}
+
+int gcmi_cut_profile(gcmi_cut* c, int enable) {
+  if (!c) { set_error("null handle"); return GCMI_EINVAL; }
+  c->prof.on = enable != 0;
+  c->prof.reset();
+  return GCMI_OK;
+}
+
+int gcmi_cut_kernel_times(gcmi_cut* c, double* ms_out, int64_t* launches_out, int n) {
+  if (!c || n < 0) { set_error("bad argument"); return GCMI_EINVAL; }
+  for (int k = 0; k < n && k < K_N; ++k) {
+    if (ms_out) ms_out[k] = c->prof.ms[k];
+    if (launches_out) launches_out[k] = c->prof.launches[k];
+  }
   return GCMI_OK;
 }
 

@@ -25,6 +25,22 @@ namespace gcmi {
 enum { D_UP = 0, D_DOWN = 1, D_LEFT = 2, D_RIGHT = 3 };
 enum { M_UP = 1, M_DOWN = 2, M_LEFT = 4, M_RIGHT = 8, M_SINK = 16, M_EXCESS = 32 };
 
+// 64-bit sums are spread over kSumSlots addresses (by block) so that
+// thousands of blocks do not serialise on one atomic; the host adds the
+// slots.  Rows: sum of min(cs, ct); sum of max(e0, 0); sum of max(e, 0).
+constexpr int kSumSlots = 64;
+// Rows 3-4 tally tile passes of the BFS and flood rounds (added by block 0
+// of each round launch) so the host learns them without a sync per round.
+enum {
+  SUM_MIN = 0,
+  SUM_POS0 = 1,
+  SUM_POSF = 2,
+  TALLY_BFS = 3,
+  TALLY_FLOOD = 4,
+  SUM_ROWS_ALL = 5
+};
+constexpr size_t kSumBytes = (size_t)SUM_ROWS_ALL * kSumSlots * sizeof(unsigned long long);
+
 struct Geo {
   int W, H;      // image
   int tw, th;    // tiles across / down
@@ -51,64 +67,34 @@ __device__ __forceinline__ long long block_sum64(long long v, long long* red) {
   return s;
 }
 
-// ----------------------------------------------------------------- init
-// Row-major int32 capacities -> tile-blocked state.  h = 0 on sink pixels
-// (e < 0), 1 elsewhere: a valid labelling, since every residual arc then
-// satisfies h(p) <= h(q) + 1.  act[t] = tile has a pixel with e > 0.
-// sums[0] += sum min(cs, ct), sums[1] += sum max(cs - ct, 0).
-template <int T, int NT>
-__global__ void __launch_bounds__(NT)
-k_init(Geo g, const int32_t* __restrict__ cs, const int32_t* __restrict__ ct,
-       const int32_t* __restrict__ right, const int32_t* __restrict__ down, int32_t* e_g,
-       int32_t* h_g, int32_t* cu_g, int32_t* cd_g, int32_t* cl_g, int32_t* cr_g, int32_t* act,
-       unsigned long long* sums) {
-  constexpr int R = T * T / NT;
-  __shared__ long long red[NT / kWave];
-  const int t = blockIdx.x;
-  const int tx = t % g.tw, ty = t / g.tw;
-  const int lx = threadIdx.x % T, chain = threadIdx.x / T;
-  const size_t base = tbase<T>(t);
-  long long smin = 0, spos = 0;
-  int any = 0;
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    const int ly = chain * R + k;
-    const int x = tx * T + lx, y = ty * T + ly;
-    const size_t i = base + ly * T + lx;
-    int ev = 0, u = 0, d = 0, l = 0, r = 0;
-    if (x < g.W && y < g.H) {
-      const size_t p = (size_t)y * g.W + x;
-      const int a = cs[p], b = ct[p];
-      smin += min(a, b);
-      ev = a - b;
-      spos += ev > 0 ? ev : 0;
-      r = x < g.W - 1 ? right[p] : 0;
-      l = x > 0 ? right[p - 1] : 0;
-      d = y < g.H - 1 ? down[p] : 0;
-      u = y > 0 ? down[p - g.W] : 0;
-    }
-    e_g[i] = ev;
-    h_g[i] = ev < 0 ? 0 : 1;
-    cu_g[i] = u;
-    cd_g[i] = d;
-    cl_g[i] = l;
-    cr_g[i] = r;
-    any |= ev > 0;
-  }
-  smin = block_sum64<NT>(smin, red);
-  spos = block_sum64<NT>(spos, red);
-  any = __syncthreads_or(any);
-  if (threadIdx.x == 0) {
-    act[t] = any;
-    if (smin) atomicAdd(&sums[0], (unsigned long long)smin);
-    if (spos) atomicAdd(&sums[1], (unsigned long long)spos);
-  }
+// Device pointers of one cut's state, passed by value to the kernels.
+struct CutDev {
+  int32_t *e, *h, *cu, *cd, *cl, *cr;  // tile-blocked state
+  uint8_t* mask;                       // residual bits + sink/excess bits
+  int32_t* pend;                       // pending edge inflow, 4*T per tile
+  int32_t* pflag;                      // tile has pending inflow
+  int32_t* act;                        // tile flagged for the next push round
+  long long* tpos;                     // per tile: sum of max(e, 0)
+  unsigned long long* sums;            // slotted 64-bit sums / tallies
+  const int32_t *cs, *ct, *right, *down;  // row-major inputs
+};
+
+__device__ __forceinline__ int pixel_mask(int e, int cu, int cd, int cl, int cr) {
+  return (cu > 0 ? M_UP : 0) | (cd > 0 ? M_DOWN : 0) | (cl > 0 ? M_LEFT : 0) |
+         (cr > 0 ? M_RIGHT : 0) | (e < 0 ? M_SINK : 0) | (e > 0 ? M_EXCESS : 0);
 }
 
 // ----------------------------------------------------------------- push
-// One residency of each listed tile: load state + pending inflow + frozen
-// halo heights, run `sweeps` synchronous push/relabel sweeps in LDS, write
-// back, and post edge outflow to the neighbours' pending slots.
+// One residency of each listed tile: load state (FIRST: build it from the
+// row-major capacities), fold pending inflow, freeze the halo heights, run
+// up to `sweeps` synchronous push/relabel sweeps in LDS, write back state,
+// residual mask and the tile's positive-excess sum, and post edge outflow to
+// the neighbours' pending slots.
+//
+// FIRST (the first round, over every tile): e = cs - ct, caps from the
+// pairwise weights, h = 0 on sink pixels (e < 0) and 1 elsewhere -- a valid
+// labelling, since every residual arc then satisfies h(p) <= h(q) + 1 --
+// and sums[SUM_MIN] / sums[SUM_POS0] collect sum min(cs, ct), sum max(e0, 0).
 //
 // A sweep: every active pixel (e > 0, h < MAX) pushes min(rem, cap) along
 // each admissible arc (h(p) == h(q) + 1) in the order down, up, right, left
@@ -117,87 +103,125 @@ k_init(Geo g, const int32_t* __restrict__ cs, const int32_t* __restrict__ ct,
 // to min(h(q) + 1) over residual arcs (MAX if none), all from the heights
 // of the sweep's start (graphcut.cl:538-577 is the same Jacobi relabel).
 // Heights are double-buffered in LDS, so a sweep needs two barriers.
-template <int T, int NT>
+//
+// Pending protocol (no fences: an agent-scope fence writes back the whole
+// L2 on this part): a sender atomically adds its edge outflow to the
+// receiver's slot and sets pflag[receiver] and act[receiver] with plain
+// stores.  A resident receiver reads pflag as a hint and, if set, exchanges
+// its slots with 0; it never clears pflag.  Whatever lands in a slot after
+// that exchange stays there with pflag still set, and k_fold (a separate
+// launch, so every store of this kernel is visible) folds it and clears
+// the flag.  No inflow is lost or counted twice.
+template <int T, int NT, bool FIRST>
 __global__ void __launch_bounds__(NT)
-k_push(Geo g, const int32_t* __restrict__ list, const int32_t* __restrict__ count, int32_t* e_g,
-       int32_t* h_g, int32_t* cu_g, int32_t* cd_g, int32_t* cl_g, int32_t* cr_g, int32_t* pend,
-       int32_t* act, int sweeps) {
+k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
+       int sweeps) {
   constexpr int R = T * T / NT;
   constexpr int HP = T + 2;
   static_assert(4 * T <= NT, "halo loaders");
   __shared__ int32_t H0[HP * HP], H1[HP * HP];
   __shared__ int32_t F[4][T * T];
   __shared__ int32_t BO[4][T];
+  __shared__ long long red[NT / kWave];
+  __shared__ int sflag;
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
   const int32_t MAXH = g.maxh;
-  const int n = *count;
+  const int n = FIRST ? g.ntiles : *count;
+  long long smin = 0, spos = 0;
   for (int li = blockIdx.x; li < n; li += gridDim.x) {
-    const int t = list[li];
+    const int t = FIRST ? li : list[li];
     const int tx = t % g.tw, ty = t / g.tw;
     const size_t base = tbase<T>(t);
     int e[R], h[R], cu[R], cd[R], cl[R], cr[R];
+    if (FIRST) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int x = tx * T + lx, y = ty * T + chain * R + k;
+        e[k] = 0; cu[k] = 0; cd[k] = 0; cl[k] = 0; cr[k] = 0;
+        if (x < g.W && y < g.H) {
+          const size_t p = (size_t)y * g.W + x;
+          const int a = s.cs[p], b = s.ct[p];
+          cr[k] = x < g.W - 1 ? s.right[p] : 0;
+          cl[k] = x > 0 ? s.right[p - 1] : 0;
+          cd[k] = y < g.H - 1 ? s.down[p] : 0;
+          cu[k] = y > 0 ? s.down[p - g.W] : 0;
+          e[k] = a - b;
+          smin += min(a, b);
+          spos += e[k] > 0 ? e[k] : 0;
+        }
+        h[k] = e[k] < 0 ? 0 : 1;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const size_t i = base + (chain * R + k) * T + lx;
+        e[k] = s.e[i];
+        h[k] = s.h[i];
+        cu[k] = s.cu[i];
+        cd[k] = s.cd[i];
+        cl[k] = s.cl[i];
+        cr[k] = s.cr[i];
+      }
+    }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      const int ly = chain * R + k;
-      const size_t i = base + ly * T + lx;
-      e[k] = e_g[i];
-      h[k] = h_g[i];
-      cu[k] = cu_g[i];
-      cd[k] = cd_g[i];
-      cl[k] = cl_g[i];
-      cr[k] = cr_g[i];
-      // pending inflow from the four neighbours (sides: 0 top, 1 bottom,
-      // 2 left, 3 right of this tile)
-      if (ly == 0) {
-        int v = atomicExch(&pend[((size_t)t * 4 + 0) * T + lx], 0);
-        e[k] += v;
-        cu[k] += v;
-      }
-      if (ly == T - 1) {
-        int v = atomicExch(&pend[((size_t)t * 4 + 1) * T + lx], 0);
-        e[k] += v;
-        cd[k] += v;
-      }
-      if (lx == 0) {
-        int v = atomicExch(&pend[((size_t)t * 4 + 2) * T + ly], 0);
-        e[k] += v;
-        cl[k] += v;
-      }
-      if (lx == T - 1) {
-        int v = atomicExch(&pend[((size_t)t * 4 + 3) * T + ly], 0);
-        e[k] += v;
-        cr[k] += v;
-      }
-      const int c = (ly + 1) * HP + lx + 1;
+      const int c = (chain * R + k + 1) * HP + lx + 1;
       H0[c] = h[k];
       H1[c] = h[k];
     }
     if (threadIdx.x < 4 * T) {
       const int side = threadIdx.x / T, j = threadIdx.x % T;
-      int v = MAXH, idx;
-      if (side == 0) {
-        if (ty > 0) v = h_g[tbase<T>(t - g.tw) + (T - 1) * T + j];
-        idx = j + 1;
-      } else if (side == 1) {
-        if (ty < g.th - 1) v = h_g[tbase<T>(t + g.tw) + j];
-        idx = (T + 1) * HP + j + 1;
-      } else if (side == 2) {
-        if (tx > 0) v = h_g[tbase<T>(t - 1) + j * T + T - 1];
-        idx = (j + 1) * HP;
-      } else {
-        if (tx < g.tw - 1) v = h_g[tbase<T>(t + 1) + j * T];
-        idx = (j + 1) * HP + T + 1;
+      int nb = -1, nx = 0, ny = 0, idx;
+      if (side == 0) { if (ty > 0) { nb = t - g.tw; nx = j; ny = T - 1; } idx = j + 1; }
+      else if (side == 1) { if (ty < g.th - 1) { nb = t + g.tw; nx = j; ny = 0; } idx = (T + 1) * HP + j + 1; }
+      else if (side == 2) { if (tx > 0) { nb = t - 1; nx = T - 1; ny = j; } idx = (j + 1) * HP; }
+      else { if (tx < g.tw - 1) { nb = t + 1; nx = 0; ny = j; } idx = (j + 1) * HP + T + 1; }
+      int v = MAXH;
+      if (nb >= 0) {
+        if (FIRST) {
+          const int x = (nb % g.tw) * T + nx, y = (nb / g.tw) * T + ny;
+          if (x < g.W && y < g.H) {
+            const size_t p = (size_t)y * g.W + x;
+            v = s.cs[p] - s.ct[p] < 0 ? 0 : 1;
+          }
+        } else {
+          v = s.h[tbase<T>(nb) + ny * T + nx];
+        }
       }
       H0[idx] = v;
       H1[idx] = v;
       BO[side][j] = 0;
     }
+    if (threadIdx.x == 0) sflag = s.pflag[t];
     __syncthreads();
+    if (sflag) {
+      // issue every exchange first, then use the results (one wait)
+      int vu[R], vd[R], vl[R], vr[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int ly = chain * R + k;
+        vu[k] = ly == 0 ? atomicExch(&s.pend[((size_t)t * 4 + 0) * T + lx], 0) : 0;
+        vd[k] = ly == T - 1 ? atomicExch(&s.pend[((size_t)t * 4 + 1) * T + lx], 0) : 0;
+        vl[k] = lx == 0 ? atomicExch(&s.pend[((size_t)t * 4 + 2) * T + ly], 0) : 0;
+        vr[k] = lx == T - 1 ? atomicExch(&s.pend[((size_t)t * 4 + 3) * T + ly], 0) : 0;
+      }
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        e[k] += vu[k] + vd[k] + vl[k] + vr[k];
+        cu[k] += vu[k];
+        cd[k] += vd[k];
+        cl[k] += vl[k];
+        cr[k] += vr[k];
+      }
+    }
+    int any0 = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) any0 |= e[k] > 0 && h[k] < MAXH;
+    int active = __syncthreads_or(any0);
 
-    int active = 1;
-    for (int s = 0; s < sweeps && active; ++s) {
-      const int32_t* Hc = (s & 1) ? H1 : H0;
-      int32_t* Hn = (s & 1) ? H0 : H1;
+    for (int sw = 0; sw < sweeps && active; ++sw) {
+      const int32_t* Hc = (sw & 1) ? H1 : H0;
+      int32_t* Hn = (sw & 1) ? H0 : H1;
 #pragma unroll
       for (int k = 0; k < R; ++k) {
         const int ly = chain * R + k;
@@ -285,64 +309,104 @@ k_push(Geo g, const int32_t* __restrict__ list, const int32_t* __restrict__ coun
       active = __syncthreads_or(any);
     }
 
+    long long tp = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const size_t i = base + (chain * R + k) * T + lx;
+      s.e[i] = e[k];
+      s.h[i] = h[k];
+      s.cu[i] = cu[k];
+      s.cd[i] = cd[k];
+      s.cl[i] = cl[k];
+      s.cr[i] = cr[k];
+      s.mask[i] = (uint8_t)pixel_mask(e[k], cu[k], cd[k], cl[k], cr[k]);
+      tp += e[k] > 0 ? e[k] : 0;
+    }
+    // edge outflow -> neighbour's pending slot (the receiving side is the
+    // opposite one); a nonzero outflow implies the arc and so the neighbour
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       const int ly = chain * R + k;
-      const size_t i = base + ly * T + lx;
-      e_g[i] = e[k];
-      h_g[i] = h[k];
-      cu_g[i] = cu[k];
-      cd_g[i] = cd[k];
-      cl_g[i] = cl[k];
-      cr_g[i] = cr[k];
-      // edge outflow -> neighbour's pending slot (the receiving side is the
-      // opposite one); a nonzero outflow implies the arc exists, so the
-      // neighbour tile exists
       if (ly == 0 && ty > 0 && BO[0][lx]) {
-        atomicAdd(&pend[((size_t)(t - g.tw) * 4 + 1) * T + lx], BO[0][lx]);
-        act[t - g.tw] = 1;
+        atomicAdd(&s.pend[((size_t)(t - g.tw) * 4 + 1) * T + lx], BO[0][lx]);
+        s.pflag[t - g.tw] = 1;
+        s.act[t - g.tw] = 1;
       }
       if (ly == T - 1 && ty < g.th - 1 && BO[1][lx]) {
-        atomicAdd(&pend[((size_t)(t + g.tw) * 4 + 0) * T + lx], BO[1][lx]);
-        act[t + g.tw] = 1;
+        atomicAdd(&s.pend[((size_t)(t + g.tw) * 4 + 0) * T + lx], BO[1][lx]);
+        s.pflag[t + g.tw] = 1;
+        s.act[t + g.tw] = 1;
       }
       if (lx == 0 && tx > 0 && BO[2][ly]) {
-        atomicAdd(&pend[((size_t)(t - 1) * 4 + 3) * T + ly], BO[2][ly]);
-        act[t - 1] = 1;
+        atomicAdd(&s.pend[((size_t)(t - 1) * 4 + 3) * T + ly], BO[2][ly]);
+        s.pflag[t - 1] = 1;
+        s.act[t - 1] = 1;
       }
       if (lx == T - 1 && tx < g.tw - 1 && BO[3][ly]) {
-        atomicAdd(&pend[((size_t)(t + 1) * 4 + 2) * T + ly], BO[3][ly]);
-        act[t + 1] = 1;
+        atomicAdd(&s.pend[((size_t)(t + 1) * 4 + 2) * T + ly], BO[3][ly]);
+        s.pflag[t + 1] = 1;
+        s.act[t + 1] = 1;
       }
     }
-    if (threadIdx.x == 0 && active) act[t] = 1;
-    __syncthreads();
+    tp = block_sum64<NT>(tp, red);
+    if (threadIdx.x == 0) {
+      s.tpos[t] = tp;
+      if (active) s.act[t] = 1;
+    }
+  }
+  if (FIRST) {
+    smin = block_sum64<NT>(smin, red);
+    spos = block_sum64<NT>(spos, red);
+    if (threadIdx.x == 0) {
+      const int slot = blockIdx.x % kSumSlots;
+      if (smin) atomicAdd(&s.sums[SUM_MIN * kSumSlots + slot], (unsigned long long)smin);
+      if (spos) atomicAdd(&s.sums[SUM_POS0 * kSumSlots + slot], (unsigned long long)spos);
+    }
   }
 }
 
-// Folds every pending slot into e and the reverse capacity (between
-// launches, when no tile is resident).  One thread per slot; corner pixels
-// own two slots, hence the atomic on e.
+// Folds the pending inflow of every flagged tile into its state (between
+// launches, when no tile is resident): e, the reverse capacity, the mask
+// byte and the tile's positive-excess sum.  One block of kFoldNT threads per
+// tile (unflagged tiles exit at once); one owner thread per edge pixel
+// (threads < 4T - 4 walk the perimeter), so corners are not raced.
+constexpr int kFoldNT = 256;
+
 template <int T>
-__global__ void k_apply_pending(Geo g, int32_t* pend, int32_t* e_g, int32_t* cu_g, int32_t* cd_g,
-                                int32_t* cl_g, int32_t* cr_g) {
-  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= (size_t)g.ntiles * 4 * T) return;
-  const int v = pend[s];
-  if (!v) return;
-  pend[s] = 0;
-  const int k = s % T;
-  const int side = (s / T) % 4;
-  const int t = (int)(s / (4 * T));
-  const size_t base = tbase<T>(t);
-  size_t i;
-  int32_t* cap;
-  if (side == 0) { i = base + k; cap = cu_g; }
-  else if (side == 1) { i = base + (T - 1) * T + k; cap = cd_g; }
-  else if (side == 2) { i = base + k * T; cap = cl_g; }
-  else { i = base + k * T + T - 1; cap = cr_g; }
-  atomicAdd(&e_g[i], v);
-  cap[i] += v;
+__global__ void __launch_bounds__(kFoldNT) k_fold(Geo g, CutDev s) {
+  static_assert(4 * T - 4 <= kFoldNT, "perimeter owners");
+  const int t = blockIdx.x;
+  if (t >= g.ntiles || !s.pflag[t]) return;
+  const int j = threadIdx.x;
+  if (j == 0) s.pflag[t] = 0;
+  if (j >= 4 * T - 4) return;
+  int x, y;
+  if (j < T) { x = j; y = 0; }
+  else if (j < 2 * T) { x = j - T; y = T - 1; }
+  else if (j < 3 * T - 2) { x = 0; y = j - 2 * T + 1; }
+  else { x = T - 1; y = j - (3 * T - 2) + 1; }
+  const size_t i = tbase<T>(t) + y * T + x;
+  int32_t* pe = s.pend + (size_t)t * 4 * T;
+  const int vu = y == 0 ? pe[0 * T + x] : 0;
+  const int vd = y == T - 1 ? pe[1 * T + x] : 0;
+  const int vl = x == 0 ? pe[2 * T + y] : 0;
+  const int vr = x == T - 1 ? pe[3 * T + y] : 0;
+  if ((vu | vd | vl | vr) == 0) return;
+  const int e0 = s.e[i];
+  const int e1 = e0 + vu + vd + vl + vr;
+  const int cu = s.cu[i] + vu, cd = s.cd[i] + vd, cl = s.cl[i] + vl, cr = s.cr[i] + vr;
+  s.e[i] = e1;
+  s.cu[i] = cu;
+  s.cd[i] = cd;
+  s.cl[i] = cl;
+  s.cr[i] = cr;
+  s.mask[i] = (uint8_t)pixel_mask(e1, cu, cd, cl, cr);
+  if (vu) pe[0 * T + x] = 0;
+  if (vd) pe[1 * T + x] = 0;
+  if (vl) pe[2 * T + y] = 0;
+  if (vr) pe[3 * T + y] = 0;
+  const long long delta = (long long)(e1 > 0 ? e1 : 0) - (e0 > 0 ? e0 : 0);
+  if (delta) atomicAdd(reinterpret_cast<unsigned long long*>(&s.tpos[t]), (unsigned long long)delta);
 }
 
 // ------------------------------------------------------ global relabel
@@ -389,28 +453,27 @@ __device__ __forceinline__ void bfs_relax(int (&d)[T * T / NT], const int (&m)[T
   }
 }
 
-// Writes halo distances of tile t into D0/D1 (threads < 4T).  `init`:
-// a neighbour pixel counts as 0 if it is a sink (e < 0), MAX otherwise;
+// Writes halo distances of tile t into D0/D1 (threads < 4T).  `init`: a
+// neighbour pixel counts as 0 if it is a sink (mask bit), MAX otherwise;
 // else its current distance from h_g.
 template <int T>
-__device__ __forceinline__ void load_halo_dist(const Geo& g, int t, const int32_t* e_g,
+__device__ __forceinline__ void load_halo_dist(const Geo& g, int t, const uint8_t* mask_g,
                                                const int32_t* h_g, bool init, int32_t* D0,
                                                int32_t* D1) {
   constexpr int HP = T + 2;
   if (threadIdx.x >= 4 * T) return;
   const int tx = t % g.tw, ty = t / g.tw;
   const int side = threadIdx.x / T, j = threadIdx.x % T;
-  int nb = -1;
-  size_t off = 0;
+  int nb = -1, nx = 0, ny = 0;  // neighbour tile and the pixel inside it
   int idx;
-  if (side == 0) { if (ty > 0) { nb = t - g.tw; off = (T - 1) * T + j; } idx = j + 1; }
-  else if (side == 1) { if (ty < g.th - 1) { nb = t + g.tw; off = j; } idx = (T + 1) * HP + j + 1; }
-  else if (side == 2) { if (tx > 0) { nb = t - 1; off = j * T + T - 1; } idx = (j + 1) * HP; }
-  else { if (tx < g.tw - 1) { nb = t + 1; off = j * T; } idx = (j + 1) * HP + T + 1; }
+  if (side == 0) { if (ty > 0) { nb = t - g.tw; nx = j; ny = T - 1; } idx = j + 1; }
+  else if (side == 1) { if (ty < g.th - 1) { nb = t + g.tw; nx = j; ny = 0; } idx = (T + 1) * HP + j + 1; }
+  else if (side == 2) { if (tx > 0) { nb = t - 1; nx = T - 1; ny = j; } idx = (j + 1) * HP; }
+  else { if (tx < g.tw - 1) { nb = t + 1; nx = 0; ny = j; } idx = (j + 1) * HP + T + 1; }
   int v = g.maxh;
   if (nb >= 0) {
-    const size_t q = tbase<T>(nb) + off;
-    v = init ? (e_g[q] < 0 ? 0 : g.maxh) : h_g[q];
+    const size_t q = tbase<T>(nb) + ny * T + nx;
+    v = init ? ((mask_g[q] & M_SINK) ? 0 : g.maxh) : h_g[q];
   }
   D0[idx] = v;
   D1[idx] = v;
@@ -427,58 +490,67 @@ __device__ __forceinline__ void flag_edge_neighbours(const Geo& g, int t, int lx
   if (lx == T - 1 && tx < g.tw - 1) flags[t + 1] = 1;
 }
 
-// First pass over all tiles: residual masks, seeds, in-tile distances.
+// First pass over all tiles, from the residual masks (pending inflow has
+// been folded): seeds, in-tile distances, and the completion test
+// (check_completion, graphcut.cl:20-60) for the tile:
+//   act[t]    = some pixel has e > 0 and a finite distance
+//   haspos[t] = some pixel has e > 0
+// Later rounds can only lower distances, so they can only switch act on.
 template <int T, int NT>
 __global__ void __launch_bounds__(NT)
-k_bfs_init(Geo g, const int32_t* __restrict__ e_g, const int32_t* __restrict__ cu_g,
-           const int32_t* __restrict__ cd_g, const int32_t* __restrict__ cl_g,
-           const int32_t* __restrict__ cr_g, int32_t* h_g, uint8_t* mask_g, int32_t* bflag,
-           int32_t* haspos) {
+k_bfs_init(Geo g, const uint8_t* __restrict__ mask_g, int32_t* h_g, int32_t* bflag,
+           int32_t* haspos, int32_t* act) {
   constexpr int R = T * T / NT;
   constexpr int HP = T + 2;
   __shared__ int32_t D0[HP * HP], D1[HP * HP];
-  const int t = blockIdx.x;
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
-  const size_t base = tbase<T>(t);
-  int d[R], m[R];
-  int any_pos = 0;
+  for (int t = blockIdx.x; t < g.ntiles; t += gridDim.x) {
+    const size_t base = tbase<T>(t);
+    int d[R], m[R];
+    int any_pos = 0;
 #pragma unroll
-  for (int k = 0; k < R; ++k) {
-    const int ly = chain * R + k;
-    const size_t i = base + ly * T + lx;
-    const int ev = e_g[i];
-    m[k] = (cu_g[i] > 0 ? M_UP : 0) | (cd_g[i] > 0 ? M_DOWN : 0) | (cl_g[i] > 0 ? M_LEFT : 0) |
-           (cr_g[i] > 0 ? M_RIGHT : 0) | (ev < 0 ? M_SINK : 0) | (ev > 0 ? M_EXCESS : 0);
-    d[k] = ev < 0 ? 0 : g.maxh;
-    D0[(ly + 1) * HP + lx + 1] = d[k];
-    any_pos |= ev > 0;
-  }
-  load_halo_dist<T>(g, t, e_g, h_g, true, D0, D1);
-  __syncthreads();
-  bfs_relax<T, NT>(d, m, D0, D1);
+    for (int k = 0; k < R; ++k) m[k] = mask_g[base + (chain * R + k) * T + lx];
 #pragma unroll
-  for (int k = 0; k < R; ++k) {
-    const int ly = chain * R + k;
-    const size_t i = base + ly * T + lx;
-    h_g[i] = d[k];
-    mask_g[i] = (uint8_t)m[k];
-    // neighbours assumed MAX for this non-sink pixel; a finite distance is news
-    if (d[k] < g.maxh && !(m[k] & M_SINK)) flag_edge_neighbours<T>(g, t, lx, ly, bflag);
+    for (int k = 0; k < R; ++k) {
+      const int ly = chain * R + k;
+      d[k] = (m[k] & M_SINK) ? 0 : g.maxh;
+      D0[(ly + 1) * HP + lx + 1] = d[k];
+      any_pos |= (m[k] & M_EXCESS) != 0;
+    }
+    load_halo_dist<T>(g, t, mask_g, h_g, true, D0, D1);
+    __syncthreads();
+    bfs_relax<T, NT>(d, m, D0, D1);
+    int any_act = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int ly = chain * R + k;
+      h_g[base + ly * T + lx] = d[k];
+      any_act |= (m[k] & M_EXCESS) && d[k] < g.maxh;
+      // neighbours assumed MAX for this non-sink pixel; a finite distance is news
+      if (d[k] < g.maxh && !(m[k] & M_SINK)) flag_edge_neighbours<T>(g, t, lx, ly, bflag);
+    }
+    any_pos = __syncthreads_or(any_pos);
+    any_act = __syncthreads_or(any_act);
+    if (threadIdx.x == 0) {
+      haspos[t] = any_pos;
+      act[t] = any_act;
+    }
   }
-  any_pos = __syncthreads_or(any_pos);
-  if (threadIdx.x == 0) haspos[t] = any_pos;
 }
 
 // Later passes over flagged tiles, from the stored distances.
 template <int T, int NT>
 __global__ void __launch_bounds__(NT)
 k_bfs_round(Geo g, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
-            int32_t* h_g, const uint8_t* __restrict__ mask_g, int32_t* bflag) {
+            int32_t* h_g, const uint8_t* __restrict__ mask_g, int32_t* bflag, int32_t* act,
+            unsigned long long* sums) {
   constexpr int R = T * T / NT;
   constexpr int HP = T + 2;
   __shared__ int32_t D0[HP * HP], D1[HP * HP];
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
   const int n = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0)
+    atomicAdd(&sums[TALLY_BFS * kSumSlots], (unsigned long long)n);
   for (int li = blockIdx.x; li < n; li += gridDim.x) {
     const int t = list[li];
     const size_t base = tbase<T>(t);
@@ -494,48 +566,19 @@ k_bfs_round(Geo g, const int32_t* __restrict__ list, const int32_t* __restrict__
     load_halo_dist<T>(g, t, nullptr, h_g, false, D0, D1);
     __syncthreads();
     bfs_relax<T, NT>(d, m, D0, D1);
+    int any_act = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       const int ly = chain * R + k;
       if (d[k] < d0[k]) {
         h_g[base + ly * T + lx] = d[k];
         flag_edge_neighbours<T>(g, t, lx, ly, bflag);
+        any_act |= (m[k] & M_EXCESS) != 0;
       }
     }
-    __syncthreads();
+    any_act = __syncthreads_or(any_act);
+    if (threadIdx.x == 0 && any_act) act[t] = 1;
   }
-}
-
-// Completion (check_completion, graphcut.cl:20-60) over tiles with excess:
-// act[t] = any pixel with e > 0 and h < MAX; sums[2] += sum max(e, 0).
-template <int T, int NT>
-__global__ void __launch_bounds__(NT)
-k_check(Geo g, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
-        const int32_t* __restrict__ e_g, const int32_t* __restrict__ h_g, int32_t* act,
-        unsigned long long* sums) {
-  constexpr int R = T * T / NT;
-  __shared__ long long red[NT / kWave];
-  const int lx = threadIdx.x % T, chain = threadIdx.x / T;
-  const int n = *count;
-  long long pos = 0;
-  for (int li = blockIdx.x; li < n; li += gridDim.x) {
-    const int t = list[li];
-    const size_t base = tbase<T>(t);
-    int any = 0;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const size_t i = base + (chain * R + k) * T + lx;
-      const int ev = e_g[i];
-      if (ev > 0) {
-        pos += ev;
-        any |= h_g[i] < g.maxh;
-      }
-    }
-    any = __syncthreads_or(any);
-    if (threadIdx.x == 0 && any) act[t] = 1;
-  }
-  pos = block_sum64<NT>(pos, red);
-  if (threadIdx.x == 0 && pos) atomicAdd(&sums[2], (unsigned long long)pos);
 }
 
 // ------------------------------------------------------ minimal source set
@@ -617,12 +660,21 @@ __device__ __forceinline__ void flag_out_arcs(const Geo& g, int t, int lx, int l
 template <int T, int NT>
 __global__ void __launch_bounds__(NT)
 k_flood(Geo g, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
-        const uint8_t* __restrict__ mask_g, uint8_t* reach_g, int32_t* fflag, int first) {
+        const uint8_t* __restrict__ mask_g, uint8_t* reach_g, int32_t* fflag, int first,
+        unsigned long long* sums, const long long* __restrict__ tpos) {
   constexpr int R = T * T / NT;
   constexpr int HP = T + 2;
   __shared__ int32_t RM[HP * HP];
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
   const int n = *count;
+  if (!first && blockIdx.x == 0 && threadIdx.x == 0 && n > 0)
+    atomicAdd(&sums[TALLY_FLOOD * kSumSlots], (unsigned long long)n);
+  // first pass (over the tiles holding excess): the final sum of max(e, 0)
+  if (first && threadIdx.x == 0) {
+    long long p = 0;
+    for (int li = blockIdx.x; li < n; li += gridDim.x) p += tpos[list[li]];
+    if (p) atomicAdd(&sums[SUM_POSF * kSumSlots + blockIdx.x % kSumSlots], (unsigned long long)p);
+  }
   for (int li = blockIdx.x; li < n; li += gridDim.x) {
     const int t = list[li];
     const size_t base = tbase<T>(t);

@@ -84,29 +84,62 @@ __global__ void __launch_bounds__(kMultiNT) k_scan_apply(int32_t* a, int n, cons
 // ------------------------------------------------------------ compaction
 
 // Compacts indices of [lo, hi) satisfying pred into list[carry...]; returns
-// the final carry.  One element per thread per step: a wave's offsets come
-// from its ballot mask, a block's from the per-wave counts in LDS.
-template <int NT>
+// the final carry.  Each thread owns IT consecutive elements per step and
+// keeps their predicate bits in one register; the block's offsets come from
+// one exclusive scan of the per-thread counts, whose wave part is a 64-lane
+// shuffle scan.
+template <int NT, int IT>
 __device__ int block_compact_range(const Pred& pred, int lo, int hi, int carry,
-                                   int32_t* list, int32_t* clear, int* wcnt) {
-  const int wid = threadIdx.x / kWave;
-  for (int base = lo; base < hi; base += NT) {
-    const int i = base + threadIdx.x;
-    const bool in = i < hi;
-    const bool f = in && pred.eval(i);
-    const unsigned long long m = __ballot(f);
-    if (lane_id() == 0) wcnt[wid] = __popcll(m);
-    __syncthreads();
-    int off = carry, total = 0;
+                                   int32_t* list, int32_t* clear, int* wsum) {
+  static_assert(IT % 4 == 0 && IT <= 32, "IT: whole int4 groups, one 32-bit mask");
+  for (int base = lo; base < hi; base += NT * IT) {
+    const int i0 = base + IT * threadIdx.x;
+    // all loads first (independent, so they overlap), then the predicate
+    int v1[IT], v2[IT];
+    const bool logical = pred.kind == Pred::LOGICAL;
+    if (i0 + IT <= hi && ((reinterpret_cast<uintptr_t>(pred.t1 + i0) & 15) == 0) &&
+        (!logical || (reinterpret_cast<uintptr_t>(pred.t2 + i0) & 15) == 0)) {
 #pragma unroll
-    for (int w = 0; w < NT / kWave; ++w) {
-      int c = wcnt[w];
-      off += (w < wid) ? c : 0;
-      total += c;
+      for (int j = 0; j < IT; j += 4) {
+        const int4 a4 = *reinterpret_cast<const int4*>(pred.t1 + i0 + j);
+        v1[j] = a4.x; v1[j + 1] = a4.y; v1[j + 2] = a4.z; v1[j + 3] = a4.w;
+        if (logical) {
+          const int4 b4 = *reinterpret_cast<const int4*>(pred.t2 + i0 + j);
+          v2[j] = b4.x; v2[j + 1] = b4.y; v2[j + 2] = b4.z; v2[j + 3] = b4.w;
+        } else {
+          v2[j] = v2[j + 1] = v2[j + 2] = v2[j + 3] = 0;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < IT; ++j) {
+        const bool in = i0 + j < hi;
+        v1[j] = in ? pred.t1[i0 + j] : 0;
+        v2[j] = (in && logical) ? pred.t2[i0 + j] : 0;
+      }
     }
-    if (f) list[off + __popcll(m & lanemask_lt())] = i;
-    if (clear && in) clear[i] = 0;
-    __syncthreads();
+    unsigned bits = 0;
+#pragma unroll
+    for (int j = 0; j < IT; ++j)
+      if (i0 + j < hi && pred.eval_values(v1[j], v2[j])) bits |= 1u << j;
+    int total = 0;
+    int off = block_excl_scan<NT>(__popc(bits), wsum, &total) + carry;
+    while (bits) {
+      const int j = __ffs(bits) - 1;
+      bits &= bits - 1;
+      list[off++] = i0 + j;
+    }
+    if (clear) {
+      if (i0 + IT <= hi && (reinterpret_cast<uintptr_t>(clear + i0) & 15) == 0) {
+#pragma unroll
+        for (int j = 0; j < IT; j += 4)
+          *reinterpret_cast<int4*>(clear + i0 + j) = make_int4(0, 0, 0, 0);
+      } else {
+#pragma unroll
+        for (int j = 0; j < IT; ++j)
+          if (i0 + j < hi) clear[i0 + j] = 0;
+      }
+    }
     carry += total;
   }
   return carry;
@@ -116,7 +149,7 @@ template <int NT>
 __global__ void __launch_bounds__(NT) k_compact_single(Pred pred, int n, int32_t* list,
                                                       int32_t* count, int32_t* clear) {
   __shared__ int wcnt[NT / kWave];
-  int c = block_compact_range<NT>(pred, 0, n, 0, list, clear, wcnt);
+  int c = block_compact_range<NT, 16>(pred, 0, n, 0, list, clear, wcnt);
   if (threadIdx.x == 0) *count = c;
 }
 
@@ -145,7 +178,7 @@ __global__ void __launch_bounds__(kMultiNT) k_compact_write(Pred pred, int n, co
   __shared__ int wcnt[kMultiNT / kWave];
   const int lo = blockIdx.x * kMultiChunk;
   const int hi = min(n, lo + kMultiChunk);
-  block_compact_range<kMultiNT>(pred, lo, hi, part[blockIdx.x], list, clear, wcnt);
+  block_compact_range<kMultiNT, 16>(pred, lo, hi, part[blockIdx.x], list, clear, wcnt);
 }
 
 // ------------------------------------------------------------ tile kernels

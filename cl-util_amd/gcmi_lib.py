@@ -81,7 +81,12 @@ SIGNATURES = {
     "gcmi_cut_stats": [_vp, _i64p, _int],
     "gcmi_cut_heights": [_vp, _vp],
     "gcmi_cut_geometry": [_vp, ctypes.POINTER(_int)],
+    "gcmi_cut_profile": [_vp, _int],
+    "gcmi_cut_kernel_times": [_vp, ctypes.POINTER(ctypes.c_double), _i64p, _int],
 }
+
+KERNEL_CLASSES = ("push_first", "push", "fold", "bfs_init", "bfs_round", "flood",
+                  "labels", "compact")
 _RESTYPE = {"gcmi_last_error": ctypes.c_char_p, "gcmi_version": ctypes.c_char_p}
 
 _lib = None

@@ -165,6 +165,16 @@ int gcmi_cut_stats(gcmi_cut *cut, int64_t *out, int n);
 int gcmi_cut_heights(gcmi_cut *cut, int32_t *h_out);
 /* Padded geometry: out[0]=padded width, out[1]=padded height, out[2]=tile. */
 int gcmi_cut_geometry(gcmi_cut *cut, int *out3);
+/* Per-kernel HIP-event timing of later runs (no reference counterpart; the
+ * reference sums OpenCL event profiles in PrefixSum.elapsed,
+ * PrefixSum.pyx:100-101).  Kernel classes, in order:
+ *   0 push_first (round 1, builds the state from the inputs)  1 push
+ *   2 fold  3 bfs_init  4 bfs_round  5 flood  6 labels  7 compact */
+#define GCMI_KERNEL_CLASSES 8
+int gcmi_cut_profile(gcmi_cut *cut, int enable);
+/* Summed milliseconds and launch counts per class for the last run. */
+int gcmi_cut_kernel_times(gcmi_cut *cut, double *ms_out, int64_t *launches_out,
+                          int n);
 
 #ifdef __cplusplus
 }

@@ -141,10 +141,12 @@ def brute_force(source, sink, right, down):
     return int(best), inter.reshape(H, W).astype(np.uint8)
 
 
-def scipy_cut(source, sink, right, down):
-    """Max-flow value by scipy's Dinic on the same graph (int32 CSR)."""
+def scipy_cut(source, sink, right, down, with_mask=False):
+    """Max-flow value by scipy's Dinic on the same graph (int32 CSR); with
+    with_mask, also the minimal source set (reachable from s in scipy's
+    final residual graph)."""
     from scipy.sparse import csr_matrix
-    from scipy.sparse.csgraph import maximum_flow
+    from scipy.sparse.csgraph import breadth_first_order, maximum_flow
     H, W = np.shape(source)
     n = W * H
     s, t = n, n + 1
@@ -170,7 +172,16 @@ def scipy_cut(source, sink, right, down):
                    shape=(n + 2, n + 2))
     g.sum_duplicates()
     res = maximum_flow(g, s, t, method="dinic")
-    return int(res.flow_value) + int(base.sum())
+    flow = int(res.flow_value) + int(base.sum())
+    if not with_mask:
+        return flow
+    resid = (g.astype(np.int64) - res.flow.astype(np.int64)).tocsr()
+    resid.data = (resid.data > 0).astype(np.int8)
+    resid.eliminate_zeros()
+    order = breadth_first_order(resid, s, directed=True, return_predecessors=False)
+    mask = np.zeros(n + 2, np.uint8)
+    mask[order] = 1
+    return flow, mask[:n].reshape(H, W)
 
 
 # ------------------------------------------------------------ primitives
