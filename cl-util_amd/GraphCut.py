@@ -37,6 +37,15 @@ def _i32(a, shape):
     return a
 
 
+def _edit_arrays(index, source, sink):
+    idx = np.ascontiguousarray(index, dtype=np.int32).reshape(-1)
+    s = np.ascontiguousarray(source, dtype=np.int32).reshape(-1)
+    t = np.ascontiguousarray(sink, dtype=np.int32).reshape(-1)
+    if not (idx.size == s.size == t.size):
+        raise ValueError("index, source and sink must have the same length")
+    return idx, s, t
+
+
 class GraphCut:
     lWorksize = (16, 16)
 
@@ -89,6 +98,14 @@ class GraphCut:
         """Keep pairwise weights for later cut(dExcess) calls."""
         s = self.shape
         self._right, self._down = _i32(right, s), _i32(down, s)
+
+    def edit(self, index, source, sink):
+        """Warm-start edit: new source/sink capacities for distinct pixels
+        (flat indices y*W + x).  The next cut() reflows from the current
+        residual graph; its result equals a from-scratch cut of the edited
+        capacities."""
+        idx, s, t = _edit_arrays(index, source, sink)
+        check(lib().gcmi_cut_edit(self._h, idx.size, host_ptr(idx), host_ptr(s), host_ptr(t)))
 
     def upload_device(self, d_source, d_sink, d_right, d_down):
         check(lib().gcmi_cut_upload_device(self._h, ptr(d_source), ptr(d_sink), ptr(d_right),

@@ -111,8 +111,13 @@ struct Pred {
   int op1, op2;
   int32_t a, b;
   int logical;
+  // Optional indirection: element i stands for id ids[i] -- the predicate
+  // reads t1[ids[i]] (NONZERO only), the output is ids[i] and clearing
+  // zeroes t1[ids[i]].
+  const int32_t* ids;
 
   __device__ __forceinline__ bool eval(int i) const {
+    if (ids) return t1[ids[i]] != 0;
     return eval_values(t1[i], kind == LOGICAL ? t2[i] : 0);
   }
 

@@ -1,0 +1,904 @@
+// Device kernels of the tiled push-relabel grid cut (gfx950).
+//
+// A pool holds one or more images (a plain cut is a pool of one).  Every
+// image is padded to whole T x T tiles and owns a contiguous range of pool
+// tiles; every per-pixel array is tile-blocked -- pool tile t occupies
+// [t*T*T, (t+1)*T*T), row-major inside the tile -- so one tile of one field
+// is T*T contiguous int32 (whole 128-byte lines).  Row-major inputs and
+// labels of image i live at pixel offset pix0(i) of their pool arrays.
+//
+// Per-pixel state (int32): e (excess; < 0 means remaining sink capacity),
+// h (height / BFS distance), cu, cd, cl, cr (residual capacity of the arc to
+// the up / down / left / right neighbour), plus a residual-mask byte.  Flow
+// pushed across a tile edge is added atomically to a pending slot of the
+// receiving tile (pend[(t*4 + side)*T + k]) instead of into its state, which
+// another work-group may hold at the same time.
+//
+// Tile kernels take a list of pool tiles (a null list means positions are
+// tile numbers: the whole pool in order) and map blocks to list positions
+// XCD by XCD (xcd_pos), so that neighbouring tiles -- whose halos a block
+// reads -- are processed on the same XCD and share its L2.
+//
+// Reference counterparts: push_up/down/left/right + add_border
+// (graphcut.cl:114-536), relabel (:538-577), init_bfs / bfs_intratile /
+// bfs_intertile (:582-1016), check_completion (:20-60).
+#pragma once
+
+#include "gcmi_common.h"
+
+namespace gcmi {
+
+enum { D_UP = 0, D_DOWN = 1, D_LEFT = 2, D_RIGHT = 3 };
+enum { M_UP = 1, M_DOWN = 2, M_LEFT = 4, M_RIGHT = 8, M_SINK = 16, M_EXCESS = 32 };
+
+// Per-image 64-bit sums, each spread over kSumSlots addresses so that many
+// tiles do not serialise on one atomic.  Rows: sum of min(cs, ct); sum of
+// max(e0, 0); sum of max(e, 0) after the last relabel.
+constexpr int kSumSlots = 64;
+enum { SUM_MIN = 0, SUM_POS0 = 1, SUM_POSF = 2, SUM_ROWS = 3 };
+// Pool-wide tallies of tile passes (added by block 0 of each round launch).
+enum { TALLY_BFS = 0, TALLY_FLOOD = 1, TALLY_N = 2 };
+// Number of XCDs blocks are dealt to round-robin.
+constexpr int kXcds = 8;
+
+struct ImgDesc {
+  int W, H;        // image
+  int tw, th;      // tiles across / down
+  int tile0;       // first pool tile
+  int ntiles;      // tw * th
+  long long pix0;  // offset of the image's row-major pixels
+};
+
+struct Geo {
+  int32_t maxh;             // MAX_HEIGHT: pool capacity in pixels (> any distance)
+  int ntiles;               // pool capacity in tiles
+  const ImgDesc* img;       // per image
+  const int32_t* tile_img;  // per pool tile: its image
+};
+
+// Device pointers of a pool's state, passed by value.
+struct CutDev {
+  int32_t *e, *h, *cu, *cd, *cl, *cr;  // tile-blocked state
+  uint8_t* mask;                       // residual bits + sink/excess bits
+  uint8_t* reach;                      // flood result
+  int32_t* pend;                       // pending edge inflow, 4*T per tile
+  int32_t* pflag;                      // tile has (possibly) pending inflow
+  int32_t* act;                        // tile flagged for the next push round
+  int32_t* bflag;                      // tile flagged for another BFS pass
+  int32_t* fflag;                      // tile flagged for another flood pass
+  int32_t* haspos;                     // tile holds excess
+  long long* tpos;                     // per tile: sum of max(e, 0)
+  unsigned long long* sums;            // per image [SUM_ROWS][kSumSlots]
+  unsigned long long* tally;           // [TALLY_N]
+  int32_t *cs, *ct;                    // row-major unaries (edits rewrite them)
+  const int32_t *right, *down;         // row-major pairwise weights
+  uint8_t* labels;                     // row-major labels
+};
+
+// Block-uniform geometry of one pool tile.
+struct TG {
+  int t, img, tx, ty, tw, th, W, H;
+  long long pix0;
+};
+
+__device__ __forceinline__ TG tile_geo(const Geo& g, int t) {
+  TG r;
+  r.t = t;
+  r.img = g.tile_img[t];
+  const ImgDesc d = g.img[r.img];
+  const int lt = t - d.tile0;
+  r.tw = d.tw;
+  r.th = d.th;
+  r.tx = lt % d.tw;
+  r.ty = lt / d.tw;
+  r.W = d.W;
+  r.H = d.H;
+  r.pix0 = d.pix0;
+  return r;
+}
+
+template <int T>
+__device__ __forceinline__ size_t tbase(int t) {
+  return (size_t)t * (T * T);
+}
+
+// Blocks b, b+8, b+16, ... are dealt to the same XCD; give each XCD a
+// contiguous run of list positions so neighbouring tiles share its L2.
+__device__ __forceinline__ int xcd_pos(int b, int grid) {
+  const int x = b % kXcds, k = b / kXcds;
+  const int per = grid / kXcds, rem = grid % kXcds;
+  return x * per + min(x, rem) + k;
+}
+
+// Block-wide sum of a 64-bit value; result valid in thread 0.
+template <int NT>
+__device__ __forceinline__ long long block_sum64(long long v, long long* red) {
+  v = wave_sum64(v);
+  if (lane_id() == 0) red[threadIdx.x / kWave] = v;
+  __syncthreads();
+  long long s = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < NT / kWave; ++w) s += red[w];
+  __syncthreads();
+  return s;
+}
+
+// Thread-0 accumulator of per-image sums that flushes (one slotted atomic)
+// whenever the image changes.
+struct ImgAcc {
+  int img = -1;
+  long long v = 0;
+  __device__ __forceinline__ void add(unsigned long long* sums, int row, int im, long long x) {
+    if (im != img) {
+      flush(sums, row);
+      img = im;
+    }
+    v += x;
+  }
+  __device__ __forceinline__ void flush(unsigned long long* sums, int row) {
+    if (img >= 0 && v)
+      atomicAdd(&sums[((size_t)img * SUM_ROWS + row) * kSumSlots + blockIdx.x % kSumSlots],
+                (unsigned long long)v);
+    v = 0;
+  }
+};
+
+__device__ __forceinline__ int pixel_mask(int e, int cu, int cd, int cl, int cr) {
+  return (cu > 0 ? M_UP : 0) | (cd > 0 ? M_DOWN : 0) | (cl > 0 ? M_LEFT : 0) |
+         (cr > 0 ? M_RIGHT : 0) | (e < 0 ? M_SINK : 0) | (e > 0 ? M_EXCESS : 0);
+}
+
+// Halo position of thread j < 4T (side = j / T): the neighbour tile (-1 if
+// outside the image), the pixel inside it, the LDS index in a (T+2)^2
+// halo-padded array and the pixel's image coordinates.
+struct Halo {
+  int nb, nx, ny, idx, gx, gy;
+};
+
+template <int T>
+__device__ __forceinline__ Halo halo_slot(const TG& q, int j) {
+  constexpr int HP = T + 2;
+  const int side = j / T, k = j % T;
+  Halo h;
+  h.nb = -1;
+  h.nx = 0;
+  h.ny = 0;
+  if (side == 0) {
+    if (q.ty > 0) { h.nb = q.t - q.tw; h.nx = k; h.ny = T - 1; }
+    h.idx = k + 1;
+    h.gx = q.tx * T + k;
+    h.gy = q.ty * T - 1;
+  } else if (side == 1) {
+    if (q.ty < q.th - 1) { h.nb = q.t + q.tw; h.nx = k; h.ny = 0; }
+    h.idx = (T + 1) * HP + k + 1;
+    h.gx = q.tx * T + k;
+    h.gy = q.ty * T + T;
+  } else if (side == 2) {
+    if (q.tx > 0) { h.nb = q.t - 1; h.nx = T - 1; h.ny = k; }
+    h.idx = (k + 1) * HP;
+    h.gx = q.tx * T - 1;
+    h.gy = q.ty * T + k;
+  } else {
+    if (q.tx < q.tw - 1) { h.nb = q.t + 1; h.nx = 0; h.ny = k; }
+    h.idx = (k + 1) * HP + T + 1;
+    h.gx = q.tx * T + T;
+    h.gy = q.ty * T + k;
+  }
+  return h;
+}
+
+// ----------------------------------------------------------------- push
+// One residency of each listed tile: load state (FIRST: build it from the
+// row-major capacities), fold pending inflow, freeze the halo heights, run
+// up to `sweeps` synchronous push/relabel sweeps in LDS, write back state,
+// residual mask and the tile's positive-excess sum, and post edge outflow to
+// the neighbours' pending slots.
+//
+// FIRST (an image's first round, over all its tiles): e = cs - ct, caps from
+// the pairwise weights, h = 0 on sink pixels (e < 0) and 1 elsewhere -- a
+// valid labelling, since every residual arc then satisfies h(p) <= h(q) + 1
+// -- and the image's sums get sum min(cs, ct) and sum max(e0, 0).
+//
+// A sweep: every active pixel (e > 0, h < MAX) pushes min(rem, cap) along
+// each admissible arc (h(p) == h(q) + 1) in the order down, up, right, left
+// (the reference's pass order, GraphCut.pyx:278-315); inflows are gathered;
+// then every pixel that is still active and has no admissible arc relabels
+// to min(h(q) + 1) over residual arcs (MAX if none), all from the heights
+// of the sweep's start (graphcut.cl:538-577 is the same Jacobi relabel).
+// Heights are double-buffered in LDS, so a sweep needs two barriers.
+//
+// Pending protocol (no fences: an agent-scope fence writes back the whole
+// L2 on this part): a sender atomically adds its edge outflow to the
+// receiver's slot and sets pflag[receiver] and act[receiver] with plain
+// stores.  A resident receiver reads pflag as a hint and, if set, exchanges
+// its slots with 0; it never clears pflag.  Whatever lands in a slot after
+// that exchange stays there with pflag still set, and k_fold (a separate
+// launch, so every store of this kernel is visible) folds it and clears
+// the flag.  No inflow is lost or counted twice.
+template <int T, int NT, bool FIRST>
+__global__ void __launch_bounds__(NT)
+k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
+       int sweeps) {
+  constexpr int R = T * T / NT;
+  constexpr int HP = T + 2;
+  static_assert(4 * T <= NT, "halo loaders");
+  __shared__ int32_t H0[HP * HP], H1[HP * HP];
+  __shared__ int32_t F[4][T * T];
+  __shared__ int32_t BO[4][T];
+  __shared__ long long red[NT / kWave];
+  __shared__ int sflag;
+  const int lx = threadIdx.x % T, chain = threadIdx.x / T;
+  const int32_t MAXH = g.maxh;
+  const int n = *count;
+  const int rpos = xcd_pos(blockIdx.x, gridDim.x);
+  ImgAcc amin, apos;
+  for (int pos = rpos; pos < n; pos += gridDim.x) {
+    const TG q = tile_geo(g, list ? list[pos] : pos);
+    const int t = q.t;
+    const size_t base = tbase<T>(t);
+    int e[R], h[R], cu[R], cd[R], cl[R], cr[R];
+    long long smin = 0, spos = 0;
+    if (FIRST) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int x = q.tx * T + lx, y = q.ty * T + chain * R + k;
+        e[k] = 0; cu[k] = 0; cd[k] = 0; cl[k] = 0; cr[k] = 0;
+        if (x < q.W && y < q.H) {
+          const size_t p = q.pix0 + (size_t)y * q.W + x;
+          const int a = s.cs[p], b = s.ct[p];
+          cr[k] = x < q.W - 1 ? s.right[p] : 0;
+          cl[k] = x > 0 ? s.right[p - 1] : 0;
+          cd[k] = y < q.H - 1 ? s.down[p] : 0;
+          cu[k] = y > 0 ? s.down[p - q.W] : 0;
+          e[k] = a - b;
+          smin += min(a, b);
+          spos += e[k] > 0 ? e[k] : 0;
+        }
+        h[k] = e[k] < 0 ? 0 : 1;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const size_t i = base + (chain * R + k) * T + lx;
+        e[k] = s.e[i];
+        h[k] = s.h[i];
+        cu[k] = s.cu[i];
+        cd[k] = s.cd[i];
+        cl[k] = s.cl[i];
+        cr[k] = s.cr[i];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int c = (chain * R + k + 1) * HP + lx + 1;
+      H0[c] = h[k];
+      H1[c] = h[k];
+    }
+    if (threadIdx.x < 4 * T) {
+      const Halo hs = halo_slot<T>(q, threadIdx.x);
+      int v = MAXH;
+      if (hs.nb >= 0) {
+        if (FIRST) {
+          if (hs.gx < q.W && hs.gy < q.H) {
+            const size_t p = q.pix0 + (size_t)hs.gy * q.W + hs.gx;
+            v = s.cs[p] - s.ct[p] < 0 ? 0 : 1;
+          }
+        } else {
+          v = s.h[tbase<T>(hs.nb) + hs.ny * T + hs.nx];
+        }
+      }
+      H0[hs.idx] = v;
+      H1[hs.idx] = v;
+      BO[threadIdx.x / T][threadIdx.x % T] = 0;
+    }
+    if (threadIdx.x == 0) sflag = s.pflag[t];
+    __syncthreads();
+    if (sflag) {
+      // issue every exchange first, then use the results (one wait)
+      int vu[R], vd[R], vl[R], vr[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int ly = chain * R + k;
+        vu[k] = ly == 0 ? atomicExch(&s.pend[((size_t)t * 4 + 0) * T + lx], 0) : 0;
+        vd[k] = ly == T - 1 ? atomicExch(&s.pend[((size_t)t * 4 + 1) * T + lx], 0) : 0;
+        vl[k] = lx == 0 ? atomicExch(&s.pend[((size_t)t * 4 + 2) * T + ly], 0) : 0;
+        vr[k] = lx == T - 1 ? atomicExch(&s.pend[((size_t)t * 4 + 3) * T + ly], 0) : 0;
+      }
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        e[k] += vu[k] + vd[k] + vl[k] + vr[k];
+        cu[k] += vu[k];
+        cd[k] += vd[k];
+        cl[k] += vl[k];
+        cr[k] += vr[k];
+      }
+    }
+    int any0 = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) any0 |= e[k] > 0 && h[k] < MAXH;
+    int active = __syncthreads_or(any0);
+
+    for (int sw = 0; sw < sweeps && active; ++sw) {
+      const int32_t* Hc = (sw & 1) ? H1 : H0;
+      int32_t* Hn = (sw & 1) ? H0 : H1;
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int ly = chain * R + k;
+        const int i = ly * T + lx;
+        const int c = (ly + 1) * HP + lx + 1;
+        int fu = 0, fd = 0, fl = 0, fr = 0;
+        const int hv = h[k];
+        if (e[k] > 0 && hv < MAXH) {
+          int rem = e[k];
+          if (cd[k] > 0 && hv == Hc[c + HP] + 1) {
+            fd = min(rem, cd[k]);
+            rem -= fd;
+            cd[k] -= fd;
+          }
+          if (rem > 0 && cu[k] > 0 && hv == Hc[c - HP] + 1) {
+            fu = min(rem, cu[k]);
+            rem -= fu;
+            cu[k] -= fu;
+          }
+          if (rem > 0 && cr[k] > 0 && hv == Hc[c + 1] + 1) {
+            fr = min(rem, cr[k]);
+            rem -= fr;
+            cr[k] -= fr;
+          }
+          if (rem > 0 && cl[k] > 0 && hv == Hc[c - 1] + 1) {
+            fl = min(rem, cl[k]);
+            rem -= fl;
+            cl[k] -= fl;
+          }
+          e[k] = rem;
+          if (ly == 0) BO[0][lx] += fu;
+          if (ly == T - 1) BO[1][lx] += fd;
+          if (lx == 0) BO[2][ly] += fl;
+          if (lx == T - 1) BO[3][ly] += fr;
+        }
+        F[D_UP][i] = fu;
+        F[D_DOWN][i] = fd;
+        F[D_LEFT][i] = fl;
+        F[D_RIGHT][i] = fr;
+      }
+      __syncthreads();
+      int any = 0;
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int ly = chain * R + k;
+        const int i = ly * T + lx;
+        const int c = (ly + 1) * HP + lx + 1;
+        if (ly > 0) {
+          const int v = F[D_DOWN][i - T];
+          e[k] += v;
+          cu[k] += v;
+        }
+        if (ly < T - 1) {
+          const int v = F[D_UP][i + T];
+          e[k] += v;
+          cd[k] += v;
+        }
+        if (lx > 0) {
+          const int v = F[D_RIGHT][i - 1];
+          e[k] += v;
+          cl[k] += v;
+        }
+        if (lx < T - 1) {
+          const int v = F[D_LEFT][i + 1];
+          e[k] += v;
+          cr[k] += v;
+        }
+        const int hv = h[k];
+        if (e[k] > 0 && hv < MAXH) {
+          const int hd = Hc[c + HP], hu = Hc[c - HP], hr = Hc[c + 1], hl = Hc[c - 1];
+          const bool adm = (cd[k] > 0 && hv == hd + 1) || (cu[k] > 0 && hv == hu + 1) ||
+                           (cr[k] > 0 && hv == hr + 1) || (cl[k] > 0 && hv == hl + 1);
+          if (!adm) {
+            int nh = MAXH;
+            if (cd[k] > 0) nh = min(nh, hd + 1);
+            if (cu[k] > 0) nh = min(nh, hu + 1);
+            if (cr[k] > 0) nh = min(nh, hr + 1);
+            if (cl[k] > 0) nh = min(nh, hl + 1);
+            h[k] = nh;
+          }
+          any |= h[k] < MAXH;
+        }
+        Hn[c] = h[k];
+      }
+      active = __syncthreads_or(any);
+    }
+
+    long long tp = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const size_t i = base + (chain * R + k) * T + lx;
+      s.e[i] = e[k];
+      s.h[i] = h[k];
+      s.cu[i] = cu[k];
+      s.cd[i] = cd[k];
+      s.cl[i] = cl[k];
+      s.cr[i] = cr[k];
+      s.mask[i] = (uint8_t)pixel_mask(e[k], cu[k], cd[k], cl[k], cr[k]);
+      tp += e[k] > 0 ? e[k] : 0;
+    }
+    // edge outflow -> neighbour's pending slot (the receiving side is the
+    // opposite one); a nonzero outflow implies the arc and so the neighbour
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int ly = chain * R + k;
+      if (ly == 0 && q.ty > 0 && BO[0][lx]) {
+        atomicAdd(&s.pend[((size_t)(t - q.tw) * 4 + 1) * T + lx], BO[0][lx]);
+        s.pflag[t - q.tw] = 1;
+        s.act[t - q.tw] = 1;
+      }
+      if (ly == T - 1 && q.ty < q.th - 1 && BO[1][lx]) {
+        atomicAdd(&s.pend[((size_t)(t + q.tw) * 4 + 0) * T + lx], BO[1][lx]);
+        s.pflag[t + q.tw] = 1;
+        s.act[t + q.tw] = 1;
+      }
+      if (lx == 0 && q.tx > 0 && BO[2][ly]) {
+        atomicAdd(&s.pend[((size_t)(t - 1) * 4 + 3) * T + ly], BO[2][ly]);
+        s.pflag[t - 1] = 1;
+        s.act[t - 1] = 1;
+      }
+      if (lx == T - 1 && q.tx < q.tw - 1 && BO[3][ly]) {
+        atomicAdd(&s.pend[((size_t)(t + 1) * 4 + 2) * T + ly], BO[3][ly]);
+        s.pflag[t + 1] = 1;
+        s.act[t + 1] = 1;
+      }
+    }
+    tp = block_sum64<NT>(tp, red);
+    if (FIRST) {
+      smin = block_sum64<NT>(smin, red);
+      spos = block_sum64<NT>(spos, red);
+    }
+    if (threadIdx.x == 0) {
+      s.tpos[t] = tp;
+      if (active) s.act[t] = 1;
+      if (FIRST) {
+        amin.add(s.sums, SUM_MIN, q.img, smin);
+        apos.add(s.sums, SUM_POS0, q.img, spos);
+      }
+    }
+  }
+  if (FIRST && threadIdx.x == 0) {
+    amin.flush(s.sums, SUM_MIN);
+    apos.flush(s.sums, SUM_POS0);
+  }
+}
+
+// Folds the pending inflow of every flagged tile into its state (between
+// launches, when no tile is resident): e, the reverse capacity, the mask
+// byte and the tile's positive-excess sum.  One block of kFoldNT threads per
+// listed tile (unflagged tiles exit at once); one owner thread per edge
+// pixel (threads < 4T - 4 walk the perimeter), so corners are not raced.
+constexpr int kFoldNT = 256;
+
+template <int T>
+__device__ __forceinline__ void fold_edge_pixel(CutDev& s, int t, int j) {
+  int x, y;
+  if (j < T) { x = j; y = 0; }
+  else if (j < 2 * T) { x = j - T; y = T - 1; }
+  else if (j < 3 * T - 2) { x = 0; y = j - 2 * T + 1; }
+  else { x = T - 1; y = j - (3 * T - 2) + 1; }
+  const size_t i = tbase<T>(t) + y * T + x;
+  int32_t* pe = s.pend + (size_t)t * 4 * T;
+  const int vu = y == 0 ? pe[0 * T + x] : 0;
+  const int vd = y == T - 1 ? pe[1 * T + x] : 0;
+  const int vl = x == 0 ? pe[2 * T + y] : 0;
+  const int vr = x == T - 1 ? pe[3 * T + y] : 0;
+  if ((vu | vd | vl | vr) == 0) return;
+  const int e0 = s.e[i];
+  const int e1 = e0 + vu + vd + vl + vr;
+  const int cu = s.cu[i] + vu, cd = s.cd[i] + vd, cl = s.cl[i] + vl, cr = s.cr[i] + vr;
+  s.e[i] = e1;
+  s.cu[i] = cu;
+  s.cd[i] = cd;
+  s.cl[i] = cl;
+  s.cr[i] = cr;
+  s.mask[i] = (uint8_t)pixel_mask(e1, cu, cd, cl, cr);
+  if (vu) pe[0 * T + x] = 0;
+  if (vd) pe[1 * T + x] = 0;
+  if (vl) pe[2 * T + y] = 0;
+  if (vr) pe[3 * T + y] = 0;
+  const long long delta = (long long)(e1 > 0 ? e1 : 0) - (e0 > 0 ? e0 : 0);
+  if (delta)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&s.tpos[t]), (unsigned long long)delta);
+}
+
+template <int T>
+__global__ void __launch_bounds__(kFoldNT)
+k_fold(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count) {
+  static_assert(4 * T - 4 <= kFoldNT, "perimeter owners");
+  const int n = *count;
+  const int j = threadIdx.x;
+  for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
+    const int t = list ? list[pos] : pos;
+    const int flagged = s.pflag[t];
+    __syncthreads();  // every thread has read the flag before it is cleared
+    if (!flagged) continue;
+    if (j == 0) s.pflag[t] = 0;
+    if (j < 4 * T - 4) fold_edge_pixel<T>(s, t, j);
+  }
+}
+
+// ------------------------------------------------------ global relabel
+// Distance of every pixel to the sink set {e < 0} over residual arcs
+// (MAX if unreachable): what init_bfs / bfs_intratile / bfs_intertile
+// (graphcut.cl:582-1016) compute, without their 30-round cap.  Inside a
+// tile the Bellman-Ford relaxation runs in LDS to its fixed point; across
+// tiles, a tile whose edge distances fell flags its neighbours for another
+// round, until no tile is flagged.
+
+// Relaxes d[] (this thread's R pixels) to the in-tile fixed point given
+// halo values already in D0/D1.  Returns after the last barrier.
+template <int T, int NT>
+__device__ __forceinline__ void bfs_relax(int (&d)[T * T / NT], const int (&m)[T * T / NT],
+                                          int32_t* D0, int32_t* D1) {
+  constexpr int R = T * T / NT;
+  constexpr int HP = T + 2;
+  const int lx = threadIdx.x % T, chain = threadIdx.x / T;
+  int32_t* Dc = D0;
+  int32_t* Dn = D1;
+  for (;;) {
+    int changed = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int ly = chain * R + k;
+      const int c = (ly + 1) * HP + lx + 1;
+      int nd = d[k];
+      if (nd > 0) {
+        if (m[k] & M_UP) nd = min(nd, Dc[c - HP] + 1);
+        if (m[k] & M_DOWN) nd = min(nd, Dc[c + HP] + 1);
+        if (m[k] & M_LEFT) nd = min(nd, Dc[c - 1] + 1);
+        if (m[k] & M_RIGHT) nd = min(nd, Dc[c + 1] + 1);
+        if (nd < d[k]) {
+          d[k] = nd;
+          changed = 1;
+        }
+      }
+      Dn[c] = d[k];
+    }
+    if (!__syncthreads_or(changed)) break;
+    int32_t* tmp = Dc;
+    Dc = Dn;
+    Dn = tmp;
+  }
+}
+
+// Flags the neighbour tiles across the edges pixel (lx, ly) lies on.
+template <int T>
+__device__ __forceinline__ void flag_edge_neighbours(const TG& q, int lx, int ly,
+                                                     int32_t* flags) {
+  if (ly == 0 && q.ty > 0) flags[q.t - q.tw] = 1;
+  if (ly == T - 1 && q.ty < q.th - 1) flags[q.t + q.tw] = 1;
+  if (lx == 0 && q.tx > 0) flags[q.t - 1] = 1;
+  if (lx == T - 1 && q.tx < q.tw - 1) flags[q.t + 1] = 1;
+}
+
+// First pass over the listed tiles, from the residual masks (pending inflow
+// has been folded): seeds, in-tile distances, and the completion test
+// (check_completion, graphcut.cl:20-60) for the tile:
+//   act[t]    = some pixel has e > 0 and a finite distance
+//   haspos[t] = some pixel has e > 0
+// Halo pixels start as 0 (sink) or MAX; later rounds only lower distances,
+// so they can only switch act on.
+template <int T, int NT>
+__global__ void __launch_bounds__(NT)
+k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count) {
+  constexpr int R = T * T / NT;
+  constexpr int HP = T + 2;
+  __shared__ int32_t D0[HP * HP], D1[HP * HP];
+  const int lx = threadIdx.x % T, chain = threadIdx.x / T;
+  const int n = *count;
+  for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
+    const TG q = tile_geo(g, list ? list[pos] : pos);
+    const size_t base = tbase<T>(q.t);
+    int d[R], m[R];
+    int any_pos = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) m[k] = s.mask[base + (chain * R + k) * T + lx];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int ly = chain * R + k;
+      d[k] = (m[k] & M_SINK) ? 0 : g.maxh;
+      D0[(ly + 1) * HP + lx + 1] = d[k];
+      any_pos |= (m[k] & M_EXCESS) != 0;
+    }
+    if (threadIdx.x < 4 * T) {
+      const Halo hs = halo_slot<T>(q, threadIdx.x);
+      int v = g.maxh;
+      if (hs.nb >= 0 && (s.mask[tbase<T>(hs.nb) + hs.ny * T + hs.nx] & M_SINK)) v = 0;
+      D0[hs.idx] = v;
+      D1[hs.idx] = v;
+    }
+    __syncthreads();
+    bfs_relax<T, NT>(d, m, D0, D1);
+    int any_act = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int ly = chain * R + k;
+      s.h[base + ly * T + lx] = d[k];
+      any_act |= (m[k] & M_EXCESS) && d[k] < g.maxh;
+      // neighbours assumed MAX for this non-sink pixel; a finite distance is news
+      if (d[k] < g.maxh && !(m[k] & M_SINK)) flag_edge_neighbours<T>(q, lx, ly, s.bflag);
+    }
+    any_pos = __syncthreads_or(any_pos);
+    any_act = __syncthreads_or(any_act);
+    if (threadIdx.x == 0) {
+      s.haspos[q.t] = any_pos;
+      s.act[q.t] = any_act;
+    }
+  }
+}
+
+// Later passes over flagged tiles, from the stored distances.
+template <int T, int NT>
+__global__ void __launch_bounds__(NT)
+k_bfs_round(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count) {
+  constexpr int R = T * T / NT;
+  constexpr int HP = T + 2;
+  __shared__ int32_t D0[HP * HP], D1[HP * HP];
+  const int lx = threadIdx.x % T, chain = threadIdx.x / T;
+  const int n = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0)
+    atomicAdd(&s.tally[TALLY_BFS], (unsigned long long)n);
+  for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
+    const TG q = tile_geo(g, list ? list[pos] : pos);
+    const size_t base = tbase<T>(q.t);
+    int d[R], d0[R], m[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int ly = chain * R + k;
+      const size_t i = base + ly * T + lx;
+      d[k] = d0[k] = s.h[i];
+      m[k] = s.mask[i];
+      D0[(ly + 1) * HP + lx + 1] = d[k];
+    }
+    if (threadIdx.x < 4 * T) {
+      const Halo hs = halo_slot<T>(q, threadIdx.x);
+      const int v = hs.nb >= 0 ? s.h[tbase<T>(hs.nb) + hs.ny * T + hs.nx] : g.maxh;
+      D0[hs.idx] = v;
+      D1[hs.idx] = v;
+    }
+    __syncthreads();
+    bfs_relax<T, NT>(d, m, D0, D1);
+    int any_act = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int ly = chain * R + k;
+      if (d[k] < d0[k]) {
+        s.h[base + ly * T + lx] = d[k];
+        flag_edge_neighbours<T>(q, lx, ly, s.bflag);
+        any_act |= (m[k] & M_EXCESS) != 0;
+      }
+    }
+    any_act = __syncthreads_or(any_act);
+    if (threadIdx.x == 0 && any_act) s.act[q.t] = 1;
+  }
+}
+
+// ------------------------------------------------------ minimal source set
+// reach = pixels reachable over residual arcs from {e > 0}: the minimal
+// source set once no pixel with e > 0 can reach a sink.  Same tile scheme
+// as the BFS: in-tile fixed point in LDS, flags across edges.
+
+// Relaxes r[] to the in-tile fixed point.  RM holds (reach | mask << 1) of
+// every tile pixel and halo pixel (halo: the neighbour's reach and mask).
+template <int T, int NT>
+__device__ __forceinline__ void flood_relax(int (&r)[T * T / NT], int32_t* RM) {
+  constexpr int R = T * T / NT;
+  constexpr int HP = T + 2;
+  const int lx = threadIdx.x % T, chain = threadIdx.x / T;
+  for (;;) {
+    int changed = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      if (r[k]) continue;
+      const int ly = chain * R + k;
+      const int c = (ly + 1) * HP + lx + 1;
+      // a neighbour q reaches p if q is reached and q's arc toward p is residual
+      const int up = RM[c - HP], dn = RM[c + HP], lf = RM[c - 1], rt = RM[c + 1];
+      if (((up & 1) && (up & (M_DOWN << 1))) || ((dn & 1) && (dn & (M_UP << 1))) ||
+          ((lf & 1) && (lf & (M_RIGHT << 1))) || ((rt & 1) && (rt & (M_LEFT << 1)))) {
+        r[k] = 1;
+        changed = 1;
+      }
+    }
+    // publish new reach bits (only ever set, so a reader racing the update
+    // sees 0 or 1, both consistent with the monotone fixed point)
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int ly = chain * R + k;
+      const int c = (ly + 1) * HP + lx + 1;
+      if (r[k]) RM[c] |= 1;
+    }
+    if (!__syncthreads_or(changed)) break;
+  }
+}
+
+// Flags the neighbour tiles that pixel (lx, ly) has a residual arc into.
+template <int T>
+__device__ __forceinline__ void flag_out_arcs(const TG& q, int lx, int ly, int m,
+                                              int32_t* flags) {
+  if (ly == 0 && q.ty > 0 && (m & M_UP)) flags[q.t - q.tw] = 1;
+  if (ly == T - 1 && q.ty < q.th - 1 && (m & M_DOWN)) flags[q.t + q.tw] = 1;
+  if (lx == 0 && q.tx > 0 && (m & M_LEFT)) flags[q.t - 1] = 1;
+  if (lx == T - 1 && q.tx < q.tw - 1 && (m & M_RIGHT)) flags[q.t + 1] = 1;
+}
+
+// first = true: over the tiles holding excess, seeds from the masks' excess
+// bit, and the images' final sum of max(e, 0) from the tile sums; else: from
+// the stored reach bytes of flagged tiles, with halo reach.
+template <int T, int NT>
+__global__ void __launch_bounds__(NT)
+k_flood(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
+        int first) {
+  constexpr int R = T * T / NT;
+  constexpr int HP = T + 2;
+  __shared__ int32_t RM[HP * HP];
+  const int lx = threadIdx.x % T, chain = threadIdx.x / T;
+  const int n = *count;
+  if (!first && blockIdx.x == 0 && threadIdx.x == 0 && n > 0)
+    atomicAdd(&s.tally[TALLY_FLOOD], (unsigned long long)n);
+  ImgAcc apos;
+  for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
+    const TG q = tile_geo(g, list ? list[pos] : pos);
+    const size_t base = tbase<T>(q.t);
+    if (first && threadIdx.x == 0) apos.add(s.sums, SUM_POSF, q.img, s.tpos[q.t]);
+    int r[R], r0[R], m[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int ly = chain * R + k;
+      const size_t i = base + ly * T + lx;
+      m[k] = s.mask[i];
+      r0[k] = first ? 0 : (s.reach[i] ? 1 : 0);
+      r[k] = r0[k] | ((m[k] & M_EXCESS) ? 1 : 0);
+      RM[(ly + 1) * HP + lx + 1] = (m[k] << 1) | r[k];
+    }
+    if (threadIdx.x < 4 * T) {
+      const Halo hs = halo_slot<T>(q, threadIdx.x);
+      int v = 0;
+      if (hs.nb >= 0) {
+        const size_t p = tbase<T>(hs.nb) + hs.ny * T + hs.nx;
+        v = (int)s.mask[p] << 1;
+        if (!first) v |= s.reach[p] ? 1 : 0;
+      }
+      RM[hs.idx] = v;
+    }
+    __syncthreads();
+    flood_relax<T, NT>(r, RM);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int ly = chain * R + k;
+      if (r[k] && !r0[k]) {
+        s.reach[base + ly * T + lx] = 1;
+        flag_out_arcs<T>(q, lx, ly, m[k], s.fflag);
+      }
+    }
+    __syncthreads();
+  }
+  if (first && threadIdx.x == 0) apos.flush(s.sums, SUM_POSF);
+}
+
+// Row-major labels of the listed tiles' images from the reach bytes (a
+// tile writes its own T x T block of its image).
+template <int T, int NT>
+__global__ void __launch_bounds__(NT)
+k_labels(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count) {
+  constexpr int R = T * T / NT;
+  const int lx = threadIdx.x % T, chain = threadIdx.x / T;
+  const int n = *count;
+  for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
+    const TG q = tile_geo(g, list ? list[pos] : pos);
+    const size_t base = tbase<T>(q.t);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int ly = chain * R + k;
+      const int x = q.tx * T + lx, y = q.ty * T + ly;
+      if (x < q.W && y < q.H) s.labels[q.pix0 + (size_t)y * q.W + x] = s.reach[base + ly * T + lx];
+    }
+  }
+}
+
+// Per listed tile: clear pending slots, pending flag, flags and reach (the
+// tile range may have held another image before).
+template <int T>
+__global__ void k_reset_tiles(CutDev s, const int32_t* __restrict__ list,
+                              const int32_t* __restrict__ count) {
+  const int n = *count;
+  for (int pos = blockIdx.x; pos < n; pos += gridDim.x) {
+    const int t = list ? list[pos] : pos;
+    for (int j = threadIdx.x; j < 4 * T; j += blockDim.x) s.pend[(size_t)t * 4 * T + j] = 0;
+    if (threadIdx.x == 0) {
+      s.pflag[t] = 0;
+      s.act[t] = 0;
+      s.bflag[t] = 0;
+      s.fflag[t] = 0;
+      s.haspos[t] = 0;
+    }
+  }
+}
+
+// Per listed tile: clear the reach bytes (before the flood).
+template <int T>
+__global__ void k_clear_reach(CutDev s, const int32_t* __restrict__ list,
+                              const int32_t* __restrict__ count) {
+  const int n = *count;
+  for (int pos = blockIdx.x; pos < n; pos += gridDim.x) {
+    uint32_t* r = reinterpret_cast<uint32_t*>(s.reach + tbase<T>(list ? list[pos] : pos));
+    for (int j = threadIdx.x; j < T * T / 4; j += blockDim.x) r[j] = 0;
+  }
+}
+
+// Tile-blocked int32 field of one image -> row-major padded image.
+template <int T>
+__global__ void k_untile(Geo g, int img, const int32_t* __restrict__ src, int32_t* out) {
+  const ImgDesc d = g.img[img];
+  const int PW = d.tw * T;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= PW) return;
+  const int t = d.tile0 + (y / T) * d.tw + x / T;
+  out[(size_t)y * PW + x] = src[tbase<T>(t) + (y % T) * T + (x % T)];
+}
+
+}  // namespace gcmi
+
+namespace gcmi {
+
+// ------------------------------------------------------------- bookkeeping
+
+// One selected image of a solve: its tiles go to sel[sel_off...] and, if
+// fresh_off >= 0 (first solve of the image), also to fresh[fresh_off...].
+struct Prep {
+  int img, sel_off, fresh_off, pad;
+};
+
+// Warm-start edit: new unaries (cs, ct) for n distinct pixels of image img
+// (row-major indices local to the image).  Updates the stored unaries and,
+// if the image holds solved state (warm), e by the change of cs - ct, the
+// mask's sink/excess bits, the tile's positive-excess sum and the image's
+// sums of min(cs, ct) and max(e0, 0).  Heights are left stale: the next
+// solve starts with a global relabel.
+template <int T>
+__global__ void __launch_bounds__(256)
+k_edit(Geo g, CutDev s, int img, int n, const int32_t* __restrict__ idx,
+       const int32_t* __restrict__ ncs, const int32_t* __restrict__ nct, int warm) {
+  __shared__ long long red[256 / kWave];
+  const ImgDesc d = g.img[img];
+  long long dmin = 0, dpos = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int p = idx[i];
+    const int x = p % d.W, y = p / d.W;
+    const size_t r = d.pix0 + (size_t)p;
+    const int a0 = s.cs[r], b0 = s.ct[r], a1 = ncs[i], b1 = nct[i];
+    s.cs[r] = a1;
+    s.ct[r] = b1;
+    if (!warm) continue;  // no state yet: the first round reads the unaries
+    const int t = d.tile0 + (y / T) * d.tw + x / T;
+    const size_t q = tbase<T>(t) + (y % T) * T + (x % T);
+    const int e0 = s.e[q];
+    const int e1 = e0 + (a1 - b1) - (a0 - b0);
+    s.e[q] = e1;
+    s.mask[q] = (uint8_t)((s.mask[q] & (M_UP | M_DOWN | M_LEFT | M_RIGHT)) |
+                          (e1 < 0 ? M_SINK : 0) | (e1 > 0 ? M_EXCESS : 0));
+    const long long dt = (long long)(e1 > 0 ? e1 : 0) - (e0 > 0 ? e0 : 0);
+    if (dt) atomicAdd(reinterpret_cast<unsigned long long*>(&s.tpos[t]), (unsigned long long)dt);
+    dmin += (long long)min(a1, b1) - min(a0, b0);
+    dpos += (long long)(a1 - b1 > 0 ? a1 - b1 : 0) - (a0 - b0 > 0 ? a0 - b0 : 0);
+  }
+  dmin = block_sum64<256>(dmin, red);
+  dpos = block_sum64<256>(dpos, red);
+  if (threadIdx.x == 0) {
+    const int slot = blockIdx.x % kSumSlots;
+    if (dmin)
+      atomicAdd(&s.sums[((size_t)img * SUM_ROWS + SUM_MIN) * kSumSlots + slot],
+                (unsigned long long)dmin);
+    if (dpos)
+      atomicAdd(&s.sums[((size_t)img * SUM_ROWS + SUM_POS0) * kSumSlots + slot],
+                (unsigned long long)dpos);
+  }
+}
+
+}  // namespace gcmi

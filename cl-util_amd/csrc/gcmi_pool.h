@@ -1,0 +1,176 @@
+// Host side of an image pool: device arrays shared by all images of the
+// pool, per-image tile/pixel ranges, and the solve driver that runs the
+// push-relabel rounds over any subset of the pool's images at once.
+//
+// A plain cut (GraphCut) is a pool of one image; a batch is a pool of many;
+// the serving loop keeps every resident session in one pool and solves the
+// sessions that have requests together.
+#pragma once
+
+#include <vector>
+
+#include "gcmi_kernels.h"
+
+namespace gcmi {
+
+constexpr int kDefaultSweeps = 4;  // in-tile sweeps per residency
+constexpr int kDefaultBfs = 1;     // push rounds per global relabel
+constexpr int kRoundBatch = 2;     // BFS / flood rounds launched per host check
+
+enum {
+  CNT_ACT = 0,
+  CNT_BFS = 1,
+  CNT_POS = 2,
+  CNT_FLOOD = 3,
+  CNT_SEL = 4,
+  CNT_FRESH = 5,
+  CNT_N = 8
+};
+
+enum {
+  STAT_ROUNDS = 0,
+  STAT_RESIDENCIES = 1,
+  STAT_GLOBAL_RELABELS = 2,
+  STAT_BFS_PASSES = 3,
+  STAT_FLOOD_PASSES = 4,
+  STAT_SYNCS = 5,
+  STAT_ALG_BYTES = 6,
+  STAT_TILES = 7,
+  STAT_N = 8
+};
+
+// Kernel classes timed by the optional HIP-event profiler.
+enum {
+  K_PUSH_FIRST = 0,
+  K_PUSH = 1,
+  K_FOLD = 2,
+  K_BFS_INIT = 3,
+  K_BFS_ROUND = 4,
+  K_FLOOD = 5,
+  K_LABELS = 6,
+  K_COMPACT = 7,
+  K_N = 8
+};
+
+// Per-launch HIP events on the pool's stream, summed per kernel class
+// after the solve's final synchronisation.
+struct Prof {
+  bool on = false;
+  int used = 0;
+  int cap = 0;
+  hipEvent_t* ev = nullptr;  // 2 per launch
+  int* cls = nullptr;
+  double ms[K_N] = {0};
+  int64_t launches[K_N] = {0};
+  void reset();
+  void begin(hipStream_t s, int k);
+  void end(hipStream_t s);
+  void collect();
+  void release();
+};
+
+struct Pool;
+
+// Kernel launchers for one tile size.
+struct PoolOps {
+  int tile;
+  int threads;
+  void (*push)(Pool*, int grid, int sweeps, bool first);
+  void (*fold)(Pool*, int grid);
+  void (*bfs_init)(Pool*, int grid);
+  void (*bfs_round)(Pool*, int grid);
+  void (*flood)(Pool*, int grid, bool first);
+  void (*labels)(Pool*, int grid);
+  void (*reset_tiles)(Pool*, int grid);
+  void (*clear_reach)(Pool*, int grid);
+  void (*edit)(Pool*, int img, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct);
+  void (*untile)(Pool*, int img, const int32_t* src, int32_t* out);
+};
+
+const PoolOps* pool_ops(int tile);
+
+struct PoolImage {
+  bool live = false;
+  int W = 0, H = 0, tw = 0, th = 0;
+  int tile0 = 0, ntiles = 0;
+  long long pix0 = 0;
+  int32_t max_pair = 0;  // largest pairwise weight uploaded
+  bool uploaded = false;  // capacities present
+  bool fresh = true;      // no solved state yet (or capacities re-uploaded)
+  bool edited = false;    // edited since the last solve: next solve is warm
+  bool solved = false;    // labels / flow valid
+  int64_t flow = 0;
+};
+
+// A first-fit allocator of [start, start + len) ranges.
+struct RangeAlloc {
+  std::vector<std::pair<long long, long long>> free_;  // (start, len), sorted
+  long long cap = 0;
+  void init(long long c);
+  long long alloc(long long len);  // -1 if no room
+  void release(long long start, long long len);
+  long long largest() const;
+};
+
+struct Pool {
+  const PoolOps* ops = nullptr;
+  int T = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int cap_tiles = 0;
+  long long cap_pix = 0;
+  int cap_imgs = 0;
+  Geo geo{};
+  CutDev dev{};
+  // device arrays (beyond those in dev)
+  ImgDesc* d_img = nullptr;
+  int32_t* tile_img = nullptr;
+  int32_t *sel = nullptr, *fresh = nullptr, *list = nullptr, *plist = nullptr, *alist = nullptr;
+  // the lists the kernels of the current solve read: null when the
+  // selection is the whole pool in tile order (positions are tiles)
+  int32_t *lsel = nullptr, *lfresh = nullptr;
+  int32_t* counts = nullptr;
+  int32_t* scratch = nullptr;
+  Prep* d_prep = nullptr;
+  long long* d_flows = nullptr;
+  int32_t* d_edit = nullptr;  // staging for edits: idx, cs, ct
+  int edit_cap = 0;
+  // pinned host buffers
+  int32_t* h_counts = nullptr;
+  Prep* h_prep = nullptr;
+  long long* h_flows = nullptr;
+  unsigned long long* h_tally = nullptr;
+  int32_t* h_edit = nullptr;
+  // host bookkeeping
+  std::vector<PoolImage> imgs;
+  RangeAlloc tiles_alloc, pix_alloc;
+  size_t device_bytes = 0;
+  int grid_cap = 2048;
+  int64_t stats[STAT_N] = {0};
+  Prof prof;
+};
+
+// Bytes of device memory a pool of the given capacity allocates.
+size_t pool_bytes(int tile, int cap_tiles, long long cap_pix, int cap_imgs);
+int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
+              hipStream_t stream);
+void pool_free(Pool* p);
+// Adds an image; returns its id in *out.  GCMI_EBUDGET if no room.
+int pool_add_image(Pool* p, int W, int H, int* out);
+int pool_remove_image(Pool* p, int img);
+int pool_upload(Pool* p, int img, const int32_t* cs, const int32_t* ct, const int32_t* right,
+                const int32_t* down, bool device_src);
+// Warm-start edit of n distinct pixels (row-major indices local to the
+// image) from host arrays.
+int pool_edit(Pool* p, int img, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct);
+// Solves the listed images together (fresh ones from their capacities,
+// edited ones warm).  flows_out (may be NULL) receives one flow per image.
+int pool_solve(Pool* p, const int* img_ids, int n, int sweeps, int bfs, int64_t max_rounds,
+               int64_t* flows_out);
+int pool_labels(Pool* p, int img, uint8_t* out, bool device_dst);
+int pool_heights(Pool* p, int img, int32_t* out);
+int validate_caps(const int32_t* cs, const int32_t* ct, const int32_t* right,
+                  const int32_t* down, size_t n, int32_t* max_pair);
+int tiles_for(int W, int H, int tile, int* tw, int* th);
+
+}  // namespace gcmi

@@ -1,0 +1,811 @@
+// Image pool: allocation, uploads, warm-start edits and the solve driver.
+//
+// Solve driver (GraphCut.cut, GraphCut.pyx:348-418), over the selected
+// images at once:
+//   prepare                  tile lists; zero the sums being recomputed
+//   fresh images: round 1    k_push<FIRST> builds state from the capacities
+//   repeat:
+//     fold                   pending edge inflow -> state
+//     global relabel         k_bfs_init + rounds until no tile is flagged;
+//                            act = tiles with excess that can reach a sink
+//     stop if act is empty
+//     push round             k_push over the active tiles
+//   minimal source set       flood from {e > 0}; labels; flow values
+// Lists are built on the device.  The host reads list lengths once per
+// relabel (the reference waits after every launch and reads a length per
+// tile-list build, GraphCut.pyx:169-323).
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "gcmi_pool.h"
+
+namespace gcmi {
+
+// ------------------------------------------------------------- profiler
+
+void Prof::reset() {
+  used = 0;
+  for (int k = 0; k < K_N; ++k) {
+    ms[k] = 0;
+    launches[k] = 0;
+  }
+}
+
+void Prof::begin(hipStream_t s, int k) {
+  if (used == cap) {
+    int ncap = cap ? cap * 2 : 256;
+    hipEvent_t* nev = (hipEvent_t*)realloc(ev, sizeof(hipEvent_t) * 2 * ncap);
+    if (nev) ev = nev;
+    int* ncls = (int*)realloc(cls, sizeof(int) * ncap);
+    if (ncls) cls = ncls;
+    if (!nev || !ncls) {
+      on = false;
+      return;
+    }
+    for (int i = 2 * cap; i < 2 * ncap; ++i) (void)hipEventCreate(&ev[i]);
+    cap = ncap;
+  }
+  cls[used] = k;
+  (void)hipEventRecord(ev[2 * used], s);
+}
+
+void Prof::end(hipStream_t s) {
+  (void)hipEventRecord(ev[2 * used + 1], s);
+  ++used;
+}
+
+void Prof::collect() {
+  for (int i = 0; i < used; ++i) {
+    float t = 0;
+    if (hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]) == hipSuccess) {
+      ms[cls[i]] += t;
+      launches[cls[i]]++;
+    }
+  }
+  used = 0;
+}
+
+void Prof::release() {
+  for (int i = 0; i < 2 * cap; ++i) (void)hipEventDestroy(ev[i]);
+  free(ev);
+  free(cls);
+  ev = nullptr;
+  cls = nullptr;
+  cap = used = 0;
+}
+
+// ------------------------------------------------------------- launchers
+
+__global__ void k_fill_i32(int32_t* p, long long n, int32_t v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_clear_listed(int32_t* flags, const int32_t* __restrict__ list,
+                               const int32_t* __restrict__ count) {
+  const int n = *count;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    flags[list[i]] = 0;
+}
+
+// Builds the solve's tile lists, zeroes the sums it recomputes (all rows of
+// a fresh image, the final-excess row of every image) and the tallies.
+__global__ void k_prepare(Geo g, CutDev s, const Prep* __restrict__ prep, int nsel_tiles,
+                          int nfresh_tiles, int32_t* sel, int32_t* fresh, int32_t* cnt_sel,
+                          int32_t* cnt_fresh) {
+  const Prep p = prep[blockIdx.x];
+  const ImgDesc d = g.img[p.img];
+  for (int j = threadIdx.x; j < d.ntiles; j += blockDim.x) {
+    sel[p.sel_off + j] = d.tile0 + j;
+    if (p.fresh_off >= 0) fresh[p.fresh_off + j] = d.tile0 + j;
+  }
+  for (int j = threadIdx.x; j < SUM_ROWS * kSumSlots; j += blockDim.x)
+    if (p.fresh_off >= 0 || j / kSumSlots == SUM_POSF)
+      s.sums[(size_t)p.img * SUM_ROWS * kSumSlots + j] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *cnt_sel = nsel_tiles;
+    *cnt_fresh = nfresh_tiles;
+    for (int k = 0; k < TALLY_N; ++k) s.tally[k] = 0;
+  }
+}
+
+// Flow value of each selected image: sum min(cs, ct) + sum max(e0, 0)
+// - sum max(e, 0) (one wave per image adds the slots).
+__global__ void k_flows(const unsigned long long* __restrict__ sums, const Prep* __restrict__ prep,
+                        long long* out) {
+  const int img = prep[blockIdx.x].img;
+  const unsigned long long* r = sums + (size_t)img * SUM_ROWS * kSumSlots;
+  long long v = 0;
+  for (int j = threadIdx.x; j < kSumSlots; j += blockDim.x)
+    v += (long long)r[SUM_MIN * kSumSlots + j] + (long long)r[SUM_POS0 * kSumSlots + j] -
+         (long long)r[SUM_POSF * kSumSlots + j];
+  v = wave_sum64(v);
+  if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
+
+template <int T, int NT>
+struct PoolKernels {
+  static void push(Pool* p, int grid, int sweeps, bool first) {
+    if (first)
+      hipLaunchKernelGGL((k_push<T, NT, true>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
+                         p->lfresh, p->counts + CNT_FRESH, sweeps);
+    else
+      hipLaunchKernelGGL((k_push<T, NT, false>), dim3(grid), dim3(NT), 0, p->stream, p->geo,
+                         p->dev, p->alist, p->counts + CNT_ACT, sweeps);
+  }
+  static void fold(Pool* p, int grid) {
+    hipLaunchKernelGGL((k_fold<T>), dim3(grid), dim3(kFoldNT), 0, p->stream, p->geo, p->dev,
+                       p->lsel, p->counts + CNT_SEL);
+  }
+  static void bfs_init(Pool* p, int grid) {
+    hipLaunchKernelGGL((k_bfs_init<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
+                       p->lsel, p->counts + CNT_SEL);
+  }
+  static void bfs_round(Pool* p, int grid) {
+    hipLaunchKernelGGL((k_bfs_round<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
+                       p->list, p->counts + CNT_BFS);
+  }
+  static void flood(Pool* p, int grid, bool first) {
+    hipLaunchKernelGGL((k_flood<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
+                       first ? p->plist : p->list, p->counts + (first ? CNT_POS : CNT_FLOOD),
+                       first ? 1 : 0);
+  }
+  static void labels(Pool* p, int grid) {
+    hipLaunchKernelGGL((k_labels<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
+                       p->lsel, p->counts + CNT_SEL);
+  }
+  static void reset_tiles(Pool* p, int grid) {
+    hipLaunchKernelGGL((k_reset_tiles<T>), dim3(grid), dim3(256), 0, p->stream, p->dev, p->lfresh,
+                       p->counts + CNT_FRESH);
+  }
+  static void clear_reach(Pool* p, int grid) {
+    hipLaunchKernelGGL((k_clear_reach<T>), dim3(grid), dim3(256), 0, p->stream, p->dev, p->lsel,
+                       p->counts + CNT_SEL);
+  }
+  static void edit(Pool* p, int img, int n, const int32_t* idx, const int32_t* cs,
+                   const int32_t* ct) {
+    const int grid = std::min((n + 255) / 256, 1024);
+    hipLaunchKernelGGL((k_edit<T>), dim3(grid > 0 ? grid : 1), dim3(256), 0, p->stream, p->geo,
+                       p->dev, img, n, idx, cs, ct, p->imgs[img].fresh ? 0 : 1);
+  }
+  static void untile(Pool* p, int img, const int32_t* src, int32_t* out) {
+    const PoolImage& im = p->imgs[img];
+    const int PW = im.tw * T;
+    hipLaunchKernelGGL((k_untile<T>), dim3((PW + 255) / 256, im.th * T), dim3(256), 0, p->stream,
+                       p->geo, img, src, out);
+  }
+};
+
+template <int T, int NT>
+static const PoolOps* ops_for() {
+  static const PoolOps ops = {T,
+                              NT,
+                              &PoolKernels<T, NT>::push,
+                              &PoolKernels<T, NT>::fold,
+                              &PoolKernels<T, NT>::bfs_init,
+                              &PoolKernels<T, NT>::bfs_round,
+                              &PoolKernels<T, NT>::flood,
+                              &PoolKernels<T, NT>::labels,
+                              &PoolKernels<T, NT>::reset_tiles,
+                              &PoolKernels<T, NT>::clear_reach,
+                              &PoolKernels<T, NT>::edit,
+                              &PoolKernels<T, NT>::untile};
+  return &ops;
+}
+
+const PoolOps* pool_ops(int tile) {
+  switch (tile) {
+    case 16: return ops_for<16, 256>();
+    case 32: return ops_for<32, 256>();
+    case 64: return ops_for<64, 1024>();
+    default: return nullptr;
+  }
+}
+
+// ------------------------------------------------------------- allocator
+
+void RangeAlloc::init(long long c) {
+  cap = c;
+  free_.clear();
+  if (c > 0) free_.push_back({0, c});
+}
+
+long long RangeAlloc::alloc(long long len) {
+  for (size_t i = 0; i < free_.size(); ++i) {
+    if (free_[i].second >= len) {
+      const long long s = free_[i].first;
+      free_[i].first += len;
+      free_[i].second -= len;
+      if (free_[i].second == 0) free_.erase(free_.begin() + i);
+      return s;
+    }
+  }
+  return -1;
+}
+
+void RangeAlloc::release(long long start, long long len) {
+  if (len <= 0) return;
+  auto it = std::lower_bound(free_.begin(), free_.end(), std::make_pair(start, (long long)0));
+  it = free_.insert(it, {start, len});
+  size_t i = it - free_.begin();
+  if (i + 1 < free_.size() && free_[i].first + free_[i].second == free_[i + 1].first) {
+    free_[i].second += free_[i + 1].second;
+    free_.erase(free_.begin() + i + 1);
+  }
+  if (i > 0 && free_[i - 1].first + free_[i - 1].second == free_[i].first) {
+    free_[i - 1].second += free_[i].second;
+    free_.erase(free_.begin() + i);
+  }
+}
+
+long long RangeAlloc::largest() const {
+  long long m = 0;
+  for (auto& f : free_) m = std::max(m, f.second);
+  return m;
+}
+
+// ------------------------------------------------------------- pool
+
+int tiles_for(int W, int H, int tile, int* tw, int* th) {
+  if (W < 1 || H < 1) {
+    set_error("width and height must be >= 1 (got %d x %d)", W, H);
+    return GCMI_EINVAL;
+  }
+  *tw = (W + tile - 1) / tile;
+  *th = (H + tile - 1) / tile;
+  return GCMI_OK;
+}
+
+size_t pool_bytes(int tile, int cap_tiles, long long cap_pix, int cap_imgs) {
+  const size_t np = (size_t)cap_tiles * tile * tile;
+  return np * (6 * 4 + 2) + (size_t)cap_tiles * (4 * tile * 4 + 11 * 4 + 8) +
+         (size_t)cap_pix * (4 * 4 + 1) +
+         (size_t)cap_imgs * (sizeof(ImgDesc) + SUM_ROWS * kSumSlots * 8 + sizeof(Prep) + 8) +
+         (size_t)compact_scratch_ints(cap_tiles) * 4 + 4096;
+}
+
+static int dev_alloc(void** ptr, size_t bytes, size_t* total) {
+  if (hipMalloc(ptr, bytes ? bytes : 4) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("device allocation of %zu bytes failed", bytes);
+    return GCMI_ENOMEM;
+  }
+  *total += bytes;
+  return GCMI_OK;
+}
+
+int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
+              hipStream_t stream) {
+  p->ops = pool_ops(tile);
+  if (!p->ops) {
+    set_error("tile must be 16, 32 or 64 (got %d)", tile);
+    return GCMI_EINVAL;
+  }
+  if (cap_tiles < 1 || cap_pix < 1 || cap_imgs < 1) {
+    set_error("empty pool");
+    return GCMI_EINVAL;
+  }
+  const long long np = (long long)cap_tiles * tile * tile;
+  if (np >= (1LL << 30)) {
+    set_error("pool of %lld padded pixels exceeds 2^30", np);
+    return GCMI_EINVAL;
+  }
+  p->T = tile;
+  p->cap_tiles = cap_tiles;
+  p->cap_pix = cap_pix;
+  p->cap_imgs = cap_imgs;
+  if (stream) {
+    p->stream = stream;
+  } else {
+    GCMI_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    p->own_stream = true;
+  }
+  int dev = 0, ncu = 0;
+  GCMI_HIP(hipGetDevice(&dev));
+  GCMI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  p->grid_cap = ncu > 0 ? ncu * 8 : 2048;
+  size_t tot = 0;
+  int rc = GCMI_OK;
+  CutDev& d = p->dev;
+#define ALLOC(ptr, bytes) \
+  if (rc == GCMI_OK) rc = dev_alloc((void**)&(ptr), (bytes), &tot)
+  ALLOC(d.e, np * 4);
+  ALLOC(d.h, np * 4);
+  ALLOC(d.cu, np * 4);
+  ALLOC(d.cd, np * 4);
+  ALLOC(d.cl, np * 4);
+  ALLOC(d.cr, np * 4);
+  ALLOC(d.mask, np);
+  ALLOC(d.reach, np);
+  ALLOC(d.pend, (size_t)cap_tiles * 4 * tile * 4);
+  ALLOC(d.pflag, (size_t)cap_tiles * 4);
+  ALLOC(d.act, (size_t)cap_tiles * 4);
+  ALLOC(d.bflag, (size_t)cap_tiles * 4);
+  ALLOC(d.fflag, (size_t)cap_tiles * 4);
+  ALLOC(d.haspos, (size_t)cap_tiles * 4);
+  ALLOC(d.tpos, (size_t)cap_tiles * 8);
+  ALLOC(d.sums, (size_t)cap_imgs * SUM_ROWS * kSumSlots * 8);
+  ALLOC(d.tally, TALLY_N * 8);
+  ALLOC(d.cs, (size_t)cap_pix * 4);
+  ALLOC(d.ct, (size_t)cap_pix * 4);
+  int32_t* right = nullptr;
+  int32_t* down = nullptr;
+  ALLOC(right, (size_t)cap_pix * 4);
+  ALLOC(down, (size_t)cap_pix * 4);
+  d.right = right;
+  d.down = down;
+  ALLOC(d.labels, (size_t)cap_pix);
+  ALLOC(p->d_img, (size_t)cap_imgs * sizeof(ImgDesc));
+  ALLOC(p->tile_img, (size_t)cap_tiles * 4);
+  ALLOC(p->sel, (size_t)cap_tiles * 4);
+  ALLOC(p->fresh, (size_t)cap_tiles * 4);
+  ALLOC(p->list, (size_t)cap_tiles * 4);
+  ALLOC(p->plist, (size_t)cap_tiles * 4);
+  ALLOC(p->alist, (size_t)cap_tiles * 4);
+  ALLOC(p->counts, CNT_N * 4);
+  ALLOC(p->scratch, (size_t)compact_scratch_ints(cap_tiles) * 4);
+  ALLOC(p->d_prep, (size_t)cap_imgs * sizeof(Prep));
+  ALLOC(p->d_flows, (size_t)cap_imgs * 8);
+#undef ALLOC
+  if (rc == GCMI_OK &&
+      (hipHostMalloc((void**)&p->h_counts, CNT_N * 4, hipHostMallocDefault) != hipSuccess ||
+       hipHostMalloc((void**)&p->h_prep, (size_t)cap_imgs * sizeof(Prep), hipHostMallocDefault) !=
+           hipSuccess ||
+       hipHostMalloc((void**)&p->h_flows, (size_t)cap_imgs * 8, hipHostMallocDefault) !=
+           hipSuccess ||
+       hipHostMalloc((void**)&p->h_tally, TALLY_N * 8, hipHostMallocDefault) != hipSuccess)) {
+    (void)hipGetLastError();
+    set_error("pinned host allocation failed");
+    rc = GCMI_ENOMEM;
+  }
+  p->device_bytes = tot;
+  if (rc != GCMI_OK) {
+    pool_free(p);
+    return rc;
+  }
+  p->geo.maxh = (int32_t)np;
+  p->geo.ntiles = cap_tiles;
+  p->geo.img = p->d_img;
+  p->geo.tile_img = p->tile_img;
+  p->imgs.assign(cap_imgs, PoolImage());
+  p->tiles_alloc.init(cap_tiles);
+  p->pix_alloc.init(cap_pix);
+  // pending slots and flags start (and stay, between solves) zero
+  GCMI_HIP(hipMemsetAsync(d.pend, 0, (size_t)cap_tiles * 4 * tile * 4, p->stream));
+  int32_t* flags[] = {d.pflag, d.act, d.bflag, d.fflag, d.haspos};
+  for (int32_t* f : flags) GCMI_HIP(hipMemsetAsync(f, 0, (size_t)cap_tiles * 4, p->stream));
+  GCMI_HIP(hipMemsetAsync(d.tpos, 0, (size_t)cap_tiles * 8, p->stream));
+  GCMI_HIP(hipStreamSynchronize(p->stream));
+  return GCMI_OK;
+}
+
+void pool_free(Pool* p) {
+  CutDev& d = p->dev;
+  void* ptrs[] = {d.e,     d.h,       d.cu,     d.cd,       d.cl,         d.cr,       d.mask,
+                  d.reach, d.pend,    d.pflag,  d.act,      d.bflag,      d.fflag,    d.haspos,
+                  d.tpos,  d.sums,    d.tally,  d.cs,       d.ct,         (void*)d.right,
+                  (void*)d.down,      d.labels, p->d_img,   p->tile_img,  p->sel,     p->fresh,
+                  p->list, p->plist,  p->alist, p->counts,  p->scratch,   p->d_prep,  p->d_flows,
+                  p->d_edit};
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  for (void* x : ptrs)
+    if (x) (void)hipFree(x);
+  void* hp[] = {p->h_counts, p->h_prep, p->h_flows, p->h_tally, p->h_edit};
+  for (void* x : hp)
+    if (x) (void)hipHostFree(x);
+  p->prof.release();
+  if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
+  *p = Pool();
+}
+
+int pool_add_image(Pool* p, int W, int H, int* out) {
+  int tw = 0, th = 0;
+  GCMI_TRY(tiles_for(W, H, p->T, &tw, &th));
+  int id = -1;
+  for (int i = 0; i < p->cap_imgs; ++i)
+    if (!p->imgs[i].live) {
+      id = i;
+      break;
+    }
+  if (id < 0) {
+    set_error("pool holds %d images already", p->cap_imgs);
+    return GCMI_EBUDGET;
+  }
+  const long long nt = (long long)tw * th;
+  const long long t0 = p->tiles_alloc.alloc(nt);
+  if (t0 < 0) {
+    set_error("no room for %lld tiles (largest free run %lld)", nt, p->tiles_alloc.largest());
+    return GCMI_EBUDGET;
+  }
+  const long long p0 = p->pix_alloc.alloc((long long)W * H);
+  if (p0 < 0) {
+    p->tiles_alloc.release(t0, nt);
+    set_error("no room for %lld pixels", (long long)W * H);
+    return GCMI_EBUDGET;
+  }
+  PoolImage& im = p->imgs[id];
+  im = PoolImage();
+  im.live = true;
+  im.W = W;
+  im.H = H;
+  im.tw = tw;
+  im.th = th;
+  im.tile0 = (int)t0;
+  im.ntiles = (int)nt;
+  im.pix0 = p0;
+  ImgDesc d{W, H, tw, th, (int)t0, (int)nt, p0};
+  GCMI_HIP(hipMemcpyAsync(p->d_img + id, &d, sizeof d, hipMemcpyHostToDevice, p->stream));
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)std::min<long long>((nt + 255) / 256, 4096)),
+                     dim3(256), 0, p->stream, p->tile_img + t0, nt, (int32_t)id);
+  GCMI_HIP(hipGetLastError());
+  GCMI_HIP(hipStreamSynchronize(p->stream));
+  *out = id;
+  return GCMI_OK;
+}
+
+int pool_remove_image(Pool* p, int img) {
+  if (img < 0 || img >= p->cap_imgs || !p->imgs[img].live) {
+    set_error("no image %d", img);
+    return GCMI_EINVAL;
+  }
+  PoolImage& im = p->imgs[img];
+  p->tiles_alloc.release(im.tile0, im.ntiles);
+  p->pix_alloc.release(im.pix0, (long long)im.W * im.H);
+  im = PoolImage();
+  return GCMI_OK;
+}
+
+int validate_caps(const int32_t* cs, const int32_t* ct, const int32_t* right,
+                  const int32_t* down, size_t n, int32_t* max_pair) {
+  int64_t mt = 0, mw = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (cs[i] < 0 || ct[i] < 0 || right[i] < 0 || down[i] < 0) {
+      set_error("capacities must be >= 0 (pixel %zu)", i);
+      return GCMI_EINVAL;
+    }
+    mt = std::max<int64_t>(mt, std::max(cs[i], ct[i]));
+    mw = std::max<int64_t>(mw, std::max(right[i], down[i]));
+  }
+  if (mt + 4 * mw >= (int64_t(1) << 31)) {
+    set_error("max(source,sink) + 4*max(pairwise) = %lld must be < 2^31",
+              (long long)(mt + 4 * mw));
+    return GCMI_EINVAL;
+  }
+  *max_pair = (int32_t)mw;
+  return GCMI_OK;
+}
+
+static int check_img(Pool* p, int img) {
+  if (img < 0 || img >= p->cap_imgs || !p->imgs[img].live) {
+    set_error("no image %d", img);
+    return GCMI_EINVAL;
+  }
+  return GCMI_OK;
+}
+
+int pool_upload(Pool* p, int img, const int32_t* cs, const int32_t* ct, const int32_t* right,
+                const int32_t* down, bool device_src) {
+  GCMI_TRY(check_img(p, img));
+  PoolImage& im = p->imgs[img];
+  const size_t n = (size_t)im.W * im.H;
+  if (!device_src) GCMI_TRY(validate_caps(cs, ct, right, down, n, &im.max_pair));
+  const hipMemcpyKind kind = device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  int32_t* dst[4] = {p->dev.cs, p->dev.ct, (int32_t*)p->dev.right, (int32_t*)p->dev.down};
+  const int32_t* src[4] = {cs, ct, right, down};
+  for (int k = 0; k < 4; ++k)
+    GCMI_HIP(hipMemcpyAsync(dst[k] + im.pix0, src[k], n * 4, kind, p->stream));
+  GCMI_HIP(hipStreamSynchronize(p->stream));
+  if (device_src) im.max_pair = -1;  // unknown: edits re-check against 2^29
+  im.uploaded = true;
+  im.fresh = true;
+  im.solved = false;
+  return GCMI_OK;
+}
+
+int pool_edit(Pool* p, int img, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct) {
+  GCMI_TRY(check_img(p, img));
+  PoolImage& im = p->imgs[img];
+  if (!im.uploaded) {
+    set_error("edit before upload");
+    return GCMI_EINVAL;
+  }
+  if (n < 0 || (n > 0 && (!idx || !cs || !ct))) {
+    set_error("bad edit arguments");
+    return GCMI_EINVAL;
+  }
+  if (n == 0) return GCMI_OK;
+  const long long npx = (long long)im.W * im.H;
+  const int64_t wmax = im.max_pair >= 0 ? im.max_pair : (1 << 27);
+  std::vector<int32_t> sorted(idx, idx + n);
+  std::sort(sorted.begin(), sorted.end());
+  for (int i = 0; i < n; ++i) {
+    if (sorted[i] < 0 || sorted[i] >= npx || (i > 0 && sorted[i] == sorted[i - 1])) {
+      set_error("edit indices must be distinct and in [0, %lld)", npx);
+      return GCMI_EINVAL;
+    }
+    if (cs[i] < 0 || ct[i] < 0 || std::max<int64_t>(cs[i], ct[i]) + 4 * wmax >= (1LL << 31)) {
+      set_error("edit %d: capacities out of range", i);
+      return GCMI_EINVAL;
+    }
+  }
+  if (p->edit_cap < n) {
+    if (p->d_edit) (void)hipFree(p->d_edit);
+    if (p->h_edit) (void)hipHostFree(p->h_edit);
+    p->d_edit = nullptr;
+    p->h_edit = nullptr;
+    p->edit_cap = 0;
+    const int cap = std::max(n, 1 << 16);
+    if (hipMalloc((void**)&p->d_edit, (size_t)cap * 12) != hipSuccess ||
+        hipHostMalloc((void**)&p->h_edit, (size_t)cap * 12, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("edit staging allocation failed");
+      return GCMI_ENOMEM;
+    }
+    p->edit_cap = cap;
+  }
+  // the staging buffer is reused: wait for the previous edit's copy
+  GCMI_HIP(hipStreamSynchronize(p->stream));
+  memcpy(p->h_edit, idx, (size_t)n * 4);
+  memcpy(p->h_edit + n, cs, (size_t)n * 4);
+  memcpy(p->h_edit + 2 * n, ct, (size_t)n * 4);
+  GCMI_HIP(hipMemcpyAsync(p->d_edit, p->h_edit, (size_t)n * 12, hipMemcpyHostToDevice, p->stream));
+  p->ops->edit(p, img, n, p->d_edit, p->d_edit + n, p->d_edit + 2 * n);
+  GCMI_HIP(hipGetLastError());
+  if (!im.fresh) im.edited = true;
+  im.solved = false;
+  return GCMI_OK;
+}
+
+// Compaction of flags over the solve's tiles (sel), output list + count.
+static int compact_sel(Pool* p, int32_t* flags, int32_t* out, int cnt, bool clear, int nsel) {
+  Pred pr{};
+  pr.t1 = flags;
+  pr.kind = Pred::NONZERO;
+  pr.ids = p->lsel;
+  Prof* P = p->prof.on ? &p->prof : nullptr;
+  if (P) P->begin(p->stream, K_COMPACT);
+  int rc = compact_async(pr, nsel, out, p->counts + cnt, clear ? flags : nullptr, p->scratch,
+                         p->stream);
+  if (P) P->end(p->stream);
+  return rc;
+}
+
+static int read_counts(Pool* p) {
+  GCMI_HIP(hipMemcpyAsync(p->h_counts, p->counts, CNT_N * 4, hipMemcpyDeviceToHost, p->stream));
+  GCMI_HIP(hipStreamSynchronize(p->stream));
+  p->stats[STAT_SYNCS]++;
+  return GCMI_OK;
+}
+
+int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_rounds,
+               int64_t* flows_out) {
+  if (n <= 0 || n > p->cap_imgs || !ids) {
+    set_error("bad image list");
+    return GCMI_EINVAL;
+  }
+  if (sweeps <= 0) sweeps = kDefaultSweeps;
+  if (bfs <= 0) bfs = kDefaultBfs;
+  const PoolOps* ops = p->ops;
+  const int T = p->T;
+  int nsel = 0, nfresh = 0;
+  for (int i = 0; i < n; ++i) {
+    GCMI_TRY(check_img(p, ids[i]));
+    PoolImage& im = p->imgs[ids[i]];
+    if (!im.uploaded) {
+      set_error("image %d solved before upload", ids[i]);
+      return GCMI_EINVAL;
+    }
+    for (int j = 0; j < i; ++j)
+      if (ids[j] == ids[i]) {
+        set_error("image %d listed twice", ids[i]);
+        return GCMI_EINVAL;
+      }
+    // the pinned prep buffer may still be read by a previous solve's copy
+    if (i == 0) GCMI_HIP(hipStreamSynchronize(p->stream));
+    // warm only if edited since the last solve; otherwise start over from
+    // the stored capacities (which include every edit so far)
+    const bool fresh = im.fresh || !im.edited;
+    p->h_prep[i] = Prep{ids[i], nsel, fresh ? nfresh : -1, 0};
+    nsel += im.ntiles;
+    if (fresh) nfresh += im.ntiles;
+  }
+  // the whole pool in tile order: kernels and compactions skip the lists
+  bool ident = nsel == p->cap_tiles;
+  for (int i = 0; i < n && ident; ++i) ident = p->imgs[ids[i]].tile0 == p->h_prep[i].sel_off;
+  p->lsel = ident ? nullptr : p->sel;
+  p->lfresh = (ident && nfresh == nsel) ? nullptr : p->fresh;
+  Prof* P = p->prof.on ? &p->prof : nullptr;
+  auto B = [&](int k) {
+    if (P) P->begin(p->stream, k);
+  };
+  auto E = [&]() {
+    if (P) P->end(p->stream);
+  };
+  if (P) P->reset();
+  for (int k = 0; k < STAT_N; ++k) p->stats[k] = 0;
+  p->stats[STAT_TILES] = nsel;
+  const int grid_cap = p->grid_cap;
+  auto grid_for = [&](int m) { return m < grid_cap ? (m > 0 ? m : 1) : grid_cap; };
+  int32_t* hc = p->h_counts;
+
+  GCMI_HIP(hipMemcpyAsync(p->d_prep, p->h_prep, (size_t)n * sizeof(Prep), hipMemcpyHostToDevice,
+                          p->stream));
+  hipLaunchKernelGGL(k_prepare, dim3(n), dim3(256), 0, p->stream, p->geo, p->dev, p->d_prep, nsel,
+                     nfresh, p->sel, p->fresh, p->counts + CNT_SEL, p->counts + CNT_FRESH);
+  GCMI_HIP(hipGetLastError());
+
+  int64_t rounds = 0;
+  if (nfresh > 0) {
+    ops->reset_tiles(p, std::min(nfresh, 4096));
+    B(K_PUSH_FIRST);
+    ops->push(p, nfresh, sweeps, true);
+    E();
+    GCMI_HIP(hipGetLastError());
+    rounds = 1;
+    p->stats[STAT_RESIDENCIES] += nfresh;
+  }
+  int32_t nact = 0;
+  for (;;) {
+    // further push rounds before the global relabel
+    for (int r = 1; r < bfs && rounds > 0; ++r) {
+      GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, true, nsel));
+      GCMI_TRY(read_counts(p));
+      nact = hc[CNT_ACT];
+      if (nact == 0) break;
+      if (max_rounds > 0 && rounds >= max_rounds) {
+        set_error("cut did not terminate within %lld push rounds", (long long)max_rounds);
+        return GCMI_ENOTCONVERGED;
+      }
+      B(K_PUSH);
+      ops->push(p, grid_for(nact), sweeps, false);
+      E();
+      GCMI_HIP(hipGetLastError());
+      ++rounds;
+      p->stats[STAT_RESIDENCIES] += nact;
+    }
+    // fold pending flow, then global relabel + completion test
+    B(K_FOLD);
+    ops->fold(p, nsel);
+    E();
+    B(K_BFS_INIT);
+    ops->bfs_init(p, nsel);
+    E();
+    GCMI_HIP(hipGetLastError());
+    p->stats[STAT_GLOBAL_RELABELS]++;
+    // later BFS rounds, kRoundBatch per host check; a round whose list is
+    // empty exits at once
+    bool list_ready = false;
+    for (;;) {
+      for (int b = 0; b < kRoundBatch; ++b) {
+        if (!list_ready) GCMI_TRY(compact_sel(p, p->dev.bflag, p->list, CNT_BFS, true, nsel));
+        list_ready = false;
+        B(K_BFS_ROUND);
+        ops->bfs_round(p, grid_cap);
+        E();
+        GCMI_HIP(hipGetLastError());
+      }
+      GCMI_TRY(compact_sel(p, p->dev.bflag, p->list, CNT_BFS, true, nsel));
+      GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, false, nsel));
+      GCMI_TRY(read_counts(p));
+      if (hc[CNT_BFS] == 0) break;
+      list_ready = true;
+    }
+    nact = hc[CNT_ACT];
+    if (nact == 0) break;
+    // the active list was built without clearing the flags (more BFS rounds
+    // could still have added tiles); clear them for the push kernel
+    hipLaunchKernelGGL(k_clear_listed, dim3((nact + 255) / 256), dim3(256), 0, p->stream,
+                       p->dev.act, p->alist, p->counts + CNT_ACT);
+    if (max_rounds > 0 && rounds >= max_rounds) {
+      set_error("cut did not terminate within %lld push rounds", (long long)max_rounds);
+      return GCMI_ENOTCONVERGED;
+    }
+    B(K_PUSH);
+    ops->push(p, grid_for(nact), sweeps, false);
+    E();
+    GCMI_HIP(hipGetLastError());
+    ++rounds;
+    p->stats[STAT_RESIDENCIES] += nact;
+  }
+  p->stats[STAT_ROUNDS] = rounds;
+
+  // minimal source set
+  GCMI_TRY(compact_sel(p, p->dev.haspos, p->plist, CNT_POS, true, nsel));
+  ops->clear_reach(p, std::min(nsel, 4096));
+  B(K_FLOOD);
+  ops->flood(p, grid_cap, true);
+  E();
+  GCMI_HIP(hipGetLastError());
+  for (;;) {
+    for (int b = 0; b < kRoundBatch; ++b) {
+      GCMI_TRY(compact_sel(p, p->dev.fflag, p->list, CNT_FLOOD, true, nsel));
+      B(K_FLOOD);
+      ops->flood(p, grid_cap, false);
+      E();
+      GCMI_HIP(hipGetLastError());
+    }
+    // anything flagged by the last round?
+    GCMI_TRY(compact_sel(p, p->dev.fflag, p->list, CNT_FLOOD, false, nsel));
+    B(K_LABELS);
+    ops->labels(p, grid_for(nsel));
+    E();
+    GCMI_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_flows, dim3(n), dim3(64), 0, p->stream, p->dev.sums, p->d_prep,
+                       p->d_flows);
+    GCMI_HIP(hipGetLastError());
+    GCMI_HIP(hipMemcpyAsync(p->h_flows, p->d_flows, (size_t)n * 8, hipMemcpyDeviceToHost,
+                            p->stream));
+    GCMI_HIP(hipMemcpyAsync(p->h_tally, p->dev.tally, TALLY_N * 8, hipMemcpyDeviceToHost,
+                            p->stream));
+    GCMI_TRY(read_counts(p));
+    if (hc[CNT_FLOOD] == 0) break;
+  }
+  if (P) P->collect();
+  for (int i = 0; i < n; ++i) {
+    PoolImage& im = p->imgs[ids[i]];
+    im.flow = p->h_flows[i];
+    im.fresh = false;
+    im.edited = false;
+    im.solved = true;
+    if (flows_out) flows_out[i] = im.flow;
+  }
+  p->stats[STAT_BFS_PASSES] = p->stats[STAT_GLOBAL_RELABELS] * nsel + (int64_t)p->h_tally[TALLY_BFS];
+  p->stats[STAT_FLOOD_PASSES] = (int64_t)p->h_tally[TALLY_FLOOD];
+  // algorithmic bytes: first round 16 per image pixel read + 25 per padded
+  // pixel written (state + mask); later residencies 49 (24 read, 25 written);
+  // relabel first pass 5 (mask read, h written); later BFS passes 9 (mask +
+  // h read, h written); flood passes 2 (mask read, reach written); labels 2.
+  int64_t img_px = 0, fresh_px = 0;
+  for (int i = 0; i < n; ++i) {
+    const PoolImage& im = p->imgs[ids[i]];
+    img_px += (int64_t)im.W * im.H;
+    if (p->h_prep[i].fresh_off >= 0) fresh_px += (int64_t)im.W * im.H;
+  }
+  const int64_t tp = (int64_t)T * T;
+  p->stats[STAT_ALG_BYTES] = fresh_px * 16 + (int64_t)nfresh * tp * 25 +
+                             (p->stats[STAT_RESIDENCIES] - nfresh) * tp * 49 +
+                             p->stats[STAT_GLOBAL_RELABELS] * nsel * tp * 5 +
+                             (int64_t)p->h_tally[TALLY_BFS] * tp * 9 +
+                             (int64_t)p->h_tally[TALLY_FLOOD] * tp * 2 + img_px * 2;
+  return GCMI_OK;
+}
+
+int pool_labels(Pool* p, int img, uint8_t* out, bool device_dst) {
+  GCMI_TRY(check_img(p, img));
+  const PoolImage& im = p->imgs[img];
+  if (!im.solved) {
+    set_error("no completed cut for image %d", img);
+    return GCMI_EINVAL;
+  }
+  GCMI_HIP(hipMemcpyAsync(out, p->dev.labels + im.pix0, (size_t)im.W * im.H,
+                          device_dst ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                          p->stream));
+  GCMI_HIP(hipStreamSynchronize(p->stream));
+  return GCMI_OK;
+}
+
+int pool_heights(Pool* p, int img, int32_t* out) {
+  GCMI_TRY(check_img(p, img));
+  const PoolImage& im = p->imgs[img];
+  const size_t np = (size_t)im.ntiles * p->T * p->T;
+  int32_t* tmp = nullptr;
+  if (hipMalloc(&tmp, np * 4) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("allocation failed");
+    return GCMI_ENOMEM;
+  }
+  p->ops->untile(p, img, p->dev.h, tmp);
+  hipError_t e1 = hipMemcpyAsync(out, tmp, np * 4, hipMemcpyDeviceToHost, p->stream);
+  hipError_t e2 = hipStreamSynchronize(p->stream);
+  (void)hipFree(tmp);
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    set_error("height readback failed");
+    return GCMI_EHIP;
+  }
+  return GCMI_OK;
+}
+
+}  // namespace gcmi

@@ -92,6 +92,40 @@ template <int NT, int IT>
 __device__ int block_compact_range(const Pred& pred, int lo, int hi, int carry,
                                    int32_t* list, int32_t* clear, int* wsum) {
   static_assert(IT % 4 == 0 && IT <= 32, "IT: whole int4 groups, one 32-bit mask");
+  if (pred.ids) {
+    // gather form: flags of listed ids
+    for (int base = lo; base < hi; base += NT * IT) {
+      const int i0 = base + IT * threadIdx.x;
+      int id[IT], v[IT];
+      if (i0 + IT <= hi && (reinterpret_cast<uintptr_t>(pred.ids + i0) & 15) == 0) {
+#pragma unroll
+        for (int j = 0; j < IT; j += 4) {
+          const int4 a4 = *reinterpret_cast<const int4*>(pred.ids + i0 + j);
+          id[j] = a4.x; id[j + 1] = a4.y; id[j + 2] = a4.z; id[j + 3] = a4.w;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < IT; ++j) id[j] = i0 + j < hi ? pred.ids[i0 + j] : -1;
+      }
+      // all gathers issued before any is used
+#pragma unroll
+      for (int j = 0; j < IT; ++j) v[j] = pred.t1[id[j] >= 0 ? id[j] : 0];
+      unsigned bits = 0;
+#pragma unroll
+      for (int j = 0; j < IT; ++j)
+        if (id[j] >= 0 && v[j] != 0) bits |= 1u << j;
+      int total = 0;
+      int off = block_excl_scan<NT>(__popc(bits), wsum, &total) + carry;
+      for (unsigned b = bits; b; b &= b - 1) list[off++] = pred.ids[i0 + __ffs(b) - 1];
+      if (clear) {
+#pragma unroll
+        for (int j = 0; j < IT; ++j)
+          if (bits & (1u << j)) clear[id[j]] = 0;
+      }
+      carry += total;
+    }
+    return carry;
+  }
   for (int base = lo; base < hi; base += NT * IT) {
     const int i0 = base + IT * threadIdx.x;
     // all loads first (independent, so they overlap), then the predicate

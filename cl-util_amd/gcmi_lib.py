@@ -83,6 +83,18 @@ SIGNATURES = {
     "gcmi_cut_geometry": [_vp, ctypes.POINTER(_int)],
     "gcmi_cut_profile": [_vp, _int],
     "gcmi_cut_kernel_times": [_vp, ctypes.POINTER(ctypes.c_double), _i64p, _int],
+    "gcmi_cut_edit": [_vp, _int, _vp, _vp, _vp],
+    "gcmi_batch_create": [_int, _vp, _vp, _int, ctypes.POINTER(_vp)],
+    "gcmi_batch_destroy": [_vp],
+    "gcmi_batch_upload": [_vp, _int, _vp, _vp, _vp, _vp],
+    "gcmi_batch_upload_device": [_vp, _int, _vp, _vp, _vp, _vp],
+    "gcmi_batch_edit": [_vp, _int, _int, _vp, _vp, _vp],
+    "gcmi_batch_run": [_vp, _int, _int, _i64, _i64p],
+    "gcmi_batch_run_subset": [_vp, _int, _vp, _int, _int, _i64, _i64p],
+    "gcmi_batch_labels": [_vp, _int, _vp],
+    "gcmi_batch_stats": [_vp, _i64p, _int],
+    "gcmi_batch_profile": [_vp, _int],
+    "gcmi_batch_kernel_times": [_vp, ctypes.POINTER(ctypes.c_double), _i64p, _int],
 }
 
 KERNEL_CLASSES = ("push_first", "push", "fold", "bfs_init", "bfs_round", "flood",

@@ -149,6 +149,14 @@ int gcmi_cut_upload_device(gcmi_cut *cut, const int32_t *d_source,
  * After it returns GCMI_OK, gcmi_cut_labels gives the minimal source set. */
 int gcmi_cut_run(gcmi_cut *cut, int sweeps, int bfs, int64_t max_rounds,
                  int64_t *flow_out);
+/* Warm-start edit (no reference counterpart; the brush tools of the
+ * reference re-upload the whole excess image, test_GraphCut.py:74-75): new
+ * source/sink capacities for n distinct pixels (h_index = y*width+x).  The
+ * next gcmi_cut_run reflows from the current residual graph instead of
+ * starting over; its flow and labels equal a from-scratch cut of the edited
+ * capacities.  Before the first run an edit only rewrites the capacities. */
+int gcmi_cut_edit(gcmi_cut *cut, int n, const int32_t *h_index,
+                  const int32_t *h_source, const int32_t *h_sink);
 /* Labels of the last cut: h_out[y*width+x] = 1 for the minimal source
  * (foreground) set -- pixels reachable from the source in the final residual
  * graph -- else 0.  (The reference exposes only dHeight, GraphCut.pyx:62.) */
@@ -175,6 +183,40 @@ int gcmi_cut_profile(gcmi_cut *cut, int enable);
 /* Summed milliseconds and launch counts per class for the last run. */
 int gcmi_cut_kernel_times(gcmi_cut *cut, double *ms_out, int64_t *launches_out,
                           int n);
+
+/* ---------------------------------------------------------------- batches
+ * Many independent images (any sizes) solved together: every kernel launch
+ * of a run covers the tiles of all images, so the launches, host syncs and
+ * tail rounds of `count` cuts are shared.  No reference counterpart (the
+ * reference cuts one image per GraphCut object).  Image i keeps its own
+ * capacities, flow and labels; a batch is a pool of images in one set of
+ * device arrays. */
+typedef struct gcmi_batch gcmi_batch;
+
+int gcmi_batch_create(int count, const int *widths, const int *heights,
+                      int tile, gcmi_batch **out);
+int gcmi_batch_destroy(gcmi_batch *batch);
+int gcmi_batch_upload(gcmi_batch *batch, int i, const int32_t *h_source,
+                      const int32_t *h_sink, const int32_t *h_right,
+                      const int32_t *h_down);
+int gcmi_batch_upload_device(gcmi_batch *batch, int i,
+                             const int32_t *d_source, const int32_t *d_sink,
+                             const int32_t *d_right, const int32_t *d_down);
+/* Warm-start edit of image i (as gcmi_cut_edit). */
+int gcmi_batch_edit(gcmi_batch *batch, int i, int n, const int32_t *h_index,
+                    const int32_t *h_source, const int32_t *h_sink);
+/* Solves all images; flows_out (count int64, may be NULL) gets each flow. */
+int gcmi_batch_run(gcmi_batch *batch, int sweeps, int bfs, int64_t max_rounds,
+                   int64_t *flows_out);
+/* Solves images which[0..n) only (flows_out gets n values). */
+int gcmi_batch_run_subset(gcmi_batch *batch, int n, const int *which,
+                          int sweeps, int bfs, int64_t max_rounds,
+                          int64_t *flows_out);
+int gcmi_batch_labels(gcmi_batch *batch, int i, uint8_t *h_out);
+int gcmi_batch_stats(gcmi_batch *batch, int64_t *out, int n);
+int gcmi_batch_profile(gcmi_batch *batch, int enable);
+int gcmi_batch_kernel_times(gcmi_batch *batch, double *ms_out,
+                            int64_t *launches_out, int n);
 
 #ifdef __cplusplus
 }
