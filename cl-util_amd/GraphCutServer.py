@@ -1,0 +1,159 @@
+"""GraphCutServer: resident grid-cut sessions behind a continuously batching
+worker (libgcmi's gcmi_server_*).
+
+Each session is one image whose capacities, residual graph and heights stay
+in device memory between requests.  A request is a new image (cut from its
+capacities) or a brush stroke (new unaries for some pixels, reflowed warm
+from the session's residual graph).  The worker thread inside the library
+solves every queued request it can -- one per session, up to `max_batch` --
+in one batched run; requests that arrive meanwhile join the next run.  The
+answer of every request is the same as a from-scratch GraphCut of the
+session's capacities at that point: the max-flow value and the minimal
+source set.
+
+mode "batched" is the serving loop; mode "plain" serves one request at a
+time, each cut from scratch, as the baseline.
+
+No counterpart in the reference, which serves one user per GraphCut object
+and re-uploads the excess image after each stroke (test_GraphCut.py:74-75).
+"""
+import ctypes
+
+import numpy as np
+
+from GraphCut import _i32
+from gcmi_lib import EBUSY, GcmiError, check, host_ptr, lib
+
+QUEUED, DONE, FAILED = 0, 1, 2
+MODES = {"batched": 0, "plain": 1}
+STAT_NAMES = ("solves", "served", "queued", "sessions", "tiles_used", "tiles_capacity",
+              "device_bytes")
+
+
+class RequestInfo:
+    """Completion record of one request (times in ns of a steady clock)."""
+
+    __slots__ = ("status", "flow", "t_arrive", "t_start", "t_done", "batch")
+
+    def __init__(self, v):
+        self.status, self.flow, self.t_arrive, self.t_start, self.t_done, self.batch = \
+            (int(x) for x in v)
+
+    @property
+    def latency_ms(self):
+        return (self.t_done - self.t_arrive) / 1e6
+
+    @property
+    def queue_ms(self):
+        return (self.t_start - self.t_arrive) / 1e6
+
+
+class GraphCutServer:
+    def __init__(self, tile=32, hbm_budget=200 * 10**9, max_sessions=256, max_batch=64,
+                 mode="batched", sweeps=0, bfs=0):
+        self.tile = int(tile[0] if isinstance(tile, (tuple, list)) else tile)
+        if mode not in MODES:
+            raise ValueError("mode must be one of %s" % sorted(MODES))
+        self.mode = mode
+        h = ctypes.c_void_p()
+        check(lib().gcmi_server_create(self.tile, int(hbm_budget), int(max_sessions),
+                                       int(max_batch), MODES[mode], ctypes.byref(h)))
+        self._h = h.value
+        self.dims = {}
+        if sweeps or bfs:
+            check(lib().gcmi_server_set_params(self._h, int(sweeps), int(bfs)))
+
+    def close_server(self):
+        if getattr(self, "_h", None):
+            lib().gcmi_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close_server()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close_server()
+
+    # ------------------------------------------------------------ sessions
+    def open(self, width, height):
+        """Admits a width x height session; GcmiError(EBUDGET) if the pool
+        has no slot or room for it."""
+        sid = ctypes.c_int()
+        check(lib().gcmi_server_open(self._h, int(width), int(height), ctypes.byref(sid)))
+        self.dims[sid.value] = (int(width), int(height))
+        return sid.value
+
+    def close(self, session):
+        check(lib().gcmi_server_close(self._h, int(session)))
+        self.dims.pop(session, None)
+
+    def _shape(self, session):
+        w, h = self.dims[session]
+        return (h, w)
+
+    # ------------------------------------------------------------ requests
+    def submit_image(self, session, source, sink, right, down):
+        s = self._shape(session)
+        a = [_i32(x, s) for x in (source, sink, right, down)]
+        rid = ctypes.c_int64()
+        check(lib().gcmi_server_submit_image(self._h, int(session), *[host_ptr(x) for x in a],
+                                             ctypes.byref(rid)))
+        return rid.value
+
+    def submit_stroke(self, session, index, source, sink):
+        idx = np.ascontiguousarray(index, dtype=np.int32).reshape(-1)
+        n = idx.size
+        src = np.ascontiguousarray(np.broadcast_to(source, (n,)), dtype=np.int32)
+        snk = np.ascontiguousarray(np.broadcast_to(sink, (n,)), dtype=np.int32)
+        rid = ctypes.c_int64()
+        check(lib().gcmi_server_submit_stroke(self._h, int(session), n, host_ptr(idx),
+                                              host_ptr(src), host_ptr(snk), ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, request, timeout=None):
+        """RequestInfo of a request; raises GcmiError(EBUSY) if it is not
+        done within `timeout` seconds (None: no limit)."""
+        info = (ctypes.c_int64 * 6)()
+        ms = -1 if timeout is None else int(timeout * 1000)
+        rc = lib().gcmi_server_wait(self._h, int(request), ms, info)
+        if rc == EBUSY:
+            raise GcmiError(rc, "request %d still in flight" % request)
+        check(rc)
+        return RequestInfo(info)
+
+    def poll(self, request):
+        """RequestInfo, status QUEUED while in flight (no wait)."""
+        info = (ctypes.c_int64 * 6)()
+        rc = lib().gcmi_server_wait(self._h, int(request), 0, info)
+        if rc not in (0, EBUSY):
+            check(rc)
+        return RequestInfo(info)
+
+    def forget(self, request):
+        check(lib().gcmi_server_forget(self._h, int(request)))
+
+    def mask_bits(self, session):
+        """(packed uint32 words, request id) of the session's last result."""
+        w, h = self.dims[session]
+        out = np.empty((w * h + 31) // 32, np.uint32)
+        rid = ctypes.c_int64()
+        check(lib().gcmi_server_mask(self._h, int(session), host_ptr(out), ctypes.byref(rid)))
+        return out, rid.value
+
+    def mask(self, session):
+        """(uint8 [H][W] labels, request id): 1 = minimal source set."""
+        w, h = self.dims[session]
+        bits, rid = self.mask_bits(session)
+        lab = np.unpackbits(bits.view(np.uint8), bitorder="little")[: w * h]
+        return lab.reshape(h, w), rid
+
+    def stats(self):
+        out = (ctypes.c_int64 * len(STAT_NAMES))()
+        check(lib().gcmi_server_stats(self._h, out, len(STAT_NAMES)))
+        return dict(zip(STAT_NAMES, (int(x) for x in out)))

@@ -133,8 +133,9 @@ struct Pool {
   int32_t* scratch = nullptr;
   Prep* d_prep = nullptr;
   long long* d_flows = nullptr;
-  int32_t* d_edit = nullptr;  // staging for edits: idx, cs, ct
-  int edit_cap = 0;
+  int32_t* d_edit = nullptr;  // staging for edits: idx, cs, ct per edit
+  long long edit_cap = 0;     // pixels
+  std::vector<uint64_t> mark;  // host bitmap for the distinct-index check
   // pinned host buffers
   int32_t* h_counts = nullptr;
   Prep* h_prep = nullptr;
@@ -163,6 +164,19 @@ int pool_upload(Pool* p, int img, const int32_t* cs, const int32_t* ct, const in
 // Warm-start edit of n distinct pixels (row-major indices local to the
 // image) from host arrays.
 int pool_edit(Pool* p, int img, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct);
+// Several images' edits with one staging copy.
+struct EditReq {
+  int img;
+  int n;
+  const int32_t *idx, *cs, *ct;
+};
+int pool_edit_many(Pool* p, int count, const EditReq* e);
+// Checks an edit of an image of npx pixels: indices distinct and in range,
+// capacities >= 0 and max(cs, ct) + 4 * wmax < 2^31.  `mark` is a zeroed
+// bitmap of at least npx bits (grown here if smaller) and is zeroed again
+// on return.
+int check_edit(long long npx, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct,
+               int64_t wmax, std::vector<uint64_t>& mark);
 // Solves the listed images together (fresh ones from their capacities,
 // edited ones warm).  flows_out (may be NULL) receives one flow per image.
 int pool_solve(Pool* p, const int* img_ids, int n, int sweeps, int bfs, int64_t max_rounds,

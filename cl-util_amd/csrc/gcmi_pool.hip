@@ -505,39 +505,79 @@ int pool_upload(Pool* p, int img, const int32_t* cs, const int32_t* ct, const in
   return GCMI_OK;
 }
 
-int pool_edit(Pool* p, int img, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct) {
-  GCMI_TRY(check_img(p, img));
-  PoolImage& im = p->imgs[img];
-  if (!im.uploaded) {
-    set_error("edit before upload");
-    return GCMI_EINVAL;
-  }
+int check_edit(long long npx, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct,
+               int64_t wmax, std::vector<uint64_t>& mark) {
   if (n < 0 || (n > 0 && (!idx || !cs || !ct))) {
     set_error("bad edit arguments");
     return GCMI_EINVAL;
   }
-  if (n == 0) return GCMI_OK;
-  const long long npx = (long long)im.W * im.H;
-  const int64_t wmax = im.max_pair >= 0 ? im.max_pair : (1 << 27);
-  std::vector<int32_t> sorted(idx, idx + n);
-  std::sort(sorted.begin(), sorted.end());
-  for (int i = 0; i < n; ++i) {
-    if (sorted[i] < 0 || sorted[i] >= npx || (i > 0 && sorted[i] == sorted[i - 1])) {
-      set_error("edit indices must be distinct and in [0, %lld)", npx);
-      return GCMI_EINVAL;
+  if ((long long)mark.size() * 64 < npx) mark.assign((size_t)((npx + 63) / 64), 0);
+  int bad = -1, kind = 0;
+  int i = 0;
+  for (; i < n; ++i) {
+    const int32_t q = idx[i];
+    if (q < 0 || q >= npx) {
+      bad = i;
+      kind = 0;
+      break;
     }
+    uint64_t& w = mark[q >> 6];
+    const uint64_t b = uint64_t(1) << (q & 63);
+    if (w & b) {
+      bad = i;
+      kind = 0;
+      break;
+    }
+    w |= b;
     if (cs[i] < 0 || ct[i] < 0 || std::max<int64_t>(cs[i], ct[i]) + 4 * wmax >= (1LL << 31)) {
-      set_error("edit %d: capacities out of range", i);
-      return GCMI_EINVAL;
+      bad = i;
+      kind = 1;
+      ++i;
+      break;
     }
   }
-  if (p->edit_cap < n) {
+  // clear the bits this call set (the first i indices, bad one excluded when
+  // it was a duplicate or out of range)
+  const int set = (bad >= 0 && kind == 0) ? bad : i;
+  for (int k = 0; k < set; ++k) mark[idx[k] >> 6] &= ~(uint64_t(1) << (idx[k] & 63));
+  if (bad >= 0) {
+    if (kind == 0)
+      set_error("edit indices must be distinct and in [0, %lld) (entry %d)", npx, bad);
+    else
+      set_error("edit %d: capacities out of range", bad);
+    return GCMI_EINVAL;
+  }
+  return GCMI_OK;
+}
+
+int pool_edit_many(Pool* p, int count, const EditReq* e) {
+  long long total = 0;
+  for (int k = 0; k < count; ++k) {
+    GCMI_TRY(check_img(p, e[k].img));
+    const PoolImage& im = p->imgs[e[k].img];
+    if (!im.uploaded) {
+      set_error("edit before upload");
+      return GCMI_EINVAL;
+    }
+    const int64_t wmax = im.max_pair >= 0 ? im.max_pair : (1 << 27);
+    GCMI_TRY(check_edit((long long)im.W * im.H, e[k].n, e[k].idx, e[k].cs, e[k].ct, wmax,
+                        p->mark));
+    total += e[k].n;
+  }
+  if (total == 0) return GCMI_OK;
+  if (total > (1LL << 30)) {
+    set_error("edit of %lld pixels too large", total);
+    return GCMI_EINVAL;
+  }
+  if (p->edit_cap < total) {
+    // the old staging may still be read by an earlier copy
+    GCMI_HIP(hipStreamSynchronize(p->stream));
     if (p->d_edit) (void)hipFree(p->d_edit);
     if (p->h_edit) (void)hipHostFree(p->h_edit);
     p->d_edit = nullptr;
     p->h_edit = nullptr;
     p->edit_cap = 0;
-    const int cap = std::max(n, 1 << 16);
+    const long long cap = std::max<long long>(total, 1 << 16);
     if (hipMalloc((void**)&p->d_edit, (size_t)cap * 12) != hipSuccess ||
         hipHostMalloc((void**)&p->h_edit, (size_t)cap * 12, hipHostMallocDefault) != hipSuccess) {
       (void)hipGetLastError();
@@ -545,18 +585,39 @@ int pool_edit(Pool* p, int img, int n, const int32_t* idx, const int32_t* cs, co
       return GCMI_ENOMEM;
     }
     p->edit_cap = cap;
+  } else {
+    // the staging buffer is reused: wait for the previous edit's copy
+    GCMI_HIP(hipStreamSynchronize(p->stream));
   }
-  // the staging buffer is reused: wait for the previous edit's copy
-  GCMI_HIP(hipStreamSynchronize(p->stream));
-  memcpy(p->h_edit, idx, (size_t)n * 4);
-  memcpy(p->h_edit + n, cs, (size_t)n * 4);
-  memcpy(p->h_edit + 2 * n, ct, (size_t)n * 4);
-  GCMI_HIP(hipMemcpyAsync(p->d_edit, p->h_edit, (size_t)n * 12, hipMemcpyHostToDevice, p->stream));
-  p->ops->edit(p, img, n, p->d_edit, p->d_edit + n, p->d_edit + 2 * n);
-  GCMI_HIP(hipGetLastError());
-  if (!im.fresh) im.edited = true;
-  im.solved = false;
+  long long off = 0;
+  for (int k = 0; k < count; ++k) {
+    const int n = e[k].n;
+    memcpy(p->h_edit + 3 * off, e[k].idx, (size_t)n * 4);
+    memcpy(p->h_edit + 3 * off + n, e[k].cs, (size_t)n * 4);
+    memcpy(p->h_edit + 3 * off + 2 * n, e[k].ct, (size_t)n * 4);
+    off += n;
+  }
+  GCMI_HIP(hipMemcpyAsync(p->d_edit, p->h_edit, (size_t)total * 12, hipMemcpyHostToDevice,
+                          p->stream));
+  off = 0;
+  for (int k = 0; k < count; ++k) {
+    const int n = e[k].n;
+    PoolImage& im = p->imgs[e[k].img];
+    if (n > 0) {
+      int32_t* d = p->d_edit + 3 * off;
+      p->ops->edit(p, e[k].img, n, d, d + n, d + 2 * n);
+      GCMI_HIP(hipGetLastError());
+      if (!im.fresh) im.edited = true;
+      im.solved = false;
+    }
+    off += n;
+  }
   return GCMI_OK;
+}
+
+int pool_edit(Pool* p, int img, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct) {
+  const EditReq e{img, n, idx, cs, ct};
+  return pool_edit_many(p, 1, &e);
 }
 
 // Compaction of flags over the solve's tiles (sel), output list + count.

@@ -1,0 +1,596 @@
+// Serving loop: resident sessions in one image pool, a request queue and a
+// worker thread that batches continuously.
+//
+// A session is one image of the pool: its capacities, residual graph,
+// heights and tile state stay in HBM between requests.  Requests are either
+// a new image (full upload; the next solve starts from it) or a brush
+// stroke (new unaries for some pixels; the next solve reflows warm from the
+// session's residual graph).  The worker takes every queued request it can
+// -- at most one per session and `max_batch` in all, oldest first --
+// applies them, solves those sessions together in one pool solve, packs
+// each mask to bits into the session's pinned host buffer and completes the
+// requests.  Requests that arrive while a solve runs wait for the next one.
+//
+// New images reach the device while the worker solves: the submitting
+// thread copies the capacities into a device staging range on a copy stream
+// of its own, and the worker's upload is then a device-to-device copy.
+//
+// Admission: a session opens only if the pool, sized from the HBM budget,
+// has a slot and room for its tiles and pixels.
+//
+// In plain mode the worker serves one request at a time and cuts it from
+// scratch: a stroke rewrites the unaries on the device and the image is cut
+// from its capacities, as a client without resident solver state would.
+//
+// No reference counterpart: the reference is a single-user tool
+// (test_GraphCut.py:74-75 re-uploads the excess image per stroke).
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "gcmi_pool.h"
+
+using namespace gcmi;
+
+namespace {
+
+int64_t now_ns() {
+  return std::chrono::duration_cast<std::chrono::nanoseconds>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+// One wave packs 64 labels into two words with a ballot; loads coalesce.
+__global__ void k_pack_bits(const uint8_t* __restrict__ labels, long long n, uint32_t* out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const bool on = i < n && labels[i] != 0;
+  const unsigned long long b = __ballot(on);
+  const long long w = (i >> 6) * 2;
+  const long long words = (n + 31) >> 5;
+  if ((threadIdx.x & 63) == 0 && w < words) {
+    out[w] = (uint32_t)b;
+    if (w + 1 < words) out[w + 1] = (uint32_t)(b >> 32);
+  }
+}
+
+enum { REQ_IMAGE = 0, REQ_STROKE = 1 };
+enum { ST_QUEUED = 0, ST_DONE = 1, ST_FAILED = 2 };
+
+constexpr long long kStageAlign = 64;  // staging offsets in int32 elements
+
+struct Request {
+  int64_t id = 0;
+  int session = 0;
+  int kind = REQ_STROKE;
+  int n = 0;                    // stroke pixels
+  std::vector<int32_t> stroke;  // idx, cs, ct
+  long long stage = -1;         // image: offset of cs, ct, right, down in staging
+  long long stage_len = 0;
+  int32_t max_pair = 0;         // image: largest pairwise weight
+  int64_t t_arrive = 0, t_start = 0, t_done = 0;
+  int64_t flow = 0;
+  int status = ST_QUEUED;
+  int batch = 0;  // sessions solved together with it
+};
+
+struct Session {
+  bool open = false;
+  bool closing = false;      // close() waits for its requests; no new ones
+  bool busy = false;         // a request of this session is being served
+  int img = -1;
+  int W = 0, H = 0;
+  int32_t max_pair = -1;     // largest pairwise weight of the submitted image
+  uint32_t* bits = nullptr;  // pinned packed mask of the last completed request
+  int64_t last_done = 0;     // id of the last completed request
+};
+
+}  // namespace
+
+struct gcmi_server {
+  Pool pool;
+  int device = 0;
+  int mode = 0;  // 0 batched warm, 1 plain per-request
+  int max_batch = 64;
+  int sweeps = 0, bfs = 0;
+  std::vector<Session> sessions;
+  uint32_t* d_bits = nullptr;  // device staging for packed masks
+  int32_t* d_stage = nullptr;  // device staging for new images
+  RangeAlloc stage_alloc;
+  hipStream_t copy_stream = nullptr;
+  // Lock order: pool_mu, then copy_mu, then mu.  The worker holds pool_mu
+  // (not mu) while it serves a batch; open/close hold pool_mu while they
+  // change the pool; submitters hold copy_mu while they stage an image.
+  std::mutex pool_mu, copy_mu, mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request*> queue;
+  std::unordered_map<int64_t, Request*> reqs;
+  int64_t next_id = 1;
+  bool stop = false;
+  std::thread worker;
+  int64_t iterations = 0, solved_requests = 0;
+  char err[256] = "";
+};
+
+namespace {
+
+// Applies the batch's requests to the pool, solves, and copies the packed
+// masks out.  Runs on the worker thread with pool_mu held.
+int serve_batch(gcmi_server* s, std::vector<Request*>& batch) {
+  Pool* p = &s->pool;
+  std::vector<int> imgs;
+  imgs.reserve(batch.size());
+  std::vector<EditReq> edits;
+  for (Request* r : batch) {
+    const Session& ss = s->sessions[r->session];
+    const size_t npx = (size_t)ss.W * ss.H;
+    PoolImage& im = p->imgs[ss.img];
+    if (r->kind == REQ_IMAGE) {
+      const int32_t* d = s->d_stage + r->stage;
+      GCMI_TRY(pool_upload(p, ss.img, d, d + npx, d + 2 * npx, d + 3 * npx, true));
+      im.max_pair = r->max_pair;
+    } else {
+      // plain mode cuts every request from scratch: the edit only rewrites
+      // the unaries
+      if (s->mode == 1) im.fresh = true;
+      const int32_t* d = r->stroke.data();
+      edits.push_back(EditReq{ss.img, r->n, d, d + r->n, d + 2 * r->n});
+    }
+    imgs.push_back(ss.img);
+  }
+  if (!edits.empty()) GCMI_TRY(pool_edit_many(p, (int)edits.size(), edits.data()));
+  GCMI_TRY(pool_solve(p, imgs.data(), (int)imgs.size(), s->sweeps, s->bfs, 0, nullptr));
+  long long off = 0;
+  for (Request* r : batch) {
+    const Session& ss = s->sessions[r->session];
+    const long long npx = (long long)ss.W * ss.H;
+    const long long words = (npx + 31) / 32;
+    const PoolImage& im = p->imgs[ss.img];
+    hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0,
+                       p->stream, p->dev.labels + im.pix0, npx, s->d_bits + off);
+    GCMI_HIP(hipGetLastError());
+    GCMI_HIP(hipMemcpyAsync(ss.bits, s->d_bits + off, words * 4, hipMemcpyDeviceToHost,
+                            p->stream));
+    off += words;
+  }
+  GCMI_HIP(hipStreamSynchronize(p->stream));
+  for (Request* r : batch) r->flow = p->imgs[s->sessions[r->session].img].flow;
+  return GCMI_OK;
+}
+
+void worker_main(gcmi_server* s) {
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+    if (s->stop) break;
+    // oldest first, one request per session, at most max_batch
+    const int cap = s->mode == 1 ? 1 : s->max_batch;
+    std::vector<Request*> batch;
+    for (auto it = s->queue.begin(); it != s->queue.end() && (int)batch.size() < cap;) {
+      Session& ss = s->sessions[(*it)->session];
+      if (ss.busy) {
+        ++it;
+        continue;
+      }
+      ss.busy = true;
+      batch.push_back(*it);
+      it = s->queue.erase(it);
+    }
+    if (batch.empty()) continue;
+    const int64_t t0 = now_ns();
+    for (Request* r : batch) r->t_start = t0;
+    lk.unlock();
+    int rc;
+    char err[sizeof s->err] = "";
+    {
+      std::lock_guard<std::mutex> plk(s->pool_mu);
+      rc = serve_batch(s, batch);
+      if (rc != GCMI_OK) snprintf(err, sizeof err, "%s", gcmi_last_error());
+    }
+    const int64_t t1 = now_ns();
+    lk.lock();
+    if (rc != GCMI_OK) memcpy(s->err, err, sizeof err);
+    for (Request* r : batch) {
+      r->t_done = t1;
+      r->status = rc == GCMI_OK ? ST_DONE : ST_FAILED;
+      r->batch = (int)batch.size();
+      r->stroke.clear();
+      r->stroke.shrink_to_fit();
+      if (r->stage >= 0) {
+        s->stage_alloc.release(r->stage, r->stage_len);
+        r->stage = -1;
+      }
+      Session& ss = s->sessions[r->session];
+      ss.busy = false;
+      if (rc == GCMI_OK) ss.last_done = r->id;
+    }
+    s->iterations++;
+    s->solved_requests += (int64_t)batch.size();
+    s->cv_done.notify_all();
+  }
+}
+
+// Queues r; the caller has set r->t_arrive.
+void enqueue(gcmi_server* s, Request* r, int64_t* id_out) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  r->id = s->next_id++;
+  s->reqs[r->id] = r;
+  s->queue.push_back(r);
+  if (id_out) *id_out = r->id;
+  s->cv_work.notify_one();
+}
+
+// Called with mu held.
+int check_session(gcmi_server* s, int session) {
+  if (session < 0 || session >= (int)s->sessions.size() || !s->sessions[session].open ||
+      s->sessions[session].closing) {
+    set_error("no open session %d", session);
+    return GCMI_EINVAL;
+  }
+  return GCMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_batch, int mode,
+                       gcmi_server** out) {
+  if (!out || max_sessions < 1 || max_batch < 1 || (mode != 0 && mode != 1)) {
+    set_error("bad server arguments");
+    return GCMI_EINVAL;
+  }
+  *out = nullptr;
+  if (!pool_ops(tile)) {
+    set_error("tile must be 16, 32 or 64 (got %d)", tile);
+    return GCMI_EINVAL;
+  }
+  GCMI_TRY(require_device());
+  // a quarter of the budget (at most 1 GiB) stages new images; the pool and
+  // the packed-mask staging share the rest
+  const size_t stage_bytes = std::min<size_t>(hbm_budget / 4, size_t(1) << 30) & ~size_t(255);
+  const long long T2 = (long long)tile * tile;
+  const int probe = 1 << 12;
+  const size_t b1 = pool_bytes(tile, probe, probe * T2, max_sessions);
+  const size_t b2 = pool_bytes(tile, 2 * probe, 2 * probe * T2, max_sessions);
+  const double per_tile = (double)(b2 - b1) / probe + T2 / 8.0;
+  const double fixed = (double)b1 - probe * (per_tile - T2 / 8.0) + 4.0 * max_sessions;
+  const double rest = (double)hbm_budget - (double)stage_bytes - fixed;
+  long long cap_tiles = rest > 0 ? (long long)(rest / per_tile) : 0;
+  cap_tiles = std::min<long long>(cap_tiles, ((1LL << 30) - (1LL << 20)) / T2);
+  if (cap_tiles < 1 || stage_bytes < 1024) {
+    set_error("HBM budget %zu too small", hbm_budget);
+    return GCMI_EBUDGET;
+  }
+  gcmi_server* s = new (std::nothrow) gcmi_server();
+  if (!s) {
+    set_error("host allocation failed");
+    return GCMI_ENOMEM;
+  }
+  s->mode = mode;
+  s->max_batch = max_batch;
+  (void)hipGetDevice(&s->device);
+  const long long cap_pix = cap_tiles * T2;
+  int rc = pool_init(&s->pool, tile, (int)cap_tiles, cap_pix, max_sessions, nullptr);
+  if (rc == GCMI_OK) {
+    const long long words = cap_pix / 32 + max_sessions;
+    if (hipMalloc((void**)&s->d_bits, words * 4) != hipSuccess ||
+        hipMalloc((void**)&s->d_stage, stage_bytes) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("server staging allocation failed");
+      rc = GCMI_ENOMEM;
+    }
+  }
+  if (rc != GCMI_OK) {
+    if (s->d_bits) (void)hipFree(s->d_bits);
+    if (s->d_stage) (void)hipFree(s->d_stage);
+    pool_free(&s->pool);
+    delete s;
+    return rc;
+  }
+  s->stage_alloc.init((long long)(stage_bytes / 4));
+  s->sessions.assign(max_sessions, Session());
+  s->worker = std::thread(worker_main, s);
+  *out = s;
+  return GCMI_OK;
+}
+
+int gcmi_server_destroy(gcmi_server* s) {
+  if (!s) return GCMI_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->cv_work.notify_all();
+  }
+  if (s->worker.joinable()) s->worker.join();
+  for (auto& kv : s->reqs) delete kv.second;
+  for (Session& ss : s->sessions)
+    if (ss.bits) (void)hipHostFree(ss.bits);
+  if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
+  if (s->d_bits) (void)hipFree(s->d_bits);
+  if (s->d_stage) (void)hipFree(s->d_stage);
+  pool_free(&s->pool);
+  delete s;
+  return GCMI_OK;
+}
+
+int gcmi_server_set_params(gcmi_server* s, int sweeps, int bfs) {
+  if (!s) {
+    set_error("null server");
+    return GCMI_EINVAL;
+  }
+  std::lock_guard<std::mutex> plk(s->pool_mu);
+  s->sweeps = sweeps;
+  s->bfs = bfs;
+  return GCMI_OK;
+}
+
+int gcmi_server_open(gcmi_server* s, int width, int height, int* session_out) {
+  if (!s || !session_out) {
+    set_error("null argument");
+    return GCMI_EINVAL;
+  }
+  std::lock_guard<std::mutex> plk(s->pool_mu);
+  std::lock_guard<std::mutex> lk(s->mu);
+  int slot = -1;
+  for (int i = 0; i < (int)s->sessions.size(); ++i)
+    if (!s->sessions[i].open) {
+      slot = i;
+      break;
+    }
+  if (slot < 0) {
+    set_error("all %zu session slots are open", s->sessions.size());
+    return GCMI_EBUDGET;
+  }
+  int img = -1;
+  GCMI_TRY(pool_add_image(&s->pool, width, height, &img));
+  Session& ss = s->sessions[slot];
+  ss = Session();
+  const size_t words = ((size_t)width * height + 31) / 32;
+  if (hipHostMalloc((void**)&ss.bits, words * 4, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    ss.bits = nullptr;
+    pool_remove_image(&s->pool, img);
+    set_error("pinned mask buffer allocation failed");
+    return GCMI_ENOMEM;
+  }
+  memset(ss.bits, 0, words * 4);
+  ss.open = true;
+  ss.img = img;
+  ss.W = width;
+  ss.H = height;
+  *session_out = slot;
+  return GCMI_OK;
+}
+
+int gcmi_server_close(gcmi_server* s, int session) {
+  if (!s) {
+    set_error("null server");
+    return GCMI_EINVAL;
+  }
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    GCMI_TRY(check_session(s, session));
+    s->sessions[session].closing = true;
+    s->cv_done.wait(lk, [&] {
+      if (s->sessions[session].busy) return false;
+      for (Request* r : s->queue)
+        if (r->session == session) return false;
+      return true;
+    });
+  }
+  std::lock_guard<std::mutex> plk(s->pool_mu);
+  std::lock_guard<std::mutex> lk(s->mu);
+  Session& ss = s->sessions[session];
+  pool_remove_image(&s->pool, ss.img);
+  if (ss.bits) (void)hipHostFree(ss.bits);
+  ss = Session();
+  return GCMI_OK;
+}
+
+int gcmi_server_submit_image(gcmi_server* s, int session, const int32_t* h_source,
+                             const int32_t* h_sink, const int32_t* h_right,
+                             const int32_t* h_down, int64_t* req_out) {
+  const int64_t t_arrive = now_ns();
+  if (!s || !h_source || !h_sink || !h_right || !h_down) {
+    set_error("null argument");
+    return GCMI_EINVAL;
+  }
+  size_t npx;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    GCMI_TRY(check_session(s, session));
+    npx = (size_t)s->sessions[session].W * s->sessions[session].H;
+  }
+  int32_t maxw = 0;
+  GCMI_TRY(validate_caps(h_source, h_sink, h_right, h_down, npx, &maxw));
+  const long long len = ((long long)(4 * npx) + kStageAlign - 1) / kStageAlign * kStageAlign;
+  if (len > s->stage_alloc.cap) {
+    set_error("image of %zu pixels exceeds the staging area", npx);
+    return GCMI_EBUDGET;
+  }
+  Request* r = new (std::nothrow) Request();
+  if (!r) {
+    set_error("host allocation failed");
+    return GCMI_ENOMEM;
+  }
+  r->session = session;
+  r->kind = REQ_IMAGE;
+  r->max_pair = maxw;
+  r->t_arrive = t_arrive;
+  r->stage_len = len;
+  {
+    // wait for staging room: the worker frees ranges as batches complete
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return (r->stage = s->stage_alloc.alloc(len)) >= 0; });
+  }
+  {
+    std::lock_guard<std::mutex> clk(s->copy_mu);
+    int32_t* d = s->d_stage + r->stage;
+    const int32_t* src[4] = {h_source, h_sink, h_right, h_down};
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 4 && e == hipSuccess; ++k)
+      e = hipMemcpyAsync(d + k * npx, src[k], npx * 4, hipMemcpyHostToDevice, s->copy_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->copy_stream);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("image staging copy failed: %s", hipGetErrorString(e));
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stage_alloc.release(r->stage, len);
+      s->cv_done.notify_all();
+      delete r;
+      return GCMI_EHIP;
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->sessions[session].max_pair = maxw;
+  }
+  enqueue(s, r, req_out);
+  return GCMI_OK;
+}
+
+int gcmi_server_submit_stroke(gcmi_server* s, int session, int n, const int32_t* h_index,
+                              const int32_t* h_source, const int32_t* h_sink,
+                              int64_t* req_out) {
+  const int64_t t_arrive = now_ns();
+  if (!s || n < 0 || (n > 0 && (!h_index || !h_source || !h_sink))) {
+    set_error("bad stroke arguments");
+    return GCMI_EINVAL;
+  }
+  int32_t wmax;
+  long long npx;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    GCMI_TRY(check_session(s, session));
+    wmax = s->sessions[session].max_pair;
+    npx = (long long)s->sessions[session].W * s->sessions[session].H;
+  }
+  // the rules of a warm edit, checked here so that both modes accept and
+  // refuse the same strokes
+  if (wmax < 0) {
+    set_error("stroke before the session's first image");
+    return GCMI_EINVAL;
+  }
+  static thread_local std::vector<uint64_t> mark;
+  GCMI_TRY(check_edit(npx, n, h_index, h_source, h_sink, wmax, mark));
+  Request* r = new (std::nothrow) Request();
+  if (!r) {
+    set_error("host allocation failed");
+    return GCMI_ENOMEM;
+  }
+  r->session = session;
+  r->kind = REQ_STROKE;
+  r->n = n;
+  r->t_arrive = t_arrive;
+  r->stroke.resize(3 * (size_t)n);
+  if (n) {
+    memcpy(r->stroke.data(), h_index, (size_t)n * 4);
+    memcpy(r->stroke.data() + n, h_source, (size_t)n * 4);
+    memcpy(r->stroke.data() + 2 * n, h_sink, (size_t)n * 4);
+  }
+  enqueue(s, r, req_out);
+  return GCMI_OK;
+}
+
+int gcmi_server_wait(gcmi_server* s, int64_t req, int timeout_ms, int64_t* info_out) {
+  if (!s) {
+    set_error("null server");
+    return GCMI_EINVAL;
+  }
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(req);
+  if (it == s->reqs.end()) {
+    set_error("unknown request %lld", (long long)req);
+    return GCMI_EINVAL;
+  }
+  Request* r = it->second;
+  auto done = [&] { return r->status != ST_QUEUED; };
+  if (timeout_ms < 0)
+    s->cv_done.wait(lk, done);
+  else
+    s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done);
+  if (info_out) {
+    info_out[0] = r->status;
+    info_out[1] = r->flow;
+    info_out[2] = r->t_arrive;
+    info_out[3] = r->t_start;
+    info_out[4] = r->t_done;
+    info_out[5] = r->batch;
+  }
+  if (r->status == ST_QUEUED) {
+    set_error("request %lld still in flight", (long long)req);
+    return GCMI_EBUSY;
+  }
+  if (r->status == ST_FAILED) {
+    set_error("request %lld failed: %s", (long long)req, s->err);
+    return GCMI_EHIP;
+  }
+  return GCMI_OK;
+}
+
+int gcmi_server_mask(gcmi_server* s, int session, uint32_t* h_bits, int64_t* req_out) {
+  if (!s) {
+    set_error("null server");
+    return GCMI_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  GCMI_TRY(check_session(s, session));
+  const Session& ss = s->sessions[session];
+  if (ss.busy) {
+    set_error("session %d is being served", session);
+    return GCMI_EBUSY;
+  }
+  if (h_bits) memcpy(h_bits, ss.bits, (((size_t)ss.W * ss.H + 31) / 32) * 4);
+  if (req_out) *req_out = ss.last_done;
+  return GCMI_OK;
+}
+
+int gcmi_server_stats(gcmi_server* s, int64_t* out, int n) {
+  if (!s || !out || n < 0) {
+    set_error("bad argument");
+    return GCMI_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  const int T = s->pool.T;
+  int64_t open = 0, used = 0;
+  for (const Session& ss : s->sessions)
+    if (ss.open) {
+      ++open;
+      used += (int64_t)((ss.W + T - 1) / T) * ((ss.H + T - 1) / T);
+    }
+  const int64_t v[7] = {s->iterations, s->solved_requests, (int64_t)s->queue.size(), open, used,
+                        s->pool.cap_tiles, (int64_t)s->pool.device_bytes};
+  for (int k = 0; k < n && k < 7; ++k) out[k] = v[k];
+  return GCMI_OK;
+}
+
+int gcmi_server_forget(gcmi_server* s, int64_t req) {
+  if (!s) {
+    set_error("null server");
+    return GCMI_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(req);
+  if (it == s->reqs.end()) return GCMI_OK;
+  if (it->second->status == ST_QUEUED) {
+    set_error("request %lld still in flight", (long long)req);
+    return GCMI_EBUSY;
+  }
+  delete it->second;
+  s->reqs.erase(it);
+  return GCMI_OK;
+}
+
+}  // extern "C"

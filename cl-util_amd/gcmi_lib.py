@@ -95,6 +95,17 @@ SIGNATURES = {
     "gcmi_batch_stats": [_vp, _i64p, _int],
     "gcmi_batch_profile": [_vp, _int],
     "gcmi_batch_kernel_times": [_vp, ctypes.POINTER(ctypes.c_double), _i64p, _int],
+    "gcmi_server_create": [_int, _sz, _int, _int, _int, ctypes.POINTER(_vp)],
+    "gcmi_server_destroy": [_vp],
+    "gcmi_server_set_params": [_vp, _int, _int],
+    "gcmi_server_open": [_vp, _int, _int, ctypes.POINTER(_int)],
+    "gcmi_server_close": [_vp, _int],
+    "gcmi_server_submit_image": [_vp, _int, _vp, _vp, _vp, _vp, _i64p],
+    "gcmi_server_submit_stroke": [_vp, _int, _int, _vp, _vp, _vp, _i64p],
+    "gcmi_server_wait": [_vp, _i64, _int, _i64p],
+    "gcmi_server_mask": [_vp, _int, _vp, _i64p],
+    "gcmi_server_stats": [_vp, _i64p, _int],
+    "gcmi_server_forget": [_vp, _i64],
 }
 
 KERNEL_CLASSES = ("push_first", "push", "fold", "bfs_init", "bfs_round", "flood",

@@ -218,6 +218,66 @@ int gcmi_batch_profile(gcmi_batch *batch, int enable);
 int gcmi_batch_kernel_times(gcmi_batch *batch, double *ms_out,
                             int64_t *launches_out, int n);
 
+/* ---------------------------------------------------------------- serving
+ * Resident sessions and a continuously batching worker.  No reference
+ * counterpart: the reference serves one interactive user
+ * (test_GraphCut.py:74-75 re-uploads the excess image and re-cuts after
+ * each brush stroke).  Each session is one image of a pool sized by an HBM
+ * budget; its residual graph, heights and tile state stay in device memory
+ * between requests.  A worker thread takes every queued request it can (one
+ * per session, at most max_batch, oldest first), applies them and solves
+ * those sessions in one pool run; requests that arrive meanwhile go into
+ * the next run.  Strokes reflow warm from the session's residual graph
+ * (as gcmi_cut_edit).  The handle is thread-safe: any thread may submit,
+ * wait and read masks.
+ *   mode 0: batched warm serving; mode 1: plain -- one request at a time,
+ *   the session's unaries re-uploaded and cut from scratch. */
+typedef struct gcmi_server gcmi_server;
+
+/* hbm_budget: device bytes the session pool may use (GCMI_EBUDGET if too
+ * small for one tile); max_sessions: session slots; max_batch: sessions
+ * per solve. */
+int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions,
+                       int max_batch, int mode, gcmi_server **out);
+/* Stops the worker and frees everything; queued requests are dropped.  No
+ * other call on this server may be in progress or follow. */
+int gcmi_server_destroy(gcmi_server *server);
+/* sweeps / bfs of every later solve (<= 0: defaults, as gcmi_cut_run). */
+int gcmi_server_set_params(gcmi_server *server, int sweeps, int bfs);
+/* Admits a width x height session if a slot and pool room are free
+ * (GCMI_EBUDGET otherwise); *session_out = its id. */
+int gcmi_server_open(gcmi_server *server, int width, int height,
+                     int *session_out);
+/* Waits for the session's queued requests, then frees its room. */
+int gcmi_server_close(gcmi_server *server, int session);
+/* Queues a new image (capacities as gcmi_cut_upload, copied at once);
+ * *req_out = request id. */
+int gcmi_server_submit_image(gcmi_server *server, int session,
+                             const int32_t *h_source, const int32_t *h_sink,
+                             const int32_t *h_right, const int32_t *h_down,
+                             int64_t *req_out);
+/* Queues a brush stroke: new source/sink capacities for n distinct pixels
+ * (checked as gcmi_cut_edit; needs a submitted image). */
+int gcmi_server_submit_stroke(gcmi_server *server, int session, int n,
+                              const int32_t *h_index, const int32_t *h_source,
+                              const int32_t *h_sink, int64_t *req_out);
+/* Waits up to timeout_ms (< 0: no limit) for a request.  info_out (6 int64,
+ * may be NULL): status (0 queued, 1 done, 2 failed), flow, arrival, start
+ * and completion times (ns, steady clock), sessions solved with it.
+ * GCMI_EBUSY if still queued at the timeout. */
+int gcmi_server_wait(gcmi_server *server, int64_t req, int timeout_ms,
+                     int64_t *info_out);
+/* Bit-packed mask of the session's last completed request (bit p%32 of
+ * word p/32 = label of pixel p, ceil(width*height/32) words); *req_out
+ * (may be NULL) = that request's id.  GCMI_EBUSY while being solved. */
+int gcmi_server_mask(gcmi_server *server, int session, uint32_t *h_bits,
+                     int64_t *req_out);
+/* [0] solves  [1] requests served  [2] queued  [3] open sessions
+ * [4] pool tiles in use  [5] pool tile capacity  [6] device bytes */
+int gcmi_server_stats(gcmi_server *server, int64_t *out, int n);
+/* Drops the record of a completed request. */
+int gcmi_server_forget(gcmi_server *server, int64_t req);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,3 +45,20 @@ def test_no_device_fails_loudly():
     from GraphCut import GraphCut
     with pytest.raises(L.GcmiError):
         GraphCut(None, None, (64, 64))
+
+
+def test_server_argument_checks():
+    h = ctypes.c_void_p()
+    assert L.lib().gcmi_server_create(48, 1 << 30, 4, 4, 0, ctypes.byref(h)) == L.EINVAL
+    assert L.lib().gcmi_server_create(32, 1 << 30, 0, 4, 0, ctypes.byref(h)) == L.EINVAL
+    assert L.lib().gcmi_server_create(32, 1 << 30, 4, 4, 2, ctypes.byref(h)) == L.EINVAL
+    assert L.lib().gcmi_server_wait(None, 1, 0, None) == L.EINVAL
+    assert L.lib().gcmi_server_destroy(None) == L.OK
+
+
+@pytest.mark.skipif(L.device_count() > 0, reason="a GPU is present")
+def test_server_without_device_fails_loudly():
+    from GraphCutServer import GraphCutServer
+    with pytest.raises(L.GcmiError) as e:
+        GraphCutServer(hbm_budget=1 << 30, max_sessions=4)
+    assert e.value.code == L.ENODEVICE
