@@ -37,6 +37,11 @@ def _i32(a, shape):
     return a
 
 
+def _points(points):
+    xy = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+    return xy
+
+
 def _edit_arrays(index, source, sink):
     idx = np.ascontiguousarray(index, dtype=np.int32).reshape(-1)
     s = np.ascontiguousarray(source, dtype=np.int32).reshape(-1)
@@ -106,6 +111,14 @@ class GraphCut:
         capacities."""
         idx, s, t = _edit_arrays(index, source, sink)
         check(lib().gcmi_cut_edit(self._h, idx.size, host_ptr(idx), host_ptr(s), host_ptr(t)))
+
+    def paint(self, points, width, source, sink):
+        """Warm-start edit by a brush stroke rasterised on the device: every
+        pixel whose centre lies within width/2 of the polyline `points`
+        ([n][2] (x, y), n <= 16) gets capacities (source, sink)."""
+        xy = _points(points)
+        check(lib().gcmi_cut_paint(self._h, xy.shape[0], host_ptr(xy), float(width),
+                                   int(source), int(sink)))
 
     def upload_device(self, d_source, d_sink, d_right, d_down):
         check(lib().gcmi_cut_upload_device(self._h, ptr(d_source), ptr(d_sink), ptr(d_right),

@@ -11,7 +11,7 @@ import ctypes
 
 import numpy as np
 
-from GraphCut import STAT_NAMES, _edit_arrays, _i32
+from GraphCut import STAT_NAMES, _edit_arrays, _i32, _points
 from gcmi_lib import KERNEL_CLASSES, check, host_ptr, lib, ptr
 
 
@@ -62,6 +62,12 @@ class GraphCutBatch:
         idx, s, t = _edit_arrays(index, source, sink)
         check(lib().gcmi_batch_edit(self._h, int(i), idx.size, host_ptr(idx), host_ptr(s),
                                     host_ptr(t)))
+
+    def paint(self, i, points, width, source, sink):
+        """Brush stroke on image i (as GraphCut.paint)."""
+        xy = _points(points)
+        check(lib().gcmi_batch_paint(self._h, int(i), xy.shape[0], host_ptr(xy), float(width),
+                                     int(source), int(sink)))
 
     def cut(self, which=None, bfs=None, sweeps=None, max_rounds=0):
         """Solve all images (or the listed ones); returns their flow values."""

@@ -11,8 +11,9 @@ answer of every request is the same as a from-scratch GraphCut of the
 session's capacities at that point: the max-flow value and the minimal
 source set.
 
-mode "batched" is the serving loop; mode "plain" serves one request at a
-time, each cut from scratch, as the baseline.
+mode "batched" is the serving loop (strokes reflow warm); mode "cold"
+batches the same way but cuts every session from scratch; mode "plain"
+serves one request at a time, each cut from scratch, as the baseline.
 
 No counterpart in the reference, which serves one user per GraphCut object
 and re-uploads the excess image after each stroke (test_GraphCut.py:74-75).
@@ -21,23 +22,24 @@ import ctypes
 
 import numpy as np
 
-from GraphCut import _i32
-from gcmi_lib import EBUSY, GcmiError, check, host_ptr, lib
+from GraphCut import _i32, _points
+from gcmi_lib import (EBUSY, REQ_IMAGE, REQ_POLYLINE, REQ_STROKE, GcmiError, Request, _i64p,
+                      check, host_ptr, lib)
 
 QUEUED, DONE, FAILED = 0, 1, 2
-MODES = {"batched": 0, "plain": 1}
+MODES = {"batched": 0, "plain": 1, "cold": 2}
 STAT_NAMES = ("solves", "served", "queued", "sessions", "tiles_used", "tiles_capacity",
-              "device_bytes")
+              "device_bytes", "ns_apply", "ns_solve", "ns_out")
 
 
 class RequestInfo:
     """Completion record of one request (times in ns of a steady clock)."""
 
-    __slots__ = ("status", "flow", "t_arrive", "t_start", "t_done", "batch")
+    __slots__ = ("status", "flow", "t_arrive", "t_start", "t_done", "batch", "t_queued")
 
     def __init__(self, v):
-        self.status, self.flow, self.t_arrive, self.t_start, self.t_done, self.batch = \
-            (int(x) for x in v)
+        (self.status, self.flow, self.t_arrive, self.t_start, self.t_done, self.batch,
+         self.t_queued) = (int(x) for x in v)
 
     @property
     def latency_ms(self):
@@ -116,10 +118,62 @@ class GraphCutServer:
                                               host_ptr(src), host_ptr(snk), ctypes.byref(rid)))
         return rid.value
 
+    def submit_polyline(self, session, points, width, source, sink):
+        """Queues a brush stroke rasterised on the device (GraphCut.paint)."""
+        xy = _points(points)
+        rid = ctypes.c_int64()
+        check(lib().gcmi_server_submit_polyline(self._h, int(session), xy.shape[0],
+                                                host_ptr(xy), float(width), int(source),
+                                                int(sink), ctypes.byref(rid)))
+        return rid.value
+
+    def replay(self, trace):
+        """Open-loop load generator: trace is a list of (t_ns, session, kind,
+        payload) with t_ns on time.monotonic_ns()'s clock (<= 0: at once),
+        kind "image" (payload: source, sink, right, down), "stroke"
+        (index, source, sink) or "polyline" (points, width, source, sink).
+        Submits each request at its time from the calling thread (the GIL
+        is released meanwhile) and returns the request ids."""
+        n = len(trace)
+        reqs = (Request * n)()
+        keep = []
+        for i, (t_ns, session, kind, payload) in enumerate(trace):
+            q = reqs[i]
+            q.t_ns = int(t_ns)
+            q.session = int(session)
+            if kind == "image":
+                shape = self._shape(session)
+                a = [_i32(x, shape) for x in payload]
+                q.kind = REQ_IMAGE
+                q.p0, q.p1, q.p2, q.p3 = (x.ctypes.data for x in a)
+            elif kind == "stroke":
+                idx = np.ascontiguousarray(payload[0], dtype=np.int32).reshape(-1)
+                m = idx.size
+                s = np.ascontiguousarray(np.broadcast_to(payload[1], (m,)), dtype=np.int32)
+                t = np.ascontiguousarray(np.broadcast_to(payload[2], (m,)), dtype=np.int32)
+                a = [idx, s, t]
+                q.kind = REQ_STROKE
+                q.n = m
+                q.p0, q.p1, q.p2 = (x.ctypes.data for x in a)
+            elif kind == "polyline":
+                xy = _points(payload[0])
+                a = [xy]
+                q.kind = REQ_POLYLINE
+                q.n = xy.shape[0]
+                q.width = float(payload[1])
+                q.source, q.sink = int(payload[2]), int(payload[3])
+                q.p0 = xy.ctypes.data
+            else:
+                raise ValueError("unknown request kind %r" % (kind,))
+            keep.append(a)
+        ids = np.zeros(n, np.int64)
+        check(lib().gcmi_server_replay(self._h, n, reqs, ids.ctypes.data_as(_i64p)))
+        return ids
+
     def wait(self, request, timeout=None):
         """RequestInfo of a request; raises GcmiError(EBUSY) if it is not
         done within `timeout` seconds (None: no limit)."""
-        info = (ctypes.c_int64 * 6)()
+        info = (ctypes.c_int64 * 7)()
         ms = -1 if timeout is None else int(timeout * 1000)
         rc = lib().gcmi_server_wait(self._h, int(request), ms, info)
         if rc == EBUSY:
@@ -129,7 +183,7 @@ class GraphCutServer:
 
     def poll(self, request):
         """RequestInfo, status QUEUED while in flight (no wait)."""
-        info = (ctypes.c_int64 * 6)()
+        info = (ctypes.c_int64 * 7)()
         rc = lib().gcmi_server_wait(self._h, int(request), 0, info)
         if rc not in (0, EBUSY):
             check(rc)

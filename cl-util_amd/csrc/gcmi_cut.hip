@@ -103,6 +103,15 @@ int gcmi_cut_edit(gcmi_cut* c, int n, const int32_t* h_index, const int32_t* h_s
   return pool_edit(&c->pool, c->img, n, h_index, h_source, h_sink);
 }
 
+int gcmi_cut_paint(gcmi_cut* c, int npts, const double* h_xy, double width, int32_t source,
+                   int32_t sink) {
+  if (!c) {
+    set_error("null handle");
+    return GCMI_EINVAL;
+  }
+  return pool_paint(&c->pool, c->img, npts, h_xy, width, source, sink);
+}
+
 int gcmi_cut_run(gcmi_cut* c, int sweeps, int bfs, int64_t max_rounds, int64_t* flow_out) {
   if (!c) {
     set_error("null handle");
@@ -259,6 +268,12 @@ int gcmi_batch_edit(gcmi_batch* b, int i, int n, const int32_t* h_index,
                     const int32_t* h_source, const int32_t* h_sink) {
   GCMI_TRY(check_batch_img(b, i));
   return pool_edit(&b->pool, b->ids[i], n, h_index, h_source, h_sink);
+}
+
+int gcmi_batch_paint(gcmi_batch* b, int i, int npts, const double* h_xy, double width,
+                     int32_t source, int32_t sink) {
+  GCMI_TRY(check_batch_img(b, i));
+  return pool_paint(&b->pool, b->ids[i], npts, h_xy, width, source, sink);
 }
 
 int gcmi_batch_run(gcmi_batch* b, int sweeps, int bfs, int64_t max_rounds, int64_t* flows_out) {

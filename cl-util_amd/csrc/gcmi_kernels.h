@@ -64,8 +64,10 @@ struct CutDev {
   int32_t* pend;                       // pending edge inflow, 4*T per tile
   int32_t* pflag;                      // tile has (possibly) pending inflow
   int32_t* act;                        // tile flagged for the next push round
-  int32_t* bflag;                      // tile flagged for another BFS pass
-  int32_t* fflag;                      // tile flagged for another flood pass
+  int32_t* stamp;                      // per tile: epoch of its last queue entry
+  int32_t* bflag;                      // tile flagged by the relabel's first pass
+  int32_t* dirty;                      // residual arcs or excess changed since the
+                                       // tile's distances were last computed
   int32_t* haspos;                     // tile holds excess
   long long* tpos;                     // per tile: sum of max(e, 0)
   unsigned long long* sums;            // per image [SUM_ROWS][kSumSlots]
@@ -430,21 +432,25 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
         atomicAdd(&s.pend[((size_t)(t - q.tw) * 4 + 1) * T + lx], BO[0][lx]);
         s.pflag[t - q.tw] = 1;
         s.act[t - q.tw] = 1;
+        s.dirty[t - q.tw] = 1;
       }
       if (ly == T - 1 && q.ty < q.th - 1 && BO[1][lx]) {
         atomicAdd(&s.pend[((size_t)(t + q.tw) * 4 + 0) * T + lx], BO[1][lx]);
         s.pflag[t + q.tw] = 1;
         s.act[t + q.tw] = 1;
+        s.dirty[t + q.tw] = 1;
       }
       if (lx == 0 && q.tx > 0 && BO[2][ly]) {
         atomicAdd(&s.pend[((size_t)(t - 1) * 4 + 3) * T + ly], BO[2][ly]);
         s.pflag[t - 1] = 1;
         s.act[t - 1] = 1;
+        s.dirty[t - 1] = 1;
       }
       if (lx == T - 1 && q.tx < q.tw - 1 && BO[3][ly]) {
         atomicAdd(&s.pend[((size_t)(t + 1) * 4 + 2) * T + ly], BO[3][ly]);
         s.pflag[t + 1] = 1;
         s.act[t + 1] = 1;
+        s.dirty[t + 1] = 1;
       }
     }
     tp = block_sum64<NT>(tp, red);
@@ -454,6 +460,7 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
     }
     if (threadIdx.x == 0) {
       s.tpos[t] = tp;
+      s.dirty[t] = 1;
       if (active) s.act[t] = 1;
       if (FIRST) {
         amin.add(s.sums, SUM_MIN, q.img, smin);
@@ -522,6 +529,75 @@ k_fold(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
   }
 }
 
+// -------------------------------------------------------- tile queues
+// The BFS and flood rounds pass work on through queues of tiles: a round
+// reads its input list and appends the neighbour tiles it changed to its
+// output list.  A tile enters an output list once: the first appender swaps
+// the round's epoch into the tile's stamp.  Appends are gathered in LDS and
+// reserved with one atomic per block flush.  Rounds rotate three lists: a
+// round reads list i, appends to list i+1 and zeroes the count of list
+// i+2 (read by the previous round, written by the next).
+struct Queue {
+  const int32_t* in;     // input tiles (null: input positions are tiles)
+  const int32_t* in_count;
+  int32_t* out;
+  int32_t* out_count;
+  int32_t* clear_count;  // zeroed by the launch (may be null)
+  int epoch;             // stamp of the tiles appended to out
+};
+
+constexpr int kQCap = 256;
+struct BlockQ {
+  int32_t buf[kQCap];
+  int n, base, dirs;
+};
+
+__device__ __forceinline__ void bq_init(BlockQ& b, const Queue& Q) {
+  if (threadIdx.x == 0) {
+    b.n = 0;
+    b.dirs = 0;
+    if (blockIdx.x == 0 && Q.clear_count) *Q.clear_count = 0;
+  }
+  __syncthreads();
+}
+
+// Neighbours of tile q across the edges pixel (lx, ly) lies on, as bits
+// 1 << D_UP ... 1 << D_RIGHT (the same bits as M_UP ... M_RIGHT).
+template <int T>
+__device__ __forceinline__ int edge_dirs(const TG& q, int lx, int ly) {
+  return (ly == 0 && q.ty > 0 ? 1 << D_UP : 0) | (ly == T - 1 && q.ty < q.th - 1 ? 1 << D_DOWN : 0) |
+         (lx == 0 && q.tx > 0 ? 1 << D_LEFT : 0) | (lx == T - 1 && q.tx < q.tw - 1 ? 1 << D_RIGHT : 0);
+}
+
+// Appends the neighbours in b.dirs (complete: a barrier has passed since the
+// block's atomicOr into it) to the block's buffer; flushes a nearly full
+// buffer.  Called by all threads.
+__device__ __forceinline__ void bq_flush(BlockQ& b, const Queue& Q);
+__device__ __forceinline__ void bq_take(BlockQ& b, const TG& q, int32_t* stamp, const Queue& Q) {
+  if (threadIdx.x < 4 && ((b.dirs >> threadIdx.x) & 1)) {
+    const int k = threadIdx.x;
+    const int nb = k == D_UP ? q.t - q.tw : k == D_DOWN ? q.t + q.tw : k == D_LEFT ? q.t - 1 : q.t + 1;
+    if (atomicExch(&stamp[nb], Q.epoch) != Q.epoch) b.buf[atomicAdd(&b.n, 1)] = nb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) b.dirs = 0;
+  if (b.n > kQCap - 4) bq_flush(b, Q);
+}
+
+// Copies the block's buffer to the output list.  Called by all threads, a
+// barrier after the last change of b.n.
+__device__ __forceinline__ void bq_flush(BlockQ& b, const Queue& Q) {
+  const int n = b.n;
+  if (n == 0) return;
+  __syncthreads();
+  if (threadIdx.x == 0) b.base = atomicAdd(Q.out_count, n);
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) Q.out[b.base + i] = b.buf[i];
+  __syncthreads();
+  if (threadIdx.x == 0) b.n = 0;
+  __syncthreads();
+}
+
 // ------------------------------------------------------ global relabel
 // Distance of every pixel to the sink set {e < 0} over residual arcs
 // (MAX if unreachable): what init_bfs / bfs_intratile / bfs_intertile
@@ -566,30 +642,30 @@ __device__ __forceinline__ void bfs_relax(int (&d)[T * T / NT], const int (&m)[T
   }
 }
 
-// Flags the neighbour tiles across the edges pixel (lx, ly) lies on.
-template <int T>
-__device__ __forceinline__ void flag_edge_neighbours(const TG& q, int lx, int ly,
-                                                     int32_t* flags) {
-  if (ly == 0 && q.ty > 0) flags[q.t - q.tw] = 1;
-  if (ly == T - 1 && q.ty < q.th - 1) flags[q.t + q.tw] = 1;
-  if (lx == 0 && q.tx > 0) flags[q.t - 1] = 1;
-  if (lx == T - 1 && q.tx < q.tw - 1) flags[q.t + 1] = 1;
-}
-
-// First pass over the listed tiles, from the residual masks (pending inflow
-// has been folded): seeds, in-tile distances, and the completion test
-// (check_completion, graphcut.cl:20-60) for the tile:
+// First pass over the listed (dirty) tiles, from the residual masks
+// (pending inflow has been folded): seeds, in-tile distances, and the
+// completion test (check_completion, graphcut.cl:20-60) for the tile:
 //   act[t]    = some pixel has e > 0 and a finite distance
 //   haspos[t] = some pixel has e > 0
-// Halo pixels start as 0 (sink) or MAX; later rounds only lower distances,
-// so they can only switch act on.
+// all != 0: every tile of the images is listed (a full relabel).
+// Halo pixels of dirty neighbours start as 0 (sink) or MAX; those of clean
+// neighbours keep their stored distances, which stay valid: a clean tile's
+// arcs and excess have not changed since its distances were computed, and
+// the rounds lower any of them that a recomputed tile now undercuts.  Later
+// rounds only lower distances, so they can only switch act on.  Tiles next
+// to an edge pixel with a finite distance get bflag (in a full relabel
+// nearly every tile does, so a flag and one compaction beat a queue append
+// per one-tile block).
 template <int T, int NT>
 __global__ void __launch_bounds__(NT)
-k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count) {
+k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
+           int all) {
   constexpr int R = T * T / NT;
   constexpr int HP = T + 2;
   __shared__ int32_t D0[HP * HP], D1[HP * HP];
+  __shared__ int sdirs;
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
+  if (threadIdx.x == 0) sdirs = 0;
   const int n = *count;
   for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
     const TG q = tile_geo(g, list ? list[pos] : pos);
@@ -606,45 +682,61 @@ k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __r
       any_pos |= (m[k] & M_EXCESS) != 0;
     }
     if (threadIdx.x < 4 * T) {
+      // a neighbour being relabelled too starts at 0 (sink) or MAX; a clean
+      // one keeps its stored (valid) distance
       const Halo hs = halo_slot<T>(q, threadIdx.x);
       int v = g.maxh;
-      if (hs.nb >= 0 && (s.mask[tbase<T>(hs.nb) + hs.ny * T + hs.nx] & M_SINK)) v = 0;
+      if (hs.nb >= 0) {
+        const size_t p = tbase<T>(hs.nb) + hs.ny * T + hs.nx;
+        v = (all || s.dirty[hs.nb]) ? ((s.mask[p] & M_SINK) ? 0 : g.maxh) : s.h[p];
+      }
       D0[hs.idx] = v;
       D1[hs.idx] = v;
     }
     __syncthreads();
     bfs_relax<T, NT>(d, m, D0, D1);
-    int any_act = 0;
+    int any_act = 0, dirs = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       const int ly = chain * R + k;
       s.h[base + ly * T + lx] = d[k];
       any_act |= (m[k] & M_EXCESS) && d[k] < g.maxh;
-      // neighbours assumed MAX for this non-sink pixel; a finite distance is news
-      if (d[k] < g.maxh && !(m[k] & M_SINK)) flag_edge_neighbours<T>(q, lx, ly, s.bflag);
+      // news for a neighbour: it may now reach a sink through this pixel (in
+      // a full relabel every neighbour already seeded this tile's sinks)
+      if (d[k] < g.maxh && !(all && (m[k] & M_SINK))) dirs |= edge_dirs<T>(q, lx, ly);
     }
+    if (dirs) atomicOr(&sdirs, dirs);
     any_pos = __syncthreads_or(any_pos);
     any_act = __syncthreads_or(any_act);
     if (threadIdx.x == 0) {
       s.haspos[q.t] = any_pos;
       s.act[q.t] = any_act;
     }
+    if (threadIdx.x < 4 && ((sdirs >> threadIdx.x) & 1)) {
+      const int k = threadIdx.x;
+      s.bflag[k == D_UP ? q.t - q.tw : k == D_DOWN ? q.t + q.tw : k == D_LEFT ? q.t - 1 : q.t + 1] =
+          1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) sdirs = 0;
   }
 }
 
-// Later passes over flagged tiles, from the stored distances.
+// Later passes over queued tiles, from the stored distances.
 template <int T, int NT>
 __global__ void __launch_bounds__(NT)
-k_bfs_round(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count) {
+k_bfs_round(Geo g, CutDev s, Queue Q) {
   constexpr int R = T * T / NT;
   constexpr int HP = T + 2;
   __shared__ int32_t D0[HP * HP], D1[HP * HP];
+  __shared__ BlockQ bq;
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
-  const int n = *count;
+  const int n = *Q.in_count;
+  bq_init(bq, Q);
   if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0)
     atomicAdd(&s.tally[TALLY_BFS], (unsigned long long)n);
   for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
-    const TG q = tile_geo(g, list ? list[pos] : pos);
+    const TG q = tile_geo(g, Q.in[pos]);
     const size_t base = tbase<T>(q.t);
     int d[R], d0[R], m[R];
 #pragma unroll
@@ -663,19 +755,22 @@ k_bfs_round(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __
     }
     __syncthreads();
     bfs_relax<T, NT>(d, m, D0, D1);
-    int any_act = 0;
+    int any_act = 0, dirs = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       const int ly = chain * R + k;
       if (d[k] < d0[k]) {
         s.h[base + ly * T + lx] = d[k];
-        flag_edge_neighbours<T>(q, lx, ly, s.bflag);
+        dirs |= edge_dirs<T>(q, lx, ly);
         any_act |= (m[k] & M_EXCESS) != 0;
       }
     }
+    if (dirs) atomicOr(&bq.dirs, dirs);
     any_act = __syncthreads_or(any_act);
     if (threadIdx.x == 0 && any_act) s.act[q.t] = 1;
+    bq_take(bq, q, s.stamp, Q);
   }
+  bq_flush(bq, Q);
 }
 
 // ------------------------------------------------------ minimal source set
@@ -717,33 +812,25 @@ __device__ __forceinline__ void flood_relax(int (&r)[T * T / NT], int32_t* RM) {
   }
 }
 
-// Flags the neighbour tiles that pixel (lx, ly) has a residual arc into.
-template <int T>
-__device__ __forceinline__ void flag_out_arcs(const TG& q, int lx, int ly, int m,
-                                              int32_t* flags) {
-  if (ly == 0 && q.ty > 0 && (m & M_UP)) flags[q.t - q.tw] = 1;
-  if (ly == T - 1 && q.ty < q.th - 1 && (m & M_DOWN)) flags[q.t + q.tw] = 1;
-  if (lx == 0 && q.tx > 0 && (m & M_LEFT)) flags[q.t - 1] = 1;
-  if (lx == T - 1 && q.tx < q.tw - 1 && (m & M_RIGHT)) flags[q.t + 1] = 1;
-}
-
 // first = true: over the tiles holding excess, seeds from the masks' excess
 // bit, and the images' final sum of max(e, 0) from the tile sums; else: from
-// the stored reach bytes of flagged tiles, with halo reach.
+// the stored reach bytes of queued tiles, with halo reach.  A pixel newly
+// reached on an edge with a residual arc across it queues that neighbour.
 template <int T, int NT>
 __global__ void __launch_bounds__(NT)
-k_flood(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
-        int first) {
+k_flood(Geo g, CutDev s, Queue Q, int first) {
   constexpr int R = T * T / NT;
   constexpr int HP = T + 2;
   __shared__ int32_t RM[HP * HP];
+  __shared__ BlockQ bq;
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
-  const int n = *count;
+  const int n = *Q.in_count;
+  bq_init(bq, Q);
   if (!first && blockIdx.x == 0 && threadIdx.x == 0 && n > 0)
     atomicAdd(&s.tally[TALLY_FLOOD], (unsigned long long)n);
   ImgAcc apos;
   for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
-    const TG q = tile_geo(g, list ? list[pos] : pos);
+    const TG q = tile_geo(g, Q.in ? Q.in[pos] : pos);
     const size_t base = tbase<T>(q.t);
     if (first && threadIdx.x == 0) apos.add(s.sums, SUM_POSF, q.img, s.tpos[q.t]);
     int r[R], r0[R], m[R];
@@ -768,17 +855,21 @@ k_flood(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __rest
     }
     __syncthreads();
     flood_relax<T, NT>(r, RM);
+    int dirs = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       const int ly = chain * R + k;
       if (r[k] && !r0[k]) {
         s.reach[base + ly * T + lx] = 1;
-        flag_out_arcs<T>(q, lx, ly, m[k], s.fflag);
+        dirs |= edge_dirs<T>(q, lx, ly) & m[k];
       }
     }
+    if (dirs) atomicOr(&bq.dirs, dirs);
     __syncthreads();
+    bq_take(bq, q, s.stamp, Q);
   }
   if (first && threadIdx.x == 0) apos.flush(s.sums, SUM_POSF);
+  bq_flush(bq, Q);
 }
 
 // Row-major labels of the listed tiles' images from the reach bytes (a
@@ -813,8 +904,6 @@ __global__ void k_reset_tiles(CutDev s, const int32_t* __restrict__ list,
     if (threadIdx.x == 0) {
       s.pflag[t] = 0;
       s.act[t] = 0;
-      s.bflag[t] = 0;
-      s.fflag[t] = 0;
       s.haspos[t] = 0;
     }
   }
@@ -855,39 +944,47 @@ struct Prep {
   int img, sel_off, fresh_off, pad;
 };
 
-// Warm-start edit: new unaries (cs, ct) for n distinct pixels of image img
-// (row-major indices local to the image).  Updates the stored unaries and,
-// if the image holds solved state (warm), e by the change of cs - ct, the
-// mask's sink/excess bits, the tile's positive-excess sum and the image's
-// sums of min(cs, ct) and max(e0, 0).  Heights are left stale: the next
-// solve starts with a global relabel.
+// One pixel's share of a warm-start edit: new unaries (a1, b1) for pixel p
+// (row-major index local to the image).  Updates the stored unaries and, if
+// the image holds solved state (warm), e by the change of cs - ct, the
+// mask's sink/excess bits and the tile's positive-excess sum; the changes
+// of min(cs, ct) and max(cs - ct, 0) are added to dmin / dpos for the
+// image's sums.  A pixel whose excess changed marks its tile active, and a
+// new deficit gets height 0; other heights stay as they are.  The next
+// solve pushes on the active tiles before its first global relabel; the
+// labelling may be invalid until then, which costs pushes but not
+// correctness, since termination is decided by an exact relabel.
 template <int T>
-__global__ void __launch_bounds__(256)
-k_edit(Geo g, CutDev s, int img, int n, const int32_t* __restrict__ idx,
-       const int32_t* __restrict__ ncs, const int32_t* __restrict__ nct, int warm) {
-  __shared__ long long red[256 / kWave];
-  const ImgDesc d = g.img[img];
-  long long dmin = 0, dpos = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int p = idx[i];
-    const int x = p % d.W, y = p / d.W;
-    const size_t r = d.pix0 + (size_t)p;
-    const int a0 = s.cs[r], b0 = s.ct[r], a1 = ncs[i], b1 = nct[i];
-    s.cs[r] = a1;
-    s.ct[r] = b1;
-    if (!warm) continue;  // no state yet: the first round reads the unaries
-    const int t = d.tile0 + (y / T) * d.tw + x / T;
-    const size_t q = tbase<T>(t) + (y % T) * T + (x % T);
-    const int e0 = s.e[q];
-    const int e1 = e0 + (a1 - b1) - (a0 - b0);
-    s.e[q] = e1;
-    s.mask[q] = (uint8_t)((s.mask[q] & (M_UP | M_DOWN | M_LEFT | M_RIGHT)) |
-                          (e1 < 0 ? M_SINK : 0) | (e1 > 0 ? M_EXCESS : 0));
-    const long long dt = (long long)(e1 > 0 ? e1 : 0) - (e0 > 0 ? e0 : 0);
-    if (dt) atomicAdd(reinterpret_cast<unsigned long long*>(&s.tpos[t]), (unsigned long long)dt);
-    dmin += (long long)min(a1, b1) - min(a0, b0);
-    dpos += (long long)(a1 - b1 > 0 ? a1 - b1 : 0) - (a0 - b0 > 0 ? a0 - b0 : 0);
-  }
+__device__ __forceinline__ void edit_pixel(const CutDev& s, const ImgDesc& d, int x, int y, int a1,
+                                           int b1, int warm, long long& dmin, long long& dpos) {
+  const size_t r = d.pix0 + (size_t)y * d.W + x;
+  const int a0 = s.cs[r], b0 = s.ct[r];
+  s.cs[r] = a1;
+  s.ct[r] = b1;
+  if (!warm) return;  // no state yet: the first round reads the unaries
+  dmin += (long long)min(a1, b1) - min(a0, b0);
+  dpos += (long long)(a1 - b1 > 0 ? a1 - b1 : 0) - (a0 - b0 > 0 ? a0 - b0 : 0);
+  const int de = (a1 - b1) - (a0 - b0);
+  if (de == 0) return;
+  const int t = d.tile0 + (y / T) * d.tw + x / T;
+  const size_t q = tbase<T>(t) + (y % T) * T + (x % T);
+  const int e0 = s.e[q];
+  const int e1 = e0 + de;
+  s.e[q] = e1;
+  s.mask[q] = (uint8_t)((s.mask[q] & (M_UP | M_DOWN | M_LEFT | M_RIGHT)) |
+                        (e1 < 0 ? M_SINK : 0) | (e1 > 0 ? M_EXCESS : 0));
+  // a new deficit is a sink at height 0; the tile pushes before the first
+  // relabel of the next solve
+  if (e1 < 0) s.h[q] = 0;
+  s.act[t] = 1;
+  s.dirty[t] = 1;
+  const long long dt = (long long)(e1 > 0 ? e1 : 0) - (e0 > 0 ? e0 : 0);
+  if (dt) atomicAdd(reinterpret_cast<unsigned long long*>(&s.tpos[t]), (unsigned long long)dt);
+}
+
+// Adds a block's edit deltas to the image's sums (slotted atomics).
+__device__ __forceinline__ void edit_sums(const CutDev& s, int img, long long dmin,
+                                          long long dpos, long long* red) {
   dmin = block_sum64<256>(dmin, red);
   dpos = block_sum64<256>(dpos, red);
   if (threadIdx.x == 0) {
@@ -899,6 +996,75 @@ k_edit(Geo g, CutDev s, int img, int n, const int32_t* __restrict__ idx,
       atomicAdd(&s.sums[((size_t)img * SUM_ROWS + SUM_POS0) * kSumSlots + slot],
                 (unsigned long long)dpos);
   }
+}
+
+// Warm-start edit of n distinct pixels of image img: idx are row-major
+// indices local to the image, (ncs, nct) their new unaries.
+template <int T>
+__global__ void __launch_bounds__(256)
+k_edit(Geo g, CutDev s, int img, int n, const int32_t* __restrict__ idx,
+       const int32_t* __restrict__ ncs, const int32_t* __restrict__ nct, int warm) {
+  __shared__ long long red[256 / kWave];
+  const ImgDesc d = g.img[img];
+  long long dmin = 0, dpos = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int p = idx[i];
+    edit_pixel<T>(s, d, p % d.W, p / d.W, ncs[i], nct[i], warm, dmin, dpos);
+  }
+  edit_sums(s, img, dmin, dpos, red);
+}
+
+// A brush stroke: the pixels whose centre (x, y) lies within width/2 of a
+// polyline -- a union of capsules, one per segment -- get unaries (cs, ct).
+constexpr int kMaxStrokePoints = 16;
+struct Polyline {
+  double x[kMaxStrokePoints], y[kMaxStrokePoints];
+  double r;  // width / 2
+  int npts;
+  int32_t cs, ct;
+  int bx0, by0, bw, bh;  // pixel box covering the stroke, clipped to the image
+};
+
+// Whether pixel centre (px, py) lies within r of segment k: the point of the
+// segment nearest to it is a + t (b - a), t = clamp(((p - a).(b - a)) /
+// |b - a|^2, 0, 1) (t = 0 for a zero-length segment), and the test is
+// dx^2 + dy^2 <= r^2.  Products and sums are rounded one at a time, in this
+// order, so that the pixel set is reproducible on any IEEE float64 host.
+__device__ __forceinline__ bool in_capsule(const Polyline& pl, int k, double px, double py) {
+#pragma clang fp contract(off)
+  const double x0 = pl.x[k], y0 = pl.y[k];
+  const double dx = pl.x[k + 1] - x0, dy = pl.y[k + 1] - y0;
+  const double L2 = dx * dx + dy * dy;
+  double t = 0.0;
+  if (L2 > 0.0) {
+    const double u = (px - x0) * dx;
+    const double v = (py - y0) * dy;
+    t = (u + v) / L2;
+    t = fmin(fmax(t, 0.0), 1.0);
+  }
+  const double qx = x0 + t * dx, qy = y0 + t * dy;
+  const double ex = px - qx, ey = py - qy;
+  const double d2 = ex * ex + ey * ey;
+  return d2 <= pl.r * pl.r;
+}
+
+// Paints a polyline stroke into image img: one thread per pixel of the
+// stroke's box.
+template <int T>
+__global__ void __launch_bounds__(256)
+k_paint(Geo g, CutDev s, int img, Polyline pl, int warm) {
+  __shared__ long long red[256 / kWave];
+  const ImgDesc d = g.img[img];
+  long long dmin = 0, dpos = 0;
+  const long long n = (long long)pl.bw * pl.bh;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int x = pl.bx0 + (int)(i % pl.bw), y = pl.by0 + (int)(i / pl.bw);
+    bool hit = false;
+    for (int k = 0; k + 1 < pl.npts && !hit; ++k) hit = in_capsule(pl, k, (double)x, (double)y);
+    if (hit) edit_pixel<T>(s, d, x, y, pl.cs, pl.ct, warm, dmin, dpos);
+  }
+  edit_sums(s, img, dmin, dpos, red);
 }
 
 }  // namespace gcmi

@@ -15,15 +15,17 @@ namespace gcmi {
 
 constexpr int kDefaultSweeps = 4;  // in-tile sweeps per residency
 constexpr int kDefaultBfs = 1;     // push rounds per global relabel
-constexpr int kRoundBatch = 2;     // BFS / flood rounds launched per host check
+constexpr int kRoundBatch = 2;     // BFS / flood rounds before the first host check
+constexpr int kRoundBatchMax = 8;  // ... and before each later one
+constexpr int kRelabelGridCap = 65536;  // blocks of a relabel's first pass
 
 enum {
   CNT_ACT = 0,
-  CNT_BFS = 1,
-  CNT_POS = 2,
-  CNT_FLOOD = 3,
-  CNT_SEL = 4,
-  CNT_FRESH = 5,
+  CNT_POS = 1,
+  CNT_SEL = 2,
+  CNT_FRESH = 3,
+  CNT_Q = 4,  // counts of the three rotating tile queues
+  CNT_DIRTY = 7,
   CNT_N = 8
 };
 
@@ -76,14 +78,15 @@ struct PoolOps {
   int tile;
   int threads;
   void (*push)(Pool*, int grid, int sweeps, bool first);
-  void (*fold)(Pool*, int grid);
-  void (*bfs_init)(Pool*, int grid);
-  void (*bfs_round)(Pool*, int grid);
-  void (*flood)(Pool*, int grid, bool first);
+  void (*fold)(Pool*, int grid, const int32_t* list, int cnt);
+  void (*bfs_init)(Pool*, int grid, const int32_t* list, int cnt, bool all);
+  void (*bfs_round)(Pool*, int grid, const Queue& q);
+  void (*flood)(Pool*, int grid, const Queue& q, bool first);
   void (*labels)(Pool*, int grid);
   void (*reset_tiles)(Pool*, int grid);
   void (*clear_reach)(Pool*, int grid);
   void (*edit)(Pool*, int img, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct);
+  void (*paint)(Pool*, int img, const Polyline& pl);
   void (*untile)(Pool*, int img, const int32_t* src, int32_t* out);
 };
 
@@ -125,7 +128,13 @@ struct Pool {
   // device arrays (beyond those in dev)
   ImgDesc* d_img = nullptr;
   int32_t* tile_img = nullptr;
-  int32_t *sel = nullptr, *fresh = nullptr, *list = nullptr, *plist = nullptr, *alist = nullptr;
+  int32_t *sel = nullptr, *fresh = nullptr, *plist = nullptr, *alist = nullptr;
+  int32_t* q = nullptr;  // three rotating tile queues of cap_tiles entries
+  int32_t* dlist = nullptr;  // dirty tiles of the current relabel
+  // Relabel only the dirty tiles (experimental: cheaper relabels of small
+  // edits, but clean tiles keep valid yet possibly stale distances).
+  bool local_relabel = false;
+  int epoch = 0;         // last queue epoch handed out (stamps hold epochs)
   // the lists the kernels of the current solve read: null when the
   // selection is the whole pool in tile order (positions are tiles)
   int32_t *lsel = nullptr, *lfresh = nullptr;
@@ -177,6 +186,15 @@ int pool_edit_many(Pool* p, int count, const EditReq* e);
 // on return.
 int check_edit(long long npx, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct,
                int64_t wmax, std::vector<uint64_t>& mark);
+// Builds a stroke for a W x H image from npts (x, y) points (xy, 2 * npts
+// doubles), checking its arguments; a stroke of fewer than 2 points or
+// outside the image gets an empty box (bw = 0).
+int make_polyline(int W, int H, int npts, const double* xy, double width, int32_t cs, int32_t ct,
+                  int64_t wmax, Polyline* out);
+// Warm-start edit by a brush stroke: the pixels within width/2 of the
+// polyline get unaries (cs, ct).
+int pool_paint(Pool* p, int img, int npts, const double* xy, double width, int32_t cs,
+               int32_t ct);
 // Solves the listed images together (fresh ones from their capacities,
 // edited ones warm).  flows_out (may be NULL) receives one flow per image.
 int pool_solve(Pool* p, const int* img_ids, int n, int sweeps, int bfs, int64_t max_rounds,

@@ -4,9 +4,13 @@
 // images at once:
 //   prepare                  tile lists; zero the sums being recomputed
 //   fresh images: round 1    k_push<FIRST> builds state from the capacities
+//   edited images: round 1   k_push over the tiles the edits made active
 //   repeat:
-//     fold                   pending edge inflow -> state
-//     global relabel         k_bfs_init + rounds until no tile is flagged;
+//     fold                   pending edge inflow -> state (dirty tiles)
+//     global relabel         k_bfs_init over the dirty tiles (those whose
+//                            arcs or excess changed since their distances
+//                            were computed), then rounds over tile queues
+//                            until a queue is empty;
 //                            act = tiles with excess that can reach a sink
 //     stop if act is empty
 //     push round             k_push over the active tiles
@@ -18,6 +22,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "gcmi_pool.h"
 
@@ -88,7 +93,7 @@ __global__ void k_clear_listed(int32_t* flags, const int32_t* __restrict__ list,
                                const int32_t* __restrict__ count) {
   const int n = *count;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    flags[list[i]] = 0;
+    flags[list ? list[i] : i] = 0;
 }
 
 // Builds the solve's tile lists, zeroes the sums it recomputes (all rows of
@@ -136,21 +141,20 @@ struct PoolKernels {
       hipLaunchKernelGGL((k_push<T, NT, false>), dim3(grid), dim3(NT), 0, p->stream, p->geo,
                          p->dev, p->alist, p->counts + CNT_ACT, sweeps);
   }
-  static void fold(Pool* p, int grid) {
+  static void fold(Pool* p, int grid, const int32_t* list, int cnt) {
     hipLaunchKernelGGL((k_fold<T>), dim3(grid), dim3(kFoldNT), 0, p->stream, p->geo, p->dev,
-                       p->lsel, p->counts + CNT_SEL);
+                       list, p->counts + cnt);
   }
-  static void bfs_init(Pool* p, int grid) {
+  static void bfs_init(Pool* p, int grid, const int32_t* list, int cnt, bool all) {
     hipLaunchKernelGGL((k_bfs_init<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
-                       p->lsel, p->counts + CNT_SEL);
+                       list, p->counts + cnt, all ? 1 : 0);
   }
-  static void bfs_round(Pool* p, int grid) {
+  static void bfs_round(Pool* p, int grid, const Queue& q) {
     hipLaunchKernelGGL((k_bfs_round<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
-                       p->list, p->counts + CNT_BFS);
+                       q);
   }
-  static void flood(Pool* p, int grid, bool first) {
-    hipLaunchKernelGGL((k_flood<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
-                       first ? p->plist : p->list, p->counts + (first ? CNT_POS : CNT_FLOOD),
+  static void flood(Pool* p, int grid, const Queue& q, bool first) {
+    hipLaunchKernelGGL((k_flood<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev, q,
                        first ? 1 : 0);
   }
   static void labels(Pool* p, int grid) {
@@ -170,6 +174,12 @@ struct PoolKernels {
     const int grid = std::min((n + 255) / 256, 1024);
     hipLaunchKernelGGL((k_edit<T>), dim3(grid > 0 ? grid : 1), dim3(256), 0, p->stream, p->geo,
                        p->dev, img, n, idx, cs, ct, p->imgs[img].fresh ? 0 : 1);
+  }
+  static void paint(Pool* p, int img, const Polyline& pl) {
+    const long long n = (long long)pl.bw * pl.bh;
+    const int grid = (int)std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL((k_paint<T>), dim3(grid > 0 ? grid : 1), dim3(256), 0, p->stream, p->geo,
+                       p->dev, img, pl, p->imgs[img].fresh ? 0 : 1);
   }
   static void untile(Pool* p, int img, const int32_t* src, int32_t* out) {
     const PoolImage& im = p->imgs[img];
@@ -192,6 +202,7 @@ static const PoolOps* ops_for() {
                               &PoolKernels<T, NT>::reset_tiles,
                               &PoolKernels<T, NT>::clear_reach,
                               &PoolKernels<T, NT>::edit,
+                              &PoolKernels<T, NT>::paint,
                               &PoolKernels<T, NT>::untile};
   return &ops;
 }
@@ -261,7 +272,7 @@ int tiles_for(int W, int H, int tile, int* tw, int* th) {
 
 size_t pool_bytes(int tile, int cap_tiles, long long cap_pix, int cap_imgs) {
   const size_t np = (size_t)cap_tiles * tile * tile;
-  return np * (6 * 4 + 2) + (size_t)cap_tiles * (4 * tile * 4 + 11 * 4 + 8) +
+  return np * (6 * 4 + 2) + (size_t)cap_tiles * (4 * tile * 4 + 15 * 4 + 8) +
          (size_t)cap_pix * (4 * 4 + 1) +
          (size_t)cap_imgs * (sizeof(ImgDesc) + SUM_ROWS * kSumSlots * 8 + sizeof(Prep) + 8) +
          (size_t)compact_scratch_ints(cap_tiles) * 4 + 4096;
@@ -323,8 +334,9 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   ALLOC(d.pend, (size_t)cap_tiles * 4 * tile * 4);
   ALLOC(d.pflag, (size_t)cap_tiles * 4);
   ALLOC(d.act, (size_t)cap_tiles * 4);
+  ALLOC(d.stamp, (size_t)cap_tiles * 4);
   ALLOC(d.bflag, (size_t)cap_tiles * 4);
-  ALLOC(d.fflag, (size_t)cap_tiles * 4);
+  ALLOC(d.dirty, (size_t)cap_tiles * 4);
   ALLOC(d.haspos, (size_t)cap_tiles * 4);
   ALLOC(d.tpos, (size_t)cap_tiles * 8);
   ALLOC(d.sums, (size_t)cap_imgs * SUM_ROWS * kSumSlots * 8);
@@ -342,7 +354,8 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   ALLOC(p->tile_img, (size_t)cap_tiles * 4);
   ALLOC(p->sel, (size_t)cap_tiles * 4);
   ALLOC(p->fresh, (size_t)cap_tiles * 4);
-  ALLOC(p->list, (size_t)cap_tiles * 4);
+  ALLOC(p->q, (size_t)cap_tiles * 3 * 4);
+  ALLOC(p->dlist, (size_t)cap_tiles * 4);
   ALLOC(p->plist, (size_t)cap_tiles * 4);
   ALLOC(p->alist, (size_t)cap_tiles * 4);
   ALLOC(p->counts, CNT_N * 4);
@@ -375,7 +388,7 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   p->pix_alloc.init(cap_pix);
   // pending slots and flags start (and stay, between solves) zero
   GCMI_HIP(hipMemsetAsync(d.pend, 0, (size_t)cap_tiles * 4 * tile * 4, p->stream));
-  int32_t* flags[] = {d.pflag, d.act, d.bflag, d.fflag, d.haspos};
+  int32_t* flags[] = {d.pflag, d.act, d.stamp, d.bflag, d.dirty, d.haspos};
   for (int32_t* f : flags) GCMI_HIP(hipMemsetAsync(f, 0, (size_t)cap_tiles * 4, p->stream));
   GCMI_HIP(hipMemsetAsync(d.tpos, 0, (size_t)cap_tiles * 8, p->stream));
   GCMI_HIP(hipStreamSynchronize(p->stream));
@@ -385,10 +398,12 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
 void pool_free(Pool* p) {
   CutDev& d = p->dev;
   void* ptrs[] = {d.e,     d.h,       d.cu,     d.cd,       d.cl,         d.cr,       d.mask,
-                  d.reach, d.pend,    d.pflag,  d.act,      d.bflag,      d.fflag,    d.haspos,
+                  d.reach, d.pend,    d.pflag,  d.act,      d.stamp,      d.bflag,    d.dirty,
+                  d.haspos,
                   d.tpos,  d.sums,    d.tally,  d.cs,       d.ct,         (void*)d.right,
                   (void*)d.down,      d.labels, p->d_img,   p->tile_img,  p->sel,     p->fresh,
-                  p->list, p->plist,  p->alist, p->counts,  p->scratch,   p->d_prep,  p->d_flows,
+                  p->q,    p->dlist,  p->plist, p->alist,   p->counts,    p->scratch, p->d_prep,
+                  p->d_flows,
                   p->d_edit};
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   for (void* x : ptrs)
@@ -620,6 +635,75 @@ int pool_edit(Pool* p, int img, int n, const int32_t* idx, const int32_t* cs, co
   return pool_edit_many(p, 1, &e);
 }
 
+int make_polyline(int W, int H, int npts, const double* xy, double width, int32_t cs, int32_t ct,
+                  int64_t wmax, Polyline* out) {
+  if (npts < 0 || npts > kMaxStrokePoints || (npts > 0 && !xy)) {
+    set_error("a stroke has 0..%d points (got %d)", kMaxStrokePoints, npts);
+    return GCMI_EINVAL;
+  }
+  if (!(width >= 0.0) || !std::isfinite(width) || width > 1e6) {
+    set_error("stroke width must be finite and in [0, 1e6]");
+    return GCMI_EINVAL;
+  }
+  if (cs < 0 || ct < 0 || std::max<int64_t>(cs, ct) + 4 * wmax >= (1LL << 31)) {
+    set_error("stroke capacities out of range");
+    return GCMI_EINVAL;
+  }
+  Polyline& pl = *out;
+  pl = Polyline();
+  pl.npts = npts;
+  pl.r = width / 2.0;
+  pl.cs = cs;
+  pl.ct = ct;
+  double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+  for (int k = 0; k < npts; ++k) {
+    const double x = xy[2 * k], y = xy[2 * k + 1];
+    if (!std::isfinite(x) || !std::isfinite(y) || std::fabs(x) > 1e9 || std::fabs(y) > 1e9) {
+      set_error("stroke point %d is not finite or too far out", k);
+      return GCMI_EINVAL;
+    }
+    pl.x[k] = x;
+    pl.y[k] = y;
+    x0 = std::min(x0, x);
+    x1 = std::max(x1, x);
+    y0 = std::min(y0, y);
+    y1 = std::max(y1, y);
+  }
+  // pixels farther than r + 1 from every point's box cannot be hit
+  const long long bx0 = std::max<long long>((long long)std::floor(x0 - pl.r), 0);
+  const long long bx1 = std::min<long long>((long long)std::ceil(x1 + pl.r), W - 1);
+  const long long by0 = std::max<long long>((long long)std::floor(y0 - pl.r), 0);
+  const long long by1 = std::min<long long>((long long)std::ceil(y1 + pl.r), H - 1);
+  if (npts < 2 || bx0 > bx1 || by0 > by1) {
+    pl.bw = pl.bh = 0;
+    return GCMI_OK;
+  }
+  pl.bx0 = (int)bx0;
+  pl.by0 = (int)by0;
+  pl.bw = (int)(bx1 - bx0 + 1);
+  pl.bh = (int)(by1 - by0 + 1);
+  return GCMI_OK;
+}
+
+int pool_paint(Pool* p, int img, int npts, const double* xy, double width, int32_t cs,
+               int32_t ct) {
+  GCMI_TRY(check_img(p, img));
+  PoolImage& im = p->imgs[img];
+  if (!im.uploaded) {
+    set_error("stroke before upload");
+    return GCMI_EINVAL;
+  }
+  Polyline pl;
+  GCMI_TRY(make_polyline(im.W, im.H, npts, xy, width, cs, ct,
+                         im.max_pair >= 0 ? im.max_pair : (1 << 27), &pl));
+  if (pl.bw == 0) return GCMI_OK;
+  p->ops->paint(p, img, pl);
+  GCMI_HIP(hipGetLastError());
+  if (!im.fresh) im.edited = true;
+  im.solved = false;
+  return GCMI_OK;
+}
+
 // Compaction of flags over the solve's tiles (sel), output list + count.
 static int compact_sel(Pool* p, int32_t* flags, int32_t* out, int cnt, bool clear, int nsel) {
   Pred pr{};
@@ -632,6 +716,30 @@ static int compact_sel(Pool* p, int32_t* flags, int32_t* out, int cnt, bool clea
                          p->stream);
   if (P) P->end(p->stream);
   return rc;
+}
+
+// Zeroes the three queue counts (a fill kernel, so the solve holds no
+// memset nodes if it is ever captured).
+static void clear_queues(Pool* p) {
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, p->stream, p->counts + CNT_Q, 3LL, 0);
+}
+
+// A fresh queue epoch; on wrap-around every stamp is reset first.
+static int next_epoch(Pool* p) {
+  if (p->epoch >= (1 << 30)) {
+    hipLaunchKernelGGL(k_fill_i32, dim3((p->cap_tiles + 255) / 256), dim3(256), 0, p->stream,
+                       p->dev.stamp, (long long)p->cap_tiles, 0);
+    p->epoch = 0;
+  }
+  return ++p->epoch;
+}
+
+// The queue of a BFS or flood round reading rotating list `cur`.
+static Queue round_queue(Pool* p, int cur) {
+  const int nxt = (cur + 1) % 3, clr = (cur + 2) % 3;
+  const size_t cap = (size_t)p->cap_tiles;
+  return Queue{p->q + cur * cap, p->counts + CNT_Q + cur, p->q + nxt * cap,
+               p->counts + CNT_Q + nxt, p->counts + CNT_Q + clr, next_epoch(p)};
 }
 
 static int read_counts(Pool* p) {
@@ -690,6 +798,9 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
   p->stats[STAT_TILES] = nsel;
   const int grid_cap = p->grid_cap;
   auto grid_for = [&](int m) { return m < grid_cap ? (m > 0 ? m : 1) : grid_cap; };
+  // the relabel's first pass runs one block per dirty tile (their number is
+  // known on the device only; surplus blocks exit at once)
+  const int grid_relabel = std::min(nsel, kRelabelGridCap);
   int32_t* hc = p->h_counts;
 
   GCMI_HIP(hipMemcpyAsync(p->d_prep, p->h_prep, (size_t)n * sizeof(Prep), hipMemcpyHostToDevice,
@@ -709,6 +820,21 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
     p->stats[STAT_RESIDENCIES] += nfresh;
   }
   int32_t nact = 0;
+  if (nsel > nfresh) {
+    // edited images: a push round over the tiles the edits made active
+    // (with any the first round flagged) before the first relabel
+    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, true, nsel));
+    GCMI_TRY(read_counts(p));
+    nact = hc[CNT_ACT];
+    if (nact > 0) {
+      B(K_PUSH);
+      ops->push(p, grid_for(nact), sweeps, false);
+      E();
+      GCMI_HIP(hipGetLastError());
+      ++rounds;
+      p->stats[STAT_RESIDENCIES] += nact;
+    }
+  }
   for (;;) {
     // further push rounds before the global relabel
     for (int r = 1; r < bfs && rounds > 0; ++r) {
@@ -727,32 +853,44 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
       ++rounds;
       p->stats[STAT_RESIDENCIES] += nact;
     }
-    // fold pending flow, then global relabel + completion test
+    // fold pending flow into the dirty tiles (every tile a push changed or
+    // sent flow to), recompute their distances, and let the rounds carry the
+    // changes outward: a global relabel, restricted to where it can differ
+    // (or, by default, over every selected tile)
+    const int32_t* glist = p->lsel;
+    int gcnt = CNT_SEL;
+    if (p->local_relabel) {
+      GCMI_TRY(compact_sel(p, p->dev.dirty, p->dlist, CNT_DIRTY, false, nsel));
+      glist = p->dlist;
+      gcnt = CNT_DIRTY;
+    }
     B(K_FOLD);
-    ops->fold(p, nsel);
+    ops->fold(p, grid_relabel, glist, gcnt);
     E();
     B(K_BFS_INIT);
-    ops->bfs_init(p, nsel);
+    ops->bfs_init(p, grid_relabel, glist, gcnt, !p->local_relabel);
     E();
     GCMI_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_clear_listed, dim3(std::min((nsel + 255) / 256, 4096)), dim3(256), 0,
+                       p->stream, p->dev.dirty, glist, p->counts + gcnt);
     p->stats[STAT_GLOBAL_RELABELS]++;
-    // later BFS rounds, kRoundBatch per host check; a round whose list is
-    // empty exits at once
-    bool list_ready = false;
-    for (;;) {
-      for (int b = 0; b < kRoundBatch; ++b) {
-        if (!list_ready) GCMI_TRY(compact_sel(p, p->dev.bflag, p->list, CNT_BFS, true, nsel));
-        list_ready = false;
+    // the flagged tiles start the first queue
+    clear_queues(p);
+    GCMI_TRY(compact_sel(p, p->dev.bflag, p->q, CNT_Q, true, nsel));
+    int cur = 0;
+    // later BFS rounds, several per host check (a round with an empty input
+    // queue exits at once); the active list rides along with the check
+    for (int batch = kRoundBatch;; batch = kRoundBatchMax) {
+      for (int b = 0; b < batch; ++b) {
         B(K_BFS_ROUND);
-        ops->bfs_round(p, grid_cap);
+        ops->bfs_round(p, grid_cap, round_queue(p, cur));
         E();
         GCMI_HIP(hipGetLastError());
+        cur = (cur + 1) % 3;
       }
-      GCMI_TRY(compact_sel(p, p->dev.bflag, p->list, CNT_BFS, true, nsel));
       GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, false, nsel));
       GCMI_TRY(read_counts(p));
-      if (hc[CNT_BFS] == 0) break;
-      list_ready = true;
+      if (hc[CNT_Q + cur] == 0) break;
     }
     nact = hc[CNT_ACT];
     if (nact == 0) break;
@@ -774,22 +912,26 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
   p->stats[STAT_ROUNDS] = rounds;
 
   // minimal source set
-  GCMI_TRY(compact_sel(p, p->dev.haspos, p->plist, CNT_POS, true, nsel));
+  // haspos persists: a clean tile keeps the flag its last relabel wrote
+  GCMI_TRY(compact_sel(p, p->dev.haspos, p->plist, CNT_POS, false, nsel));
   ops->clear_reach(p, std::min(nsel, 4096));
+  clear_queues(p);
+  int cur = 0;
   B(K_FLOOD);
-  ops->flood(p, grid_cap, true);
+  ops->flood(p, grid_cap,
+             Queue{p->plist, p->counts + CNT_POS, p->q, p->counts + CNT_Q, nullptr, next_epoch(p)},
+             true);
   E();
   GCMI_HIP(hipGetLastError());
-  for (;;) {
-    for (int b = 0; b < kRoundBatch; ++b) {
-      GCMI_TRY(compact_sel(p, p->dev.fflag, p->list, CNT_FLOOD, true, nsel));
+  for (int batch = kRoundBatch;; batch = kRoundBatchMax) {
+    for (int b = 0; b < batch; ++b) {
       B(K_FLOOD);
-      ops->flood(p, grid_cap, false);
+      ops->flood(p, grid_cap, round_queue(p, cur), false);
       E();
       GCMI_HIP(hipGetLastError());
+      cur = (cur + 1) % 3;
     }
-    // anything flagged by the last round?
-    GCMI_TRY(compact_sel(p, p->dev.fflag, p->list, CNT_FLOOD, false, nsel));
+    // labels and flows ride along with the check
     B(K_LABELS);
     ops->labels(p, grid_for(nsel));
     E();
@@ -802,7 +944,7 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
     GCMI_HIP(hipMemcpyAsync(p->h_tally, p->dev.tally, TALLY_N * 8, hipMemcpyDeviceToHost,
                             p->stream));
     GCMI_TRY(read_counts(p));
-    if (hc[CNT_FLOOD] == 0) break;
+    if (hc[CNT_Q + cur] == 0) break;
   }
   if (P) P->collect();
   for (int i = 0; i < n; ++i) {

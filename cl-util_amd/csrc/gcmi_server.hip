@@ -21,6 +21,7 @@
 // In plain mode the worker serves one request at a time and cuts it from
 // scratch: a stroke rewrites the unaries on the device and the image is cut
 // from its capacities, as a client without resident solver state would.
+// Cold mode batches like the warm mode but cuts every session from scratch.
 //
 // No reference counterpart: the reference is a single-user tool
 // (test_GraphCut.py:74-75 re-uploads the excess image per stroke).
@@ -63,7 +64,7 @@ __global__ void k_pack_bits(const uint8_t* __restrict__ labels, long long n, uin
   }
 }
 
-enum { REQ_IMAGE = 0, REQ_STROKE = 1 };
+enum { REQ_IMAGE = GCMI_REQ_IMAGE, REQ_STROKE = GCMI_REQ_STROKE, REQ_POLYLINE = GCMI_REQ_POLYLINE };
 enum { ST_QUEUED = 0, ST_DONE = 1, ST_FAILED = 2 };
 
 constexpr long long kStageAlign = 64;  // staging offsets in int32 elements
@@ -77,7 +78,11 @@ struct Request {
   long long stage = -1;         // image: offset of cs, ct, right, down in staging
   long long stage_len = 0;
   int32_t max_pair = 0;         // image: largest pairwise weight
+  std::vector<double> xy;       // polyline: points
+  double width = 0;             // polyline
+  int32_t pcs = 0, pct = 0;     // polyline: unaries painted
   int64_t t_arrive = 0, t_start = 0, t_done = 0;
+  int64_t t_queued = 0;  // when it entered the queue (after staging)
   int64_t flow = 0;
   int status = ST_QUEUED;
   int batch = 0;  // sessions solved together with it
@@ -99,7 +104,7 @@ struct Session {
 struct gcmi_server {
   Pool pool;
   int device = 0;
-  int mode = 0;  // 0 batched warm, 1 plain per-request
+  int mode = 0;  // GCMI_SERVE_WARM, GCMI_SERVE_PLAIN or GCMI_SERVE_COLD
   int max_batch = 64;
   int sweeps = 0, bfs = 0;
   std::vector<Session> sessions;
@@ -118,6 +123,7 @@ struct gcmi_server {
   bool stop = false;
   std::thread worker;
   int64_t iterations = 0, solved_requests = 0;
+  int64_t ns_apply = 0, ns_solve = 0, ns_out = 0;  // worker time per phase
   char err[256] = "";
 };
 
@@ -126,6 +132,7 @@ namespace {
 // Applies the batch's requests to the pool, solves, and copies the packed
 // masks out.  Runs on the worker thread with pool_mu held.
 int serve_batch(gcmi_server* s, std::vector<Request*>& batch) {
+  const int64_t t0 = now_ns();
   Pool* p = &s->pool;
   std::vector<int> imgs;
   imgs.reserve(batch.size());
@@ -139,16 +146,22 @@ int serve_batch(gcmi_server* s, std::vector<Request*>& batch) {
       GCMI_TRY(pool_upload(p, ss.img, d, d + npx, d + 2 * npx, d + 3 * npx, true));
       im.max_pair = r->max_pair;
     } else {
-      // plain mode cuts every request from scratch: the edit only rewrites
-      // the unaries
-      if (s->mode == 1) im.fresh = true;
-      const int32_t* d = r->stroke.data();
-      edits.push_back(EditReq{ss.img, r->n, d, d + r->n, d + 2 * r->n});
+      // plain and cold modes cut every request from scratch: the edit only
+      // rewrites the unaries
+      if (s->mode != GCMI_SERVE_WARM) im.fresh = true;
+      if (r->kind == REQ_POLYLINE) {
+        GCMI_TRY(pool_paint(p, ss.img, r->n, r->xy.data(), r->width, r->pcs, r->pct));
+      } else {
+        const int32_t* d = r->stroke.data();
+        edits.push_back(EditReq{ss.img, r->n, d, d + r->n, d + 2 * r->n});
+      }
     }
     imgs.push_back(ss.img);
   }
   if (!edits.empty()) GCMI_TRY(pool_edit_many(p, (int)edits.size(), edits.data()));
+  const int64_t t1 = now_ns();
   GCMI_TRY(pool_solve(p, imgs.data(), (int)imgs.size(), s->sweeps, s->bfs, 0, nullptr));
+  const int64_t t2 = now_ns();
   long long off = 0;
   for (Request* r : batch) {
     const Session& ss = s->sessions[r->session];
@@ -164,6 +177,9 @@ int serve_batch(gcmi_server* s, std::vector<Request*>& batch) {
   }
   GCMI_HIP(hipStreamSynchronize(p->stream));
   for (Request* r : batch) r->flow = p->imgs[s->sessions[r->session].img].flow;
+  s->ns_apply += t1 - t0;
+  s->ns_solve += t2 - t1;
+  s->ns_out += now_ns() - t2;
   return GCMI_OK;
 }
 
@@ -174,7 +190,7 @@ void worker_main(gcmi_server* s) {
     s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
     if (s->stop) break;
     // oldest first, one request per session, at most max_batch
-    const int cap = s->mode == 1 ? 1 : s->max_batch;
+    const int cap = s->mode == GCMI_SERVE_PLAIN ? 1 : s->max_batch;
     std::vector<Request*> batch;
     for (auto it = s->queue.begin(); it != s->queue.end() && (int)batch.size() < cap;) {
       Session& ss = s->sessions[(*it)->session];
@@ -224,6 +240,7 @@ void worker_main(gcmi_server* s) {
 void enqueue(gcmi_server* s, Request* r, int64_t* id_out) {
   std::lock_guard<std::mutex> lk(s->mu);
   r->id = s->next_id++;
+  r->t_queued = now_ns();
   s->reqs[r->id] = r;
   s->queue.push_back(r);
   if (id_out) *id_out = r->id;
@@ -246,7 +263,8 @@ extern "C" {
 
 int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_batch, int mode,
                        gcmi_server** out) {
-  if (!out || max_sessions < 1 || max_batch < 1 || (mode != 0 && mode != 1)) {
+  if (!out || max_sessions < 1 || max_batch < 1 ||
+      (mode != GCMI_SERVE_WARM && mode != GCMI_SERVE_PLAIN && mode != GCMI_SERVE_COLD)) {
     set_error("bad server arguments");
     return GCMI_EINVAL;
   }
@@ -399,10 +417,13 @@ int gcmi_server_close(gcmi_server* s, int session) {
   return GCMI_OK;
 }
 
-int gcmi_server_submit_image(gcmi_server* s, int session, const int32_t* h_source,
-                             const int32_t* h_sink, const int32_t* h_right,
-                             const int32_t* h_down, int64_t* req_out) {
-  const int64_t t_arrive = now_ns();
+}  // extern "C"
+
+namespace {
+
+int submit_image_at(gcmi_server* s, int session, const int32_t* h_source, const int32_t* h_sink,
+                    const int32_t* h_right, const int32_t* h_down, int64_t t_arrive,
+                    int64_t* req_out) {
   if (!s || !h_source || !h_sink || !h_right || !h_down) {
     set_error("null argument");
     return GCMI_EINVAL;
@@ -461,10 +482,9 @@ int gcmi_server_submit_image(gcmi_server* s, int session, const int32_t* h_sourc
   return GCMI_OK;
 }
 
-int gcmi_server_submit_stroke(gcmi_server* s, int session, int n, const int32_t* h_index,
-                              const int32_t* h_source, const int32_t* h_sink,
-                              int64_t* req_out) {
-  const int64_t t_arrive = now_ns();
+int submit_stroke_at(gcmi_server* s, int session, int n, const int32_t* h_index,
+                     const int32_t* h_source, const int32_t* h_sink, int64_t t_arrive,
+                     int64_t* req_out) {
   if (!s || n < 0 || (n > 0 && (!h_index || !h_source || !h_sink))) {
     set_error("bad stroke arguments");
     return GCMI_EINVAL;
@@ -504,6 +524,104 @@ int gcmi_server_submit_stroke(gcmi_server* s, int session, int n, const int32_t*
   return GCMI_OK;
 }
 
+int submit_polyline_at(gcmi_server* s, int session, int npts, const double* h_xy, double width,
+                       int32_t source, int32_t sink, int64_t t_arrive, int64_t* req_out) {
+  if (!s) {
+    set_error("null server");
+    return GCMI_EINVAL;
+  }
+  int32_t wmax;
+  int W, H;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    GCMI_TRY(check_session(s, session));
+    wmax = s->sessions[session].max_pair;
+    W = s->sessions[session].W;
+    H = s->sessions[session].H;
+  }
+  if (wmax < 0) {
+    set_error("stroke before the session's first image");
+    return GCMI_EINVAL;
+  }
+  Polyline pl;
+  GCMI_TRY(make_polyline(W, H, npts, h_xy, width, source, sink, wmax, &pl));
+  Request* r = new (std::nothrow) Request();
+  if (!r) {
+    set_error("host allocation failed");
+    return GCMI_ENOMEM;
+  }
+  r->session = session;
+  r->kind = REQ_POLYLINE;
+  r->n = npts;
+  r->t_arrive = t_arrive;
+  r->xy.assign(h_xy, h_xy + 2 * (size_t)npts);
+  r->width = width;
+  r->pcs = source;
+  r->pct = sink;
+  enqueue(s, r, req_out);
+  return GCMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gcmi_server_submit_image(gcmi_server* s, int session, const int32_t* h_source,
+                             const int32_t* h_sink, const int32_t* h_right,
+                             const int32_t* h_down, int64_t* req_out) {
+  return submit_image_at(s, session, h_source, h_sink, h_right, h_down, now_ns(), req_out);
+}
+
+int gcmi_server_submit_stroke(gcmi_server* s, int session, int n, const int32_t* h_index,
+                              const int32_t* h_source, const int32_t* h_sink,
+                              int64_t* req_out) {
+  return submit_stroke_at(s, session, n, h_index, h_source, h_sink, now_ns(), req_out);
+}
+
+int gcmi_server_submit_polyline(gcmi_server* s, int session, int npts, const double* h_xy,
+                                double width, int32_t source, int32_t sink, int64_t* req_out) {
+  return submit_polyline_at(s, session, npts, h_xy, width, source, sink, now_ns(), req_out);
+}
+
+int gcmi_server_replay(gcmi_server* s, int count, const gcmi_request* reqs, int64_t* req_out) {
+  if (!s || count < 0 || (count > 0 && !reqs)) {
+    set_error("bad replay arguments");
+    return GCMI_EINVAL;
+  }
+  for (int i = 0; i < count; ++i) {
+    const gcmi_request& q = reqs[i];
+    int64_t t = q.t_ns;
+    if (t > 0) {
+      std::this_thread::sleep_until(
+          std::chrono::steady_clock::time_point(std::chrono::nanoseconds(t)));
+    } else {
+      t = now_ns();
+    }
+    int64_t id = 0;
+    int rc;
+    switch (q.kind) {
+      case GCMI_REQ_IMAGE:
+        rc = submit_image_at(s, q.session, (const int32_t*)q.p0, (const int32_t*)q.p1,
+                             (const int32_t*)q.p2, (const int32_t*)q.p3, t, &id);
+        break;
+      case GCMI_REQ_STROKE:
+        rc = submit_stroke_at(s, q.session, q.n, (const int32_t*)q.p0, (const int32_t*)q.p1,
+                              (const int32_t*)q.p2, t, &id);
+        break;
+      case GCMI_REQ_POLYLINE:
+        rc = submit_polyline_at(s, q.session, q.n, (const double*)q.p0, q.width, q.source, q.sink,
+                                t, &id);
+        break;
+      default:
+        set_error("request %d: unknown kind %d", i, q.kind);
+        rc = GCMI_EINVAL;
+    }
+    if (rc != GCMI_OK) return rc;
+    if (req_out) req_out[i] = id;
+  }
+  return GCMI_OK;
+}
+
 int gcmi_server_wait(gcmi_server* s, int64_t req, int timeout_ms, int64_t* info_out) {
   if (!s) {
     set_error("null server");
@@ -528,6 +646,7 @@ int gcmi_server_wait(gcmi_server* s, int64_t req, int timeout_ms, int64_t* info_
     info_out[3] = r->t_start;
     info_out[4] = r->t_done;
     info_out[5] = r->batch;
+    info_out[6] = r->t_queued;
   }
   if (r->status == ST_QUEUED) {
     set_error("request %lld still in flight", (long long)req);
@@ -570,9 +689,10 @@ int gcmi_server_stats(gcmi_server* s, int64_t* out, int n) {
       ++open;
       used += (int64_t)((ss.W + T - 1) / T) * ((ss.H + T - 1) / T);
     }
-  const int64_t v[7] = {s->iterations, s->solved_requests, (int64_t)s->queue.size(), open, used,
-                        s->pool.cap_tiles, (int64_t)s->pool.device_bytes};
-  for (int k = 0; k < n && k < 7; ++k) out[k] = v[k];
+  const int64_t v[10] = {s->iterations, s->solved_requests, (int64_t)s->queue.size(), open, used,
+                         s->pool.cap_tiles, (int64_t)s->pool.device_bytes, s->ns_apply,
+                         s->ns_solve, s->ns_out};
+  for (int k = 0; k < n && k < 10; ++k) out[k] = v[k];
   return GCMI_OK;
 }
 

@@ -84,6 +84,10 @@ SIGNATURES = {
     "gcmi_cut_profile": [_vp, _int],
     "gcmi_cut_kernel_times": [_vp, ctypes.POINTER(ctypes.c_double), _i64p, _int],
     "gcmi_cut_edit": [_vp, _int, _vp, _vp, _vp],
+    "gcmi_cut_paint": [_vp, _int, _vp, ctypes.c_double, _i32, _i32],
+    "gcmi_batch_paint": [_vp, _int, _int, _vp, ctypes.c_double, _i32, _i32],
+    "gcmi_server_submit_polyline": [_vp, _int, _int, _vp, ctypes.c_double, _i32, _i32, _i64p],
+    "gcmi_server_replay": [_vp, _int, _vp, _i64p],
     "gcmi_batch_create": [_int, _vp, _vp, _int, ctypes.POINTER(_vp)],
     "gcmi_batch_destroy": [_vp],
     "gcmi_batch_upload": [_vp, _int, _vp, _vp, _vp, _vp],
@@ -107,6 +111,19 @@ SIGNATURES = {
     "gcmi_server_stats": [_vp, _i64p, _int],
     "gcmi_server_forget": [_vp, _i64],
 }
+
+REQ_IMAGE, REQ_STROKE, REQ_POLYLINE = 0, 1, 2
+
+
+class Request(ctypes.Structure):
+    """struct gcmi_request of gcmi.h (one request of a replayed trace)."""
+    _fields_ = [("t_ns", ctypes.c_int64), ("session", ctypes.c_int32),
+                ("kind", ctypes.c_int32), ("n", ctypes.c_int32),
+                ("source", ctypes.c_int32), ("sink", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("width", ctypes.c_double),
+                ("p0", ctypes.c_void_p), ("p1", ctypes.c_void_p),
+                ("p2", ctypes.c_void_p), ("p3", ctypes.c_void_p)]
+
 
 KERNEL_CLASSES = ("push_first", "push", "fold", "bfs_init", "bfs_round", "flood",
                   "labels", "compact")

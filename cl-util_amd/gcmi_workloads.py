@@ -144,11 +144,18 @@ def tiny(W, H, seed, cap_hi=20, w_hi=10):
 
 
 class Stroke:
-    """A brush stroke: flat pixel indices it covers and its side."""
+    """A brush stroke: flat pixel indices it covers and its side, plus the
+    polyline (points, width) it was rasterised from, if any."""
 
-    def __init__(self, index, foreground):
+    def __init__(self, index, foreground, points=None, width=None):
         self.index = index            # int32 flat indices y*W + x, ascending
         self.foreground = foreground  # True: hard source; False: hard sink
+        self.points = points          # float64 [npts][2] (x, y)
+        self.width = width
+
+    def capacities(self):
+        """(source, sink) capacity painted by the stroke."""
+        return (HARD, 0) if self.foreground else (0, HARD)
 
     def values(self):
         n = self.index.size
@@ -182,9 +189,10 @@ def rasterise_polyline(W, H, pts, width):
     return np.flatnonzero(hit).astype(np.int32)
 
 
-def strokes(W, H, count=100, seed=3, first_foreground=True):
+def strokes(W, H, count=100, seed=3, first_foreground=True, raster=True):
     """`count` seeded strokes: polylines of 3-8 segments with vertices uniform
-    in the image, width 8-32 px, alternating foreground/background."""
+    in the image, width 8-32 px, alternating foreground/background.  With
+    raster=False the pixel sets are not computed (index is None)."""
     rng = np.random.default_rng(seed + 7919)
     out = []
     for i in range(count):
@@ -193,7 +201,8 @@ def strokes(W, H, count=100, seed=3, first_foreground=True):
                        rng.uniform(0, H - 1, nseg + 1)))
         width = int(rng.integers(8, 33))
         fg = (i % 2 == 0) == first_foreground
-        out.append(Stroke(rasterise_polyline(W, H, pts, width), fg))
+        idx = rasterise_polyline(W, H, pts, width) if raster else None
+        out.append(Stroke(idx, fg, np.array(pts, dtype=np.float64), width))
     return out
 
 

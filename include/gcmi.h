@@ -157,6 +157,15 @@ int gcmi_cut_run(gcmi_cut *cut, int sweeps, int bfs, int64_t max_rounds,
  * capacities.  Before the first run an edit only rewrites the capacities. */
 int gcmi_cut_edit(gcmi_cut *cut, int n, const int32_t *h_index,
                   const int32_t *h_source, const int32_t *h_sink);
+/* Warm-start edit by a brush stroke, rasterised on the device (no reference
+ * counterpart; the reference's brush paints the excess image on the host,
+ * test_GraphCut.py:74-75).  h_xy holds npts (x, y) points (2*npts doubles,
+ * npts <= 16); every pixel whose centre (x, y) lies within width/2 of the
+ * polyline -- a union of capsules, one per segment, each test rounded in
+ * float64 in the order documented in the library -- gets source/sink
+ * capacities (source, sink).  Fewer than 2 points paint nothing. */
+int gcmi_cut_paint(gcmi_cut *cut, int npts, const double *h_xy, double width,
+                   int32_t source, int32_t sink);
 /* Labels of the last cut: h_out[y*width+x] = 1 for the minimal source
  * (foreground) set -- pixels reachable from the source in the final residual
  * graph -- else 0.  (The reference exposes only dHeight, GraphCut.pyx:62.) */
@@ -205,6 +214,9 @@ int gcmi_batch_upload_device(gcmi_batch *batch, int i,
 /* Warm-start edit of image i (as gcmi_cut_edit). */
 int gcmi_batch_edit(gcmi_batch *batch, int i, int n, const int32_t *h_index,
                     const int32_t *h_source, const int32_t *h_sink);
+/* Brush stroke on image i (as gcmi_cut_paint). */
+int gcmi_batch_paint(gcmi_batch *batch, int i, int npts, const double *h_xy,
+                     double width, int32_t source, int32_t sink);
 /* Solves all images; flows_out (count int64, may be NULL) gets each flow. */
 int gcmi_batch_run(gcmi_batch *batch, int sweeps, int bfs, int64_t max_rounds,
                    int64_t *flows_out);
@@ -229,9 +241,10 @@ int gcmi_batch_kernel_times(gcmi_batch *batch, double *ms_out,
  * those sessions in one pool run; requests that arrive meanwhile go into
  * the next run.  Strokes reflow warm from the session's residual graph
  * (as gcmi_cut_edit).  The handle is thread-safe: any thread may submit,
- * wait and read masks.
- *   mode 0: batched warm serving; mode 1: plain -- one request at a time,
- *   the session's unaries re-uploaded and cut from scratch. */
+ * wait and read masks.  Modes: */
+#define GCMI_SERVE_WARM 0  /* batched; strokes reflow warm                */
+#define GCMI_SERVE_PLAIN 1 /* one request at a time, cut from scratch     */
+#define GCMI_SERVE_COLD 2  /* batched; every session cut from scratch     */
 typedef struct gcmi_server gcmi_server;
 
 /* hbm_budget: device bytes the session pool may use (GCMI_EBUDGET if too
@@ -261,9 +274,39 @@ int gcmi_server_submit_image(gcmi_server *server, int session,
 int gcmi_server_submit_stroke(gcmi_server *server, int session, int n,
                               const int32_t *h_index, const int32_t *h_source,
                               const int32_t *h_sink, int64_t *req_out);
-/* Waits up to timeout_ms (< 0: no limit) for a request.  info_out (6 int64,
+/* Queues a brush stroke given as a polyline (as gcmi_cut_paint). */
+int gcmi_server_submit_polyline(gcmi_server *server, int session, int npts,
+                                const double *h_xy, double width,
+                                int32_t source, int32_t sink,
+                                int64_t *req_out);
+
+/* One request of a replayed trace. */
+#define GCMI_REQ_IMAGE 0    /* p0..p3: source, sink, right, down          */
+#define GCMI_REQ_STROKE 1   /* n pixels; p0..p2: index, source, sink      */
+#define GCMI_REQ_POLYLINE 2 /* n points; p0: xy; width, source, sink     */
+typedef struct gcmi_request {
+  int64_t t_ns; /* submit at this steady-clock (CLOCK_MONOTONIC) time, in
+                   ns; it is the request's arrival time.  <= 0: at once */
+  int32_t session;
+  int32_t kind;
+  int32_t n;
+  int32_t source, sink;
+  int32_t reserved;
+  double width;
+  const void *p0, *p1, *p2, *p3;
+} gcmi_request;
+/* Open-loop load generator: submits reqs[0..count) in order from the
+ * calling thread, each at its time (sleeping until then; a late submission
+ * keeps the scheduled arrival time, so the lag counts as latency).
+ * req_out (count int64, may be NULL) gets the request ids.  Stops at the
+ * first refused request and returns its status. */
+int gcmi_server_replay(gcmi_server *server, int count, const gcmi_request *reqs,
+                       int64_t *req_out);
+
+/* Waits up to timeout_ms (< 0: no limit) for a request.  info_out (7 int64,
  * may be NULL): status (0 queued, 1 done, 2 failed), flow, arrival, start
- * and completion times (ns, steady clock), sessions solved with it.
+ * and completion times (ns, steady clock), sessions solved with it, and the
+ * time it entered the queue (after a new image's staging copy).
  * GCMI_EBUSY if still queued at the timeout. */
 int gcmi_server_wait(gcmi_server *server, int64_t req, int timeout_ms,
                      int64_t *info_out);
@@ -273,7 +316,8 @@ int gcmi_server_wait(gcmi_server *server, int64_t req, int timeout_ms,
 int gcmi_server_mask(gcmi_server *server, int session, uint32_t *h_bits,
                      int64_t *req_out);
 /* [0] solves  [1] requests served  [2] queued  [3] open sessions
- * [4] pool tiles in use  [5] pool tile capacity  [6] device bytes */
+ * [4] pool tiles in use  [5] pool tile capacity  [6] device bytes
+ * [7] [8] [9] worker ns applying requests, solving, reading masks out */
 int gcmi_server_stats(gcmi_server *server, int64_t *out, int n);
 /* Drops the record of a completed request. */
 int gcmi_server_forget(gcmi_server *server, int64_t req);

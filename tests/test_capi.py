@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 import gcmi_lib as L
@@ -51,7 +52,7 @@ def test_server_argument_checks():
     h = ctypes.c_void_p()
     assert L.lib().gcmi_server_create(48, 1 << 30, 4, 4, 0, ctypes.byref(h)) == L.EINVAL
     assert L.lib().gcmi_server_create(32, 1 << 30, 0, 4, 0, ctypes.byref(h)) == L.EINVAL
-    assert L.lib().gcmi_server_create(32, 1 << 30, 4, 4, 2, ctypes.byref(h)) == L.EINVAL
+    assert L.lib().gcmi_server_create(32, 1 << 30, 4, 4, 3, ctypes.byref(h)) == L.EINVAL
     assert L.lib().gcmi_server_wait(None, 1, 0, None) == L.EINVAL
     assert L.lib().gcmi_server_destroy(None) == L.OK
 
@@ -62,3 +63,30 @@ def test_server_without_device_fails_loudly():
     with pytest.raises(L.GcmiError) as e:
         GraphCutServer(hbm_budget=1 << 30, max_sessions=4)
     assert e.value.code == L.ENODEVICE
+
+
+def test_request_struct_layout():
+    """gcmi_request: int64, six int32, a double, four pointers (72 bytes)."""
+    assert ctypes.sizeof(L.Request) == 72
+    assert L.Request.width.offset == 32
+    assert L.Request.p0.offset == 40
+
+
+def test_stroke_polyline_rasteriser_cases():
+    """The host rasteriser (the oracle of the device brush) on cases fixed
+    by geometry alone."""
+    import gcmi_workloads as G
+    # horizontal segment on y = 3.5, width 1: rows 3 and 4 are exactly r away
+    idx = G.rasterise_polyline(10, 8, [(2.0, 3.5), (6.0, 3.5)], 1)
+    ys, xs = np.divmod(idx, 10)
+    assert set(ys) == {3, 4} and set(xs) == set(range(2, 7))
+    # width 0 on the diagonal: exactly the centres on it
+    idx = G.rasterise_polyline(6, 6, [(0.0, 0.0), (5.0, 5.0)], 0)
+    assert list(idx) == [0, 7, 14, 21, 28, 35]
+    # one point paints nothing; a disc of radius 2 around a repeated point
+    assert G.rasterise_polyline(9, 9, [(4.0, 4.0)], 5).size == 0
+    idx = G.rasterise_polyline(9, 9, [(4.0, 4.0), (4.0, 4.0)], 4)
+    ys, xs = np.divmod(idx, 9)
+    assert ((xs - 4) ** 2 + (ys - 4) ** 2 <= 4).all() and idx.size == 13
+    # outside the image
+    assert G.rasterise_polyline(9, 9, [(-30.0, -30.0), (-10.0, -20.0)], 8).size == 0
