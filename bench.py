@@ -1,13 +1,25 @@
 """Benchmark of the MI355X tiled push-relabel grid cut.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--tile 32]
+                    [--no-batch] [--no-warm] [--no-serve] [--no-cpu] ...
 
 One step = one plain cut of the 4096x4096 workload (three foreground
 ellipses, noise sigma 30, seed 1): init from the row-major int32 capacities
 already resident in HBM, push-relabel to termination, minimal source set,
-labels written to HBM.  Prints one JSON line; `value` is Mpix/s of the plain
-cut.  The CPU baseline times the oracle's restatement of the reference's
-push-relabel on the 512x512 workload, on one host core.
+labels written to HBM.  `value` is Mpix/s of that plain cut.  Beside it, in
+the same JSON line:
+
+  batched   64 ragged images (W, H ~ U[256, 2048]) solved together, tile /
+            sweeps / push rounds per relabel picked by the autotuner, against
+            64 sequential plain cuts;
+  warm      a 2048x2048 image and 100 seeded brush strokes: stroke-to-mask
+            latency (device brush, reflow, packed mask in host memory) warm
+            vs from scratch;
+  served    256 resident sessions (512..4096 squared, weights 4:3:2:1),
+            Poisson arrivals, 10% new images / 90% strokes: stroke latency
+            p50/p99 and sustained stroke rate per serving mode;
+  cpu_baseline  the oracle's restatement of the reference push-relabel on
+            the 512x512 workload, on one host core.
 """
 import argparse
 import json
@@ -35,6 +47,10 @@ HBM_PEAK_GBS = 8000.0
 ALG_BYTES_PER_PX = {"push": 49, "bfs_init": 5, "bfs_round": 9, "labels": 2}
 
 
+def log(msg):
+    print("# " + msg, file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -47,8 +63,15 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-parity", action="store_true", help="skip the oracle check")
     ap.add_argument("--no-sweep", action="store_true", help="skip the tile sweep")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--no-warm", action="store_true", help="skip the warm-start leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
+
+
+# ------------------------------------------------------------------ plain
 
 
 def time_cut(gc, steps, warmup, sweeps, bfs, profile=False):
@@ -56,7 +79,6 @@ def time_cut(gc, steps, warmup, sweeps, bfs, profile=False):
         gc.cut(sweeps=sweeps, bfs=bfs)
     gc.profile(profile)
     totals = {}
-    stats = None
     t0 = time.perf_counter()
     for _ in range(steps):
         gc.cut(sweeps=sweeps, bfs=bfs)
@@ -73,10 +95,9 @@ def time_cut(gc, steps, warmup, sweeps, bfs, profile=False):
 
 def roofline(totals, stats, tile, W, H, steps):
     """Dominant kernel class by total time; algorithmic bytes per launch over
-    its mean launch duration."""
+    its mean launch duration (HIP events on the cut's stream)."""
     T2 = tile * tile
     ntiles = stats["tiles"]
-    # algorithmic bytes each class moves per run
     per_run = {
         "push_first": 16 * W * H + 25 * ntiles * T2,
         "push": ALG_BYTES_PER_PX["push"] * (stats["residencies"] - ntiles) * T2,
@@ -96,6 +117,171 @@ def roofline(totals, stats, tile, W, H, steps):
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "alg_bytes_per_launch": int(per_launch_bytes),
             "avg_launch_us": round(avg_s * 1e6, 2)}
+
+
+# ---------------------------------------------------------------- batched
+
+
+def batched_leg(args):
+    """64 ragged images: autotuned batched run vs 64 sequential plain cuts."""
+    import gcmi_workloads as G
+    from GraphCut import GraphCut
+    from GraphCutBatch import GraphCutBatch
+    from gcmi_lib import DeviceBuffer
+    log("batched: generating 64 ragged images")
+    works = G.ragged_batch(64, 256, 2048, seed=2)
+    dims = [(w.shape[1], w.shape[0]) for w in works]
+    pixels = sum(w.pixels for w in works)
+    dev = [[DeviceBuffer(a.nbytes).upload(a) for a in w.arrays()] for w in works]
+
+    def make_batch(tile):
+        b = GraphCutBatch(dims, tile=tile)
+        for i, d in enumerate(dev):
+            b.upload_device(i, *d)
+        return b
+
+    def time_batch(b, sweeps, bfs, reps):
+        # an image not edited since its last solve is cut from scratch again
+        b.cut(sweeps=sweeps, bfs=bfs)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            b.cut(sweeps=sweeps, bfs=bfs)
+        return (time.perf_counter() - t0) / reps
+
+    # autotuner: tile x sweeps x push rounds per relabel, one timed run each
+    log("batched: autotuning")
+    tune = []
+    batches = {}
+    for tile in (16, 32, 64):
+        batches[tile] = make_batch(tile)
+        for sweeps in (2, 4, 8):
+            for bfs in (1, 2):
+                dt = time_batch(batches[tile], sweeps, bfs, 2)
+                tune.append({"tile": tile, "sweeps": sweeps, "bfs": bfs,
+                             "ms": round(dt * 1e3, 3)})
+    best = min(tune, key=lambda r: r["ms"])
+    b = batches[best["tile"]]
+    reps = max(3, args.steps // 2)
+    t_batch = time_batch(b, best["sweeps"], best["bfs"], reps)
+    flows = b.cut(sweeps=best["sweeps"], bfs=best["bfs"])
+    # sequential plain cuts with the same settings
+    log("batched: 64 sequential plain cuts")
+    cuts = []
+    for i, d in enumerate(dev):
+        g = GraphCut(None, None, dims[i], tile=best["tile"])
+        g.upload_device(*d)
+        cuts.append(g)
+    for g in cuts:
+        g.cut(sweeps=best["sweeps"], bfs=best["bfs"])
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for g in cuts:
+            g.cut(sweeps=best["sweeps"], bfs=best["bfs"])
+    t_seq = (time.perf_counter() - t0) / reps
+    same = all(g.flow == f for g, f in zip(cuts, flows))
+    for g in cuts:
+        g.close()
+    for x in batches.values():
+        x.close()
+    return {"workload": "64 images, W,H ~ U[256,2048] (seed 2), centred ellipse, sigma 20",
+            "pixels": pixels, "autotune": best, "autotune_trials": len(tune),
+            "batched_ms": round(t_batch * 1e3, 3),
+            "batched_mpix_s": round(pixels / t_batch / 1e6, 1),
+            "sequential_ms": round(t_seq * 1e3, 3),
+            "sequential_mpix_s": round(pixels / t_seq / 1e6, 1),
+            "speedup": round(t_seq / t_batch, 2), "flows_equal_sequential": bool(same)}
+
+
+# ------------------------------------------------------------------- warm
+
+
+def warm_leg(args):
+    """2048x2048, 100 strokes: warm reflow vs from scratch, per stroke
+    paint + cut + packed mask readback."""
+    import gcmi_workloads as G
+    from GraphCut import GraphCut
+    W = H = 2048
+    log("warm: generating the 2048x2048 image and 100 strokes")
+    work = G.centred(W, H, seed=3)
+    strokes = G.strokes(W, H, count=100, seed=3, raster=False)
+    res = {}
+    masks = {}
+    flows = {}
+    for mode in ("warm", "scratch"):
+        gc = GraphCut(None, None, (W, H), tile=args.tile)
+        gc.upload(*work.arrays())
+        gc.cut()
+        gc.labels_packed()
+        lat = []
+        fl = []
+        for st in strokes:
+            s, t = st.capacities()
+            t0 = time.perf_counter()
+            gc.paint(st.points, st.width, s, t)
+            if mode == "scratch":
+                gc.forget()
+            fl.append(gc.cut())
+            bits = gc.labels_packed()
+            lat.append((time.perf_counter() - t0) * 1e3)
+        masks[mode] = bits
+        flows[mode] = fl
+        res[mode] = {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+                     "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                     "mean_ms": round(float(np.mean(lat)), 3)}
+        gc.close()
+    out = {"workload": "2048x2048 centred ellipse (seed 3), 100 strokes (seed 3), "
+                       "device brush, packed mask to host", "tile": args.tile,
+           "warm": res["warm"], "scratch": res["scratch"],
+           "speedup_mean": round(res["scratch"]["mean_ms"] / res["warm"]["mean_ms"], 2),
+           "flows_equal": flows["warm"] == flows["scratch"],
+           "final_mask_equal": bool((masks["warm"] == masks["scratch"]).all())}
+    if not args.no_parity:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import oracle as O
+        cur = work.copy()
+        for st in G.strokes(W, H, count=100, seed=3):
+            G.apply_stroke(cur, st)
+        fo, mo = O.bk_cut(*cur.arrays())
+        packed = np.packbits(mo.reshape(-1), bitorder="little")
+        packed = np.pad(packed, (0, (-packed.size) % 4)).view(np.uint32)
+        out["parity_last_stroke"] = {"flow_equal": fo == flows["warm"][-1],
+                                     "mask_equal": bool((packed == masks["warm"]).all())}
+    return out
+
+
+# ---------------------------------------------------------------- serving
+
+
+def serve_leg(args):
+    import gcmi_serving as S
+    from GraphCutServer import GraphCutServer
+    log("served: generating the 256-session mix")
+    mix = S.Mix(nsess=256, seed=4)
+    rates = {"plain": (500, 1000, 1500), "cold": (1000, 2000, 4000),
+             "batched": (1000, 2000, 4000)}
+    out = {"workload": "256 sessions, sizes {512,1024,2048,4096}^2 weights 4:3:2:1 (seed 4), "
+                       "Poisson arrivals, 10%% new images / 90%% polyline strokes, "
+                       "%.1f s per rate" % args.serve_seconds,
+           "resident_pixels": mix.pixels, "hbm_budget_bytes": 200 * 10**9, "modes": {}}
+    for mode, rs in rates.items():
+        log("served: mode %s" % mode)
+        with GraphCutServer(tile=32, hbm_budget=200 * 10**9, max_sessions=256, max_batch=64,
+                            mode=mode) as srv:
+            sids = S.open_sessions(srv, mix)
+            pts = [S.run_trace(srv, sids, mix, r, duration=args.serve_seconds, seed=11 + k)
+                   for k, r in enumerate(rs)]
+            out["modes"][mode] = {"points": pts, "sustained_2048_p99_20ms": S.sustained(pts),
+                                  "device_bytes": srv.stats()["device_bytes"]}
+    sus = {m: (v["sustained_2048_p99_20ms"] or {}).get("stroke_rate")
+           for m, v in out["modes"].items()}
+    out["sustained_stroke_rate"] = sus
+    if sus.get("plain"):
+        best = max((v for m, v in sus.items() if m != "plain" and v), default=None)
+        out["served_vs_plain"] = round(best / sus["plain"], 2) if best else None
+    return out
+
+
+# ------------------------------------------------------------------- CPU
 
 
 def cpu_baseline(seconds):
@@ -130,14 +316,14 @@ def main():
     from gcmi_lib import DeviceBuffer
 
     W = H = args.size
-    print("# generating %dx%d workload" % (W, H), file=sys.stderr, flush=True)
+    log("generating %dx%d workload" % (W, H))
     work = G.three_ellipses(W, H, seed=1)
     dev = [DeviceBuffer(a.nbytes).upload(a) for a in work.arrays()]
     gc = GraphCut(None, None, (W, H), tile=args.tile)
     gc.upload_device(*dev)
     sweeps = args.sweeps or None
     bfs = args.bfs or None
-    print("# timing", file=sys.stderr, flush=True)
+    log("timing the plain cut")
     t_step, totals, stats = time_cut(gc, args.steps, args.warmup, sweeps, bfs, profile=True)
     t_plain, _, _ = time_cut(gc, args.steps, 1, sweeps, bfs, profile=False)
     value = W * H / t_plain / 1e6
@@ -172,7 +358,16 @@ def main():
             sweep[str(t)] = round(W * H / ts / 1e6, 1)
             g2.close()
         out["tile_sweep_mpix_s"] = sweep
+    gc.close()
+    del dev
+    if not args.no_batch:
+        out["batched"] = batched_leg(args)
+    if not args.no_warm:
+        out["warm"] = warm_leg(args)
+    if not args.no_serve:
+        out["served"] = serve_leg(args)
     if not args.no_cpu:
+        log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)
     return 0

@@ -153,6 +153,17 @@ class GraphCut:
         check(lib().gcmi_cut_labels(self._h, host_ptr(out)))
         return out
 
+    def labels_packed(self):
+        """uint32 words, bit p % 32 of word p // 32 = label of pixel p."""
+        out = np.empty((self.shape[0] * self.shape[1] + 31) // 32, np.uint32)
+        check(lib().gcmi_cut_labels_packed(self._h, host_ptr(out)))
+        return out
+
+    def forget(self):
+        """Drop the solved state: the next cut() starts from the stored
+        capacities (every edit included) instead of reflowing warm."""
+        check(lib().gcmi_cut_forget(self._h))
+
     def labels_device(self, d_out):
         check(lib().gcmi_cut_labels_device(self._h, ptr(d_out)))
 

@@ -112,6 +112,22 @@ int gcmi_cut_paint(gcmi_cut* c, int npts, const double* h_xy, double width, int3
   return pool_paint(&c->pool, c->img, npts, h_xy, width, source, sink);
 }
 
+int gcmi_cut_labels_packed(gcmi_cut* c, uint32_t* h_bits) {
+  if (!c || !h_bits) {
+    set_error("null argument");
+    return GCMI_EINVAL;
+  }
+  return pool_labels_packed(&c->pool, c->img, h_bits);
+}
+
+int gcmi_cut_forget(gcmi_cut* c) {
+  if (!c) {
+    set_error("null handle");
+    return GCMI_EINVAL;
+  }
+  return pool_forget(&c->pool, c->img);
+}
+
 int gcmi_cut_run(gcmi_cut* c, int sweeps, int bfs, int64_t max_rounds, int64_t* flow_out) {
   if (!c) {
     set_error("null handle");

@@ -151,6 +151,8 @@ struct Pool {
   long long* h_flows = nullptr;
   unsigned long long* h_tally = nullptr;
   int32_t* h_edit = nullptr;
+  uint32_t *d_bits = nullptr, *h_bits = nullptr;  // packed-label staging
+  long long bits_cap = 0;                           // words
   // host bookkeeping
   std::vector<PoolImage> imgs;
   RangeAlloc tiles_alloc, pix_alloc;
@@ -200,6 +202,12 @@ int pool_paint(Pool* p, int img, int npts, const double* xy, double width, int32
 int pool_solve(Pool* p, const int* img_ids, int n, int sweeps, int bfs, int64_t max_rounds,
                int64_t* flows_out);
 int pool_labels(Pool* p, int img, uint8_t* out, bool device_dst);
+// Labels packed to bits (bit p % 32 of word p / 32 = label of pixel p).
+int pool_labels_packed(Pool* p, int img, uint32_t* out);
+// Launches the packing of an image's labels into d_out (ceil(W*H/32) words).
+int pack_labels_async(Pool* p, int img, uint32_t* d_out);
+// Drops the solved state: the next solve starts from the capacities.
+int pool_forget(Pool* p, int img);
 int pool_heights(Pool* p, int img, int32_t* out);
 int validate_caps(const int32_t* cs, const int32_t* ct, const int32_t* right,
                   const int32_t* down, size_t n, int32_t* max_pair);

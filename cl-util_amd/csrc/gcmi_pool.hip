@@ -408,7 +408,8 @@ void pool_free(Pool* p) {
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   for (void* x : ptrs)
     if (x) (void)hipFree(x);
-  void* hp[] = {p->h_counts, p->h_prep, p->h_flows, p->h_tally, p->h_edit};
+  if (p->d_bits) (void)hipFree(p->d_bits);
+  void* hp[] = {p->h_counts, p->h_prep, p->h_flows, p->h_tally, p->h_edit, p->h_bits};
   for (void* x : hp)
     if (x) (void)hipHostFree(x);
   p->prof.release();
@@ -987,6 +988,68 @@ int pool_labels(Pool* p, int img, uint8_t* out, bool device_dst) {
                           device_dst ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                           p->stream));
   GCMI_HIP(hipStreamSynchronize(p->stream));
+  return GCMI_OK;
+}
+
+// One wave packs 64 labels into two words with a ballot; loads coalesce.
+__global__ void k_pack_bits(const uint8_t* __restrict__ labels, long long n, uint32_t* out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const bool on = i < n && labels[i] != 0;
+  const unsigned long long b = __ballot(on);
+  const long long w = (i >> 6) * 2;
+  const long long words = (n + 31) >> 5;
+  if ((threadIdx.x & 63) == 0 && w < words) {
+    out[w] = (uint32_t)b;
+    if (w + 1 < words) out[w + 1] = (uint32_t)(b >> 32);
+  }
+}
+
+int pack_labels_async(Pool* p, int img, uint32_t* d_out) {
+  const PoolImage& im = p->imgs[img];
+  const long long n = (long long)im.W * im.H;
+  hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream,
+                     p->dev.labels + im.pix0, n, d_out);
+  GCMI_HIP(hipGetLastError());
+  return GCMI_OK;
+}
+
+int pool_labels_packed(Pool* p, int img, uint32_t* out) {
+  GCMI_TRY(check_img(p, img));
+  const PoolImage& im = p->imgs[img];
+  if (!im.solved) {
+    set_error("no completed cut for image %d", img);
+    return GCMI_EINVAL;
+  }
+  const long long words = ((long long)im.W * im.H + 31) / 32;
+  if (p->bits_cap < words) {
+    GCMI_HIP(hipStreamSynchronize(p->stream));
+    if (p->d_bits) (void)hipFree(p->d_bits);
+    if (p->h_bits) (void)hipHostFree(p->h_bits);
+    p->d_bits = nullptr;
+    p->h_bits = nullptr;
+    p->bits_cap = 0;
+    if (hipMalloc((void**)&p->d_bits, (size_t)words * 4) != hipSuccess ||
+        hipHostMalloc((void**)&p->h_bits, (size_t)words * 4, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("packed-label staging allocation failed");
+      return GCMI_ENOMEM;
+    }
+    p->bits_cap = words;
+  }
+  GCMI_TRY(pack_labels_async(p, img, p->d_bits));
+  GCMI_HIP(hipMemcpyAsync(p->h_bits, p->d_bits, (size_t)words * 4, hipMemcpyDeviceToHost,
+                          p->stream));
+  GCMI_HIP(hipStreamSynchronize(p->stream));
+  memcpy(out, p->h_bits, (size_t)words * 4);
+  return GCMI_OK;
+}
+
+int pool_forget(Pool* p, int img) {
+  GCMI_TRY(check_img(p, img));
+  PoolImage& im = p->imgs[img];
+  im.fresh = true;
+  im.edited = false;
+  im.solved = false;
   return GCMI_OK;
 }
 

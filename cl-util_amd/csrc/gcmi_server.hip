@@ -51,19 +51,6 @@ int64_t now_ns() {
       .count();
 }
 
-// One wave packs 64 labels into two words with a ballot; loads coalesce.
-__global__ void k_pack_bits(const uint8_t* __restrict__ labels, long long n, uint32_t* out) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  const bool on = i < n && labels[i] != 0;
-  const unsigned long long b = __ballot(on);
-  const long long w = (i >> 6) * 2;
-  const long long words = (n + 31) >> 5;
-  if ((threadIdx.x & 63) == 0 && w < words) {
-    out[w] = (uint32_t)b;
-    if (w + 1 < words) out[w + 1] = (uint32_t)(b >> 32);
-  }
-}
-
 enum { REQ_IMAGE = GCMI_REQ_IMAGE, REQ_STROKE = GCMI_REQ_STROKE, REQ_POLYLINE = GCMI_REQ_POLYLINE };
 enum { ST_QUEUED = 0, ST_DONE = 1, ST_FAILED = 2 };
 
@@ -167,10 +154,7 @@ int serve_batch(gcmi_server* s, std::vector<Request*>& batch) {
     const Session& ss = s->sessions[r->session];
     const long long npx = (long long)ss.W * ss.H;
     const long long words = (npx + 31) / 32;
-    const PoolImage& im = p->imgs[ss.img];
-    hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0,
-                       p->stream, p->dev.labels + im.pix0, npx, s->d_bits + off);
-    GCMI_HIP(hipGetLastError());
+    GCMI_TRY(pack_labels_async(p, ss.img, s->d_bits + off));
     GCMI_HIP(hipMemcpyAsync(ss.bits, s->d_bits + off, words * 4, hipMemcpyDeviceToHost,
                             p->stream));
     off += words;

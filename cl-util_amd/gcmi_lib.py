@@ -85,6 +85,8 @@ SIGNATURES = {
     "gcmi_cut_kernel_times": [_vp, ctypes.POINTER(ctypes.c_double), _i64p, _int],
     "gcmi_cut_edit": [_vp, _int, _vp, _vp, _vp],
     "gcmi_cut_paint": [_vp, _int, _vp, ctypes.c_double, _i32, _i32],
+    "gcmi_cut_labels_packed": [_vp, _vp],
+    "gcmi_cut_forget": [_vp],
     "gcmi_batch_paint": [_vp, _int, _int, _vp, ctypes.c_double, _i32, _i32],
     "gcmi_server_submit_polyline": [_vp, _int, _int, _vp, ctypes.c_double, _i32, _i32, _i64p],
     "gcmi_server_replay": [_vp, _int, _vp, _i64p],

@@ -1,0 +1,144 @@
+"""Serving workload: resident sessions, Poisson request traces, latency
+statistics (host side).
+
+The mix: `nsess` sessions with square sizes drawn from SIZES with weights
+4:3:2:1 (seeded).  Session i starts from base image i mod K of its size
+(three_ellipses with a per-size seed; K images per size), so that the
+4096x4096 images are not generated 26 times.  Requests arrive as a Poisson
+process; each picks a session uniformly; 10% are new images (the session's
+next base image, cut from scratch) and 90% are brush strokes: polylines
+from a seeded bank per size (3-8 segments, width 8-32, alternating
+foreground/background hard constraints), rasterised on the device.
+
+Latency of a request = completion (mask in host memory) - scheduled
+arrival, so any lag of the load generator counts against the server.
+"""
+import threading
+import time
+
+import numpy as np
+
+import gcmi_workloads as G
+
+SIZES = (512, 1024, 2048, 4096)
+WEIGHTS = (4, 3, 2, 1)
+BASES = {512: 4, 1024: 3, 2048: 2, 4096: 2}
+
+
+class Mix:
+    def __init__(self, nsess=256, seed=4, bank=32, sizes=SIZES, weights=WEIGHTS, bases=BASES):
+        rng = np.random.default_rng(seed)
+        w = np.asarray(weights, float)
+        self.sizes = [int(x) for x in rng.choice(sizes, size=nsess, p=w / w.sum())]
+        self.bases = {n: [G.three_ellipses(n, n, seed=4000 + 10 * n + k) for k in range(bases[n])]
+                      for n in sizes if n in self.sizes}
+        self.bank = {n: G.strokes(n, n, count=bank, seed=5000 + n, raster=False)
+                     for n in self.bases}
+        self.nsess = nsess
+
+    @property
+    def pixels(self):
+        return sum(n * n for n in self.sizes)
+
+    def base(self, i, k=0):
+        n = self.sizes[i]
+        b = self.bases[n]
+        return b[(i + k) % len(b)]
+
+
+def open_sessions(srv, mix):
+    """Opens every session and cuts its first image; returns the ids."""
+    sids = [srv.open(n, n) for n in mix.sizes]
+    trace = [(0, sids[i], "image", mix.base(i).arrays()) for i in range(mix.nsess)]
+    for r in srv.replay(trace):
+        srv.wait(int(r))
+        srv.forget(int(r))
+    return sids
+
+
+def make_trace(mix, rate, duration, seed, t0_ns, image_frac=0.1):
+    """(strokes, images): lists of (t_ns, session index, kind, payload, size)."""
+    rng = np.random.default_rng(seed)
+    n = max(1, int(rate * duration))
+    ts = t0_ns + np.cumsum(rng.exponential(1e9 / rate, size=n)).astype(np.int64)
+    strokes, images = [], []
+    nimg = np.zeros(mix.nsess, int)
+    for k in range(n):
+        i = int(rng.integers(mix.nsess))
+        size = mix.sizes[i]
+        if rng.random() < image_frac:
+            nimg[i] += 1
+            images.append((int(ts[k]), i, "image", mix.base(i, int(nimg[i])).arrays(), size))
+        else:
+            st = mix.bank[size][int(rng.integers(len(mix.bank[size])))]
+            s, t = st.capacities()
+            strokes.append((int(ts[k]), i, "polyline", (st.points, st.width, s, t), size))
+    return strokes, images
+
+
+def run_trace(srv, sids, mix, rate, duration=2.0, seed=11, image_frac=0.1, lead=0.2):
+    """Replays a Poisson trace (strokes and new images from two generator
+    threads, so an image's host-side staging does not delay the strokes
+    behind it) and returns latency statistics in ms."""
+    t0 = time.monotonic_ns() + int(lead * 1e9)
+    strokes, images = make_trace(mix, rate, duration, seed, t0, image_frac)
+    ids = {}
+    errors = []
+
+    def go(name, tr):
+        try:
+            ids[name] = srv.replay([(t, sids[i], k, p) for (t, i, k, p, _) in tr]) \
+                if tr else np.zeros(0, np.int64)
+        except Exception as e:  # reported by the caller
+            errors.append(e)
+
+    th = [threading.Thread(target=go, args=("s", strokes)),
+          threading.Thread(target=go, args=("i", images))]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    if errors:
+        raise errors[0]
+    lat = {"stroke": [], "image": []}
+    by_size = {}
+    batch = []
+    t_end = {"s": t0, "i": t0}
+    for name, tr in (("s", strokes), ("i", images)):
+        for rid, item in zip(ids[name], tr):
+            info = srv.wait(int(rid))
+            srv.forget(int(rid))
+            ms = (info.t_done - item[0]) / 1e6
+            lat["stroke" if name == "s" else "image"].append(ms)
+            if name == "s":
+                by_size.setdefault(item[4], []).append(ms)
+            batch.append(info.batch)
+            t_end[name] = max(t_end[name], info.t_done)
+
+    def pct(v, q):
+        return round(float(np.percentile(v, q)), 3) if v else None
+
+    out = {"rate": rate, "requests": len(strokes) + len(images),
+           "strokes": len(strokes), "images": len(images),
+           "stroke_rate": round(rate * len(strokes) / max(1, len(strokes) + len(images)), 1),
+           "stroke_throughput": round(len(strokes) / max((t_end["s"] - t0) / 1e9, 1e-9), 1),
+           "stroke_p50_ms": pct(lat["stroke"], 50), "stroke_p99_ms": pct(lat["stroke"], 99),
+           "image_p50_ms": pct(lat["image"], 50), "image_p99_ms": pct(lat["image"], 99),
+           "mean_batch": round(float(np.mean(batch)), 2)}
+    for size in sorted(by_size):
+        out["stroke_p50_ms_%d" % size] = pct(by_size[size], 50)
+        out["stroke_p99_ms_%d" % size] = pct(by_size[size], 99)
+    return out
+
+
+def sustained(points, p99_ms=20.0, size=2048):
+    """Highest offered rate whose strokes on `size` sessions kept p99 within
+    p99_ms and whose stroke throughput kept up with the offer (within 10%)."""
+    best = None
+    for p in points:
+        p99 = p.get("stroke_p99_ms_%d" % size)
+        if p99 is not None and p99 <= p99_ms and \
+                p["stroke_throughput"] >= 0.9 * p["stroke_rate"]:
+            if best is None or p["rate"] > best["rate"]:
+                best = p
+    return best

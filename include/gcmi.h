@@ -172,6 +172,13 @@ int gcmi_cut_paint(gcmi_cut *cut, int npts, const double *h_xy, double width,
 int gcmi_cut_labels(gcmi_cut *cut, uint8_t *h_out);
 /* Same into device memory (width*height bytes). */
 int gcmi_cut_labels_device(gcmi_cut *cut, uint8_t *d_out);
+/* Same packed to bits: bit p%32 of h_bits[p/32] is the label of pixel
+ * p = y*width+x (ceil(width*height/32) words; no reference counterpart). */
+int gcmi_cut_labels_packed(gcmi_cut *cut, uint32_t *h_bits);
+/* Drops the solved state, so the next gcmi_cut_run starts from the stored
+ * capacities (every edit included) instead of reflowing warm.  (The
+ * reference's GraphCut.reset, GraphCut.pyx:420-421, does nothing.) */
+int gcmi_cut_forget(gcmi_cut *cut);
 /* Counters of the last run:
  *   [0] push rounds  [1] tile residencies  [2] global relabels
  *   [3] BFS tile passes  [4] flood tile passes  [5] host syncs

@@ -134,3 +134,28 @@ def test_server_replay_polylines(mode):
             mask, rid = srv.mask(sid)
             assert rid == ids[-len(dims) + i]
             assert (mask == O.bk_cut(*cur[i].arrays())[1]).all(), (mode, i)
+
+
+def test_packed_labels_and_forget():
+    """labels_packed() is labels() packed little-endian into uint32 words;
+    forget() makes the next cut start from the capacities (same answer)."""
+    W, H = 333, 97  # W*H not a multiple of 32
+    work = G.centred(W, H, seed=44)
+    gc = GraphCut(None, None, (W, H))
+    gc.upload(*work.arrays())
+    f0 = gc.cut()
+    bits = gc.labels_packed()
+    lab = gc.labels()
+    ref = np.packbits(lab.reshape(-1), bitorder="little")
+    ref = np.pad(ref, (0, (-ref.size) % 4)).view(np.uint32)
+    assert bits.shape == ((W * H + 31) // 32,)
+    assert (bits == ref).all()
+    st = G.strokes(W, H, count=1, seed=45)[0]
+    s, t = st.capacities()
+    gc.paint(st.points, st.width, s, t)
+    gc.forget()
+    f1 = gc.cut()
+    G.apply_stroke(work, st)
+    fo, mo = O.bk_cut(*work.arrays())
+    assert f1 == fo and f0 != f1
+    assert (gc.labels() == mo).all()
