@@ -112,11 +112,29 @@ def roofline(totals, stats, tile, W, H, steps):
     per_launch_bytes = per_run[dom] * steps / launches
     avg_s = ms / launches / 1e3
     achieved = per_launch_bytes / avg_s / 1e9
+    traffic, source = pmc_traffic(dom, W, tile) if W == H else (None, None)
     return {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": source,
             "alg_bytes_per_launch": int(per_launch_bytes),
             "avg_launch_us": round(avg_s * 1e6, 2)}
+
+
+def pmc_traffic(kernel, size, tile):
+    """HBM bytes per launch of `kernel` on the size x size workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each counter scaled
+    by what it reads for a copy of known size), or None if not profiled."""
+    path = os.path.join(HERE, "profiles", "plain_cut_%d_t%d_pmc_traffic.md" % (size, tile))
+    key = "traffic_bytes_per_launch %s %d %d =" % (kernel, size, tile)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(key):
+                    return int(line.split("=")[1]), os.path.relpath(path, HERE)
+    except OSError:
+        pass
+    return None, None
 
 
 # ---------------------------------------------------------------- batched
