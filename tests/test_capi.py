@@ -90,3 +90,17 @@ def test_stroke_polyline_rasteriser_cases():
     assert ((xs - 4) ** 2 + (ys - 4) ** 2 <= 4).all() and idx.size == 13
     # outside the image
     assert G.rasterise_polyline(9, 9, [(-30.0, -30.0), (-10.0, -20.0)], 8).size == 0
+
+
+PRIM_BENCH = os.path.join(REPO, "cl-util_amd", "gcmi_prim_bench")
+
+
+@pytest.mark.skipif(L.device_count() > 0, reason="a GPU is present")
+def test_prim_bench_without_device():
+    """The native primitive benchmark is built and exits with status 3 (no
+    usable GPU) instead of measuring anything."""
+    import subprocess
+    assert os.path.exists(PRIM_BENCH)
+    r = subprocess.run([PRIM_BENCH, "1"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 3
+    assert "no HIP device" in r.stderr

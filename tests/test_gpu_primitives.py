@@ -134,3 +134,18 @@ def test_tile_list_reset_increment_default_build():
     assert it == 1
     tl.reset()
     assert tl.iteration == 0 and (tl.tiles() == -1).all()
+
+
+def test_native_prim_bench():
+    """The native benchmark (scan, compact, tile-list build over the 11 image
+    sizes x tiles 16, 32 sweep) checks every result on the host."""
+    import os
+    import subprocess
+    from conftest import REPO
+    exe = os.path.join(REPO, "cl-util_amd", "gcmi_prim_bench")
+    r = subprocess.run([exe, "2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = r.stdout.strip().splitlines()
+    assert rows[0].startswith("img dim,mp,tile dim,num tiles")
+    assert len(rows) == 1 + 22
+    assert all(row.endswith(",ok") for row in rows[1:])
