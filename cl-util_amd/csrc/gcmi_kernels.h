@@ -217,19 +217,49 @@ __device__ __forceinline__ Halo halo_slot(const TG& q, int j) {
 // that exchange stays there with pflag still set, and k_fold (a separate
 // launch, so every store of this kernel is visible) folds it and clears
 // the flag.  No inflow is lost or counted twice.
+// R consecutive int32 of a 4R-byte aligned address, and back.
+template <int R>
+__device__ __forceinline__ void load_row(const int32_t* __restrict__ p, int (&v)[R]) {
+  if constexpr (R == 4) {
+    const int4 a = *reinterpret_cast<const int4*>(p);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  } else if constexpr (R == 2) {
+    const int2 a = *reinterpret_cast<const int2*>(p);
+    v[0] = a.x; v[1] = a.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = p[k];
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void store_row(int32_t* p, const int (&v)[R]) {
+  if constexpr (R == 4) {
+    *reinterpret_cast<int4*>(p) = make_int4(v[0], v[1], v[2], v[3]);
+  } else if constexpr (R == 2) {
+    *reinterpret_cast<int2*>(p) = make_int2(v[0], v[1]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < R; ++k) p[k] = v[k];
+  }
+}
+
 template <int T, int NT, bool FIRST>
 __global__ void __launch_bounds__(NT)
 k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
        int sweeps) {
+  // thread -> R consecutive pixels of one tile row: row ly, columns x0 .. x0+R-1
   constexpr int R = T * T / NT;
+  constexpr int CPR = T / R;  // threads per row
   constexpr int HP = T + 2;
   static_assert(4 * T <= NT, "halo loaders");
+  static_assert(T % R == 0, "row segments");
   __shared__ int32_t H0[HP * HP], H1[HP * HP];
   __shared__ int32_t F[4][T * T];
   __shared__ int32_t BO[4][T];
   __shared__ long long red[NT / kWave];
   __shared__ int sflag;
-  const int lx = threadIdx.x % T, chain = threadIdx.x / T;
+  const int ly = threadIdx.x / CPR, x0 = (threadIdx.x % CPR) * R;
   const int32_t MAXH = g.maxh;
   const int n = *count;
   const int rpos = xcd_pos(blockIdx.x, gridDim.x);
@@ -237,42 +267,71 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
   for (int pos = rpos; pos < n; pos += gridDim.x) {
     const TG q = tile_geo(g, list ? list[pos] : pos);
     const int t = q.t;
-    const size_t base = tbase<T>(t);
+    const size_t row = tbase<T>(t) + ly * T + x0;  // this thread's segment, tile-blocked
     int e[R], h[R], cu[R], cd[R], cl[R], cr[R];
     long long smin = 0, spos = 0;
     if (FIRST) {
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        const int x = q.tx * T + lx, y = q.ty * T + chain * R + k;
         e[k] = 0; cu[k] = 0; cd[k] = 0; cl[k] = 0; cr[k] = 0;
-        if (x < q.W && y < q.H) {
-          const size_t p = q.pix0 + (size_t)y * q.W + x;
-          const int a = s.cs[p], b = s.ct[p];
-          cr[k] = x < q.W - 1 ? s.right[p] : 0;
-          cl[k] = x > 0 ? s.right[p - 1] : 0;
-          cd[k] = y < q.H - 1 ? s.down[p] : 0;
-          cu[k] = y > 0 ? s.down[p - q.W] : 0;
-          e[k] = a - b;
-          smin += min(a, b);
-          spos += e[k] > 0 ? e[k] : 0;
-        }
-        h[k] = e[k] < 0 ? 0 : 1;
       }
-    } else {
+      const int y = q.ty * T + ly, xg = q.tx * T + x0;
+      if (y < q.H && xg < q.W) {
+        const size_t p = q.pix0 + (size_t)y * q.W + xg;
+        if (R > 1 && q.W % R == 0) {
+          // whole aligned segment (pixel ranges start on a multiple of 4)
+          int a[R], b[R], rr[R], dd[R], du[R];
+          load_row<R>(s.cs + p, a);
+          load_row<R>(s.ct + p, b);
+          load_row<R>(s.right + p, rr);
+          load_row<R>(s.down + p, dd);
+          if (y > 0) {
+            load_row<R>(s.down + p - q.W, du);
+          } else {
 #pragma unroll
-      for (int k = 0; k < R; ++k) {
-        const size_t i = base + (chain * R + k) * T + lx;
-        e[k] = s.e[i];
-        h[k] = s.h[i];
-        cu[k] = s.cu[i];
-        cd[k] = s.cd[i];
-        cl[k] = s.cl[i];
-        cr[k] = s.cr[i];
+            for (int k = 0; k < R; ++k) du[k] = 0;
+          }
+          const int rl = xg > 0 ? s.right[p - 1] : 0;
+#pragma unroll
+          for (int k = 0; k < R; ++k) {
+            cr[k] = xg + k < q.W - 1 ? rr[k] : 0;
+            cl[k] = k == 0 ? rl : rr[k - 1];
+            cd[k] = y < q.H - 1 ? dd[k] : 0;
+            cu[k] = du[k];
+            e[k] = a[k] - b[k];
+            smin += min(a[k], b[k]);
+            spos += e[k] > 0 ? e[k] : 0;
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < R; ++k) {
+            const int x = xg + k;
+            if (x < q.W) {
+              const int a = s.cs[p + k], b = s.ct[p + k];
+              cr[k] = x < q.W - 1 ? s.right[p + k] : 0;
+              cl[k] = x > 0 ? s.right[p + k - 1] : 0;
+              cd[k] = y < q.H - 1 ? s.down[p + k] : 0;
+              cu[k] = y > 0 ? s.down[p + k - q.W] : 0;
+              e[k] = a - b;
+              smin += min(a, b);
+              spos += e[k] > 0 ? e[k] : 0;
+            }
+          }
+        }
       }
+#pragma unroll
+      for (int k = 0; k < R; ++k) h[k] = e[k] < 0 ? 0 : 1;
+    } else {
+      load_row<R>(s.e + row, e);
+      load_row<R>(s.h + row, h);
+      load_row<R>(s.cu + row, cu);
+      load_row<R>(s.cd + row, cd);
+      load_row<R>(s.cl + row, cl);
+      load_row<R>(s.cr + row, cr);
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      const int c = (chain * R + k + 1) * HP + lx + 1;
+      const int c = (ly + 1) * HP + x0 + k + 1;
       H0[c] = h[k];
       H1[c] = h[k];
     }
@@ -293,18 +352,19 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
       H1[hs.idx] = v;
       BO[threadIdx.x / T][threadIdx.x % T] = 0;
     }
-    if (threadIdx.x == 0) sflag = s.pflag[t];
+    // round 1 starts from reset tiles: nothing pending
+    if (threadIdx.x == 0) sflag = FIRST ? 0 : s.pflag[t];
     __syncthreads();
     if (sflag) {
       // issue every exchange first, then use the results (one wait)
       int vu[R], vd[R], vl[R], vr[R];
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        const int ly = chain * R + k;
-        vu[k] = ly == 0 ? atomicExch(&s.pend[((size_t)t * 4 + 0) * T + lx], 0) : 0;
-        vd[k] = ly == T - 1 ? atomicExch(&s.pend[((size_t)t * 4 + 1) * T + lx], 0) : 0;
-        vl[k] = lx == 0 ? atomicExch(&s.pend[((size_t)t * 4 + 2) * T + ly], 0) : 0;
-        vr[k] = lx == T - 1 ? atomicExch(&s.pend[((size_t)t * 4 + 3) * T + ly], 0) : 0;
+        const int x = x0 + k;
+        vu[k] = ly == 0 ? atomicExch(&s.pend[((size_t)t * 4 + 0) * T + x], 0) : 0;
+        vd[k] = ly == T - 1 ? atomicExch(&s.pend[((size_t)t * 4 + 1) * T + x], 0) : 0;
+        vl[k] = x == 0 ? atomicExch(&s.pend[((size_t)t * 4 + 2) * T + ly], 0) : 0;
+        vr[k] = x == T - 1 ? atomicExch(&s.pend[((size_t)t * 4 + 3) * T + ly], 0) : 0;
       }
 #pragma unroll
       for (int k = 0; k < R; ++k) {
@@ -325,9 +385,9 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
       int32_t* Hn = (sw & 1) ? H0 : H1;
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        const int ly = chain * R + k;
-        const int i = ly * T + lx;
-        const int c = (ly + 1) * HP + lx + 1;
+        const int x = x0 + k;
+        const int i = ly * T + x;
+        const int c = (ly + 1) * HP + x + 1;
         int fu = 0, fd = 0, fl = 0, fr = 0;
         const int hv = h[k];
         if (e[k] > 0 && hv < MAXH) {
@@ -353,10 +413,10 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
             cl[k] -= fl;
           }
           e[k] = rem;
-          if (ly == 0) BO[0][lx] += fu;
-          if (ly == T - 1) BO[1][lx] += fd;
-          if (lx == 0) BO[2][ly] += fl;
-          if (lx == T - 1) BO[3][ly] += fr;
+          if (ly == 0) BO[0][x] += fu;
+          if (ly == T - 1) BO[1][x] += fd;
+          if (x == 0) BO[2][ly] += fl;
+          if (x == T - 1) BO[3][ly] += fr;
         }
         F[D_UP][i] = fu;
         F[D_DOWN][i] = fd;
@@ -367,9 +427,9 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
       int any = 0;
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        const int ly = chain * R + k;
-        const int i = ly * T + lx;
-        const int c = (ly + 1) * HP + lx + 1;
+        const int x = x0 + k;
+        const int i = ly * T + x;
+        const int c = (ly + 1) * HP + x + 1;
         if (ly > 0) {
           const int v = F[D_DOWN][i - T];
           e[k] += v;
@@ -380,12 +440,12 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
           e[k] += v;
           cd[k] += v;
         }
-        if (lx > 0) {
+        if (x > 0) {
           const int v = F[D_RIGHT][i - 1];
           e[k] += v;
           cl[k] += v;
         }
-        if (lx < T - 1) {
+        if (x < T - 1) {
           const int v = F[D_LEFT][i + 1];
           e[k] += v;
           cr[k] += v;
@@ -411,42 +471,50 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
     }
 
     long long tp = 0;
+    uint32_t mk = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      const size_t i = base + (chain * R + k) * T + lx;
-      s.e[i] = e[k];
-      s.h[i] = h[k];
-      s.cu[i] = cu[k];
-      s.cd[i] = cd[k];
-      s.cl[i] = cl[k];
-      s.cr[i] = cr[k];
-      s.mask[i] = (uint8_t)pixel_mask(e[k], cu[k], cd[k], cl[k], cr[k]);
+      mk |= (uint32_t)pixel_mask(e[k], cu[k], cd[k], cl[k], cr[k]) << (8 * k);
       tp += e[k] > 0 ? e[k] : 0;
+    }
+    store_row<R>(s.e + row, e);
+    store_row<R>(s.h + row, h);
+    store_row<R>(s.cu + row, cu);
+    store_row<R>(s.cd + row, cd);
+    store_row<R>(s.cl + row, cl);
+    store_row<R>(s.cr + row, cr);
+    if constexpr (R == 4) {
+      *reinterpret_cast<uint32_t*>(s.mask + row) = mk;
+    } else if constexpr (R == 2) {
+      *reinterpret_cast<uint16_t*>(s.mask + row) = (uint16_t)mk;
+    } else {
+#pragma unroll
+      for (int k = 0; k < R; ++k) s.mask[row + k] = (uint8_t)(mk >> (8 * k));
     }
     // edge outflow -> neighbour's pending slot (the receiving side is the
     // opposite one); a nonzero outflow implies the arc and so the neighbour
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      const int ly = chain * R + k;
-      if (ly == 0 && q.ty > 0 && BO[0][lx]) {
-        atomicAdd(&s.pend[((size_t)(t - q.tw) * 4 + 1) * T + lx], BO[0][lx]);
+      const int x = x0 + k;
+      if (ly == 0 && q.ty > 0 && BO[0][x]) {
+        atomicAdd(&s.pend[((size_t)(t - q.tw) * 4 + 1) * T + x], BO[0][x]);
         s.pflag[t - q.tw] = 1;
         s.act[t - q.tw] = 1;
         s.dirty[t - q.tw] = 1;
       }
-      if (ly == T - 1 && q.ty < q.th - 1 && BO[1][lx]) {
-        atomicAdd(&s.pend[((size_t)(t + q.tw) * 4 + 0) * T + lx], BO[1][lx]);
+      if (ly == T - 1 && q.ty < q.th - 1 && BO[1][x]) {
+        atomicAdd(&s.pend[((size_t)(t + q.tw) * 4 + 0) * T + x], BO[1][x]);
         s.pflag[t + q.tw] = 1;
         s.act[t + q.tw] = 1;
         s.dirty[t + q.tw] = 1;
       }
-      if (lx == 0 && q.tx > 0 && BO[2][ly]) {
+      if (x == 0 && q.tx > 0 && BO[2][ly]) {
         atomicAdd(&s.pend[((size_t)(t - 1) * 4 + 3) * T + ly], BO[2][ly]);
         s.pflag[t - 1] = 1;
         s.act[t - 1] = 1;
         s.dirty[t - 1] = 1;
       }
-      if (lx == T - 1 && q.tx < q.tw - 1 && BO[3][ly]) {
+      if (x == T - 1 && q.tx < q.tw - 1 && BO[3][ly]) {
         atomicAdd(&s.pend[((size_t)(t + 1) * 4 + 2) * T + ly], BO[3][ly]);
         s.pflag[t + 1] = 1;
         s.act[t + 1] = 1;
@@ -606,8 +674,39 @@ __device__ __forceinline__ void bq_flush(BlockQ& b, const Queue& Q) {
 // tiles, a tile whose edge distances fell flags its neighbours for another
 // round, until no tile is flagged.
 
-// Relaxes d[] (this thread's R pixels) to the in-tile fixed point given
-// halo values already in D0/D1.  Returns after the last barrier.
+// Iterations of an in-tile relaxation before its row scans switch on.
+constexpr int kScanAfter = 2;
+
+// One row of a tile spans T consecutive lanes of a wave (lane % T = column).
+// row_min_scan relaxes each pixel of the row along straight runs of
+// residual arcs: d(x) = min over j of d(x -/+ j) + j where the j arcs from x
+// toward x -/+ j are all residual (the pixel's own M_LEFT / M_RIGHT bits
+// first).  Doubling steps: after the step of size s every lane holds the
+// best value over runs shorter than 2s and whether its run of 2s arcs is
+// open.
+template <int T>
+__device__ __forceinline__ int row_min_scan(int d, int m, int lx) {
+  int a = d, oa = (m & M_LEFT) ? 1 : 0;
+  int b = d, ob = (m & M_RIGHT) ? 1 : 0;
+#pragma unroll
+  for (int s = 1; s < T; s <<= 1) {
+    const int au = __shfl_up(a, s, T), ou = __shfl_up(oa, s, T);
+    const int bd = __shfl_down(b, s, T), od = __shfl_down(ob, s, T);
+    const bool lo = lx >= s, hi = lx + s < T;
+    if (lo && oa) a = min(a, au + s);
+    oa = lo ? (oa & ou) : 0;
+    if (hi && ob) b = min(b, bd + s);
+    ob = hi ? (ob & od) : 0;
+  }
+  return min(a, b);
+}
+
+// Relaxes d[] (this thread's R pixels, rows chain*R .. chain*R+R-1 of
+// column lx) to the in-tile fixed point given halo values already in
+// D0/D1.  Each iteration: one Jacobi step from the LDS copy (across chains
+// and into the halo), then sweeps along the thread's column segment in
+// registers and straight-run scans along every row -- the fixed point is
+// the same, with far fewer barriers.  Returns after the last barrier.
 template <int T, int NT>
 __device__ __forceinline__ void bfs_relax(int (&d)[T * T / NT], const int (&m)[T * T / NT],
                                           int32_t* D0, int32_t* D1) {
@@ -616,7 +715,7 @@ __device__ __forceinline__ void bfs_relax(int (&d)[T * T / NT], const int (&m)[T
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
   int32_t* Dc = D0;
   int32_t* Dn = D1;
-  for (;;) {
+  for (int it = 0;; ++it) {
     int changed = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -633,8 +732,34 @@ __device__ __forceinline__ void bfs_relax(int (&d)[T * T / NT], const int (&m)[T
           changed = 1;
         }
       }
-      Dn[c] = d[k];
     }
+    // column segment, both ways
+#pragma unroll
+    for (int k = 1; k < R; ++k)
+      if ((m[k] & M_UP) && d[k - 1] + 1 < d[k]) {
+        d[k] = d[k - 1] + 1;
+        changed = 1;
+      }
+#pragma unroll
+    for (int k = R - 2; k >= 0; --k)
+      if ((m[k] & M_DOWN) && d[k + 1] + 1 < d[k]) {
+        d[k] = d[k + 1] + 1;
+        changed = 1;
+      }
+    // rows, once the tile has shown paths longer than a few steps (short
+    // ones settle in a couple of iterations without the scans)
+    if (it >= kScanAfter) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int v = row_min_scan<T>(d[k], m[k], lx);
+        if (v < d[k]) {
+          d[k] = v;
+          changed = 1;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) Dn[(chain * R + k + 1) * HP + lx + 1] = d[k];
     if (!__syncthreads_or(changed)) break;
     int32_t* tmp = Dc;
     Dc = Dn;
@@ -778,14 +903,41 @@ k_bfs_round(Geo g, CutDev s, Queue Q) {
 // source set once no pixel with e > 0 can reach a sink.  Same tile scheme
 // as the BFS: in-tile fixed point in LDS, flags across edges.
 
+// Reach along one row (T consecutive lanes): a reached pixel reaches its
+// neighbours along straight runs of residual arcs.  With the row's reach and
+// arc bits as masks (bit x = column x) this is a carry-style fill: per
+// doubling step s, G |= (G << s) & P and P &= P << s, where P marks the
+// columns whose arc from the left (for the rightward fill) is residual.
+template <int T>
+__device__ __forceinline__ int row_reach(int r, int m, int lx) {
+  constexpr unsigned long long seg = T == 64 ? ~0ull : ((1ull << T) - 1);
+  const int base = lane_id() & ~(T - 1) & 63;
+  const unsigned long long g0 = (__ballot(r != 0) >> base) & seg;
+  unsigned long long pr = ((__ballot((m & M_RIGHT) != 0) >> base) << 1) & seg;  // arc x-1 -> x
+  unsigned long long pl = ((__ballot((m & M_LEFT) != 0) >> base) & seg) >> 1;   // arc x+1 -> x
+  unsigned long long gr = g0, gl = g0;
+#pragma unroll
+  for (int s = 1; s < T; s <<= 1) {
+    gr |= (gr << s) & pr;
+    pr &= pr << s;
+    gl |= (gl >> s) & pl;
+    pl &= pl >> s;
+  }
+  return (int)(((gr | gl) & seg) >> lx) & 1;
+}
+
 // Relaxes r[] to the in-tile fixed point.  RM holds (reach | mask << 1) of
 // every tile pixel and halo pixel (halo: the neighbour's reach and mask).
+// Each iteration: one step from the LDS copy (across chains and from the
+// halo), then sweeps along the thread's column segment and fills along
+// every row.
 template <int T, int NT>
-__device__ __forceinline__ void flood_relax(int (&r)[T * T / NT], int32_t* RM) {
+__device__ __forceinline__ void flood_relax(int (&r)[T * T / NT], const int (&m)[T * T / NT],
+                                           int32_t* RM) {
   constexpr int R = T * T / NT;
   constexpr int HP = T + 2;
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
-  for (;;) {
+  for (int it = 0;; ++it) {
     int changed = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -798,6 +950,30 @@ __device__ __forceinline__ void flood_relax(int (&r)[T * T / NT], int32_t* RM) {
           ((lf & 1) && (lf & (M_RIGHT << 1))) || ((rt & 1) && (rt & (M_LEFT << 1)))) {
         r[k] = 1;
         changed = 1;
+      }
+    }
+    // column segment, both ways
+#pragma unroll
+    for (int k = 1; k < R; ++k)
+      if (!r[k] && r[k - 1] && (m[k - 1] & M_DOWN)) {
+        r[k] = 1;
+        changed = 1;
+      }
+#pragma unroll
+    for (int k = R - 2; k >= 0; --k)
+      if (!r[k] && r[k + 1] && (m[k + 1] & M_UP)) {
+        r[k] = 1;
+        changed = 1;
+      }
+    // rows (as in bfs_relax, after the first iterations)
+    if (it >= kScanAfter) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int v = row_reach<T>(r[k], m[k], lx);
+        if (v && !r[k]) {
+          r[k] = 1;
+          changed = 1;
+        }
       }
     }
     // publish new reach bits (only ever set, so a reader racing the update
@@ -854,7 +1030,7 @@ k_flood(Geo g, CutDev s, Queue Q, int first) {
       RM[hs.idx] = v;
     }
     __syncthreads();
-    flood_relax<T, NT>(r, RM);
+    flood_relax<T, NT>(r, m, RM);
     int dirs = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {

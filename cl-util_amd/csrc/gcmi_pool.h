@@ -92,6 +92,13 @@ struct PoolOps {
 
 const PoolOps* pool_ops(int tile);
 
+// Row-major pixel ranges start on a multiple of kPixAlign elements, so the
+// rows of an image whose width is a multiple of 4 load as 16-byte vectors.
+constexpr long long kPixAlign = 4;
+inline long long pix_len(int W, int H) {
+  return ((long long)W * H + kPixAlign - 1) / kPixAlign * kPixAlign;
+}
+
 struct PoolImage {
   bool live = false;
   int W = 0, H = 0, tw = 0, th = 0;

@@ -207,10 +207,15 @@ static const PoolOps* ops_for() {
   return &ops;
 }
 
+// Threads per tile for each tile size (a build-time tuning constant).
+#ifndef GCMI_NT32
+#define GCMI_NT32 256
+#endif
+
 const PoolOps* pool_ops(int tile) {
   switch (tile) {
     case 16: return ops_for<16, 256>();
-    case 32: return ops_for<32, 256>();
+    case 32: return ops_for<32, GCMI_NT32>();
     case 64: return ops_for<64, 1024>();
     default: return nullptr;
   }
@@ -274,7 +279,8 @@ size_t pool_bytes(int tile, int cap_tiles, long long cap_pix, int cap_imgs) {
   const size_t np = (size_t)cap_tiles * tile * tile;
   return np * (6 * 4 + 2) + (size_t)cap_tiles * (4 * tile * 4 + 15 * 4 + 8) +
          (size_t)cap_pix * (4 * 4 + 1) +
-         (size_t)cap_imgs * (sizeof(ImgDesc) + SUM_ROWS * kSumSlots * 8 + sizeof(Prep) + 8) +
+         (size_t)cap_imgs *
+             (sizeof(ImgDesc) + SUM_ROWS * kSumSlots * 8 + sizeof(Prep) + 8 + kPixAlign * 17) +
          (size_t)compact_scratch_ints(cap_tiles) * 4 + 4096;
 }
 
@@ -290,6 +296,8 @@ static int dev_alloc(void** ptr, size_t bytes, size_t* total) {
 
 int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
               hipStream_t stream) {
+  // room for every image's range to start on a multiple of kPixAlign
+  cap_pix = (cap_pix + kPixAlign - 1) / kPixAlign * kPixAlign + (long long)kPixAlign * cap_imgs;
   p->ops = pool_ops(tile);
   if (!p->ops) {
     set_error("tile must be 16, 32 or 64 (got %d)", tile);
@@ -436,7 +444,7 @@ int pool_add_image(Pool* p, int W, int H, int* out) {
     set_error("no room for %lld tiles (largest free run %lld)", nt, p->tiles_alloc.largest());
     return GCMI_EBUDGET;
   }
-  const long long p0 = p->pix_alloc.alloc((long long)W * H);
+  const long long p0 = p->pix_alloc.alloc(pix_len(W, H));
   if (p0 < 0) {
     p->tiles_alloc.release(t0, nt);
     set_error("no room for %lld pixels", (long long)W * H);
@@ -469,7 +477,7 @@ int pool_remove_image(Pool* p, int img) {
   }
   PoolImage& im = p->imgs[img];
   p->tiles_alloc.release(im.tile0, im.ntiles);
-  p->pix_alloc.release(im.pix0, (long long)im.W * im.H);
+  p->pix_alloc.release(im.pix0, pix_len(im.W, im.H));
   im = PoolImage();
   return GCMI_OK;
 }
