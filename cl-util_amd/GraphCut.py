@@ -20,7 +20,7 @@ import ctypes
 
 import numpy as np
 
-from gcmi_lib import DeviceBuffer, check, host_ptr, lib, ptr
+from gcmi_lib import OPT_LOCAL_RELABEL, DeviceBuffer, check, host_ptr, lib, ptr
 
 TILE_DEFAULT = (32, 32)
 BFS_DEFAULT = 0       # push rounds per global relabel (0: library default, 1)
@@ -171,6 +171,11 @@ class GraphCut:
         a = (ctypes.c_int64 * len(STAT_NAMES))()
         check(lib().gcmi_cut_stats(self._h, a, len(STAT_NAMES)))
         return dict(zip(STAT_NAMES, [int(v) for v in a]))
+
+    def set_local_relabel(self, enable=True):
+        """Reflows of edits relabel only the changed tiles (and a ring around
+        them) for their first relabels; the cut is the same either way."""
+        check(lib().gcmi_cut_set_option(self._h, OPT_LOCAL_RELABEL, 1 if enable else 0))
 
     def profile(self, enable=True):
         """Time every later run's kernels with HIP events on the cut's stream."""

@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 
 from GraphCut import STAT_NAMES, _edit_arrays, _i32, _points
-from gcmi_lib import KERNEL_CLASSES, check, host_ptr, lib, ptr
+from gcmi_lib import KERNEL_CLASSES, OPT_LOCAL_RELABEL, check, host_ptr, lib, ptr
 
 
 class GraphCutBatch:
@@ -95,6 +95,10 @@ class GraphCutBatch:
         a = (ctypes.c_int64 * len(STAT_NAMES))()
         check(lib().gcmi_batch_stats(self._h, a, len(STAT_NAMES)))
         return dict(zip(STAT_NAMES, [int(v) for v in a]))
+
+    def set_local_relabel(self, enable=True):
+        """As GraphCut.set_local_relabel."""
+        check(lib().gcmi_batch_set_option(self._h, OPT_LOCAL_RELABEL, 1 if enable else 0))
 
     def profile(self, enable=True):
         check(lib().gcmi_batch_profile(self._h, 1 if enable else 0))

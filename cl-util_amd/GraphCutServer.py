@@ -23,8 +23,8 @@ import ctypes
 import numpy as np
 
 from GraphCut import _i32, _points
-from gcmi_lib import (EBUSY, REQ_IMAGE, REQ_POLYLINE, REQ_STROKE, GcmiError, Request, _i64p,
-                      check, host_ptr, lib)
+from gcmi_lib import (EBUSY, OPT_LOCAL_RELABEL, REQ_IMAGE, REQ_POLYLINE, REQ_STROKE, GcmiError,
+                      Request, _i64p, check, host_ptr, lib)
 
 QUEUED, DONE, FAILED = 0, 1, 2
 MODES = {"batched": 0, "plain": 1, "cold": 2}
@@ -52,7 +52,7 @@ class RequestInfo:
 
 class GraphCutServer:
     def __init__(self, tile=32, hbm_budget=200 * 10**9, max_sessions=256, max_batch=64,
-                 mode="batched", sweeps=0, bfs=0):
+                 mode="batched", sweeps=0, bfs=0, local_relabel=False):
         self.tile = int(tile[0] if isinstance(tile, (tuple, list)) else tile)
         if mode not in MODES:
             raise ValueError("mode must be one of %s" % sorted(MODES))
@@ -64,6 +64,8 @@ class GraphCutServer:
         self.dims = {}
         if sweeps or bfs:
             check(lib().gcmi_server_set_params(self._h, int(sweeps), int(bfs)))
+        if local_relabel:
+            check(lib().gcmi_server_set_option(self._h, OPT_LOCAL_RELABEL, 1))
 
     def close_server(self):
         if getattr(self, "_h", None):

@@ -194,6 +194,14 @@ int gcmi_cut_profile(gcmi_cut* c, int enable) {
   return GCMI_OK;
 }
 
+int gcmi_cut_set_option(gcmi_cut* c, int option, int value) {
+  if (!c) {
+    set_error("null handle");
+    return GCMI_EINVAL;
+  }
+  return pool_set_option(&c->pool, option, value);
+}
+
 int gcmi_cut_kernel_times(gcmi_cut* c, double* ms_out, int64_t* launches_out, int n) {
   if (!c || n < 0) {
     set_error("bad argument");
@@ -347,6 +355,14 @@ int gcmi_batch_profile(gcmi_batch* b, int enable) {
   b->pool.prof.on = enable != 0;
   b->pool.prof.reset();
   return GCMI_OK;
+}
+
+int gcmi_batch_set_option(gcmi_batch* b, int option, int value) {
+  if (!b) {
+    set_error("null handle");
+    return GCMI_EINVAL;
+  }
+  return pool_set_option(&b->pool, option, value);
 }
 
 int gcmi_batch_kernel_times(gcmi_batch* b, double* ms_out, int64_t* launches_out, int n) {

@@ -68,6 +68,7 @@ struct CutDev {
   int32_t* bflag;                      // tile flagged by the relabel's first pass
   int32_t* dirty;                      // residual arcs or excess changed since the
                                        // tile's distances were last computed
+  int32_t* rflag;                      // tile being recomputed by a local relabel
   int32_t* haspos;                     // tile holds excess
   long long* tpos;                     // per tile: sum of max(e, 0)
   unsigned long long* sums;            // per image [SUM_ROWS][kSumSlots]
@@ -808,8 +809,9 @@ __device__ __forceinline__ void bfs_relax(int (&d)[T * T / NT], const int (&m)[T
 // completion test (check_completion, graphcut.cl:20-60) for the tile:
 //   act[t]    = some pixel has e > 0 and a finite distance
 //   haspos[t] = some pixel has e > 0
-// all != 0: every tile of the images is listed (a full relabel).
-// Halo pixels of dirty neighbours start as 0 (sink) or MAX; those of clean
+// all != 0: every tile of the images is listed (a full relabel); else the
+// listed tiles are those with rflag set.
+// Halo pixels of listed neighbours start as 0 (sink) or MAX; those of clean
 // neighbours keep their stored distances, which stay valid: a clean tile's
 // arcs and excess have not changed since its distances were computed, and
 // the rounds lower any of them that a recomputed tile now undercuts.  Later
@@ -849,7 +851,7 @@ k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __r
       int v = g.maxh;
       if (hs.nb >= 0) {
         const size_t p = tbase<T>(hs.nb) + hs.ny * T + hs.nx;
-        v = (all || s.dirty[hs.nb]) ? ((s.mask[p] & M_SINK) ? 0 : g.maxh) : s.h[p];
+        v = (all || s.rflag[hs.nb]) ? ((s.mask[p] & M_SINK) ? 0 : g.maxh) : s.h[p];
       }
       D0[hs.idx] = v;
       D1[hs.idx] = v;

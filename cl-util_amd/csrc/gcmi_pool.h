@@ -18,6 +18,7 @@ constexpr int kDefaultBfs = 1;     // push rounds per global relabel
 constexpr int kRoundBatch = 2;     // BFS / flood rounds before the first host check
 constexpr int kRoundBatchMax = 8;  // ... and before each later one
 constexpr int kRelabelGridCap = 65536;  // blocks of a relabel's first pass
+constexpr int kLocalRelabels = 6;       // local relabels per solve, then full ones
 
 enum {
   CNT_ACT = 0,
@@ -138,8 +139,11 @@ struct Pool {
   int32_t *sel = nullptr, *fresh = nullptr, *plist = nullptr, *alist = nullptr;
   int32_t* q = nullptr;  // three rotating tile queues of cap_tiles entries
   int32_t* dlist = nullptr;  // dirty tiles of the current relabel
-  // Relabel only the dirty tiles (experimental: cheaper relabels of small
-  // edits, but clean tiles keep valid yet possibly stale distances).
+  // In solves of edited images only: relabel the tiles that changed (and
+  // one ring around them) instead of every tile.  Clean tiles keep their
+  // stored distances, valid but possibly stale; the solve switches to full
+  // relabels after kLocalRelabels.  Exact either way: termination is
+  // decided by a relabel over a valid labelling.
   bool local_relabel = false;
   int epoch = 0;         // last queue epoch handed out (stamps hold epochs)
   // the lists the kernels of the current solve read: null when the
@@ -215,6 +219,8 @@ int pool_labels_packed(Pool* p, int img, uint32_t* out);
 int pack_labels_async(Pool* p, int img, uint32_t* d_out);
 // Drops the solved state: the next solve starts from the capacities.
 int pool_forget(Pool* p, int img);
+// GCMI_OPT_* options of the C ABI.
+int pool_set_option(Pool* p, int option, int value);
 int pool_heights(Pool* p, int img, int32_t* out);
 int validate_caps(const int32_t* cs, const int32_t* ct, const int32_t* right,
                   const int32_t* down, size_t n, int32_t* max_pair);

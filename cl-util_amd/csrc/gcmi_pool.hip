@@ -96,6 +96,24 @@ __global__ void k_clear_listed(int32_t* flags, const int32_t* __restrict__ list,
     flags[list ? list[i] : i] = 0;
 }
 
+// Local relabel set: rflag[t] = t or one of its four neighbours is dirty
+// (one ring around every changed tile, so a distance that rose next to an
+// edit is recomputed rather than left to the stale neighbours).
+__global__ void k_dilate(Geo g, CutDev s, const int32_t* __restrict__ list,
+                         const int32_t* __restrict__ count) {
+  const int n = *count;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const TG q = tile_geo(g, list ? list[i] : i);
+    const int t = q.t;
+    int f = s.dirty[t];
+    if (!f && q.ty > 0) f = s.dirty[t - q.tw];
+    if (!f && q.ty < q.th - 1) f = s.dirty[t + q.tw];
+    if (!f && q.tx > 0) f = s.dirty[t - 1];
+    if (!f && q.tx < q.tw - 1) f = s.dirty[t + 1];
+    s.rflag[t] = f;
+  }
+}
+
 // Builds the solve's tile lists, zeroes the sums it recomputes (all rows of
 // a fresh image, the final-excess row of every image) and the tallies.
 __global__ void k_prepare(Geo g, CutDev s, const Prep* __restrict__ prep, int nsel_tiles,
@@ -277,7 +295,7 @@ int tiles_for(int W, int H, int tile, int* tw, int* th) {
 
 size_t pool_bytes(int tile, int cap_tiles, long long cap_pix, int cap_imgs) {
   const size_t np = (size_t)cap_tiles * tile * tile;
-  return np * (6 * 4 + 2) + (size_t)cap_tiles * (4 * tile * 4 + 15 * 4 + 8) +
+  return np * (6 * 4 + 2) + (size_t)cap_tiles * (4 * tile * 4 + 16 * 4 + 8) +
          (size_t)cap_pix * (4 * 4 + 1) +
          (size_t)cap_imgs *
              (sizeof(ImgDesc) + SUM_ROWS * kSumSlots * 8 + sizeof(Prep) + 8 + kPixAlign * 17) +
@@ -345,6 +363,7 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   ALLOC(d.stamp, (size_t)cap_tiles * 4);
   ALLOC(d.bflag, (size_t)cap_tiles * 4);
   ALLOC(d.dirty, (size_t)cap_tiles * 4);
+  ALLOC(d.rflag, (size_t)cap_tiles * 4);
   ALLOC(d.haspos, (size_t)cap_tiles * 4);
   ALLOC(d.tpos, (size_t)cap_tiles * 8);
   ALLOC(d.sums, (size_t)cap_imgs * SUM_ROWS * kSumSlots * 8);
@@ -396,7 +415,7 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   p->pix_alloc.init(cap_pix);
   // pending slots and flags start (and stay, between solves) zero
   GCMI_HIP(hipMemsetAsync(d.pend, 0, (size_t)cap_tiles * 4 * tile * 4, p->stream));
-  int32_t* flags[] = {d.pflag, d.act, d.stamp, d.bflag, d.dirty, d.haspos};
+  int32_t* flags[] = {d.pflag, d.act, d.stamp, d.bflag, d.dirty, d.rflag, d.haspos};
   for (int32_t* f : flags) GCMI_HIP(hipMemsetAsync(f, 0, (size_t)cap_tiles * 4, p->stream));
   GCMI_HIP(hipMemsetAsync(d.tpos, 0, (size_t)cap_tiles * 8, p->stream));
   GCMI_HIP(hipStreamSynchronize(p->stream));
@@ -407,7 +426,7 @@ void pool_free(Pool* p) {
   CutDev& d = p->dev;
   void* ptrs[] = {d.e,     d.h,       d.cu,     d.cd,       d.cl,         d.cr,       d.mask,
                   d.reach, d.pend,    d.pflag,  d.act,      d.stamp,      d.bflag,    d.dirty,
-                  d.haspos,
+                  d.rflag, d.haspos,
                   d.tpos,  d.sums,    d.tally,  d.cs,       d.ct,         (void*)d.right,
                   (void*)d.down,      d.labels, p->d_img,   p->tile_img,  p->sel,     p->fresh,
                   p->q,    p->dlist,  p->plist, p->alist,   p->counts,    p->scratch, p->d_prep,
@@ -829,6 +848,7 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
     p->stats[STAT_RESIDENCIES] += nfresh;
   }
   int32_t nact = 0;
+  int nlocal = 0;  // local relabels so far
   if (nsel > nfresh) {
     // edited images: a push round over the tiles the edits made active
     // (with any the first round flagged) before the first relabel
@@ -866,22 +886,33 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
     // sent flow to), recompute their distances, and let the rounds carry the
     // changes outward: a global relabel, restricted to where it can differ
     // (or, by default, over every selected tile)
+    // (or over every selected tile: always when an image is cut from
+    // scratch, and after kLocalRelabels local ones, which bounds the cost of
+    // the stale distances a local relabel leaves in clean tiles)
     const int32_t* glist = p->lsel;
     int gcnt = CNT_SEL;
-    if (p->local_relabel) {
-      GCMI_TRY(compact_sel(p, p->dev.dirty, p->dlist, CNT_DIRTY, false, nsel));
+    const bool local = p->local_relabel && nfresh == 0 && nlocal < kLocalRelabels;
+    if (local) {
+      hipLaunchKernelGGL(k_dilate, dim3(std::min((nsel + 255) / 256, 4096)), dim3(256), 0,
+                         p->stream, p->geo, p->dev, p->lsel, p->counts + CNT_SEL);
+      GCMI_TRY(compact_sel(p, p->dev.rflag, p->dlist, CNT_DIRTY, false, nsel));
       glist = p->dlist;
       gcnt = CNT_DIRTY;
+      ++nlocal;
     }
     B(K_FOLD);
     ops->fold(p, grid_relabel, glist, gcnt);
     E();
     B(K_BFS_INIT);
-    ops->bfs_init(p, grid_relabel, glist, gcnt, !p->local_relabel);
+    ops->bfs_init(p, grid_relabel, glist, gcnt, !local);
     E();
     GCMI_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_clear_listed, dim3(std::min((nsel + 255) / 256, 4096)), dim3(256), 0,
-                       p->stream, p->dev.dirty, glist, p->counts + gcnt);
+    const int gclear = std::min((nsel + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_clear_listed, dim3(gclear), dim3(256), 0, p->stream, p->dev.dirty, glist,
+                       p->counts + gcnt);
+    if (local)
+      hipLaunchKernelGGL(k_clear_listed, dim3(gclear), dim3(256), 0, p->stream, p->dev.rflag,
+                         glist, p->counts + gcnt);
     p->stats[STAT_GLOBAL_RELABELS]++;
     // the flagged tiles start the first queue
     clear_queues(p);
@@ -1050,6 +1081,17 @@ int pool_labels_packed(Pool* p, int img, uint32_t* out) {
   GCMI_HIP(hipStreamSynchronize(p->stream));
   memcpy(out, p->h_bits, (size_t)words * 4);
   return GCMI_OK;
+}
+
+int pool_set_option(Pool* p, int option, int value) {
+  switch (option) {
+    case GCMI_OPT_LOCAL_RELABEL:
+      p->local_relabel = value != 0;
+      return GCMI_OK;
+    default:
+      set_error("unknown option %d", option);
+      return GCMI_EINVAL;
+  }
 }
 
 int pool_forget(Pool* p, int img) {

@@ -338,6 +338,15 @@ int gcmi_server_set_params(gcmi_server* s, int sweeps, int bfs) {
   return GCMI_OK;
 }
 
+int gcmi_server_set_option(gcmi_server* s, int option, int value) {
+  if (!s) {
+    set_error("null server");
+    return GCMI_EINVAL;
+  }
+  std::lock_guard<std::mutex> plk(s->pool_mu);
+  return pool_set_option(&s->pool, option, value);
+}
+
 int gcmi_server_open(gcmi_server* s, int width, int height, int* session_out) {
   if (!s || !session_out) {
     set_error("null argument");

@@ -87,6 +87,9 @@ SIGNATURES = {
     "gcmi_cut_paint": [_vp, _int, _vp, ctypes.c_double, _i32, _i32],
     "gcmi_cut_labels_packed": [_vp, _vp],
     "gcmi_cut_forget": [_vp],
+    "gcmi_cut_set_option": [_vp, _int, _int],
+    "gcmi_batch_set_option": [_vp, _int, _int],
+    "gcmi_server_set_option": [_vp, _int, _int],
     "gcmi_batch_paint": [_vp, _int, _int, _vp, ctypes.c_double, _i32, _i32],
     "gcmi_server_submit_polyline": [_vp, _int, _int, _vp, ctypes.c_double, _i32, _i32, _i64p],
     "gcmi_server_replay": [_vp, _int, _vp, _i64p],
@@ -115,6 +118,7 @@ SIGNATURES = {
 }
 
 REQ_IMAGE, REQ_STROKE, REQ_POLYLINE = 0, 1, 2
+OPT_LOCAL_RELABEL = 1
 
 
 class Request(ctypes.Structure):

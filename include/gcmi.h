@@ -196,6 +196,13 @@ int gcmi_cut_geometry(gcmi_cut *cut, int *out3);
  *   2 fold  3 bfs_init  4 bfs_round  5 flood  6 labels  7 compact */
 #define GCMI_KERNEL_CLASSES 8
 int gcmi_cut_profile(gcmi_cut *cut, int enable);
+/* Solver options (no reference counterpart).
+ * GCMI_OPT_LOCAL_RELABEL (default 0): in runs that only reflow edits, the
+ *   global relabel recomputes the tiles whose arcs or excess changed (and
+ *   one ring of tiles around them) instead of every tile, for at most a few
+ *   relabels per run; the result is the same exact cut either way. */
+#define GCMI_OPT_LOCAL_RELABEL 1
+int gcmi_cut_set_option(gcmi_cut *cut, int option, int value);
 /* Summed milliseconds and launch counts per class for the last run. */
 int gcmi_cut_kernel_times(gcmi_cut *cut, double *ms_out, int64_t *launches_out,
                           int n);
@@ -234,6 +241,8 @@ int gcmi_batch_run_subset(gcmi_batch *batch, int n, const int *which,
 int gcmi_batch_labels(gcmi_batch *batch, int i, uint8_t *h_out);
 int gcmi_batch_stats(gcmi_batch *batch, int64_t *out, int n);
 int gcmi_batch_profile(gcmi_batch *batch, int enable);
+/* GCMI_OPT_* options, as gcmi_cut_set_option. */
+int gcmi_batch_set_option(gcmi_batch *batch, int option, int value);
 int gcmi_batch_kernel_times(gcmi_batch *batch, double *ms_out,
                             int64_t *launches_out, int n);
 
@@ -264,6 +273,8 @@ int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions,
 int gcmi_server_destroy(gcmi_server *server);
 /* sweeps / bfs of every later solve (<= 0: defaults, as gcmi_cut_run). */
 int gcmi_server_set_params(gcmi_server *server, int sweeps, int bfs);
+/* GCMI_OPT_* options of every later solve, as gcmi_cut_set_option. */
+int gcmi_server_set_option(gcmi_server *server, int option, int value);
 /* Admits a width x height session if a slot and pool room are free
  * (GCMI_EBUDGET otherwise); *session_out = its id. */
 int gcmi_server_open(gcmi_server *server, int width, int height,

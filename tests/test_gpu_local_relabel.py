@@ -1,0 +1,113 @@
+"""Warm reflow with the local relabel option (only the tiles that changed,
+and a ring around them, are relabelled for the first relabels of a run):
+after every edit the cut equals a from-scratch oracle cut, exactly as with
+full relabels."""
+import numpy as np
+import pytest
+
+import gcmi_workloads as G
+import oracle as O
+from GraphCut import GraphCut
+from GraphCutBatch import GraphCutBatch
+from GraphCutServer import GraphCutServer
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("tile", [16, 32, 64])
+def test_local_relabel_pixel_strokes(tile):
+    W, H = 384, 256
+    w = G.centred(W, H, seed=13)
+    gc = GraphCut(None, None, (W, H), tile=tile)
+    gc.set_local_relabel(True)
+    gc.upload(*w.arrays())
+    gc.cut()
+    cur = w.copy()
+    for k, st in enumerate(G.strokes(W, H, count=10, seed=15)):
+        G.apply_stroke(cur, st)
+        s, t = st.values()
+        gc.edit(st.index, s, t)
+        f = gc.cut()
+        fo, mo = O.bk_cut(*cur.arrays())
+        assert f == fo, (tile, k)
+        assert (gc.labels() == mo).all(), (tile, k)
+
+
+def test_local_relabel_soft_edits_and_random_grids():
+    """Soft unary changes (not only hard constraints) on random grids."""
+    rng = np.random.default_rng(5)
+    for trial in range(4):
+        W, H = [(97, 61), (200, 33), (64, 64), (130, 170)][trial]
+        w = G.tiny(W, H, 40 + trial, cap_hi=300, w_hi=60)
+        gc = GraphCut(None, None, (W, H), tile=16)
+        gc.set_local_relabel(True)
+        gc.upload(*w.arrays())
+        gc.cut()
+        cur = w.copy()
+        for k in range(6):
+            n = int(rng.integers(1, 200))
+            idx = rng.choice(W * H, size=n, replace=False).astype(np.int32)
+            s = rng.integers(0, 400, size=n).astype(np.int32)
+            t = rng.integers(0, 400, size=n).astype(np.int32)
+            cur.source.reshape(-1)[idx] = s
+            cur.sink.reshape(-1)[idx] = t
+            gc.edit(idx, s, t)
+            f = gc.cut()
+            fo, mo = O.bk_cut(*cur.arrays())
+            assert f == fo, (trial, k)
+            assert (gc.labels() == mo).all(), (trial, k)
+
+
+def test_local_relabel_2048_brush():
+    W = H = 2048
+    work = G.centred(W, H, seed=3)
+    strokes = G.strokes(W, H, count=6, seed=21)
+    gc = GraphCut(None, None, (W, H), tile=32)
+    gc.set_local_relabel(True)
+    gc.upload(*work.arrays())
+    gc.cut()
+    for k, st in enumerate(strokes):
+        s, t = st.capacities()
+        gc.paint(st.points, st.width, s, t)
+        G.apply_stroke(work, st)
+        f = gc.cut()
+        fo, mo = O.bk_cut(*work.arrays())
+        assert f == fo, k
+        assert (gc.labels() == mo).all(), k
+
+
+def test_local_relabel_batch_and_server():
+    dims = [(256, 256), (300, 200), (128, 128), (400, 100)]
+    works = [G.three_ellipses(w, h, seed=30 + i) for i, (w, h) in enumerate(dims)]
+    strokes = [G.strokes(w, h, count=3, seed=40 + i) for i, (w, h) in enumerate(dims)]
+    b = GraphCutBatch(dims, tile=32)
+    b.set_local_relabel(True)
+    for i, w in enumerate(works):
+        b.upload(i, *w.arrays())
+    b.cut()
+    cur = [w.copy() for w in works]
+    for r in range(3):
+        for i in range(len(dims)):
+            st = strokes[i][r]
+            s, t = st.capacities()
+            b.paint(i, st.points, st.width, s, t)
+            G.apply_stroke(cur[i], st)
+        flows = b.cut()
+        for i in range(len(dims)):
+            fo, mo = O.bk_cut(*cur[i].arrays())
+            assert flows[i] == fo and (b.labels(i) == mo).all(), (r, i)
+    with GraphCutServer(tile=32, hbm_budget=1 << 30, max_sessions=8,
+                        local_relabel=True) as srv:
+        sids = [srv.open(w, h) for w, h in dims]
+        trace = [(0, sids[i], "image", works[i].arrays()) for i in range(len(dims))]
+        for r in range(3):
+            for i in range(len(dims)):
+                st = strokes[i][r]
+                s, t = st.capacities()
+                trace.append((0, sids[i], "polyline", (st.points, st.width, s, t)))
+        ids = srv.replay(trace)
+        for r in ids:
+            srv.wait(int(r), timeout=60)
+        for i, sid in enumerate(sids):
+            mask, _ = srv.mask(sid)
+            assert (mask == O.bk_cut(*cur[i].arrays())[1]).all(), i
