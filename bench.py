@@ -166,7 +166,9 @@ def batched_leg(args):
             b.cut(sweeps=sweeps, bfs=bfs)
         return (time.perf_counter() - t0) / reps
 
-    # autotuner: tile x sweeps x push rounds per relabel, one timed run each
+    # autotuner: tile x sweeps x push rounds per relabel x recording (0:
+    # host-driven; r > 0: one replayed hipGraph with r BFS rounds per
+    # relabel), two timed runs each
     log("batched: autotuning")
     tune = []
     batches = {}
@@ -174,14 +176,19 @@ def batched_leg(args):
         batches[tile] = make_batch(tile)
         for sweeps in (2, 4, 8):
             for bfs in (1, 2):
-                dt = time_batch(batches[tile], sweeps, bfs, 2)
-                tune.append({"tile": tile, "sweeps": sweeps, "bfs": bfs,
-                             "ms": round(dt * 1e3, 3)})
+                for graph in (0, 4, 8):
+                    batches[tile].set_graph(graph)
+                    dt = time_batch(batches[tile], sweeps, bfs, 2)
+                    tune.append({"tile": tile, "sweeps": sweeps, "bfs": bfs, "graph": graph,
+                                 "ms": round(dt * 1e3, 3)})
     best = min(tune, key=lambda r: r["ms"])
     b = batches[best["tile"]]
+    b.set_graph(best["graph"])
     reps = max(3, args.steps // 2)
     t_batch = time_batch(b, best["sweeps"], best["bfs"], reps)
+    host = min((r for r in tune if r["graph"] == 0), key=lambda r: r["ms"])
     flows = b.cut(sweeps=best["sweeps"], bfs=best["bfs"])
+    stats = b.stats()
     # sequential plain cuts with the same settings
     log("batched: 64 sequential plain cuts")
     cuts = []
@@ -207,7 +214,9 @@ def batched_leg(args):
             "batched_mpix_s": round(pixels / t_batch / 1e6, 1),
             "sequential_ms": round(t_seq * 1e3, 3),
             "sequential_mpix_s": round(pixels / t_seq / 1e6, 1),
-            "speedup": round(t_seq / t_batch, 2), "flows_equal_sequential": bool(same)}
+            "speedup": round(t_seq / t_batch, 2), "flows_equal_sequential": bool(same),
+            "best_host_driven": host, "host_syncs": stats["host_syncs"],
+            "rounds": stats["rounds"], "global_relabels": stats["global_relabels"]}
 
 
 # ------------------------------------------------------------------- warm

@@ -20,7 +20,7 @@ import ctypes
 
 import numpy as np
 
-from gcmi_lib import OPT_LOCAL_RELABEL, DeviceBuffer, check, host_ptr, lib, ptr
+from gcmi_lib import OPT_DEVICE_ROUNDS, OPT_GRAPH, OPT_LOCAL_RELABEL, DeviceBuffer, check, host_ptr, lib, ptr
 
 TILE_DEFAULT = (32, 32)
 BFS_DEFAULT = 0       # push rounds per global relabel (0: library default, 1)
@@ -176,6 +176,26 @@ class GraphCut:
         """Reflows of edits relabel only the changed tiles (and a ring around
         them) for their first relabels; the cut is the same either way."""
         check(lib().gcmi_cut_set_option(self._h, OPT_LOCAL_RELABEL, 1 if enable else 0))
+
+    def set_device_rounds(self, rounds=8):
+        """rounds > 0: device-driven runs -- fixed launch shapes with `rounds`
+        BFS / flood rounds per relabel and one host synchronisation at the
+        end (the host loop finishes a run that needs more); 0: host-driven.
+        The cut is the same either way."""
+        check(lib().gcmi_cut_set_option(self._h, OPT_DEVICE_ROUNDS, int(rounds)))
+
+    def set_graph(self, rounds=8):
+        """rounds > 0: device-driven runs with `rounds` rounds per relabel,
+        recorded once as a hipGraph and replayed for later runs of the same
+        images; 0: host-driven runs, no recording."""
+        if int(rounds) > 0:
+            check(lib().gcmi_cut_set_option(self._h, OPT_DEVICE_ROUNDS, int(rounds)))
+            check(lib().gcmi_cut_set_option(self._h, OPT_GRAPH, 1))
+        elif int(rounds) == 0:
+            check(lib().gcmi_cut_set_option(self._h, OPT_GRAPH, 0))
+            check(lib().gcmi_cut_set_option(self._h, OPT_DEVICE_ROUNDS, 0))
+        else:
+            check(lib().gcmi_cut_set_option(self._h, OPT_DEVICE_ROUNDS, int(rounds)))
 
     def profile(self, enable=True):
         """Time every later run's kernels with HIP events on the cut's stream."""

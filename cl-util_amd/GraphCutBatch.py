@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 
 from GraphCut import STAT_NAMES, _edit_arrays, _i32, _points
-from gcmi_lib import KERNEL_CLASSES, OPT_LOCAL_RELABEL, check, host_ptr, lib, ptr
+from gcmi_lib import KERNEL_CLASSES, OPT_DEVICE_ROUNDS, OPT_GRAPH, OPT_LOCAL_RELABEL, check, host_ptr, lib, ptr
 
 
 class GraphCutBatch:
@@ -99,6 +99,26 @@ class GraphCutBatch:
     def set_local_relabel(self, enable=True):
         """As GraphCut.set_local_relabel."""
         check(lib().gcmi_batch_set_option(self._h, OPT_LOCAL_RELABEL, 1 if enable else 0))
+
+    def set_device_rounds(self, rounds=8):
+        """rounds > 0: device-driven runs -- fixed launch shapes with `rounds`
+        BFS / flood rounds per relabel and one host synchronisation at the
+        end (the host loop finishes a run that needs more); 0: host-driven.
+        The cut is the same either way."""
+        check(lib().gcmi_batch_set_option(self._h, OPT_DEVICE_ROUNDS, int(rounds)))
+
+    def set_graph(self, rounds=8):
+        """rounds > 0: device-driven runs with `rounds` rounds per relabel,
+        recorded once as a hipGraph and replayed for later runs of the same
+        images; 0: host-driven runs, no recording."""
+        if int(rounds) > 0:
+            check(lib().gcmi_batch_set_option(self._h, OPT_DEVICE_ROUNDS, int(rounds)))
+            check(lib().gcmi_batch_set_option(self._h, OPT_GRAPH, 1))
+        elif int(rounds) == 0:
+            check(lib().gcmi_batch_set_option(self._h, OPT_GRAPH, 0))
+            check(lib().gcmi_batch_set_option(self._h, OPT_DEVICE_ROUNDS, 0))
+        else:
+            check(lib().gcmi_batch_set_option(self._h, OPT_DEVICE_ROUNDS, int(rounds)))
 
     def profile(self, enable=True):
         check(lib().gcmi_batch_profile(self._h, 1 if enable else 0))

@@ -23,7 +23,7 @@ import ctypes
 import numpy as np
 
 from GraphCut import _i32, _points
-from gcmi_lib import (EBUSY, OPT_LOCAL_RELABEL, REQ_IMAGE, REQ_POLYLINE, REQ_STROKE, GcmiError,
+from gcmi_lib import (EBUSY, OPT_DEVICE_ROUNDS, OPT_LOCAL_RELABEL, REQ_IMAGE, REQ_POLYLINE, REQ_STROKE, GcmiError,
                       Request, _i64p, check, host_ptr, lib)
 
 QUEUED, DONE, FAILED = 0, 1, 2
@@ -52,7 +52,7 @@ class RequestInfo:
 
 class GraphCutServer:
     def __init__(self, tile=32, hbm_budget=200 * 10**9, max_sessions=256, max_batch=64,
-                 mode="batched", sweeps=0, bfs=0, local_relabel=False):
+                 mode="batched", sweeps=0, bfs=0, local_relabel=False, device_rounds=0):
         self.tile = int(tile[0] if isinstance(tile, (tuple, list)) else tile)
         if mode not in MODES:
             raise ValueError("mode must be one of %s" % sorted(MODES))
@@ -66,6 +66,8 @@ class GraphCutServer:
             check(lib().gcmi_server_set_params(self._h, int(sweeps), int(bfs)))
         if local_relabel:
             check(lib().gcmi_server_set_option(self._h, OPT_LOCAL_RELABEL, 1))
+        if device_rounds:
+            check(lib().gcmi_server_set_option(self._h, OPT_DEVICE_ROUNDS, int(device_rounds)))
 
     def close_server(self):
         if getattr(self, "_h", None):

@@ -36,8 +36,20 @@ enum { M_UP = 1, M_DOWN = 2, M_LEFT = 4, M_RIGHT = 8, M_SINK = 16, M_EXCESS = 32
 // max(e0, 0); sum of max(e, 0) after the last relabel.
 constexpr int kSumSlots = 64;
 enum { SUM_MIN = 0, SUM_POS0 = 1, SUM_POSF = 2, SUM_ROWS = 3 };
-// Pool-wide tallies of tile passes (added by block 0 of each round launch).
-enum { TALLY_BFS = 0, TALLY_FLOOD = 1, TALLY_N = 2 };
+// Pool-wide tallies (added by block 0 of each launch): tiles processed by later BFS / flood passes, and their in-tile relaxation
+// iterations; push launches with work and their tiles; relabel first passes
+// with work and their tiles
+enum {
+  TALLY_BFS = 0,
+  TALLY_FLOOD = 1,
+  TALLY_BFS_ITERS = 2,
+  TALLY_FLOOD_ITERS = 3,
+  TALLY_PUSH_TILES = 4,
+  TALLY_PUSH_ROUNDS = 5,
+  TALLY_RELABELS = 6,
+  TALLY_RELABEL_TILES = 7,
+  TALLY_N = 8
+};
 // Number of XCDs blocks are dealt to round-robin.
 constexpr int kXcds = 8;
 
@@ -273,6 +285,10 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
   const int ly = threadIdx.x / CPR, x0 = (threadIdx.x % CPR) * R;
   const int32_t MAXH = g.maxh;
   const int n = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) {
+    atomicAdd(&s.tally[TALLY_PUSH_TILES], (unsigned long long)n);
+    atomicAdd(&s.tally[TALLY_PUSH_ROUNDS], 1ull);
+  }
   const int rpos = xcd_pos(blockIdx.x, gridDim.x);
   ImgAcc amin, apos;
   for (int pos = rpos; pos < n; pos += gridDim.x) {
@@ -642,26 +658,35 @@ k_fold(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
 // reserved with one atomic per block flush.  Rounds rotate three lists: a
 // round reads list i, appends to list i+1 and zeroes the count of list
 // i+2 (read by the previous round, written by the next).
+// The epochs live on the device, so that a recorded sequence of rounds can
+// be replayed: a round reads its epoch from one slot and writes epoch + 1 to
+// the other, which the next round reads (the slot it writes was last read by
+// the previous round).
 struct Queue {
   const int32_t* in;     // input tiles (null: input positions are tiles)
   const int32_t* in_count;
   int32_t* out;
   int32_t* out_count;
   int32_t* clear_count;  // zeroed by the launch (may be null)
-  int epoch;             // stamp of the tiles appended to out
+  const int32_t* ep_in;  // this round's epoch: the stamp of tiles appended to out
+  int32_t* ep_out;       // receives epoch + 1
 };
 
 constexpr int kQCap = 256;
 struct BlockQ {
   int32_t buf[kQCap];
-  int n, base, dirs;
+  int n, base, dirs, epoch;
 };
 
 __device__ __forceinline__ void bq_init(BlockQ& b, const Queue& Q) {
   if (threadIdx.x == 0) {
     b.n = 0;
     b.dirs = 0;
-    if (blockIdx.x == 0 && Q.clear_count) *Q.clear_count = 0;
+    b.epoch = *Q.ep_in;
+    if (blockIdx.x == 0) {
+      if (Q.clear_count) *Q.clear_count = 0;
+      *Q.ep_out = b.epoch + 1;
+    }
   }
   __syncthreads();
 }
@@ -682,7 +707,7 @@ __device__ __forceinline__ void bq_take(BlockQ& b, const TG& q, int32_t* stamp, 
   if (threadIdx.x < 4 && ((b.dirs >> threadIdx.x) & 1)) {
     const int k = threadIdx.x;
     const int nb = k == D_UP ? q.t - q.tw : k == D_DOWN ? q.t + q.tw : k == D_LEFT ? q.t - 1 : q.t + 1;
-    if (atomicExch(&stamp[nb], Q.epoch) != Q.epoch) b.buf[atomicAdd(&b.n, 1)] = nb;
+    if (atomicExch(&stamp[nb], b.epoch) != b.epoch) b.buf[atomicAdd(&b.n, 1)] = nb;
   }
   __syncthreads();
   if (threadIdx.x == 0) b.dirs = 0;
@@ -743,9 +768,10 @@ __device__ __forceinline__ int row_min_scan(int d, int m, int lx) {
 // D0/D1.  Each iteration: one Jacobi step from the LDS copy (across chains
 // and into the halo), then sweeps along the thread's column segment in
 // registers and straight-run scans along every row -- the fixed point is
-// the same, with far fewer barriers.  Returns after the last barrier.
+// the same, with far fewer barriers.  Returns after the last barrier, with
+// the number of iterations.
 template <int T, int NT>
-__device__ __forceinline__ void bfs_relax(int (&d)[T * T / NT], const int (&m)[T * T / NT],
+__device__ __forceinline__ int bfs_relax(int (&d)[T * T / NT], const int (&m)[T * T / NT],
                                           int32_t* D0, int32_t* D1) {
   constexpr int R = T * T / NT;
   constexpr int HP = T + 2;
@@ -797,7 +823,7 @@ __device__ __forceinline__ void bfs_relax(int (&d)[T * T / NT], const int (&m)[T
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) Dn[(chain * R + k + 1) * HP + lx + 1] = d[k];
-    if (!__syncthreads_or(changed)) break;
+    if (!__syncthreads_or(changed)) return it + 1;
     int32_t* tmp = Dc;
     Dc = Dn;
     Dn = tmp;
@@ -830,6 +856,10 @@ k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __r
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
   if (threadIdx.x == 0) sdirs = 0;
   const int n = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) {
+    atomicAdd(&s.tally[TALLY_RELABELS], 1ull);
+    atomicAdd(&s.tally[TALLY_RELABEL_TILES], (unsigned long long)n);
+  }
   for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
     const TG q = tile_geo(g, list ? list[pos] : pos);
     const size_t base = tbase<T>(q.t);
@@ -917,7 +947,8 @@ k_bfs_round(Geo g, CutDev s, Queue Q) {
       D1[hs.idx] = v;
     }
     __syncthreads();
-    bfs_relax<T, NT>(d, m, D0, D1);
+    const int iters = bfs_relax<T, NT>(d, m, D0, D1);
+    if (threadIdx.x == 0) atomicAdd(&s.tally[TALLY_BFS_ITERS], (unsigned long long)iters);
     int any_act = 0, dirs = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -970,8 +1001,8 @@ __device__ __forceinline__ int row_reach(int r, int m, int lx) {
 // halo), then sweeps along the thread's column segment and fills along
 // every row.
 template <int T, int NT>
-__device__ __forceinline__ void flood_relax(int (&r)[T * T / NT], const int (&m)[T * T / NT],
-                                           int32_t* RM) {
+__device__ __forceinline__ int flood_relax(int (&r)[T * T / NT], const int (&m)[T * T / NT],
+                                          int32_t* RM) {
   constexpr int R = T * T / NT;
   constexpr int HP = T + 2;
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
@@ -1022,7 +1053,7 @@ __device__ __forceinline__ void flood_relax(int (&r)[T * T / NT], const int (&m)
       const int c = (ly + 1) * HP + lx + 1;
       if (r[k]) RM[c] |= 1;
     }
-    if (!__syncthreads_or(changed)) break;
+    if (!__syncthreads_or(changed)) return it + 1;
   }
 }
 
@@ -1068,7 +1099,9 @@ k_flood(Geo g, CutDev s, Queue Q, int first) {
       RM[hs.idx] = v;
     }
     __syncthreads();
-    flood_relax<T, NT>(r, m, RM);
+    const int iters = flood_relax<T, NT>(r, m, RM);
+    if (threadIdx.x == 0 && !first)
+      atomicAdd(&s.tally[TALLY_FLOOD_ITERS], (unsigned long long)iters);
     int dirs = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {

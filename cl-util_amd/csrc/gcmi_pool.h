@@ -19,6 +19,8 @@ constexpr int kRoundBatch = 2;     // BFS / flood rounds before the first host c
 constexpr int kRoundBatchMax = 8;  // ... and before each later one
 constexpr int kRelabelGridCap = 65536;  // blocks of a relabel's first pass
 constexpr int kLocalRelabels = 6;       // local relabels per solve, then full ones
+constexpr int kDeviceRoundsMax = 64;     // BFS / flood rounds per relabel, device-driven
+constexpr int kDeviceRoundsDefault = 8;
 
 enum {
   CNT_ACT = 0,
@@ -27,7 +29,10 @@ enum {
   CNT_FRESH = 3,
   CNT_Q = 4,  // counts of the three rotating tile queues
   CNT_DIRTY = 7,
-  CNT_N = 8
+  CNT_EP = 8,     // two alternating queue-epoch slots
+  CNT_DONE = 10,  // a complete relabel found no active tile
+  CNT_GSEL = 11,  // selected tiles, or 0 once the solve is done
+  CNT_N = 12
 };
 
 enum {
@@ -145,7 +150,20 @@ struct Pool {
   // relabels after kLocalRelabels.  Exact either way: termination is
   // decided by a relabel over a valid labelling.
   bool local_relabel = false;
-  int epoch = 0;         // last queue epoch handed out (stamps hold epochs)
+  bool trace = false;  // GCMI_TRACE=1: per-phase lines on stderr, one round per host check
+  int ep_par = 0;         // epoch slot the next queue round reads
+  bool ep_reset = false;  // restart stamps and epochs before the next solve
+  // Device-driven solves (dd_rounds > 0): every launch of a fixed shape,
+  // dd_rounds BFS / flood rounds per relabel, gates on the device instead of
+  // host checks, one synchronisation at the end (the host loop finishes a
+  // solve the sequence did not).  graph: record that sequence for an image
+  // list once as a hipGraph and replay it while the list and parameters
+  // stay the same.
+  int dd_rounds = 0;
+  bool graph = false;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<int> graph_key;
+  int graph_flood_cur = 0, graph_ep_par = 0;
   // the lists the kernels of the current solve read: null when the
   // selection is the whole pool in tile order (positions are tiles)
   int32_t *lsel = nullptr, *lfresh = nullptr;

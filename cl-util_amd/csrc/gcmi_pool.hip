@@ -115,10 +115,10 @@ __global__ void k_dilate(Geo g, CutDev s, const int32_t* __restrict__ list,
 }
 
 // Builds the solve's tile lists, zeroes the sums it recomputes (all rows of
-// a fresh image, the final-excess row of every image) and the tallies.
+// a fresh image, the final-excess row of every image) and the tallies, sets
+// the counts and the first queue epoch.
 __global__ void k_prepare(Geo g, CutDev s, const Prep* __restrict__ prep, int nsel_tiles,
-                          int nfresh_tiles, int32_t* sel, int32_t* fresh, int32_t* cnt_sel,
-                          int32_t* cnt_fresh) {
+                          int nfresh_tiles, int32_t* sel, int32_t* fresh, int32_t* counts) {
   const Prep p = prep[blockIdx.x];
   const ImgDesc d = g.img[p.img];
   for (int j = threadIdx.x; j < d.ntiles; j += blockDim.x) {
@@ -129,10 +129,21 @@ __global__ void k_prepare(Geo g, CutDev s, const Prep* __restrict__ prep, int ns
     if (p.fresh_off >= 0 || j / kSumSlots == SUM_POSF)
       s.sums[(size_t)p.img * SUM_ROWS * kSumSlots + j] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *cnt_sel = nsel_tiles;
-    *cnt_fresh = nfresh_tiles;
+    counts[CNT_SEL] = nsel_tiles;
+    counts[CNT_FRESH] = nfresh_tiles;
+    counts[CNT_GSEL] = nsel_tiles;
+    counts[CNT_DONE] = 0;
+    // the next epoch: above every stamp written so far (stamps start at 0)
+    counts[CNT_EP] = max(max(counts[CNT_EP], counts[CNT_EP + 1]), 1);
     for (int k = 0; k < TALLY_N; ++k) s.tally[k] = 0;
   }
+}
+
+// Zeroes the final-excess sums of the selected images: the first flood
+// pass adds them up (a run whose recording stopped short floods again).
+__global__ void k_zero_posf(unsigned long long* sums, const Prep* __restrict__ prep) {
+  unsigned long long* r = sums + (size_t)prep[blockIdx.x].img * SUM_ROWS * kSumSlots;
+  for (int j = threadIdx.x; j < kSumSlots; j += blockDim.x) r[SUM_POSF * kSumSlots + j] = 0;
 }
 
 // Flow value of each selected image: sum min(cs, ct) + sum max(e0, 0)
@@ -344,6 +355,10 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   GCMI_HIP(hipGetDevice(&dev));
   GCMI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   p->grid_cap = ncu > 0 ? ncu * 8 : 2048;
+  {
+    const char* tr = getenv("GCMI_TRACE");
+    p->trace = tr && tr[0] == '1';
+  }
   size_t tot = 0;
   int rc = GCMI_OK;
   CutDev& d = p->dev;
@@ -418,6 +433,7 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   int32_t* flags[] = {d.pflag, d.act, d.stamp, d.bflag, d.dirty, d.rflag, d.haspos};
   for (int32_t* f : flags) GCMI_HIP(hipMemsetAsync(f, 0, (size_t)cap_tiles * 4, p->stream));
   GCMI_HIP(hipMemsetAsync(d.tpos, 0, (size_t)cap_tiles * 8, p->stream));
+  GCMI_HIP(hipMemsetAsync(p->counts, 0, CNT_N * 4, p->stream));
   GCMI_HIP(hipStreamSynchronize(p->stream));
   return GCMI_OK;
 }
@@ -433,6 +449,8 @@ void pool_free(Pool* p) {
                   p->d_flows,
                   p->d_edit};
   if (p->stream) (void)hipStreamSynchronize(p->stream);
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  p->graph_exec = nullptr;
   for (void* x : ptrs)
     if (x) (void)hipFree(x);
   if (p->d_bits) (void)hipFree(p->d_bits);
@@ -752,28 +770,374 @@ static void clear_queues(Pool* p) {
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, p->stream, p->counts + CNT_Q, 3LL, 0);
 }
 
-// A fresh queue epoch; on wrap-around every stamp is reset first.
-static int next_epoch(Pool* p) {
-  if (p->epoch >= (1 << 30)) {
-    hipLaunchKernelGGL(k_fill_i32, dim3((p->cap_tiles + 255) / 256), dim3(256), 0, p->stream,
-                       p->dev.stamp, (long long)p->cap_tiles, 0);
-    p->epoch = 0;
-  }
-  return ++p->epoch;
-}
-
-// The queue of a BFS or flood round reading rotating list `cur`.
+// The queue of a BFS or flood round reading rotating list `cur`; the epoch
+// slots alternate with every round.
 static Queue round_queue(Pool* p, int cur) {
   const int nxt = (cur + 1) % 3, clr = (cur + 2) % 3;
   const size_t cap = (size_t)p->cap_tiles;
-  return Queue{p->q + cur * cap, p->counts + CNT_Q + cur, p->q + nxt * cap,
-               p->counts + CNT_Q + nxt, p->counts + CNT_Q + clr, next_epoch(p)};
+  const Queue q{p->q + cur * cap,          p->counts + CNT_Q + cur,
+                p->q + nxt * cap,          p->counts + CNT_Q + nxt,
+                p->counts + CNT_Q + clr,   p->counts + CNT_EP + p->ep_par,
+                p->counts + CNT_EP + 1 - p->ep_par};
+  p->ep_par ^= 1;
+  return q;
 }
 
 static int read_counts(Pool* p) {
   GCMI_HIP(hipMemcpyAsync(p->h_counts, p->counts, CNT_N * 4, hipMemcpyDeviceToHost, p->stream));
   GCMI_HIP(hipStreamSynchronize(p->stream));
   p->stats[STAT_SYNCS]++;
+  // stamps hold epochs: restart both before the counter nears overflow
+  if (std::max(p->h_counts[CNT_EP], p->h_counts[CNT_EP + 1]) >= (1 << 30)) p->ep_reset = true;
+  return GCMI_OK;
+}
+
+// Sets the solve's done flag once a complete relabel (its queue, list
+// `qidx`, drained) found no active tile; a done solve zeroes the active and
+// relabel counts, so every later push and relabel launch of a recorded solve
+// exits at once.
+__global__ void k_gate(int32_t* c, int qidx) {
+  if (threadIdx.x == 0) {
+    if (c[CNT_Q + qidx] == 0 && c[CNT_ACT] == 0) c[CNT_DONE] = 1;
+    if (c[CNT_DONE]) {
+      c[CNT_ACT] = 0;
+      c[CNT_GSEL] = 0;
+    }
+  }
+}
+
+namespace {
+
+// One solve's fixed facts, shared by its phases.
+struct Solve {
+  Pool* p;
+  const PoolOps* ops;
+  int n, nsel, nfresh, sweeps, bfs;
+  int grid_relabel;
+  bool dd;  // device-driven: fixed launch shapes, no host reads (recordable)
+  Prof* P;
+  void B(int k) const {
+    if (P) P->begin(p->stream, k);
+  }
+  void E() const {
+    if (P) P->end(p->stream);
+  }
+  int grid_for(int m) const {
+    return dd ? p->grid_cap : (m < p->grid_cap ? (m > 0 ? m : 1) : p->grid_cap);
+  }
+  int grid_clear() const { return std::min((nsel + 255) / 256, 4096); }
+};
+
+// Start of every solve: the image lists, sums, tallies and counts.
+int phase_prepare(const Solve& S) {
+  Pool* p = S.p;
+  GCMI_HIP(hipMemcpyAsync(p->d_prep, p->h_prep, (size_t)S.n * sizeof(Prep),
+                          hipMemcpyHostToDevice, p->stream));
+  hipLaunchKernelGGL(k_prepare, dim3(S.n), dim3(256), 0, p->stream, p->geo, p->dev, p->d_prep,
+                     S.nsel, S.nfresh, p->sel, p->fresh, p->counts);
+  GCMI_HIP(hipGetLastError());
+  p->ep_par = 0;
+  return GCMI_OK;
+}
+
+// The first push round: every tile of the fresh images, then the tiles the
+// edits of the warm images made active.
+int phase_first(const Solve& S, int32_t* nact) {
+  Pool* p = S.p;
+  if (S.nfresh > 0) {
+    S.ops->reset_tiles(p, std::min(S.nfresh, 4096));
+    S.B(K_PUSH_FIRST);
+    S.ops->push(p, S.nfresh, S.sweeps, true);
+    S.E();
+    GCMI_HIP(hipGetLastError());
+  }
+  if (S.nsel > S.nfresh) {
+    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, true, S.nsel));
+    if (!S.dd) {
+      GCMI_TRY(read_counts(p));
+      *nact = p->h_counts[CNT_ACT];
+      if (p->trace) fprintf(stderr, "gcmi: warm pre-push over %d tiles\n", *nact);
+    }
+    if (S.dd || *nact > 0) {
+      S.B(K_PUSH);
+      S.ops->push(p, S.grid_for(*nact), S.sweeps, false);
+      S.E();
+      GCMI_HIP(hipGetLastError());
+    }
+  }
+  return GCMI_OK;
+}
+
+// A relabel and its BFS rounds; returns the rotating list the next round
+// would read in *cur.  Host-driven: rounds run until the queue drains, the
+// active list is compacted and its count read (*nact).  Device-driven: a
+// fixed number of rounds, then the compaction and the done gate.
+int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32_t* nact) {
+  Pool* p = S.p;
+  const int32_t* glist = p->lsel;
+  int gcnt = S.dd ? CNT_GSEL : CNT_SEL;
+  if (local) {
+    hipLaunchKernelGGL(k_dilate, dim3(S.grid_clear()), dim3(256), 0, p->stream, p->geo, p->dev,
+                       p->lsel, p->counts + CNT_SEL);
+    GCMI_TRY(compact_sel(p, p->dev.rflag, p->dlist, CNT_DIRTY, false, S.nsel));
+    glist = p->dlist;
+    gcnt = CNT_DIRTY;
+  }
+  S.B(K_FOLD);
+  S.ops->fold(p, S.grid_relabel, glist, gcnt);
+  S.E();
+  S.B(K_BFS_INIT);
+  S.ops->bfs_init(p, S.grid_relabel, glist, gcnt, !local);
+  S.E();
+  GCMI_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_clear_listed, dim3(S.grid_clear()), dim3(256), 0, p->stream, p->dev.dirty,
+                     glist, p->counts + gcnt);
+  if (local)
+    hipLaunchKernelGGL(k_clear_listed, dim3(S.grid_clear()), dim3(256), 0, p->stream,
+                       p->dev.rflag, glist, p->counts + gcnt);
+  // the flagged tiles start the first queue
+  clear_queues(p);
+  GCMI_TRY(compact_sel(p, p->dev.bflag, p->q, CNT_Q, true, S.nsel));
+  int cur = 0;
+  const int32_t* hc = p->h_counts;
+  if (S.dd) {
+    for (int b = 0; b < rounds_dd; ++b) {
+      S.ops->bfs_round(p, p->grid_cap, round_queue(p, cur));
+      cur = (cur + 1) % 3;
+    }
+    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, false, S.nsel));
+    hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, p->stream, p->counts, cur);
+    GCMI_HIP(hipGetLastError());
+    *cur_out = cur;
+    return GCMI_OK;
+  }
+  if (p->trace) {
+    GCMI_TRY(read_counts(p));
+    fprintf(stderr, "gcmi: relabel %s over %d tiles, first queue %d\n", local ? "local" : "full",
+            local ? hc[CNT_DIRTY] : S.nsel, hc[CNT_Q]);
+  }
+  // later BFS rounds, several per host check (a round with an empty input
+  // queue exits at once); the active list rides along with the check
+  for (int batch = p->trace ? 1 : kRoundBatch;; batch = p->trace ? 1 : kRoundBatchMax) {
+    for (int b = 0; b < batch; ++b) {
+      S.B(K_BFS_ROUND);
+      S.ops->bfs_round(p, p->grid_cap, round_queue(p, cur));
+      S.E();
+      GCMI_HIP(hipGetLastError());
+      cur = (cur + 1) % 3;
+    }
+    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, false, S.nsel));
+    if (p->trace)
+      GCMI_HIP(hipMemcpyAsync(p->h_tally, p->dev.tally, TALLY_N * 8, hipMemcpyDeviceToHost,
+                              p->stream));
+    GCMI_TRY(read_counts(p));
+    if (p->trace)
+      fprintf(stderr, "gcmi:   bfs round -> queue %d (tiles so far %llu, iterations %llu)\n",
+              hc[CNT_Q + cur], p->h_tally[TALLY_BFS], p->h_tally[TALLY_BFS_ITERS]);
+    if (hc[CNT_Q + cur] == 0) break;
+  }
+  *nact = hc[CNT_ACT];
+  *cur_out = cur;
+  return GCMI_OK;
+}
+
+// A push round over the active list built (without clearing its flags) by
+// the last relabel.
+int phase_push(const Solve& S, int32_t nact) {
+  Pool* p = S.p;
+  hipLaunchKernelGGL(k_clear_listed, dim3(S.dd ? S.grid_clear() : (nact + 255) / 256), dim3(256),
+                     0, p->stream, p->dev.act, p->alist, p->counts + CNT_ACT);
+  S.B(K_PUSH);
+  S.ops->push(p, S.grid_for(nact), S.sweeps, false);
+  S.E();
+  GCMI_HIP(hipGetLastError());
+  return GCMI_OK;
+}
+
+// Further push rounds before a relabel (bfs > 1).  Host-driven: stops early
+// (returning false) when no tile is active.
+int phase_extra_pushes(const Solve& S, bool* any) {
+  Pool* p = S.p;
+  *any = true;
+  for (int r = 1; r < S.bfs; ++r) {
+    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, true, S.nsel));
+    int32_t nact = 0;
+    if (!S.dd) {
+      GCMI_TRY(read_counts(p));
+      nact = p->h_counts[CNT_ACT];
+      if (nact == 0) {
+        *any = false;
+        return GCMI_OK;
+      }
+    }
+    S.B(K_PUSH);
+    S.ops->push(p, S.grid_for(nact), S.sweeps, false);
+    S.E();
+    GCMI_HIP(hipGetLastError());
+  }
+  return GCMI_OK;
+}
+
+// Minimal source set: the first flood pass from the tiles holding excess,
+// then rounds (device-driven: a fixed number), labels and flows.  Returns
+// the list the next round would read in *cur_out.
+int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
+  Pool* p = S.p;
+  // haspos persists: a clean tile keeps the flag its last relabel wrote
+  GCMI_TRY(compact_sel(p, p->dev.haspos, p->plist, CNT_POS, false, S.nsel));
+  S.ops->clear_reach(p, std::min(S.nsel, 4096));
+  hipLaunchKernelGGL(k_zero_posf, dim3(S.n), dim3(64), 0, p->stream, p->dev.sums, p->d_prep);
+  clear_queues(p);
+  S.B(K_FLOOD);
+  const Queue first{p->plist,
+                    p->counts + CNT_POS,
+                    p->q,
+                    p->counts + CNT_Q,
+                    nullptr,
+                    p->counts + CNT_EP + p->ep_par,
+                    p->counts + CNT_EP + 1 - p->ep_par};
+  p->ep_par ^= 1;
+  S.ops->flood(p, p->grid_cap, first, true);
+  S.E();
+  GCMI_HIP(hipGetLastError());
+  int cur = 0;
+  for (int b = 0; b < rounds_dd; ++b) {
+    S.B(K_FLOOD);
+    S.ops->flood(p, p->grid_cap, round_queue(p, cur), false);
+    S.E();
+    GCMI_HIP(hipGetLastError());
+    cur = (cur + 1) % 3;
+  }
+  *cur_out = cur;
+  return GCMI_OK;
+}
+
+// Labels, flows and tallies to the host buffers (no synchronisation).
+int phase_results(const Solve& S) {
+  Pool* p = S.p;
+  S.B(K_LABELS);
+  S.ops->labels(p, S.grid_for(S.nsel));
+  S.E();
+  GCMI_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_flows, dim3(S.n), dim3(64), 0, p->stream, p->dev.sums, p->d_prep,
+                     p->d_flows);
+  GCMI_HIP(hipGetLastError());
+  GCMI_HIP(hipMemcpyAsync(p->h_flows, p->d_flows, (size_t)S.n * 8, hipMemcpyDeviceToHost,
+                          p->stream));
+  GCMI_HIP(hipMemcpyAsync(p->h_tally, p->dev.tally, TALLY_N * 8, hipMemcpyDeviceToHost,
+                          p->stream));
+  return GCMI_OK;
+}
+
+// Host-driven flood rounds from list `cur`, each batch closed by labels,
+// flows and a count check.
+int flood_rounds_host(const Solve& S, int cur) {
+  Pool* p = S.p;
+  const int32_t* hc = p->h_counts;
+  for (int batch = p->trace ? 1 : kRoundBatch;; batch = p->trace ? 1 : kRoundBatchMax) {
+    for (int b = 0; b < batch; ++b) {
+      S.B(K_FLOOD);
+      S.ops->flood(p, p->grid_cap, round_queue(p, cur), false);
+      S.E();
+      GCMI_HIP(hipGetLastError());
+      cur = (cur + 1) % 3;
+    }
+    // labels and flows ride along with the check
+    GCMI_TRY(phase_results(S));
+    GCMI_TRY(read_counts(p));
+    if (p->trace)
+      fprintf(stderr, "gcmi:   flood round -> queue %d (tiles so far %llu, iterations %llu)\n",
+              hc[CNT_Q + cur], p->h_tally[TALLY_FLOOD], p->h_tally[TALLY_FLOOD_ITERS]);
+    if (hc[CNT_Q + cur] == 0) break;
+  }
+  return GCMI_OK;
+}
+
+// Push/relabel iterations of a device-driven solve (the host loop finishes
+// a solve that needs more): runs from scratch mostly finish after the first
+// relabel or the second, warm reflows after the second or the third.
+int dd_iters(const Solve& S) { return S.nfresh == S.nsel ? 2 : 3; }
+
+// The device-driven solve (the sequence a recording holds): fixed launch
+// shapes, `rounds` BFS and flood rounds per relabel, no host reads.
+// *flood_cur: the flood list the next round would read.
+int emit_device_driven(const Solve& S, int rounds, int* flood_cur) {
+  int32_t nact = 0;
+  GCMI_TRY(phase_prepare(S));
+  GCMI_TRY(phase_first(S, &nact));
+  for (int it = 0; it < dd_iters(S); ++it) {
+    if (it > 0) {
+      bool any;
+      GCMI_TRY(phase_extra_pushes(S, &any));
+    }
+    // local relabels as in the host loop (their dirty lists are empty, so
+    // they are gated too, once the solve is done)
+    const bool local = S.p->local_relabel && S.nfresh == 0 && it < kLocalRelabels;
+    int cur;
+    GCMI_TRY(phase_relabel(S, local, rounds, &cur, &nact));
+    GCMI_TRY(phase_push(S, 0));
+  }
+  GCMI_TRY(phase_flood_start(S, rounds, flood_cur));
+  return phase_results(S);
+}
+
+}  // namespace
+
+// Records the device-driven solve of the current image list (or reuses the
+// recording of the same list and parameters) and replays it.  The caller
+// reads the counts and finishes the solve if the recording did not.
+static int run_recorded(Pool* p, const Solve& S, int* flood_cur) {
+  std::vector<int> key;
+  key.reserve(4 * S.n + 8);
+  key.push_back(S.n);
+  key.push_back(S.nsel);
+  key.push_back(S.nfresh);
+  for (int i = 0; i < S.n; ++i) {
+    const PoolImage& im = p->imgs[p->h_prep[i].img];
+    key.push_back(p->h_prep[i].img);
+    key.push_back(p->h_prep[i].fresh_off);
+    key.push_back(im.tile0);
+    key.push_back(im.ntiles);
+  }
+  key.push_back(S.sweeps);
+  key.push_back(S.bfs);
+  key.push_back(p->dd_rounds);
+  key.push_back(p->lsel == nullptr);
+  key.push_back(p->lfresh == nullptr);
+  key.push_back(p->local_relabel);
+  if (!p->graph_exec || key != p->graph_key) {
+    if (p->graph_exec) {
+      (void)hipGraphExecDestroy(p->graph_exec);
+      p->graph_exec = nullptr;
+    }
+    GCMI_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeRelaxed));
+    int cur = 0;
+    const int rc = emit_device_driven(S, p->dd_rounds, &cur);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(p->stream, &graph);
+    if (rc != GCMI_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      set_error("solve recording failed: %s", hipGetErrorString(e));
+      return GCMI_EHIP;
+    }
+    const hipError_t ei = hipGraphInstantiate(&p->graph_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      p->graph_exec = nullptr;
+      set_error("solve recording could not be instantiated: %s", hipGetErrorString(ei));
+      return GCMI_EHIP;
+    }
+    p->graph_key = key;
+    p->graph_flood_cur = cur;
+    p->graph_ep_par = p->ep_par;
+  }
+  GCMI_HIP(hipGraphLaunch(p->graph_exec, p->stream));
+  // the epoch slot a round after the recording reads
+  p->ep_par = p->graph_ep_par;
+  *flood_cur = p->graph_flood_cur;
   return GCMI_OK;
 }
 
@@ -785,7 +1149,6 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
   }
   if (sweeps <= 0) sweeps = kDefaultSweeps;
   if (bfs <= 0) bfs = kDefaultBfs;
-  const PoolOps* ops = p->ops;
   const int T = p->T;
   int nsel = 0, nfresh = 0;
   for (int i = 0; i < n; ++i) {
@@ -815,176 +1178,75 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
   p->lsel = ident ? nullptr : p->sel;
   p->lfresh = (ident && nfresh == nsel) ? nullptr : p->fresh;
   Prof* P = p->prof.on ? &p->prof : nullptr;
-  auto B = [&](int k) {
-    if (P) P->begin(p->stream, k);
-  };
-  auto E = [&]() {
-    if (P) P->end(p->stream);
-  };
   if (P) P->reset();
   for (int k = 0; k < STAT_N; ++k) p->stats[k] = 0;
   p->stats[STAT_TILES] = nsel;
-  const int grid_cap = p->grid_cap;
-  auto grid_for = [&](int m) { return m < grid_cap ? (m > 0 ? m : 1) : grid_cap; };
-  // the relabel's first pass runs one block per dirty tile (their number is
-  // known on the device only; surplus blocks exit at once)
-  const int grid_relabel = std::min(nsel, kRelabelGridCap);
-  int32_t* hc = p->h_counts;
-
-  GCMI_HIP(hipMemcpyAsync(p->d_prep, p->h_prep, (size_t)n * sizeof(Prep), hipMemcpyHostToDevice,
-                          p->stream));
-  hipLaunchKernelGGL(k_prepare, dim3(n), dim3(256), 0, p->stream, p->geo, p->dev, p->d_prep, nsel,
-                     nfresh, p->sel, p->fresh, p->counts + CNT_SEL, p->counts + CNT_FRESH);
-  GCMI_HIP(hipGetLastError());
-
-  int64_t rounds = 0;
-  if (nfresh > 0) {
-    ops->reset_tiles(p, std::min(nfresh, 4096));
-    B(K_PUSH_FIRST);
-    ops->push(p, nfresh, sweeps, true);
-    E();
+  if (p->ep_reset) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(std::min((p->cap_tiles + 255) / 256, 4096)), dim3(256), 0,
+                       p->stream, p->dev.stamp, (long long)p->cap_tiles, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, p->stream, p->counts + CNT_EP, 2LL, 0);
     GCMI_HIP(hipGetLastError());
-    rounds = 1;
-    p->stats[STAT_RESIDENCIES] += nfresh;
+    p->ep_reset = false;
   }
+  // the relabel's first pass runs one block per listed tile (their number
+  // is known on the device only; surplus blocks exit at once)
+  Solve S{p, p->ops, n, nsel, nfresh, sweeps, bfs, std::min(nsel, kRelabelGridCap), false, P};
+  const int32_t* hc = p->h_counts;
   int32_t nact = 0;
-  int nlocal = 0;  // local relabels so far
-  if (nsel > nfresh) {
-    // edited images: a push round over the tiles the edits made active
-    // (with any the first round flagged) before the first relabel
-    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, true, nsel));
+  int64_t rounds = 0;
+  bool solved = false, flooded = false;
+  int cur = 0;
+  const bool dd_run = p->dd_rounds > 0 && !P && !p->trace;
+  if (dd_run) {
+    Solve R = S;
+    R.dd = true;
+    // gated launches of a done solve exit at once: one block per CU slot
+    R.grid_relabel = std::min(nsel, p->grid_cap);
+    if (p->graph)
+      GCMI_TRY(run_recorded(p, R, &cur));
+    else
+      GCMI_TRY(emit_device_driven(R, p->dd_rounds, &cur));
     GCMI_TRY(read_counts(p));
-    nact = hc[CNT_ACT];
-    if (nact > 0) {
-      B(K_PUSH);
-      ops->push(p, grid_for(nact), sweeps, false);
-      E();
-      GCMI_HIP(hipGetLastError());
-      ++rounds;
-      p->stats[STAT_RESIDENCIES] += nact;
+    solved = hc[CNT_DONE] != 0;
+    flooded = solved && hc[CNT_Q + cur] == 0;
+    rounds = dd_iters(R) + 1;
+    if (!solved) {
+      // go on from the state after the recording's last push round
+      nact = 1;
     }
+  } else {
+    GCMI_TRY(phase_prepare(S));
+    GCMI_TRY(phase_first(S, &nact));
+    rounds = (nfresh > 0) + (nsel > nfresh && nact > 0);
   }
-  for (;;) {
-    // further push rounds before the global relabel
-    for (int r = 1; r < bfs && rounds > 0; ++r) {
-      GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, true, nsel));
-      GCMI_TRY(read_counts(p));
-      nact = hc[CNT_ACT];
-      if (nact == 0) break;
-      if (max_rounds > 0 && rounds >= max_rounds) {
-        set_error("cut did not terminate within %lld push rounds", (long long)max_rounds);
-        return GCMI_ENOTCONVERGED;
-      }
-      B(K_PUSH);
-      ops->push(p, grid_for(nact), sweeps, false);
-      E();
-      GCMI_HIP(hipGetLastError());
-      ++rounds;
-      p->stats[STAT_RESIDENCIES] += nact;
-    }
-    // fold pending flow into the dirty tiles (every tile a push changed or
-    // sent flow to), recompute their distances, and let the rounds carry the
-    // changes outward: a global relabel, restricted to where it can differ
-    // (or, by default, over every selected tile)
-    // (or over every selected tile: always when an image is cut from
-    // scratch, and after kLocalRelabels local ones, which bounds the cost of
-    // the stale distances a local relabel leaves in clean tiles)
-    const int32_t* glist = p->lsel;
-    int gcnt = CNT_SEL;
+  int nlocal = dd_run ? dd_iters(S) : 0;  // relabels that may have been local so far
+  while (!solved) {
+    bool any = true;
+    if (rounds > 0) GCMI_TRY(phase_extra_pushes(S, &any));
+    // a global relabel: over every selected tile, or (local) only over the
+    // tiles that changed and a ring around them -- only in solves of edited
+    // images, and for kLocalRelabels relabels at most, which bounds the
+    // cost of the stale distances a local relabel leaves in clean tiles
     const bool local = p->local_relabel && nfresh == 0 && nlocal < kLocalRelabels;
-    if (local) {
-      hipLaunchKernelGGL(k_dilate, dim3(std::min((nsel + 255) / 256, 4096)), dim3(256), 0,
-                         p->stream, p->geo, p->dev, p->lsel, p->counts + CNT_SEL);
-      GCMI_TRY(compact_sel(p, p->dev.rflag, p->dlist, CNT_DIRTY, false, nsel));
-      glist = p->dlist;
-      gcnt = CNT_DIRTY;
-      ++nlocal;
-    }
-    B(K_FOLD);
-    ops->fold(p, grid_relabel, glist, gcnt);
-    E();
-    B(K_BFS_INIT);
-    ops->bfs_init(p, grid_relabel, glist, gcnt, !local);
-    E();
-    GCMI_HIP(hipGetLastError());
-    const int gclear = std::min((nsel + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_clear_listed, dim3(gclear), dim3(256), 0, p->stream, p->dev.dirty, glist,
-                       p->counts + gcnt);
-    if (local)
-      hipLaunchKernelGGL(k_clear_listed, dim3(gclear), dim3(256), 0, p->stream, p->dev.rflag,
-                         glist, p->counts + gcnt);
-    p->stats[STAT_GLOBAL_RELABELS]++;
-    // the flagged tiles start the first queue
-    clear_queues(p);
-    GCMI_TRY(compact_sel(p, p->dev.bflag, p->q, CNT_Q, true, nsel));
-    int cur = 0;
-    // later BFS rounds, several per host check (a round with an empty input
-    // queue exits at once); the active list rides along with the check
-    for (int batch = kRoundBatch;; batch = kRoundBatchMax) {
-      for (int b = 0; b < batch; ++b) {
-        B(K_BFS_ROUND);
-        ops->bfs_round(p, grid_cap, round_queue(p, cur));
-        E();
-        GCMI_HIP(hipGetLastError());
-        cur = (cur + 1) % 3;
-      }
-      GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, false, nsel));
-      GCMI_TRY(read_counts(p));
-      if (hc[CNT_Q + cur] == 0) break;
-    }
-    nact = hc[CNT_ACT];
+    if (local) ++nlocal;
+    GCMI_TRY(phase_relabel(S, local, 0, &cur, &nact));
+    if (p->trace) fprintf(stderr, "gcmi: push round over %d active tiles\n", nact);
     if (nact == 0) break;
-    // the active list was built without clearing the flags (more BFS rounds
-    // could still have added tiles); clear them for the push kernel
-    hipLaunchKernelGGL(k_clear_listed, dim3((nact + 255) / 256), dim3(256), 0, p->stream,
-                       p->dev.act, p->alist, p->counts + CNT_ACT);
     if (max_rounds > 0 && rounds >= max_rounds) {
       set_error("cut did not terminate within %lld push rounds", (long long)max_rounds);
       return GCMI_ENOTCONVERGED;
     }
-    B(K_PUSH);
-    ops->push(p, grid_for(nact), sweeps, false);
-    E();
-    GCMI_HIP(hipGetLastError());
+    GCMI_TRY(phase_push(S, nact));
     ++rounds;
-    p->stats[STAT_RESIDENCIES] += nact;
   }
-  p->stats[STAT_ROUNDS] = rounds;
-
-  // minimal source set
-  // haspos persists: a clean tile keeps the flag its last relabel wrote
-  GCMI_TRY(compact_sel(p, p->dev.haspos, p->plist, CNT_POS, false, nsel));
-  ops->clear_reach(p, std::min(nsel, 4096));
-  clear_queues(p);
-  int cur = 0;
-  B(K_FLOOD);
-  ops->flood(p, grid_cap,
-             Queue{p->plist, p->counts + CNT_POS, p->q, p->counts + CNT_Q, nullptr, next_epoch(p)},
-             true);
-  E();
-  GCMI_HIP(hipGetLastError());
-  for (int batch = kRoundBatch;; batch = kRoundBatchMax) {
-    for (int b = 0; b < batch; ++b) {
-      B(K_FLOOD);
-      ops->flood(p, grid_cap, round_queue(p, cur), false);
-      E();
-      GCMI_HIP(hipGetLastError());
-      cur = (cur + 1) % 3;
+  if (!flooded) {
+    if (solved) {
+      // the recording's flood did not drain: its rounds go on
+      GCMI_TRY(flood_rounds_host(S, cur));
+    } else {
+      GCMI_TRY(phase_flood_start(S, 0, &cur));
+      GCMI_TRY(flood_rounds_host(S, cur));
     }
-    // labels and flows ride along with the check
-    B(K_LABELS);
-    ops->labels(p, grid_for(nsel));
-    E();
-    GCMI_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_flows, dim3(n), dim3(64), 0, p->stream, p->dev.sums, p->d_prep,
-                       p->d_flows);
-    GCMI_HIP(hipGetLastError());
-    GCMI_HIP(hipMemcpyAsync(p->h_flows, p->d_flows, (size_t)n * 8, hipMemcpyDeviceToHost,
-                            p->stream));
-    GCMI_HIP(hipMemcpyAsync(p->h_tally, p->dev.tally, TALLY_N * 8, hipMemcpyDeviceToHost,
-                            p->stream));
-    GCMI_TRY(read_counts(p));
-    if (hc[CNT_Q + cur] == 0) break;
   }
   if (P) P->collect();
   for (int i = 0; i < n; ++i) {
@@ -995,8 +1257,12 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
     im.solved = true;
     if (flows_out) flows_out[i] = im.flow;
   }
-  p->stats[STAT_BFS_PASSES] = p->stats[STAT_GLOBAL_RELABELS] * nsel + (int64_t)p->h_tally[TALLY_BFS];
-  p->stats[STAT_FLOOD_PASSES] = (int64_t)p->h_tally[TALLY_FLOOD];
+  const unsigned long long* ht = p->h_tally;
+  p->stats[STAT_ROUNDS] = (int64_t)ht[TALLY_PUSH_ROUNDS];
+  p->stats[STAT_RESIDENCIES] = (int64_t)ht[TALLY_PUSH_TILES];
+  p->stats[STAT_GLOBAL_RELABELS] = (int64_t)ht[TALLY_RELABELS];
+  p->stats[STAT_BFS_PASSES] = (int64_t)(ht[TALLY_RELABEL_TILES] + ht[TALLY_BFS]);
+  p->stats[STAT_FLOOD_PASSES] = (int64_t)ht[TALLY_FLOOD];
   // algorithmic bytes: first round 16 per image pixel read + 25 per padded
   // pixel written (state + mask); later residencies 49 (24 read, 25 written);
   // relabel first pass 5 (mask read, h written); later BFS passes 9 (mask +
@@ -1010,9 +1276,9 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
   const int64_t tp = (int64_t)T * T;
   p->stats[STAT_ALG_BYTES] = fresh_px * 16 + (int64_t)nfresh * tp * 25 +
                              (p->stats[STAT_RESIDENCIES] - nfresh) * tp * 49 +
-                             p->stats[STAT_GLOBAL_RELABELS] * nsel * tp * 5 +
-                             (int64_t)p->h_tally[TALLY_BFS] * tp * 9 +
-                             (int64_t)p->h_tally[TALLY_FLOOD] * tp * 2 + img_px * 2;
+                             (int64_t)ht[TALLY_RELABEL_TILES] * tp * 5 +
+                             (int64_t)ht[TALLY_BFS] * tp * 9 +
+                             (int64_t)ht[TALLY_FLOOD] * tp * 2 + img_px * 2;
   return GCMI_OK;
 }
 
@@ -1087,6 +1353,17 @@ int pool_set_option(Pool* p, int option, int value) {
   switch (option) {
     case GCMI_OPT_LOCAL_RELABEL:
       p->local_relabel = value != 0;
+      return GCMI_OK;
+    case GCMI_OPT_DEVICE_ROUNDS:
+      if (value < 0 || value > kDeviceRoundsMax) {
+        set_error("device-driven rounds must be in [0, %d] (got %d)", kDeviceRoundsMax, value);
+        return GCMI_EINVAL;
+      }
+      p->dd_rounds = value;
+      return GCMI_OK;
+    case GCMI_OPT_GRAPH:
+      p->graph = value != 0;
+      if (p->graph && p->dd_rounds == 0) p->dd_rounds = kDeviceRoundsDefault;
       return GCMI_OK;
     default:
       set_error("unknown option %d", option);

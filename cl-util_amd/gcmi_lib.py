@@ -119,6 +119,8 @@ SIGNATURES = {
 
 REQ_IMAGE, REQ_STROKE, REQ_POLYLINE = 0, 1, 2
 OPT_LOCAL_RELABEL = 1
+OPT_DEVICE_ROUNDS = 2
+OPT_GRAPH = 3
 
 
 class Request(ctypes.Structure):

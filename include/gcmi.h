@@ -202,6 +202,20 @@ int gcmi_cut_profile(gcmi_cut *cut, int enable);
  *   one ring of tiles around them) instead of every tile, for at most a few
  *   relabels per run; the result is the same exact cut either way. */
 #define GCMI_OPT_LOCAL_RELABEL 1
+/* GCMI_OPT_DEVICE_ROUNDS (default 0, at most 64): value > 0 makes runs
+ *   device-driven: two push/relabel iterations (three when edits are
+ *   reflowed) with `value` BFS rounds per relabel, then the minimal-source-
+ *   set flood with `value` rounds, every launch of a fixed shape that exits
+ *   at once when the run is already done, and one host synchronisation at
+ *   the end.  A run the sequence does not finish goes on host-driven from
+ *   where it stopped.  0: host-driven (the host reads counts between
+ *   phases).  The result is the same exact cut either way.
+ * GCMI_OPT_GRAPH (default 0): 1 records the device-driven sequence of a run
+ *   as one hipGraph and replays it for later runs of the same images and
+ *   parameters (with 8 device rounds unless set).
+ * Both are ignored while profiling. */
+#define GCMI_OPT_DEVICE_ROUNDS 2
+#define GCMI_OPT_GRAPH 3
 int gcmi_cut_set_option(gcmi_cut *cut, int option, int value);
 /* Summed milliseconds and launch counts per class for the last run. */
 int gcmi_cut_kernel_times(gcmi_cut *cut, double *ms_out, int64_t *launches_out,
