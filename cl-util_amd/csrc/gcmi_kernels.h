@@ -1270,6 +1270,11 @@ struct Polyline {
   int npts;
   int32_t cs, ct;
   int bx0, by0, bw, bh;  // pixel box covering the stroke, clipped to the image
+  int W;                 // image width
+  // rows of the image each segment's capsule can reach: segment k covers
+  // rows ya[k] .. ya[k] + (row0[k + 1] - row0[k]) - 1; nrows in all
+  int ya[kMaxStrokePoints], row0[kMaxStrokePoints];
+  int nrows;
 };
 
 // Whether pixel centre (px, py) lies within r of segment k: the point of the
@@ -1295,21 +1300,63 @@ __device__ __forceinline__ bool in_capsule(const Polyline& pl, int k, double px,
   return d2 <= pl.r * pl.r;
 }
 
-// Paints a polyline stroke into image img: one thread per pixel of the
-// stroke's box.
+// Columns of row y that segment k's capsule can reach: the capsule's points
+// in the row lie within r of segment points whose y is within r of the row,
+// so [xl, xr] is the x range of the segment's part with |y(t) - y| <= r + 1,
+// widened by r + 1 (the extra pixel absorbs rounding).  False: none.
+__device__ __forceinline__ bool seg_row_span(const Polyline& pl, int k, double y, double* xl,
+                                             double* xr) {
+  const double r1 = pl.r + 1.0;
+  const double x0 = pl.x[k], y0 = pl.y[k], x1 = pl.x[k + 1], y1 = pl.y[k + 1];
+  const double dy = y1 - y0;
+  double ta = 0.0, tb = 1.0;
+  if (dy != 0.0) {
+    double t0 = (y - r1 - y0) / dy, t1 = (y + r1 - y0) / dy;
+    if (t0 > t1) {
+      const double tmp = t0;
+      t0 = t1;
+      t1 = tmp;
+    }
+    ta = fmax(t0, 0.0);
+    tb = fmin(t1, 1.0);
+    if (ta > tb) return false;
+  } else if (fabs(y - y0) > r1) {
+    return false;
+  }
+  const double xa = x0 + ta * (x1 - x0), xb = x0 + tb * (x1 - x0);
+  *xl = fmin(xa, xb) - r1;
+  *xr = fmax(xa, xb) + r1;
+  return true;
+}
+
+// Paints a polyline stroke into image img: one wave per (segment, row) the
+// segment's capsule can reach, its lanes over the row's candidate columns.
+// A pixel is painted by the first segment whose capsule holds it, so every
+// pixel of the union is edited exactly once.
 template <int T>
 __global__ void __launch_bounds__(256)
 k_paint(Geo g, CutDev s, int img, Polyline pl, int warm) {
   __shared__ long long red[256 / kWave];
   const ImgDesc d = g.img[img];
   long long dmin = 0, dpos = 0;
-  const long long n = (long long)pl.bw * pl.bh;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    const int x = pl.bx0 + (int)(i % pl.bw), y = pl.by0 + (int)(i / pl.bw);
-    bool hit = false;
-    for (int k = 0; k + 1 < pl.npts && !hit; ++k) hit = in_capsule(pl, k, (double)x, (double)y);
-    if (hit) edit_pixel<T>(s, d, x, y, pl.cs, pl.ct, warm, dmin, dpos);
+  const int lane = threadIdx.x % kWave, waves = blockDim.x / kWave;
+  for (int item = blockIdx.x * waves + threadIdx.x / kWave; item < pl.nrows;
+       item += gridDim.x * waves) {
+    int k = 0;
+    while (k + 2 < pl.npts && item >= pl.row0[k + 1]) ++k;
+    const int y = pl.ya[k] + (item - pl.row0[k]);
+    double xl, xr;
+    if (!seg_row_span(pl, k, (double)y, &xl, &xr)) continue;
+    xl = fmax(xl, 0.0);
+    xr = fmin(xr, (double)(pl.W - 1));
+    if (xl > xr) continue;
+    const int xa = (int)floor(xl), xb = (int)ceil(xr);
+    for (int x = xa + lane; x <= xb; x += kWave) {
+      if (!in_capsule(pl, k, (double)x, (double)y)) continue;
+      bool earlier = false;
+      for (int j = 0; j < k && !earlier; ++j) earlier = in_capsule(pl, j, (double)x, (double)y);
+      if (!earlier) edit_pixel<T>(s, d, x, y, pl.cs, pl.ct, warm, dmin, dpos);
+    }
   }
   edit_sums(s, img, dmin, dpos, red);
 }

@@ -233,8 +233,9 @@ int pool_solve(Pool* p, const int* img_ids, int n, int sweeps, int bfs, int64_t 
 int pool_labels(Pool* p, int img, uint8_t* out, bool device_dst);
 // Labels packed to bits (bit p % 32 of word p / 32 = label of pixel p).
 int pool_labels_packed(Pool* p, int img, uint32_t* out);
-// Launches the packing of an image's labels into d_out (ceil(W*H/32) words).
-int pack_labels_async(Pool* p, int img, uint32_t* d_out);
+// Launches the packing of an image's labels into d_out (ceil(W*H/32) words)
+// on `stream` (null: the pool's stream).
+int pack_labels_async(Pool* p, int img, uint32_t* d_out, hipStream_t stream = nullptr);
 // Drops the solved state: the next solve starts from the capacities.
 int pool_forget(Pool* p, int img);
 // GCMI_OPT_* options of the C ABI.

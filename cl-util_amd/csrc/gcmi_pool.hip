@@ -205,8 +205,8 @@ struct PoolKernels {
                        p->dev, img, n, idx, cs, ct, p->imgs[img].fresh ? 0 : 1);
   }
   static void paint(Pool* p, int img, const Polyline& pl) {
-    const long long n = (long long)pl.bw * pl.bh;
-    const int grid = (int)std::min<long long>((n + 255) / 256, 4096);
+    // a wave per (segment, row), four per block
+    const int grid = std::min((pl.nrows + 3) / 4, 4096);
     hipLaunchKernelGGL((k_paint<T>), dim3(grid > 0 ? grid : 1), dim3(256), 0, p->stream, p->geo,
                        p->dev, img, pl, p->imgs[img].fresh ? 0 : 1);
   }
@@ -728,6 +728,27 @@ int make_polyline(int W, int H, int npts, const double* xy, double width, int32_
   pl.by0 = (int)by0;
   pl.bw = (int)(bx1 - bx0 + 1);
   pl.bh = (int)(by1 - by0 + 1);
+  pl.W = W;
+  // rows each segment's capsule can reach (as the box, per segment)
+  long long total = 0;
+  for (int k = 0; k + 1 < npts; ++k) {
+    const double ylo = std::min(pl.y[k], pl.y[k + 1]) - pl.r;
+    const double yhi = std::max(pl.y[k], pl.y[k + 1]) + pl.r;
+    const double xlo = std::min(pl.x[k], pl.x[k + 1]) - pl.r;
+    const double xhi = std::max(pl.x[k], pl.x[k + 1]) + pl.r;
+    const long long ya = std::max<long long>((long long)std::floor(ylo), 0);
+    const long long yb = std::min<long long>((long long)std::ceil(yhi), H - 1);
+    const bool cols = (long long)std::ceil(xhi) >= 0 && (long long)std::floor(xlo) <= W - 1;
+    const long long rows = (ya <= yb && cols) ? yb - ya + 1 : 0;
+    pl.ya[k] = rows > 0 ? (int)ya : 0;
+    pl.row0[k] = (int)std::min<long long>(total, INT32_MAX);
+    total += rows;
+  }
+  if (total > INT32_MAX) {
+    set_error("stroke spans too many rows");
+    return GCMI_EINVAL;
+  }
+  pl.nrows = (int)total;
   return GCMI_OK;
 }
 
@@ -1309,11 +1330,11 @@ __global__ void k_pack_bits(const uint8_t* __restrict__ labels, long long n, uin
   }
 }
 
-int pack_labels_async(Pool* p, int img, uint32_t* d_out) {
+int pack_labels_async(Pool* p, int img, uint32_t* d_out, hipStream_t stream) {
   const PoolImage& im = p->imgs[img];
   const long long n = (long long)im.W * im.H;
-  hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream,
-                     p->dev.labels + im.pix0, n, d_out);
+  hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     stream ? stream : p->stream, p->dev.labels + im.pix0, n, d_out);
   GCMI_HIP(hipGetLastError());
   return GCMI_OK;
 }

@@ -88,6 +88,12 @@ struct Session {
 
 }  // namespace
 
+// A solved batch whose masks are still being copied out.
+struct Pending {
+  std::vector<Request*> batch;
+  hipEvent_t done = nullptr;  // recorded on out_stream after the last copy
+};
+
 struct gcmi_server {
   Pool pool;
   int device = 0;
@@ -99,11 +105,20 @@ struct gcmi_server {
   int32_t* d_stage = nullptr;  // device staging for new images
   RangeAlloc stage_alloc;
   hipStream_t copy_stream = nullptr;
+  // Masks are packed and copied out on out_stream, after the solve that
+  // produced them, while the worker goes on with the next batch; the
+  // finisher thread completes a batch's requests when its copies are done.
+  hipStream_t out_stream = nullptr;
+  hipEvent_t solved = nullptr;  // the last solve, on the pool's stream
+  std::deque<Pending> outq;
+  std::vector<hipEvent_t> free_events;
+  bool stop_out = false;
+  std::thread finisher;
   // Lock order: pool_mu, then copy_mu, then mu.  The worker holds pool_mu
   // (not mu) while it serves a batch; open/close hold pool_mu while they
   // change the pool; submitters hold copy_mu while they stage an image.
   std::mutex pool_mu, copy_mu, mu;
-  std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_work, cv_done, cv_out;
   std::deque<Request*> queue;
   std::unordered_map<int64_t, Request*> reqs;
   int64_t next_id = 1;
@@ -116,9 +131,12 @@ struct gcmi_server {
 
 namespace {
 
-// Applies the batch's requests to the pool, solves, and copies the packed
-// masks out.  Runs on the worker thread with pool_mu held.
-int serve_batch(gcmi_server* s, std::vector<Request*>& batch) {
+// Applies the batch's requests to the pool, solves, and queues the packing
+// and copy-out of the masks on out_stream, recording *done after them.
+// Runs on the worker thread with pool_mu held.  The batch's sessions stay
+// busy until the finisher has seen *done, so no later solve rewrites their
+// labels before the copies have read them.
+int serve_batch(gcmi_server* s, std::vector<Request*>& batch, hipEvent_t done) {
   const int64_t t0 = now_ns();
   Pool* p = &s->pool;
   std::vector<int> imgs;
@@ -149,17 +167,21 @@ int serve_batch(gcmi_server* s, std::vector<Request*>& batch) {
   const int64_t t1 = now_ns();
   GCMI_TRY(pool_solve(p, imgs.data(), (int)imgs.size(), s->sweeps, s->bfs, 0, nullptr));
   const int64_t t2 = now_ns();
+  // (the solve has synchronised: the flows are on the host; the staging
+  // after the previous batch's copies on out_stream is free to reuse)
+  GCMI_HIP(hipEventRecord(s->solved, p->stream));
+  GCMI_HIP(hipStreamWaitEvent(s->out_stream, s->solved, 0));
   long long off = 0;
   for (Request* r : batch) {
     const Session& ss = s->sessions[r->session];
     const long long npx = (long long)ss.W * ss.H;
     const long long words = (npx + 31) / 32;
-    GCMI_TRY(pack_labels_async(p, ss.img, s->d_bits + off));
+    GCMI_TRY(pack_labels_async(p, ss.img, s->d_bits + off, s->out_stream));
     GCMI_HIP(hipMemcpyAsync(ss.bits, s->d_bits + off, words * 4, hipMemcpyDeviceToHost,
-                            p->stream));
+                            s->out_stream));
     off += words;
   }
-  GCMI_HIP(hipStreamSynchronize(p->stream));
+  GCMI_HIP(hipEventRecord(done, s->out_stream));
   for (Request* r : batch) r->flow = p->imgs[s->sessions[r->session].img].flow;
   s->ns_apply += t1 - t0;
   s->ns_solve += t2 - t1;
@@ -167,11 +189,55 @@ int serve_batch(gcmi_server* s, std::vector<Request*>& batch) {
   return GCMI_OK;
 }
 
+// Completes the requests of a batch (called with mu held).
+void complete(gcmi_server* s, std::vector<Request*>& batch, bool ok, int64_t t) {
+  for (Request* r : batch) {
+    r->t_done = t;
+    r->status = ok ? ST_DONE : ST_FAILED;
+    r->batch = (int)batch.size();
+    r->stroke.clear();
+    r->stroke.shrink_to_fit();
+    Session& ss = s->sessions[r->session];
+    ss.busy = false;
+    if (ok) ss.last_done = r->id;
+  }
+  s->iterations++;
+  s->solved_requests += (int64_t)batch.size();
+  s->cv_done.notify_all();
+  s->cv_work.notify_all();
+}
+
+void finisher_main(gcmi_server* s) {
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    s->cv_out.wait(lk, [&] { return s->stop_out || !s->outq.empty(); });
+    if (s->outq.empty()) break;  // stopping, and nothing left to finish
+    Pending pd = std::move(s->outq.front());
+    s->outq.pop_front();
+    lk.unlock();
+    const hipError_t e = hipEventSynchronize(pd.done);
+    const int64_t t = now_ns();
+    lk.lock();
+    if (e != hipSuccess)
+      snprintf(s->err, sizeof s->err, "mask copy failed: %s", hipGetErrorString(e));
+    s->free_events.push_back(pd.done);
+    complete(s, pd.batch, e == hipSuccess, t);
+  }
+}
+
+// Whether some queued request's session is free (called with mu held).
+bool work_ready(const gcmi_server* s) {
+  for (const Request* r : s->queue)
+    if (!s->sessions[r->session].busy) return true;
+  return false;
+}
+
 void worker_main(gcmi_server* s) {
   (void)hipSetDevice(s->device);
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
-    s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+    s->cv_work.wait(lk, [&] { return s->stop || work_ready(s); });
     if (s->stop) break;
     // oldest first, one request per session, at most max_batch
     const int cap = s->mode == GCMI_SERVE_PLAIN ? 1 : s->max_batch;
@@ -189,33 +255,44 @@ void worker_main(gcmi_server* s) {
     if (batch.empty()) continue;
     const int64_t t0 = now_ns();
     for (Request* r : batch) r->t_start = t0;
-    lk.unlock();
-    int rc;
-    char err[sizeof s->err] = "";
-    {
-      std::lock_guard<std::mutex> plk(s->pool_mu);
-      rc = serve_batch(s, batch);
-      if (rc != GCMI_OK) snprintf(err, sizeof err, "%s", gcmi_last_error());
+    hipEvent_t done = nullptr;
+    if (!s->free_events.empty()) {
+      done = s->free_events.back();
+      s->free_events.pop_back();
     }
-    const int64_t t1 = now_ns();
+    lk.unlock();
+    int rc = GCMI_OK;
+    char err[sizeof s->err] = "";
+    if (!done && hipEventCreateWithFlags(&done, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      done = nullptr;
+      rc = GCMI_EHIP;
+      snprintf(err, sizeof err, "event creation failed");
+    }
+    if (rc == GCMI_OK) {
+      std::lock_guard<std::mutex> plk(s->pool_mu);
+      rc = serve_batch(s, batch, done);
+      if (rc != GCMI_OK) {
+        snprintf(err, sizeof err, "%s", gcmi_last_error());
+        // copies of this batch may have been queued: let them finish
+        (void)hipStreamSynchronize(s->out_stream);
+      }
+    }
     lk.lock();
-    if (rc != GCMI_OK) memcpy(s->err, err, sizeof err);
-    for (Request* r : batch) {
-      r->t_done = t1;
-      r->status = rc == GCMI_OK ? ST_DONE : ST_FAILED;
-      r->batch = (int)batch.size();
-      r->stroke.clear();
-      r->stroke.shrink_to_fit();
+    // the images' staging ranges were read by the solve, which has finished
+    for (Request* r : batch)
       if (r->stage >= 0) {
         s->stage_alloc.release(r->stage, r->stage_len);
         r->stage = -1;
       }
-      Session& ss = s->sessions[r->session];
-      ss.busy = false;
-      if (rc == GCMI_OK) ss.last_done = r->id;
+    if (rc != GCMI_OK) {
+      memcpy(s->err, err, sizeof err);
+      if (done) s->free_events.push_back(done);
+      complete(s, batch, false, now_ns());
+    } else {
+      s->outq.push_back(Pending{std::move(batch), done});
+      s->cv_out.notify_one();
     }
-    s->iterations++;
-    s->solved_requests += (int64_t)batch.size();
     s->cv_done.notify_all();
   }
 }
@@ -288,7 +365,9 @@ int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_ba
     const long long words = cap_pix / 32 + max_sessions;
     if (hipMalloc((void**)&s->d_bits, words * 4) != hipSuccess ||
         hipMalloc((void**)&s->d_stage, stage_bytes) != hipSuccess ||
-        hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+        hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->out_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s->solved, hipEventDisableTiming) != hipSuccess) {
       (void)hipGetLastError();
       set_error("server staging allocation failed");
       rc = GCMI_ENOMEM;
@@ -297,6 +376,9 @@ int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_ba
   if (rc != GCMI_OK) {
     if (s->d_bits) (void)hipFree(s->d_bits);
     if (s->d_stage) (void)hipFree(s->d_stage);
+    if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
+    if (s->out_stream) (void)hipStreamDestroy(s->out_stream);
+    if (s->solved) (void)hipEventDestroy(s->solved);
     pool_free(&s->pool);
     delete s;
     return rc;
@@ -304,6 +386,7 @@ int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_ba
   s->stage_alloc.init((long long)(stage_bytes / 4));
   s->sessions.assign(max_sessions, Session());
   s->worker = std::thread(worker_main, s);
+  s->finisher = std::thread(finisher_main, s);
   *out = s;
   return GCMI_OK;
 }
@@ -316,9 +399,19 @@ int gcmi_server_destroy(gcmi_server* s) {
     s->cv_work.notify_all();
   }
   if (s->worker.joinable()) s->worker.join();
+  {
+    // the finisher completes what is queued, then stops
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop_out = true;
+    s->cv_out.notify_all();
+  }
+  if (s->finisher.joinable()) s->finisher.join();
   for (auto& kv : s->reqs) delete kv.second;
   for (Session& ss : s->sessions)
     if (ss.bits) (void)hipHostFree(ss.bits);
+  for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
+  if (s->solved) (void)hipEventDestroy(s->solved);
+  if (s->out_stream) (void)hipStreamDestroy(s->out_stream);
   if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
   if (s->d_bits) (void)hipFree(s->d_bits);
   if (s->d_stage) (void)hipFree(s->d_stage);
