@@ -20,7 +20,8 @@ import ctypes
 
 import numpy as np
 
-from gcmi_lib import OPT_DEVICE_ROUNDS, OPT_GRAPH, OPT_LOCAL_RELABEL, DeviceBuffer, check, host_ptr, lib, ptr
+from gcmi_lib import (OPT_DEVICE_ROUNDS, OPT_GRAPH, OPT_LOCAL_RELABEL, DeviceBuffer, check, host_ptr,
+                      lib, ptr, rgb_image, rgb_model)
 
 TILE_DEFAULT = (32, 32)
 BFS_DEFAULT = 0       # push rounds per global relabel (0: library default, 1)
@@ -73,6 +74,7 @@ class GraphCut:
         self._h = h.value
         self._right = None
         self._down = None
+        self._device_pairwise = False
         self.flow = None
 
     @property
@@ -98,6 +100,27 @@ class GraphCut:
         a = [_i32(x, s) for x in (source, sink, right, down)]
         check(lib().gcmi_cut_upload(self._h, *[host_ptr(x) for x in a]))
         self._right, self._down = a[2], a[3]
+        self._device_pairwise = False
+
+    def upload_rgb(self, image, model=None):
+        """Builds the capacities on the device from a uint8 [H][W][3] (or
+        RGBA [H][W][4]) image: colour-distance unaries and pairwise weights
+        (the reference builds its weights from the RGBA image on the device,
+        graphcut.cl:1018-1131).  model: an RgbModel (gcmi_lib.rgb_model);
+        default: the synthetic workloads' model."""
+        a, ch = rgb_image(image)
+        if a.shape[:2] != self.shape:
+            raise ValueError("expected an image of shape %s" % (self.shape,))
+        m = rgb_model() if model is None else model
+        check(lib().gcmi_cut_upload_rgb(self._h, host_ptr(a), ch, ctypes.byref(m)))
+        self._right = self._down = None
+        self._device_pairwise = True
+
+    def capacities(self):
+        """(source, sink, right, down) int32 [H][W] as stored on the device."""
+        out = [np.empty(self.shape, np.int32) for _ in range(4)]
+        check(lib().gcmi_cut_capacities(self._h, *[host_ptr(a) for a in out]))
+        return tuple(out)
 
     def upload_pairwise(self, right, down):
         """Keep pairwise weights for later cut(dExcess) calls."""
@@ -131,6 +154,15 @@ class GraphCut:
         dExcess (optional): int32 [H][W] signed excess, source - sink as in
         test_GraphCut.py:44-47; it sets source = max(e, 0), sink = max(-e, 0)
         with the pairwise weights of the last upload / upload_pairwise."""
+        if dExcess is not None and self._right is None and self._device_pairwise:
+            # pairwise weights built on the device: new unaries for every
+            # pixel, cut from scratch
+            e = np.asarray(dExcess.get() if hasattr(dExcess, "get") else dExcess)
+            e = _i32(e, self.shape).reshape(-1)
+            self.forget()
+            self.edit(np.arange(e.size, dtype=np.int32), np.maximum(e, 0),
+                      np.maximum(-e.astype(np.int64), 0).astype(np.int32))
+            dExcess = None
         if dExcess is not None:
             if self._right is None:
                 raise ValueError("upload_pairwise() before cut(dExcess)")

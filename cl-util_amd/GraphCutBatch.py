@@ -12,7 +12,8 @@ import ctypes
 import numpy as np
 
 from GraphCut import STAT_NAMES, _edit_arrays, _i32, _points
-from gcmi_lib import KERNEL_CLASSES, OPT_DEVICE_ROUNDS, OPT_GRAPH, OPT_LOCAL_RELABEL, check, host_ptr, lib, ptr
+from gcmi_lib import (KERNEL_CLASSES, OPT_DEVICE_ROUNDS, OPT_GRAPH, OPT_LOCAL_RELABEL, check,
+                      host_ptr, lib, ptr, rgb_image, rgb_model)
 
 
 class GraphCutBatch:
@@ -53,6 +54,15 @@ class GraphCutBatch:
         s = self._shape(i)
         a = [_i32(x, s) for x in (source, sink, right, down)]
         check(lib().gcmi_batch_upload(self._h, int(i), *[host_ptr(x) for x in a]))
+
+    def upload_rgb(self, i, image, model=None):
+        """Image i's capacities built on the device from a colour image (as
+        GraphCut.upload_rgb)."""
+        a, ch = rgb_image(image)
+        if a.shape[:2] != self._shape(i):
+            raise ValueError("expected an image of shape %s" % (self._shape(i),))
+        m = rgb_model() if model is None else model
+        check(lib().gcmi_batch_upload_rgb(self._h, int(i), host_ptr(a), ch, ctypes.byref(m)))
 
     def upload_device(self, i, d_source, d_sink, d_right, d_down):
         check(lib().gcmi_batch_upload_device(self._h, int(i), ptr(d_source), ptr(d_sink),

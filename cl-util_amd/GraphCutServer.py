@@ -23,8 +23,8 @@ import ctypes
 import numpy as np
 
 from GraphCut import _i32, _points
-from gcmi_lib import (EBUSY, OPT_DEVICE_ROUNDS, OPT_LOCAL_RELABEL, REQ_IMAGE, REQ_POLYLINE, REQ_STROKE, GcmiError,
-                      Request, _i64p, check, host_ptr, lib)
+from gcmi_lib import (EBUSY, OPT_DEVICE_ROUNDS, OPT_LOCAL_RELABEL, REQ_IMAGE, REQ_POLYLINE, REQ_RGB,
+                      REQ_STROKE, GcmiError, Request, _i64p, check, host_ptr, lib, rgb_image)
 
 QUEUED, DONE, FAILED = 0, 1, 2
 MODES = {"batched": 0, "plain": 1, "cold": 2}
@@ -112,6 +112,21 @@ class GraphCutServer:
                                              ctypes.byref(rid)))
         return rid.value
 
+    def set_rgb_model(self, model):
+        """Colour model (gcmi_lib.rgb_model) of later submit_rgb requests."""
+        check(lib().gcmi_server_set_rgb_model(self._h, ctypes.byref(model)))
+
+    def submit_rgb(self, session, image):
+        """Queues a new image given as uint8 [H][W][3 or 4] colour; its
+        capacities are built on the device (GraphCut.upload_rgb)."""
+        a, ch = rgb_image(image)
+        if a.shape[:2] != self._shape(session):
+            raise ValueError("expected an image of shape %s" % (self._shape(session),))
+        rid = ctypes.c_int64()
+        check(lib().gcmi_server_submit_rgb(self._h, int(session), host_ptr(a), ch,
+                                           ctypes.byref(rid)))
+        return rid.value
+
     def submit_stroke(self, session, index, source, sink):
         idx = np.ascontiguousarray(index, dtype=np.int32).reshape(-1)
         n = idx.size
@@ -134,8 +149,9 @@ class GraphCutServer:
     def replay(self, trace):
         """Open-loop load generator: trace is a list of (t_ns, session, kind,
         payload) with t_ns on time.monotonic_ns()'s clock (<= 0: at once),
-        kind "image" (payload: source, sink, right, down), "stroke"
-        (index, source, sink) or "polyline" (points, width, source, sink).
+        kind "image" (payload: source, sink, right, down), "rgb" (a uint8
+        colour image), "stroke" (index, source, sink) or "polyline" (points,
+        width, source, sink).
         Submits each request at its time from the calling thread (the GIL
         is released meanwhile) and returns the request ids."""
         n = len(trace)
@@ -159,6 +175,14 @@ class GraphCutServer:
                 q.kind = REQ_STROKE
                 q.n = m
                 q.p0, q.p1, q.p2 = (x.ctypes.data for x in a)
+            elif kind == "rgb":
+                a, ch = rgb_image(payload)
+                if a.shape[:2] != self._shape(session):
+                    raise ValueError("request %d: image shape %s" % (i, a.shape))
+                q.kind = REQ_RGB
+                q.n = ch
+                q.p0 = a.ctypes.data
+                a = [a]
             elif kind == "polyline":
                 xy = _points(payload[0])
                 a = [xy]

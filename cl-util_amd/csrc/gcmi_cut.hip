@@ -85,6 +85,41 @@ int gcmi_cut_upload(gcmi_cut* c, const int32_t* h_source, const int32_t* h_sink,
   return pool_upload(&c->pool, c->img, h_source, h_sink, h_right, h_down, false);
 }
 
+void gcmi_rgb_model_default(gcmi_rgb_model* m) {
+  if (!m) return;
+  *m = gcmi_rgb_model{};
+  const double bg[3] = {60.0, 80.0, 100.0}, fg[3] = {180.0, 150.0, 90.0};
+  for (int k = 0; k < 3; ++k) {
+    m->mu_bg[k] = bg[k];
+    m->mu_fg[k] = fg[k];
+  }
+  m->unary_scale = 10.0;
+  m->unary_div = 100.0;
+  m->unary_cap = 100.0;
+  m->pair_kind = GCMI_PAIR_EXP;
+  m->pair_scale = 50.0;
+  m->inv_beta = 0.1;
+  m->inv_eps = 0.05;
+}
+
+int gcmi_cut_upload_rgb(gcmi_cut* c, const uint8_t* h_rgb, int channels,
+                        const gcmi_rgb_model* model) {
+  if (!c) {
+    set_error("null handle");
+    return GCMI_EINVAL;
+  }
+  return pool_upload_rgb_host(&c->pool, c->img, h_rgb, channels, model);
+}
+
+int gcmi_cut_capacities(gcmi_cut* c, int32_t* h_source, int32_t* h_sink, int32_t* h_right,
+                        int32_t* h_down) {
+  if (!c || !h_source || !h_sink || !h_right || !h_down) {
+    set_error("null argument");
+    return GCMI_EINVAL;
+  }
+  return pool_capacities(&c->pool, c->img, h_source, h_sink, h_right, h_down);
+}
+
 int gcmi_cut_upload_device(gcmi_cut* c, const int32_t* d_source, const int32_t* d_sink,
                            const int32_t* d_right, const int32_t* d_down) {
   if (!c || !d_source || !d_sink || !d_right || !d_down) {
@@ -275,6 +310,12 @@ int gcmi_batch_upload(gcmi_batch* b, int i, const int32_t* h_source, const int32
     return GCMI_EINVAL;
   }
   return pool_upload(&b->pool, b->ids[i], h_source, h_sink, h_right, h_down, false);
+}
+
+int gcmi_batch_upload_rgb(gcmi_batch* b, int i, const uint8_t* h_rgb, int channels,
+                          const gcmi_rgb_model* model) {
+  GCMI_TRY(check_batch_img(b, i));
+  return pool_upload_rgb_host(&b->pool, b->ids[i], h_rgb, channels, model);
 }
 
 int gcmi_batch_upload_device(gcmi_batch* b, int i, const int32_t* d_source,

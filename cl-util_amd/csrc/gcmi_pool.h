@@ -144,8 +144,9 @@ struct Pool {
   int32_t *sel = nullptr, *fresh = nullptr, *plist = nullptr, *alist = nullptr;
   int32_t* q = nullptr;  // three rotating tile queues of cap_tiles entries
   int32_t* dlist = nullptr;  // dirty tiles of the current relabel
-  // In solves of edited images only: relabel the tiles that changed (and
-  // one ring around them) instead of every tile.  Clean tiles keep their
+  // In solves with edited images: relabel the tiles that changed (and one
+  // ring around them; every tile of a fresh image has) instead of every
+  // tile.  Clean tiles keep their
   // stored distances, valid but possibly stale; the solve switches to full
   // relabels after kLocalRelabels.  Exact either way: termination is
   // decided by a relabel over a valid labelling.
@@ -172,6 +173,9 @@ struct Pool {
   Prep* d_prep = nullptr;
   long long* d_flows = nullptr;
   int32_t* d_edit = nullptr;  // staging for edits: idx, cs, ct per edit
+  unsigned long long* d_rgbsum = nullptr;  // pair-difference sum of a colour upload
+  uint8_t* d_rgb = nullptr;                // staging for host colour images
+  long long rgb_cap = 0;                   // bytes
   long long edit_cap = 0;     // pixels
   std::vector<uint64_t> mark;  // host bitmap for the distinct-index check
   // pinned host buffers
@@ -201,6 +205,17 @@ int pool_add_image(Pool* p, int W, int H, int* out);
 int pool_remove_image(Pool* p, int img);
 int pool_upload(Pool* p, int img, const int32_t* cs, const int32_t* ct, const int32_t* right,
                 const int32_t* down, bool device_src);
+// Copies an image's stored capacities (row-major) to host memory.
+int pool_capacities(Pool* p, int img, int32_t* cs, int32_t* ct, int32_t* right, int32_t* down);
+// Checks a colour model; *max_pair (may be null) gets its largest weight.
+int check_rgb_model(const gcmi_rgb_model* m, int32_t* max_pair);
+// Builds an image's capacities on the device from a colour image in device
+// memory ([H][W][channels] uint8), on the pool's stream (no synchronisation).
+int pool_upload_rgb(Pool* p, int img, const uint8_t* d_rgb, int channels,
+                    const gcmi_rgb_model* m);
+// The same from host memory (copied; synchronises).
+int pool_upload_rgb_host(Pool* p, int img, const uint8_t* h_rgb, int channels,
+                         const gcmi_rgb_model* m);
 // Warm-start edit of n distinct pixels (row-major indices local to the
 // image) from host arrays.
 int pool_edit(Pool* p, int img, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct);

@@ -139,6 +139,75 @@ __global__ void k_prepare(Geo g, CutDev s, const Prep* __restrict__ prep, int ns
   }
 }
 
+// ---------------------------------------------------- capacities from colour
+// Squared colour difference of two pixels (channels 0..2).
+__device__ __forceinline__ int rgb_d2(const uint8_t* a, const uint8_t* b) {
+  const int dr = (int)a[0] - (int)b[0], dg = (int)a[1] - (int)b[1], db = (int)a[2] - (int)b[2];
+  return dr * dr + dg * dg + db * db;
+}
+
+// The sum of d2 over an image's horizontal and vertical neighbour pairs
+// (beta's numerator; an exact integer).  One block per row in turn.
+__global__ void __launch_bounds__(256)
+k_rgb_pair_sum(const uint8_t* __restrict__ rgb, int W, int H, int ch, unsigned long long* sum) {
+  __shared__ long long red[256 / kWave];
+  long long acc = 0;
+  for (int y = blockIdx.x; y < H; y += gridDim.x) {
+    const uint8_t* row = rgb + (size_t)y * W * ch;
+    for (int x = threadIdx.x; x < W; x += blockDim.x) {
+      const uint8_t* q = row + (size_t)x * ch;
+      if (x + 1 < W) acc += rgb_d2(q, q + ch);
+      if (y + 1 < H) acc += rgb_d2(q, q + (size_t)W * ch);
+    }
+  }
+  acc = block_sum64<256>(acc, red);
+  if (threadIdx.x == 0 && acc) atomicAdd(sum, (unsigned long long)acc);
+}
+
+// Weight of a pair with squared colour difference d2 (float64, each
+// operation rounded once, in the order the model states).
+__device__ __forceinline__ int32_t rgb_pair_weight(const gcmi_rgb_model& m, int d2,
+                                                   double two_beta) {
+#pragma clang fp contract(off)
+  const double d = (double)d2;
+  if (m.pair_kind == GCMI_PAIR_EXP) return (int32_t)rint(m.pair_scale * exp(-d / two_beta));
+  return (int32_t)rint(m.pair_scale / (m.inv_beta * sqrt(d) + m.inv_eps));
+}
+
+__device__ __forceinline__ int32_t rgb_unary(const gcmi_rgb_model& m, const double* mu, double r,
+                                             double g, double b) {
+#pragma clang fp contract(off)
+  const double e0 = r - mu[0], e1 = g - mu[1], e2 = b - mu[2];
+  const double d = (e0 * e0 + e1 * e1) + e2 * e2;
+  return (int32_t)rint(m.unary_scale * fmin(d / m.unary_div, m.unary_cap));
+}
+
+// Row-major int32 capacities of one image from its colour image.
+__global__ void __launch_bounds__(256)
+k_rgb_caps(const uint8_t* __restrict__ rgb, int W, int H, int ch, gcmi_rgb_model m,
+           const unsigned long long* __restrict__ pair_sum, int32_t* __restrict__ cs,
+           int32_t* __restrict__ ct, int32_t* __restrict__ right, int32_t* __restrict__ down) {
+#pragma clang fp contract(off)
+  double beta = 1.0;
+  if (m.pair_kind == GCMI_PAIR_EXP) {
+    const long long cnt = (long long)(W - 1) * H + (long long)W * (H - 1);
+    if (cnt > 0) beta = (double)*pair_sum / (double)cnt;
+    if (!(beta > 0.0)) beta = 1.0;
+  }
+  const double two_beta = 2.0 * beta;
+  for (int y = blockIdx.x; y < H; y += gridDim.x) {
+    const size_t r0 = (size_t)y * W;
+    for (int x = threadIdx.x; x < W; x += blockDim.x) {
+      const uint8_t* q = rgb + (r0 + x) * ch;
+      const double r = q[0], g = q[1], b = q[2];
+      cs[r0 + x] = rgb_unary(m, m.mu_bg, r, g, b);
+      ct[r0 + x] = rgb_unary(m, m.mu_fg, r, g, b);
+      right[r0 + x] = x + 1 < W ? rgb_pair_weight(m, rgb_d2(q, q + ch), two_beta) : 0;
+      down[r0 + x] = y + 1 < H ? rgb_pair_weight(m, rgb_d2(q, q + (size_t)W * ch), two_beta) : 0;
+    }
+  }
+}
+
 // Zeroes the final-excess sums of the selected images: the first flood
 // pass adds them up (a run whose recording stopped short floods again).
 __global__ void k_zero_posf(unsigned long long* sums, const Prep* __restrict__ prep) {
@@ -401,6 +470,7 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   ALLOC(p->plist, (size_t)cap_tiles * 4);
   ALLOC(p->alist, (size_t)cap_tiles * 4);
   ALLOC(p->counts, CNT_N * 4);
+  ALLOC(p->d_rgbsum, 8);
   ALLOC(p->scratch, (size_t)compact_scratch_ints(cap_tiles) * 4);
   ALLOC(p->d_prep, (size_t)cap_imgs * sizeof(Prep));
   ALLOC(p->d_flows, (size_t)cap_imgs * 8);
@@ -447,7 +517,7 @@ void pool_free(Pool* p) {
                   (void*)d.down,      d.labels, p->d_img,   p->tile_img,  p->sel,     p->fresh,
                   p->q,    p->dlist,  p->plist, p->alist,   p->counts,    p->scratch, p->d_prep,
                   p->d_flows,
-                  p->d_edit};
+                  p->d_edit, p->d_rgbsum, p->d_rgb};
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   p->graph_exec = nullptr;
@@ -563,6 +633,108 @@ int pool_upload(Pool* p, int img, const int32_t* cs, const int32_t* ct, const in
   im.uploaded = true;
   im.fresh = true;
   im.solved = false;
+  return GCMI_OK;
+}
+
+int pool_capacities(Pool* p, int img, int32_t* cs, int32_t* ct, int32_t* right, int32_t* down) {
+  GCMI_TRY(check_img(p, img));
+  const PoolImage& im = p->imgs[img];
+  if (!im.uploaded) {
+    set_error("image %d has no capacities", img);
+    return GCMI_EINVAL;
+  }
+  const size_t n = (size_t)im.W * im.H;
+  const int32_t* src[4] = {p->dev.cs, p->dev.ct, (const int32_t*)p->dev.right,
+                           (const int32_t*)p->dev.down};
+  int32_t* dst[4] = {cs, ct, right, down};
+  for (int k = 0; k < 4; ++k)
+    GCMI_HIP(hipMemcpyAsync(dst[k], src[k] + im.pix0, n * 4, hipMemcpyDeviceToHost, p->stream));
+  GCMI_HIP(hipStreamSynchronize(p->stream));
+  return GCMI_OK;
+}
+
+int check_rgb_model(const gcmi_rgb_model* m, int32_t* max_pair) {
+  if (!m) {
+    set_error("null colour model");
+    return GCMI_EINVAL;
+  }
+  bool ok = m->pair_kind == GCMI_PAIR_EXP || m->pair_kind == GCMI_PAIR_INV;
+  const double v[] = {m->mu_bg[0], m->mu_bg[1], m->mu_bg[2], m->mu_fg[0], m->mu_fg[1],
+                      m->mu_fg[2], m->unary_scale, m->unary_div, m->unary_cap, m->pair_scale,
+                      m->inv_beta, m->inv_eps};
+  for (double x : v) ok = ok && std::isfinite(x);
+  ok = ok && m->unary_scale >= 0 && m->unary_div > 0 && m->unary_cap >= 0 &&
+       m->unary_scale * m->unary_cap <= double(1 << 26) && m->pair_scale >= 0;
+  double wmax = 0;
+  if (ok && m->pair_kind == GCMI_PAIR_EXP) wmax = m->pair_scale;
+  if (ok && m->pair_kind == GCMI_PAIR_INV) {
+    ok = m->inv_beta >= 0 && m->inv_eps > 0;
+    if (ok) wmax = m->pair_scale / m->inv_eps;
+  }
+  ok = ok && wmax <= double(1 << 24);
+  if (!ok) {
+    set_error("colour model out of range");
+    return GCMI_EINVAL;
+  }
+  if (max_pair) *max_pair = (int32_t)std::ceil(wmax + 0.5);
+  return GCMI_OK;
+}
+
+int pool_upload_rgb(Pool* p, int img, const uint8_t* d_rgb, int channels,
+                    const gcmi_rgb_model* m) {
+  GCMI_TRY(check_img(p, img));
+  if (channels != 3 && channels != 4) {
+    set_error("a colour image has 3 or 4 channels (got %d)", channels);
+    return GCMI_EINVAL;
+  }
+  int32_t wmax = 0;
+  GCMI_TRY(check_rgb_model(m, &wmax));
+  PoolImage& im = p->imgs[img];
+  const int rows = std::min(im.H, 8192);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, p->stream, (int32_t*)p->d_rgbsum, 2LL, 0);
+  hipLaunchKernelGGL(k_rgb_pair_sum, dim3(rows), dim3(256), 0, p->stream, d_rgb, im.W, im.H,
+                     channels, p->d_rgbsum);
+  hipLaunchKernelGGL(k_rgb_caps, dim3(rows), dim3(256), 0, p->stream, d_rgb, im.W, im.H, channels,
+                     *m, (const unsigned long long*)p->d_rgbsum, p->dev.cs + im.pix0,
+                     p->dev.ct + im.pix0, (int32_t*)p->dev.right + im.pix0,
+                     (int32_t*)p->dev.down + im.pix0);
+  GCMI_HIP(hipGetLastError());
+  im.max_pair = wmax;
+  im.uploaded = true;
+  im.fresh = true;
+  im.solved = false;
+  return GCMI_OK;
+}
+
+int pool_upload_rgb_host(Pool* p, int img, const uint8_t* h_rgb, int channels,
+                         const gcmi_rgb_model* m) {
+  GCMI_TRY(check_img(p, img));
+  if (!h_rgb) {
+    set_error("null colour image");
+    return GCMI_EINVAL;
+  }
+  if (channels != 3 && channels != 4) {
+    set_error("a colour image has 3 or 4 channels (got %d)", channels);
+    return GCMI_EINVAL;
+  }
+  GCMI_TRY(check_rgb_model(m, nullptr));
+  const PoolImage& im = p->imgs[img];
+  const long long bytes = (long long)im.W * im.H * channels;
+  if (p->rgb_cap < bytes) {
+    GCMI_HIP(hipStreamSynchronize(p->stream));
+    if (p->d_rgb) (void)hipFree(p->d_rgb);
+    p->d_rgb = nullptr;
+    p->rgb_cap = 0;
+    if (hipMalloc((void**)&p->d_rgb, (size_t)bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("device allocation of %lld bytes failed", bytes);
+      return GCMI_ENOMEM;
+    }
+    p->rgb_cap = bytes;
+  }
+  GCMI_HIP(hipMemcpyAsync(p->d_rgb, h_rgb, (size_t)bytes, hipMemcpyHostToDevice, p->stream));
+  GCMI_TRY(pool_upload_rgb(p, img, p->d_rgb, channels, m));
+  GCMI_HIP(hipStreamSynchronize(p->stream));
   return GCMI_OK;
 }
 
@@ -1093,7 +1265,7 @@ int emit_device_driven(const Solve& S, int rounds, int* flood_cur) {
     }
     // local relabels as in the host loop (their dirty lists are empty, so
     // they are gated too, once the solve is done)
-    const bool local = S.p->local_relabel && S.nfresh == 0 && it < kLocalRelabels;
+    const bool local = S.p->local_relabel && S.nfresh < S.nsel && it < kLocalRelabels;
     int cur;
     GCMI_TRY(phase_relabel(S, local, rounds, &cur, &nact));
     GCMI_TRY(phase_push(S, 0));
@@ -1245,10 +1417,12 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
     bool any = true;
     if (rounds > 0) GCMI_TRY(phase_extra_pushes(S, &any));
     // a global relabel: over every selected tile, or (local) only over the
-    // tiles that changed and a ring around them -- only in solves of edited
-    // images, and for kLocalRelabels relabels at most, which bounds the
-    // cost of the stale distances a local relabel leaves in clean tiles
-    const bool local = p->local_relabel && nfresh == 0 && nlocal < kLocalRelabels;
+    // tiles that changed and a ring around them -- in solves with edited
+    // images (a fresh image's tiles are all changed: every tile a push
+    // round processed is), and for kLocalRelabels relabels at most, which
+    // bounds the cost of the stale distances a local relabel leaves in
+    // clean tiles
+    const bool local = p->local_relabel && nfresh < nsel && nlocal < kLocalRelabels;
     if (local) ++nlocal;
     GCMI_TRY(phase_relabel(S, local, 0, &cur, &nact));
     if (p->trace) fprintf(stderr, "gcmi: push round over %d active tiles\n", nact);

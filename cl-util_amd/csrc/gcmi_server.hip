@@ -51,7 +51,12 @@ int64_t now_ns() {
       .count();
 }
 
-enum { REQ_IMAGE = GCMI_REQ_IMAGE, REQ_STROKE = GCMI_REQ_STROKE, REQ_POLYLINE = GCMI_REQ_POLYLINE };
+enum {
+  REQ_IMAGE = GCMI_REQ_IMAGE,
+  REQ_STROKE = GCMI_REQ_STROKE,
+  REQ_POLYLINE = GCMI_REQ_POLYLINE,
+  REQ_RGB = GCMI_REQ_RGB
+};
 enum { ST_QUEUED = 0, ST_DONE = 1, ST_FAILED = 2 };
 
 constexpr long long kStageAlign = 64;  // staging offsets in int32 elements
@@ -68,6 +73,7 @@ struct Request {
   std::vector<double> xy;       // polyline: points
   double width = 0;             // polyline
   int32_t pcs = 0, pct = 0;     // polyline: unaries painted
+  gcmi_rgb_model model{};       // colour image: the capacity model (n: channels)
   int64_t t_arrive = 0, t_start = 0, t_done = 0;
   int64_t t_queued = 0;  // when it entered the queue (after staging)
   int64_t flow = 0;
@@ -100,6 +106,7 @@ struct gcmi_server {
   int mode = 0;  // GCMI_SERVE_WARM, GCMI_SERVE_PLAIN or GCMI_SERVE_COLD
   int max_batch = 64;
   int sweeps = 0, bfs = 0;
+  gcmi_rgb_model rgb_model{};  // of later colour-image requests
   std::vector<Session> sessions;
   uint32_t* d_bits = nullptr;  // device staging for packed masks
   int32_t* d_stage = nullptr;  // device staging for new images
@@ -150,6 +157,9 @@ int serve_batch(gcmi_server* s, std::vector<Request*>& batch, hipEvent_t done) {
       const int32_t* d = s->d_stage + r->stage;
       GCMI_TRY(pool_upload(p, ss.img, d, d + npx, d + 2 * npx, d + 3 * npx, true));
       im.max_pair = r->max_pair;
+    } else if (r->kind == REQ_RGB) {
+      GCMI_TRY(pool_upload_rgb(p, ss.img, (const uint8_t*)(s->d_stage + r->stage), r->n,
+                               &r->model));
     } else {
       // plain and cold modes cut every request from scratch: the edit only
       // rewrites the unaries
@@ -358,6 +368,7 @@ int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_ba
   }
   s->mode = mode;
   s->max_batch = max_batch;
+  gcmi_rgb_model_default(&s->rgb_model);
   (void)hipGetDevice(&s->device);
   const long long cap_pix = cap_tiles * T2;
   int rc = pool_init(&s->pool, tile, (int)cap_tiles, cap_pix, max_sessions, nullptr);
@@ -568,6 +579,71 @@ int submit_image_at(gcmi_server* s, int session, const int32_t* h_source, const 
   return GCMI_OK;
 }
 
+int submit_rgb_at(gcmi_server* s, int session, const uint8_t* h_rgb, int channels,
+                  int64_t t_arrive, int64_t* req_out) {
+  if (!s || !h_rgb) {
+    set_error("null argument");
+    return GCMI_EINVAL;
+  }
+  if (channels != 3 && channels != 4) {
+    set_error("a colour image has 3 or 4 channels (got %d)", channels);
+    return GCMI_EINVAL;
+  }
+  size_t npx;
+  gcmi_rgb_model model;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    GCMI_TRY(check_session(s, session));
+    npx = (size_t)s->sessions[session].W * s->sessions[session].H;
+    model = s->rgb_model;
+  }
+  int32_t wmax = 0;
+  GCMI_TRY(check_rgb_model(&model, &wmax));
+  const long long bytes = (long long)npx * channels;
+  const long long len = ((bytes + 3) / 4 + kStageAlign - 1) / kStageAlign * kStageAlign;
+  if (len > s->stage_alloc.cap) {
+    set_error("image of %zu pixels exceeds the staging area", npx);
+    return GCMI_EBUDGET;
+  }
+  Request* r = new (std::nothrow) Request();
+  if (!r) {
+    set_error("host allocation failed");
+    return GCMI_ENOMEM;
+  }
+  r->session = session;
+  r->kind = REQ_RGB;
+  r->n = channels;
+  r->model = model;
+  r->max_pair = wmax;
+  r->t_arrive = t_arrive;
+  r->stage_len = len;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return (r->stage = s->stage_alloc.alloc(len)) >= 0; });
+  }
+  {
+    std::lock_guard<std::mutex> clk(s->copy_mu);
+    hipError_t e = hipMemcpyAsync(s->d_stage + r->stage, h_rgb, (size_t)bytes,
+                                  hipMemcpyHostToDevice, s->copy_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->copy_stream);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("image staging copy failed: %s", hipGetErrorString(e));
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stage_alloc.release(r->stage, len);
+      s->cv_done.notify_all();
+      delete r;
+      return GCMI_EHIP;
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->sessions[session].max_pair = wmax;
+  }
+  enqueue(s, r, req_out);
+  return GCMI_OK;
+}
+
 int submit_stroke_at(gcmi_server* s, int session, int n, const int32_t* h_index,
                      const int32_t* h_source, const int32_t* h_sink, int64_t t_arrive,
                      int64_t* req_out) {
@@ -658,6 +734,22 @@ int gcmi_server_submit_image(gcmi_server* s, int session, const int32_t* h_sourc
   return submit_image_at(s, session, h_source, h_sink, h_right, h_down, now_ns(), req_out);
 }
 
+int gcmi_server_set_rgb_model(gcmi_server* s, const gcmi_rgb_model* model) {
+  if (!s) {
+    set_error("null server");
+    return GCMI_EINVAL;
+  }
+  GCMI_TRY(check_rgb_model(model, nullptr));
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->rgb_model = *model;
+  return GCMI_OK;
+}
+
+int gcmi_server_submit_rgb(gcmi_server* s, int session, const uint8_t* h_rgb, int channels,
+                           int64_t* req_out) {
+  return submit_rgb_at(s, session, h_rgb, channels, now_ns(), req_out);
+}
+
 int gcmi_server_submit_stroke(gcmi_server* s, int session, int n, const int32_t* h_index,
                               const int32_t* h_source, const int32_t* h_sink,
                               int64_t* req_out) {
@@ -697,6 +789,9 @@ int gcmi_server_replay(gcmi_server* s, int count, const gcmi_request* reqs, int6
       case GCMI_REQ_POLYLINE:
         rc = submit_polyline_at(s, q.session, q.n, (const double*)q.p0, q.width, q.source, q.sink,
                                 t, &id);
+        break;
+      case GCMI_REQ_RGB:
+        rc = submit_rgb_at(s, q.session, (const uint8_t*)q.p0, q.n, t, &id);
         break;
       default:
         set_error("request %d: unknown kind %d", i, q.kind);

@@ -115,12 +115,52 @@ SIGNATURES = {
     "gcmi_server_mask": [_vp, _int, _vp, _i64p],
     "gcmi_server_stats": [_vp, _i64p, _int],
     "gcmi_server_forget": [_vp, _i64],
+    "gcmi_rgb_model_default": [_vp],
+    "gcmi_cut_upload_rgb": [_vp, _vp, _int, _vp],
+    "gcmi_cut_capacities": [_vp, _vp, _vp, _vp, _vp],
+    "gcmi_batch_upload_rgb": [_vp, _int, _vp, _int, _vp],
+    "gcmi_server_set_rgb_model": [_vp, _vp],
+    "gcmi_server_submit_rgb": [_vp, _int, _vp, _int, _i64p],
 }
 
-REQ_IMAGE, REQ_STROKE, REQ_POLYLINE = 0, 1, 2
+REQ_IMAGE, REQ_STROKE, REQ_POLYLINE, REQ_RGB = 0, 1, 2, 3
+PAIR_EXP, PAIR_INV = 0, 1
 OPT_LOCAL_RELABEL = 1
 OPT_DEVICE_ROUNDS = 2
 OPT_GRAPH = 3
+
+
+class RgbModel(ctypes.Structure):
+    """struct gcmi_rgb_model of gcmi.h: capacities from an 8-bit colour image."""
+    _fields_ = [("mu_bg", ctypes.c_double * 3), ("mu_fg", ctypes.c_double * 3),
+                ("unary_scale", ctypes.c_double), ("unary_div", ctypes.c_double),
+                ("unary_cap", ctypes.c_double), ("pair_kind", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("pair_scale", ctypes.c_double),
+                ("inv_beta", ctypes.c_double), ("inv_eps", ctypes.c_double)]
+
+
+def rgb_model(**fields):
+    """The library's default colour model with the given fields replaced
+    (mu_bg / mu_fg as 3-sequences, pair_kind PAIR_EXP or PAIR_INV)."""
+    m = RgbModel()
+    lib().gcmi_rgb_model_default(ctypes.byref(m))
+    for k, v in fields.items():
+        if k in ("mu_bg", "mu_fg"):
+            getattr(m, k)[:] = [float(x) for x in v]
+        elif not hasattr(m, k):
+            raise TypeError("unknown colour-model field %r" % k)
+        else:
+            setattr(m, k, v)
+    return m
+
+
+def rgb_image(image):
+    """A C-contiguous uint8 [H][W][3 or 4] array and its channel count."""
+    a = np.ascontiguousarray(image, dtype=np.uint8)
+    if a.ndim != 3 or a.shape[2] not in (3, 4):
+        raise ValueError("expected a [H][W][3] or [H][W][4] uint8 image, got shape %s"
+                         % (a.shape,))
+    return a, a.shape[2]
 
 
 class Request(ctypes.Structure):
@@ -135,7 +175,8 @@ class Request(ctypes.Structure):
 
 KERNEL_CLASSES = ("push_first", "push", "fold", "bfs_init", "bfs_round", "flood",
                   "labels", "compact")
-_RESTYPE = {"gcmi_last_error": ctypes.c_char_p, "gcmi_version": ctypes.c_char_p}
+_RESTYPE = {"gcmi_last_error": ctypes.c_char_p, "gcmi_version": ctypes.c_char_p,
+            "gcmi_rgb_model_default": None}
 
 _lib = None
 

@@ -93,6 +93,41 @@ def pairwise(image):
     return right, down
 
 
+def caps_from_rgb(image, mu_bg=MU_BG, mu_fg=MU_FG, unary_scale=10.0, unary_div=100.0,
+                  unary_cap=100.0, pair="exp", pair_scale=50.0, inv_beta=0.1, inv_eps=0.05):
+    """Capacities of a uint8 [H][W][3 or 4] colour image under a colour
+    model (the host statement of gcmi_rgb_model; float64, each operation
+    rounded once): unaries rint(unary_scale * min(|I - mu|^2 / unary_div,
+    unary_cap)); pairwise "exp": rint(pair_scale * exp(-d2 / (2 beta))),
+    beta = mean d2 (1 if 0); "inv": rint(pair_scale / (inv_beta * sqrt(d2) +
+    inv_eps)), the reference's weight (graphcut.cl:1018-1031)."""
+    img = image[..., :3].astype(np.float64)
+    H, W, _ = img.shape
+
+    def unary(mu):
+        e = img - np.asarray(mu, dtype=np.float64)
+        d = (e[..., 0] * e[..., 0] + e[..., 1] * e[..., 1]) + e[..., 2] * e[..., 2]
+        return np.rint(unary_scale * np.minimum(d / unary_div, unary_cap)).astype(np.int32)
+
+    dh = ((img[:, 1:] - img[:, :-1]) ** 2).sum(-1)
+    dv = ((img[1:, :] - img[:-1, :]) ** 2).sum(-1)
+    if pair == "exp":
+        count = dh.size + dv.size
+        beta = (dh.sum() + dv.sum()) / count if count else 1.0
+        if beta <= 0:
+            beta = 1.0
+        weight = lambda d: np.rint(pair_scale * np.exp(-d / (2.0 * beta)))
+    elif pair == "inv":
+        weight = lambda d: np.rint(pair_scale / (inv_beta * np.sqrt(d) + inv_eps))
+    else:
+        raise ValueError(pair)
+    right = np.zeros((H, W), dtype=np.int32)
+    down = np.zeros((H, W), dtype=np.int32)
+    right[:, :-1] = weight(dh).astype(np.int32)
+    down[:-1, :] = weight(dv).astype(np.int32)
+    return Workload(image, unary(mu_bg), unary(mu_fg), right, down)
+
+
 def from_image(image):
     source, sink = unaries(image)
     right, down = pairwise(image)

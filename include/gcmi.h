@@ -141,6 +141,42 @@ int gcmi_cut_upload(gcmi_cut *cut, const int32_t *h_source,
 int gcmi_cut_upload_device(gcmi_cut *cut, const int32_t *d_source,
                            const int32_t *d_sink, const int32_t *d_right,
                            const int32_t *d_down);
+/* Capacities built on the device from an 8-bit colour image (the
+ * reference builds its pairwise weights on the device from an RGBA image:
+ * weight / load_tiles, graphcut.cl:1018-1131; GraphCut.pyx:80-154).  For a
+ * pixel colour I (channels 0..2; a 4th channel is ignored) and the squared
+ * colour difference d2 of two 4-neighbours, in float64:
+ *   source = rint(unary_scale * min(|I - mu_bg|^2 / unary_div, unary_cap))
+ *   sink   = rint(unary_scale * min(|I - mu_fg|^2 / unary_div, unary_cap))
+ *   GCMI_PAIR_EXP: w = rint(pair_scale * exp(-d2 / (2 beta))), beta = the
+ *     mean d2 over all horizontal and vertical pairs of the image (1 if 0)
+ *   GCMI_PAIR_INV: w = rint(pair_scale / (inv_beta * sqrt(d2) + inv_eps)),
+ *     the reference's weight with pair_scale = lamda (GraphCut.pyx:80,
+ *     graphcut.cl:1030, BETA 1/10, EPSILON 0.05)
+ * rint rounds half to even.  Bounds: unary_scale * unary_cap <= 2^26 and
+ * the largest weight (pair_scale, or pair_scale / inv_eps) <= 2^24. */
+#define GCMI_PAIR_EXP 0
+#define GCMI_PAIR_INV 1
+typedef struct gcmi_rgb_model {
+  double mu_bg[3];
+  double mu_fg[3];
+  double unary_scale, unary_div, unary_cap;
+  int32_t pair_kind;
+  int32_t reserved;
+  double pair_scale, inv_beta, inv_eps;
+} gcmi_rgb_model;
+/* The synthetic workloads' model: mu_bg (60,80,100), mu_fg (180,150,90),
+ * unaries 10 * min(d / 100, 100), GCMI_PAIR_EXP with pair_scale 50
+ * (inv_beta 0.1, inv_eps 0.05 for GCMI_PAIR_INV). */
+void gcmi_rgb_model_default(gcmi_rgb_model *model);
+/* Builds and uploads the capacities from an [height][width][channels]
+ * uint8 image (channels 3 or 4) in host memory (copied). */
+int gcmi_cut_upload_rgb(gcmi_cut *cut, const uint8_t *h_rgb, int channels,
+                        const gcmi_rgb_model *model);
+/* Copies the stored capacities (as uploaded, built or edited; each array
+ * [height][width] int32) to host memory. */
+int gcmi_cut_capacities(gcmi_cut *cut, int32_t *h_source, int32_t *h_sink,
+                        int32_t *h_right, int32_t *h_down);
 /* GraphCut.cut (GraphCut.pyx:348-418): run push-relabel to termination.
  *   sweeps      in-tile push/relabel sweeps per tile residency (<=0: default)
  *   bfs         push rounds between global relabels (<=0: default)
@@ -239,6 +275,9 @@ int gcmi_batch_upload(gcmi_batch *batch, int i, const int32_t *h_source,
 int gcmi_batch_upload_device(gcmi_batch *batch, int i,
                              const int32_t *d_source, const int32_t *d_sink,
                              const int32_t *d_right, const int32_t *d_down);
+/* Capacities of image i built from a colour image (as gcmi_cut_upload_rgb). */
+int gcmi_batch_upload_rgb(gcmi_batch *batch, int i, const uint8_t *h_rgb,
+                          int channels, const gcmi_rgb_model *model);
 /* Warm-start edit of image i (as gcmi_cut_edit). */
 int gcmi_batch_edit(gcmi_batch *batch, int i, int n, const int32_t *h_index,
                     const int32_t *h_source, const int32_t *h_sink);
@@ -301,6 +340,14 @@ int gcmi_server_submit_image(gcmi_server *server, int session,
                              const int32_t *h_source, const int32_t *h_sink,
                              const int32_t *h_right, const int32_t *h_down,
                              int64_t *req_out);
+/* The colour model of later gcmi_server_submit_rgb requests (default:
+ * gcmi_rgb_model_default). */
+int gcmi_server_set_rgb_model(gcmi_server *server, const gcmi_rgb_model *model);
+/* Queues a new image given as an [height][width][channels] uint8 colour
+ * image (channels 3 or 4, copied at once); its capacities are built on the
+ * device with the server's colour model. */
+int gcmi_server_submit_rgb(gcmi_server *server, int session, const uint8_t *h_rgb,
+                           int channels, int64_t *req_out);
 /* Queues a brush stroke: new source/sink capacities for n distinct pixels
  * (checked as gcmi_cut_edit; needs a submitted image). */
 int gcmi_server_submit_stroke(gcmi_server *server, int session, int n,
@@ -316,6 +363,7 @@ int gcmi_server_submit_polyline(gcmi_server *server, int session, int npts,
 #define GCMI_REQ_IMAGE 0    /* p0..p3: source, sink, right, down          */
 #define GCMI_REQ_STROKE 1   /* n pixels; p0..p2: index, source, sink      */
 #define GCMI_REQ_POLYLINE 2 /* n points; p0: xy; width, source, sink     */
+#define GCMI_REQ_RGB 3      /* n channels; p0: colour image               */
 typedef struct gcmi_request {
   int64_t t_ns; /* submit at this steady-clock (CLOCK_MONOTONIC) time, in
                    ns; it is the request's arrival time.  <= 0: at once */

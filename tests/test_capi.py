@@ -104,3 +104,36 @@ def test_prim_bench_without_device():
     r = subprocess.run([PRIM_BENCH, "1"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 3
     assert "no HIP device" in r.stderr
+
+
+def test_rgb_model_layout_and_default():
+    """struct gcmi_rgb_model as ctypes sees it; the default model is the
+    synthetic workloads' (host-only calls, no device needed)."""
+    assert ctypes.sizeof(L.RgbModel) == 104
+    assert L.RgbModel.pair_kind.offset == 72 and L.RgbModel.pair_scale.offset == 80
+    m = L.rgb_model()
+    assert list(m.mu_bg) == [60.0, 80.0, 100.0] and list(m.mu_fg) == [180.0, 150.0, 90.0]
+    assert (m.unary_scale, m.unary_div, m.unary_cap) == (10.0, 100.0, 100.0)
+    assert m.pair_kind == L.PAIR_EXP and m.pair_scale == 50.0
+    assert (m.inv_beta, m.inv_eps) == (0.1, 0.05)
+    m2 = L.rgb_model(pair_kind=L.PAIR_INV, mu_fg=(1, 2, 3))
+    assert m2.pair_kind == L.PAIR_INV and list(m2.mu_fg) == [1.0, 2.0, 3.0]
+    with pytest.raises(TypeError):
+        L.rgb_model(lamda=60)
+
+
+def test_rgb_host_model_matches_generator():
+    """The colour model's host statement reproduces the generator's
+    capacities, and its inverse weight follows the reference's formula at
+    closed-form points (d2 = 0: lamda / eps; d2 = 100: lamda / (1 + eps))."""
+    import gcmi_workloads as G
+    w = G.centred(64, 48, seed=2)
+    c = G.caps_from_rgb(w.image)
+    for a, b in zip(w.arrays(), c.arrays()):
+        assert (a == b).all()
+    img = np.zeros((1, 3, 3), np.uint8)
+    img[0, 1] = (6, 8, 0)  # d2 = 100 to both neighbours
+    inv = G.caps_from_rgb(img, pair="inv", pair_scale=60.0)
+    assert inv.right[0, 0] == round(60.0 / 1.05) == 57 and inv.right[0, 1] == 57
+    flat = G.caps_from_rgb(np.zeros((2, 2, 3), np.uint8), pair="inv", pair_scale=60.0)
+    assert flat.right[0, 0] == 1200 and flat.down[0, 0] == 1200 and flat.right[0, 1] == 0

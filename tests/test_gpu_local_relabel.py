@@ -111,3 +111,36 @@ def test_local_relabel_batch_and_server():
         for i, sid in enumerate(sids):
             mask, _ = srv.mask(sid)
             assert (mask == O.bk_cut(*cur[i].arrays())[1]).all(), i
+
+
+@pytest.mark.parametrize("device_rounds", [0, 4])
+def test_local_relabel_mixed_fresh_and_warm(device_rounds):
+    """A run that cuts some images from new capacities and reflows strokes on
+    the others, with local relabels (host- or device-driven)."""
+    dims = [(300, 220), (128, 300), (257, 129), (200, 200), (64, 500)]
+    works = [G.three_ellipses(w, h, seed=60 + i) for i, (w, h) in enumerate(dims)]
+    strokes = [G.strokes(w, h, count=4, seed=70 + i) for i, (w, h) in enumerate(dims)]
+    b = GraphCutBatch(dims, tile=32)
+    b.set_local_relabel(True)
+    if device_rounds:
+        b.set_device_rounds(device_rounds)
+    for i, w in enumerate(works):
+        b.upload(i, *w.arrays())
+    b.cut()
+    cur = [w.copy() for w in works]
+    rng = np.random.default_rng(device_rounds)
+    for r in range(4):
+        for i in range(len(dims)):
+            if (i + r) % 3 == 0:
+                cur[i] = G.tiny(dims[i][0], dims[i][1], 500 + 10 * r + i, cap_hi=400, w_hi=50)
+                b.upload(i, *cur[i].arrays())
+            else:
+                st = strokes[i][r]
+                s, t = st.capacities()
+                b.paint(i, st.points, st.width, s, t)
+                G.apply_stroke(cur[i], st)
+        which = sorted(rng.choice(len(dims), size=4, replace=False).tolist()) if r == 2 else None
+        flows = b.cut(which=which)
+        for k, i in enumerate(which if which is not None else range(len(dims))):
+            fo, mo = O.bk_cut(*cur[i].arrays())
+            assert flows[k] == fo and (b.labels(i) == mo).all(), (r, i)
