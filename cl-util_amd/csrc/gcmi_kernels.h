@@ -603,7 +603,7 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
 constexpr int kFoldNT = 256;
 
 template <int T>
-__device__ __forceinline__ void fold_edge_pixel(CutDev& s, int t, int j) {
+__device__ __forceinline__ long long fold_edge_pixel(CutDev& s, int t, int j) {
   int x, y;
   if (j < T) { x = j; y = 0; }
   else if (j < 2 * T) { x = j - T; y = T - 1; }
@@ -615,7 +615,7 @@ __device__ __forceinline__ void fold_edge_pixel(CutDev& s, int t, int j) {
   const int vd = y == T - 1 ? pe[1 * T + x] : 0;
   const int vl = x == 0 ? pe[2 * T + y] : 0;
   const int vr = x == T - 1 ? pe[3 * T + y] : 0;
-  if ((vu | vd | vl | vr) == 0) return;
+  if ((vu | vd | vl | vr) == 0) return 0;
   const int e0 = s.e[i];
   const int e1 = e0 + vu + vd + vl + vr;
   const int cu = s.cu[i] + vu, cd = s.cd[i] + vd, cl = s.cl[i] + vl, cr = s.cr[i] + vr;
@@ -629,9 +629,7 @@ __device__ __forceinline__ void fold_edge_pixel(CutDev& s, int t, int j) {
   if (vd) pe[1 * T + x] = 0;
   if (vl) pe[2 * T + y] = 0;
   if (vr) pe[3 * T + y] = 0;
-  const long long delta = (long long)(e1 > 0 ? e1 : 0) - (e0 > 0 ? e0 : 0);
-  if (delta)
-    atomicAdd(reinterpret_cast<unsigned long long*>(&s.tpos[t]), (unsigned long long)delta);
+  return (long long)(e1 > 0 ? e1 : 0) - (e0 > 0 ? e0 : 0);
 }
 
 template <int T>
@@ -646,7 +644,11 @@ k_fold(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
     __syncthreads();  // every thread has read the flag before it is cleared
     if (!flagged) continue;
     if (j == 0) s.pflag[t] = 0;
-    if (j < 4 * T - 4) fold_edge_pixel<T>(s, t, j);
+    // the change of the tile's positive-excess sum, one atomic per wave
+    long long delta = j < 4 * T - 4 ? fold_edge_pixel<T>(s, t, j) : 0;
+    delta = wave_sum64(delta);
+    if (lane_id() == 0 && delta)
+      atomicAdd(reinterpret_cast<unsigned long long*>(&s.tpos[t]), (unsigned long long)delta);
   }
 }
 
@@ -1223,10 +1225,10 @@ __device__ __forceinline__ void edit_pixel(const CutDev& s, const ImgDesc& d, in
   // a new deficit is a sink at height 0; the tile pushes before the first
   // relabel of the next solve
   if (e1 < 0) s.h[q] = 0;
+  // the tile is active: the solve's first push round over the active tiles
+  // recomputes its positive-excess sum tpos before anything reads it
   s.act[t] = 1;
   s.dirty[t] = 1;
-  const long long dt = (long long)(e1 > 0 ? e1 : 0) - (e0 > 0 ? e0 : 0);
-  if (dt) atomicAdd(reinterpret_cast<unsigned long long*>(&s.tpos[t]), (unsigned long long)dt);
 }
 
 // Adds a block's edit deltas to the image's sums (slotted atomics).
@@ -1275,6 +1277,9 @@ struct Polyline {
   // rows ya[k] .. ya[k] + (row0[k + 1] - row0[k]) - 1; nrows in all
   int ya[kMaxStrokePoints], row0[kMaxStrokePoints];
   int nrows;
+  // pixel box of each segment's capsule (a pixel outside it is not in it)
+  int sx0[kMaxStrokePoints], sx1[kMaxStrokePoints], sy0[kMaxStrokePoints],
+      sy1[kMaxStrokePoints];
 };
 
 // Whether pixel centre (px, py) lies within r of segment k: the point of the
@@ -1354,7 +1359,9 @@ k_paint(Geo g, CutDev s, int img, Polyline pl, int warm) {
     for (int x = xa + lane; x <= xb; x += kWave) {
       if (!in_capsule(pl, k, (double)x, (double)y)) continue;
       bool earlier = false;
-      for (int j = 0; j < k && !earlier; ++j) earlier = in_capsule(pl, j, (double)x, (double)y);
+      for (int j = 0; j < k && !earlier; ++j)
+        earlier = x >= pl.sx0[j] && x <= pl.sx1[j] && y >= pl.sy0[j] && y <= pl.sy1[j] &&
+                  in_capsule(pl, j, (double)x, (double)y);
       if (!earlier) edit_pixel<T>(s, d, x, y, pl.cs, pl.ct, warm, dmin, dpos);
     }
   }

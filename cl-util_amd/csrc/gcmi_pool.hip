@@ -913,6 +913,11 @@ int make_polyline(int W, int H, int npts, const double* xy, double width, int32_
     const bool cols = (long long)std::ceil(xhi) >= 0 && (long long)std::floor(xlo) <= W - 1;
     const long long rows = (ya <= yb && cols) ? yb - ya + 1 : 0;
     pl.ya[k] = rows > 0 ? (int)ya : 0;
+    // the capsule's pixel box, one pixel wider each way than needed
+    pl.sx0[k] = (int)std::max<long long>((long long)std::floor(xlo) - 1, -1);
+    pl.sx1[k] = (int)std::min<long long>((long long)std::ceil(xhi) + 1, W);
+    pl.sy0[k] = (int)std::max<long long>((long long)std::floor(ylo) - 1, -1);
+    pl.sy1[k] = (int)std::min<long long>((long long)std::ceil(yhi) + 1, H);
     pl.row0[k] = (int)std::min<long long>(total, INT32_MAX);
     total += rows;
   }
