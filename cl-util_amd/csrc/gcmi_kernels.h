@@ -82,6 +82,10 @@ struct CutDev {
                                        // tile's distances were last computed
   int32_t* rflag;                      // tile being recomputed by a local relabel
   int32_t* haspos;                     // tile holds excess
+  int32_t* fchg;                       // arcs or excess changed since the tile's reach was
+                                       // last computed
+  int32_t* imp;                        // the tile's reach is not all reached from its own
+                                       // excess through its own arcs (some came over an edge)
   long long* tpos;                     // per tile: sum of max(e, 0)
   unsigned long long* sums;            // per image [SUM_ROWS][kSumSlots]
   unsigned long long* tally;           // [TALLY_N]
@@ -1059,10 +1063,13 @@ __device__ __forceinline__ int flood_relax(int (&r)[T * T / NT], const int (&m)[
   }
 }
 
-// first = true: over the tiles holding excess, seeds from the masks' excess
-// bit, and the images' final sum of max(e, 0) from the tile sums; else: from
-// the stored reach bytes of queued tiles, with halo reach.  A pixel newly
-// reached on an edge with a residual arc across it queues that neighbour.
+// first = true: over listed tiles whose reach bytes were cleared, seeds
+// from the masks' excess bit, halo reach from the neighbours' stored reach
+// bytes; the tile's import flag is set when its reach differs from what its
+// own excess reaches through its own arcs (no halo).  Else: over queued
+// tiles, from their stored reach bytes, halo reach; a tile that gains reach
+// has imported it.  A pixel newly reached on an edge with a residual arc
+// across it queues that neighbour.
 template <int T, int NT>
 __global__ void __launch_bounds__(NT)
 k_flood(Geo g, CutDev s, Queue Q, int first) {
@@ -1075,12 +1082,10 @@ k_flood(Geo g, CutDev s, Queue Q, int first) {
   bq_init(bq, Q);
   if (!first && blockIdx.x == 0 && threadIdx.x == 0 && n > 0)
     atomicAdd(&s.tally[TALLY_FLOOD], (unsigned long long)n);
-  ImgAcc apos;
   for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
     const TG q = tile_geo(g, Q.in ? Q.in[pos] : pos);
     const size_t base = tbase<T>(q.t);
-    if (first && threadIdx.x == 0) apos.add(s.sums, SUM_POSF, q.img, s.tpos[q.t]);
-    int r[R], r0[R], m[R];
+    int r[R], r0[R], m[R], rl[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       const int ly = chain * R + k;
@@ -1090,35 +1095,63 @@ k_flood(Geo g, CutDev s, Queue Q, int first) {
       r[k] = r0[k] | ((m[k] & M_EXCESS) ? 1 : 0);
       RM[(ly + 1) * HP + lx + 1] = (m[k] << 1) | r[k];
     }
+    int hv = 0;
+    Halo hs{};
     if (threadIdx.x < 4 * T) {
-      const Halo hs = halo_slot<T>(q, threadIdx.x);
-      int v = 0;
+      hs = halo_slot<T>(q, threadIdx.x);
       if (hs.nb >= 0) {
         const size_t p = tbase<T>(hs.nb) + hs.ny * T + hs.nx;
-        v = (int)s.mask[p] << 1;
-        if (!first) v |= s.reach[p] ? 1 : 0;
+        hv = ((int)s.mask[p] << 1) | (s.reach[p] ? 1 : 0);
       }
-      RM[hs.idx] = v;
+      // a first pass relaxes without halo reach first: the tile's own reach
+      RM[hs.idx] = first ? (hv & ~1) : hv;
     }
     __syncthreads();
-    const int iters = flood_relax<T, NT>(r, m, RM);
+    int iters = 0;
+    if (first) {
+      iters += flood_relax<T, NT>(r, m, RM);
+#pragma unroll
+      for (int k = 0; k < R; ++k) rl[k] = r[k];
+      if (threadIdx.x < 4 * T) RM[hs.idx] = hv;
+      __syncthreads();
+    }
+    iters += flood_relax<T, NT>(r, m, RM);
     if (threadIdx.x == 0 && !first)
       atomicAdd(&s.tally[TALLY_FLOOD_ITERS], (unsigned long long)iters);
-    int dirs = 0;
+    int dirs = 0, imported = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       const int ly = chain * R + k;
       if (r[k] && !r0[k]) {
         s.reach[base + ly * T + lx] = 1;
         dirs |= edge_dirs<T>(q, lx, ly) & m[k];
+        imported |= first ? !rl[k] : 1;
       }
     }
     if (dirs) atomicOr(&bq.dirs, dirs);
-    __syncthreads();
+    imported = __syncthreads_or(imported);
+    if (threadIdx.x == 0) {
+      if (first)
+        s.imp[q.t] = imported;
+      else if (imported)
+        s.imp[q.t] = 1;
+    }
     bq_take(bq, q, s.stamp, Q);
   }
-  if (first && threadIdx.x == 0) apos.flush(s.sums, SUM_POSF);
   bq_flush(bq, Q);
+}
+
+// The images' final sum of max(e, 0) from the listed tiles' sums.
+template <int T>
+__global__ void k_posf(Geo g, CutDev s, const int32_t* __restrict__ list,
+                       const int32_t* __restrict__ count) {
+  const int n = *count;
+  ImgAcc apos;
+  for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < n; pos += gridDim.x * blockDim.x) {
+    const int t = list ? list[pos] : pos;
+    apos.add(s.sums, SUM_POSF, g.tile_img[t], s.tpos[t]);
+  }
+  apos.flush(s.sums, SUM_POSF);
 }
 
 // Row-major labels of the listed tiles' images from the reach bytes (a
@@ -1158,14 +1191,17 @@ __global__ void k_reset_tiles(CutDev s, const int32_t* __restrict__ list,
   }
 }
 
-// Per listed tile: clear the reach bytes (before the flood).
+// Per listed tile: clear the reach bytes and the import flag (before the
+// flood).
 template <int T>
 __global__ void k_clear_reach(CutDev s, const int32_t* __restrict__ list,
                               const int32_t* __restrict__ count) {
   const int n = *count;
   for (int pos = blockIdx.x; pos < n; pos += gridDim.x) {
-    uint32_t* r = reinterpret_cast<uint32_t*>(s.reach + tbase<T>(list ? list[pos] : pos));
+    const int t = list ? list[pos] : pos;
+    uint32_t* r = reinterpret_cast<uint32_t*>(s.reach + tbase<T>(t));
     for (int j = threadIdx.x; j < T * T / 4; j += blockDim.x) r[j] = 0;
+    if (threadIdx.x == 0) s.imp[t] = 0;
   }
 }
 

@@ -90,7 +90,8 @@ struct PoolOps {
   void (*flood)(Pool*, int grid, const Queue& q, bool first);
   void (*labels)(Pool*, int grid);
   void (*reset_tiles)(Pool*, int grid);
-  void (*clear_reach)(Pool*, int grid);
+  void (*clear_reach)(Pool*, int grid, const int32_t* list, int cnt);
+  void (*posf)(Pool*, int grid);
   void (*edit)(Pool*, int img, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct);
   void (*paint)(Pool*, int img, const Polyline& pl);
   void (*untile)(Pool*, int img, const int32_t* src, int32_t* out);

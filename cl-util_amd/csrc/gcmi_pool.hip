@@ -96,6 +96,52 @@ __global__ void k_clear_listed(int32_t* flags, const int32_t* __restrict__ list,
     flags[list ? list[i] : i] = 0;
 }
 
+// Per listed tile: a dirty tile (arcs or excess changed since its last
+// relabel) is marked changed for the next flood, and its dirty flag cleared.
+__global__ void k_take_dirty(CutDev s, const int32_t* __restrict__ list,
+                             const int32_t* __restrict__ count) {
+  const int n = *count;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int t = list ? list[i] : i;
+    if (s.dirty[t]) {
+      s.fchg[t] = 1;
+      s.dirty[t] = 0;
+    }
+  }
+}
+
+// The tiles whose reach a flood must recompute: the changed tiles (fchg, the
+// `count` tiles of `init`) and, transitively, every tile that imported
+// reach (imp) next to one -- its reach may have come through the change.
+// One block, a worklist of the tiles added last (in buf0 / buf1, each room
+// for every tile); a tile enters once (the fchg swap).  Ends when a pass
+// adds nothing, after at most as many passes as there are tiles.
+__global__ void __launch_bounds__(1024)
+k_imp_closure(Geo g, CutDev s, const int32_t* __restrict__ init, const int32_t* __restrict__ count,
+              int32_t* buf0, int32_t* buf1) {
+  __shared__ int nnext;
+  const int32_t* cur = init;
+  int n = *count;
+  int32_t* next = buf0;
+  while (n > 0) {
+    if (threadIdx.x == 0) nnext = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const TG q = tile_geo(g, cur[i]);
+      const int nb[4] = {q.ty > 0 ? q.t - q.tw : -1, q.ty < q.th - 1 ? q.t + q.tw : -1,
+                         q.tx > 0 ? q.t - 1 : -1, q.tx < q.tw - 1 ? q.t + 1 : -1};
+      for (int k = 0; k < 4; ++k)
+        if (nb[k] >= 0 && s.imp[nb[k]] && atomicExch(&s.fchg[nb[k]], 1) == 0)
+          next[atomicAdd(&nnext, 1)] = nb[k];
+    }
+    __syncthreads();
+    n = nnext;
+    cur = next;
+    next = next == buf0 ? buf1 : buf0;
+    __syncthreads();
+  }
+}
+
 // Local relabel set: rflag[t] = t or one of its four neighbours is dirty
 // (one ring around every changed tile, so a distance that rose next to an
 // edit is recomputed rather than left to the stale neighbours).
@@ -263,8 +309,12 @@ struct PoolKernels {
     hipLaunchKernelGGL((k_reset_tiles<T>), dim3(grid), dim3(256), 0, p->stream, p->dev, p->lfresh,
                        p->counts + CNT_FRESH);
   }
-  static void clear_reach(Pool* p, int grid) {
-    hipLaunchKernelGGL((k_clear_reach<T>), dim3(grid), dim3(256), 0, p->stream, p->dev, p->lsel,
+  static void clear_reach(Pool* p, int grid, const int32_t* list, int cnt) {
+    hipLaunchKernelGGL((k_clear_reach<T>), dim3(grid), dim3(256), 0, p->stream, p->dev, list,
+                       p->counts + cnt);
+  }
+  static void posf(Pool* p, int grid) {
+    hipLaunchKernelGGL((k_posf<T>), dim3(grid), dim3(256), 0, p->stream, p->geo, p->dev, p->lsel,
                        p->counts + CNT_SEL);
   }
   static void edit(Pool* p, int img, int n, const int32_t* idx, const int32_t* cs,
@@ -299,6 +349,7 @@ static const PoolOps* ops_for() {
                               &PoolKernels<T, NT>::labels,
                               &PoolKernels<T, NT>::reset_tiles,
                               &PoolKernels<T, NT>::clear_reach,
+                              &PoolKernels<T, NT>::posf,
                               &PoolKernels<T, NT>::edit,
                               &PoolKernels<T, NT>::paint,
                               &PoolKernels<T, NT>::untile};
@@ -449,6 +500,8 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   ALLOC(d.dirty, (size_t)cap_tiles * 4);
   ALLOC(d.rflag, (size_t)cap_tiles * 4);
   ALLOC(d.haspos, (size_t)cap_tiles * 4);
+  ALLOC(d.fchg, (size_t)cap_tiles * 4);
+  ALLOC(d.imp, (size_t)cap_tiles * 4);
   ALLOC(d.tpos, (size_t)cap_tiles * 8);
   ALLOC(d.sums, (size_t)cap_imgs * SUM_ROWS * kSumSlots * 8);
   ALLOC(d.tally, TALLY_N * 8);
@@ -500,7 +553,7 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   p->pix_alloc.init(cap_pix);
   // pending slots and flags start (and stay, between solves) zero
   GCMI_HIP(hipMemsetAsync(d.pend, 0, (size_t)cap_tiles * 4 * tile * 4, p->stream));
-  int32_t* flags[] = {d.pflag, d.act, d.stamp, d.bflag, d.dirty, d.rflag, d.haspos};
+  int32_t* flags[] = {d.pflag, d.act, d.stamp, d.bflag, d.dirty, d.rflag, d.haspos, d.fchg, d.imp};
   for (int32_t* f : flags) GCMI_HIP(hipMemsetAsync(f, 0, (size_t)cap_tiles * 4, p->stream));
   GCMI_HIP(hipMemsetAsync(d.tpos, 0, (size_t)cap_tiles * 8, p->stream));
   GCMI_HIP(hipMemsetAsync(p->counts, 0, CNT_N * 4, p->stream));
@@ -512,7 +565,7 @@ void pool_free(Pool* p) {
   CutDev& d = p->dev;
   void* ptrs[] = {d.e,     d.h,       d.cu,     d.cd,       d.cl,         d.cr,       d.mask,
                   d.reach, d.pend,    d.pflag,  d.act,      d.stamp,      d.bflag,    d.dirty,
-                  d.rflag, d.haspos,
+                  d.rflag, d.haspos,  d.fchg,   d.imp,
                   d.tpos,  d.sums,    d.tally,  d.cs,       d.ct,         (void*)d.right,
                   (void*)d.down,      d.labels, p->d_img,   p->tile_img,  p->sel,     p->fresh,
                   p->q,    p->dlist,  p->plist, p->alist,   p->counts,    p->scratch, p->d_prep,
@@ -1088,8 +1141,8 @@ int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32
   S.ops->bfs_init(p, S.grid_relabel, glist, gcnt, !local);
   S.E();
   GCMI_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_clear_listed, dim3(S.grid_clear()), dim3(256), 0, p->stream, p->dev.dirty,
-                     glist, p->counts + gcnt);
+  hipLaunchKernelGGL(k_take_dirty, dim3(S.grid_clear()), dim3(256), 0, p->stream, p->dev, glist,
+                     p->counts + gcnt);
   if (local)
     hipLaunchKernelGGL(k_clear_listed, dim3(S.grid_clear()), dim3(256), 0, p->stream,
                        p->dev.rflag, glist, p->counts + gcnt);
@@ -1181,10 +1234,28 @@ int phase_extra_pushes(const Solve& S, bool* any) {
 // the list the next round would read in *cur_out.
 int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
   Pool* p = S.p;
-  // haspos persists: a clean tile keeps the flag its last relabel wrote
-  GCMI_TRY(compact_sel(p, p->dev.haspos, p->plist, CNT_POS, false, S.nsel));
-  S.ops->clear_reach(p, std::min(S.nsel, 4096));
+  // the images' final sum of max(e, 0)
   hipLaunchKernelGGL(k_zero_posf, dim3(S.n), dim3(64), 0, p->stream, p->dev.sums, p->d_prep);
+  S.ops->posf(p, std::min((S.nsel + 255) / 256, 1024));
+  const size_t cap = (size_t)p->cap_tiles;
+  if (S.nfresh < S.nsel) {
+    // incremental: recompute the changed tiles and the tiles whose reach
+    // may have come through them; every other tile keeps its reach bytes
+    // (they can only grow, and the rounds grow them)
+    GCMI_TRY(compact_sel(p, p->dev.fchg, p->q, CNT_Q, false, S.nsel));
+    hipLaunchKernelGGL(k_imp_closure, dim3(1), dim3(1024), 0, p->stream, p->geo, p->dev, p->q,
+                       p->counts + CNT_Q, p->q + cap, p->q + 2 * cap);
+    GCMI_TRY(compact_sel(p, p->dev.fchg, p->plist, CNT_POS, true, S.nsel));
+    S.ops->clear_reach(p, std::min(S.nsel, 4096), p->plist, CNT_POS);
+  } else {
+    // from scratch: the tiles holding excess seed the first pass (haspos
+    // persists: a clean tile keeps the flag its last relabel wrote)
+    GCMI_TRY(compact_sel(p, p->dev.haspos, p->plist, CNT_POS, false, S.nsel));
+    S.ops->clear_reach(p, std::min(S.nsel, 4096), p->lsel, CNT_SEL);
+    hipLaunchKernelGGL(k_clear_listed, dim3(S.grid_clear()), dim3(256), 0, p->stream,
+                       p->dev.fchg, p->lsel, p->counts + CNT_SEL);
+  }
+  GCMI_HIP(hipGetLastError());
   clear_queues(p);
   S.B(K_FLOOD);
   const Queue first{p->plist,
