@@ -234,8 +234,14 @@ def warm_leg(args):
     res = {}
     masks = {}
     flows = {}
-    for mode in ("warm", "scratch"):
+    # warm reflows host-driven with full relabels, with local relabels, and
+    # device-driven with local relabels; scratch: every stroke from scratch
+    variants = {"warm_host": {}, "warm_local": dict(local=True),
+                "warm_device": dict(local=True, rounds=8), "scratch": {}}
+    for mode, opt in variants.items():
         gc = GraphCut(None, None, (W, H), tile=args.tile)
+        gc.set_local_relabel(opt.get("local", False))
+        gc.set_device_rounds(opt.get("rounds", 0))
         gc.upload(*work.arrays())
         gc.cut()
         gc.labels_packed()
@@ -256,12 +262,14 @@ def warm_leg(args):
                      "p99_ms": round(float(np.percentile(lat, 99)), 3),
                      "mean_ms": round(float(np.mean(lat)), 3)}
         gc.close()
+    best = min((m for m in variants if m != "scratch"), key=lambda m: res[m]["mean_ms"])
     out = {"workload": "2048x2048 centred ellipse (seed 3), 100 strokes (seed 3), "
                        "device brush, packed mask to host", "tile": args.tile,
-           "warm": res["warm"], "scratch": res["scratch"],
-           "speedup_mean": round(res["scratch"]["mean_ms"] / res["warm"]["mean_ms"], 2),
-           "flows_equal": flows["warm"] == flows["scratch"],
-           "final_mask_equal": bool((masks["warm"] == masks["scratch"]).all())}
+           "warm": dict(res[best], variant=best), "scratch": res["scratch"],
+           "variants": res,
+           "speedup_mean": round(res["scratch"]["mean_ms"] / res[best]["mean_ms"], 2),
+           "flows_equal": all(flows[m] == flows["scratch"] for m in variants),
+           "final_mask_equal": all(bool((masks[m] == masks["scratch"]).all()) for m in variants)}
     if not args.no_parity:
         sys.path.insert(0, os.path.join(HERE, "oracle"))
         import oracle as O
@@ -271,8 +279,8 @@ def warm_leg(args):
         fo, mo = O.bk_cut(*cur.arrays())
         packed = np.packbits(mo.reshape(-1), bitorder="little")
         packed = np.pad(packed, (0, (-packed.size) % 4)).view(np.uint32)
-        out["parity_last_stroke"] = {"flow_equal": fo == flows["warm"][-1],
-                                     "mask_equal": bool((packed == masks["warm"]).all())}
+        out["parity_last_stroke"] = {"flow_equal": fo == flows[best][-1],
+                                     "mask_equal": bool((packed == masks[best]).all())}
     return out
 
 
@@ -280,31 +288,56 @@ def warm_leg(args):
 
 
 def serve_leg(args):
+    """The 256-session mix at swept Poisson rates.  Modes: plain (one request
+    at a time, each cut from scratch: the baseline), cold (continuous
+    batching, every request cut from scratch) and batched (continuous
+    batching, strokes reflowed warm: device-driven runs with local
+    relabels).  New images arrive as 8-bit colour images in every mode."""
     import gcmi_serving as S
     from GraphCutServer import GraphCutServer
     log("served: generating the 256-session mix")
     mix = S.Mix(nsess=256, seed=4)
-    rates = {"plain": (500, 1000, 1500), "cold": (1000, 2000, 4000),
-             "batched": (1000, 2000, 4000)}
+    modes = {"plain": (dict(mode="plain"), (1000, 1500, 2000, 2500)),
+             "cold": (dict(mode="cold", device_rounds=8), (2000, 3000, 4000, 6000)),
+             "batched": (dict(mode="batched", local_relabel=True, device_rounds=8),
+                         (4000, 6000, 7000, 8000, 10000))}
+    stroke_only = {"plain": (1500, 2000, 2500, 3000), "batched": (8000, 10000, 13000, 16000)}
     out = {"workload": "256 sessions, sizes {512,1024,2048,4096}^2 weights 4:3:2:1 (seed 4), "
-                       "Poisson arrivals, 10%% new images / 90%% polyline strokes, "
-                       "%.1f s per rate" % args.serve_seconds,
-           "resident_pixels": mix.pixels, "hbm_budget_bytes": 200 * 10**9, "modes": {}}
-    for mode, rs in rates.items():
+                       "Poisson arrivals, 10%% new images (8-bit colour, capacities built on "
+                       "the device) / 90%% polyline strokes, %.1f s per rate"
+                       % args.serve_seconds,
+           "resident_pixels": mix.pixels, "hbm_budget_bytes": 200 * 10**9, "modes": {},
+           "stroke_only": {}}
+    for mode, (kw, rs) in modes.items():
         log("served: mode %s" % mode)
         with GraphCutServer(tile=32, hbm_budget=200 * 10**9, max_sessions=256, max_batch=64,
-                            mode=mode) as srv:
-            sids = S.open_sessions(srv, mix)
-            pts = [S.run_trace(srv, sids, mix, r, duration=args.serve_seconds, seed=11 + k)
-                   for k, r in enumerate(rs)]
+                            **kw) as srv:
+            sids = S.open_sessions(srv, mix, rgb=True)
+            pts = [S.run_trace(srv, sids, mix, r, duration=args.serve_seconds, seed=11 + k,
+                               rgb=True) for k, r in enumerate(rs)]
             out["modes"][mode] = {"points": pts, "sustained_2048_p99_20ms": S.sustained(pts),
                                   "device_bytes": srv.stats()["device_bytes"]}
-    sus = {m: (v["sustained_2048_p99_20ms"] or {}).get("stroke_rate")
-           for m, v in out["modes"].items()}
+            if mode in stroke_only:
+                # the same sessions, strokes only
+                log("served: mode %s, strokes only" % mode)
+                pts = [S.run_trace(srv, sids, mix, r, duration=args.serve_seconds,
+                                   seed=51 + k, image_frac=0.0)
+                       for k, r in enumerate(stroke_only[mode])]
+                out["stroke_only"][mode] = {"points": pts,
+                                            "sustained_2048_p99_20ms": S.sustained(pts)}
+
+    def rate(entry):
+        return (entry["sustained_2048_p99_20ms"] or {}).get("stroke_rate")
+
+    sus = {m: rate(v) for m, v in out["modes"].items()}
     out["sustained_stroke_rate"] = sus
     if sus.get("plain"):
         best = max((v for m, v in sus.items() if m != "plain" and v), default=None)
         out["served_vs_plain"] = round(best / sus["plain"], 2) if best else None
+    so = {m: rate(v) for m, v in out["stroke_only"].items()}
+    out["stroke_only_sustained_stroke_rate"] = so
+    if so.get("plain") and so.get("batched"):
+        out["stroke_only_served_vs_plain"] = round(so["batched"] / so["plain"], 2)
     return out
 
 

@@ -36,19 +36,17 @@ enum { M_UP = 1, M_DOWN = 2, M_LEFT = 4, M_RIGHT = 8, M_SINK = 16, M_EXCESS = 32
 // max(e0, 0); sum of max(e, 0) after the last relabel.
 constexpr int kSumSlots = 64;
 enum { SUM_MIN = 0, SUM_POS0 = 1, SUM_POSF = 2, SUM_ROWS = 3 };
-// Pool-wide tallies (added by block 0 of each launch): tiles processed by later BFS / flood passes, and their in-tile relaxation
-// iterations; push launches with work and their tiles; relabel first passes
-// with work and their tiles
+// Pool-wide tallies (added by block 0 of each launch): tiles processed by
+// later BFS / flood passes; push launches with work and their tiles;
+// relabel first passes with work and their tiles
 enum {
   TALLY_BFS = 0,
   TALLY_FLOOD = 1,
-  TALLY_BFS_ITERS = 2,
-  TALLY_FLOOD_ITERS = 3,
-  TALLY_PUSH_TILES = 4,
-  TALLY_PUSH_ROUNDS = 5,
-  TALLY_RELABELS = 6,
-  TALLY_RELABEL_TILES = 7,
-  TALLY_N = 8
+  TALLY_PUSH_TILES = 2,
+  TALLY_PUSH_ROUNDS = 3,
+  TALLY_RELABELS = 4,
+  TALLY_RELABEL_TILES = 5,
+  TALLY_N = 6
 };
 // Number of XCDs blocks are dealt to round-robin.
 constexpr int kXcds = 8;
@@ -953,8 +951,7 @@ k_bfs_round(Geo g, CutDev s, Queue Q) {
       D1[hs.idx] = v;
     }
     __syncthreads();
-    const int iters = bfs_relax<T, NT>(d, m, D0, D1);
-    if (threadIdx.x == 0) atomicAdd(&s.tally[TALLY_BFS_ITERS], (unsigned long long)iters);
+    bfs_relax<T, NT>(d, m, D0, D1);
     int any_act = 0, dirs = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -1063,16 +1060,21 @@ __device__ __forceinline__ int flood_relax(int (&r)[T * T / NT], const int (&m)[
   }
 }
 
-// first = true: over listed tiles whose reach bytes were cleared, seeds
-// from the masks' excess bit, halo reach from the neighbours' stored reach
+// mode FLOOD_FIRST: over the tiles holding excess, reach bytes all cleared:
+// seeds from the masks' excess bit, no halo reach, no import.
+// FLOOD_FIRST_INC: over listed tiles whose reach bytes were cleared (other
+// tiles keep theirs): seeds, halo reach from the neighbours' stored reach
 // bytes; the tile's import flag is set when its reach differs from what its
-// own excess reaches through its own arcs (no halo).  Else: over queued
-// tiles, from their stored reach bytes, halo reach; a tile that gains reach
-// has imported it.  A pixel newly reached on an edge with a residual arc
-// across it queues that neighbour.
+// own excess reaches through its own arcs (no halo).  FLOOD_ROUND: over
+// queued tiles, from their stored reach bytes, halo reach; a tile that
+// gains reach has imported it.  A pixel newly reached on an edge with a
+// residual arc across it queues that neighbour.
+enum { FLOOD_ROUND = 0, FLOOD_FIRST = 1, FLOOD_FIRST_INC = 2 };
+
 template <int T, int NT>
 __global__ void __launch_bounds__(NT)
-k_flood(Geo g, CutDev s, Queue Q, int first) {
+k_flood(Geo g, CutDev s, Queue Q, int mode) {
+  const bool first = mode != FLOOD_ROUND;
   constexpr int R = T * T / NT;
   constexpr int HP = T + 2;
   __shared__ int32_t RM[HP * HP];
@@ -1101,23 +1103,28 @@ k_flood(Geo g, CutDev s, Queue Q, int first) {
       hs = halo_slot<T>(q, threadIdx.x);
       if (hs.nb >= 0) {
         const size_t p = tbase<T>(hs.nb) + hs.ny * T + hs.nx;
-        hv = ((int)s.mask[p] << 1) | (s.reach[p] ? 1 : 0);
+        hv = (int)s.mask[p] << 1;
+        if (mode != FLOOD_FIRST) hv |= s.reach[p] ? 1 : 0;
       }
       // a first pass relaxes without halo reach first: the tile's own reach
       RM[hs.idx] = first ? (hv & ~1) : hv;
     }
-    __syncthreads();
-    int iters = 0;
-    if (first) {
-      iters += flood_relax<T, NT>(r, m, RM);
+    // (the barrier publishes RM; with no reached halo pixel the tile's own
+    // reach is all it gets)
+    const bool two =
+        __syncthreads_or(mode == FLOOD_FIRST_INC && threadIdx.x < 4 * T && (hv & 1)) != 0;
+    if (two) {
+      flood_relax<T, NT>(r, m, RM);
 #pragma unroll
       for (int k = 0; k < R; ++k) rl[k] = r[k];
       if (threadIdx.x < 4 * T) RM[hs.idx] = hv;
       __syncthreads();
     }
-    iters += flood_relax<T, NT>(r, m, RM);
-    if (threadIdx.x == 0 && !first)
-      atomicAdd(&s.tally[TALLY_FLOOD_ITERS], (unsigned long long)iters);
+    flood_relax<T, NT>(r, m, RM);
+    if (first && !two) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) rl[k] = r[k];
+    }
     int dirs = 0, imported = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -1141,17 +1148,33 @@ k_flood(Geo g, CutDev s, Queue Q, int first) {
   bq_flush(bq, Q);
 }
 
-// The images' final sum of max(e, 0) from the listed tiles' sums.
+// The images' final sum of max(e, 0) from the listed tiles' sums: a wave
+// whose 64 tiles are of one image (the lists hold each image's tiles in a
+// run) adds them with one atomic.
 template <int T>
 __global__ void k_posf(Geo g, CutDev s, const int32_t* __restrict__ list,
                        const int32_t* __restrict__ count) {
   const int n = *count;
-  ImgAcc apos;
-  for (int pos = blockIdx.x * blockDim.x + threadIdx.x; pos < n; pos += gridDim.x * blockDim.x) {
-    const int t = list ? list[pos] : pos;
-    apos.add(s.sums, SUM_POSF, g.tile_img[t], s.tpos[t]);
+  const int stride = gridDim.x * blockDim.x;
+  for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    const int pos = base + threadIdx.x;
+    int img = -1;
+    long long v = 0;
+    if (pos < n) {
+      const int t = list ? list[pos] : pos;
+      img = g.tile_img[t];
+      v = s.tpos[t];
+    }
+    const int img0 = __shfl(img, 0);
+    const unsigned long long row = SUM_POSF * kSumSlots + blockIdx.x % kSumSlots;
+    if (__all(img == img0 || img < 0)) {
+      v = wave_sum64(v);
+      if (lane_id() == 0 && img0 >= 0 && v)
+        atomicAdd(&s.sums[(size_t)img0 * SUM_ROWS * kSumSlots + row], (unsigned long long)v);
+    } else if (img >= 0 && v) {
+      atomicAdd(&s.sums[(size_t)img * SUM_ROWS * kSumSlots + row], (unsigned long long)v);
+    }
   }
-  apos.flush(s.sums, SUM_POSF);
 }
 
 // Row-major labels of the listed tiles' images from the reach bytes (a
@@ -1191,8 +1214,8 @@ __global__ void k_reset_tiles(CutDev s, const int32_t* __restrict__ list,
   }
 }
 
-// Per listed tile: clear the reach bytes and the import flag (before the
-// flood).
+// Per listed tile: clear the reach bytes, the import flag and the changed
+// flag (before a flood recomputes the tile).
 template <int T>
 __global__ void k_clear_reach(CutDev s, const int32_t* __restrict__ list,
                               const int32_t* __restrict__ count) {
@@ -1201,7 +1224,10 @@ __global__ void k_clear_reach(CutDev s, const int32_t* __restrict__ list,
     const int t = list ? list[pos] : pos;
     uint32_t* r = reinterpret_cast<uint32_t*>(s.reach + tbase<T>(t));
     for (int j = threadIdx.x; j < T * T / 4; j += blockDim.x) r[j] = 0;
-    if (threadIdx.x == 0) s.imp[t] = 0;
+    if (threadIdx.x == 0) {
+      s.imp[t] = 0;
+      s.fchg[t] = 0;
+    }
   }
 }
 

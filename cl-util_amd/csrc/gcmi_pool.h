@@ -87,7 +87,7 @@ struct PoolOps {
   void (*fold)(Pool*, int grid, const int32_t* list, int cnt);
   void (*bfs_init)(Pool*, int grid, const int32_t* list, int cnt, bool all);
   void (*bfs_round)(Pool*, int grid, const Queue& q);
-  void (*flood)(Pool*, int grid, const Queue& q, bool first);
+  void (*flood)(Pool*, int grid, const Queue& q, int mode);
   void (*labels)(Pool*, int grid);
   void (*reset_tiles)(Pool*, int grid);
   void (*clear_reach)(Pool*, int grid, const int32_t* list, int cnt);

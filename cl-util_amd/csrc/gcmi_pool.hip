@@ -297,9 +297,9 @@ struct PoolKernels {
     hipLaunchKernelGGL((k_bfs_round<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
                        q);
   }
-  static void flood(Pool* p, int grid, const Queue& q, bool first) {
+  static void flood(Pool* p, int grid, const Queue& q, int mode) {
     hipLaunchKernelGGL((k_flood<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev, q,
-                       first ? 1 : 0);
+                       mode);
   }
   static void labels(Pool* p, int grid) {
     hipLaunchKernelGGL((k_labels<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
@@ -1183,8 +1183,8 @@ int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32
                               p->stream));
     GCMI_TRY(read_counts(p));
     if (p->trace)
-      fprintf(stderr, "gcmi:   bfs round -> queue %d (tiles so far %llu, iterations %llu)\n",
-              hc[CNT_Q + cur], p->h_tally[TALLY_BFS], p->h_tally[TALLY_BFS_ITERS]);
+      fprintf(stderr, "gcmi:   bfs round -> queue %d (tiles so far %llu)\n", hc[CNT_Q + cur],
+              p->h_tally[TALLY_BFS]);
     if (hc[CNT_Q + cur] == 0) break;
   }
   *nact = hc[CNT_ACT];
@@ -1238,7 +1238,8 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
   hipLaunchKernelGGL(k_zero_posf, dim3(S.n), dim3(64), 0, p->stream, p->dev.sums, p->d_prep);
   S.ops->posf(p, std::min((S.nsel + 255) / 256, 1024));
   const size_t cap = (size_t)p->cap_tiles;
-  if (S.nfresh < S.nsel) {
+  const bool inc = S.nfresh < S.nsel;
+  if (inc) {
     // incremental: recompute the changed tiles and the tiles whose reach
     // may have come through them; every other tile keeps its reach bytes
     // (they can only grow, and the rounds grow them)
@@ -1252,8 +1253,6 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
     // persists: a clean tile keeps the flag its last relabel wrote)
     GCMI_TRY(compact_sel(p, p->dev.haspos, p->plist, CNT_POS, false, S.nsel));
     S.ops->clear_reach(p, std::min(S.nsel, 4096), p->lsel, CNT_SEL);
-    hipLaunchKernelGGL(k_clear_listed, dim3(S.grid_clear()), dim3(256), 0, p->stream,
-                       p->dev.fchg, p->lsel, p->counts + CNT_SEL);
   }
   GCMI_HIP(hipGetLastError());
   clear_queues(p);
@@ -1266,13 +1265,13 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
                     p->counts + CNT_EP + p->ep_par,
                     p->counts + CNT_EP + 1 - p->ep_par};
   p->ep_par ^= 1;
-  S.ops->flood(p, p->grid_cap, first, true);
+  S.ops->flood(p, p->grid_cap, first, inc ? FLOOD_FIRST_INC : FLOOD_FIRST);
   S.E();
   GCMI_HIP(hipGetLastError());
   int cur = 0;
   for (int b = 0; b < rounds_dd; ++b) {
     S.B(K_FLOOD);
-    S.ops->flood(p, p->grid_cap, round_queue(p, cur), false);
+    S.ops->flood(p, p->grid_cap, round_queue(p, cur), FLOOD_ROUND);
     S.E();
     GCMI_HIP(hipGetLastError());
     cur = (cur + 1) % 3;
@@ -1306,7 +1305,7 @@ int flood_rounds_host(const Solve& S, int cur) {
   for (int batch = p->trace ? 1 : kRoundBatch;; batch = p->trace ? 1 : kRoundBatchMax) {
     for (int b = 0; b < batch; ++b) {
       S.B(K_FLOOD);
-      S.ops->flood(p, p->grid_cap, round_queue(p, cur), false);
+      S.ops->flood(p, p->grid_cap, round_queue(p, cur), FLOOD_ROUND);
       S.E();
       GCMI_HIP(hipGetLastError());
       cur = (cur + 1) % 3;
@@ -1315,8 +1314,8 @@ int flood_rounds_host(const Solve& S, int cur) {
     GCMI_TRY(phase_results(S));
     GCMI_TRY(read_counts(p));
     if (p->trace)
-      fprintf(stderr, "gcmi:   flood round -> queue %d (tiles so far %llu, iterations %llu)\n",
-              hc[CNT_Q + cur], p->h_tally[TALLY_FLOOD], p->h_tally[TALLY_FLOOD_ITERS]);
+      fprintf(stderr, "gcmi:   flood round -> queue %d (tiles so far %llu)\n", hc[CNT_Q + cur],
+              p->h_tally[TALLY_FLOOD]);
     if (hc[CNT_Q + cur] == 0) break;
   }
   return GCMI_OK;

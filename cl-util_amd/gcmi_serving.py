@@ -6,7 +6,9 @@ The mix: `nsess` sessions with square sizes drawn from SIZES with weights
 (three_ellipses with a per-size seed; K images per size), so that the
 4096x4096 images are not generated 26 times.  Requests arrive as a Poisson
 process; each picks a session uniformly; 10% are new images (the session's
-next base image, cut from scratch) and 90% are brush strokes: polylines
+next base image, cut from scratch; sent as its 8-bit colour image, whose
+capacities the server builds on the device, or as the four int32 capacity
+arrays) and 90% are brush strokes: polylines
 from a seeded bank per size (3-8 segments, width 8-32, alternating
 foreground/background hard constraints), rasterised on the device.
 
@@ -46,17 +48,22 @@ class Mix:
         return b[(i + k) % len(b)]
 
 
-def open_sessions(srv, mix):
+def image_request(work, rgb):
+    """(kind, payload) of a new-image request for a workload."""
+    return ("rgb", work.image) if rgb else ("image", work.arrays())
+
+
+def open_sessions(srv, mix, rgb=False):
     """Opens every session and cuts its first image; returns the ids."""
     sids = [srv.open(n, n) for n in mix.sizes]
-    trace = [(0, sids[i], "image", mix.base(i).arrays()) for i in range(mix.nsess)]
+    trace = [(0, sids[i]) + image_request(mix.base(i), rgb) for i in range(mix.nsess)]
     for r in srv.replay(trace):
         srv.wait(int(r))
         srv.forget(int(r))
     return sids
 
 
-def make_trace(mix, rate, duration, seed, t0_ns, image_frac=0.1):
+def make_trace(mix, rate, duration, seed, t0_ns, image_frac=0.1, rgb=False):
     """(strokes, images): lists of (t_ns, session index, kind, payload, size)."""
     rng = np.random.default_rng(seed)
     n = max(1, int(rate * duration))
@@ -68,7 +75,8 @@ def make_trace(mix, rate, duration, seed, t0_ns, image_frac=0.1):
         size = mix.sizes[i]
         if rng.random() < image_frac:
             nimg[i] += 1
-            images.append((int(ts[k]), i, "image", mix.base(i, int(nimg[i])).arrays(), size))
+            kind, payload = image_request(mix.base(i, int(nimg[i])), rgb)
+            images.append((int(ts[k]), i, kind, payload, size))
         else:
             st = mix.bank[size][int(rng.integers(len(mix.bank[size])))]
             s, t = st.capacities()
@@ -76,12 +84,12 @@ def make_trace(mix, rate, duration, seed, t0_ns, image_frac=0.1):
     return strokes, images
 
 
-def run_trace(srv, sids, mix, rate, duration=2.0, seed=11, image_frac=0.1, lead=0.2):
+def run_trace(srv, sids, mix, rate, duration=2.0, seed=11, image_frac=0.1, lead=0.2, rgb=False):
     """Replays a Poisson trace (strokes and new images from two generator
     threads, so an image's host-side staging does not delay the strokes
     behind it) and returns latency statistics in ms."""
     t0 = time.monotonic_ns() + int(lead * 1e9)
-    strokes, images = make_trace(mix, rate, duration, seed, t0, image_frac)
+    strokes, images = make_trace(mix, rate, duration, seed, t0, image_frac, rgb)
     ids = {}
     errors = []
 
