@@ -1068,12 +1068,13 @@ __device__ __forceinline__ int flood_relax(int (&r)[T * T / NT], const int (&m)[
 // own excess reaches through its own arcs (no halo).  FLOOD_ROUND: over
 // queued tiles, from their stored reach bytes, halo reach; a tile that
 // gains reach has imported it.  A pixel newly reached on an edge with a
-// residual arc across it queues that neighbour.
+// residual arc across it queues that neighbour.  lmark (may be null): every
+// processed tile is flagged there (its labels are rewritten after).
 enum { FLOOD_ROUND = 0, FLOOD_FIRST = 1, FLOOD_FIRST_INC = 2 };
 
 template <int T, int NT>
 __global__ void __launch_bounds__(NT)
-k_flood(Geo g, CutDev s, Queue Q, int mode) {
+k_flood(Geo g, CutDev s, Queue Q, int mode, int32_t* lmark) {
   const bool first = mode != FLOOD_ROUND;
   constexpr int R = T * T / NT;
   constexpr int HP = T + 2;
@@ -1142,6 +1143,7 @@ k_flood(Geo g, CutDev s, Queue Q, int mode) {
         s.imp[q.t] = imported;
       else if (imported)
         s.imp[q.t] = 1;
+      if (lmark) lmark[q.t] = 1;
     }
     bq_take(bq, q, s.stamp, Q);
   }
@@ -1396,27 +1398,46 @@ __device__ __forceinline__ bool seg_row_span(const Polyline& pl, int k, double y
   return true;
 }
 
-// Paints a polyline stroke into image img: one wave per (segment, row) the
-// segment's capsule can reach, its lanes over the row's candidate columns.
-// A pixel is painted by the first segment whose capsule holds it, so every
-// pixel of the union is edited exactly once.
+// A stroke of a paint launch: its polyline, image, whether the image has
+// solved state to reflow (warm), and the stroke's first row among all the
+// launch's (segment, row) items.
+struct PaintJob {
+  Polyline pl;
+  int img;
+  int warm;
+  int row0;
+};
+
+// Paints the strokes of `jobs` (row0 ascending): one wave per (stroke,
+// segment, row) a segment's capsule can reach, its lanes over the row's
+// candidate columns.  A pixel is painted by the first segment of its stroke
+// whose capsule holds it, so every pixel of a stroke's union is edited once
+// (strokes of one launch are of different images).
 template <int T>
 __global__ void __launch_bounds__(256)
-k_paint(Geo g, CutDev s, int img, Polyline pl, int warm) {
-  __shared__ long long red[256 / kWave];
-  const ImgDesc d = g.img[img];
-  long long dmin = 0, dpos = 0;
+k_paint(Geo g, CutDev s, const PaintJob* __restrict__ jobs, int njobs, int nrows) {
   const int lane = threadIdx.x % kWave, waves = blockDim.x / kWave;
-  for (int item = blockIdx.x * waves + threadIdx.x / kWave; item < pl.nrows;
+  for (int item = blockIdx.x * waves + threadIdx.x / kWave; item < nrows;
        item += gridDim.x * waves) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].row0 <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const PaintJob& J = jobs[lo];
+    const Polyline& pl = J.pl;
+    const int local = item - J.row0;
     int k = 0;
-    while (k + 2 < pl.npts && item >= pl.row0[k + 1]) ++k;
-    const int y = pl.ya[k] + (item - pl.row0[k]);
+    while (k + 2 < pl.npts && local >= pl.row0[k + 1]) ++k;
+    const int y = pl.ya[k] + (local - pl.row0[k]);
     double xl, xr;
     if (!seg_row_span(pl, k, (double)y, &xl, &xr)) continue;
     xl = fmax(xl, 0.0);
     xr = fmin(xr, (double)(pl.W - 1));
     if (xl > xr) continue;
+    const ImgDesc d = g.img[J.img];
+    long long dmin = 0, dpos = 0;
     const int xa = (int)floor(xl), xb = (int)ceil(xr);
     for (int x = xa + lane; x <= xb; x += kWave) {
       if (!in_capsule(pl, k, (double)x, (double)y)) continue;
@@ -1424,10 +1445,19 @@ k_paint(Geo g, CutDev s, int img, Polyline pl, int warm) {
       for (int j = 0; j < k && !earlier; ++j)
         earlier = x >= pl.sx0[j] && x <= pl.sx1[j] && y >= pl.sy0[j] && y <= pl.sy1[j] &&
                   in_capsule(pl, j, (double)x, (double)y);
-      if (!earlier) edit_pixel<T>(s, d, x, y, pl.cs, pl.ct, warm, dmin, dpos);
+      if (!earlier) edit_pixel<T>(s, d, x, y, pl.cs, pl.ct, J.warm, dmin, dpos);
+    }
+    if (J.warm) {
+      // the row's changes of the image's sums (one slotted atomic per wave)
+      dmin = wave_sum64(dmin);
+      dpos = wave_sum64(dpos);
+      if (lane == 0) {
+        const size_t slot = (size_t)J.img * SUM_ROWS * kSumSlots + blockIdx.x % kSumSlots;
+        if (dmin) atomicAdd(&s.sums[slot + SUM_MIN * kSumSlots], (unsigned long long)dmin);
+        if (dpos) atomicAdd(&s.sums[slot + SUM_POS0 * kSumSlots], (unsigned long long)dpos);
+      }
     }
   }
-  edit_sums(s, img, dmin, dpos, red);
 }
 
 }  // namespace gcmi

@@ -87,13 +87,13 @@ struct PoolOps {
   void (*fold)(Pool*, int grid, const int32_t* list, int cnt);
   void (*bfs_init)(Pool*, int grid, const int32_t* list, int cnt, bool all);
   void (*bfs_round)(Pool*, int grid, const Queue& q);
-  void (*flood)(Pool*, int grid, const Queue& q, int mode);
-  void (*labels)(Pool*, int grid);
+  void (*flood)(Pool*, int grid, const Queue& q, int mode, int32_t* lmark);
+  void (*labels)(Pool*, int grid, const int32_t* list, int cnt);
   void (*reset_tiles)(Pool*, int grid);
   void (*clear_reach)(Pool*, int grid, const int32_t* list, int cnt);
   void (*posf)(Pool*, int grid);
   void (*edit)(Pool*, int img, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct);
-  void (*paint)(Pool*, int img, const Polyline& pl);
+  void (*paint)(Pool*, const PaintJob* d_jobs, int njobs, int nrows);
   void (*untile)(Pool*, int img, const int32_t* src, int32_t* out);
 };
 
@@ -154,6 +154,7 @@ struct Pool {
   bool local_relabel = false;
   bool trace = false;  // GCMI_TRACE=1: per-phase lines on stderr, one round per host check
   int ep_par = 0;         // epoch slot the next queue round reads
+  bool flood_inc = false;  // the current flood recomputes changed tiles only
   bool ep_reset = false;  // restart stamps and epochs before the next solve
   // Device-driven solves (dd_rounds > 0): every launch of a fixed shape,
   // dd_rounds BFS / flood rounds per relabel, gates on the device instead of
@@ -177,6 +178,8 @@ struct Pool {
   unsigned long long* d_rgbsum = nullptr;  // pair-difference sum of a colour upload
   uint8_t* d_rgb = nullptr;                // staging for host colour images
   long long rgb_cap = 0;                   // bytes
+  PaintJob *h_jobs = nullptr, *d_jobs = nullptr;  // staging for paint launches
+  int jobs_cap = 0;
   long long edit_cap = 0;     // pixels
   std::vector<uint64_t> mark;  // host bitmap for the distinct-index check
   // pinned host buffers
@@ -242,6 +245,16 @@ int make_polyline(int W, int H, int npts, const double* xy, double width, int32_
 // polyline get unaries (cs, ct).
 int pool_paint(Pool* p, int img, int npts, const double* xy, double width, int32_t cs,
                int32_t ct);
+// Several strokes (of different images) painted by one launch; all are
+// checked before any is painted.
+struct PaintReq {
+  int img;
+  int npts;
+  const double* xy;
+  double width;
+  int32_t cs, ct;
+};
+int pool_paint_many(Pool* p, int count, const PaintReq* reqs);
 // Solves the listed images together (fresh ones from their capacities,
 // edited ones warm).  flows_out (may be NULL) receives one flow per image.
 int pool_solve(Pool* p, const int* img_ids, int n, int sweeps, int bfs, int64_t max_rounds,

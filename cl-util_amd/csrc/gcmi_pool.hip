@@ -297,13 +297,13 @@ struct PoolKernels {
     hipLaunchKernelGGL((k_bfs_round<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
                        q);
   }
-  static void flood(Pool* p, int grid, const Queue& q, int mode) {
+  static void flood(Pool* p, int grid, const Queue& q, int mode, int32_t* lmark) {
     hipLaunchKernelGGL((k_flood<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev, q,
-                       mode);
+                       mode, lmark);
   }
-  static void labels(Pool* p, int grid) {
+  static void labels(Pool* p, int grid, const int32_t* list, int cnt) {
     hipLaunchKernelGGL((k_labels<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
-                       p->lsel, p->counts + CNT_SEL);
+                       list, p->counts + cnt);
   }
   static void reset_tiles(Pool* p, int grid) {
     hipLaunchKernelGGL((k_reset_tiles<T>), dim3(grid), dim3(256), 0, p->stream, p->dev, p->lfresh,
@@ -323,11 +323,11 @@ struct PoolKernels {
     hipLaunchKernelGGL((k_edit<T>), dim3(grid > 0 ? grid : 1), dim3(256), 0, p->stream, p->geo,
                        p->dev, img, n, idx, cs, ct, p->imgs[img].fresh ? 0 : 1);
   }
-  static void paint(Pool* p, int img, const Polyline& pl) {
-    // a wave per (segment, row), four per block
-    const int grid = std::min((pl.nrows + 3) / 4, 4096);
+  static void paint(Pool* p, const PaintJob* d_jobs, int njobs, int nrows) {
+    // a wave per (stroke, segment, row), four per block
+    const int grid = std::min((nrows + 3) / 4, 8192);
     hipLaunchKernelGGL((k_paint<T>), dim3(grid > 0 ? grid : 1), dim3(256), 0, p->stream, p->geo,
-                       p->dev, img, pl, p->imgs[img].fresh ? 0 : 1);
+                       p->dev, d_jobs, njobs, nrows);
   }
   static void untile(Pool* p, int img, const int32_t* src, int32_t* out) {
     const PoolImage& im = p->imgs[img];
@@ -570,14 +570,14 @@ void pool_free(Pool* p) {
                   (void*)d.down,      d.labels, p->d_img,   p->tile_img,  p->sel,     p->fresh,
                   p->q,    p->dlist,  p->plist, p->alist,   p->counts,    p->scratch, p->d_prep,
                   p->d_flows,
-                  p->d_edit, p->d_rgbsum, p->d_rgb};
+                  p->d_edit, p->d_rgbsum, p->d_rgb, p->d_jobs};
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   p->graph_exec = nullptr;
   for (void* x : ptrs)
     if (x) (void)hipFree(x);
   if (p->d_bits) (void)hipFree(p->d_bits);
-  void* hp[] = {p->h_counts, p->h_prep, p->h_flows, p->h_tally, p->h_edit, p->h_bits};
+  void* hp[] = {p->h_counts, p->h_prep, p->h_flows, p->h_tally, p->h_edit, p->h_bits, p->h_jobs};
   for (void* x : hp)
     if (x) (void)hipHostFree(x);
   p->prof.release();
@@ -984,21 +984,79 @@ int make_polyline(int W, int H, int npts, const double* xy, double width, int32_
 
 int pool_paint(Pool* p, int img, int npts, const double* xy, double width, int32_t cs,
                int32_t ct) {
-  GCMI_TRY(check_img(p, img));
-  PoolImage& im = p->imgs[img];
-  if (!im.uploaded) {
-    set_error("stroke before upload");
+  const PaintReq r{img, npts, xy, width, cs, ct};
+  return pool_paint_many(p, 1, &r);
+}
+
+int pool_paint_many(Pool* p, int count, const PaintReq* reqs) {
+  if (count <= 0) return GCMI_OK;
+  if (!reqs) {
+    set_error("null stroke list");
     return GCMI_EINVAL;
   }
-  Polyline pl;
-  GCMI_TRY(make_polyline(im.W, im.H, npts, xy, width, cs, ct,
-                         im.max_pair >= 0 ? im.max_pair : (1 << 27), &pl));
-  if (pl.bw == 0) return GCMI_OK;
-  p->ops->paint(p, img, pl);
-  GCMI_HIP(hipGetLastError());
-  if (!im.fresh) im.edited = true;
-  im.solved = false;
-  return GCMI_OK;
+  // check every stroke first, so that a refused one paints nothing
+  std::vector<Polyline> pls((size_t)count);
+  for (int i = 0; i < count; ++i) {
+    const PaintReq& r = reqs[i];
+    GCMI_TRY(check_img(p, r.img));
+    const PoolImage& im = p->imgs[r.img];
+    if (!im.uploaded) {
+      set_error("stroke before upload");
+      return GCMI_EINVAL;
+    }
+    GCMI_TRY(make_polyline(im.W, im.H, r.npts, r.xy, r.width, r.cs, r.ct,
+                           im.max_pair >= 0 ? im.max_pair : (1 << 27), &pls[i]));
+  }
+  if (p->jobs_cap < count) {
+    GCMI_HIP(hipStreamSynchronize(p->stream));
+    if (p->h_jobs) (void)hipHostFree(p->h_jobs);
+    if (p->d_jobs) (void)hipFree(p->d_jobs);
+    p->h_jobs = nullptr;
+    p->d_jobs = nullptr;
+    p->jobs_cap = 0;
+    const int cap = std::max(count, 64);
+    if (hipHostMalloc((void**)&p->h_jobs, (size_t)cap * sizeof(PaintJob), hipHostMallocDefault) !=
+            hipSuccess ||
+        hipMalloc((void**)&p->d_jobs, (size_t)cap * sizeof(PaintJob)) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("paint staging allocation failed");
+      return GCMI_ENOMEM;
+    }
+    p->jobs_cap = cap;
+  }
+  // the staging may still be read by a previous launch's copy
+  GCMI_HIP(hipStreamSynchronize(p->stream));
+  std::vector<char> seen((size_t)p->cap_imgs, 0);
+  int m = 0, base = 0;
+  long long rows = 0;
+  auto launch = [&]() -> int {
+    if (m == base) return GCMI_OK;
+    GCMI_HIP(hipMemcpyAsync(p->d_jobs + base, p->h_jobs + base, (size_t)(m - base) * sizeof(PaintJob),
+                            hipMemcpyHostToDevice, p->stream));
+    p->ops->paint(p, p->d_jobs + base, m - base, (int)rows);
+    GCMI_HIP(hipGetLastError());
+    std::fill(seen.begin(), seen.end(), 0);
+    base = m;
+    rows = 0;
+    return GCMI_OK;
+  };
+  for (int i = 0; i < count; ++i) {
+    const Polyline& pl = pls[i];
+    if (pl.bw == 0 || pl.nrows == 0) continue;
+    // one launch paints each image once; rows are counted in int
+    if (seen[reqs[i].img] || rows + pl.nrows > INT32_MAX) GCMI_TRY(launch());
+    seen[reqs[i].img] = 1;
+    PoolImage& im = p->imgs[reqs[i].img];
+    PaintJob& J = p->h_jobs[m++];
+    J.pl = pl;
+    J.img = reqs[i].img;
+    J.warm = im.fresh ? 0 : 1;
+    J.row0 = (int)rows;
+    rows += pl.nrows;
+    if (!im.fresh) im.edited = true;
+    im.solved = false;
+  }
+  return launch();
 }
 
 // Compaction of flags over the solve's tiles (sel), output list + count.
@@ -1239,6 +1297,10 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
   S.ops->posf(p, std::min((S.nsel + 255) / 256, 1024));
   const size_t cap = (size_t)p->cap_tiles;
   const bool inc = S.nfresh < S.nsel;
+  p->flood_inc = inc;
+  // (an incremental flood flags the tiles it processes in bflag, all zero
+  // since the last relabel's compaction, for their labels)
+  int32_t* lmark = inc ? p->dev.bflag : nullptr;
   if (inc) {
     // incremental: recompute the changed tiles and the tiles whose reach
     // may have come through them; every other tile keeps its reach bytes
@@ -1265,13 +1327,13 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
                     p->counts + CNT_EP + p->ep_par,
                     p->counts + CNT_EP + 1 - p->ep_par};
   p->ep_par ^= 1;
-  S.ops->flood(p, p->grid_cap, first, inc ? FLOOD_FIRST_INC : FLOOD_FIRST);
+  S.ops->flood(p, p->grid_cap, first, inc ? FLOOD_FIRST_INC : FLOOD_FIRST, lmark);
   S.E();
   GCMI_HIP(hipGetLastError());
   int cur = 0;
   for (int b = 0; b < rounds_dd; ++b) {
     S.B(K_FLOOD);
-    S.ops->flood(p, p->grid_cap, round_queue(p, cur), FLOOD_ROUND);
+    S.ops->flood(p, p->grid_cap, round_queue(p, cur), FLOOD_ROUND, lmark);
     S.E();
     GCMI_HIP(hipGetLastError());
     cur = (cur + 1) % 3;
@@ -1283,8 +1345,16 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
 // Labels, flows and tallies to the host buffers (no synchronisation).
 int phase_results(const Solve& S) {
   Pool* p = S.p;
-  S.B(K_LABELS);
-  S.ops->labels(p, S.grid_for(S.nsel));
+  if (p->flood_inc) {
+    // the labels of the tiles the flood processed (the others' reach, and
+    // so their labels, did not change); the active list is free by now
+    GCMI_TRY(compact_sel(p, p->dev.bflag, p->alist, CNT_ACT, true, S.nsel));
+    S.B(K_LABELS);
+    S.ops->labels(p, S.grid_for(S.nsel), p->alist, CNT_ACT);
+  } else {
+    S.B(K_LABELS);
+    S.ops->labels(p, S.grid_for(S.nsel), p->lsel, CNT_SEL);
+  }
   S.E();
   GCMI_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_flows, dim3(S.n), dim3(64), 0, p->stream, p->dev.sums, p->d_prep,
@@ -1305,7 +1375,8 @@ int flood_rounds_host(const Solve& S, int cur) {
   for (int batch = p->trace ? 1 : kRoundBatch;; batch = p->trace ? 1 : kRoundBatchMax) {
     for (int b = 0; b < batch; ++b) {
       S.B(K_FLOOD);
-      S.ops->flood(p, p->grid_cap, round_queue(p, cur), FLOOD_ROUND);
+      S.ops->flood(p, p->grid_cap, round_queue(p, cur), FLOOD_ROUND,
+                   p->flood_inc ? p->dev.bflag : nullptr);
       S.E();
       GCMI_HIP(hipGetLastError());
       cur = (cur + 1) % 3;

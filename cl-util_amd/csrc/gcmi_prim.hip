@@ -15,7 +15,8 @@ namespace gcmi {
 constexpr int kSingleNT = 1024;            // threads of the one-block kernels
 constexpr int kSingleMax = 1 << 16;        // one-block path up to this n
 constexpr int kMultiNT = 256;              // threads of the multi-block kernels
-constexpr int kMultiChunk = kMultiNT * 64; // elements per block (16384)
+constexpr int kMultiIT = 16;                    // elements per thread of a multi-block pass
+constexpr int kMultiChunk = kMultiNT * kMultiIT; // elements per block (4096): one step each
 
 __device__ __forceinline__ unsigned long long lanemask_lt() {
   const int lane = lane_id();
@@ -191,13 +192,17 @@ __global__ void __launch_bounds__(kMultiNT) k_compact_count(Pred pred, int n, in
   __shared__ int wsum[kMultiNT / kWave];
   const int lo = blockIdx.x * kMultiChunk;
   const int hi = min(n, lo + kMultiChunk);
-  int s = 0;
-  for (int base = lo; base < hi; base += kMultiNT) {
-    const int i = base + threadIdx.x;
-    const bool f = i < hi && pred.eval(i);
-    const unsigned long long m = __ballot(f);
-    s += __popcll(m);
+  // every predicate of the chunk issued before any is used (the gathers of
+  // a selection-list predicate overlap)
+  bool f[kMultiIT];
+#pragma unroll
+  for (int j = 0; j < kMultiIT; ++j) {
+    const int i = lo + j * kMultiNT + threadIdx.x;
+    f[j] = i < hi && pred.eval(i);
   }
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < kMultiIT; ++j) s += __popcll(__ballot(f[j]));
   if (lane_id() == 0) wsum[threadIdx.x / kWave] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -212,7 +217,7 @@ __global__ void __launch_bounds__(kMultiNT) k_compact_write(Pred pred, int n, co
   __shared__ int wcnt[kMultiNT / kWave];
   const int lo = blockIdx.x * kMultiChunk;
   const int hi = min(n, lo + kMultiChunk);
-  block_compact_range<kMultiNT, 16>(pred, lo, hi, part[blockIdx.x], list, clear, wcnt);
+  block_compact_range<kMultiNT, kMultiIT>(pred, lo, hi, part[blockIdx.x], list, clear, wcnt);
 }
 
 // ------------------------------------------------------------ tile kernels

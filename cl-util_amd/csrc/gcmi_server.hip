@@ -149,23 +149,19 @@ int serve_batch(gcmi_server* s, std::vector<Request*>& batch, hipEvent_t done) {
   std::vector<int> imgs;
   imgs.reserve(batch.size());
   std::vector<EditReq> edits;
+  std::vector<PaintReq> paints;
+  std::vector<Request*> images;
   for (Request* r : batch) {
     const Session& ss = s->sessions[r->session];
-    const size_t npx = (size_t)ss.W * ss.H;
     PoolImage& im = p->imgs[ss.img];
-    if (r->kind == REQ_IMAGE) {
-      const int32_t* d = s->d_stage + r->stage;
-      GCMI_TRY(pool_upload(p, ss.img, d, d + npx, d + 2 * npx, d + 3 * npx, true));
-      im.max_pair = r->max_pair;
-    } else if (r->kind == REQ_RGB) {
-      GCMI_TRY(pool_upload_rgb(p, ss.img, (const uint8_t*)(s->d_stage + r->stage), r->n,
-                               &r->model));
+    if (r->kind == REQ_IMAGE || r->kind == REQ_RGB) {
+      images.push_back(r);
     } else {
       // plain and cold modes cut every request from scratch: the edit only
       // rewrites the unaries
       if (s->mode != GCMI_SERVE_WARM) im.fresh = true;
       if (r->kind == REQ_POLYLINE) {
-        GCMI_TRY(pool_paint(p, ss.img, r->n, r->xy.data(), r->width, r->pcs, r->pct));
+        paints.push_back(PaintReq{ss.img, r->n, r->xy.data(), r->width, r->pcs, r->pct});
       } else {
         const int32_t* d = r->stroke.data();
         edits.push_back(EditReq{ss.img, r->n, d, d + r->n, d + 2 * r->n});
@@ -173,7 +169,23 @@ int serve_batch(gcmi_server* s, std::vector<Request*>& batch, hipEvent_t done) {
     }
     imgs.push_back(ss.img);
   }
+  // strokes first (their staging waits for the stream, idle until now), one
+  // launch for all polylines; then the new images (one request per session,
+  // so no image is both edited and replaced)
+  if (!paints.empty()) GCMI_TRY(pool_paint_many(p, (int)paints.size(), paints.data()));
   if (!edits.empty()) GCMI_TRY(pool_edit_many(p, (int)edits.size(), edits.data()));
+  for (Request* r : images) {
+    const Session& ss = s->sessions[r->session];
+    const size_t npx = (size_t)ss.W * ss.H;
+    if (r->kind == REQ_IMAGE) {
+      const int32_t* d = s->d_stage + r->stage;
+      GCMI_TRY(pool_upload(p, ss.img, d, d + npx, d + 2 * npx, d + 3 * npx, true));
+      p->imgs[ss.img].max_pair = r->max_pair;
+    } else {
+      GCMI_TRY(pool_upload_rgb(p, ss.img, (const uint8_t*)(s->d_stage + r->stage), r->n,
+                               &r->model));
+    }
+  }
   const int64_t t1 = now_ns();
   GCMI_TRY(pool_solve(p, imgs.data(), (int)imgs.size(), s->sweeps, s->bfs, 0, nullptr));
   const int64_t t2 = now_ns();
