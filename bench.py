@@ -297,11 +297,12 @@ def serve_leg(args):
     from GraphCutServer import GraphCutServer
     log("served: generating the 256-session mix")
     mix = S.Mix(nsess=256, seed=4)
-    modes = {"plain": (dict(mode="plain"), (1000, 1500, 2000, 2500)),
+    modes = {"plain": (dict(mode="plain"), (1500, 2000, 2500)),
              "cold": (dict(mode="cold", device_rounds=8), (2000, 3000, 4000, 6000)),
              "batched": (dict(mode="batched", local_relabel=True, device_rounds=8),
-                         (4000, 6000, 7000, 8000, 10000))}
-    stroke_only = {"plain": (1500, 2000, 2500, 3000), "batched": (8000, 10000, 13000, 16000)}
+                         (8000, 10000, 12000, 14000))}
+    stroke_only = {"plain": (1500, 2000, 2500, 3000),
+                   "batched": (16000, 20000, 25000, 30000)}
     out = {"workload": "256 sessions, sizes {512,1024,2048,4096}^2 weights 4:3:2:1 (seed 4), "
                        "Poisson arrivals, 10%% new images (8-bit colour, capacities built on "
                        "the device) / 90%% polyline strokes, %.1f s per rate"

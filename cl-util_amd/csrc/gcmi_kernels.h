@@ -80,6 +80,8 @@ struct CutDev {
                                        // tile's distances were last computed
   int32_t* rflag;                      // tile being recomputed by a local relabel
   int32_t* haspos;                     // tile holds excess
+  uint32_t* packed;                    // per pool pixel: its label as one bit (row-major,
+                                       // each image from a word of its own)
   int32_t* fchg;                       // arcs or excess changed since the tile's reach was
                                        // last computed
   int32_t* imp;                        // the tile's reach is not all reached from its own
@@ -1179,13 +1181,40 @@ __global__ void k_posf(Geo g, CutDev s, const int32_t* __restrict__ list,
   }
 }
 
+// Writes bits `b` under `m` (bit j: pixel bitpos + j, T <= 64 of them) into
+// a bit-packed mask.  A word wholly under the mask belongs to this row
+// segment alone and is stored; a partial one is merged atomically (other
+// segments own its other bits).
+__device__ __forceinline__ void put_bits(uint32_t* packed, long long bitpos,
+                                         unsigned long long b, unsigned long long m) {
+  uint32_t* w = packed + (bitpos >> 5);
+  const int off = (int)(bitpos & 31);
+  const unsigned long long mlo = m << off, blo = b << off;
+  const uint32_t mw[3] = {(uint32_t)mlo, (uint32_t)(mlo >> 32),
+                          off ? (uint32_t)(m >> (64 - off)) : 0u};
+  const uint32_t bw[3] = {(uint32_t)blo, (uint32_t)(blo >> 32),
+                          off ? (uint32_t)(b >> (64 - off)) : 0u};
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    if (mw[j] == 0xffffffffu) {
+      w[j] = bw[j];
+    } else if (mw[j]) {
+      atomicAnd(&w[j], ~mw[j]);
+      if (bw[j] & mw[j]) atomicOr(&w[j], bw[j] & mw[j]);
+    }
+  }
+}
+
 // Row-major labels of the listed tiles' images from the reach bytes (a
-// tile writes its own T x T block of its image).
+// tile writes its own T x T block of its image), as bytes and as bits of
+// the bit-packed masks (a tile row's bits gathered by one ballot).
 template <int T, int NT>
 __global__ void __launch_bounds__(NT)
 k_labels(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count) {
   constexpr int R = T * T / NT;
+  constexpr unsigned long long seg = T == 64 ? ~0ull : ((1ull << T) - 1);
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
+  const int sh = lane_id() & ~(T - 1) & 63;  // this row segment's first lane
   const int n = *count;
   for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
     const TG q = tile_geo(g, list ? list[pos] : pos);
@@ -1194,7 +1223,13 @@ k_labels(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __res
     for (int k = 0; k < R; ++k) {
       const int ly = chain * R + k;
       const int x = q.tx * T + lx, y = q.ty * T + ly;
-      if (x < q.W && y < q.H) s.labels[q.pix0 + (size_t)y * q.W + x] = s.reach[base + ly * T + lx];
+      const bool in = x < q.W && y < q.H;
+      const uint8_t v = s.reach[base + ly * T + lx];
+      if (in) s.labels[q.pix0 + (size_t)y * q.W + x] = v;
+      const unsigned long long bits = (__ballot(in && v) >> sh) & seg;
+      const unsigned long long mask = (__ballot(in) >> sh) & seg;
+      if (lx == 0 && mask)
+        put_bits(s.packed, q.pix0 + (long long)y * q.W + q.tx * T, bits, mask);
     }
   }
 }

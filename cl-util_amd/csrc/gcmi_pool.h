@@ -99,9 +99,10 @@ struct PoolOps {
 
 const PoolOps* pool_ops(int tile);
 
-// Row-major pixel ranges start on a multiple of kPixAlign elements, so the
-// rows of an image whose width is a multiple of 4 load as 16-byte vectors.
-constexpr long long kPixAlign = 4;
+// Row-major pixel ranges start on a multiple of kPixAlign elements: the
+// rows of an image whose width is a multiple of 4 load as 16-byte vectors,
+// and an image's bit-packed mask starts at bit 0 of a word of its own.
+constexpr long long kPixAlign = 32;
 inline long long pix_len(int W, int H) {
   return ((long long)W * H + kPixAlign - 1) / kPixAlign * kPixAlign;
 }
@@ -188,8 +189,6 @@ struct Pool {
   long long* h_flows = nullptr;
   unsigned long long* h_tally = nullptr;
   int32_t* h_edit = nullptr;
-  uint32_t *d_bits = nullptr, *h_bits = nullptr;  // packed-label staging
-  long long bits_cap = 0;                           // words
   // host bookkeeping
   std::vector<PoolImage> imgs;
   RangeAlloc tiles_alloc, pix_alloc;
@@ -262,9 +261,9 @@ int pool_solve(Pool* p, const int* img_ids, int n, int sweeps, int bfs, int64_t 
 int pool_labels(Pool* p, int img, uint8_t* out, bool device_dst);
 // Labels packed to bits (bit p % 32 of word p / 32 = label of pixel p).
 int pool_labels_packed(Pool* p, int img, uint32_t* out);
-// Launches the packing of an image's labels into d_out (ceil(W*H/32) words)
-// on `stream` (null: the pool's stream).
-int pack_labels_async(Pool* p, int img, uint32_t* d_out, hipStream_t stream = nullptr);
+// The device address of an image's bit-packed mask (ceil(W*H/32) words, bit
+// p % 32 of word p / 32 = label of pixel p), kept up to date by every solve.
+const uint32_t* packed_mask(const Pool* p, int img);
 // Drops the solved state: the next solve starts from the capacities.
 int pool_forget(Pool* p, int img);
 // GCMI_OPT_* options of the C ABI.

@@ -427,7 +427,7 @@ int tiles_for(int W, int H, int tile, int* tw, int* th) {
 size_t pool_bytes(int tile, int cap_tiles, long long cap_pix, int cap_imgs) {
   const size_t np = (size_t)cap_tiles * tile * tile;
   return np * (6 * 4 + 2) + (size_t)cap_tiles * (4 * tile * 4 + 16 * 4 + 8) +
-         (size_t)cap_pix * (4 * 4 + 1) +
+         (size_t)cap_pix * (4 * 4 + 1) + (size_t)cap_pix / 8 +
          (size_t)cap_imgs *
              (sizeof(ImgDesc) + SUM_ROWS * kSumSlots * 8 + sizeof(Prep) + 8 + kPixAlign * 17) +
          (size_t)compact_scratch_ints(cap_tiles) * 4 + 4096;
@@ -500,6 +500,7 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   ALLOC(d.dirty, (size_t)cap_tiles * 4);
   ALLOC(d.rflag, (size_t)cap_tiles * 4);
   ALLOC(d.haspos, (size_t)cap_tiles * 4);
+  ALLOC(d.packed, (size_t)cap_pix / 8);
   ALLOC(d.fchg, (size_t)cap_tiles * 4);
   ALLOC(d.imp, (size_t)cap_tiles * 4);
   ALLOC(d.tpos, (size_t)cap_tiles * 8);
@@ -565,7 +566,7 @@ void pool_free(Pool* p) {
   CutDev& d = p->dev;
   void* ptrs[] = {d.e,     d.h,       d.cu,     d.cd,       d.cl,         d.cr,       d.mask,
                   d.reach, d.pend,    d.pflag,  d.act,      d.stamp,      d.bflag,    d.dirty,
-                  d.rflag, d.haspos,  d.fchg,   d.imp,
+                  d.rflag, d.haspos,  d.fchg,   d.imp,      d.packed,
                   d.tpos,  d.sums,    d.tally,  d.cs,       d.ct,         (void*)d.right,
                   (void*)d.down,      d.labels, p->d_img,   p->tile_img,  p->sel,     p->fresh,
                   p->q,    p->dlist,  p->plist, p->alist,   p->counts,    p->scratch, p->d_prep,
@@ -576,8 +577,7 @@ void pool_free(Pool* p) {
   p->graph_exec = nullptr;
   for (void* x : ptrs)
     if (x) (void)hipFree(x);
-  if (p->d_bits) (void)hipFree(p->d_bits);
-  void* hp[] = {p->h_counts, p->h_prep, p->h_flows, p->h_tally, p->h_edit, p->h_bits, p->h_jobs};
+  void* hp[] = {p->h_counts, p->h_prep, p->h_flows, p->h_tally, p->h_edit, p->h_jobs};
   for (void* x : hp)
     if (x) (void)hipHostFree(x);
   p->prof.release();
@@ -622,6 +622,10 @@ int pool_add_image(Pool* p, int W, int H, int* out) {
   im.pix0 = p0;
   ImgDesc d{W, H, tw, th, (int)t0, (int)nt, p0};
   GCMI_HIP(hipMemcpyAsync(p->d_img + id, &d, sizeof d, hipMemcpyHostToDevice, p->stream));
+  // a clean packed mask (its last word's bits past W * H stay 0)
+  const long long words = ((long long)W * H + 31) / 32;
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)std::min<long long>((words + 255) / 256, 4096)),
+                     dim3(256), 0, p->stream, (int32_t*)(p->dev.packed + p0 / 32), words, 0);
   hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)std::min<long long>((nt + 255) / 256, 4096)),
                      dim3(256), 0, p->stream, p->tile_img + t0, nt, (int32_t)id);
   GCMI_HIP(hipGetLastError());
@@ -1637,26 +1641,8 @@ int pool_labels(Pool* p, int img, uint8_t* out, bool device_dst) {
   return GCMI_OK;
 }
 
-// One wave packs 64 labels into two words with a ballot; loads coalesce.
-__global__ void k_pack_bits(const uint8_t* __restrict__ labels, long long n, uint32_t* out) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  const bool on = i < n && labels[i] != 0;
-  const unsigned long long b = __ballot(on);
-  const long long w = (i >> 6) * 2;
-  const long long words = (n + 31) >> 5;
-  if ((threadIdx.x & 63) == 0 && w < words) {
-    out[w] = (uint32_t)b;
-    if (w + 1 < words) out[w + 1] = (uint32_t)(b >> 32);
-  }
-}
-
-int pack_labels_async(Pool* p, int img, uint32_t* d_out, hipStream_t stream) {
-  const PoolImage& im = p->imgs[img];
-  const long long n = (long long)im.W * im.H;
-  hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     stream ? stream : p->stream, p->dev.labels + im.pix0, n, d_out);
-  GCMI_HIP(hipGetLastError());
-  return GCMI_OK;
+const uint32_t* packed_mask(const Pool* p, int img) {
+  return p->dev.packed + p->imgs[img].pix0 / 32;
 }
 
 int pool_labels_packed(Pool* p, int img, uint32_t* out) {
@@ -1667,26 +1653,9 @@ int pool_labels_packed(Pool* p, int img, uint32_t* out) {
     return GCMI_EINVAL;
   }
   const long long words = ((long long)im.W * im.H + 31) / 32;
-  if (p->bits_cap < words) {
-    GCMI_HIP(hipStreamSynchronize(p->stream));
-    if (p->d_bits) (void)hipFree(p->d_bits);
-    if (p->h_bits) (void)hipHostFree(p->h_bits);
-    p->d_bits = nullptr;
-    p->h_bits = nullptr;
-    p->bits_cap = 0;
-    if (hipMalloc((void**)&p->d_bits, (size_t)words * 4) != hipSuccess ||
-        hipHostMalloc((void**)&p->h_bits, (size_t)words * 4, hipHostMallocDefault) != hipSuccess) {
-      (void)hipGetLastError();
-      set_error("packed-label staging allocation failed");
-      return GCMI_ENOMEM;
-    }
-    p->bits_cap = words;
-  }
-  GCMI_TRY(pack_labels_async(p, img, p->d_bits));
-  GCMI_HIP(hipMemcpyAsync(p->h_bits, p->d_bits, (size_t)words * 4, hipMemcpyDeviceToHost,
+  GCMI_HIP(hipMemcpyAsync(out, packed_mask(p, img), (size_t)words * 4, hipMemcpyDeviceToHost,
                           p->stream));
   GCMI_HIP(hipStreamSynchronize(p->stream));
-  memcpy(out, p->h_bits, (size_t)words * 4);
   return GCMI_OK;
 }
 

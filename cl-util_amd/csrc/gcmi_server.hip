@@ -108,7 +108,6 @@ struct gcmi_server {
   int sweeps = 0, bfs = 0;
   gcmi_rgb_model rgb_model{};  // of later colour-image requests
   std::vector<Session> sessions;
-  uint32_t* d_bits = nullptr;  // device staging for packed masks
   int32_t* d_stage = nullptr;  // device staging for new images
   RangeAlloc stage_alloc;
   hipStream_t copy_stream = nullptr;
@@ -189,19 +188,15 @@ int serve_batch(gcmi_server* s, std::vector<Request*>& batch, hipEvent_t done) {
   const int64_t t1 = now_ns();
   GCMI_TRY(pool_solve(p, imgs.data(), (int)imgs.size(), s->sweeps, s->bfs, 0, nullptr));
   const int64_t t2 = now_ns();
-  // (the solve has synchronised: the flows are on the host; the staging
-  // after the previous batch's copies on out_stream is free to reuse)
+  // (the solve has synchronised: the flows are on the host) the masks,
+  // kept packed on the device by the solve, are copied out after it
   GCMI_HIP(hipEventRecord(s->solved, p->stream));
   GCMI_HIP(hipStreamWaitEvent(s->out_stream, s->solved, 0));
-  long long off = 0;
   for (Request* r : batch) {
     const Session& ss = s->sessions[r->session];
-    const long long npx = (long long)ss.W * ss.H;
-    const long long words = (npx + 31) / 32;
-    GCMI_TRY(pack_labels_async(p, ss.img, s->d_bits + off, s->out_stream));
-    GCMI_HIP(hipMemcpyAsync(ss.bits, s->d_bits + off, words * 4, hipMemcpyDeviceToHost,
+    const long long words = ((long long)ss.W * ss.H + 31) / 32;
+    GCMI_HIP(hipMemcpyAsync(ss.bits, packed_mask(p, ss.img), words * 4, hipMemcpyDeviceToHost,
                             s->out_stream));
-    off += words;
   }
   GCMI_HIP(hipEventRecord(done, s->out_stream));
   for (Request* r : batch) r->flow = p->imgs[s->sessions[r->session].img].flow;
@@ -357,15 +352,15 @@ int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_ba
     return GCMI_EINVAL;
   }
   GCMI_TRY(require_device());
-  // a quarter of the budget (at most 1 GiB) stages new images; the pool and
-  // the packed-mask staging share the rest
+  // a quarter of the budget (at most 1 GiB) stages new images; the pool
+  // (its packed masks included) gets the rest
   const size_t stage_bytes = std::min<size_t>(hbm_budget / 4, size_t(1) << 30) & ~size_t(255);
   const long long T2 = (long long)tile * tile;
   const int probe = 1 << 12;
   const size_t b1 = pool_bytes(tile, probe, probe * T2, max_sessions);
   const size_t b2 = pool_bytes(tile, 2 * probe, 2 * probe * T2, max_sessions);
-  const double per_tile = (double)(b2 - b1) / probe + T2 / 8.0;
-  const double fixed = (double)b1 - probe * (per_tile - T2 / 8.0) + 4.0 * max_sessions;
+  const double per_tile = (double)(b2 - b1) / probe;
+  const double fixed = (double)b1 - probe * per_tile + 4.0 * max_sessions;
   const double rest = (double)hbm_budget - (double)stage_bytes - fixed;
   long long cap_tiles = rest > 0 ? (long long)(rest / per_tile) : 0;
   cap_tiles = std::min<long long>(cap_tiles, ((1LL << 30) - (1LL << 20)) / T2);
@@ -385,9 +380,7 @@ int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_ba
   const long long cap_pix = cap_tiles * T2;
   int rc = pool_init(&s->pool, tile, (int)cap_tiles, cap_pix, max_sessions, nullptr);
   if (rc == GCMI_OK) {
-    const long long words = cap_pix / 32 + max_sessions;
-    if (hipMalloc((void**)&s->d_bits, words * 4) != hipSuccess ||
-        hipMalloc((void**)&s->d_stage, stage_bytes) != hipSuccess ||
+    if (hipMalloc((void**)&s->d_stage, stage_bytes) != hipSuccess ||
         hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&s->out_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&s->solved, hipEventDisableTiming) != hipSuccess) {
@@ -397,7 +390,6 @@ int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_ba
     }
   }
   if (rc != GCMI_OK) {
-    if (s->d_bits) (void)hipFree(s->d_bits);
     if (s->d_stage) (void)hipFree(s->d_stage);
     if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     if (s->out_stream) (void)hipStreamDestroy(s->out_stream);
@@ -436,7 +428,6 @@ int gcmi_server_destroy(gcmi_server* s) {
   if (s->solved) (void)hipEventDestroy(s->solved);
   if (s->out_stream) (void)hipStreamDestroy(s->out_stream);
   if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
-  if (s->d_bits) (void)hipFree(s->d_bits);
   if (s->d_stage) (void)hipFree(s->d_stage);
   pool_free(&s->pool);
   delete s;
