@@ -67,6 +67,12 @@ def parse():
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--serve-modes", default="plain,cold,batched",
+                    help="serving modes to run (comma-separated)")
+    ap.add_argument("--serve-rates", default="",
+                    help="replace each mode's rate sweep (comma-separated requests/s)")
+    ap.add_argument("--serve-image-frac", type=float, default=0.1,
+                    help="share of new-image requests in the serving mix")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
 
@@ -304,18 +310,28 @@ def serve_leg(args):
     stroke_only = {"plain": (1500, 2000, 2500, 3000),
                    "batched": (16000, 20000, 25000, 30000)}
     out = {"workload": "256 sessions, sizes {512,1024,2048,4096}^2 weights 4:3:2:1 (seed 4), "
-                       "Poisson arrivals, 10%% new images (8-bit colour, capacities built on "
-                       "the device) / 90%% polyline strokes, %.1f s per rate"
-                       % args.serve_seconds,
+                       "Poisson arrivals, %g%% new images (8-bit colour, capacities built on "
+                       "the device) / the rest polyline strokes, %.1f s per rate"
+                       % (100 * args.serve_image_frac, args.serve_seconds),
            "resident_pixels": mix.pixels, "hbm_budget_bytes": 200 * 10**9, "modes": {},
            "stroke_only": {}}
+    only = [m for m in args.serve_modes.split(",") if m]
+    modes = {m: v for m, v in modes.items() if m in only}
+    if args.serve_rates:
+        rates = tuple(float(x) for x in args.serve_rates.split(","))
+        modes = {m: (kw, rates) for m, (kw, _) in modes.items()}
+        stroke_only = {}
     for mode, (kw, rs) in modes.items():
         log("served: mode %s" % mode)
         with GraphCutServer(tile=32, hbm_budget=200 * 10**9, max_sessions=256, max_batch=64,
                             **kw) as srv:
             sids = S.open_sessions(srv, mix, rgb=True)
+            # an unmeasured warm-up trace (first-use allocations, page-in)
+            S.run_trace(srv, sids, mix, rs[0], duration=0.5, seed=7,
+                        image_frac=args.serve_image_frac, rgb=True)
             pts = [S.run_trace(srv, sids, mix, r, duration=args.serve_seconds, seed=11 + k,
-                               rgb=True) for k, r in enumerate(rs)]
+                               image_frac=args.serve_image_frac, rgb=True)
+                   for k, r in enumerate(rs)]
             out["modes"][mode] = {"points": pts, "sustained_2048_p99_20ms": S.sustained(pts),
                                   "device_bytes": srv.stats()["device_bytes"]}
             if mode in stroke_only:

@@ -1252,10 +1252,11 @@ __global__ void k_reset_tiles(CutDev s, const int32_t* __restrict__ list,
 }
 
 // Per listed tile: clear the reach bytes, the import flag and the changed
-// flag (before a flood recomputes the tile).
+// flag (before a flood recomputes the tile); flag it in lmark (may be null)
+// for its labels.
 template <int T>
 __global__ void k_clear_reach(CutDev s, const int32_t* __restrict__ list,
-                              const int32_t* __restrict__ count) {
+                              const int32_t* __restrict__ count, int32_t* lmark) {
   const int n = *count;
   for (int pos = blockIdx.x; pos < n; pos += gridDim.x) {
     const int t = list ? list[pos] : pos;
@@ -1264,6 +1265,7 @@ __global__ void k_clear_reach(CutDev s, const int32_t* __restrict__ list,
     if (threadIdx.x == 0) {
       s.imp[t] = 0;
       s.fchg[t] = 0;
+      if (lmark) lmark[t] = 1;
     }
   }
 }

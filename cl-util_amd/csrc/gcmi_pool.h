@@ -32,7 +32,8 @@ enum {
   CNT_EP = 8,     // two alternating queue-epoch slots
   CNT_DONE = 10,  // a complete relabel found no active tile
   CNT_GSEL = 11,  // selected tiles, or 0 once the solve is done
-  CNT_N = 12
+  CNT_FIRST = 12,  // tiles of an incremental flood's first pass
+  CNT_N = 13
 };
 
 enum {
@@ -90,7 +91,7 @@ struct PoolOps {
   void (*flood)(Pool*, int grid, const Queue& q, int mode, int32_t* lmark);
   void (*labels)(Pool*, int grid, const int32_t* list, int cnt);
   void (*reset_tiles)(Pool*, int grid);
-  void (*clear_reach)(Pool*, int grid, const int32_t* list, int cnt);
+  void (*clear_reach)(Pool*, int grid, const int32_t* list, int cnt, int32_t* lmark);
   void (*posf)(Pool*, int grid);
   void (*edit)(Pool*, int img, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct);
   void (*paint)(Pool*, const PaintJob* d_jobs, int njobs, int nrows);
@@ -176,6 +177,7 @@ struct Pool {
   Prep* d_prep = nullptr;
   long long* d_flows = nullptr;
   int32_t* d_edit = nullptr;  // staging for edits: idx, cs, ct per edit
+  int32_t* d_imgfresh = nullptr;           // per image: cut from scratch in this solve
   unsigned long long* d_rgbsum = nullptr;  // pair-difference sum of a colour upload
   uint8_t* d_rgb = nullptr;                // staging for host colour images
   long long rgb_cap = 0;                   // bytes

@@ -164,8 +164,10 @@ __global__ void k_dilate(Geo g, CutDev s, const int32_t* __restrict__ list,
 // a fresh image, the final-excess row of every image) and the tallies, sets
 // the counts and the first queue epoch.
 __global__ void k_prepare(Geo g, CutDev s, const Prep* __restrict__ prep, int nsel_tiles,
-                          int nfresh_tiles, int32_t* sel, int32_t* fresh, int32_t* counts) {
+                          int nfresh_tiles, int32_t* sel, int32_t* fresh, int32_t* counts,
+                          int32_t* imgfresh) {
   const Prep p = prep[blockIdx.x];
+  if (threadIdx.x == 0) imgfresh[p.img] = p.fresh_off >= 0;
   const ImgDesc d = g.img[p.img];
   for (int j = threadIdx.x; j < d.ntiles; j += blockDim.x) {
     sel[p.sel_off + j] = d.tile0 + j;
@@ -256,9 +258,37 @@ k_rgb_caps(const uint8_t* __restrict__ rgb, int W, int H, int ch, gcmi_rgb_model
 
 // Zeroes the final-excess sums of the selected images: the first flood
 // pass adds them up (a run whose recording stopped short floods again).
-__global__ void k_zero_posf(unsigned long long* sums, const Prep* __restrict__ prep) {
+__global__ void k_zero_posf(unsigned long long* sums, const Prep* __restrict__ prep,
+                            int32_t* counts) {
   unsigned long long* r = sums + (size_t)prep[blockIdx.x].img * SUM_ROWS * kSumSlots;
   for (int j = threadIdx.x; j < kSumSlots; j += blockDim.x) r[SUM_POSF * kSumSlots + j] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) counts[CNT_FIRST] = 0;
+}
+
+// The first pass of an incremental flood: of the recompute set `list`, the
+// tiles of images reflowed warm (their neighbours may hold reach to import)
+// and the tiles holding excess of images cut from scratch (the others get
+// reach only from their own image's tiles, through the rounds).  Appended
+// to `out` in any order, count in counts[CNT_FIRST].
+__global__ void k_first_pass_list(Geo g, CutDev s, const int32_t* __restrict__ list,
+                                  const int32_t* __restrict__ count,
+                                  const int32_t* __restrict__ imgfresh, int32_t* out,
+                                  int32_t* counts) {
+  const int n = *count;
+  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const int i = base + threadIdx.x;
+    bool keep = false;
+    int t = 0;
+    if (i < n) {
+      t = list[i];
+      keep = s.haspos[t] || !imgfresh[g.tile_img[t]];
+    }
+    const unsigned long long m = __ballot(keep);
+    int at = 0;
+    if (lane_id() == 0 && m) at = atomicAdd(&counts[CNT_FIRST], __popcll(m));
+    at = __shfl(at, 0);
+    if (keep) out[at + __popcll(m & ((1ull << lane_id()) - 1))] = t;
+  }
 }
 
 // Flow value of each selected image: sum min(cs, ct) + sum max(e0, 0)
@@ -309,9 +339,9 @@ struct PoolKernels {
     hipLaunchKernelGGL((k_reset_tiles<T>), dim3(grid), dim3(256), 0, p->stream, p->dev, p->lfresh,
                        p->counts + CNT_FRESH);
   }
-  static void clear_reach(Pool* p, int grid, const int32_t* list, int cnt) {
+  static void clear_reach(Pool* p, int grid, const int32_t* list, int cnt, int32_t* lmark) {
     hipLaunchKernelGGL((k_clear_reach<T>), dim3(grid), dim3(256), 0, p->stream, p->dev, list,
-                       p->counts + cnt);
+                       p->counts + cnt, lmark);
   }
   static void posf(Pool* p, int grid) {
     hipLaunchKernelGGL((k_posf<T>), dim3(grid), dim3(256), 0, p->stream, p->geo, p->dev, p->lsel,
@@ -525,6 +555,7 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   ALLOC(p->alist, (size_t)cap_tiles * 4);
   ALLOC(p->counts, CNT_N * 4);
   ALLOC(p->d_rgbsum, 8);
+  ALLOC(p->d_imgfresh, (size_t)cap_imgs * 4);
   ALLOC(p->scratch, (size_t)compact_scratch_ints(cap_tiles) * 4);
   ALLOC(p->d_prep, (size_t)cap_imgs * sizeof(Prep));
   ALLOC(p->d_flows, (size_t)cap_imgs * 8);
@@ -571,7 +602,7 @@ void pool_free(Pool* p) {
                   (void*)d.down,      d.labels, p->d_img,   p->tile_img,  p->sel,     p->fresh,
                   p->q,    p->dlist,  p->plist, p->alist,   p->counts,    p->scratch, p->d_prep,
                   p->d_flows,
-                  p->d_edit, p->d_rgbsum, p->d_rgb, p->d_jobs};
+                  p->d_edit, p->d_rgbsum, p->d_rgb, p->d_jobs, p->d_imgfresh};
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   p->graph_exec = nullptr;
@@ -1147,7 +1178,7 @@ int phase_prepare(const Solve& S) {
   GCMI_HIP(hipMemcpyAsync(p->d_prep, p->h_prep, (size_t)S.n * sizeof(Prep),
                           hipMemcpyHostToDevice, p->stream));
   hipLaunchKernelGGL(k_prepare, dim3(S.n), dim3(256), 0, p->stream, p->geo, p->dev, p->d_prep,
-                     S.nsel, S.nfresh, p->sel, p->fresh, p->counts);
+                     S.nsel, S.nfresh, p->sel, p->fresh, p->counts, p->d_imgfresh);
   GCMI_HIP(hipGetLastError());
   p->ep_par = 0;
   return GCMI_OK;
@@ -1297,7 +1328,8 @@ int phase_extra_pushes(const Solve& S, bool* any) {
 int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
   Pool* p = S.p;
   // the images' final sum of max(e, 0)
-  hipLaunchKernelGGL(k_zero_posf, dim3(S.n), dim3(64), 0, p->stream, p->dev.sums, p->d_prep);
+  hipLaunchKernelGGL(k_zero_posf, dim3(S.n), dim3(64), 0, p->stream, p->dev.sums, p->d_prep,
+                     p->counts);
   S.ops->posf(p, std::min((S.nsel + 255) / 256, 1024));
   const size_t cap = (size_t)p->cap_tiles;
   const bool inc = S.nfresh < S.nsel;
@@ -1313,18 +1345,21 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
     hipLaunchKernelGGL(k_imp_closure, dim3(1), dim3(1024), 0, p->stream, p->geo, p->dev, p->q,
                        p->counts + CNT_Q, p->q + cap, p->q + 2 * cap);
     GCMI_TRY(compact_sel(p, p->dev.fchg, p->plist, CNT_POS, true, S.nsel));
-    S.ops->clear_reach(p, std::min(S.nsel, 4096), p->plist, CNT_POS);
+    S.ops->clear_reach(p, std::min(S.nsel, 4096), p->plist, CNT_POS, lmark);
+    hipLaunchKernelGGL(k_first_pass_list, dim3(std::min((S.nsel + 255) / 256, 2048)), dim3(256),
+                       0, p->stream, p->geo, p->dev, p->plist, p->counts + CNT_POS,
+                       p->d_imgfresh, p->alist, p->counts);
   } else {
     // from scratch: the tiles holding excess seed the first pass (haspos
     // persists: a clean tile keeps the flag its last relabel wrote)
     GCMI_TRY(compact_sel(p, p->dev.haspos, p->plist, CNT_POS, false, S.nsel));
-    S.ops->clear_reach(p, std::min(S.nsel, 4096), p->lsel, CNT_SEL);
+    S.ops->clear_reach(p, std::min(S.nsel, 4096), p->lsel, CNT_SEL, nullptr);
   }
   GCMI_HIP(hipGetLastError());
   clear_queues(p);
   S.B(K_FLOOD);
-  const Queue first{p->plist,
-                    p->counts + CNT_POS,
+  const Queue first{inc ? p->alist : p->plist,
+                    p->counts + (inc ? CNT_FIRST : CNT_POS),
                     p->q,
                     p->counts + CNT_Q,
                     nullptr,
