@@ -1367,6 +1367,7 @@ k_edit(Geo g, CutDev s, int img, int n, const int32_t* __restrict__ idx,
 // A brush stroke: the pixels whose centre (x, y) lies within width/2 of a
 // polyline -- a union of capsules, one per segment -- get unaries (cs, ct).
 constexpr int kMaxStrokePoints = 16;
+constexpr int kPaintRows = 8;  // rows of one segment painted by one wave
 struct Polyline {
   double x[kMaxStrokePoints], y[kMaxStrokePoints];
   double r;  // width / 2
@@ -1375,9 +1376,10 @@ struct Polyline {
   int bx0, by0, bw, bh;  // pixel box covering the stroke, clipped to the image
   int W;                 // image width
   // rows of the image each segment's capsule can reach: segment k covers
-  // rows ya[k] .. ya[k] + (row0[k + 1] - row0[k]) - 1; nrows in all
-  int ya[kMaxStrokePoints], row0[kMaxStrokePoints];
-  int nrows;
+  // nr[k] rows from ya[k]; they form chunks of kPaintRows rows, segment k's
+  // from chunk c0[k] of the stroke; nchunks in all
+  int ya[kMaxStrokePoints], nr[kMaxStrokePoints], c0[kMaxStrokePoints];
+  int nchunks;
   // pixel box of each segment's capsule (a pixel outside it is not in it)
   int sx0[kMaxStrokePoints], sx1[kMaxStrokePoints], sy0[kMaxStrokePoints],
       sy1[kMaxStrokePoints];
@@ -1436,56 +1438,69 @@ __device__ __forceinline__ bool seg_row_span(const Polyline& pl, int k, double y
 }
 
 // A stroke of a paint launch: its polyline, image, whether the image has
-// solved state to reflow (warm), and the stroke's first row among all the
-// launch's (segment, row) items.
+// solved state to reflow (warm), and the stroke's first chunk among all the
+// launch's (stroke, segment, chunk of rows) items.
 struct PaintJob {
   Polyline pl;
   int img;
   int warm;
-  int row0;
+  int chunk0;
 };
 
-// Paints the strokes of `jobs` (row0 ascending): one wave per (stroke,
-// segment, row) a segment's capsule can reach, its lanes over the row's
-// candidate columns.  A pixel is painted by the first segment of its stroke
-// whose capsule holds it, so every pixel of a stroke's union is edited once
-// (strokes of one launch are of different images).
+constexpr int kPaintJobsLds = 256;  // jobs whose first chunks a block stages in LDS
+
+// Paints the strokes of `jobs` (chunk0 ascending): one wave per (stroke,
+// segment, kPaintRows rows) a segment's capsule can reach, its lanes over
+// each row's candidate columns.  A pixel is painted by the first segment of
+// its stroke whose capsule holds it, so every pixel of a stroke's union is
+// edited once (strokes of one launch are of different images).
 template <int T>
 __global__ void __launch_bounds__(256)
-k_paint(Geo g, CutDev s, const PaintJob* __restrict__ jobs, int njobs, int nrows) {
+k_paint(Geo g, CutDev s, const PaintJob* __restrict__ jobs, int njobs, int nchunks) {
+  __shared__ int sc0[kPaintJobsLds];
+  const bool lds = njobs <= kPaintJobsLds;
+  if (lds)
+    for (int j = threadIdx.x; j < njobs; j += blockDim.x) sc0[j] = jobs[j].chunk0;
+  __syncthreads();
   const int lane = threadIdx.x % kWave, waves = blockDim.x / kWave;
-  for (int item = blockIdx.x * waves + threadIdx.x / kWave; item < nrows;
+  for (int item = blockIdx.x * waves + threadIdx.x / kWave; item < nchunks;
        item += gridDim.x * waves) {
     int lo = 0, hi = njobs - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (jobs[mid].row0 <= item) lo = mid;
+      if ((lds ? sc0[mid] : jobs[mid].chunk0) <= item) lo = mid;
       else hi = mid - 1;
     }
     const PaintJob& J = jobs[lo];
     const Polyline& pl = J.pl;
-    const int local = item - J.row0;
-    int k = 0;
-    while (k + 2 < pl.npts && local >= pl.row0[k + 1]) ++k;
-    const int y = pl.ya[k] + (local - pl.row0[k]);
-    double xl, xr;
-    if (!seg_row_span(pl, k, (double)y, &xl, &xr)) continue;
-    xl = fmax(xl, 0.0);
-    xr = fmin(xr, (double)(pl.W - 1));
-    if (xl > xr) continue;
+    const int local = item - J.chunk0;
+    // the segment: the last one whose first chunk is <= local (one load per
+    // lane, then a ballot)
+    const int nseg = pl.npts - 1;
+    const unsigned long long le = __ballot(lane < nseg && pl.c0[lane] <= local);
+    const int k = 63 - __clzll(le);
+    const int yfirst = pl.ya[k] + (local - pl.c0[k]) * kPaintRows;
+    const int ylast = min(yfirst + kPaintRows, pl.ya[k] + pl.nr[k]);
     const ImgDesc d = g.img[J.img];
     long long dmin = 0, dpos = 0;
-    const int xa = (int)floor(xl), xb = (int)ceil(xr);
-    for (int x = xa + lane; x <= xb; x += kWave) {
-      if (!in_capsule(pl, k, (double)x, (double)y)) continue;
-      bool earlier = false;
-      for (int j = 0; j < k && !earlier; ++j)
-        earlier = x >= pl.sx0[j] && x <= pl.sx1[j] && y >= pl.sy0[j] && y <= pl.sy1[j] &&
-                  in_capsule(pl, j, (double)x, (double)y);
-      if (!earlier) edit_pixel<T>(s, d, x, y, pl.cs, pl.ct, J.warm, dmin, dpos);
+    for (int y = yfirst; y < ylast; ++y) {
+      double xl, xr;
+      if (!seg_row_span(pl, k, (double)y, &xl, &xr)) continue;
+      xl = fmax(xl, 0.0);
+      xr = fmin(xr, (double)(pl.W - 1));
+      if (xl > xr) continue;
+      const int xa = (int)floor(xl), xb = (int)ceil(xr);
+      for (int x = xa + lane; x <= xb; x += kWave) {
+        if (!in_capsule(pl, k, (double)x, (double)y)) continue;
+        bool earlier = false;
+        for (int j = 0; j < k && !earlier; ++j)
+          earlier = x >= pl.sx0[j] && x <= pl.sx1[j] && y >= pl.sy0[j] && y <= pl.sy1[j] &&
+                    in_capsule(pl, j, (double)x, (double)y);
+        if (!earlier) edit_pixel<T>(s, d, x, y, pl.cs, pl.ct, J.warm, dmin, dpos);
+      }
     }
     if (J.warm) {
-      // the row's changes of the image's sums (one slotted atomic per wave)
+      // the chunk's changes of the image's sums (one slotted atomic per wave)
       dmin = wave_sum64(dmin);
       dpos = wave_sum64(dpos);
       if (lane == 0) {

@@ -353,11 +353,11 @@ struct PoolKernels {
     hipLaunchKernelGGL((k_edit<T>), dim3(grid > 0 ? grid : 1), dim3(256), 0, p->stream, p->geo,
                        p->dev, img, n, idx, cs, ct, p->imgs[img].fresh ? 0 : 1);
   }
-  static void paint(Pool* p, const PaintJob* d_jobs, int njobs, int nrows) {
-    // a wave per (stroke, segment, row), four per block
-    const int grid = std::min((nrows + 3) / 4, 8192);
+  static void paint(Pool* p, const PaintJob* d_jobs, int njobs, int nchunks) {
+    // a wave per (stroke, segment, chunk of rows), four per block
+    const int grid = std::min((nchunks + 3) / 4, 8192);
     hipLaunchKernelGGL((k_paint<T>), dim3(grid > 0 ? grid : 1), dim3(256), 0, p->stream, p->geo,
-                       p->dev, d_jobs, njobs, nrows);
+                       p->dev, d_jobs, njobs, nchunks);
   }
   static void untile(Pool* p, int img, const int32_t* src, int32_t* out) {
     const PoolImage& im = p->imgs[img];
@@ -989,7 +989,8 @@ int make_polyline(int W, int H, int npts, const double* xy, double width, int32_
   pl.bw = (int)(bx1 - bx0 + 1);
   pl.bh = (int)(by1 - by0 + 1);
   pl.W = W;
-  // rows each segment's capsule can reach (as the box, per segment)
+  // rows each segment's capsule can reach (as the box, per segment), in
+  // chunks of kPaintRows
   long long total = 0;
   for (int k = 0; k + 1 < npts; ++k) {
     const double ylo = std::min(pl.y[k], pl.y[k + 1]) - pl.r;
@@ -1001,19 +1002,20 @@ int make_polyline(int W, int H, int npts, const double* xy, double width, int32_
     const bool cols = (long long)std::ceil(xhi) >= 0 && (long long)std::floor(xlo) <= W - 1;
     const long long rows = (ya <= yb && cols) ? yb - ya + 1 : 0;
     pl.ya[k] = rows > 0 ? (int)ya : 0;
+    pl.nr[k] = (int)rows;
     // the capsule's pixel box, one pixel wider each way than needed
     pl.sx0[k] = (int)std::max<long long>((long long)std::floor(xlo) - 1, -1);
     pl.sx1[k] = (int)std::min<long long>((long long)std::ceil(xhi) + 1, W);
     pl.sy0[k] = (int)std::max<long long>((long long)std::floor(ylo) - 1, -1);
     pl.sy1[k] = (int)std::min<long long>((long long)std::ceil(yhi) + 1, H);
-    pl.row0[k] = (int)std::min<long long>(total, INT32_MAX);
-    total += rows;
+    pl.c0[k] = (int)std::min<long long>(total, INT32_MAX);
+    total += (rows + kPaintRows - 1) / kPaintRows;
   }
   if (total > INT32_MAX) {
     set_error("stroke spans too many rows");
     return GCMI_EINVAL;
   }
-  pl.nrows = (int)total;
+  pl.nchunks = (int)total;
   return GCMI_OK;
 }
 
@@ -1077,35 +1079,64 @@ int pool_paint_many(Pool* p, int count, const PaintReq* reqs) {
   };
   for (int i = 0; i < count; ++i) {
     const Polyline& pl = pls[i];
-    if (pl.bw == 0 || pl.nrows == 0) continue;
-    // one launch paints each image once; rows are counted in int
-    if (seen[reqs[i].img] || rows + pl.nrows > INT32_MAX) GCMI_TRY(launch());
+    if (pl.bw == 0 || pl.nchunks == 0) continue;
+    // one launch paints each image once; chunks are counted in int
+    if (seen[reqs[i].img] || rows + pl.nchunks > INT32_MAX) GCMI_TRY(launch());
     seen[reqs[i].img] = 1;
     PoolImage& im = p->imgs[reqs[i].img];
     PaintJob& J = p->h_jobs[m++];
     J.pl = pl;
     J.img = reqs[i].img;
     J.warm = im.fresh ? 0 : 1;
-    J.row0 = (int)rows;
-    rows += pl.nrows;
+    J.chunk0 = (int)rows;
+    rows += pl.nchunks;
     if (!im.fresh) im.edited = true;
     im.solved = false;
   }
   return launch();
 }
 
-// Compaction of flags over the solve's tiles (sel), output list + count.
-static int compact_sel(Pool* p, int32_t* flags, int32_t* out, int cnt, bool clear, int nsel) {
-  Pred pr{};
-  pr.t1 = flags;
-  pr.kind = Pred::NONZERO;
-  pr.ids = p->lsel;
+// The selected tiles (positions of `sel`, or tiles 0..n-1) whose flag is
+// set, appended to `out` in any order with one atomic per wave; *count must
+// be zero before.  clear: each listed flag is zeroed.  imgskip (may be
+// null): tiles of images flagged there are left out (and not cleared).
+__global__ void k_flag_list(const int32_t* __restrict__ sel, int n, int32_t* flags, int clear,
+                            int32_t* out, int32_t* count, const int32_t* __restrict__ tile_img,
+                            const int32_t* __restrict__ imgskip) {
+  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const int i = base + threadIdx.x;
+    int t = 0;
+    bool keep = false;
+    if (i < n) {
+      t = sel ? sel[i] : i;
+      keep = flags[t] != 0 && !(imgskip && imgskip[tile_img[t]]);
+    }
+    const unsigned long long m = __ballot(keep);
+    if (m == 0) continue;
+    int at = 0;
+    if (lane_id() == 0) at = atomicAdd(count, __popcll(m));
+    at = __shfl(at, 0);
+    if (keep) {
+      out[at + __popcll(m & ((1ull << lane_id()) - 1))] = t;
+      if (clear) flags[t] = 0;
+    }
+  }
+}
+
+// The solve's tiles (sel) whose flag is set (leaving out the images flagged
+// in imgskip, if given), as a list + count for the solver's own launches,
+// which do not depend on the list's order.
+static int compact_sel(Pool* p, int32_t* flags, int32_t* out, int cnt, bool clear, int nsel,
+                       const int32_t* imgskip = nullptr) {
   Prof* P = p->prof.on ? &p->prof : nullptr;
   if (P) P->begin(p->stream, K_COMPACT);
-  int rc = compact_async(pr, nsel, out, p->counts + cnt, clear ? flags : nullptr, p->scratch,
-                         p->stream);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, p->stream, p->counts + cnt, 1LL, 0);
+  hipLaunchKernelGGL(k_flag_list, dim3(std::max(1, std::min((nsel + 255) / 256, 2048))),
+                     dim3(256), 0, p->stream, p->lsel, nsel, flags, clear ? 1 : 0, out,
+                     p->counts + cnt, p->tile_img, imgskip);
   if (P) P->end(p->stream);
-  return rc;
+  GCMI_HIP(hipGetLastError());
+  return GCMI_OK;
 }
 
 // Zeroes the three queue counts (a fill kernel, so the solve holds no
@@ -1341,7 +1372,9 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
     // incremental: recompute the changed tiles and the tiles whose reach
     // may have come through them; every other tile keeps its reach bytes
     // (they can only grow, and the rounds grow them)
-    GCMI_TRY(compact_sel(p, p->dev.fchg, p->q, CNT_Q, false, S.nsel));
+    // (a fresh image's tiles are all in the set already, and so are their
+    // neighbours: the closure starts from the warm images' changed tiles)
+    GCMI_TRY(compact_sel(p, p->dev.fchg, p->q, CNT_Q, false, S.nsel, p->d_imgfresh));
     hipLaunchKernelGGL(k_imp_closure, dim3(1), dim3(1024), 0, p->stream, p->geo, p->dev, p->q,
                        p->counts + CNT_Q, p->q + cap, p->q + 2 * cap);
     GCMI_TRY(compact_sel(p, p->dev.fchg, p->plist, CNT_POS, true, S.nsel));
