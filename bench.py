@@ -306,9 +306,9 @@ def serve_leg(args):
     modes = {"plain": (dict(mode="plain"), (1500, 2000, 2500)),
              "cold": (dict(mode="cold", device_rounds=8), (2000, 3000, 4000, 6000)),
              "batched": (dict(mode="batched", local_relabel=True, device_rounds=8),
-                         (8000, 10000, 12000, 14000))}
+                         (10000, 12000, 14000, 16000, 18000))}
     stroke_only = {"plain": (1500, 2000, 2500, 3000),
-                   "batched": (16000, 20000, 25000, 30000)}
+                   "batched": (20000, 25000, 30000, 35000, 40000)}
     out = {"workload": "256 sessions, sizes {512,1024,2048,4096}^2 weights 4:3:2:1 (seed 4), "
                        "Poisson arrivals, %g%% new images (8-bit colour, capacities built on "
                        "the device) / the rest polyline strokes, %.1f s per rate"

@@ -256,14 +256,30 @@ void worker_main(gcmi_server* s) {
   for (;;) {
     s->cv_work.wait(lk, [&] { return s->stop || work_ready(s); });
     if (s->stop) break;
-    // oldest first, one request per session, at most max_batch
+    // oldest first, one request per session, at most max_batch.  In the
+    // warm mode a batch holds either new images (cut from scratch) or
+    // strokes (reflowed warm), the class of the oldest request it can take,
+    // so strokes are not held to the longer solve of a new image; a session
+    // whose next request is of the other class waits for a later batch.
     const int cap = s->mode == GCMI_SERVE_PLAIN ? 1 : s->max_batch;
     std::vector<Request*> batch;
+    int cls = -1;  // 1: new images, 0: strokes
+    std::vector<char> held;
     for (auto it = s->queue.begin(); it != s->queue.end() && (int)batch.size() < cap;) {
       Session& ss = s->sessions[(*it)->session];
-      if (ss.busy) {
+      if (ss.busy || (!held.empty() && held[(*it)->session])) {
         ++it;
         continue;
+      }
+      const int c = ((*it)->kind == REQ_IMAGE || (*it)->kind == REQ_RGB) ? 1 : 0;
+      if (s->mode == GCMI_SERVE_WARM) {
+        if (cls < 0) cls = c;
+        if (c != cls) {
+          if (held.empty()) held.assign(s->sessions.size(), 0);
+          held[(*it)->session] = 1;
+          ++it;
+          continue;
+        }
       }
       ss.busy = true;
       batch.push_back(*it);
