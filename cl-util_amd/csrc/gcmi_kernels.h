@@ -1367,7 +1367,7 @@ k_edit(Geo g, CutDev s, int img, int n, const int32_t* __restrict__ idx,
 // A brush stroke: the pixels whose centre (x, y) lies within width/2 of a
 // polyline -- a union of capsules, one per segment -- get unaries (cs, ct).
 constexpr int kMaxStrokePoints = 16;
-constexpr int kPaintRows = 8;  // rows of one segment painted by one wave
+constexpr int kPaintCols = 256;  // columns of one row of one segment painted by one wave
 struct Polyline {
   double x[kMaxStrokePoints], y[kMaxStrokePoints];
   double r;  // width / 2
@@ -1376,9 +1376,10 @@ struct Polyline {
   int bx0, by0, bw, bh;  // pixel box covering the stroke, clipped to the image
   int W;                 // image width
   // rows of the image each segment's capsule can reach: segment k covers
-  // nr[k] rows from ya[k]; they form chunks of kPaintRows rows, segment k's
-  // from chunk c0[k] of the stroke; nchunks in all
-  int ya[kMaxStrokePoints], nr[kMaxStrokePoints], c0[kMaxStrokePoints];
+  // nr[k] rows from ya[k], each in xc[k] pieces of kPaintCols columns (its
+  // box's width); segment k's pieces start at piece c0[k] of the stroke,
+  // row by row; nchunks pieces in all
+  int ya[kMaxStrokePoints], nr[kMaxStrokePoints], xc[kMaxStrokePoints], c0[kMaxStrokePoints];
   int nchunks;
   // pixel box of each segment's capsule (a pixel outside it is not in it)
   int sx0[kMaxStrokePoints], sx1[kMaxStrokePoints], sy0[kMaxStrokePoints],
@@ -1450,10 +1451,11 @@ struct PaintJob {
 constexpr int kPaintJobsLds = 256;  // jobs whose first chunks a block stages in LDS
 
 // Paints the strokes of `jobs` (chunk0 ascending): one wave per (stroke,
-// segment, kPaintRows rows) a segment's capsule can reach, its lanes over
-// each row's candidate columns.  A pixel is painted by the first segment of
-// its stroke whose capsule holds it, so every pixel of a stroke's union is
-// edited once (strokes of one launch are of different images).
+// segment, row, piece of kPaintCols columns) a segment's capsule can reach,
+// its lanes over the piece's candidate columns.  A pixel is painted by the
+// first segment of its stroke whose capsule holds it, so every pixel of a
+// stroke's union is edited once (strokes of one launch are of different
+// images).
 template <int T>
 __global__ void __launch_bounds__(256)
 k_paint(Geo g, CutDev s, const PaintJob* __restrict__ jobs, int njobs, int nchunks) {
@@ -1479,24 +1481,26 @@ k_paint(Geo g, CutDev s, const PaintJob* __restrict__ jobs, int njobs, int nchun
     const int nseg = pl.npts - 1;
     const unsigned long long le = __ballot(lane < nseg && pl.c0[lane] <= local);
     const int k = 63 - __clzll(le);
-    const int yfirst = pl.ya[k] + (local - pl.c0[k]) * kPaintRows;
-    const int ylast = min(yfirst + kPaintRows, pl.ya[k] + pl.nr[k]);
+    const int piece = local - pl.c0[k];
+    const int y = pl.ya[k] + piece / pl.xc[k];
+    const int xpiece = piece % pl.xc[k];
     const ImgDesc d = g.img[J.img];
     long long dmin = 0, dpos = 0;
-    for (int y = yfirst; y < ylast; ++y) {
-      double xl, xr;
-      if (!seg_row_span(pl, k, (double)y, &xl, &xr)) continue;
+    double xl, xr;
+    if (seg_row_span(pl, k, (double)y, &xl, &xr)) {
       xl = fmax(xl, 0.0);
       xr = fmin(xr, (double)(pl.W - 1));
-      if (xl > xr) continue;
-      const int xa = (int)floor(xl), xb = (int)ceil(xr);
-      for (int x = xa + lane; x <= xb; x += kWave) {
-        if (!in_capsule(pl, k, (double)x, (double)y)) continue;
-        bool earlier = false;
-        for (int j = 0; j < k && !earlier; ++j)
-          earlier = x >= pl.sx0[j] && x <= pl.sx1[j] && y >= pl.sy0[j] && y <= pl.sy1[j] &&
-                    in_capsule(pl, j, (double)x, (double)y);
-        if (!earlier) edit_pixel<T>(s, d, x, y, pl.cs, pl.ct, J.warm, dmin, dpos);
+      if (xl <= xr) {
+        const int x0 = (int)floor(xl) + xpiece * kPaintCols;
+        const int x1 = min((int)ceil(xr), x0 + kPaintCols - 1);
+        for (int x = x0 + lane; x <= x1; x += kWave) {
+          if (!in_capsule(pl, k, (double)x, (double)y)) continue;
+          bool earlier = false;
+          for (int j = 0; j < k && !earlier; ++j)
+            earlier = x >= pl.sx0[j] && x <= pl.sx1[j] && y >= pl.sy0[j] && y <= pl.sy1[j] &&
+                      in_capsule(pl, j, (double)x, (double)y);
+          if (!earlier) edit_pixel<T>(s, d, x, y, pl.cs, pl.ct, J.warm, dmin, dpos);
+        }
       }
     }
     if (J.warm) {

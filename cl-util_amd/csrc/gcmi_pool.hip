@@ -989,8 +989,8 @@ int make_polyline(int W, int H, int npts, const double* xy, double width, int32_
   pl.bw = (int)(bx1 - bx0 + 1);
   pl.bh = (int)(by1 - by0 + 1);
   pl.W = W;
-  // rows each segment's capsule can reach (as the box, per segment), in
-  // chunks of kPaintRows
+  // rows each segment's capsule can reach (as the box, per segment), each
+  // in pieces of kPaintCols columns covering the box's width
   long long total = 0;
   for (int k = 0; k + 1 < npts; ++k) {
     const double ylo = std::min(pl.y[k], pl.y[k + 1]) - pl.r;
@@ -1003,13 +1003,23 @@ int make_polyline(int W, int H, int npts, const double* xy, double width, int32_
     const long long rows = (ya <= yb && cols) ? yb - ya + 1 : 0;
     pl.ya[k] = rows > 0 ? (int)ya : 0;
     pl.nr[k] = (int)rows;
+    // a row's candidate columns: the x extent of the segment's part within
+    // r + 1 rows of it, widened by r + 1 each way, plus rounding; at most
+    // the box's width
+    const long long xa = std::max<long long>((long long)std::floor(xlo) - 1, 0);
+    const long long xb = std::min<long long>((long long)std::ceil(xhi) + 1, W - 1);
+    const double r1 = pl.r + 1.0;
+    const double adx = std::fabs(pl.x[k + 1] - pl.x[k]), ady = std::fabs(pl.y[k + 1] - pl.y[k]);
+    const double span = (ady > 0.0 ? adx * std::min(1.0, 2.0 * r1 / ady) : adx) + 2.0 * r1 + 4.0;
+    const double width = std::min(span, (double)(xb - xa + 1));
+    pl.xc[k] = std::max(1, (int)std::ceil(width / kPaintCols));
     // the capsule's pixel box, one pixel wider each way than needed
     pl.sx0[k] = (int)std::max<long long>((long long)std::floor(xlo) - 1, -1);
     pl.sx1[k] = (int)std::min<long long>((long long)std::ceil(xhi) + 1, W);
     pl.sy0[k] = (int)std::max<long long>((long long)std::floor(ylo) - 1, -1);
     pl.sy1[k] = (int)std::min<long long>((long long)std::ceil(yhi) + 1, H);
     pl.c0[k] = (int)std::min<long long>(total, INT32_MAX);
-    total += (rows + kPaintRows - 1) / kPaintRows;
+    total += rows * pl.xc[k];
   }
   if (total > INT32_MAX) {
     set_error("stroke spans too many rows");
