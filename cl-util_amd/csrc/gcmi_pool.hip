@@ -1178,10 +1178,16 @@ static int read_counts(Pool* p) {
 }
 
 // Sets the solve's done flag once a complete relabel (its queue, list
-// `qidx`, drained) found no active tile; a done solve zeroes the active and
-// relabel counts, so every later push and relabel launch of a recorded solve
-// exits at once.
-__global__ void k_gate(int32_t* c, int qidx) {
+// `qidx` at `q`, drained) found no active tile; a done solve zeroes the
+// active and relabel counts, so every later push and relabel launch of a
+// recorded solve exits at once.  A relabel whose queue did not drain marks
+// the queued tiles dirty: their distances are the only ones that may
+// violate h(p) <= h(q) + 1 across an arc (a neighbour's fell and they were
+// not recomputed yet), so a local relabel that recomputes them -- and the
+// rounds that follow it -- starts again from valid stored distances.
+__global__ void k_gate(int32_t* c, int qidx, const int32_t* __restrict__ q, int32_t* dirty) {
+  const int nq = c[CNT_Q + qidx];
+  for (int i = threadIdx.x; i < nq; i += blockDim.x) dirty[q[i]] = 1;
   if (threadIdx.x == 0) {
     if (c[CNT_Q + qidx] == 0 && c[CNT_ACT] == 0) c[CNT_DONE] = 1;
     if (c[CNT_DONE]) {
@@ -1291,7 +1297,8 @@ int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32
       cur = (cur + 1) % 3;
     }
     GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, false, S.nsel));
-    hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, p->stream, p->counts, cur);
+    hipLaunchKernelGGL(k_gate, dim3(1), dim3(1024), 0, p->stream, p->counts, cur,
+                       p->q + (size_t)cur * p->cap_tiles, p->dev.dirty);
     GCMI_HIP(hipGetLastError());
     *cur_out = cur;
     return GCMI_OK;

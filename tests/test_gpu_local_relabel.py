@@ -144,3 +144,38 @@ def test_local_relabel_mixed_fresh_and_warm(device_rounds):
         for k, i in enumerate(which if which is not None else range(len(dims))):
             fo, mo = O.bk_cut(*cur[i].arrays())
             assert flows[k] == fo and (b.labels(i) == mo).all(), (r, i)
+
+
+@pytest.mark.parametrize("tile,device_rounds", [(16, 2), (16, 6), (32, 3)])
+def test_far_excess_behind_zero_excess_corridor(tile, device_rounds):
+    """A sink appears at one end of a long strip whose only excess sits at
+    the other end, past pixels with no excess (nothing there pushes, so no
+    tile between them turns dirty).  The relabel's distances must reach the
+    far excess however few BFS rounds a device-driven relabel runs: a
+    relabel whose queue did not drain hands its queued tiles to the next
+    one."""
+    W, H = 700, 40
+    z = np.zeros((H, W), np.int32)
+    src, snk = z.copy(), z.copy()
+    src[:, W - 20:] = 9  # excess only at the far right
+    right = np.full((H, W), 6, np.int32)
+    down = np.full((H, W), 6, np.int32)
+    right[:, -1] = 0
+    down[-1, :] = 0
+    w = G.Workload(None, src, snk, right, down)
+    gc = GraphCut(None, None, (W, H), tile=tile)
+    gc.set_local_relabel(True)
+    gc.set_device_rounds(device_rounds)
+    gc.upload(*w.arrays())
+    fo, mo = O.bk_cut(*w.arrays())
+    assert fo == 0 and gc.cut() == 0 and (gc.labels() == mo).all()
+    cur = w.copy()
+    # sinks at the far left, then also in the middle, then the left one goes
+    for k, (x, t) in enumerate([(3, 1 << 20), (350, 50), (3, 0), (200, 7)]):
+        idx = np.arange(H, dtype=np.int32) * W + x
+        cur.sink.reshape(-1)[idx] = t
+        gc.edit(idx, np.zeros(H, np.int32), np.full(H, t, np.int32))
+        fo, mo = O.bk_cut(*cur.arrays())
+        assert fo > 0
+        assert gc.cut() == fo, (tile, device_rounds, k)
+        assert (gc.labels() == mo).all(), (tile, device_rounds, k)
