@@ -243,7 +243,7 @@ def warm_leg(args):
     # warm reflows host-driven with full relabels, with local relabels, and
     # device-driven with local relabels; scratch: every stroke from scratch
     variants = {"warm_host": {}, "warm_local": dict(local=True),
-                "warm_device": dict(local=True, rounds=8), "scratch": {}}
+                "warm_device": dict(local=True, rounds=2), "scratch": {}}
     for mode, opt in variants.items():
         gc = GraphCut(None, None, (W, H), tile=args.tile)
         gc.set_local_relabel(opt.get("local", False))
@@ -305,7 +305,7 @@ def serve_leg(args):
     mix = S.Mix(nsess=256, seed=4)
     modes = {"plain": (dict(mode="plain"), (1500, 2000, 2500)),
              "cold": (dict(mode="cold", device_rounds=8), (2000, 3000, 4000, 6000)),
-             "batched": (dict(mode="batched", local_relabel=True, device_rounds=8),
+             "batched": (dict(mode="batched", local_relabel=True, device_rounds=2),
                          (10000, 12000, 14000, 16000, 18000))}
     stroke_only = {"plain": (1500, 2000, 2500, 3000),
                    "batched": (20000, 25000, 30000, 35000, 40000)}
