@@ -178,7 +178,8 @@ struct Pool {
   long long* d_flows = nullptr;
   int32_t* d_edit = nullptr;  // staging for edits: idx, cs, ct per edit
   int32_t* d_imgfresh = nullptr;           // per image: cut from scratch in this solve
-  unsigned long long* d_rgbsum = nullptr;  // pair-difference sum of a colour upload
+  unsigned long long* d_rgbsum = nullptr;  // pair-difference sum of a colour upload (kSumSlots slots)
+  int32_t* d_rgblut = nullptr;  // pairwise weight per squared colour difference of a colour upload
   uint8_t* d_rgb = nullptr;                // staging for host colour images
   long long rgb_cap = 0;                   // bytes
   PaintJob *h_jobs = nullptr, *d_jobs = nullptr;  // staging for paint launches

@@ -98,9 +98,15 @@ __global__ void k_clear_listed(int32_t* flags, const int32_t* __restrict__ list,
 
 // Per listed tile: a dirty tile (arcs or excess changed since its last
 // relabel) is marked changed for the next flood, and its dirty flag cleared.
+// Also zeroes the queue counts and the active count (counts) for the
+// compactions that follow it in a relabel.
 __global__ void k_take_dirty(CutDev s, const int32_t* __restrict__ list,
-                             const int32_t* __restrict__ count) {
+                             const int32_t* __restrict__ count, int32_t* counts) {
   const int n = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    counts[CNT_Q] = counts[CNT_Q + 1] = counts[CNT_Q + 2] = 0;
+    counts[CNT_ACT] = 0;
+  }
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int t = list ? list[i] : i;
     if (s.dirty[t]) {
@@ -144,10 +150,12 @@ k_imp_closure(Geo g, CutDev s, const int32_t* __restrict__ init, const int32_t* 
 
 // Local relabel set: rflag[t] = t or one of its four neighbours is dirty
 // (one ring around every changed tile, so a distance that rose next to an
-// edit is recomputed rather than left to the stale neighbours).
+// edit is recomputed rather than left to the stale neighbours).  Zeroes
+// *zero (the count of the set's list).
 __global__ void k_dilate(Geo g, CutDev s, const int32_t* __restrict__ list,
-                         const int32_t* __restrict__ count) {
+                         const int32_t* __restrict__ count, int32_t* zero) {
   const int n = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const TG q = tile_geo(g, list ? list[i] : i);
     const int t = q.t;
@@ -181,6 +189,7 @@ __global__ void k_prepare(Geo g, CutDev s, const Prep* __restrict__ prep, int ns
     counts[CNT_FRESH] = nfresh_tiles;
     counts[CNT_GSEL] = nsel_tiles;
     counts[CNT_DONE] = 0;
+    counts[CNT_ACT] = 0;
     // the next epoch: above every stamp written so far (stamps start at 0)
     counts[CNT_EP] = max(max(counts[CNT_EP], counts[CNT_EP + 1]), 1);
     for (int k = 0; k < TALLY_N; ++k) s.tally[k] = 0;
@@ -194,22 +203,111 @@ __device__ __forceinline__ int rgb_d2(const uint8_t* a, const uint8_t* b) {
   return dr * dr + dg * dg + db * db;
 }
 
+// Consecutive row-major pixels per thread of the colour kernels (their loads
+// are independent, so a thread has them all in flight at once).
+constexpr int kRgbPx = 4;
+constexpr int kRgbNT = 256;
+
+// Column of pixel i of a row-major image W wide (32-bit arithmetic when the
+// image allows it).
+__device__ __forceinline__ int rgb_col(long long i, int W, bool small) {
+  return small ? (int)((unsigned)i % (unsigned)W) : (int)(i % W);
+}
+
+// A block of the colour kernels covers kRgbNT * kRgbPx consecutive pixels
+// and stages the bytes it reads in LDS: its pixels and the next one (right
+// neighbours), and the pixels one row below them.  Words per staged range
+// (up to 4 channels, plus the range's offset inside its first word).
+constexpr int kRgbBlockPx = kRgbNT * kRgbPx;
+constexpr int kRgbStageWords = (kRgbBlockPx + 1) + 1;
+
+// Bytes [b0, b1) of a colour image of nb bytes (4-byte aligned), clipped to
+// the image, into dst: byte i at dst[i - (b0 & ~3)].  Whole words where the
+// image holds them, a last partial word byte by byte.  All threads call it.
+__device__ __forceinline__ void rgb_stage(const uint8_t* __restrict__ rgb, long long nb,
+                                          long long b0, long long b1, uint32_t* dst) {
+  const long long a0 = b0 & ~3LL;
+  b1 = b1 < nb ? b1 : nb;
+  const int nw = b1 > a0 ? (int)((b1 - a0 + 3) >> 2) : 0;
+  for (int k = threadIdx.x; k < nw; k += blockDim.x) {
+    const long long o = a0 + 4LL * k;
+    uint32_t v = 0;
+    if (o + 4 <= nb) {
+      v = *reinterpret_cast<const uint32_t*>(rgb + o);
+    } else {
+      for (int j = 0; j < 4 && o + j < nb; ++j) v |= (uint32_t)rgb[o + j] << (8 * j);
+    }
+    dst[k] = v;
+  }
+}
+
+// Stages a block's bytes: own[] holds pixels p0 .. p0 + kRgbBlockPx (one
+// past, for right neighbours), dn[] the pixels a row below p0 ..; returns
+// the byte offsets of own[0] and dn[0] in the image.
+__device__ __forceinline__ void rgb_stage_block(const uint8_t* __restrict__ rgb, long long n,
+                                                int W, int ch, long long p0, uint32_t* own,
+                                                uint32_t* dn, long long* own0, long long* dn0) {
+  const long long nb = n * ch;
+  rgb_stage(rgb, nb, p0 * ch, (p0 + kRgbBlockPx + 1) * ch, own);
+  rgb_stage(rgb, nb, (p0 + W) * ch, (p0 + W + kRgbBlockPx) * ch, dn);
+  *own0 = (p0 * ch) & ~3LL;
+  *dn0 = ((p0 + W) * ch) & ~3LL;
+  __syncthreads();
+}
+
 // The sum of d2 over an image's horizontal and vertical neighbour pairs
-// (beta's numerator; an exact integer).  One block per row in turn.
-__global__ void __launch_bounds__(256)
+// (beta's numerator; an exact integer), spread over kSumSlots slots.  Every
+// thread takes kRgbPx pixels; the grid covers the image once.
+__global__ void __launch_bounds__(kRgbNT)
 k_rgb_pair_sum(const uint8_t* __restrict__ rgb, int W, int H, int ch, unsigned long long* sum) {
-  __shared__ long long red[256 / kWave];
+  __shared__ long long red[kRgbNT / kWave];
+  __shared__ uint32_t own[kRgbStageWords], dn[kRgbStageWords];
+  const long long n = (long long)W * H;
+  const long long p0 = (long long)blockIdx.x * kRgbBlockPx;
+  long long own0, dn0;
+  rgb_stage_block(rgb, n, W, ch, p0, own, dn, &own0, &dn0);
+  const uint8_t* O = reinterpret_cast<const uint8_t*>(own);
+  const uint8_t* D = reinterpret_cast<const uint8_t*>(dn);
+  const long long i0 = p0 + (long long)threadIdx.x * kRgbPx;
+  int x = i0 < n ? rgb_col(i0, W, n <= 0x7fffffffLL) : 0;
   long long acc = 0;
-  for (int y = blockIdx.x; y < H; y += gridDim.x) {
-    const uint8_t* row = rgb + (size_t)y * W * ch;
-    for (int x = threadIdx.x; x < W; x += blockDim.x) {
-      const uint8_t* q = row + (size_t)x * ch;
+#pragma unroll
+  for (int j = 0; j < kRgbPx; ++j) {
+    const long long i = i0 + j;
+    if (i < n) {
+      const uint8_t* q = O + (i * ch - own0);
       if (x + 1 < W) acc += rgb_d2(q, q + ch);
-      if (y + 1 < H) acc += rgb_d2(q, q + (size_t)W * ch);
+      if (i + W < n) acc += rgb_d2(q, D + ((i + W) * ch - dn0));
+    }
+    if (++x == W) x = 0;
+  }
+  acc = block_sum64<kRgbNT>(acc, red);
+  if (threadIdx.x == 0 && acc) atomicAdd(&sum[blockIdx.x % kSumSlots], (unsigned long long)acc);
+}
+
+// Largest squared colour difference of two pixels: 3 * 255^2.
+constexpr int kD2Max = 3 * 255 * 255;
+
+// 2 beta of the exponential pairwise model from the pair sum's slots (1 if
+// the image has no pairs or they are all equal); 0 (unused) for the inverse
+// model.  Called by every thread of a block; result in every thread.
+__device__ double rgb_two_beta(const gcmi_rgb_model& m, int W, int H,
+                               const unsigned long long* __restrict__ pair_sum, double* slot) {
+#pragma clang fp contract(off)
+  if (threadIdx.x < kWave) {
+    unsigned long long v = 0;
+    for (int j = threadIdx.x; j < kSumSlots; j += kWave) v += pair_sum[j];
+    v = (unsigned long long)wave_sum64((long long)v);
+    if (threadIdx.x == 0) {
+      double beta = 1.0;
+      const long long cnt = (long long)(W - 1) * H + (long long)W * (H - 1);
+      if (cnt > 0) beta = (double)v / (double)cnt;
+      if (!(beta > 0.0)) beta = 1.0;
+      *slot = 2.0 * beta;
     }
   }
-  acc = block_sum64<256>(acc, red);
-  if (threadIdx.x == 0 && acc) atomicAdd(sum, (unsigned long long)acc);
+  __syncthreads();
+  return *slot;
 }
 
 // Weight of a pair with squared colour difference d2 (float64, each
@@ -230,51 +328,95 @@ __device__ __forceinline__ int32_t rgb_unary(const gcmi_rgb_model& m, const doub
   return (int32_t)rint(m.unary_scale * fmin(d / m.unary_div, m.unary_cap));
 }
 
-// Row-major int32 capacities of one image from its colour image.
-__global__ void __launch_bounds__(256)
+// The pairwise weight of every squared colour difference 0..kD2Max for one
+// image (the exponential model's beta is the image's).
+__global__ void __launch_bounds__(kRgbNT)
+k_rgb_lut(gcmi_rgb_model m, int W, int H, const unsigned long long* __restrict__ pair_sum,
+          int32_t* __restrict__ lut) {
+  __shared__ double s_two_beta;
+  const double two_beta = m.pair_kind == GCMI_PAIR_EXP
+                              ? rgb_two_beta(m, W, H, pair_sum, &s_two_beta)
+                              : 0.0;
+  const int d2 = blockIdx.x * kRgbNT + threadIdx.x;
+  if (d2 <= kD2Max) lut[d2] = rgb_pair_weight(m, d2, two_beta);
+}
+
+// Row-major int32 capacities of one image from its colour image (pairwise
+// weights from the image's table); kRgbPx pixels per thread, stored as one
+// 16-byte vector per array when whole (the outputs start 16-byte aligned:
+// pool pixel offsets are multiples of 32).
+__global__ void __launch_bounds__(kRgbNT)
 k_rgb_caps(const uint8_t* __restrict__ rgb, int W, int H, int ch, gcmi_rgb_model m,
-           const unsigned long long* __restrict__ pair_sum, int32_t* __restrict__ cs,
-           int32_t* __restrict__ ct, int32_t* __restrict__ right, int32_t* __restrict__ down) {
+           const int32_t* __restrict__ lut, int32_t* __restrict__ cs, int32_t* __restrict__ ct,
+           int32_t* __restrict__ right, int32_t* __restrict__ down) {
 #pragma clang fp contract(off)
-  double beta = 1.0;
-  if (m.pair_kind == GCMI_PAIR_EXP) {
-    const long long cnt = (long long)(W - 1) * H + (long long)W * (H - 1);
-    if (cnt > 0) beta = (double)*pair_sum / (double)cnt;
-    if (!(beta > 0.0)) beta = 1.0;
+  __shared__ uint32_t own[kRgbStageWords], dn[kRgbStageWords];
+  const long long n = (long long)W * H;
+  const long long p0 = (long long)blockIdx.x * kRgbBlockPx;
+  long long own0, dn0;
+  rgb_stage_block(rgb, n, W, ch, p0, own, dn, &own0, &dn0);
+  const uint8_t* O = reinterpret_cast<const uint8_t*>(own);
+  const uint8_t* D = reinterpret_cast<const uint8_t*>(dn);
+  const long long i0 = p0 + (long long)threadIdx.x * kRgbPx;
+  int x = i0 < n ? rgb_col(i0, W, n <= 0x7fffffffLL) : 0;
+  int32_t a[kRgbPx], b[kRgbPx], rr[kRgbPx], dd[kRgbPx];
+#pragma unroll
+  for (int j = 0; j < kRgbPx; ++j) {
+    const long long i = i0 + j;
+    a[j] = b[j] = rr[j] = dd[j] = 0;
+    if (i < n) {
+      const uint8_t* q = O + (i * ch - own0);
+      const double r = q[0], g = q[1], bl = q[2];
+      a[j] = rgb_unary(m, m.mu_bg, r, g, bl);
+      b[j] = rgb_unary(m, m.mu_fg, r, g, bl);
+      if (x + 1 < W) rr[j] = lut[rgb_d2(q, q + ch)];
+      if (i + W < n) dd[j] = lut[rgb_d2(q, D + ((i + W) * ch - dn0))];
+    }
+    if (++x == W) x = 0;
   }
-  const double two_beta = 2.0 * beta;
-  for (int y = blockIdx.x; y < H; y += gridDim.x) {
-    const size_t r0 = (size_t)y * W;
-    for (int x = threadIdx.x; x < W; x += blockDim.x) {
-      const uint8_t* q = rgb + (r0 + x) * ch;
-      const double r = q[0], g = q[1], b = q[2];
-      cs[r0 + x] = rgb_unary(m, m.mu_bg, r, g, b);
-      ct[r0 + x] = rgb_unary(m, m.mu_fg, r, g, b);
-      right[r0 + x] = x + 1 < W ? rgb_pair_weight(m, rgb_d2(q, q + ch), two_beta) : 0;
-      down[r0 + x] = y + 1 < H ? rgb_pair_weight(m, rgb_d2(q, q + (size_t)W * ch), two_beta) : 0;
+  static_assert(kRgbPx == 4, "one int4 per array and thread");
+  if (i0 + kRgbPx <= n) {
+    *reinterpret_cast<int4*>(cs + i0) = make_int4(a[0], a[1], a[2], a[3]);
+    *reinterpret_cast<int4*>(ct + i0) = make_int4(b[0], b[1], b[2], b[3]);
+    *reinterpret_cast<int4*>(right + i0) = make_int4(rr[0], rr[1], rr[2], rr[3]);
+    *reinterpret_cast<int4*>(down + i0) = make_int4(dd[0], dd[1], dd[2], dd[3]);
+  } else {
+    for (int j = 0; j < kRgbPx && i0 + j < n; ++j) {
+      cs[i0 + j] = a[j];
+      ct[i0 + j] = b[j];
+      right[i0 + j] = rr[j];
+      down[i0 + j] = dd[j];
     }
   }
 }
 
 // Zeroes the final-excess sums of the selected images: the first flood
 // pass adds them up (a run whose recording stopped short floods again).
+// Also zeroes the counts the flood's compactions and first pass append to,
+// and the active count its labels list is built in.
 __global__ void k_zero_posf(unsigned long long* sums, const Prep* __restrict__ prep,
                             int32_t* counts) {
   unsigned long long* r = sums + (size_t)prep[blockIdx.x].img * SUM_ROWS * kSumSlots;
   for (int j = threadIdx.x; j < kSumSlots; j += blockDim.x) r[SUM_POSF * kSumSlots + j] = 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) counts[CNT_FIRST] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    counts[CNT_FIRST] = counts[CNT_POS] = counts[CNT_ACT] = 0;
+    counts[CNT_Q] = counts[CNT_Q + 1] = counts[CNT_Q + 2] = 0;
+  }
 }
 
 // The first pass of an incremental flood: of the recompute set `list`, the
 // tiles of images reflowed warm (their neighbours may hold reach to import)
 // and the tiles holding excess of images cut from scratch (the others get
 // reach only from their own image's tiles, through the rounds).  Appended
-// to `out` in any order, count in counts[CNT_FIRST].
+// to `out` in any order, count in counts[CNT_FIRST].  Zeroes the queue
+// counts for the flood's rounds.
 __global__ void k_first_pass_list(Geo g, CutDev s, const int32_t* __restrict__ list,
                                   const int32_t* __restrict__ count,
                                   const int32_t* __restrict__ imgfresh, int32_t* out,
                                   int32_t* counts) {
   const int n = *count;
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    counts[CNT_Q] = counts[CNT_Q + 1] = counts[CNT_Q + 2] = 0;
   for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const int i = base + threadIdx.x;
     bool keep = false;
@@ -460,7 +602,7 @@ size_t pool_bytes(int tile, int cap_tiles, long long cap_pix, int cap_imgs) {
          (size_t)cap_pix * (4 * 4 + 1) + (size_t)cap_pix / 8 +
          (size_t)cap_imgs *
              (sizeof(ImgDesc) + SUM_ROWS * kSumSlots * 8 + sizeof(Prep) + 8 + kPixAlign * 17) +
-         (size_t)compact_scratch_ints(cap_tiles) * 4 + 4096;
+         (size_t)compact_scratch_ints(cap_tiles) * 4 + 8 * kSumSlots + 4 * (kD2Max + 1) + 4096;
 }
 
 static int dev_alloc(void** ptr, size_t bytes, size_t* total) {
@@ -554,7 +696,8 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   ALLOC(p->plist, (size_t)cap_tiles * 4);
   ALLOC(p->alist, (size_t)cap_tiles * 4);
   ALLOC(p->counts, CNT_N * 4);
-  ALLOC(p->d_rgbsum, 8);
+  ALLOC(p->d_rgbsum, 8 * kSumSlots);
+  ALLOC(p->d_rgblut, 4 * (kD2Max + 1));
   ALLOC(p->d_imgfresh, (size_t)cap_imgs * 4);
   ALLOC(p->scratch, (size_t)compact_scratch_ints(cap_tiles) * 4);
   ALLOC(p->d_prep, (size_t)cap_imgs * sizeof(Prep));
@@ -602,7 +745,7 @@ void pool_free(Pool* p) {
                   (void*)d.down,      d.labels, p->d_img,   p->tile_img,  p->sel,     p->fresh,
                   p->q,    p->dlist,  p->plist, p->alist,   p->counts,    p->scratch, p->d_prep,
                   p->d_flows,
-                  p->d_edit, p->d_rgbsum, p->d_rgb, p->d_jobs, p->d_imgfresh};
+                  p->d_edit, p->d_rgbsum, p->d_rgblut, p->d_rgb, p->d_jobs, p->d_imgfresh};
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   p->graph_exec = nullptr;
@@ -778,12 +921,25 @@ int pool_upload_rgb(Pool* p, int img, const uint8_t* d_rgb, int channels,
   int32_t wmax = 0;
   GCMI_TRY(check_rgb_model(m, &wmax));
   PoolImage& im = p->imgs[img];
-  const int rows = std::min(im.H, 8192);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, p->stream, (int32_t*)p->d_rgbsum, 2LL, 0);
-  hipLaunchKernelGGL(k_rgb_pair_sum, dim3(rows), dim3(256), 0, p->stream, d_rgb, im.W, im.H,
-                     channels, p->d_rgbsum);
-  hipLaunchKernelGGL(k_rgb_caps, dim3(rows), dim3(256), 0, p->stream, d_rgb, im.W, im.H, channels,
-                     *m, (const unsigned long long*)p->d_rgbsum, p->dev.cs + im.pix0,
+  static_assert(kPixAlign % 4 == 0, "colour capacities are stored four pixels at a time");
+  if (reinterpret_cast<uintptr_t>(d_rgb) % 4 != 0) {
+    set_error("a colour image on the device must start 4-byte aligned");
+    return GCMI_EINVAL;
+  }
+  const long long npx = (long long)im.W * im.H;
+  const long long blocks = (npx + kRgbNT * kRgbPx - 1) / (kRgbNT * kRgbPx);
+  if (blocks > 0x7fffffffLL) {
+    set_error("colour image too large");
+    return GCMI_EINVAL;
+  }
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, p->stream, (int32_t*)p->d_rgbsum,
+                     2LL * kSumSlots, 0);
+  hipLaunchKernelGGL(k_rgb_pair_sum, dim3((unsigned)blocks), dim3(kRgbNT), 0, p->stream, d_rgb,
+                     im.W, im.H, channels, p->d_rgbsum);
+  hipLaunchKernelGGL(k_rgb_lut, dim3((kD2Max + kRgbNT) / kRgbNT), dim3(kRgbNT), 0, p->stream, *m,
+                     im.W, im.H, (const unsigned long long*)p->d_rgbsum, p->d_rgblut);
+  hipLaunchKernelGGL(k_rgb_caps, dim3((unsigned)blocks), dim3(kRgbNT), 0, p->stream, d_rgb, im.W,
+                     im.H, channels, *m, (const int32_t*)p->d_rgblut, p->dev.cs + im.pix0,
                      p->dev.ct + im.pix0, (int32_t*)p->dev.right + im.pix0,
                      (int32_t*)p->dev.down + im.pix0);
   GCMI_HIP(hipGetLastError());
@@ -1107,52 +1263,87 @@ int pool_paint_many(Pool* p, int count, const PaintReq* reqs) {
 }
 
 // The selected tiles (positions of `sel`, or tiles 0..n-1) whose flag is
-// set, appended to `out` in any order with one atomic per wave; *count must
-// be zero before.  clear: each listed flag is zeroed.  imgskip (may be
-// null): tiles of images flagged there are left out (and not cleared).
-__global__ void k_flag_list(const int32_t* __restrict__ sel, int n, int32_t* flags, int clear,
-                            int32_t* out, int32_t* count, const int32_t* __restrict__ tile_img,
-                            const int32_t* __restrict__ imgskip) {
-  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-    const int i = base + threadIdx.x;
-    int t = 0;
-    bool keep = false;
-    if (i < n) {
-      t = sel ? sel[i] : i;
-      keep = flags[t] != 0 && !(imgskip && imgskip[tile_img[t]]);
+// set, appended to `out` in any order with one atomic per block of
+// kFlagNT * kFlagPer positions; *count must be zero before.  clear: each
+// listed flag is zeroed.  imgskip (may be null): tiles of images flagged
+// there are left out (and not cleared).
+constexpr int kFlagNT = 256;
+constexpr int kFlagPer = 4;
+
+__global__ void __launch_bounds__(kFlagNT)
+k_flag_list(const int32_t* __restrict__ sel, int n, int32_t* flags, int clear, int32_t* out,
+            int32_t* count, const int32_t* __restrict__ tile_img,
+            const int32_t* __restrict__ imgskip) {
+  __shared__ int s_off[kFlagNT / kWave];
+  __shared__ int s_base;
+  const int lane = lane_id(), wave = threadIdx.x / kWave;
+  const unsigned long long below = (1ull << lane) - 1;
+  for (int base = blockIdx.x * kFlagNT * kFlagPer; base < n;
+       base += gridDim.x * kFlagNT * kFlagPer) {
+    int t[kFlagPer];
+    bool keep[kFlagPer];
+#pragma unroll
+    for (int j = 0; j < kFlagPer; ++j) {
+      const int i = base + j * kFlagNT + threadIdx.x;
+      t[j] = 0;
+      keep[j] = false;
+      if (i < n) t[j] = sel ? sel[i] : i;
     }
-    const unsigned long long m = __ballot(keep);
-    if (m == 0) continue;
-    int at = 0;
-    if (lane_id() == 0) at = atomicAdd(count, __popcll(m));
-    at = __shfl(at, 0);
-    if (keep) {
-      out[at + __popcll(m & ((1ull << lane_id()) - 1))] = t;
-      if (clear) flags[t] = 0;
+#pragma unroll
+    for (int j = 0; j < kFlagPer; ++j)
+      if (base + j * kFlagNT + threadIdx.x < n)
+        keep[j] = flags[t[j]] != 0 && !(imgskip && imgskip[tile_img[t[j]]]);
+    unsigned long long m[kFlagPer];
+    int tot = 0;
+#pragma unroll
+    for (int j = 0; j < kFlagPer; ++j) {
+      m[j] = __ballot(keep[j]);
+      tot += __popcll(m[j]);
     }
+    if (lane == 0) s_off[wave] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int sum = 0;
+      for (int w = 0; w < kFlagNT / kWave; ++w) {
+        const int c = s_off[w];
+        s_off[w] = sum;
+        sum += c;
+      }
+      s_base = sum ? atomicAdd(count, sum) : 0;
+    }
+    __syncthreads();
+    int at = s_base + s_off[wave];
+#pragma unroll
+    for (int j = 0; j < kFlagPer; ++j) {
+      if (keep[j]) {
+        out[at + __popcll(m[j] & below)] = t[j];
+        if (clear) flags[t[j]] = 0;
+      }
+      at += __popcll(m[j]);
+    }
+    __syncthreads();  // s_off and s_base are rewritten by the next pass
   }
 }
 
 // The solve's tiles (sel) whose flag is set (leaving out the images flagged
 // in imgskip, if given), as a list + count for the solver's own launches,
-// which do not depend on the list's order.
+// which do not depend on the list's order.  zeroed: an earlier launch of
+// the solve already zeroed the count (no fill launch).
+enum { COUNT_FILL = 0, COUNT_ZEROED = 1 };
 static int compact_sel(Pool* p, int32_t* flags, int32_t* out, int cnt, bool clear, int nsel,
-                       const int32_t* imgskip = nullptr) {
+                       const int32_t* imgskip, int zeroed) {
   Prof* P = p->prof.on ? &p->prof : nullptr;
   if (P) P->begin(p->stream, K_COMPACT);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, p->stream, p->counts + cnt, 1LL, 0);
-  hipLaunchKernelGGL(k_flag_list, dim3(std::max(1, std::min((nsel + 255) / 256, 2048))),
-                     dim3(256), 0, p->stream, p->lsel, nsel, flags, clear ? 1 : 0, out,
+  if (zeroed != COUNT_ZEROED)
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, p->stream, p->counts + cnt, 1LL, 0);
+  constexpr int per_block = kFlagNT * kFlagPer;
+  hipLaunchKernelGGL(k_flag_list,
+                     dim3(std::max(1, std::min((nsel + per_block - 1) / per_block, 2048))),
+                     dim3(kFlagNT), 0, p->stream, p->lsel, nsel, flags, clear ? 1 : 0, out,
                      p->counts + cnt, p->tile_img, imgskip);
   if (P) P->end(p->stream);
   GCMI_HIP(hipGetLastError());
   return GCMI_OK;
-}
-
-// Zeroes the three queue counts (a fill kernel, so the solve holds no
-// memset nodes if it is ever captured).
-static void clear_queues(Pool* p) {
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, p->stream, p->counts + CNT_Q, 3LL, 0);
 }
 
 // The queue of a BFS or flood round reading rotating list `cur`; the epoch
@@ -1243,7 +1434,8 @@ int phase_first(const Solve& S, int32_t* nact) {
     GCMI_HIP(hipGetLastError());
   }
   if (S.nsel > S.nfresh) {
-    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, true, S.nsel));
+    // (k_prepare zeroed the count)
+    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, true, S.nsel, nullptr, COUNT_ZEROED));
     if (!S.dd) {
       GCMI_TRY(read_counts(p));
       *nact = p->h_counts[CNT_ACT];
@@ -1269,8 +1461,9 @@ int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32
   int gcnt = S.dd ? CNT_GSEL : CNT_SEL;
   if (local) {
     hipLaunchKernelGGL(k_dilate, dim3(S.grid_clear()), dim3(256), 0, p->stream, p->geo, p->dev,
-                       p->lsel, p->counts + CNT_SEL);
-    GCMI_TRY(compact_sel(p, p->dev.rflag, p->dlist, CNT_DIRTY, false, S.nsel));
+                       p->lsel, p->counts + CNT_SEL, p->counts + CNT_DIRTY);
+    GCMI_TRY(
+        compact_sel(p, p->dev.rflag, p->dlist, CNT_DIRTY, false, S.nsel, nullptr, COUNT_ZEROED));
     glist = p->dlist;
     gcnt = CNT_DIRTY;
   }
@@ -1282,13 +1475,13 @@ int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32
   S.E();
   GCMI_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_take_dirty, dim3(S.grid_clear()), dim3(256), 0, p->stream, p->dev, glist,
-                     p->counts + gcnt);
+                     p->counts + gcnt, p->counts);
   if (local)
     hipLaunchKernelGGL(k_clear_listed, dim3(S.grid_clear()), dim3(256), 0, p->stream,
                        p->dev.rflag, glist, p->counts + gcnt);
-  // the flagged tiles start the first queue
-  clear_queues(p);
-  GCMI_TRY(compact_sel(p, p->dev.bflag, p->q, CNT_Q, true, S.nsel));
+  // the flagged tiles start the first queue (k_take_dirty zeroed the queue
+  // and active counts)
+  GCMI_TRY(compact_sel(p, p->dev.bflag, p->q, CNT_Q, true, S.nsel, nullptr, COUNT_ZEROED));
   int cur = 0;
   const int32_t* hc = p->h_counts;
   if (S.dd) {
@@ -1296,7 +1489,7 @@ int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32
       S.ops->bfs_round(p, p->grid_cap, round_queue(p, cur));
       cur = (cur + 1) % 3;
     }
-    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, false, S.nsel));
+    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, false, S.nsel, nullptr, COUNT_ZEROED));
     hipLaunchKernelGGL(k_gate, dim3(1), dim3(1024), 0, p->stream, p->counts, cur,
                        p->q + (size_t)cur * p->cap_tiles, p->dev.dirty);
     GCMI_HIP(hipGetLastError());
@@ -1310,6 +1503,7 @@ int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32
   }
   // later BFS rounds, several per host check (a round with an empty input
   // queue exits at once); the active list rides along with the check
+  int zeroed = COUNT_ZEROED;
   for (int batch = p->trace ? 1 : kRoundBatch;; batch = p->trace ? 1 : kRoundBatchMax) {
     for (int b = 0; b < batch; ++b) {
       S.B(K_BFS_ROUND);
@@ -1318,7 +1512,8 @@ int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32
       GCMI_HIP(hipGetLastError());
       cur = (cur + 1) % 3;
     }
-    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, false, S.nsel));
+    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, false, S.nsel, nullptr, zeroed));
+    zeroed = COUNT_FILL;
     if (p->trace)
       GCMI_HIP(hipMemcpyAsync(p->h_tally, p->dev.tally, TALLY_N * 8, hipMemcpyDeviceToHost,
                               p->stream));
@@ -1352,7 +1547,7 @@ int phase_extra_pushes(const Solve& S, bool* any) {
   Pool* p = S.p;
   *any = true;
   for (int r = 1; r < S.bfs; ++r) {
-    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, true, S.nsel));
+    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, true, S.nsel, nullptr, COUNT_FILL));
     int32_t nact = 0;
     if (!S.dd) {
       GCMI_TRY(read_counts(p));
@@ -1391,10 +1586,11 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
     // (they can only grow, and the rounds grow them)
     // (a fresh image's tiles are all in the set already, and so are their
     // neighbours: the closure starts from the warm images' changed tiles)
-    GCMI_TRY(compact_sel(p, p->dev.fchg, p->q, CNT_Q, false, S.nsel, p->d_imgfresh));
+    // (k_zero_posf zeroed the counts of these compactions)
+    GCMI_TRY(compact_sel(p, p->dev.fchg, p->q, CNT_Q, false, S.nsel, p->d_imgfresh, COUNT_ZEROED));
     hipLaunchKernelGGL(k_imp_closure, dim3(1), dim3(1024), 0, p->stream, p->geo, p->dev, p->q,
                        p->counts + CNT_Q, p->q + cap, p->q + 2 * cap);
-    GCMI_TRY(compact_sel(p, p->dev.fchg, p->plist, CNT_POS, true, S.nsel));
+    GCMI_TRY(compact_sel(p, p->dev.fchg, p->plist, CNT_POS, true, S.nsel, nullptr, COUNT_ZEROED));
     S.ops->clear_reach(p, std::min(S.nsel, 4096), p->plist, CNT_POS, lmark);
     hipLaunchKernelGGL(k_first_pass_list, dim3(std::min((S.nsel + 255) / 256, 2048)), dim3(256),
                        0, p->stream, p->geo, p->dev, p->plist, p->counts + CNT_POS,
@@ -1402,11 +1598,13 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
   } else {
     // from scratch: the tiles holding excess seed the first pass (haspos
     // persists: a clean tile keeps the flag its last relabel wrote)
-    GCMI_TRY(compact_sel(p, p->dev.haspos, p->plist, CNT_POS, false, S.nsel));
+    GCMI_TRY(
+        compact_sel(p, p->dev.haspos, p->plist, CNT_POS, false, S.nsel, nullptr, COUNT_ZEROED));
     S.ops->clear_reach(p, std::min(S.nsel, 4096), p->lsel, CNT_SEL, nullptr);
   }
   GCMI_HIP(hipGetLastError());
-  clear_queues(p);
+  // (the queue counts are zero: k_zero_posf, and k_first_pass_list after the
+  // incremental path's use of the queue as a compaction target)
   S.B(K_FLOOD);
   const Queue first{inc ? p->alist : p->plist,
                     p->counts + (inc ? CNT_FIRST : CNT_POS),
@@ -1432,12 +1630,13 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
 }
 
 // Labels, flows and tallies to the host buffers (no synchronisation).
-int phase_results(const Solve& S) {
+// zeroed: the active count is zero (right after the flood's start).
+int phase_results(const Solve& S, int zeroed) {
   Pool* p = S.p;
   if (p->flood_inc) {
     // the labels of the tiles the flood processed (the others' reach, and
     // so their labels, did not change); the active list is free by now
-    GCMI_TRY(compact_sel(p, p->dev.bflag, p->alist, CNT_ACT, true, S.nsel));
+    GCMI_TRY(compact_sel(p, p->dev.bflag, p->alist, CNT_ACT, true, S.nsel, nullptr, zeroed));
     S.B(K_LABELS);
     S.ops->labels(p, S.grid_for(S.nsel), p->alist, CNT_ACT);
   } else {
@@ -1471,7 +1670,7 @@ int flood_rounds_host(const Solve& S, int cur) {
       cur = (cur + 1) % 3;
     }
     // labels and flows ride along with the check
-    GCMI_TRY(phase_results(S));
+    GCMI_TRY(phase_results(S, COUNT_FILL));
     GCMI_TRY(read_counts(p));
     if (p->trace)
       fprintf(stderr, "gcmi:   flood round -> queue %d (tiles so far %llu)\n", hc[CNT_Q + cur],
@@ -1506,7 +1705,7 @@ int emit_device_driven(const Solve& S, int rounds, int* flood_cur) {
     GCMI_TRY(phase_push(S, 0));
   }
   GCMI_TRY(phase_flood_start(S, rounds, flood_cur));
-  return phase_results(S);
+  return phase_results(S, COUNT_ZEROED);
 }
 
 }  // namespace
