@@ -705,6 +705,58 @@ __device__ __forceinline__ int edge_dirs(const TG& q, int lx, int ly) {
          (lx == 0 && q.tx > 0 ? 1 << D_LEFT : 0) | (lx == T - 1 && q.tx < q.tw - 1 ? 1 << D_RIGHT : 0);
 }
 
+// LDS index of the halo pixel across side `side` from edge pixel (lx, ly),
+// in a (T+2)^2 halo-padded array, and its halo slot (as halo_slot's j).
+template <int T>
+__device__ __forceinline__ int halo_idx(int side, int lx, int ly) {
+  constexpr int HP = T + 2;
+  return side == D_UP ? lx + 1
+         : side == D_DOWN ? (T + 1) * HP + lx + 1
+         : side == D_LEFT ? (ly + 1) * HP
+                          : (ly + 1) * HP + T + 1;
+}
+template <int T>
+__device__ __forceinline__ int halo_j(int side, int lx, int ly) {
+  return side * T + (side <= D_DOWN ? lx : ly);
+}
+
+// Arc bit of a halo pixel on side `side` that points into this tile.
+__device__ __forceinline__ int arc_into(int side) { return 1 << (side ^ 1); }
+
+// BFS news for the neighbours of edge pixel (lx, ly) at distance d: a side
+// counts if the neighbour pixel across it has a residual arc into this pixel
+// (hin bit 0) and d + 1 beats the value the halo holds for it (D).  hin bit
+// 1: the neighbour is being recomputed in the same pass from a halo of 0 at
+// sinks and MAX elsewhere, so a sink here is no news to it.  Distances only
+// fall, so a stale halo value can only make a side count needlessly.
+template <int T>
+__device__ __forceinline__ int bfs_news(const TG& q, int lx, int ly, int d, bool sink,
+                                        const int32_t* D, const uint8_t* hin) {
+  const int e = edge_dirs<T>(q, lx, ly);
+  int dirs = 0;
+#pragma unroll
+  for (int side = 0; side < 4; ++side) {
+    if (!((e >> side) & 1)) continue;
+    const int h = hin[halo_j<T>(side, lx, ly)];
+    if ((h & 1) && !((h & 2) && sink) && d + 1 < D[halo_idx<T>(side, lx, ly)]) dirs |= 1 << side;
+  }
+  return dirs;
+}
+
+// Flood news for the neighbours of newly reached edge pixel (lx, ly) with
+// arc bits m: a side counts if the arc across it is residual and the halo
+// pixel there is not known to be reached (RM bit 0; reach only grows, so a
+// stale 0 can only make a side count needlessly).
+template <int T>
+__device__ __forceinline__ int flood_news(const TG& q, int lx, int ly, int m, const int32_t* RM) {
+  const int e = edge_dirs<T>(q, lx, ly) & m;
+  int dirs = 0;
+#pragma unroll
+  for (int side = 0; side < 4; ++side)
+    if (((e >> side) & 1) && !(RM[halo_idx<T>(side, lx, ly)] & 1)) dirs |= 1 << side;
+  return dirs;
+}
+
 // Appends the neighbours in b.dirs (complete: a barrier has passed since the
 // block's atomicOr into it) to the block's buffer; flushes a nearly full
 // buffer.  Called by all threads.
@@ -858,6 +910,7 @@ k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __r
   constexpr int R = T * T / NT;
   constexpr int HP = T + 2;
   __shared__ int32_t D0[HP * HP], D1[HP * HP];
+  __shared__ uint8_t hin[4 * T];
   __shared__ int sdirs;
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
   if (threadIdx.x == 0) sdirs = 0;
@@ -884,13 +937,17 @@ k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __r
       // a neighbour being relabelled too starts at 0 (sink) or MAX; a clean
       // one keeps its stored (valid) distance
       const Halo hs = halo_slot<T>(q, threadIdx.x);
-      int v = g.maxh;
+      int v = g.maxh, in = 0;
       if (hs.nb >= 0) {
         const size_t p = tbase<T>(hs.nb) + hs.ny * T + hs.nx;
-        v = (all || s.rflag[hs.nb]) ? ((s.mask[p] & M_SINK) ? 0 : g.maxh) : s.h[p];
+        const int mq = s.mask[p];
+        const bool listed = all || s.rflag[hs.nb];
+        v = listed ? ((mq & M_SINK) ? 0 : g.maxh) : s.h[p];
+        in = ((mq & arc_into(threadIdx.x / T)) ? 1 : 0) | (listed ? 2 : 0);
       }
       D0[hs.idx] = v;
       D1[hs.idx] = v;
+      hin[threadIdx.x] = (uint8_t)in;
     }
     __syncthreads();
     bfs_relax<T, NT>(d, m, D0, D1);
@@ -900,9 +957,8 @@ k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __r
       const int ly = chain * R + k;
       s.h[base + ly * T + lx] = d[k];
       any_act |= (m[k] & M_EXCESS) && d[k] < g.maxh;
-      // news for a neighbour: it may now reach a sink through this pixel (in
-      // a full relabel every neighbour already seeded this tile's sinks)
-      if (d[k] < g.maxh && !(all && (m[k] & M_SINK))) dirs |= edge_dirs<T>(q, lx, ly);
+      // news for a neighbour: it may now reach a sink through this pixel
+      if (d[k] < g.maxh) dirs |= bfs_news<T>(q, lx, ly, d[k], (m[k] & M_SINK) != 0, D0, hin);
     }
     if (dirs) atomicOr(&sdirs, dirs);
     any_pos = __syncthreads_or(any_pos);
@@ -928,6 +984,7 @@ k_bfs_round(Geo g, CutDev s, Queue Q) {
   constexpr int R = T * T / NT;
   constexpr int HP = T + 2;
   __shared__ int32_t D0[HP * HP], D1[HP * HP];
+  __shared__ uint8_t hin[4 * T];
   __shared__ BlockQ bq;
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
   const int n = *Q.in_count;
@@ -948,9 +1005,15 @@ k_bfs_round(Geo g, CutDev s, Queue Q) {
     }
     if (threadIdx.x < 4 * T) {
       const Halo hs = halo_slot<T>(q, threadIdx.x);
-      const int v = hs.nb >= 0 ? s.h[tbase<T>(hs.nb) + hs.ny * T + hs.nx] : g.maxh;
+      int v = g.maxh, in = 0;
+      if (hs.nb >= 0) {
+        const size_t p = tbase<T>(hs.nb) + hs.ny * T + hs.nx;
+        v = s.h[p];
+        in = (s.mask[p] & arc_into(threadIdx.x / T)) ? 1 : 0;
+      }
       D0[hs.idx] = v;
       D1[hs.idx] = v;
+      hin[threadIdx.x] = (uint8_t)in;
     }
     __syncthreads();
     bfs_relax<T, NT>(d, m, D0, D1);
@@ -960,7 +1023,7 @@ k_bfs_round(Geo g, CutDev s, Queue Q) {
       const int ly = chain * R + k;
       if (d[k] < d0[k]) {
         s.h[base + ly * T + lx] = d[k];
-        dirs |= edge_dirs<T>(q, lx, ly);
+        dirs |= bfs_news<T>(q, lx, ly, d[k], false, D0, hin);
         any_act |= (m[k] & M_EXCESS) != 0;
       }
     }
@@ -1134,7 +1197,7 @@ k_flood(Geo g, CutDev s, Queue Q, int mode, int32_t* lmark) {
       const int ly = chain * R + k;
       if (r[k] && !r0[k]) {
         s.reach[base + ly * T + lx] = 1;
-        dirs |= edge_dirs<T>(q, lx, ly) & m[k];
+        dirs |= flood_news<T>(q, lx, ly, m[k], RM);
         imported |= first ? !rl[k] : 1;
       }
     }
