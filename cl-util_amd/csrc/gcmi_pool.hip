@@ -1699,7 +1699,7 @@ int emit_device_driven(const Solve& S, int rounds, int* flood_cur) {
     }
     // local relabels as in the host loop (their dirty lists are empty, so
     // they are gated too, once the solve is done)
-    const bool local = S.p->local_relabel && S.nfresh < S.nsel && it < kLocalRelabels;
+    const bool local = S.p->local_relabel && (S.nfresh < S.nsel || it > 0) && it < kLocalRelabels;
     int cur;
     GCMI_TRY(phase_relabel(S, local, rounds, &cur, &nact));
     GCMI_TRY(phase_push(S, 0));
@@ -1847,17 +1847,20 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
     rounds = (nfresh > 0) + (nsel > nfresh && nact > 0);
   }
   int nlocal = dd_run ? dd_iters(S) : 0;  // relabels that may have been local so far
+  int relabels = dd_run ? dd_iters(S) : 0;
   while (!solved) {
     bool any = true;
     if (rounds > 0) GCMI_TRY(phase_extra_pushes(S, &any));
     // a global relabel: over every selected tile, or (local) only over the
     // tiles that changed and a ring around them -- in solves with edited
-    // images (a fresh image's tiles are all changed: every tile a push
-    // round processed is), and for kLocalRelabels relabels at most, which
+    // images, or after a first relabel (before it, a fresh image's tiles
+    // hold no distances), and for kLocalRelabels relabels at most, which
     // bounds the cost of the stale distances a local relabel leaves in
     // clean tiles
-    const bool local = p->local_relabel && nfresh < nsel && nlocal < kLocalRelabels;
+    const bool local =
+        p->local_relabel && (nfresh < nsel || relabels > 0) && nlocal < kLocalRelabels;
     if (local) ++nlocal;
+    ++relabels;
     GCMI_TRY(phase_relabel(S, local, 0, &cur, &nact));
     if (p->trace) fprintf(stderr, "gcmi: push round over %d active tiles\n", nact);
     if (nact == 0) break;
