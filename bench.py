@@ -60,6 +60,10 @@ def parse():
     ap.add_argument("--tile", type=int, default=32)
     ap.add_argument("--sweeps", type=int, default=0)
     ap.add_argument("--bfs", type=int, default=0)
+    ap.add_argument("--local", type=int, default=1,
+                    help="plain cut: local relabels after the first (1) or full relabels (0)")
+    ap.add_argument("--rounds", type=int, default=2,
+                    help="plain cut: device-driven BFS/flood rounds per relabel (0: host-driven)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-parity", action="store_true", help="skip the oracle check")
     ap.add_argument("--no-sweep", action="store_true", help="skip the tile sweep")
@@ -174,7 +178,7 @@ def batched_leg(args):
 
     # autotuner: tile x sweeps x push rounds per relabel x recording (0:
     # host-driven; r > 0: one replayed hipGraph with r BFS rounds per
-    # relabel), two timed runs each
+    # relabel) x relabels after the first (full or local), two timed runs each
     log("batched: autotuning")
     tune = []
     batches = {}
@@ -182,24 +186,30 @@ def batched_leg(args):
         batches[tile] = make_batch(tile)
         for sweeps in (2, 4, 8):
             for bfs in (1, 2):
-                for graph in (0, 4, 8):
-                    batches[tile].set_graph(graph)
-                    dt = time_batch(batches[tile], sweeps, bfs, 2)
-                    tune.append({"tile": tile, "sweeps": sweeps, "bfs": bfs, "graph": graph,
-                                 "ms": round(dt * 1e3, 3)})
+                for graph in (0, 2, 4, 8):
+                    for local in (0, 1):
+                        batches[tile].set_graph(graph)
+                        batches[tile].set_local_relabel(bool(local))
+                        dt = time_batch(batches[tile], sweeps, bfs, 2)
+                        tune.append({"tile": tile, "sweeps": sweeps, "bfs": bfs, "graph": graph,
+                                     "local": local, "ms": round(dt * 1e3, 3)})
     best = min(tune, key=lambda r: r["ms"])
     b = batches[best["tile"]]
     b.set_graph(best["graph"])
+    b.set_local_relabel(bool(best["local"]))
     reps = max(3, args.steps // 2)
     t_batch = time_batch(b, best["sweeps"], best["bfs"], reps)
     host = min((r for r in tune if r["graph"] == 0), key=lambda r: r["ms"])
     flows = b.cut(sweeps=best["sweeps"], bfs=best["bfs"])
     stats = b.stats()
-    # sequential plain cuts with the same settings
+    # sequential plain cuts with the same tile, sweeps and push rounds, and
+    # the plain cut's own run settings (local relabels, 2 device rounds)
     log("batched: 64 sequential plain cuts")
     cuts = []
     for i, d in enumerate(dev):
         g = GraphCut(None, None, dims[i], tile=best["tile"])
+        g.set_local_relabel(bool(args.local))
+        g.set_device_rounds(args.rounds)
         g.upload_device(*d)
         cuts.append(g)
     for g in cuts:
@@ -397,6 +407,8 @@ def main():
     work = G.three_ellipses(W, H, seed=1)
     dev = [DeviceBuffer(a.nbytes).upload(a) for a in work.arrays()]
     gc = GraphCut(None, None, (W, H), tile=args.tile)
+    gc.set_local_relabel(bool(args.local))
+    gc.set_device_rounds(args.rounds)
     gc.upload_device(*dev)
     sweeps = args.sweeps or None
     bfs = args.bfs or None
@@ -412,6 +424,7 @@ def main():
         "dtype": "int32", "data": "synthetic",
         "config": {"workload": "plain single cut, %dx%d, 3 ellipses, sigma 30, seed 1" % (W, H),
                    "tile": args.tile, "sweeps": sweeps or 4, "bfs": bfs or 1,
+                   "local_relabel": bool(args.local), "device_rounds": args.rounds,
                    "pixels": W * H},
         "flow": flow,
         "run_stats": stats,
@@ -430,6 +443,8 @@ def main():
         sweep = {}
         for t in (16, 32, 64):
             g2 = GraphCut(None, None, (W, H), tile=t)
+            g2.set_local_relabel(bool(args.local))
+            g2.set_device_rounds(args.rounds)
             g2.upload_device(*dev)
             ts, _, _ = time_cut(g2, max(3, args.steps // 4), 1, sweeps, bfs)
             sweep[str(t)] = round(W * H / ts / 1e6, 1)
