@@ -142,6 +142,14 @@ __device__ __forceinline__ long long block_sum64(long long v, long long* red) {
   return s;
 }
 
+// Zeroes counts[i] for every bit i of `mask` (the solver's list counts a
+// later launch appends to), from thread 0 of block 0.
+__device__ __forceinline__ void zero_counts(int32_t* counts, unsigned mask) {
+  if (counts && blockIdx.x == 0 && threadIdx.x == 0)
+    for (int i = 0; mask; ++i, mask >>= 1)
+      if (mask & 1) counts[i] = 0;
+}
+
 // Thread-0 accumulator of per-image sums that flushes (one slotted atomic)
 // whenever the image changes.
 struct ImgAcc {
@@ -899,21 +907,20 @@ __device__ __forceinline__ int bfs_relax(int (&d)[T * T / NT], const int (&m)[T 
 // neighbours keep their stored distances, which stay valid: a clean tile's
 // arcs and excess have not changed since its distances were computed, and
 // the rounds lower any of them that a recomputed tile now undercuts.  Later
-// rounds only lower distances, so they can only switch act on.  Tiles next
-// to an edge pixel with a finite distance get bflag (in a full relabel
-// nearly every tile does, so a flag and one compaction beat a queue append
-// per one-tile block).
+// rounds only lower distances, so they can only switch act on.  The
+// neighbours this tile's distances may improve start the rounds' queue
+// (Q.out; Q.in is not read).
 template <int T, int NT>
 __global__ void __launch_bounds__(NT)
 k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
-           int all) {
+           int all, Queue Q) {
   constexpr int R = T * T / NT;
   constexpr int HP = T + 2;
   __shared__ int32_t D0[HP * HP], D1[HP * HP];
   __shared__ uint8_t hin[4 * T];
-  __shared__ int sdirs;
+  __shared__ BlockQ bq;
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
-  if (threadIdx.x == 0) sdirs = 0;
+  bq_init(bq, Q);
   const int n = *count;
   if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) {
     atomicAdd(&s.tally[TALLY_RELABELS], 1ull);
@@ -960,21 +967,16 @@ k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __r
       // news for a neighbour: it may now reach a sink through this pixel
       if (d[k] < g.maxh) dirs |= bfs_news<T>(q, lx, ly, d[k], (m[k] & M_SINK) != 0, D0, hin);
     }
-    if (dirs) atomicOr(&sdirs, dirs);
+    if (dirs) atomicOr(&bq.dirs, dirs);
     any_pos = __syncthreads_or(any_pos);
     any_act = __syncthreads_or(any_act);
     if (threadIdx.x == 0) {
       s.haspos[q.t] = any_pos;
       s.act[q.t] = any_act;
     }
-    if (threadIdx.x < 4 && ((sdirs >> threadIdx.x) & 1)) {
-      const int k = threadIdx.x;
-      s.bflag[k == D_UP ? q.t - q.tw : k == D_DOWN ? q.t + q.tw : k == D_LEFT ? q.t - 1 : q.t + 1] =
-          1;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) sdirs = 0;
+    bq_take(bq, q, s.stamp, Q);
   }
+  bq_flush(bq, Q);
 }
 
 // Later passes over queued tiles, from the stored distances.
@@ -1217,10 +1219,11 @@ k_flood(Geo g, CutDev s, Queue Q, int mode, int32_t* lmark) {
 
 // The images' final sum of max(e, 0) from the listed tiles' sums: a wave
 // whose 64 tiles are of one image (the lists hold each image's tiles in a
-// run) adds them with one atomic.
+// run) adds them with one atomic.  Also zeroes the counts in zmask.
 template <int T>
 __global__ void k_posf(Geo g, CutDev s, const int32_t* __restrict__ list,
-                       const int32_t* __restrict__ count) {
+                       const int32_t* __restrict__ count, int32_t* counts, unsigned zmask) {
+  zero_counts(counts, zmask);
   const int n = *count;
   const int stride = gridDim.x * blockDim.x;
   for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {

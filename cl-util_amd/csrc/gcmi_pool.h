@@ -86,13 +86,13 @@ struct PoolOps {
   int threads;
   void (*push)(Pool*, int grid, int sweeps, bool first);
   void (*fold)(Pool*, int grid, const int32_t* list, int cnt);
-  void (*bfs_init)(Pool*, int grid, const int32_t* list, int cnt, bool all);
+  void (*bfs_init)(Pool*, int grid, const int32_t* list, int cnt, bool all, const Queue& q);
   void (*bfs_round)(Pool*, int grid, const Queue& q);
   void (*flood)(Pool*, int grid, const Queue& q, int mode, int32_t* lmark);
   void (*labels)(Pool*, int grid, const int32_t* list, int cnt);
   void (*reset_tiles)(Pool*, int grid);
   void (*clear_reach)(Pool*, int grid, const int32_t* list, int cnt, int32_t* lmark);
-  void (*posf)(Pool*, int grid);
+  void (*posf)(Pool*, int grid, unsigned zmask);
   void (*edit)(Pool*, int img, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct);
   void (*paint)(Pool*, const PaintJob* d_jobs, int njobs, int nrows);
   void (*untile)(Pool*, int img, const int32_t* src, int32_t* out);
@@ -157,6 +157,7 @@ struct Pool {
   bool trace = false;  // GCMI_TRACE=1: per-phase lines on stderr, one round per host check
   int ep_par = 0;         // epoch slot the next queue round reads
   bool flood_inc = false;  // the current flood recomputes changed tiles only
+  bool flood_ran = false;  // a flood has added to the final-excess sums in this solve
   bool ep_reset = false;  // restart stamps and epochs before the next solve
   // Device-driven solves (dd_rounds > 0): every launch of a fixed shape,
   // dd_rounds BFS / flood rounds per relabel, gates on the device instead of

@@ -97,22 +97,21 @@ __global__ void k_clear_listed(int32_t* flags, const int32_t* __restrict__ list,
 }
 
 // Per listed tile: a dirty tile (arcs or excess changed since its last
-// relabel) is marked changed for the next flood, and its dirty flag cleared.
-// Also zeroes the queue counts and the active count (counts) for the
-// compactions that follow it in a relabel.
+// relabel) is marked changed for the next flood, and its dirty flag cleared;
+// clear (may be null): a flag zeroed for every listed tile.  Also zeroes the
+// active count for the compaction that follows it in a relabel.
 __global__ void k_take_dirty(CutDev s, const int32_t* __restrict__ list,
-                             const int32_t* __restrict__ count, int32_t* counts) {
+                             const int32_t* __restrict__ count, int32_t* counts,
+                             int32_t* clear) {
   const int n = *count;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    counts[CNT_Q] = counts[CNT_Q + 1] = counts[CNT_Q + 2] = 0;
-    counts[CNT_ACT] = 0;
-  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) counts[CNT_ACT] = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int t = list ? list[i] : i;
     if (s.dirty[t]) {
       s.fchg[t] = 1;
       s.dirty[t] = 0;
     }
+    if (clear) clear[t] = 0;
   }
 }
 
@@ -150,21 +149,31 @@ k_imp_closure(Geo g, CutDev s, const int32_t* __restrict__ init, const int32_t* 
 
 // Local relabel set: rflag[t] = t or one of its four neighbours is dirty
 // (one ring around every changed tile, so a distance that rose next to an
-// edit is recomputed rather than left to the stale neighbours).  Zeroes
-// *zero (the count of the set's list).
+// edit is recomputed rather than left to the stale neighbours); the set's
+// tiles are appended to `out` in any order (*out_count zero before).
 __global__ void k_dilate(Geo g, CutDev s, const int32_t* __restrict__ list,
-                         const int32_t* __restrict__ count, int32_t* zero) {
+                         const int32_t* __restrict__ count, int32_t* out, int32_t* out_count) {
   const int n = *count;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *zero = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const TG q = tile_geo(g, list ? list[i] : i);
-    const int t = q.t;
-    int f = s.dirty[t];
-    if (!f && q.ty > 0) f = s.dirty[t - q.tw];
-    if (!f && q.ty < q.th - 1) f = s.dirty[t + q.tw];
-    if (!f && q.tx > 0) f = s.dirty[t - 1];
-    if (!f && q.tx < q.tw - 1) f = s.dirty[t + 1];
-    s.rflag[t] = f;
+  const unsigned long long below = (1ull << lane_id()) - 1;
+  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const int i = base + threadIdx.x;
+    int f = 0, t = 0;
+    if (i < n) {
+      const TG q = tile_geo(g, list ? list[i] : i);
+      t = q.t;
+      f = s.dirty[t];
+      if (!f && q.ty > 0) f = s.dirty[t - q.tw];
+      if (!f && q.ty < q.th - 1) f = s.dirty[t + q.tw];
+      if (!f && q.tx > 0) f = s.dirty[t - 1];
+      if (!f && q.tx < q.tw - 1) f = s.dirty[t + 1];
+      s.rflag[t] = f;
+    }
+    const unsigned long long m = __ballot(f != 0);
+    if (m == 0) continue;
+    int at = 0;
+    if (lane_id() == 0) at = atomicAdd(out_count, __popcll(m));
+    at = __shfl(at, 0);
+    if (f) out[at + __popcll(m & below)] = t;
   }
 }
 
@@ -190,6 +199,9 @@ __global__ void k_prepare(Geo g, CutDev s, const Prep* __restrict__ prep, int ns
     counts[CNT_GSEL] = nsel_tiles;
     counts[CNT_DONE] = 0;
     counts[CNT_ACT] = 0;
+    // the first relabel's queue and local-set counts (a device-driven run
+    // appends to them without a fill)
+    counts[CNT_Q] = counts[CNT_Q + 1] = counts[CNT_Q + 2] = counts[CNT_DIRTY] = 0;
     // the next epoch: above every stamp written so far (stamps start at 0)
     counts[CNT_EP] = max(max(counts[CNT_EP], counts[CNT_EP + 1]), 1);
     for (int k = 0; k < TALLY_N; ++k) s.tally[k] = 0;
@@ -461,9 +473,10 @@ struct PoolKernels {
     hipLaunchKernelGGL((k_fold<T>), dim3(grid), dim3(kFoldNT), 0, p->stream, p->geo, p->dev,
                        list, p->counts + cnt);
   }
-  static void bfs_init(Pool* p, int grid, const int32_t* list, int cnt, bool all) {
+  static void bfs_init(Pool* p, int grid, const int32_t* list, int cnt, bool all,
+                       const Queue& q) {
     hipLaunchKernelGGL((k_bfs_init<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
-                       list, p->counts + cnt, all ? 1 : 0);
+                       list, p->counts + cnt, all ? 1 : 0, q);
   }
   static void bfs_round(Pool* p, int grid, const Queue& q) {
     hipLaunchKernelGGL((k_bfs_round<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
@@ -485,9 +498,9 @@ struct PoolKernels {
     hipLaunchKernelGGL((k_clear_reach<T>), dim3(grid), dim3(256), 0, p->stream, p->dev, list,
                        p->counts + cnt, lmark);
   }
-  static void posf(Pool* p, int grid) {
+  static void posf(Pool* p, int grid, unsigned zmask) {
     hipLaunchKernelGGL((k_posf<T>), dim3(grid), dim3(256), 0, p->stream, p->geo, p->dev, p->lsel,
-                       p->counts + CNT_SEL);
+                       p->counts + CNT_SEL, p->counts, zmask);
   }
   static void edit(Pool* p, int img, int n, const int32_t* idx, const int32_t* cs,
                    const int32_t* ct) {
@@ -496,7 +509,8 @@ struct PoolKernels {
                        p->dev, img, n, idx, cs, ct, p->imgs[img].fresh ? 0 : 1);
   }
   static void paint(Pool* p, const PaintJob* d_jobs, int njobs, int nchunks) {
-    // a wave per (stroke, segment, chunk of rows), four per block
+    // a wave per (stroke, segment, row, piece of kPaintCols columns), four
+    // per block
     const int grid = std::min((nchunks + 3) / 4, 8192);
     hipLaunchKernelGGL((k_paint<T>), dim3(grid > 0 ? grid : 1), dim3(256), 0, p->stream, p->geo,
                        p->dev, d_jobs, njobs, nchunks);
@@ -1375,16 +1389,20 @@ static int read_counts(Pool* p) {
 // the queued tiles dirty: their distances are the only ones that may
 // violate h(p) <= h(q) + 1 across an arc (a neighbour's fell and they were
 // not recomputed yet), so a local relabel that recomputes them -- and the
-// rounds that follow it -- starts again from valid stored distances.
+// rounds that follow it -- starts again from valid stored distances.  Also
+// zeroes the counts the next relabel appends to.
 __global__ void k_gate(int32_t* c, int qidx, const int32_t* __restrict__ q, int32_t* dirty) {
   const int nq = c[CNT_Q + qidx];
   for (int i = threadIdx.x; i < nq; i += blockDim.x) dirty[q[i]] = 1;
+  __syncthreads();  // every thread has read the count before it is zeroed
   if (threadIdx.x == 0) {
-    if (c[CNT_Q + qidx] == 0 && c[CNT_ACT] == 0) c[CNT_DONE] = 1;
+    if (nq == 0 && c[CNT_ACT] == 0) c[CNT_DONE] = 1;
     if (c[CNT_DONE]) {
       c[CNT_ACT] = 0;
       c[CNT_GSEL] = 0;
     }
+    // the next relabel's queue and local-set counts
+    c[CNT_Q] = c[CNT_Q + 1] = c[CNT_Q + 2] = c[CNT_DIRTY] = 0;
   }
 }
 
@@ -1419,6 +1437,7 @@ int phase_prepare(const Solve& S) {
                      S.nsel, S.nfresh, p->sel, p->fresh, p->counts, p->d_imgfresh);
   GCMI_HIP(hipGetLastError());
   p->ep_par = 0;
+  p->flood_ran = false;
   return GCMI_OK;
 }
 
@@ -1459,29 +1478,35 @@ int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32
   Pool* p = S.p;
   const int32_t* glist = p->lsel;
   int gcnt = S.dd ? CNT_GSEL : CNT_SEL;
+  // the queue counts and the local set's count start at zero (a
+  // device-driven run: zeroed by k_prepare or the last relabel's gate)
+  static_assert(CNT_DIRTY == CNT_Q + 3, "one fill zeroes the queue and local-set counts");
+  if (!S.dd)
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, p->stream, p->counts + CNT_Q, 4LL, 0);
   if (local) {
     hipLaunchKernelGGL(k_dilate, dim3(S.grid_clear()), dim3(256), 0, p->stream, p->geo, p->dev,
-                       p->lsel, p->counts + CNT_SEL, p->counts + CNT_DIRTY);
-    GCMI_TRY(
-        compact_sel(p, p->dev.rflag, p->dlist, CNT_DIRTY, false, S.nsel, nullptr, COUNT_ZEROED));
+                       p->lsel, p->counts + CNT_SEL, p->dlist, p->counts + CNT_DIRTY);
     glist = p->dlist;
     gcnt = CNT_DIRTY;
   }
   S.B(K_FOLD);
   S.ops->fold(p, S.grid_relabel, glist, gcnt);
   S.E();
+  // the neighbours the first pass's distances may improve start queue 0
+  const Queue first{nullptr,
+                    nullptr,
+                    p->q,
+                    p->counts + CNT_Q,
+                    nullptr,
+                    p->counts + CNT_EP + p->ep_par,
+                    p->counts + CNT_EP + 1 - p->ep_par};
+  p->ep_par ^= 1;
   S.B(K_BFS_INIT);
-  S.ops->bfs_init(p, S.grid_relabel, glist, gcnt, !local);
+  S.ops->bfs_init(p, S.grid_relabel, glist, gcnt, !local, first);
   S.E();
   GCMI_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_take_dirty, dim3(S.grid_clear()), dim3(256), 0, p->stream, p->dev, glist,
-                     p->counts + gcnt, p->counts);
-  if (local)
-    hipLaunchKernelGGL(k_clear_listed, dim3(S.grid_clear()), dim3(256), 0, p->stream,
-                       p->dev.rflag, glist, p->counts + gcnt);
-  // the flagged tiles start the first queue (k_take_dirty zeroed the queue
-  // and active counts)
-  GCMI_TRY(compact_sel(p, p->dev.bflag, p->q, CNT_Q, true, S.nsel, nullptr, COUNT_ZEROED));
+                     p->counts + gcnt, p->counts, local ? p->dev.rflag : nullptr);
   int cur = 0;
   const int32_t* hc = p->h_counts;
   if (S.dd) {
@@ -1570,10 +1595,16 @@ int phase_extra_pushes(const Solve& S, bool* any) {
 // the list the next round would read in *cur_out.
 int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
   Pool* p = S.p;
-  // the images' final sum of max(e, 0)
-  hipLaunchKernelGGL(k_zero_posf, dim3(S.n), dim3(64), 0, p->stream, p->dev.sums, p->d_prep,
-                     p->counts);
-  S.ops->posf(p, std::min((S.nsel + 255) / 256, 1024));
+  // the images' final sum of max(e, 0) (k_prepare zeroed it; a flood run
+  // before in the same solve added to it) and the counts the flood's
+  // compactions, first pass, queues and labels list append to
+  if (p->flood_ran)
+    hipLaunchKernelGGL(k_zero_posf, dim3(S.n), dim3(64), 0, p->stream, p->dev.sums, p->d_prep,
+                       p->counts);
+  p->flood_ran = true;
+  S.ops->posf(p, std::min((S.nsel + 255) / 256, 1024),
+              (1u << CNT_FIRST) | (1u << CNT_POS) | (1u << CNT_ACT) | (1u << CNT_Q) |
+                  (1u << (CNT_Q + 1)) | (1u << (CNT_Q + 2)));
   const size_t cap = (size_t)p->cap_tiles;
   const bool inc = S.nfresh < S.nsel;
   p->flood_inc = inc;
@@ -1834,6 +1865,7 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
     else
       GCMI_TRY(emit_device_driven(R, p->dd_rounds, &cur));
     GCMI_TRY(read_counts(p));
+    p->flood_ran = true;  // (a replayed recording ran its flood too)
     solved = hc[CNT_DONE] != 0;
     flooded = solved && hc[CNT_Q + cur] == 0;
     rounds = dd_iters(R) + 1;
