@@ -251,9 +251,12 @@ def warm_leg(args):
     masks = {}
     flows = {}
     # warm reflows host-driven with full relabels, with local relabels, and
-    # device-driven with local relabels; scratch: every stroke from scratch
+    # device-driven with local relabels (2 or 1 rounds per relabel); scratch:
+    # every stroke from scratch, with the plain cut's settings
     variants = {"warm_host": {}, "warm_local": dict(local=True),
-                "warm_device": dict(local=True, rounds=2), "scratch": {}}
+                "warm_device": dict(local=True, rounds=2),
+                "warm_device1": dict(local=True, rounds=1),
+                "scratch": dict(local=True, rounds=2)}
     for mode, opt in variants.items():
         gc = GraphCut(None, None, (W, H), tile=args.tile)
         gc.set_local_relabel(opt.get("local", False))

@@ -1713,8 +1713,10 @@ int flood_rounds_host(const Solve& S, int cur) {
 
 // Push/relabel iterations of a device-driven solve (the host loop finishes
 // a solve that needs more): runs from scratch mostly finish after the first
-// relabel or the second, warm reflows after the second or the third.
-int dd_iters(const Solve& S) { return S.nfresh == S.nsel ? 2 : 3; }
+// relabel or the second, and so do warm reflows of one or two images; in a
+// larger batch of reflows some image mostly needs a third, which costs less
+// than a host round trip.
+int dd_iters(const Solve& S) { return S.nfresh == S.nsel || S.n <= 2 ? 2 : 3; }
 
 // The device-driven solve (the sequence a recording holds): fixed launch
 // shapes, `rounds` BFS and flood rounds per relabel, no host reads.
