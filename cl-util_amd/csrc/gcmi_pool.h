@@ -180,7 +180,10 @@ struct Pool {
   int32_t* d_edit = nullptr;  // staging for edits: idx, cs, ct per edit
   int32_t* d_imgfresh = nullptr;           // per image: cut from scratch in this solve
   unsigned long long* d_rgbsum = nullptr;  // pair-difference sum of a colour upload (kSumSlots slots)
-  int32_t* d_rgblut = nullptr;  // pairwise weight per squared colour difference of a colour upload
+  // colour uploads: the job table of a launch's images (allocated on first
+  // use)
+  void* h_rgbjobs = nullptr;
+  void* d_rgbjobs = nullptr;
   uint8_t* d_rgb = nullptr;                // staging for host colour images
   long long rgb_cap = 0;                   // bytes
   PaintJob *h_jobs = nullptr, *d_jobs = nullptr;  // staging for paint launches
@@ -218,6 +221,15 @@ int pool_capacities(Pool* p, int img, int32_t* cs, int32_t* ct, int32_t* right, 
 int check_rgb_model(const gcmi_rgb_model* m, int32_t* max_pair);
 // Builds an image's capacities on the device from a colour image in device
 // memory ([H][W][channels] uint8), on the pool's stream (no synchronisation).
+// Colour uploads of several distinct images in one set of launches (device
+// bytes, 4-byte aligned).
+struct RgbReq {
+  int img;
+  const uint8_t* d_rgb;
+  int channels;
+  const gcmi_rgb_model* model;
+};
+int pool_upload_rgb_many(Pool* p, int count, const RgbReq* reqs);
 int pool_upload_rgb(Pool* p, int img, const uint8_t* d_rgb, int channels,
                     const gcmi_rgb_model* m);
 // The same from host memory (copied; synchronises).

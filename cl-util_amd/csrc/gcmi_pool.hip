@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 
 #include "gcmi_pool.h"
 
@@ -236,20 +237,32 @@ constexpr int kRgbStageWords = (kRgbBlockPx + 1) + 1;
 // Bytes [b0, b1) of a colour image of nb bytes (4-byte aligned), clipped to
 // the image, into dst: byte i at dst[i - (b0 & ~3)].  Whole words where the
 // image holds them, a last partial word byte by byte.  All threads call it.
+constexpr int kRgbStageIters = (kRgbStageWords + kRgbNT - 1) / kRgbNT;
+
 __device__ __forceinline__ void rgb_stage(const uint8_t* __restrict__ rgb, long long nb,
                                           long long b0, long long b1, uint32_t* dst) {
   const long long a0 = b0 & ~3LL;
   b1 = b1 < nb ? b1 : nb;
   const int nw = b1 > a0 ? (int)((b1 - a0 + 3) >> 2) : 0;
-  for (int k = threadIdx.x; k < nw; k += blockDim.x) {
+  // every load issued before the first store (a block's words in flight at once)
+  uint32_t v[kRgbStageIters];
+#pragma unroll
+  for (int j = 0; j < kRgbStageIters; ++j) {
+    const int k = threadIdx.x + j * kRgbNT;
     const long long o = a0 + 4LL * k;
-    uint32_t v = 0;
-    if (o + 4 <= nb) {
-      v = *reinterpret_cast<const uint32_t*>(rgb + o);
-    } else {
-      for (int j = 0; j < 4 && o + j < nb; ++j) v |= (uint32_t)rgb[o + j] << (8 * j);
+    v[j] = 0;
+    if (k < nw) {
+      if (o + 4 <= nb) {
+        v[j] = *reinterpret_cast<const uint32_t*>(rgb + o);
+      } else {
+        for (int i = 0; i < 4 && o + i < nb; ++i) v[j] |= (uint32_t)rgb[o + i] << (8 * i);
+      }
     }
-    dst[k] = v;
+  }
+#pragma unroll
+  for (int j = 0; j < kRgbStageIters; ++j) {
+    const int k = threadIdx.x + j * kRgbNT;
+    if (k < nw) dst[k] = v[j];
   }
 }
 
@@ -267,15 +280,53 @@ __device__ __forceinline__ void rgb_stage_block(const uint8_t* __restrict__ rgb,
   __syncthreads();
 }
 
-// The sum of d2 over an image's horizontal and vertical neighbour pairs
-// (beta's numerator; an exact integer), spread over kSumSlots slots.  Every
-// thread takes kRgbPx pixels; the grid covers the image once.
+// One colour image of a launch: its bytes, pool pixel offset, size and
+// model, and its first block in the launch (blocks of kRgbBlockPx pixels,
+// images in turn).
+struct RgbJob {
+  const uint8_t* rgb;
+  long long pix0;
+  int W, H, ch;
+  int block0;
+  gcmi_rgb_model m;
+};
+constexpr int kRgbJobsMax = 64;  // images per launch
+
+// A launch's job table: the first blocks of its jobs (ascending, from 0),
+// then the jobs.
+struct RgbJobs {
+  int block0[kRgbJobsMax];
+  RgbJob job[kRgbJobsMax];
+};
+
+// The job this block belongs to: one ballot of wave 0 over the first blocks
+// (njobs <= kWave).  All threads call it.
+__device__ __forceinline__ int rgb_job(const RgbJobs* __restrict__ J, int njobs, int* s_j) {
+  static_assert(kRgbJobsMax <= kWave, "one lane per job");
+  if (threadIdx.x < kWave) {
+    const int b0 = threadIdx.x < njobs ? J->block0[threadIdx.x] : 0x7fffffff;
+    const unsigned long long m = __ballot(b0 <= (int)blockIdx.x);
+    if (threadIdx.x == 0) *s_j = __popcll(m) - 1;
+  }
+  __syncthreads();
+  return *s_j;
+}
+
+// Per image of the launch, the sum of d2 over its horizontal and vertical
+// neighbour pairs (beta's numerator; an exact integer), spread over
+// kSumSlots slots at sums[job * kSumSlots].  Every thread takes kRgbPx
+// pixels; the grid covers every image once.
 __global__ void __launch_bounds__(kRgbNT)
-k_rgb_pair_sum(const uint8_t* __restrict__ rgb, int W, int H, int ch, unsigned long long* sum) {
+k_rgb_pair_sum(const RgbJobs* __restrict__ J, int njobs, unsigned long long* sums) {
   __shared__ long long red[kRgbNT / kWave];
   __shared__ uint32_t own[kRgbStageWords], dn[kRgbStageWords];
+  __shared__ int s_j;
+  const int j = rgb_job(J, njobs, &s_j);
+  const RgbJob* jobs = J->job;
+  const uint8_t* rgb = jobs[j].rgb;
+  const int W = jobs[j].W, H = jobs[j].H, ch = jobs[j].ch;
   const long long n = (long long)W * H;
-  const long long p0 = (long long)blockIdx.x * kRgbBlockPx;
+  const long long p0 = (long long)(blockIdx.x - jobs[j].block0) * kRgbBlockPx;
   long long own0, dn0;
   rgb_stage_block(rgb, n, W, ch, p0, own, dn, &own0, &dn0);
   const uint8_t* O = reinterpret_cast<const uint8_t*>(own);
@@ -284,8 +335,8 @@ k_rgb_pair_sum(const uint8_t* __restrict__ rgb, int W, int H, int ch, unsigned l
   int x = i0 < n ? rgb_col(i0, W, n <= 0x7fffffffLL) : 0;
   long long acc = 0;
 #pragma unroll
-  for (int j = 0; j < kRgbPx; ++j) {
-    const long long i = i0 + j;
+  for (int k = 0; k < kRgbPx; ++k) {
+    const long long i = i0 + k;
     if (i < n) {
       const uint8_t* q = O + (i * ch - own0);
       if (x + 1 < W) acc += rgb_d2(q, q + ch);
@@ -294,17 +345,15 @@ k_rgb_pair_sum(const uint8_t* __restrict__ rgb, int W, int H, int ch, unsigned l
     if (++x == W) x = 0;
   }
   acc = block_sum64<kRgbNT>(acc, red);
-  if (threadIdx.x == 0 && acc) atomicAdd(&sum[blockIdx.x % kSumSlots], (unsigned long long)acc);
+  if (threadIdx.x == 0 && acc)
+    atomicAdd(&sums[(size_t)j * kSumSlots + blockIdx.x % kSumSlots], (unsigned long long)acc);
 }
 
-// Largest squared colour difference of two pixels: 3 * 255^2.
-constexpr int kD2Max = 3 * 255 * 255;
-
-// 2 beta of the exponential pairwise model from the pair sum's slots (1 if
-// the image has no pairs or they are all equal); 0 (unused) for the inverse
-// model.  Called by every thread of a block; result in every thread.
-__device__ double rgb_two_beta(const gcmi_rgb_model& m, int W, int H,
-                               const unsigned long long* __restrict__ pair_sum, double* slot) {
+// 2 beta of the exponential pairwise model from an image's pair-sum slots
+// (beta = 1 if the image has no pairs or they are all equal).  Called by
+// every thread of a block; result in every thread.
+__device__ double rgb_two_beta(int W, int H, const unsigned long long* __restrict__ pair_sum,
+                               double* slot) {
 #pragma clang fp contract(off)
   if (threadIdx.x < kWave) {
     unsigned long long v = 0;
@@ -340,31 +389,32 @@ __device__ __forceinline__ int32_t rgb_unary(const gcmi_rgb_model& m, const doub
   return (int32_t)rint(m.unary_scale * fmin(d / m.unary_div, m.unary_cap));
 }
 
-// The pairwise weight of every squared colour difference 0..kD2Max for one
-// image (the exponential model's beta is the image's).
+// Row-major int32 capacities of the launch's images from their colour
+// images (beta from each image's pair sum); kRgbPx pixels per thread, stored
+// as one 16-byte vector per array when whole (the outputs start 16-byte
+// aligned: pool pixel offsets are multiples of 32).
 __global__ void __launch_bounds__(kRgbNT)
-k_rgb_lut(gcmi_rgb_model m, int W, int H, const unsigned long long* __restrict__ pair_sum,
-          int32_t* __restrict__ lut) {
-  __shared__ double s_two_beta;
-  const double two_beta = m.pair_kind == GCMI_PAIR_EXP
-                              ? rgb_two_beta(m, W, H, pair_sum, &s_two_beta)
-                              : 0.0;
-  const int d2 = blockIdx.x * kRgbNT + threadIdx.x;
-  if (d2 <= kD2Max) lut[d2] = rgb_pair_weight(m, d2, two_beta);
-}
-
-// Row-major int32 capacities of one image from its colour image (pairwise
-// weights from the image's table); kRgbPx pixels per thread, stored as one
-// 16-byte vector per array when whole (the outputs start 16-byte aligned:
-// pool pixel offsets are multiples of 32).
-__global__ void __launch_bounds__(kRgbNT)
-k_rgb_caps(const uint8_t* __restrict__ rgb, int W, int H, int ch, gcmi_rgb_model m,
-           const int32_t* __restrict__ lut, int32_t* __restrict__ cs, int32_t* __restrict__ ct,
-           int32_t* __restrict__ right, int32_t* __restrict__ down) {
+k_rgb_caps(const RgbJobs* __restrict__ J, int njobs, const unsigned long long* __restrict__ sums,
+           int32_t* __restrict__ cs, int32_t* __restrict__ ct, int32_t* __restrict__ right,
+           int32_t* __restrict__ down) {
 #pragma clang fp contract(off)
   __shared__ uint32_t own[kRgbStageWords], dn[kRgbStageWords];
+  __shared__ int s_j;
+  const int j = rgb_job(J, njobs, &s_j);
+  const RgbJob* jobs = J->job;
+  const uint8_t* rgb = jobs[j].rgb;
+  const int W = jobs[j].W, H = jobs[j].H, ch = jobs[j].ch;
+  const gcmi_rgb_model& m = jobs[j].m;
+  __shared__ double s_two_beta;
+  const double two_beta = m.pair_kind == GCMI_PAIR_EXP
+      ? rgb_two_beta(W, H, sums + (size_t)j * kSumSlots, &s_two_beta) : 0.0;
+  const long long pix0 = jobs[j].pix0;
+  cs += pix0;
+  ct += pix0;
+  right += pix0;
+  down += pix0;
   const long long n = (long long)W * H;
-  const long long p0 = (long long)blockIdx.x * kRgbBlockPx;
+  const long long p0 = (long long)(blockIdx.x - jobs[j].block0) * kRgbBlockPx;
   long long own0, dn0;
   rgb_stage_block(rgb, n, W, ch, p0, own, dn, &own0, &dn0);
   const uint8_t* O = reinterpret_cast<const uint8_t*>(own);
@@ -373,16 +423,16 @@ k_rgb_caps(const uint8_t* __restrict__ rgb, int W, int H, int ch, gcmi_rgb_model
   int x = i0 < n ? rgb_col(i0, W, n <= 0x7fffffffLL) : 0;
   int32_t a[kRgbPx], b[kRgbPx], rr[kRgbPx], dd[kRgbPx];
 #pragma unroll
-  for (int j = 0; j < kRgbPx; ++j) {
-    const long long i = i0 + j;
-    a[j] = b[j] = rr[j] = dd[j] = 0;
+  for (int k = 0; k < kRgbPx; ++k) {
+    const long long i = i0 + k;
+    a[k] = b[k] = rr[k] = dd[k] = 0;
     if (i < n) {
       const uint8_t* q = O + (i * ch - own0);
       const double r = q[0], g = q[1], bl = q[2];
-      a[j] = rgb_unary(m, m.mu_bg, r, g, bl);
-      b[j] = rgb_unary(m, m.mu_fg, r, g, bl);
-      if (x + 1 < W) rr[j] = lut[rgb_d2(q, q + ch)];
-      if (i + W < n) dd[j] = lut[rgb_d2(q, D + ((i + W) * ch - dn0))];
+      a[k] = rgb_unary(m, m.mu_bg, r, g, bl);
+      b[k] = rgb_unary(m, m.mu_fg, r, g, bl);
+      if (x + 1 < W) rr[k] = rgb_pair_weight(m, rgb_d2(q, q + ch), two_beta);
+      if (i + W < n) dd[k] = rgb_pair_weight(m, rgb_d2(q, D + ((i + W) * ch - dn0)), two_beta);
     }
     if (++x == W) x = 0;
   }
@@ -393,11 +443,11 @@ k_rgb_caps(const uint8_t* __restrict__ rgb, int W, int H, int ch, gcmi_rgb_model
     *reinterpret_cast<int4*>(right + i0) = make_int4(rr[0], rr[1], rr[2], rr[3]);
     *reinterpret_cast<int4*>(down + i0) = make_int4(dd[0], dd[1], dd[2], dd[3]);
   } else {
-    for (int j = 0; j < kRgbPx && i0 + j < n; ++j) {
-      cs[i0 + j] = a[j];
-      ct[i0 + j] = b[j];
-      right[i0 + j] = rr[j];
-      down[i0 + j] = dd[j];
+    for (int k = 0; k < kRgbPx && i0 + k < n; ++k) {
+      cs[i0 + k] = a[k];
+      ct[i0 + k] = b[k];
+      right[i0 + k] = rr[k];
+      down[i0 + k] = dd[k];
     }
   }
 }
@@ -616,7 +666,7 @@ size_t pool_bytes(int tile, int cap_tiles, long long cap_pix, int cap_imgs) {
          (size_t)cap_pix * (4 * 4 + 1) + (size_t)cap_pix / 8 +
          (size_t)cap_imgs *
              (sizeof(ImgDesc) + SUM_ROWS * kSumSlots * 8 + sizeof(Prep) + 8 + kPixAlign * 17) +
-         (size_t)compact_scratch_ints(cap_tiles) * 4 + 8 * kSumSlots + 4 * (kD2Max + 1) + 4096;
+         (size_t)compact_scratch_ints(cap_tiles) * 4 + 8 * kSumSlots * kRgbJobsMax + 4096;
 }
 
 static int dev_alloc(void** ptr, size_t bytes, size_t* total) {
@@ -710,8 +760,7 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   ALLOC(p->plist, (size_t)cap_tiles * 4);
   ALLOC(p->alist, (size_t)cap_tiles * 4);
   ALLOC(p->counts, CNT_N * 4);
-  ALLOC(p->d_rgbsum, 8 * kSumSlots);
-  ALLOC(p->d_rgblut, 4 * (kD2Max + 1));
+  ALLOC(p->d_rgbsum, 8 * kSumSlots * kRgbJobsMax);
   ALLOC(p->d_imgfresh, (size_t)cap_imgs * 4);
   ALLOC(p->scratch, (size_t)compact_scratch_ints(cap_tiles) * 4);
   ALLOC(p->d_prep, (size_t)cap_imgs * sizeof(Prep));
@@ -759,13 +808,13 @@ void pool_free(Pool* p) {
                   (void*)d.down,      d.labels, p->d_img,   p->tile_img,  p->sel,     p->fresh,
                   p->q,    p->dlist,  p->plist, p->alist,   p->counts,    p->scratch, p->d_prep,
                   p->d_flows,
-                  p->d_edit, p->d_rgbsum, p->d_rgblut, p->d_rgb, p->d_jobs, p->d_imgfresh};
+                  p->d_edit, p->d_rgbsum, p->d_rgbjobs, p->d_rgb, p->d_jobs, p->d_imgfresh};
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   p->graph_exec = nullptr;
   for (void* x : ptrs)
     if (x) (void)hipFree(x);
-  void* hp[] = {p->h_counts, p->h_prep, p->h_flows, p->h_tally, p->h_edit, p->h_jobs};
+  void* hp[] = {p->h_counts, p->h_prep, p->h_flows, p->h_tally, p->h_edit, p->h_jobs, p->h_rgbjobs};
   for (void* x : hp)
     if (x) (void)hipHostFree(x);
   p->prof.release();
@@ -925,43 +974,83 @@ int check_rgb_model(const gcmi_rgb_model* m, int32_t* max_pair) {
   return GCMI_OK;
 }
 
+int pool_upload_rgb_many(Pool* p, int count, const RgbReq* reqs) {
+  if (count <= 0) return GCMI_OK;
+  static_assert(kPixAlign % 4 == 0, "colour capacities are stored four pixels at a time");
+  std::vector<int32_t> wmax(count);
+  for (int i = 0; i < count; ++i) {
+    const RgbReq& r = reqs[i];
+    GCMI_TRY(check_img(p, r.img));
+    if (r.channels != 3 && r.channels != 4) {
+      set_error("a colour image has 3 or 4 channels (got %d)", r.channels);
+      return GCMI_EINVAL;
+    }
+    GCMI_TRY(check_rgb_model(r.model, &wmax[i]));
+    if (!r.d_rgb || reinterpret_cast<uintptr_t>(r.d_rgb) % 4 != 0) {
+      set_error("a colour image on the device must start 4-byte aligned");
+      return GCMI_EINVAL;
+    }
+    for (int k = 0; k < i; ++k)
+      if (reqs[k].img == r.img) {
+        set_error("image %d has two colour uploads in one call", r.img);
+        return GCMI_EINVAL;
+      }
+  }
+  // staging: the job table of kRgbJobsMax images (allocated on first use)
+  if (!p->h_rgbjobs) {
+    GCMI_HIP(hipStreamSynchronize(p->stream));
+    if (hipHostMalloc((void**)&p->h_rgbjobs, sizeof(RgbJobs), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&p->d_rgbjobs, sizeof(RgbJobs)) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("colour staging allocation failed");
+      return GCMI_ENOMEM;
+    }
+  }
+  RgbJobs* hj = reinterpret_cast<RgbJobs*>(p->h_rgbjobs);
+  const RgbJobs* dj = reinterpret_cast<const RgbJobs*>(p->d_rgbjobs);
+  for (int base = 0; base < count; base += kRgbJobsMax) {
+    const int m = std::min(count - base, kRgbJobsMax);
+    // the job table may still be read by the previous launch's copy
+    GCMI_HIP(hipStreamSynchronize(p->stream));
+    long long blocks = 0;
+    for (int i = 0; i < m; ++i) {
+      const RgbReq& r = reqs[base + i];
+      const PoolImage& im = p->imgs[r.img];
+      hj->block0[i] = (int)blocks;
+      hj->job[i] = RgbJob{r.d_rgb, im.pix0, im.W, im.H, r.channels, (int)blocks, *r.model};
+      blocks += ((long long)im.W * im.H + kRgbBlockPx - 1) / kRgbBlockPx;
+      if (blocks > 0x7fffffffLL) {
+        set_error("colour images too large for one launch");
+        return GCMI_EINVAL;
+      }
+    }
+    GCMI_HIP(hipMemcpyAsync(p->d_rgbjobs, hj, offsetof(RgbJobs, job) + (size_t)m * sizeof(RgbJob),
+                            hipMemcpyHostToDevice, p->stream));
+    hipLaunchKernelGGL(k_fill_i32, dim3(std::max(1, m * kSumSlots * 2 / 256)), dim3(256), 0,
+                       p->stream, (int32_t*)p->d_rgbsum, 2LL * kSumSlots * m, 0);
+    if (blocks > 0) {
+      hipLaunchKernelGGL(k_rgb_pair_sum, dim3((unsigned)blocks), dim3(kRgbNT), 0, p->stream, dj,
+                         m, p->d_rgbsum);
+      hipLaunchKernelGGL(k_rgb_caps, dim3((unsigned)blocks), dim3(kRgbNT), 0, p->stream, dj, m,
+                         (const unsigned long long*)p->d_rgbsum, p->dev.cs, p->dev.ct,
+                         (int32_t*)p->dev.right, (int32_t*)p->dev.down);
+    }
+    GCMI_HIP(hipGetLastError());
+    for (int i = 0; i < m; ++i) {
+      PoolImage& im = p->imgs[reqs[base + i].img];
+      im.max_pair = wmax[base + i];
+      im.uploaded = true;
+      im.fresh = true;
+      im.solved = false;
+    }
+  }
+  return GCMI_OK;
+}
+
 int pool_upload_rgb(Pool* p, int img, const uint8_t* d_rgb, int channels,
                     const gcmi_rgb_model* m) {
-  GCMI_TRY(check_img(p, img));
-  if (channels != 3 && channels != 4) {
-    set_error("a colour image has 3 or 4 channels (got %d)", channels);
-    return GCMI_EINVAL;
-  }
-  int32_t wmax = 0;
-  GCMI_TRY(check_rgb_model(m, &wmax));
-  PoolImage& im = p->imgs[img];
-  static_assert(kPixAlign % 4 == 0, "colour capacities are stored four pixels at a time");
-  if (reinterpret_cast<uintptr_t>(d_rgb) % 4 != 0) {
-    set_error("a colour image on the device must start 4-byte aligned");
-    return GCMI_EINVAL;
-  }
-  const long long npx = (long long)im.W * im.H;
-  const long long blocks = (npx + kRgbNT * kRgbPx - 1) / (kRgbNT * kRgbPx);
-  if (blocks > 0x7fffffffLL) {
-    set_error("colour image too large");
-    return GCMI_EINVAL;
-  }
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, p->stream, (int32_t*)p->d_rgbsum,
-                     2LL * kSumSlots, 0);
-  hipLaunchKernelGGL(k_rgb_pair_sum, dim3((unsigned)blocks), dim3(kRgbNT), 0, p->stream, d_rgb,
-                     im.W, im.H, channels, p->d_rgbsum);
-  hipLaunchKernelGGL(k_rgb_lut, dim3((kD2Max + kRgbNT) / kRgbNT), dim3(kRgbNT), 0, p->stream, *m,
-                     im.W, im.H, (const unsigned long long*)p->d_rgbsum, p->d_rgblut);
-  hipLaunchKernelGGL(k_rgb_caps, dim3((unsigned)blocks), dim3(kRgbNT), 0, p->stream, d_rgb, im.W,
-                     im.H, channels, *m, (const int32_t*)p->d_rgblut, p->dev.cs + im.pix0,
-                     p->dev.ct + im.pix0, (int32_t*)p->dev.right + im.pix0,
-                     (int32_t*)p->dev.down + im.pix0);
-  GCMI_HIP(hipGetLastError());
-  im.max_pair = wmax;
-  im.uploaded = true;
-  im.fresh = true;
-  im.solved = false;
-  return GCMI_OK;
+  const RgbReq r{img, d_rgb, channels, m};
+  return pool_upload_rgb_many(p, 1, &r);
 }
 
 int pool_upload_rgb_host(Pool* p, int img, const uint8_t* h_rgb, int channels,

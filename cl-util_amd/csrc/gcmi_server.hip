@@ -173,6 +173,7 @@ int serve_batch(gcmi_server* s, std::vector<Request*>& batch, hipEvent_t done) {
   // so no image is both edited and replaced)
   if (!paints.empty()) GCMI_TRY(pool_paint_many(p, (int)paints.size(), paints.data()));
   if (!edits.empty()) GCMI_TRY(pool_edit_many(p, (int)edits.size(), edits.data()));
+  std::vector<RgbReq> rgbs;
   for (Request* r : images) {
     const Session& ss = s->sessions[r->session];
     const size_t npx = (size_t)ss.W * ss.H;
@@ -181,10 +182,11 @@ int serve_batch(gcmi_server* s, std::vector<Request*>& batch, hipEvent_t done) {
       GCMI_TRY(pool_upload(p, ss.img, d, d + npx, d + 2 * npx, d + 3 * npx, true));
       p->imgs[ss.img].max_pair = r->max_pair;
     } else {
-      GCMI_TRY(pool_upload_rgb(p, ss.img, (const uint8_t*)(s->d_stage + r->stage), r->n,
-                               &r->model));
+      rgbs.push_back(RgbReq{ss.img, (const uint8_t*)(s->d_stage + r->stage), r->n, &r->model});
     }
   }
+  // every colour image of the batch in one set of launches
+  if (!rgbs.empty()) GCMI_TRY(pool_upload_rgb_many(p, (int)rgbs.size(), rgbs.data()));
   const int64_t t1 = now_ns();
   GCMI_TRY(pool_solve(p, imgs.data(), (int)imgs.size(), s->sweeps, s->bfs, 0, nullptr));
   const int64_t t2 = now_ns();
