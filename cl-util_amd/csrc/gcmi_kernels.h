@@ -64,6 +64,9 @@ struct Geo {
   int ntiles;               // pool capacity in tiles
   const ImgDesc* img;       // per image
   const int32_t* tile_img;  // per pool tile: its image
+  // per pool tile, its geometry in two int4 (one 32-byte line):
+  // {img, tx | ty << 16, tw | th << 16, W}, {H, pix0 low, pix0 high, 0}
+  const int4* tgeo;
 };
 
 // Device pointers of a pool's state, passed by value.
@@ -100,21 +103,23 @@ struct TG {
   long long pix0;
 };
 
+// (one load of the tile's geometry line, no dependent load of its image's
+// descriptor)
 __device__ __forceinline__ TG tile_geo(const Geo& g, int t) {
+  const int4 a = g.tgeo[2 * (size_t)t], b = g.tgeo[2 * (size_t)t + 1];
   TG r;
   r.t = t;
-  r.img = g.tile_img[t];
-  const ImgDesc d = g.img[r.img];
-  const int lt = t - d.tile0;
-  r.tw = d.tw;
-  r.th = d.th;
-  r.tx = lt % d.tw;
-  r.ty = lt / d.tw;
-  r.W = d.W;
-  r.H = d.H;
-  r.pix0 = d.pix0;
+  r.img = a.x;
+  r.tx = a.y & 0xffff;
+  r.ty = (int)((unsigned)a.y >> 16);
+  r.tw = a.z & 0xffff;
+  r.th = (int)((unsigned)a.z >> 16);
+  r.W = a.w;
+  r.H = b.x;
+  r.pix0 = (long long)(((unsigned long long)(unsigned)b.z << 32) | (unsigned)b.y);
   return r;
 }
+
 
 template <int T>
 __device__ __forceinline__ size_t tbase(int t) {
@@ -947,9 +952,10 @@ k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __r
       int v = g.maxh, in = 0;
       if (hs.nb >= 0) {
         const size_t p = tbase<T>(hs.nb) + hs.ny * T + hs.nx;
-        const int mq = s.mask[p];
+        // (all three loads issued together)
+        const int mq = s.mask[p], hq = s.h[p];
         const bool listed = all || s.rflag[hs.nb];
-        v = listed ? ((mq & M_SINK) ? 0 : g.maxh) : s.h[p];
+        v = listed ? ((mq & M_SINK) ? 0 : g.maxh) : hq;
         in = ((mq & arc_into(threadIdx.x / T)) ? 1 : 0) | (listed ? 2 : 0);
       }
       D0[hs.idx] = v;

@@ -144,6 +144,7 @@ struct Pool {
   // device arrays (beyond those in dev)
   ImgDesc* d_img = nullptr;
   int32_t* tile_img = nullptr;
+  int4* tgeo = nullptr;  // per tile: two int4 of geometry (tile_geo)
   int32_t *sel = nullptr, *fresh = nullptr, *plist = nullptr, *alist = nullptr;
   int32_t* q = nullptr;  // three rotating tile queues of cap_tiles entries
   int32_t* dlist = nullptr;  // dirty tiles of the current relabel

@@ -90,6 +90,19 @@ __global__ void k_fill_i32(int32_t* p, long long n, int32_t v) {
     p[i] = v;
 }
 
+// Writes the geometry lines of image `img`'s tiles.
+__global__ void k_tgeo_fill(int4* tgeo, ImgDesc d, int img) {
+  for (int lt = blockIdx.x * blockDim.x + threadIdx.x; lt < d.ntiles;
+       lt += gridDim.x * blockDim.x) {
+    const size_t t = (size_t)d.tile0 + lt;
+    const int tx = lt % d.tw, ty = lt / d.tw;
+    tgeo[2 * t] = make_int4(img, (int)((unsigned)tx | ((unsigned)ty << 16)),
+                            (int)((unsigned)d.tw | ((unsigned)d.th << 16)), d.W);
+    tgeo[2 * t + 1] = make_int4(d.H, (int)(unsigned)(unsigned long long)d.pix0,
+                                (int)(unsigned)((unsigned long long)d.pix0 >> 32), 0);
+  }
+}
+
 __global__ void k_clear_listed(int32_t* flags, const int32_t* __restrict__ list,
                                const int32_t* __restrict__ count) {
   const int n = *count;
@@ -657,12 +670,16 @@ int tiles_for(int W, int H, int tile, int* tw, int* th) {
   }
   *tw = (W + tile - 1) / tile;
   *th = (H + tile - 1) / tile;
+  if (*tw > 0xffff || *th > 0xffff) {
+    set_error("an image is at most 65535 tiles across and down (got %d x %d)", *tw, *th);
+    return GCMI_EINVAL;
+  }
   return GCMI_OK;
 }
 
 size_t pool_bytes(int tile, int cap_tiles, long long cap_pix, int cap_imgs) {
   const size_t np = (size_t)cap_tiles * tile * tile;
-  return np * (6 * 4 + 2) + (size_t)cap_tiles * (4 * tile * 4 + 16 * 4 + 8) +
+  return np * (6 * 4 + 2) + (size_t)cap_tiles * (4 * tile * 4 + 16 * 4 + 8 + 32) +
          (size_t)cap_pix * (4 * 4 + 1) + (size_t)cap_pix / 8 +
          (size_t)cap_imgs *
              (sizeof(ImgDesc) + SUM_ROWS * kSumSlots * 8 + sizeof(Prep) + 8 + kPixAlign * 17) +
@@ -753,6 +770,7 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   ALLOC(d.labels, (size_t)cap_pix);
   ALLOC(p->d_img, (size_t)cap_imgs * sizeof(ImgDesc));
   ALLOC(p->tile_img, (size_t)cap_tiles * 4);
+  ALLOC(p->tgeo, (size_t)cap_tiles * 32);
   ALLOC(p->sel, (size_t)cap_tiles * 4);
   ALLOC(p->fresh, (size_t)cap_tiles * 4);
   ALLOC(p->q, (size_t)cap_tiles * 3 * 4);
@@ -786,6 +804,7 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   p->geo.ntiles = cap_tiles;
   p->geo.img = p->d_img;
   p->geo.tile_img = p->tile_img;
+  p->geo.tgeo = p->tgeo;
   p->imgs.assign(cap_imgs, PoolImage());
   p->tiles_alloc.init(cap_tiles);
   p->pix_alloc.init(cap_pix);
@@ -807,7 +826,7 @@ void pool_free(Pool* p) {
                   d.tpos,  d.sums,    d.tally,  d.cs,       d.ct,         (void*)d.right,
                   (void*)d.down,      d.labels, p->d_img,   p->tile_img,  p->sel,     p->fresh,
                   p->q,    p->dlist,  p->plist, p->alist,   p->counts,    p->scratch, p->d_prep,
-                  p->d_flows,
+                  p->d_flows, p->tgeo,
                   p->d_edit, p->d_rgbsum, p->d_rgbjobs, p->d_rgb, p->d_jobs, p->d_imgfresh};
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
@@ -865,6 +884,8 @@ int pool_add_image(Pool* p, int W, int H, int* out) {
                      dim3(256), 0, p->stream, (int32_t*)(p->dev.packed + p0 / 32), words, 0);
   hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)std::min<long long>((nt + 255) / 256, 4096)),
                      dim3(256), 0, p->stream, p->tile_img + t0, nt, (int32_t)id);
+  hipLaunchKernelGGL(k_tgeo_fill, dim3((unsigned)std::min<long long>((nt + 255) / 256, 4096)),
+                     dim3(256), 0, p->stream, p->tgeo, d, id);
   GCMI_HIP(hipGetLastError());
   GCMI_HIP(hipStreamSynchronize(p->stream));
   *out = id;
