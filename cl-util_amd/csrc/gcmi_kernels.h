@@ -1380,7 +1380,13 @@ template <int T>
 __device__ __forceinline__ void edit_pixel(const CutDev& s, const ImgDesc& d, int x, int y, int a1,
                                            int b1, int warm, long long& dmin, long long& dpos) {
   const size_t r = d.pix0 + (size_t)y * d.W + x;
+  const int t = d.tile0 + (y / T) * d.tw + x / T;
+  const size_t q = tbase<T>(t) + (y % T) * T + (x % T);
+  // (the state is read with the unaries, in one round trip, whether or not
+  // the excess changes)
   const int a0 = s.cs[r], b0 = s.ct[r];
+  const int e0 = warm ? s.e[q] : 0;
+  const int m0 = warm ? s.mask[q] : 0;
   s.cs[r] = a1;
   s.ct[r] = b1;
   if (!warm) return;  // no state yet: the first round reads the unaries
@@ -1388,13 +1394,10 @@ __device__ __forceinline__ void edit_pixel(const CutDev& s, const ImgDesc& d, in
   dpos += (long long)(a1 - b1 > 0 ? a1 - b1 : 0) - (a0 - b0 > 0 ? a0 - b0 : 0);
   const int de = (a1 - b1) - (a0 - b0);
   if (de == 0) return;
-  const int t = d.tile0 + (y / T) * d.tw + x / T;
-  const size_t q = tbase<T>(t) + (y % T) * T + (x % T);
-  const int e0 = s.e[q];
   const int e1 = e0 + de;
   s.e[q] = e1;
-  s.mask[q] = (uint8_t)((s.mask[q] & (M_UP | M_DOWN | M_LEFT | M_RIGHT)) |
-                        (e1 < 0 ? M_SINK : 0) | (e1 > 0 ? M_EXCESS : 0));
+  s.mask[q] = (uint8_t)((m0 & (M_UP | M_DOWN | M_LEFT | M_RIGHT)) | (e1 < 0 ? M_SINK : 0) |
+                        (e1 > 0 ? M_EXCESS : 0));
   // a new deficit is a sink at height 0; the tile pushes before the first
   // relabel of the next solve
   if (e1 < 0) s.h[q] = 0;
@@ -1440,6 +1443,7 @@ k_edit(Geo g, CutDev s, int img, int n, const int32_t* __restrict__ idx,
 // polyline -- a union of capsules, one per segment -- get unaries (cs, ct).
 constexpr int kMaxStrokePoints = 16;
 constexpr int kPaintCols = 256;  // columns of one row of one segment painted by one wave
+constexpr int kPaintRows = 4;    // rows of one segment painted by one wave
 struct Polyline {
   double x[kMaxStrokePoints], y[kMaxStrokePoints];
   double r;  // width / 2
@@ -1448,9 +1452,9 @@ struct Polyline {
   int bx0, by0, bw, bh;  // pixel box covering the stroke, clipped to the image
   int W;                 // image width
   // rows of the image each segment's capsule can reach: segment k covers
-  // nr[k] rows from ya[k], each in xc[k] pieces of kPaintCols columns (its
-  // box's width); segment k's pieces start at piece c0[k] of the stroke,
-  // row by row; nchunks pieces in all
+  // nr[k] rows from ya[k], in groups of kPaintRows rows, each row in xc[k]
+  // pieces of kPaintCols columns; segment k's (group, piece) chunks start at
+  // chunk c0[k] of the stroke, group by group; nchunks chunks in all
   int ya[kMaxStrokePoints], nr[kMaxStrokePoints], xc[kMaxStrokePoints], c0[kMaxStrokePoints];
   int nchunks;
   // pixel box of each segment's capsule (a pixel outside it is not in it)
@@ -1523,11 +1527,11 @@ struct PaintJob {
 constexpr int kPaintJobsLds = 256;  // jobs whose first chunks a block stages in LDS
 
 // Paints the strokes of `jobs` (chunk0 ascending): one wave per (stroke,
-// segment, row, piece of kPaintCols columns) a segment's capsule can reach,
-// its lanes over the piece's candidate columns.  A pixel is painted by the
-// first segment of its stroke whose capsule holds it, so every pixel of a
-// stroke's union is edited once (strokes of one launch are of different
-// images).
+// segment, kPaintRows rows, piece of kPaintCols columns) a segment's capsule
+// can reach, its lanes over the piece's candidate columns of each row in
+// turn.  A pixel is painted by the first segment of its stroke whose capsule
+// holds it, so every pixel of a stroke's union is edited once (strokes of
+// one launch are of different images).
 template <int T>
 __global__ void __launch_bounds__(256)
 k_paint(Geo g, CutDev s, const PaintJob* __restrict__ jobs, int njobs, int nchunks) {
@@ -1554,25 +1558,26 @@ k_paint(Geo g, CutDev s, const PaintJob* __restrict__ jobs, int njobs, int nchun
     const unsigned long long le = __ballot(lane < nseg && pl.c0[lane] <= local);
     const int k = 63 - __clzll(le);
     const int piece = local - pl.c0[k];
-    const int y = pl.ya[k] + piece / pl.xc[k];
+    const int ya = pl.ya[k] + (piece / pl.xc[k]) * kPaintRows;
+    const int yb = min(ya + kPaintRows, pl.ya[k] + pl.nr[k]);
     const int xpiece = piece % pl.xc[k];
     const ImgDesc d = g.img[J.img];
     long long dmin = 0, dpos = 0;
-    double xl, xr;
-    if (seg_row_span(pl, k, (double)y, &xl, &xr)) {
+    for (int y = ya; y < yb; ++y) {
+      double xl, xr;
+      if (!seg_row_span(pl, k, (double)y, &xl, &xr)) continue;
       xl = fmax(xl, 0.0);
       xr = fmin(xr, (double)(pl.W - 1));
-      if (xl <= xr) {
-        const int x0 = (int)floor(xl) + xpiece * kPaintCols;
-        const int x1 = min((int)ceil(xr), x0 + kPaintCols - 1);
-        for (int x = x0 + lane; x <= x1; x += kWave) {
-          if (!in_capsule(pl, k, (double)x, (double)y)) continue;
-          bool earlier = false;
-          for (int j = 0; j < k && !earlier; ++j)
-            earlier = x >= pl.sx0[j] && x <= pl.sx1[j] && y >= pl.sy0[j] && y <= pl.sy1[j] &&
-                      in_capsule(pl, j, (double)x, (double)y);
-          if (!earlier) edit_pixel<T>(s, d, x, y, pl.cs, pl.ct, J.warm, dmin, dpos);
-        }
+      if (xl > xr) continue;
+      const int x0 = (int)floor(xl) + xpiece * kPaintCols;
+      const int x1 = min((int)ceil(xr), x0 + kPaintCols - 1);
+      for (int x = x0 + lane; x <= x1; x += kWave) {
+        if (!in_capsule(pl, k, (double)x, (double)y)) continue;
+        bool earlier = false;
+        for (int j = 0; j < k && !earlier; ++j)
+          earlier = x >= pl.sx0[j] && x <= pl.sx1[j] && y >= pl.sy0[j] && y <= pl.sy1[j] &&
+                    in_capsule(pl, j, (double)x, (double)y);
+        if (!earlier) edit_pixel<T>(s, d, x, y, pl.cs, pl.ct, J.warm, dmin, dpos);
       }
     }
     if (J.warm) {

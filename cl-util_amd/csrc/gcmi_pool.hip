@@ -572,8 +572,8 @@ struct PoolKernels {
                        p->dev, img, n, idx, cs, ct, p->imgs[img].fresh ? 0 : 1);
   }
   static void paint(Pool* p, const PaintJob* d_jobs, int njobs, int nchunks) {
-    // a wave per (stroke, segment, row, piece of kPaintCols columns), four
-    // per block
+    // a wave per (stroke, segment, kPaintRows rows, piece of kPaintCols
+    // columns), four per block
     const int grid = std::min((nchunks + 3) / 4, 8192);
     hipLaunchKernelGGL((k_paint<T>), dim3(grid > 0 ? grid : 1), dim3(256), 0, p->stream, p->geo,
                        p->dev, d_jobs, njobs, nchunks);
@@ -1299,7 +1299,7 @@ int make_polyline(int W, int H, int npts, const double* xy, double width, int32_
     pl.sy0[k] = (int)std::max<long long>((long long)std::floor(ylo) - 1, -1);
     pl.sy1[k] = (int)std::min<long long>((long long)std::ceil(yhi) + 1, H);
     pl.c0[k] = (int)std::min<long long>(total, INT32_MAX);
-    total += rows * pl.xc[k];
+    total += (rows + kPaintRows - 1) / kPaintRows * pl.xc[k];
   }
   if (total > INT32_MAX) {
     set_error("stroke spans too many rows");
