@@ -205,8 +205,9 @@ class GraphCut:
         return dict(zip(STAT_NAMES, [int(v) for v in a]))
 
     def set_local_relabel(self, enable=True):
-        """Reflows of edits relabel only the changed tiles (and a ring around
-        them) for their first relabels; the cut is the same either way."""
+        """Reflows of edits, and every run after its first relabel, relabel
+        only the changed tiles for their first relabels; the cut is the same
+        either way."""
         check(lib().gcmi_cut_set_option(self._h, OPT_LOCAL_RELABEL, 1 if enable else 0))
 
     def set_device_rounds(self, rounds=8):

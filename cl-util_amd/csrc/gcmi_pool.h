@@ -148,12 +148,12 @@ struct Pool {
   int32_t *sel = nullptr, *fresh = nullptr, *plist = nullptr, *alist = nullptr;
   int32_t* q = nullptr;  // three rotating tile queues of cap_tiles entries
   int32_t* dlist = nullptr;  // dirty tiles of the current relabel
-  // In solves with edited images: relabel the tiles that changed (and one
-  // ring around them; every tile of a fresh image has) instead of every
-  // tile.  Clean tiles keep their
-  // stored distances, valid but possibly stale; the solve switches to full
-  // relabels after kLocalRelabels.  Exact either way: termination is
-  // decided by a relabel over a valid labelling.
+  // In solves with edited images, and after a solve's first relabel:
+  // relabel the tiles that changed (every tile of a fresh image has)
+  // instead of every tile.  Clean tiles keep their stored distances, valid
+  // but possibly stale; the solve switches to full relabels after
+  // kLocalRelabels.  Exact either way: termination is decided by a relabel
+  // over a valid labelling.
   bool local_relabel = false;
   bool trace = false;  // GCMI_TRACE=1: per-phase lines on stderr, one round per host check
   int ep_par = 0;         // epoch slot the next queue round reads

@@ -161,10 +161,12 @@ k_imp_closure(Geo g, CutDev s, const int32_t* __restrict__ init, const int32_t* 
   }
 }
 
-// Local relabel set: rflag[t] = t or one of its four neighbours is dirty
-// (one ring around every changed tile, so a distance that rose next to an
-// edit is recomputed rather than left to the stale neighbours); the set's
-// tiles are appended to `out` in any order (*out_count zero before).
+// Local relabel set: rflag[t] = t is dirty (its arcs or excess changed; a
+// push dirties both tiles of every arc it changes), the set's tiles
+// appended to `out` in any order (*out_count zero before).  No ring of clean
+// tiles is needed: a distance that rose in a dirty tile only loosens the
+// constraint h(c) <= h(d) + 1 on a clean neighbour's stored distance, and
+// one that fell is passed on by the rounds.
 __global__ void k_dilate(Geo g, CutDev s, const int32_t* __restrict__ list,
                          const int32_t* __restrict__ count, int32_t* out, int32_t* out_count) {
   const int n = *count;
@@ -176,10 +178,6 @@ __global__ void k_dilate(Geo g, CutDev s, const int32_t* __restrict__ list,
       const TG q = tile_geo(g, list ? list[i] : i);
       t = q.t;
       f = s.dirty[t];
-      if (!f && q.ty > 0) f = s.dirty[t - q.tw];
-      if (!f && q.ty < q.th - 1) f = s.dirty[t + q.tw];
-      if (!f && q.tx > 0) f = s.dirty[t - 1];
-      if (!f && q.tx < q.tw - 1) f = s.dirty[t + 1];
       s.rflag[t] = f;
     }
     const unsigned long long m = __ballot(f != 0);

@@ -235,9 +235,8 @@ int gcmi_cut_profile(gcmi_cut *cut, int enable);
 /* Solver options (no reference counterpart).
  * GCMI_OPT_LOCAL_RELABEL (default 0): in runs that reflow edits, and after
  *   the first relabel of any run, the global relabel recomputes the tiles
- *   whose arcs or excess changed (and one ring of tiles around them) instead
- *   of every tile, for at most a few relabels per run; the result is the
- *   same exact cut either way. */
+ *   whose arcs or excess changed instead of every tile, for at most a few
+ *   relabels per run; the result is the same exact cut either way. */
 #define GCMI_OPT_LOCAL_RELABEL 1
 /* GCMI_OPT_DEVICE_ROUNDS (default 0, at most 64): value > 0 makes runs
  *   device-driven: two push/relabel iterations (three when edits of more

@@ -1,5 +1,5 @@
-"""Warm reflow with the local relabel option (only the tiles that changed,
-and a ring around them, are relabelled for the first relabels of a run):
+"""Warm reflow with the local relabel option (only the tiles that changed
+are relabelled for the first relabels of a run):
 after every edit the cut equals a from-scratch oracle cut, exactly as with
 full relabels."""
 import numpy as np
