@@ -81,7 +81,7 @@ struct CutDev {
   int32_t* bflag;                      // tile flagged by the relabel's first pass
   int32_t* dirty;                      // residual arcs or excess changed since the
                                        // tile's distances were last computed
-  int32_t* rflag;                      // tile being recomputed by a local relabel
+  int32_t* rflag;                      // epoch of the local relabel recomputing the tile
   int32_t* haspos;                     // tile holds excess
   uint32_t* packed;                    // per pool pixel: its label as one bit (row-major,
                                        // each image from a word of its own)
@@ -907,7 +907,10 @@ __device__ __forceinline__ int bfs_relax(int (&d)[T * T / NT], const int (&m)[T 
 //   act[t]    = some pixel has e > 0 and a finite distance
 //   haspos[t] = some pixel has e > 0
 // all != 0: every tile of the images is listed (a full relabel); else the
-// listed tiles are those with rflag set.
+// listed tiles are those whose rflag holds this pass's epoch (*Q.ep_in).
+// A listed tile that was dirty is marked changed for the next flood and
+// its dirty flag cleared; *zero (the count the relabel's active list is
+// compacted into) is zeroed.
 // Halo pixels of listed neighbours start as 0 (sink) or MAX; those of clean
 // neighbours keep their stored distances, which stay valid: a clean tile's
 // arcs and excess have not changed since its distances were computed, and
@@ -918,7 +921,7 @@ __device__ __forceinline__ int bfs_relax(int (&d)[T * T / NT], const int (&m)[T 
 template <int T, int NT>
 __global__ void __launch_bounds__(NT)
 k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
-           int all, Queue Q) {
+           int all, Queue Q, int32_t* zero) {
   constexpr int R = T * T / NT;
   constexpr int HP = T + 2;
   __shared__ int32_t D0[HP * HP], D1[HP * HP];
@@ -927,10 +930,14 @@ k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __r
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
   bq_init(bq, Q);
   const int n = *count;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) {
-    atomicAdd(&s.tally[TALLY_RELABELS], 1ull);
-    atomicAdd(&s.tally[TALLY_RELABEL_TILES], (unsigned long long)n);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *zero = 0;
+    if (n > 0) {
+      atomicAdd(&s.tally[TALLY_RELABELS], 1ull);
+      atomicAdd(&s.tally[TALLY_RELABEL_TILES], (unsigned long long)n);
+    }
   }
+  const int epoch = bq.epoch;
   for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
     const TG q = tile_geo(g, list ? list[pos] : pos);
     const size_t base = tbase<T>(q.t);
@@ -954,7 +961,7 @@ k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __r
         const size_t p = tbase<T>(hs.nb) + hs.ny * T + hs.nx;
         // (all three loads issued together)
         const int mq = s.mask[p], hq = s.h[p];
-        const bool listed = all || s.rflag[hs.nb];
+        const bool listed = all || s.rflag[hs.nb] == epoch;
         v = listed ? ((mq & M_SINK) ? 0 : g.maxh) : hq;
         in = ((mq & arc_into(threadIdx.x / T)) ? 1 : 0) | (listed ? 2 : 0);
       }
@@ -979,6 +986,11 @@ k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __r
     if (threadIdx.x == 0) {
       s.haspos[q.t] = any_pos;
       s.act[q.t] = any_act;
+      // (no other block reads this tile's dirty flag during the pass)
+      if (s.dirty[q.t]) {
+        s.fchg[q.t] = 1;
+        s.dirty[q.t] = 0;
+      }
     }
     bq_take(bq, q, s.stamp, Q);
   }

@@ -110,24 +110,6 @@ __global__ void k_clear_listed(int32_t* flags, const int32_t* __restrict__ list,
     flags[list ? list[i] : i] = 0;
 }
 
-// Per listed tile: a dirty tile (arcs or excess changed since its last
-// relabel) is marked changed for the next flood, and its dirty flag cleared;
-// clear (may be null): a flag zeroed for every listed tile.  Also zeroes the
-// active count for the compaction that follows it in a relabel.
-__global__ void k_take_dirty(CutDev s, const int32_t* __restrict__ list,
-                             const int32_t* __restrict__ count, int32_t* counts,
-                             int32_t* clear) {
-  const int n = *count;
-  if (blockIdx.x == 0 && threadIdx.x == 0) counts[CNT_ACT] = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int t = list ? list[i] : i;
-    if (s.dirty[t]) {
-      s.fchg[t] = 1;
-      s.dirty[t] = 0;
-    }
-    if (clear) clear[t] = 0;
-  }
-}
 
 // The tiles whose reach a flood must recompute: the changed tiles (fchg, the
 // `count` tiles of `init`) and, transitively, every tile that imported
@@ -161,15 +143,19 @@ k_imp_closure(Geo g, CutDev s, const int32_t* __restrict__ init, const int32_t* 
   }
 }
 
-// Local relabel set: rflag[t] = t is dirty (its arcs or excess changed; a
-// push dirties both tiles of every arc it changes), the set's tiles
-// appended to `out` in any order (*out_count zero before).  No ring of clean
+// Local relabel set: the dirty tiles (arcs or excess changed; a push dirties
+// both tiles of every arc it changes), appended to `out` in any order
+// (*out_count zero before) and marked rflag[t] = the epoch the relabel's
+// first pass runs with (*epoch; older marks never equal it, so none is
+// cleared).  No ring of clean
 // tiles is needed: a distance that rose in a dirty tile only loosens the
 // constraint h(c) <= h(d) + 1 on a clean neighbour's stored distance, and
 // one that fell is passed on by the rounds.
 __global__ void k_dilate(Geo g, CutDev s, const int32_t* __restrict__ list,
-                         const int32_t* __restrict__ count, int32_t* out, int32_t* out_count) {
+                         const int32_t* __restrict__ count, int32_t* out, int32_t* out_count,
+                         const int32_t* __restrict__ epoch) {
   const int n = *count;
+  const int ep = *epoch;
   const unsigned long long below = (1ull << lane_id()) - 1;
   for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const int i = base + threadIdx.x;
@@ -178,7 +164,7 @@ __global__ void k_dilate(Geo g, CutDev s, const int32_t* __restrict__ list,
       const TG q = tile_geo(g, list ? list[i] : i);
       t = q.t;
       f = s.dirty[t];
-      s.rflag[t] = f;
+      if (f) s.rflag[t] = ep;
     }
     const unsigned long long m = __ballot(f != 0);
     if (m == 0) continue;
@@ -537,7 +523,7 @@ struct PoolKernels {
   static void bfs_init(Pool* p, int grid, const int32_t* list, int cnt, bool all,
                        const Queue& q) {
     hipLaunchKernelGGL((k_bfs_init<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
-                       list, p->counts + cnt, all ? 1 : 0, q);
+                       list, p->counts + cnt, all ? 1 : 0, q, p->counts + CNT_ACT);
   }
   static void bfs_round(Pool* p, int grid, const Queue& q) {
     hipLaunchKernelGGL((k_bfs_round<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
@@ -1592,8 +1578,10 @@ int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32
   if (!S.dd)
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, p->stream, p->counts + CNT_Q, 4LL, 0);
   if (local) {
+    // (marked with the epoch slot the first pass will read)
     hipLaunchKernelGGL(k_dilate, dim3(S.grid_clear()), dim3(256), 0, p->stream, p->geo, p->dev,
-                       p->lsel, p->counts + CNT_SEL, p->dlist, p->counts + CNT_DIRTY);
+                       p->lsel, p->counts + CNT_SEL, p->dlist, p->counts + CNT_DIRTY,
+                       p->counts + CNT_EP + p->ep_par);
     glist = p->dlist;
     gcnt = CNT_DIRTY;
   }
@@ -1613,8 +1601,6 @@ int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32
   S.ops->bfs_init(p, S.grid_relabel, glist, gcnt, !local, first);
   S.E();
   GCMI_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_take_dirty, dim3(S.grid_clear()), dim3(256), 0, p->stream, p->dev, glist,
-                     p->counts + gcnt, p->counts, local ? p->dev.rflag : nullptr);
   int cur = 0;
   const int32_t* hc = p->h_counts;
   if (S.dd) {
@@ -1950,8 +1936,11 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
   for (int k = 0; k < STAT_N; ++k) p->stats[k] = 0;
   p->stats[STAT_TILES] = nsel;
   if (p->ep_reset) {
+    // stamps and local-relabel marks hold epochs
     hipLaunchKernelGGL(k_fill_i32, dim3(std::min((p->cap_tiles + 255) / 256, 4096)), dim3(256), 0,
                        p->stream, p->dev.stamp, (long long)p->cap_tiles, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(std::min((p->cap_tiles + 255) / 256, 4096)), dim3(256), 0,
+                       p->stream, p->dev.rflag, (long long)p->cap_tiles, 0);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, p->stream, p->counts + CNT_EP, 2LL, 0);
     GCMI_HIP(hipGetLastError());
     p->ep_reset = false;
