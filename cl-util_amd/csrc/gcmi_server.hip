@@ -94,10 +94,20 @@ struct Session {
 
 }  // namespace
 
-// A solved batch whose masks are still being copied out.
+// One mask copy of a solved batch: the session's packed bits on the device
+// to its host buffer.
+struct MaskCopy {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+
+// A solved batch whose masks are still to be copied out.
 struct Pending {
   std::vector<Request*> batch;
-  hipEvent_t done = nullptr;  // recorded on out_stream after the last copy
+  std::vector<MaskCopy> copies;
+  hipEvent_t solved = nullptr;  // recorded on the pool's stream after the solve
+  hipEvent_t done = nullptr;    // recorded on out_stream after the last copy
 };
 
 struct gcmi_server {
@@ -111,11 +121,10 @@ struct gcmi_server {
   int32_t* d_stage = nullptr;  // device staging for new images
   RangeAlloc stage_alloc;
   hipStream_t copy_stream = nullptr;
-  // Masks are packed and copied out on out_stream, after the solve that
-  // produced them, while the worker goes on with the next batch; the
-  // finisher thread completes a batch's requests when its copies are done.
+  // Masks are copied out on out_stream, after the solve that produced them:
+  // the finisher thread issues a batch's copies and completes its requests
+  // when they are done, while the worker already runs the next batch.
   hipStream_t out_stream = nullptr;
-  hipEvent_t solved = nullptr;  // the last solve, on the pool's stream
   std::deque<Pending> outq;
   std::vector<hipEvent_t> free_events;
   bool stop_out = false;
@@ -137,12 +146,13 @@ struct gcmi_server {
 
 namespace {
 
-// Applies the batch's requests to the pool, solves, and queues the packing
-// and copy-out of the masks on out_stream, recording *done after them.
-// Runs on the worker thread with pool_mu held.  The batch's sessions stay
-// busy until the finisher has seen *done, so no later solve rewrites their
-// labels before the copies have read them.
-int serve_batch(gcmi_server* s, std::vector<Request*>& batch, hipEvent_t done) {
+// Applies the batch's requests to the pool, solves, records `solved` on the
+// pool's stream after the solve and lists the masks' copies for the
+// finisher.  Runs on the worker thread with pool_mu held.  The batch's
+// sessions stay busy until the finisher has completed them, so no later
+// solve rewrites their labels before the copies have read them.
+int serve_batch(gcmi_server* s, std::vector<Request*>& batch, hipEvent_t solved,
+                std::vector<MaskCopy>& copies) {
   const int64_t t0 = now_ns();
   Pool* p = &s->pool;
   std::vector<int> imgs;
@@ -191,16 +201,15 @@ int serve_batch(gcmi_server* s, std::vector<Request*>& batch, hipEvent_t done) {
   GCMI_TRY(pool_solve(p, imgs.data(), (int)imgs.size(), s->sweeps, s->bfs, 0, nullptr));
   const int64_t t2 = now_ns();
   // (the solve has synchronised: the flows are on the host) the masks,
-  // kept packed on the device by the solve, are copied out after it
-  GCMI_HIP(hipEventRecord(s->solved, p->stream));
-  GCMI_HIP(hipStreamWaitEvent(s->out_stream, s->solved, 0));
+  // kept packed on the device by the solve, are copied out after it by the
+  // finisher
+  GCMI_HIP(hipEventRecord(solved, p->stream));
+  copies.clear();
   for (Request* r : batch) {
     const Session& ss = s->sessions[r->session];
     const long long words = ((long long)ss.W * ss.H + 31) / 32;
-    GCMI_HIP(hipMemcpyAsync(ss.bits, packed_mask(p, ss.img), words * 4, hipMemcpyDeviceToHost,
-                            s->out_stream));
+    copies.push_back(MaskCopy{ss.bits, packed_mask(p, ss.img), (size_t)words * 4});
   }
-  GCMI_HIP(hipEventRecord(done, s->out_stream));
   for (Request* r : batch) r->flow = p->imgs[s->sessions[r->session].img].flow;
   s->ns_apply += t1 - t0;
   s->ns_solve += t2 - t1;
@@ -235,11 +244,21 @@ void finisher_main(gcmi_server* s) {
     Pending pd = std::move(s->outq.front());
     s->outq.pop_front();
     lk.unlock();
-    const hipError_t e = hipEventSynchronize(pd.done);
+    hipError_t e = hipStreamWaitEvent(s->out_stream, pd.solved, 0);
+    for (const MaskCopy& c : pd.copies)
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, s->out_stream);
+    if (e == hipSuccess) e = hipEventRecord(pd.done, s->out_stream);
+    if (e == hipSuccess) e = hipEventSynchronize(pd.done);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipStreamSynchronize(s->out_stream);
+    }
     const int64_t t = now_ns();
     lk.lock();
     if (e != hipSuccess)
       snprintf(s->err, sizeof s->err, "mask copy failed: %s", hipGetErrorString(e));
+    s->free_events.push_back(pd.solved);
     s->free_events.push_back(pd.done);
     complete(s, pd.batch, e == hipSuccess, t);
   }
@@ -290,28 +309,27 @@ void worker_main(gcmi_server* s) {
     if (batch.empty()) continue;
     const int64_t t0 = now_ns();
     for (Request* r : batch) r->t_start = t0;
-    hipEvent_t done = nullptr;
-    if (!s->free_events.empty()) {
-      done = s->free_events.back();
-      s->free_events.pop_back();
-    }
+    hipEvent_t ev[2] = {nullptr, nullptr};  // solved, done
+    for (hipEvent_t& e : ev)
+      if (!s->free_events.empty()) {
+        e = s->free_events.back();
+        s->free_events.pop_back();
+      }
     lk.unlock();
     int rc = GCMI_OK;
     char err[sizeof s->err] = "";
-    if (!done && hipEventCreateWithFlags(&done, hipEventDisableTiming) != hipSuccess) {
-      (void)hipGetLastError();
-      done = nullptr;
-      rc = GCMI_EHIP;
-      snprintf(err, sizeof err, "event creation failed");
-    }
+    for (hipEvent_t& e : ev)
+      if (rc == GCMI_OK && !e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        e = nullptr;
+        rc = GCMI_EHIP;
+        snprintf(err, sizeof err, "event creation failed");
+      }
+    std::vector<MaskCopy> copies;
     if (rc == GCMI_OK) {
       std::lock_guard<std::mutex> plk(s->pool_mu);
-      rc = serve_batch(s, batch, done);
-      if (rc != GCMI_OK) {
-        snprintf(err, sizeof err, "%s", gcmi_last_error());
-        // copies of this batch may have been queued: let them finish
-        (void)hipStreamSynchronize(s->out_stream);
-      }
+      rc = serve_batch(s, batch, ev[0], copies);
+      if (rc != GCMI_OK) snprintf(err, sizeof err, "%s", gcmi_last_error());
     }
     lk.lock();
     // the images' staging ranges were read by the solve, which has finished
@@ -322,10 +340,11 @@ void worker_main(gcmi_server* s) {
       }
     if (rc != GCMI_OK) {
       memcpy(s->err, err, sizeof err);
-      if (done) s->free_events.push_back(done);
+      for (hipEvent_t e : ev)
+        if (e) s->free_events.push_back(e);
       complete(s, batch, false, now_ns());
     } else {
-      s->outq.push_back(Pending{std::move(batch), done});
+      s->outq.push_back(Pending{std::move(batch), std::move(copies), ev[0], ev[1]});
       s->cv_out.notify_one();
     }
     s->cv_done.notify_all();
@@ -400,8 +419,7 @@ int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_ba
   if (rc == GCMI_OK) {
     if (hipMalloc((void**)&s->d_stage, stage_bytes) != hipSuccess ||
         hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&s->out_stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&s->solved, hipEventDisableTiming) != hipSuccess) {
+        hipStreamCreateWithFlags(&s->out_stream, hipStreamNonBlocking) != hipSuccess) {
       (void)hipGetLastError();
       set_error("server staging allocation failed");
       rc = GCMI_ENOMEM;
@@ -411,7 +429,6 @@ int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_ba
     if (s->d_stage) (void)hipFree(s->d_stage);
     if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     if (s->out_stream) (void)hipStreamDestroy(s->out_stream);
-    if (s->solved) (void)hipEventDestroy(s->solved);
     pool_free(&s->pool);
     delete s;
     return rc;
@@ -443,7 +460,6 @@ int gcmi_server_destroy(gcmi_server* s) {
   for (Session& ss : s->sessions)
     if (ss.bits) (void)hipHostFree(ss.bits);
   for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
-  if (s->solved) (void)hipEventDestroy(s->solved);
   if (s->out_stream) (void)hipStreamDestroy(s->out_stream);
   if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
   if (s->d_stage) (void)hipFree(s->d_stage);
