@@ -319,7 +319,7 @@ def serve_leg(args):
     # every mode runs the plain cut's fastest settings (local relabels after
     # the first, 2 device rounds per relabel)
     fast = dict(local_relabel=True, device_rounds=2)
-    modes = {"plain": (dict(mode="plain", **fast), (1500, 2000, 2500, 3000)),
+    modes = {"plain": (dict(mode="plain", **fast), (2000, 2500, 3000, 3500, 4000)),
              "cold": (dict(mode="cold", **fast), (2000, 3000, 4000, 6000)),
              "batched": (dict(mode="batched", **fast),
                          (16000, 20000, 22000, 24000, 26000, 28000, 30000))}
@@ -358,6 +358,9 @@ def serve_leg(args):
                        for k, r in enumerate(stroke_only[mode])]
                 out["stroke_only"][mode] = {"points": pts,
                                             "sustained_2048_p99_20ms": S.sustained(pts)}
+        # the closed server's device memory is released (and cleared by the
+        # driver) in the background: let that finish before the next mode
+        time.sleep(3.0)
 
     def rate(entry):
         return (entry["sustained_2048_p99_20ms"] or {}).get("stroke_rate")
