@@ -131,6 +131,8 @@ struct RangeAlloc {
   long long largest() const;
 };
 
+enum { STAGE_PREP = 0, STAGE_EDIT = 1, STAGE_JOBS = 2, STAGE_RGBJOBS = 3, STAGE_N = 4 };
+
 struct Pool {
   const PoolOps* ops = nullptr;
   int T = 0;
@@ -159,6 +161,9 @@ struct Pool {
   int ep_par = 0;         // epoch slot the next queue round reads
   bool flood_inc = false;  // the current flood recomputes changed tiles only
   bool flood_ran = false;  // a flood has added to the final-excess sums in this solve
+  // pinned staging buffers with a host-to-device copy queued on the stream
+  // since its last synchronisation (a buffer is rewritten only after it)
+  bool stage_pending[STAGE_N] = {};
   bool ep_reset = false;  // restart stamps and epochs before the next solve
   // Device-driven solves (dd_rounds > 0): every launch of a fixed shape,
   // dd_rounds BFS / flood rounds per relabel, gates on the device instead of

@@ -670,6 +670,19 @@ size_t pool_bytes(int tile, int cap_tiles, long long cap_pix, int cap_imgs) {
          (size_t)compact_scratch_ints(cap_tiles) * 4 + 8 * kSumSlots * kRgbJobsMax + 4096;
 }
 
+// Synchronises the pool's stream: every staging copy queued on it is done.
+static int stream_sync(Pool* p) {
+  GCMI_HIP(hipStreamSynchronize(p->stream));
+  for (bool& b : p->stage_pending) b = false;
+  return GCMI_OK;
+}
+
+// Before staging buffer k is rewritten: waits for the stream only if a copy
+// from it was queued since the last synchronisation.
+static int stage_wait(Pool* p, int k) {
+  return p->stage_pending[k] ? stream_sync(p) : GCMI_OK;
+}
+
 static int dev_alloc(void** ptr, size_t bytes, size_t* total) {
   if (hipMalloc(ptr, bytes ? bytes : 4) != hipSuccess) {
     (void)hipGetLastError();
@@ -1016,7 +1029,7 @@ int pool_upload_rgb_many(Pool* p, int count, const RgbReq* reqs) {
   for (int base = 0; base < count; base += kRgbJobsMax) {
     const int m = std::min(count - base, kRgbJobsMax);
     // the job table may still be read by the previous launch's copy
-    GCMI_HIP(hipStreamSynchronize(p->stream));
+    GCMI_TRY(stage_wait(p, STAGE_RGBJOBS));
     long long blocks = 0;
     for (int i = 0; i < m; ++i) {
       const RgbReq& r = reqs[base + i];
@@ -1031,6 +1044,7 @@ int pool_upload_rgb_many(Pool* p, int count, const RgbReq* reqs) {
     }
     GCMI_HIP(hipMemcpyAsync(p->d_rgbjobs, hj, offsetof(RgbJobs, job) + (size_t)m * sizeof(RgbJob),
                             hipMemcpyHostToDevice, p->stream));
+    p->stage_pending[STAGE_RGBJOBS] = true;
     hipLaunchKernelGGL(k_fill_i32, dim3(std::max(1, m * kSumSlots * 2 / 256)), dim3(256), 0,
                        p->stream, (int32_t*)p->d_rgbsum, 2LL * kSumSlots * m, 0);
     if (blocks > 0) {
@@ -1172,7 +1186,7 @@ int pool_edit_many(Pool* p, int count, const EditReq* e) {
     p->edit_cap = cap;
   } else {
     // the staging buffer is reused: wait for the previous edit's copy
-    GCMI_HIP(hipStreamSynchronize(p->stream));
+    GCMI_TRY(stage_wait(p, STAGE_EDIT));
   }
   long long off = 0;
   for (int k = 0; k < count; ++k) {
@@ -1184,6 +1198,7 @@ int pool_edit_many(Pool* p, int count, const EditReq* e) {
   }
   GCMI_HIP(hipMemcpyAsync(p->d_edit, p->h_edit, (size_t)total * 12, hipMemcpyHostToDevice,
                           p->stream));
+  p->stage_pending[STAGE_EDIT] = true;
   off = 0;
   for (int k = 0; k < count; ++k) {
     const int n = e[k].n;
@@ -1336,7 +1351,7 @@ int pool_paint_many(Pool* p, int count, const PaintReq* reqs) {
     p->jobs_cap = cap;
   }
   // the staging may still be read by a previous launch's copy
-  GCMI_HIP(hipStreamSynchronize(p->stream));
+  GCMI_TRY(stage_wait(p, STAGE_JOBS));
   std::vector<char> seen((size_t)p->cap_imgs, 0);
   int m = 0, base = 0;
   long long rows = 0;
@@ -1344,6 +1359,7 @@ int pool_paint_many(Pool* p, int count, const PaintReq* reqs) {
     if (m == base) return GCMI_OK;
     GCMI_HIP(hipMemcpyAsync(p->d_jobs + base, p->h_jobs + base, (size_t)(m - base) * sizeof(PaintJob),
                             hipMemcpyHostToDevice, p->stream));
+    p->stage_pending[STAGE_JOBS] = true;
     p->ops->paint(p, p->d_jobs + base, m - base, (int)rows);
     GCMI_HIP(hipGetLastError());
     std::fill(seen.begin(), seen.end(), 0);
@@ -1469,7 +1485,7 @@ static Queue round_queue(Pool* p, int cur) {
 
 static int read_counts(Pool* p) {
   GCMI_HIP(hipMemcpyAsync(p->h_counts, p->counts, CNT_N * 4, hipMemcpyDeviceToHost, p->stream));
-  GCMI_HIP(hipStreamSynchronize(p->stream));
+  GCMI_TRY(stream_sync(p));
   p->stats[STAT_SYNCS]++;
   // stamps hold epochs: restart both before the counter nears overflow
   if (std::max(p->h_counts[CNT_EP], p->h_counts[CNT_EP + 1]) >= (1 << 30)) p->ep_reset = true;
@@ -1527,6 +1543,7 @@ int phase_prepare(const Solve& S) {
   Pool* p = S.p;
   GCMI_HIP(hipMemcpyAsync(p->d_prep, p->h_prep, (size_t)S.n * sizeof(Prep),
                           hipMemcpyHostToDevice, p->stream));
+  p->stage_pending[STAGE_PREP] = true;
   hipLaunchKernelGGL(k_prepare, dim3(S.n), dim3(256), 0, p->stream, p->geo, p->dev, p->d_prep,
                      S.nsel, S.nfresh, p->sel, p->fresh, p->counts, p->d_imgfresh);
   GCMI_HIP(hipGetLastError());
@@ -1918,7 +1935,7 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
         return GCMI_EINVAL;
       }
     // the pinned prep buffer may still be read by a previous solve's copy
-    if (i == 0) GCMI_HIP(hipStreamSynchronize(p->stream));
+    if (i == 0) GCMI_TRY(stage_wait(p, STAGE_PREP));
     // warm only if edited since the last solve; otherwise start over from
     // the stored capacities (which include every edit so far)
     const bool fresh = im.fresh || !im.edited;
