@@ -78,6 +78,8 @@ def parse():
     ap.add_argument("--serve-image-frac", type=float, default=0.1,
                     help="share of new-image requests in the serving mix")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--serve-max-batch", type=int, default=64,
+                    help="most requests per served solve")
     return ap.parse_args()
 
 
@@ -339,8 +341,8 @@ def serve_leg(args):
         stroke_only = {}
     for mode, (kw, rs) in modes.items():
         log("served: mode %s" % mode)
-        with GraphCutServer(tile=32, hbm_budget=200 * 10**9, max_sessions=256, max_batch=64,
-                            **kw) as srv:
+        with GraphCutServer(tile=32, hbm_budget=200 * 10**9, max_sessions=256,
+                            max_batch=args.serve_max_batch, **kw) as srv:
             sids = S.open_sessions(srv, mix, rgb=True)
             # an unmeasured warm-up trace (first-use allocations, page-in)
             S.run_trace(srv, sids, mix, rs[0], duration=0.5, seed=7,
