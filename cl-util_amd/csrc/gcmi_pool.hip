@@ -506,8 +506,17 @@ __global__ void k_flows(const unsigned long long* __restrict__ sums, const Prep*
   if (threadIdx.x == 0) out[blockIdx.x] = v;
 }
 
+// Threads per tile of the relabel kernels: at T = 32, two waves (8 pixels
+// per thread) instead of the push's four, so a CU keeps more tiles in
+// flight (their time per tile is the latency of a few dependent loads).
+// The flood and labels kernels lose at two waves (more registers per
+// thread for the flood, and its occupancy halves), so they keep the push's.
+template <int T, int NT>
+constexpr int kTileNT = T == 32 ? 128 : NT;
+
 template <int T, int NT>
 struct PoolKernels {
+  static constexpr int TN = kTileNT<T, NT>;
   static void push(Pool* p, int grid, int sweeps, bool first) {
     if (first)
       hipLaunchKernelGGL((k_push<T, NT, true>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
@@ -522,11 +531,11 @@ struct PoolKernels {
   }
   static void bfs_init(Pool* p, int grid, const int32_t* list, int cnt, bool all,
                        const Queue& q) {
-    hipLaunchKernelGGL((k_bfs_init<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
+    hipLaunchKernelGGL((k_bfs_init<T, TN>), dim3(grid), dim3(TN), 0, p->stream, p->geo, p->dev,
                        list, p->counts + cnt, all ? 1 : 0, q, p->counts + CNT_ACT);
   }
   static void bfs_round(Pool* p, int grid, const Queue& q) {
-    hipLaunchKernelGGL((k_bfs_round<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
+    hipLaunchKernelGGL((k_bfs_round<T, TN>), dim3(grid), dim3(TN), 0, p->stream, p->geo, p->dev,
                        q);
   }
   static void flood(Pool* p, int grid, const Queue& q, int mode, int32_t* lmark) {
