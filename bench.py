@@ -324,7 +324,7 @@ def serve_leg(args):
     modes = {"plain": (dict(mode="plain", **fast), (2000, 2500, 3000, 3500, 4000)),
              "cold": (dict(mode="cold", **fast), (2000, 3000, 4000, 6000)),
              "batched": (dict(mode="batched", **fast),
-                         (16000, 20000, 22000, 24000, 26000, 28000, 30000))}
+                         (16000, 20000, 24000, 26000, 28000, 30000, 32000, 34000))}
     stroke_only = {"plain": (1500, 2000, 2500, 3000, 3500),
                    "batched": (30000, 40000, 45000, 50000, 55000)}
     out = {"workload": "256 sessions, sizes {512,1024,2048,4096}^2 weights 4:3:2:1 (seed 4), "

@@ -614,10 +614,13 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
 
 // Folds the pending inflow of every flagged tile into its state (between
 // launches, when no tile is resident): e, the reverse capacity, the mask
-// byte and the tile's positive-excess sum.  One block of kFoldNT threads per
-// listed tile (unflagged tiles exit at once); one owner thread per edge
-// pixel (threads < 4T - 4 walk the perimeter), so corners are not raced.
-constexpr int kFoldNT = 256;
+// byte and the tile's positive-excess sum.  One block per listed tile
+// (unflagged tiles exit at once); one owner thread per edge pixel (threads
+// < 4T - 4 walk the perimeter), so corners are not raced.  The block is the
+// perimeter rounded up to whole waves: 64, 128 and 256 threads at T = 16,
+// 32 and 64.
+template <int T>
+constexpr int kFoldNT = (4 * T - 4 + 63) / 64 * 64;
 
 template <int T>
 __device__ __forceinline__ long long fold_edge_pixel(CutDev& s, int t, int j) {
@@ -650,9 +653,9 @@ __device__ __forceinline__ long long fold_edge_pixel(CutDev& s, int t, int j) {
 }
 
 template <int T>
-__global__ void __launch_bounds__(kFoldNT)
+__global__ void __launch_bounds__(kFoldNT<T>)
 k_fold(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count) {
-  static_assert(4 * T - 4 <= kFoldNT, "perimeter owners");
+  static_assert(4 * T - 4 <= kFoldNT<T>, "perimeter owners");
   const int n = *count;
   const int j = threadIdx.x;
   for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {

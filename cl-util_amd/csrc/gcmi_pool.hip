@@ -526,7 +526,7 @@ struct PoolKernels {
                          p->dev, p->alist, p->counts + CNT_ACT, sweeps);
   }
   static void fold(Pool* p, int grid, const int32_t* list, int cnt) {
-    hipLaunchKernelGGL((k_fold<T>), dim3(grid), dim3(kFoldNT), 0, p->stream, p->geo, p->dev,
+    hipLaunchKernelGGL((k_fold<T>), dim3(grid), dim3(kFoldNT<T>), 0, p->stream, p->geo, p->dev,
                        list, p->counts + cnt);
   }
   static void bfs_init(Pool* p, int grid, const int32_t* list, int cnt, bool all,
