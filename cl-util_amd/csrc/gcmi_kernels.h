@@ -1381,6 +1381,51 @@ struct Prep {
   int img, sel_off, fresh_off, pad;
 };
 
+// A pixel's edit in two steps, so that a thread can have the loads of
+// several pixels in flight before it stores any: edit_load reads the
+// pixel's unaries and state (in one round trip, whether or not the excess
+// changes), edit_apply writes the new values.
+struct PixEdit {
+  size_t r, q;  // row-major unary index, tiled state index
+  int t;        // tile
+  int a0, b0, e0, m0;
+};
+
+template <int T>
+__device__ __forceinline__ void edit_load(const CutDev& s, const ImgDesc& d, int x, int y,
+                                          int warm, PixEdit& E) {
+  E.r = d.pix0 + (size_t)y * d.W + x;
+  E.t = d.tile0 + (y / T) * d.tw + x / T;
+  E.q = tbase<T>(E.t) + (y % T) * T + (x % T);
+  E.a0 = s.cs[E.r];
+  E.b0 = s.ct[E.r];
+  E.e0 = warm ? s.e[E.q] : 0;
+  E.m0 = warm ? s.mask[E.q] : 0;
+}
+
+__device__ __forceinline__ void edit_apply(const CutDev& s, const PixEdit& E, int a1, int b1,
+                                           int warm, long long& dmin, long long& dpos) {
+  const int a0 = E.a0, b0 = E.b0;
+  s.cs[E.r] = a1;
+  s.ct[E.r] = b1;
+  if (!warm) return;  // no state yet: the first round reads the unaries
+  dmin += (long long)min(a1, b1) - min(a0, b0);
+  dpos += (long long)(a1 - b1 > 0 ? a1 - b1 : 0) - (a0 - b0 > 0 ? a0 - b0 : 0);
+  const int de = (a1 - b1) - (a0 - b0);
+  if (de == 0) return;
+  const int e1 = E.e0 + de;
+  s.e[E.q] = e1;
+  s.mask[E.q] = (uint8_t)((E.m0 & (M_UP | M_DOWN | M_LEFT | M_RIGHT)) | (e1 < 0 ? M_SINK : 0) |
+                          (e1 > 0 ? M_EXCESS : 0));
+  // a new deficit is a sink at height 0; the tile pushes before the first
+  // relabel of the next solve
+  if (e1 < 0) s.h[E.q] = 0;
+  // the tile is active: the solve's first push round over the active tiles
+  // recomputes its positive-excess sum tpos before anything reads it
+  s.act[E.t] = 1;
+  s.dirty[E.t] = 1;
+}
+
 // One pixel's share of a warm-start edit: new unaries (a1, b1) for pixel p
 // (row-major index local to the image).  Updates the stored unaries and, if
 // the image holds solved state (warm), e by the change of cs - ct, the
@@ -1394,32 +1439,9 @@ struct Prep {
 template <int T>
 __device__ __forceinline__ void edit_pixel(const CutDev& s, const ImgDesc& d, int x, int y, int a1,
                                            int b1, int warm, long long& dmin, long long& dpos) {
-  const size_t r = d.pix0 + (size_t)y * d.W + x;
-  const int t = d.tile0 + (y / T) * d.tw + x / T;
-  const size_t q = tbase<T>(t) + (y % T) * T + (x % T);
-  // (the state is read with the unaries, in one round trip, whether or not
-  // the excess changes)
-  const int a0 = s.cs[r], b0 = s.ct[r];
-  const int e0 = warm ? s.e[q] : 0;
-  const int m0 = warm ? s.mask[q] : 0;
-  s.cs[r] = a1;
-  s.ct[r] = b1;
-  if (!warm) return;  // no state yet: the first round reads the unaries
-  dmin += (long long)min(a1, b1) - min(a0, b0);
-  dpos += (long long)(a1 - b1 > 0 ? a1 - b1 : 0) - (a0 - b0 > 0 ? a0 - b0 : 0);
-  const int de = (a1 - b1) - (a0 - b0);
-  if (de == 0) return;
-  const int e1 = e0 + de;
-  s.e[q] = e1;
-  s.mask[q] = (uint8_t)((m0 & (M_UP | M_DOWN | M_LEFT | M_RIGHT)) | (e1 < 0 ? M_SINK : 0) |
-                        (e1 > 0 ? M_EXCESS : 0));
-  // a new deficit is a sink at height 0; the tile pushes before the first
-  // relabel of the next solve
-  if (e1 < 0) s.h[q] = 0;
-  // the tile is active: the solve's first push round over the active tiles
-  // recomputes its positive-excess sum tpos before anything reads it
-  s.act[t] = 1;
-  s.dirty[t] = 1;
+  PixEdit E;
+  edit_load<T>(s, d, x, y, warm, E);
+  edit_apply(s, E, a1, b1, warm, dmin, dpos);
 }
 
 // Adds a block's edit deltas to the image's sums (slotted atomics).
@@ -1472,9 +1494,9 @@ struct Polyline {
   // chunk c0[k] of the stroke, group by group; nchunks chunks in all
   int ya[kMaxStrokePoints], nr[kMaxStrokePoints], xc[kMaxStrokePoints], c0[kMaxStrokePoints];
   int nchunks;
-  // pixel box of each segment's capsule (a pixel outside it is not in it)
-  int sx0[kMaxStrokePoints], sx1[kMaxStrokePoints], sy0[kMaxStrokePoints],
-      sy1[kMaxStrokePoints];
+  // pixel box (x0, x1, y0, y1) of each segment's capsule: a pixel outside it
+  // is not in it (one 16-byte scalar load per segment)
+  int4 sbox[kMaxStrokePoints];
 };
 
 // Whether pixel centre (px, py) lies within r of segment k: the point of the
@@ -1555,16 +1577,18 @@ k_paint(Geo g, CutDev s, const PaintJob* __restrict__ jobs, int njobs, int nchun
   if (lds)
     for (int j = threadIdx.x; j < njobs; j += blockDim.x) sc0[j] = jobs[j].chunk0;
   __syncthreads();
+  // item, job and segment are wave-uniform: the stroke's fields are read
+  // through the scalar cache
   const int lane = threadIdx.x % kWave, waves = blockDim.x / kWave;
-  for (int item = blockIdx.x * waves + threadIdx.x / kWave; item < nchunks;
-       item += gridDim.x * waves) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  for (int item = blockIdx.x * waves + wave; item < nchunks; item += gridDim.x * waves) {
     int lo = 0, hi = njobs - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if ((lds ? sc0[mid] : jobs[mid].chunk0) <= item) lo = mid;
       else hi = mid - 1;
     }
-    const PaintJob& J = jobs[lo];
+    const PaintJob& J = jobs[__builtin_amdgcn_readfirstlane(lo)];
     const Polyline& pl = J.pl;
     const int local = item - J.chunk0;
     // the segment: the last one whose first chunk is <= local (one load per
@@ -1578,22 +1602,44 @@ k_paint(Geo g, CutDev s, const PaintJob* __restrict__ jobs, int njobs, int nchun
     const int xpiece = piece % pl.xc[k];
     const ImgDesc d = g.img[J.img];
     long long dmin = 0, dpos = 0;
-    for (int y = ya; y < yb; ++y) {
+    // each row's candidate columns [rx0, rx1] in this piece (none: rx1 < rx0)
+    int rx0[kPaintRows], rx1[kPaintRows], passes = 0;
+#pragma unroll
+    for (int ri = 0; ri < kPaintRows; ++ri) {
+      const int y = ya + ri;
+      int x0 = 0, x1 = -1;
       double xl, xr;
-      if (!seg_row_span(pl, k, (double)y, &xl, &xr)) continue;
-      xl = fmax(xl, 0.0);
-      xr = fmin(xr, (double)(pl.W - 1));
-      if (xl > xr) continue;
-      const int x0 = (int)floor(xl) + xpiece * kPaintCols;
-      const int x1 = min((int)ceil(xr), x0 + kPaintCols - 1);
-      for (int x = x0 + lane; x <= x1; x += kWave) {
-        if (!in_capsule(pl, k, (double)x, (double)y)) continue;
-        bool earlier = false;
-        for (int j = 0; j < k && !earlier; ++j)
-          earlier = x >= pl.sx0[j] && x <= pl.sx1[j] && y >= pl.sy0[j] && y <= pl.sy1[j] &&
-                    in_capsule(pl, j, (double)x, (double)y);
-        if (!earlier) edit_pixel<T>(s, d, x, y, pl.cs, pl.ct, J.warm, dmin, dpos);
+      if (y < yb && seg_row_span(pl, k, (double)y, &xl, &xr)) {
+        xl = fmax(xl, 0.0);
+        xr = fmin(xr, (double)(pl.W - 1));
+        if (xl <= xr) {
+          x0 = (int)floor(xl) + xpiece * kPaintCols;
+          x1 = min((int)ceil(xr), x0 + kPaintCols - 1);
+        }
       }
+      rx0[ri] = __builtin_amdgcn_readfirstlane(x0);
+      rx1[ri] = __builtin_amdgcn_readfirstlane(x1);
+      passes = max(passes, (rx1[ri] - rx0[ri] + kWave) / kWave);
+    }
+    // a pass takes one column of every row: the loads of all its pixels are
+    // in flight before the first store
+    for (int p = 0; p < passes; ++p) {
+      PixEdit E[kPaintRows];
+      bool hit[kPaintRows];
+#pragma unroll
+      for (int ri = 0; ri < kPaintRows; ++ri) {
+        const int x = rx0[ri] + p * kWave + lane, y = ya + ri;
+        hit[ri] = x <= rx1[ri] && in_capsule(pl, k, (double)x, (double)y);
+        for (int j = 0; j < k && hit[ri]; ++j) {
+          const int4 b = pl.sbox[j];
+          hit[ri] = !(x >= b.x && x <= b.y && y >= b.z && y <= b.w &&
+                      in_capsule(pl, j, (double)x, (double)y));
+        }
+        if (hit[ri]) edit_load<T>(s, d, x, y, J.warm, E[ri]);
+      }
+#pragma unroll
+      for (int ri = 0; ri < kPaintRows; ++ri)
+        if (hit[ri]) edit_apply(s, E[ri], pl.cs, pl.ct, J.warm, dmin, dpos);
     }
     if (J.warm) {
       // the chunk's changes of the image's sums (one slotted atomic per wave)

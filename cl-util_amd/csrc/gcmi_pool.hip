@@ -224,62 +224,93 @@ __device__ __forceinline__ int rgb_col(long long i, int W, bool small) {
   return small ? (int)((unsigned)i % (unsigned)W) : (int)(i % W);
 }
 
-// A block of the colour kernels covers kRgbNT * kRgbPx consecutive pixels
-// and stages the bytes it reads in LDS: its pixels and the next one (right
-// neighbours), and the pixels one row below them.  Words per staged range
-// (up to 4 channels, plus the range's offset inside its first word).
+// A chunk of the colour kernels is kRgbNT * kRgbPx consecutive pixels; a
+// block covers kRgbChunks consecutive chunks of one image and stages the
+// bytes of each in LDS: its pixels and the next one (right neighbours), and
+// the pixels one row below them.  The bytes of the next chunk are loaded
+// into registers while the block works on the current one.  Words per
+// staged range (up to 4 channels, plus the range's offset inside its first
+// word).
 constexpr int kRgbBlockPx = kRgbNT * kRgbPx;
+constexpr int kRgbChunks = 4;
 constexpr int kRgbStageWords = (kRgbBlockPx + 1) + 1;
-
-// Bytes [b0, b1) of a colour image of nb bytes (4-byte aligned), clipped to
-// the image, into dst: byte i at dst[i - (b0 & ~3)].  Whole words where the
-// image holds them, a last partial word byte by byte.  All threads call it.
 constexpr int kRgbStageIters = (kRgbStageWords + kRgbNT - 1) / kRgbNT;
 
-__device__ __forceinline__ void rgb_stage(const uint8_t* __restrict__ rgb, long long nb,
-                                          long long b0, long long b1, uint32_t* dst) {
+// One thread's share of a staged byte range: words threadIdx.x + j * kRgbNT.
+struct RgbWords {
+  uint32_t v[kRgbStageIters];
+  int nw;  // words in the range
+};
+
+// Loads bytes [b0, b1) of a colour image of nb bytes (4-byte aligned),
+// clipped to the image: whole words where the image holds them, a last
+// partial word byte by byte.  Byte i lands at word (i - (b0 & ~3)) / 4.
+__device__ __forceinline__ void rgb_load(const uint8_t* __restrict__ rgb, long long nb,
+                                         long long b0, long long b1, RgbWords& w) {
   const long long a0 = b0 & ~3LL;
   b1 = b1 < nb ? b1 : nb;
-  const int nw = b1 > a0 ? (int)((b1 - a0 + 3) >> 2) : 0;
-  // every load issued before the first store (a block's words in flight at once)
-  uint32_t v[kRgbStageIters];
+  w.nw = b1 > a0 ? (int)((b1 - a0 + 3) >> 2) : 0;
 #pragma unroll
   for (int j = 0; j < kRgbStageIters; ++j) {
     const int k = threadIdx.x + j * kRgbNT;
     const long long o = a0 + 4LL * k;
-    v[j] = 0;
-    if (k < nw) {
+    w.v[j] = 0;
+    if (k < w.nw) {
       if (o + 4 <= nb) {
-        v[j] = *reinterpret_cast<const uint32_t*>(rgb + o);
+        w.v[j] = *reinterpret_cast<const uint32_t*>(rgb + o);
       } else {
-        for (int i = 0; i < 4 && o + i < nb; ++i) v[j] |= (uint32_t)rgb[o + i] << (8 * i);
+        for (int i = 0; i < 4 && o + i < nb; ++i) w.v[j] |= (uint32_t)rgb[o + i] << (8 * i);
       }
     }
   }
+}
+
+__device__ __forceinline__ void rgb_store(const RgbWords& w, uint32_t* dst) {
 #pragma unroll
   for (int j = 0; j < kRgbStageIters; ++j) {
     const int k = threadIdx.x + j * kRgbNT;
-    if (k < nw) dst[k] = v[j];
+    if (k < w.nw) dst[k] = w.v[j];
   }
 }
 
-// Stages a block's bytes: own[] holds pixels p0 .. p0 + kRgbBlockPx (one
-// past, for right neighbours), dn[] the pixels a row below p0 ..; returns
-// the byte offsets of own[0] and dn[0] in the image.
-__device__ __forceinline__ void rgb_stage_block(const uint8_t* __restrict__ rgb, long long n,
-                                                int W, int ch, long long p0, uint32_t* own,
-                                                uint32_t* dn, long long* own0, long long* dn0) {
+// The loads of the chunk at pixel p0: own holds pixels p0 .. p0 +
+// kRgbBlockPx (one past, for right neighbours), dn the pixels a row below.
+__device__ __forceinline__ void rgb_load_chunk(const uint8_t* __restrict__ rgb, long long n,
+                                               int W, int ch, long long p0, RgbWords& own,
+                                               RgbWords& dn) {
   const long long nb = n * ch;
-  rgb_stage(rgb, nb, p0 * ch, (p0 + kRgbBlockPx + 1) * ch, own);
-  rgb_stage(rgb, nb, (p0 + W) * ch, (p0 + W + kRgbBlockPx) * ch, dn);
-  *own0 = (p0 * ch) & ~3LL;
-  *dn0 = ((p0 + W) * ch) & ~3LL;
-  __syncthreads();
+  rgb_load(rgb, nb, p0 * ch, (p0 + kRgbBlockPx + 1) * ch, own);
+  rgb_load(rgb, nb, (p0 + W) * ch, (p0 + W + kRgbBlockPx) * ch, dn);
+}
+
+// Walks a block's chunks of image n = W * H pixels: f(p0, O, own0, D, dn0)
+// for each chunk at pixel p0 that starts inside the image, with its bytes
+// in LDS (pixel i at O + (i * ch - own0) for p0 <= i <= p0 + kRgbBlockPx,
+// the pixel a row below at D + ((i + W) * ch - dn0)).  All threads call it.
+template <class F>
+__device__ __forceinline__ void rgb_chunks(const uint8_t* __restrict__ rgb, long long n, int W,
+                                           int ch, long long first, F&& f) {
+  __shared__ uint32_t own[kRgbStageWords], dn[kRgbStageWords];
+  const uint8_t* O = reinterpret_cast<const uint8_t*>(own);
+  const uint8_t* D = reinterpret_cast<const uint8_t*>(dn);
+  RgbWords wo, wd;
+  rgb_load_chunk(rgb, n, W, ch, first, wo, wd);
+  for (int c = 0; c < kRgbChunks; ++c) {
+    const long long p0 = first + (long long)c * kRgbBlockPx;
+    if (p0 >= n) break;  // uniform
+    rgb_store(wo, own);
+    rgb_store(wd, dn);
+    __syncthreads();
+    const long long p1 = p0 + kRgbBlockPx;
+    if (c + 1 < kRgbChunks && p1 < n) rgb_load_chunk(rgb, n, W, ch, p1, wo, wd);
+    f(p0, O, (p0 * ch) & ~3LL, D, ((p0 + W) * ch) & ~3LL);
+    __syncthreads();  // the chunk's bytes are read before the next store
+  }
 }
 
 // One colour image of a launch: its bytes, pool pixel offset, size and
-// model, and its first block in the launch (blocks of kRgbBlockPx pixels,
-// images in turn).
+// model, and its first block in the launch (blocks of kRgbChunks *
+// kRgbBlockPx pixels, images in turn).
 struct RgbJob {
   const uint8_t* rgb;
   long long pix0;
@@ -297,7 +328,7 @@ struct RgbJobs {
 };
 
 // The job this block belongs to: one ballot of wave 0 over the first blocks
-// (njobs <= kWave).  All threads call it.
+// (njobs <= kWave); the result is wave-uniform.  All threads call it.
 __device__ __forceinline__ int rgb_job(const RgbJobs* __restrict__ J, int njobs, int* s_j) {
   static_assert(kRgbJobsMax <= kWave, "one lane per job");
   if (threadIdx.x < kWave) {
@@ -306,41 +337,40 @@ __device__ __forceinline__ int rgb_job(const RgbJobs* __restrict__ J, int njobs,
     if (threadIdx.x == 0) *s_j = __popcll(m) - 1;
   }
   __syncthreads();
-  return *s_j;
+  return __builtin_amdgcn_readfirstlane(*s_j);
 }
 
 // Per image of the launch, the sum of d2 over its horizontal and vertical
 // neighbour pairs (beta's numerator; an exact integer), spread over
 // kSumSlots slots at sums[job * kSumSlots].  Every thread takes kRgbPx
-// pixels; the grid covers every image once.
+// pixels of each chunk; the grid covers every image once.
 __global__ void __launch_bounds__(kRgbNT)
 k_rgb_pair_sum(const RgbJobs* __restrict__ J, int njobs, unsigned long long* sums) {
   __shared__ long long red[kRgbNT / kWave];
-  __shared__ uint32_t own[kRgbStageWords], dn[kRgbStageWords];
   __shared__ int s_j;
   const int j = rgb_job(J, njobs, &s_j);
   const RgbJob* jobs = J->job;
   const uint8_t* rgb = jobs[j].rgb;
   const int W = jobs[j].W, H = jobs[j].H, ch = jobs[j].ch;
   const long long n = (long long)W * H;
-  const long long p0 = (long long)(blockIdx.x - jobs[j].block0) * kRgbBlockPx;
-  long long own0, dn0;
-  rgb_stage_block(rgb, n, W, ch, p0, own, dn, &own0, &dn0);
-  const uint8_t* O = reinterpret_cast<const uint8_t*>(own);
-  const uint8_t* D = reinterpret_cast<const uint8_t*>(dn);
-  const long long i0 = p0 + (long long)threadIdx.x * kRgbPx;
-  int x = i0 < n ? rgb_col(i0, W, n <= 0x7fffffffLL) : 0;
+  const bool small = n <= 0x7fffffffLL;
+  const long long first = (long long)(blockIdx.x - jobs[j].block0) * kRgbChunks * kRgbBlockPx;
   long long acc = 0;
+  rgb_chunks(rgb, n, W, ch, first,
+             [&](long long p0, const uint8_t* O, long long own0, const uint8_t* D, long long dn0) {
+               const long long i0 = p0 + (long long)threadIdx.x * kRgbPx;
+               int x = i0 < n ? rgb_col(i0, W, small) : 0;
 #pragma unroll
-  for (int k = 0; k < kRgbPx; ++k) {
-    const long long i = i0 + k;
-    if (i < n) {
-      const uint8_t* q = O + (i * ch - own0);
-      if (x + 1 < W) acc += rgb_d2(q, q + ch);
-      if (i + W < n) acc += rgb_d2(q, D + ((i + W) * ch - dn0));
-    }
-    if (++x == W) x = 0;
-  }
+               for (int k = 0; k < kRgbPx; ++k) {
+                 const long long i = i0 + k;
+                 if (i < n) {
+                   const uint8_t* q = O + (i * ch - own0);
+                   if (x + 1 < W) acc += rgb_d2(q, q + ch);
+                   if (i + W < n) acc += rgb_d2(q, D + ((i + W) * ch - dn0));
+                 }
+                 if (++x == W) x = 0;
+               }
+             });
   acc = block_sum64<kRgbNT>(acc, red);
   if (threadIdx.x == 0 && acc)
     atomicAdd(&sums[(size_t)j * kSumSlots + blockIdx.x % kSumSlots], (unsigned long long)acc);
@@ -387,15 +417,14 @@ __device__ __forceinline__ int32_t rgb_unary(const gcmi_rgb_model& m, const doub
 }
 
 // Row-major int32 capacities of the launch's images from their colour
-// images (beta from each image's pair sum); kRgbPx pixels per thread, stored
-// as one 16-byte vector per array when whole (the outputs start 16-byte
-// aligned: pool pixel offsets are multiples of 32).
+// images (beta from each image's pair sum); kRgbPx pixels per thread and
+// chunk, stored as one 16-byte vector per array when whole (the outputs
+// start 16-byte aligned: pool pixel offsets are multiples of 32).
 __global__ void __launch_bounds__(kRgbNT)
 k_rgb_caps(const RgbJobs* __restrict__ J, int njobs, const unsigned long long* __restrict__ sums,
            int32_t* __restrict__ cs, int32_t* __restrict__ ct, int32_t* __restrict__ right,
            int32_t* __restrict__ down) {
 #pragma clang fp contract(off)
-  __shared__ uint32_t own[kRgbStageWords], dn[kRgbStageWords];
   __shared__ int s_j;
   const int j = rgb_job(J, njobs, &s_j);
   const RgbJob* jobs = J->job;
@@ -411,42 +440,44 @@ k_rgb_caps(const RgbJobs* __restrict__ J, int njobs, const unsigned long long* _
   right += pix0;
   down += pix0;
   const long long n = (long long)W * H;
-  const long long p0 = (long long)(blockIdx.x - jobs[j].block0) * kRgbBlockPx;
-  long long own0, dn0;
-  rgb_stage_block(rgb, n, W, ch, p0, own, dn, &own0, &dn0);
-  const uint8_t* O = reinterpret_cast<const uint8_t*>(own);
-  const uint8_t* D = reinterpret_cast<const uint8_t*>(dn);
-  const long long i0 = p0 + (long long)threadIdx.x * kRgbPx;
-  int x = i0 < n ? rgb_col(i0, W, n <= 0x7fffffffLL) : 0;
-  int32_t a[kRgbPx], b[kRgbPx], rr[kRgbPx], dd[kRgbPx];
+  const bool small = n <= 0x7fffffffLL;
+  const long long first = (long long)(blockIdx.x - jobs[j].block0) * kRgbChunks * kRgbBlockPx;
+  rgb_chunks(rgb, n, W, ch, first,
+             [&](long long p0, const uint8_t* O, long long own0, const uint8_t* D, long long dn0) {
+#pragma clang fp contract(off)
+               const long long i0 = p0 + (long long)threadIdx.x * kRgbPx;
+               int x = i0 < n ? rgb_col(i0, W, small) : 0;
+               int32_t a[kRgbPx], b[kRgbPx], rr[kRgbPx], dd[kRgbPx];
 #pragma unroll
-  for (int k = 0; k < kRgbPx; ++k) {
-    const long long i = i0 + k;
-    a[k] = b[k] = rr[k] = dd[k] = 0;
-    if (i < n) {
-      const uint8_t* q = O + (i * ch - own0);
-      const double r = q[0], g = q[1], bl = q[2];
-      a[k] = rgb_unary(m, m.mu_bg, r, g, bl);
-      b[k] = rgb_unary(m, m.mu_fg, r, g, bl);
-      if (x + 1 < W) rr[k] = rgb_pair_weight(m, rgb_d2(q, q + ch), two_beta);
-      if (i + W < n) dd[k] = rgb_pair_weight(m, rgb_d2(q, D + ((i + W) * ch - dn0)), two_beta);
-    }
-    if (++x == W) x = 0;
-  }
-  static_assert(kRgbPx == 4, "one int4 per array and thread");
-  if (i0 + kRgbPx <= n) {
-    *reinterpret_cast<int4*>(cs + i0) = make_int4(a[0], a[1], a[2], a[3]);
-    *reinterpret_cast<int4*>(ct + i0) = make_int4(b[0], b[1], b[2], b[3]);
-    *reinterpret_cast<int4*>(right + i0) = make_int4(rr[0], rr[1], rr[2], rr[3]);
-    *reinterpret_cast<int4*>(down + i0) = make_int4(dd[0], dd[1], dd[2], dd[3]);
-  } else {
-    for (int k = 0; k < kRgbPx && i0 + k < n; ++k) {
-      cs[i0 + k] = a[k];
-      ct[i0 + k] = b[k];
-      right[i0 + k] = rr[k];
-      down[i0 + k] = dd[k];
-    }
-  }
+               for (int k = 0; k < kRgbPx; ++k) {
+                 const long long i = i0 + k;
+                 a[k] = b[k] = rr[k] = dd[k] = 0;
+                 if (i < n) {
+                   const uint8_t* q = O + (i * ch - own0);
+                   const double r = q[0], g = q[1], bl = q[2];
+                   a[k] = rgb_unary(m, m.mu_bg, r, g, bl);
+                   b[k] = rgb_unary(m, m.mu_fg, r, g, bl);
+                   if (x + 1 < W) rr[k] = rgb_pair_weight(m, rgb_d2(q, q + ch), two_beta);
+                   if (i + W < n)
+                     dd[k] = rgb_pair_weight(m, rgb_d2(q, D + ((i + W) * ch - dn0)), two_beta);
+                 }
+                 if (++x == W) x = 0;
+               }
+               static_assert(kRgbPx == 4, "one int4 per array and thread");
+               if (i0 + kRgbPx <= n) {
+                 *reinterpret_cast<int4*>(cs + i0) = make_int4(a[0], a[1], a[2], a[3]);
+                 *reinterpret_cast<int4*>(ct + i0) = make_int4(b[0], b[1], b[2], b[3]);
+                 *reinterpret_cast<int4*>(right + i0) = make_int4(rr[0], rr[1], rr[2], rr[3]);
+                 *reinterpret_cast<int4*>(down + i0) = make_int4(dd[0], dd[1], dd[2], dd[3]);
+               } else {
+                 for (int k = 0; k < kRgbPx && i0 + k < n; ++k) {
+                   cs[i0 + k] = a[k];
+                   ct[i0 + k] = b[k];
+                   right[i0 + k] = rr[k];
+                   down[i0 + k] = dd[k];
+                 }
+               }
+             });
 }
 
 // Zeroes the final-excess sums of the selected images: the first flood
@@ -1045,7 +1076,8 @@ int pool_upload_rgb_many(Pool* p, int count, const RgbReq* reqs) {
       const PoolImage& im = p->imgs[r.img];
       hj->block0[i] = (int)blocks;
       hj->job[i] = RgbJob{r.d_rgb, im.pix0, im.W, im.H, r.channels, (int)blocks, *r.model};
-      blocks += ((long long)im.W * im.H + kRgbBlockPx - 1) / kRgbBlockPx;
+      constexpr long long px_per_block = (long long)kRgbChunks * kRgbBlockPx;
+      blocks += ((long long)im.W * im.H + px_per_block - 1) / px_per_block;
       if (blocks > 0x7fffffffLL) {
         set_error("colour images too large for one launch");
         return GCMI_EINVAL;
@@ -1302,10 +1334,10 @@ int make_polyline(int W, int H, int npts, const double* xy, double width, int32_
     const double width = std::min(span, (double)(xb - xa + 1));
     pl.xc[k] = std::max(1, (int)std::ceil(width / kPaintCols));
     // the capsule's pixel box, one pixel wider each way than needed
-    pl.sx0[k] = (int)std::max<long long>((long long)std::floor(xlo) - 1, -1);
-    pl.sx1[k] = (int)std::min<long long>((long long)std::ceil(xhi) + 1, W);
-    pl.sy0[k] = (int)std::max<long long>((long long)std::floor(ylo) - 1, -1);
-    pl.sy1[k] = (int)std::min<long long>((long long)std::ceil(yhi) + 1, H);
+    pl.sbox[k] = make_int4((int)std::max<long long>((long long)std::floor(xlo) - 1, -1),
+                           (int)std::min<long long>((long long)std::ceil(xhi) + 1, W),
+                           (int)std::max<long long>((long long)std::floor(ylo) - 1, -1),
+                           (int)std::min<long long>((long long)std::ceil(yhi) + 1, H));
     pl.c0[k] = (int)std::min<long long>(total, INT32_MAX);
     total += (rows + kPaintRows - 1) / kPaintRows * pl.xc[k];
   }
