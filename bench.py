@@ -326,7 +326,7 @@ def serve_leg(args):
              "batched": (dict(mode="batched", **fast),
                          (16000, 20000, 24000, 26000, 28000, 30000, 32000, 34000))}
     stroke_only = {"plain": (1500, 2000, 2500, 3000, 3500),
-                   "batched": (30000, 40000, 45000, 50000, 55000)}
+                   "batched": (30000, 35000, 40000, 45000, 50000)}
     out = {"workload": "256 sessions, sizes {512,1024,2048,4096}^2 weights 4:3:2:1 (seed 4), "
                        "Poisson arrivals, %g%% new images (8-bit colour, capacities built on "
                        "the device) / the rest polyline strokes, %.1f s per rate"

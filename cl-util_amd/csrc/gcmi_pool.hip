@@ -1565,7 +1565,6 @@ struct Solve {
   const PoolOps* ops;
   int n, nsel, nfresh, sweeps, bfs;
   int grid_relabel;
-  int grid_fold;  // the relabel's fold (a light kernel: more blocks in flight)
   bool dd;  // device-driven: fixed launch shapes, no host reads (recordable)
   Prof* P;
   void B(int k) const {
@@ -1645,7 +1644,7 @@ int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32
     gcnt = CNT_DIRTY;
   }
   S.B(K_FOLD);
-  S.ops->fold(p, S.grid_fold, glist, gcnt);
+  S.ops->fold(p, S.grid_relabel, glist, gcnt);
   S.E();
   // the neighbours the first pass's distances may improve start queue 0
   const Queue first{nullptr,
@@ -2006,8 +2005,7 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
   }
   // the relabel's first pass runs one block per listed tile (their number
   // is known on the device only; surplus blocks exit at once)
-  const int grid_first = std::min(nsel, kRelabelGridCap);
-  Solve S{p, p->ops, n, nsel, nfresh, sweeps, bfs, grid_first, grid_first, false, P};
+  Solve S{p, p->ops, n, nsel, nfresh, sweeps, bfs, std::min(nsel, kRelabelGridCap), false, P};
   const int32_t* hc = p->h_counts;
   int32_t nact = 0;
   int64_t rounds = 0;
@@ -2019,7 +2017,6 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
     R.dd = true;
     // gated launches of a done solve exit at once: one block per CU slot
     R.grid_relabel = std::min(nsel, p->grid_cap);
-    R.grid_fold = std::min(nsel, 3 * p->grid_cap);
     if (p->graph)
       GCMI_TRY(run_recorded(p, R, &cur));
     else
