@@ -1666,7 +1666,9 @@ int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32
       S.ops->bfs_round(p, p->grid_cap, round_queue(p, cur));
       cur = (cur + 1) % 3;
     }
-    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, false, S.nsel, nullptr, COUNT_ZEROED));
+    // the one compaction of a device-driven relabel clears the flags it
+    // lists (nothing reads them before the push round that follows)
+    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, true, S.nsel, nullptr, COUNT_ZEROED));
     hipLaunchKernelGGL(k_gate, dim3(1), dim3(1024), 0, p->stream, p->counts, cur,
                        p->q + (size_t)cur * p->cap_tiles, p->dev.dirty);
     GCMI_HIP(hipGetLastError());
@@ -1705,12 +1707,14 @@ int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32
   return GCMI_OK;
 }
 
-// A push round over the active list built (without clearing its flags) by
-// the last relabel.
+// A push round over the active list built by the last relabel.  A
+// host-driven relabel compacts after every batch of rounds without clearing
+// the flags, so they are cleared here; a device-driven one cleared them.
 int phase_push(const Solve& S, int32_t nact) {
   Pool* p = S.p;
-  hipLaunchKernelGGL(k_clear_listed, dim3(S.dd ? S.grid_clear() : (nact + 255) / 256), dim3(256),
-                     0, p->stream, p->dev.act, p->alist, p->counts + CNT_ACT);
+  if (!S.dd)
+    hipLaunchKernelGGL(k_clear_listed, dim3((nact + 255) / 256), dim3(256), 0, p->stream,
+                       p->dev.act, p->alist, p->counts + CNT_ACT);
   S.B(K_PUSH);
   S.ops->push(p, S.grid_for(nact), S.sweeps, false);
   S.E();
