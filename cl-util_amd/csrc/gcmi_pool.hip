@@ -131,9 +131,18 @@ k_imp_closure(Geo g, CutDev s, const int32_t* __restrict__ init, const int32_t* 
       const TG q = tile_geo(g, cur[i]);
       const int nb[4] = {q.ty > 0 ? q.t - q.tw : -1, q.ty < q.th - 1 ? q.t + q.tw : -1,
                          q.tx > 0 ? q.t - 1 : -1, q.tx < q.tw - 1 ? q.t + 1 : -1};
+      // the four neighbours' loads, then their claims, each group in flight
+      // together: unconditional, a missing neighbour standing in as the
+      // tile itself (fchg holds 0 or 1, so or-ing 0 only reads it)
+      int im[4], was[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) im[k] = s.imp[nb[k] >= 0 ? nb[k] : q.t] != 0 && nb[k] >= 0;
+#pragma unroll
       for (int k = 0; k < 4; ++k)
-        if (nb[k] >= 0 && s.imp[nb[k]] && atomicExch(&s.fchg[nb[k]], 1) == 0)
-          next[atomicAdd(&nnext, 1)] = nb[k];
+        was[k] = atomicOr(&s.fchg[nb[k] >= 0 ? nb[k] : q.t], im[k] ? 1 : 0);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (im[k] && was[k] == 0) next[atomicAdd(&nnext, 1)] = nb[k];
     }
     __syncthreads();
     n = nnext;
