@@ -126,6 +126,13 @@ __device__ __forceinline__ size_t tbase(int t) {
   return (size_t)t * (T * T);
 }
 
+// The tile at position pos of a tile list (no list: position = tile).
+// The per-tile loops load the next position's entry while they work on the
+// current tile, so a tile's first dependent load is not the list read.
+__device__ __forceinline__ int list_tile(const int32_t* __restrict__ list, int pos) {
+  return list ? list[pos] : pos;
+}
+
 // Blocks b, b+8, b+16, ... are dealt to the same XCD; give each XCD a
 // contiguous run of list positions so neighbouring tiles share its L2.
 __device__ __forceinline__ int xcd_pos(int b, int grid) {
@@ -308,8 +315,10 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
   }
   const int rpos = xcd_pos(blockIdx.x, gridDim.x);
   ImgAcc amin, apos;
+  int tnext = rpos < n ? list_tile(list, rpos) : 0;
   for (int pos = rpos; pos < n; pos += gridDim.x) {
-    const TG q = tile_geo(g, list ? list[pos] : pos);
+    const TG q = tile_geo(g, tnext);
+    if (pos + (int)gridDim.x < n) tnext = list_tile(list, pos + gridDim.x);
     const int t = q.t;
     const size_t row = tbase<T>(t) + ly * T + x0;  // this thread's segment, tile-blocked
     int e[R], h[R], cu[R], cd[R], cl[R], cr[R];
@@ -658,8 +667,11 @@ k_fold(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
   static_assert(4 * T - 4 <= kFoldNT<T>, "perimeter owners");
   const int n = *count;
   const int j = threadIdx.x;
-  for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
-    const int t = list ? list[pos] : pos;
+  const int p0 = xcd_pos(blockIdx.x, gridDim.x);
+  int tnext = p0 < n ? list_tile(list, p0) : 0;
+  for (int pos = p0; pos < n; pos += gridDim.x) {
+    const int t = tnext;
+    if (pos + (int)gridDim.x < n) tnext = list_tile(list, pos + gridDim.x);
     const int flagged = s.pflag[t];
     __syncthreads();  // every thread has read the flag before it is cleared
     if (!flagged) continue;
@@ -941,8 +953,11 @@ k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __r
     }
   }
   const int epoch = bq.epoch;
-  for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
-    const TG q = tile_geo(g, list ? list[pos] : pos);
+  const int p0 = xcd_pos(blockIdx.x, gridDim.x);
+  int tnext = p0 < n ? list_tile(list, p0) : 0;
+  for (int pos = p0; pos < n; pos += gridDim.x) {
+    const TG q = tile_geo(g, tnext);
+    if (pos + (int)gridDim.x < n) tnext = list_tile(list, pos + gridDim.x);
     const size_t base = tbase<T>(q.t);
     int d[R], m[R];
     int any_pos = 0;
@@ -1014,8 +1029,11 @@ k_bfs_round(Geo g, CutDev s, Queue Q) {
   bq_init(bq, Q);
   if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0)
     atomicAdd(&s.tally[TALLY_BFS], (unsigned long long)n);
-  for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
-    const TG q = tile_geo(g, Q.in[pos]);
+  const int p0 = xcd_pos(blockIdx.x, gridDim.x);
+  int tnext = p0 < n ? Q.in[p0] : 0;
+  for (int pos = p0; pos < n; pos += gridDim.x) {
+    const TG q = tile_geo(g, tnext);
+    if (pos + (int)gridDim.x < n) tnext = Q.in[pos + gridDim.x];
     const size_t base = tbase<T>(q.t);
     int d[R], d0[R], m[R];
 #pragma unroll
@@ -1173,8 +1191,11 @@ k_flood(Geo g, CutDev s, Queue Q, int mode, int32_t* lmark) {
   bq_init(bq, Q);
   if (!first && blockIdx.x == 0 && threadIdx.x == 0 && n > 0)
     atomicAdd(&s.tally[TALLY_FLOOD], (unsigned long long)n);
-  for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
-    const TG q = tile_geo(g, Q.in ? Q.in[pos] : pos);
+  const int p0 = xcd_pos(blockIdx.x, gridDim.x);
+  int tnext = p0 < n ? list_tile(Q.in, p0) : 0;
+  for (int pos = p0; pos < n; pos += gridDim.x) {
+    const TG q = tile_geo(g, tnext);
+    if (pos + (int)gridDim.x < n) tnext = list_tile(Q.in, pos + gridDim.x);
     const size_t base = tbase<T>(q.t);
     int r[R], r0[R], m[R], rl[R];
 #pragma unroll
@@ -1303,8 +1324,11 @@ k_labels(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __res
   const int lx = threadIdx.x % T, chain = threadIdx.x / T;
   const int sh = lane_id() & ~(T - 1) & 63;  // this row segment's first lane
   const int n = *count;
-  for (int pos = xcd_pos(blockIdx.x, gridDim.x); pos < n; pos += gridDim.x) {
-    const TG q = tile_geo(g, list ? list[pos] : pos);
+  const int p0 = xcd_pos(blockIdx.x, gridDim.x);
+  int tnext = p0 < n ? list_tile(list, p0) : 0;
+  for (int pos = p0; pos < n; pos += gridDim.x) {
+    const TG q = tile_geo(g, tnext);
+    if (pos + (int)gridDim.x < n) tnext = list_tile(list, pos + gridDim.x);
     const size_t base = tbase<T>(q.t);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
