@@ -33,7 +33,8 @@ enum {
   CNT_DONE = 10,  // a complete relabel found no active tile
   CNT_GSEL = 11,  // selected tiles, or 0 once the solve is done
   CNT_FIRST = 12,  // tiles of an incremental flood's first pass
-  CNT_N = 13
+  CNT_BLOCKS = 13,  // blocks of a gated compaction that have finished (0 between launches)
+  CNT_N = 14
 };
 
 enum {

@@ -1436,6 +1436,54 @@ int pool_paint_many(Pool* p, int count, const PaintReq* reqs) {
   return launch();
 }
 
+// The done gate of a device-driven relabel, run by the last block of the
+// relabel's compaction of the active flags (c: null for a plain compaction).
+// Sets the solve's done flag once a complete relabel (its queue, list
+// `qidx` at `q`, drained) found no active tile; a done solve zeroes the
+// active and relabel counts, so every later push and relabel launch of a
+// recorded solve exits at once.  A relabel whose queue did not drain marks
+// the queued tiles dirty: their distances are the only ones that may
+// violate h(p) <= h(q) + 1 across an arc (a neighbour's fell and they were
+// not recomputed yet), so a local relabel that recomputes them -- and the
+// rounds that follow it -- starts again from valid stored distances.  Also
+// zeroes the counts the next relabel appends to.
+struct Gate {
+  int32_t* c;
+  int qidx;
+  const int32_t* q;
+  int32_t* dirty;
+};
+
+__device__ __forceinline__ int load_count(const int32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by every thread of every block after its last append.  The last
+// block to finish runs the gate: each block's count append returned before
+// its finish was counted, so the active count it reads is complete.
+__device__ __forceinline__ void gate_last_block(const Gate& G) {
+  __shared__ int s_last;
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(&G.c[CNT_BLOCKS], 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (!s_last) return;
+  int32_t* c = G.c;
+  const int nq = load_count(c + CNT_Q + G.qidx);
+  for (int i = threadIdx.x; i < nq; i += blockDim.x) G.dirty[G.q[i]] = 1;
+  __syncthreads();  // every thread has read the count before it is zeroed
+  if (threadIdx.x == 0) {
+    if (nq == 0 && load_count(c + CNT_ACT) == 0) c[CNT_DONE] = 1;
+    if (load_count(c + CNT_DONE)) {
+      c[CNT_ACT] = 0;
+      c[CNT_GSEL] = 0;
+    }
+    // the next relabel's queue and local-set counts; the next gated
+    // compaction's block count
+    c[CNT_Q] = c[CNT_Q + 1] = c[CNT_Q + 2] = c[CNT_DIRTY] = 0;
+    c[CNT_BLOCKS] = 0;
+  }
+}
+
 // The selected tiles (positions of `sel`, or tiles 0..n-1) whose flag is
 // set, appended to `out` in any order with one atomic per block of
 // kFlagNT * kFlagPer positions; *count must be zero before.  clear: each
@@ -1447,7 +1495,7 @@ constexpr int kFlagPer = 4;
 __global__ void __launch_bounds__(kFlagNT)
 k_flag_list(const int32_t* __restrict__ sel, int n, int32_t* flags, int clear, int32_t* out,
             int32_t* count, const int32_t* __restrict__ tile_img,
-            const int32_t* __restrict__ imgskip) {
+            const int32_t* __restrict__ imgskip, Gate G) {
   __shared__ int s_off[kFlagNT / kWave];
   __shared__ int s_base;
   const int lane = lane_id(), wave = threadIdx.x / kWave;
@@ -1497,6 +1545,7 @@ k_flag_list(const int32_t* __restrict__ sel, int n, int32_t* flags, int clear, i
     }
     __syncthreads();  // s_off and s_base are rewritten by the next pass
   }
+  if (G.c) gate_last_block(G);
 }
 
 // The solve's tiles (sel) whose flag is set (leaving out the images flagged
@@ -1505,7 +1554,7 @@ k_flag_list(const int32_t* __restrict__ sel, int n, int32_t* flags, int clear, i
 // the solve already zeroed the count (no fill launch).
 enum { COUNT_FILL = 0, COUNT_ZEROED = 1 };
 static int compact_sel(Pool* p, int32_t* flags, int32_t* out, int cnt, bool clear, int nsel,
-                       const int32_t* imgskip, int zeroed) {
+                       const int32_t* imgskip, int zeroed, const Gate& gate = Gate{}) {
   Prof* P = p->prof.on ? &p->prof : nullptr;
   if (P) P->begin(p->stream, K_COMPACT);
   if (zeroed != COUNT_ZEROED)
@@ -1514,7 +1563,7 @@ static int compact_sel(Pool* p, int32_t* flags, int32_t* out, int cnt, bool clea
   hipLaunchKernelGGL(k_flag_list,
                      dim3(std::max(1, std::min((nsel + per_block - 1) / per_block, 2048))),
                      dim3(kFlagNT), 0, p->stream, p->lsel, nsel, flags, clear ? 1 : 0, out,
-                     p->counts + cnt, p->tile_img, imgskip);
+                     p->counts + cnt, p->tile_img, imgskip, gate);
   if (P) P->end(p->stream);
   GCMI_HIP(hipGetLastError());
   return GCMI_OK;
@@ -1540,30 +1589,6 @@ static int read_counts(Pool* p) {
   // stamps hold epochs: restart both before the counter nears overflow
   if (std::max(p->h_counts[CNT_EP], p->h_counts[CNT_EP + 1]) >= (1 << 30)) p->ep_reset = true;
   return GCMI_OK;
-}
-
-// Sets the solve's done flag once a complete relabel (its queue, list
-// `qidx` at `q`, drained) found no active tile; a done solve zeroes the
-// active and relabel counts, so every later push and relabel launch of a
-// recorded solve exits at once.  A relabel whose queue did not drain marks
-// the queued tiles dirty: their distances are the only ones that may
-// violate h(p) <= h(q) + 1 across an arc (a neighbour's fell and they were
-// not recomputed yet), so a local relabel that recomputes them -- and the
-// rounds that follow it -- starts again from valid stored distances.  Also
-// zeroes the counts the next relabel appends to.
-__global__ void k_gate(int32_t* c, int qidx, const int32_t* __restrict__ q, int32_t* dirty) {
-  const int nq = c[CNT_Q + qidx];
-  for (int i = threadIdx.x; i < nq; i += blockDim.x) dirty[q[i]] = 1;
-  __syncthreads();  // every thread has read the count before it is zeroed
-  if (threadIdx.x == 0) {
-    if (nq == 0 && c[CNT_ACT] == 0) c[CNT_DONE] = 1;
-    if (c[CNT_DONE]) {
-      c[CNT_ACT] = 0;
-      c[CNT_GSEL] = 0;
-    }
-    // the next relabel's queue and local-set counts
-    c[CNT_Q] = c[CNT_Q + 1] = c[CNT_Q + 2] = c[CNT_DIRTY] = 0;
-  }
 }
 
 namespace {
@@ -1676,11 +1701,11 @@ int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32
       cur = (cur + 1) % 3;
     }
     // the one compaction of a device-driven relabel clears the flags it
-    // lists (nothing reads them before the push round that follows)
-    GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, true, S.nsel, nullptr, COUNT_ZEROED));
-    hipLaunchKernelGGL(k_gate, dim3(1), dim3(1024), 0, p->stream, p->counts, cur,
-                       p->q + (size_t)cur * p->cap_tiles, p->dev.dirty);
-    GCMI_HIP(hipGetLastError());
+    // lists (nothing reads them before the push round that follows), and
+    // its last block runs the done gate
+    const Gate gate{p->counts, cur, p->q + (size_t)cur * p->cap_tiles, p->dev.dirty};
+    GCMI_TRY(
+        compact_sel(p, p->dev.act, p->alist, CNT_ACT, true, S.nsel, nullptr, COUNT_ZEROED, gate));
     *cur_out = cur;
     return GCMI_OK;
   }
