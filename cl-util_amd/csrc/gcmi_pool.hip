@@ -407,21 +407,45 @@ __device__ double rgb_two_beta(int W, int H, const unsigned long long* __restric
   return *slot;
 }
 
-// Weight of a pair with squared colour difference d2 (float64, each
-// operation rounded once, in the order the model states).
+// The models round each float64 operation once, in the order they state,
+// and end with rint.  Their divisions by a per-image constant c are first
+// tried as a product with 1/c (rounded once): that quotient is within 3
+// units of roundoff (u = 2^-53) of the correctly rounded one, so the value
+// handed to rint moves by less than 5u of itself for the unary and less
+// than (3 * 750 + 6)u = 2^-41.9 of itself for the exponential weight (exp's
+// argument is at most about 750 in size where its result is not 0).  When
+// that value lies further than 16 times the bound (2^-46 and 2^-38
+// relative, plus the same absolute) from every half-integer, rint gives the
+// same integer as with the correctly rounded division; otherwise the
+// division is done.  The capacities are those of the divisions, bit for
+// bit.
+__device__ __forceinline__ bool rint_settled(double v, double rel) {
+  const double k = rint(v);
+  return 0.5 - fabs(v - k) > rel * (fabs(v) + 1.0);
+}
+
+// Weight of a pair with squared colour difference d2; inv_tb = 1 / two_beta.
 __device__ __forceinline__ int32_t rgb_pair_weight(const gcmi_rgb_model& m, int d2,
-                                                   double two_beta) {
+                                                   double two_beta, double inv_tb) {
 #pragma clang fp contract(off)
   const double d = (double)d2;
-  if (m.pair_kind == GCMI_PAIR_EXP) return (int32_t)rint(m.pair_scale * exp(-d / two_beta));
+  if (m.pair_kind == GCMI_PAIR_EXP) {
+    const double w = m.pair_scale * exp(-(d * inv_tb));
+    if (rint_settled(w, 0x1p-38)) return (int32_t)rint(w);
+    return (int32_t)rint(m.pair_scale * exp(-d / two_beta));
+  }
   return (int32_t)rint(m.pair_scale / (m.inv_beta * sqrt(d) + m.inv_eps));
 }
 
+// Unary of a pixel of colour (r, g, b) against mean mu; inv_div = 1 /
+// unary_div.
 __device__ __forceinline__ int32_t rgb_unary(const gcmi_rgb_model& m, const double* mu, double r,
-                                             double g, double b) {
+                                             double g, double b, double inv_div) {
 #pragma clang fp contract(off)
   const double e0 = r - mu[0], e1 = g - mu[1], e2 = b - mu[2];
   const double d = (e0 * e0 + e1 * e1) + e2 * e2;
+  const double t = m.unary_scale * fmin(d * inv_div, m.unary_cap);
+  if (rint_settled(t, 0x1p-46)) return (int32_t)rint(t);
   return (int32_t)rint(m.unary_scale * fmin(d / m.unary_div, m.unary_cap));
 }
 
@@ -443,6 +467,8 @@ k_rgb_caps(const RgbJobs* __restrict__ J, int njobs, const unsigned long long* _
   __shared__ double s_two_beta;
   const double two_beta = m.pair_kind == GCMI_PAIR_EXP
       ? rgb_two_beta(W, H, sums + (size_t)j * kSumSlots, &s_two_beta) : 0.0;
+  const double inv_tb = m.pair_kind == GCMI_PAIR_EXP ? 1.0 / two_beta : 0.0;
+  const double inv_div = 1.0 / m.unary_div;
   const long long pix0 = jobs[j].pix0;
   cs += pix0;
   ct += pix0;
@@ -464,11 +490,13 @@ k_rgb_caps(const RgbJobs* __restrict__ J, int njobs, const unsigned long long* _
                  if (i < n) {
                    const uint8_t* q = O + (i * ch - own0);
                    const double r = q[0], g = q[1], bl = q[2];
-                   a[k] = rgb_unary(m, m.mu_bg, r, g, bl);
-                   b[k] = rgb_unary(m, m.mu_fg, r, g, bl);
-                   if (x + 1 < W) rr[k] = rgb_pair_weight(m, rgb_d2(q, q + ch), two_beta);
+                   a[k] = rgb_unary(m, m.mu_bg, r, g, bl, inv_div);
+                   b[k] = rgb_unary(m, m.mu_fg, r, g, bl, inv_div);
+                   if (x + 1 < W)
+                     rr[k] = rgb_pair_weight(m, rgb_d2(q, q + ch), two_beta, inv_tb);
                    if (i + W < n)
-                     dd[k] = rgb_pair_weight(m, rgb_d2(q, D + ((i + W) * ch - dn0)), two_beta);
+                     dd[k] = rgb_pair_weight(m, rgb_d2(q, D + ((i + W) * ch - dn0)), two_beta,
+                                             inv_tb);
                  }
                  if (++x == W) x = 0;
                }
