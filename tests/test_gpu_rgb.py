@@ -25,6 +25,13 @@ MODELS = {
                     unary_div=37.0, unary_cap=250.0, pair_scale=123.0)),
     "inv_custom": (dict(pair_kind=PAIR_INV, pair_scale=9.5, inv_beta=0.37, inv_eps=0.011),
                    dict(pair="inv", pair_scale=9.5, inv_beta=0.37, inv_eps=0.011)),
+    # integer means, scale 1 and divisor 2: every odd squared distance puts
+    # the unary exactly on a half-integer, where the device must fall back
+    # from its reciprocal product to the division and round ties to even
+    "halves": (dict(mu_bg=(60.0, 80.0, 100.0), mu_fg=(200.0, 10.0, 33.0), unary_scale=1.0,
+                    unary_div=2.0, unary_cap=1e6),
+               dict(mu_bg=(60.0, 80.0, 100.0), mu_fg=(200.0, 10.0, 33.0), unary_scale=1.0,
+                    unary_div=2.0, unary_cap=1e6)),
 }
 
 
