@@ -68,6 +68,42 @@ def test_paint_before_first_cut_and_soft_values():
     assert (gc.labels() == O.bk_cut(*work.arrays())[1]).all()
 
 
+def wide_strokes(W, H):
+    """Polylines whose rows span several 64-column passes and several
+    256-column pieces, and whose segments overlap the earlier ones."""
+    return [
+        ([(3.0, 10.0), (W - 4.0, 12.5)], 9, True),                        # near-horizontal, full width
+        ([(W - 2.0, 40.0), (5.5, 48.25), (W - 30.0, 55.0)], 21, False),   # doubles back over itself
+        ([(10.0, 5.0), (W / 2.0, H - 5.0), (W - 10.0, 5.0), (12.0, 6.0)], 31, True),
+        ([(0.0, H / 2.0), (W - 1.0, H / 2.0 + 0.5)], 64, False),          # 64 wide, rows of W
+        ([(W / 3.0, 0.0), (W / 3.0 + 0.25, H - 1.0)], 3, True),            # steep: one pass per row
+        ([(-30.0, -8.0), (W + 30.0, H + 8.0)], 12, False),                # clipped at both ends
+    ]
+
+
+@pytest.mark.parametrize("tile", [16, 32, 64])
+def test_paint_pixel_set_direct(tile):
+    """The painted pixels read back from the device's unaries equal the host
+    rasteriser's, pixel for pixel (an image not cut yet: painting only
+    rewrites its unaries), for strokes wider than a pass and a piece."""
+    W, H = 1100, 90
+    work = G.centred(W, H, seed=47)
+    gc = GraphCut(None, None, (W, H), tile=(tile, tile))
+    for k, (pts, width, fg) in enumerate(wide_strokes(W, H)):
+        gc.upload(*work.arrays())
+        s, t = (1000 + k, 3 + k) if fg else (5 + k, 2000 + k)
+        gc.paint(pts, width, s, t)
+        src, snk, right, down = gc.capacities()
+        es, et = work.source.copy(), work.sink.copy()
+        idx = G.rasterise_polyline(W, H, pts, width)
+        es.reshape(-1)[idx] = s
+        et.reshape(-1)[idx] = t
+        assert idx.size > 0, k
+        assert (src == es).all() and (snk == et).all(), (tile, k, int((src != es).sum()))
+        assert (right == work.right).all() and (down == work.down).all(), (tile, k)
+    gc.close()
+
+
 @pytest.mark.parametrize("tile", [32, 64])
 def test_paint_strokes_2048(tile):
     """The 2048x2048 warm-start workload painted on the device: after
