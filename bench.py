@@ -9,19 +9,30 @@ already resident in HBM, push-relabel to termination, minimal source set,
 labels written to HBM.  `value` is Mpix/s of that plain cut.  Beside it, in
 the same JSON line:
 
-  batched   64 ragged images (W, H ~ U[256, 2048]) solved together, tile /
-            sweeps / push rounds per relabel picked by the autotuner, against
-            64 sequential plain cuts;
+  roofline  the dominant kernel's algorithmic bytes per launch over its mean
+            launch time (HIP events on the cut's stream), and the whole
+            cut's algorithmic bytes over its step time;
+  tiles     the same cut at tiles 16, 32 and 64;
   warm      a 2048x2048 image and 100 seeded brush strokes: stroke-to-mask
             latency (device brush, reflow, packed mask in host memory) warm
             vs from scratch;
   served    256 resident sessions (512..4096 squared, weights 4:3:2:1),
-            Poisson arrivals, 10% new images / 90% strokes: stroke latency
-            p50/p99 and sustained stroke rate per serving mode;
+            Poisson arrivals, 10% new images / 90% strokes drawn afresh:
+            stroke latency p50/p99 and the sustained stroke rate per serving
+            mode, found by bisection;
+  batched   64 ragged images (W, H ~ U[256, 2048]) solved together, tile /
+            sweeps / push rounds per relabel picked by the autotuner, against
+            64 sequential plain cuts;
   cpu_baseline  the oracle's restatement of the reference push-relabel on
-            the 512x512 workload, on one host core.
+            the 512x512 workload, on the host's cores;
+  parity    (last) every checked result against the oracle's answers for
+            these workloads (tests/golden/bench_expected.json, written by
+            tests/golden/make_bench_golden.py): flow values and sha256 of the
+            packed minimal source sets.
 """
 import argparse
+import concurrent.futures as cf
+import glob
 import json
 import os
 import sys
@@ -35,16 +46,17 @@ import numpy as np  # noqa: E402
 METRIC = ("graph-cut Mpix/s + p50/p99 stroke-to-mask latency, "
           "served/batched vs plain")
 HBM_PEAK_GBS = 8000.0
+EXPECTED = os.path.join(HERE, "tests", "golden", "bench_expected.json")
 
 # Algorithmic HBM bytes per pixel touched, per kernel class:
 #   push_first 16 read per image pixel (source, sink, right, down) + 25 written
 #              per padded pixel (e, h, 4 caps, residual mask)
 #   push       49 per pixel per later residency: 24 read + 25 written
-#   bfs_init    5: residual mask read, h written
-#   bfs_round   9: mask + h read, h written, per tile pass
-#   flood       2: mask read, reach written, per tile pass
-#   labels      2: reach read, label written
-ALG_BYTES_PER_PX = {"push": 49, "bfs_init": 5, "bfs_round": 9, "labels": 2}
+#   bfs_init    5 per pixel of a relabel's first pass: residual mask read, h written
+#   bfs_round   9 per pixel of a later BFS pass: mask + h read, h written
+#   flood       2 per pixel of a flood pass: mask read, reach written
+#   labels      2 per image pixel: reach read, label written (+ 1 bit packed)
+ALG_BYTES_PER_PX = {"push": 49, "bfs_init": 5, "bfs_round": 9, "flood": 2, "labels": 2}
 
 
 def log(msg):
@@ -65,7 +77,6 @@ def parse():
     ap.add_argument("--rounds", type=int, default=2,
                     help="plain cut: device-driven BFS/flood rounds per relabel (0: host-driven)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-parity", action="store_true", help="skip the oracle check")
     ap.add_argument("--no-sweep", action="store_true", help="skip the tile sweep")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start leg")
@@ -74,13 +85,28 @@ def parse():
     ap.add_argument("--serve-modes", default="plain,cold,batched",
                     help="serving modes to run (comma-separated)")
     ap.add_argument("--serve-rates", default="",
-                    help="replace each mode's rate sweep (comma-separated requests/s)")
+                    help="probe these rates (comma-separated requests/s) instead of bisecting")
     ap.add_argument("--serve-image-frac", type=float, default=0.1,
                     help="share of new-image requests in the serving mix")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0: the usable cores, at most 16)")
     ap.add_argument("--serve-max-batch", type=int, default=64,
                     help="most requests per served solve")
     return ap.parse_args()
+
+
+def load_expected():
+    with open(EXPECTED) as f:
+        return json.load(f)
+
+
+def answer(flow, bits, exp):
+    """Parity record of one result against the oracle's answer."""
+    import gcmi_serving as S
+    sha = S.mask_sha(bits)
+    return {"flow_equal": int(flow) == exp["flow"], "mask_equal": sha == exp["mask_sha"],
+            "mask_sha": sha}
 
 
 # ------------------------------------------------------------------ plain
@@ -105,32 +131,51 @@ def time_cut(gc, steps, warmup, sweeps, bfs, profile=False):
     return dt / steps, totals, stats
 
 
-def roofline(totals, stats, tile, W, H, steps):
-    """Dominant kernel class by total time; algorithmic bytes per launch over
-    its mean launch duration (HIP events on the cut's stream)."""
+def class_bytes(stats, tile, W, H):
+    """Algorithmic bytes per run of each kernel class (from the run's
+    counters)."""
     T2 = tile * tile
     ntiles = stats["tiles"]
-    per_run = {
+    return {
         "push_first": 16 * W * H + 25 * ntiles * T2,
         "push": ALG_BYTES_PER_PX["push"] * (stats["residencies"] - ntiles) * T2,
-        "bfs_init": ALG_BYTES_PER_PX["bfs_init"] * stats["global_relabels"] * ntiles * T2,
-        "bfs_round": ALG_BYTES_PER_PX["bfs_round"]
-        * (stats["bfs_passes"] - stats["global_relabels"] * ntiles) * T2,
+        "bfs_init": ALG_BYTES_PER_PX["bfs_init"] * stats["relabel_tiles"] * T2,
+        "bfs_round": ALG_BYTES_PER_PX["bfs_round"] * stats["bfs_round_passes"] * T2,
+        "flood": ALG_BYTES_PER_PX["flood"] * stats["flood_passes"] * T2,
         "labels": ALG_BYTES_PER_PX["labels"] * W * H,
     }
-    dom = max((k for k in totals if per_run.get(k) and totals[k][1]),
-              key=lambda k: totals[k][0])
-    ms, launches = totals[dom]
-    per_launch_bytes = per_run[dom] * steps / launches
-    avg_s = ms / launches / 1e3
-    achieved = per_launch_bytes / avg_s / 1e9
-    traffic, source = pmc_traffic(dom, W, tile) if W == H else (None, None)
-    return {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "traffic_source": source,
-            "alg_bytes_per_launch": int(per_launch_bytes),
-            "avg_launch_us": round(avg_s * 1e6, 2)}
+
+
+def kernel_table(totals, stats, tile, W, H, steps):
+    """Per kernel class of a profiled (host-driven) run: time per step, mean
+    launch time, algorithmic bytes per launch, GB/s and fraction of peak."""
+    per_run = class_bytes(stats, tile, W, H)
+    out = {}
+    for k, (ms, launches) in totals.items():
+        if not launches:
+            continue
+        row = {"ms_per_step": round(ms / steps, 4), "launches_per_step": launches / steps,
+               "avg_launch_us": round(ms / launches * 1e3, 2)}
+        if per_run.get(k):
+            b = per_run[k] * steps / launches
+            gbs = b / (ms / launches / 1e3) / 1e9
+            row.update(alg_bytes_per_launch=int(b), achieved_gbs=round(gbs, 1),
+                       frac=round(gbs / HBM_PEAK_GBS, 4))
+        out[k] = row
+    return out
+
+
+def roofline(table, W, tile):
+    """The dominant kernel class (by time) that has an algorithmic byte
+    count."""
+    dom = max((k for k in table if "achieved_gbs" in table[k]),
+              key=lambda k: table[k]["ms_per_step"])
+    r = table[dom]
+    traffic, source = pmc_traffic(dom, W, tile)
+    return {"kernel": dom, "bound": "hbm", "achieved": r["achieved_gbs"],
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["frac"], "traffic": traffic,
+            "traffic_source": source, "alg_bytes_per_launch": r["alg_bytes_per_launch"],
+            "avg_launch_us": r["avg_launch_us"]}
 
 
 def pmc_traffic(kernel, size, tile):
@@ -149,11 +194,206 @@ def pmc_traffic(kernel, size, tile):
     return None, None
 
 
+def tile_leg(args, dev, W, H, exp, sweeps, bfs):
+    """The plain cut at tiles 16, 32 and 64: Mpix/s, whole-cut algorithmic
+    GB/s, the first push round's GB/s, flow and mask against the oracle."""
+    from GraphCut import GraphCut
+    out = {}
+    for t in (16, 32, 64):
+        g = GraphCut(None, None, (W, H), tile=t)
+        g.set_local_relabel(bool(args.local))
+        g.set_device_rounds(args.rounds)
+        g.upload_device(*dev)
+        n = max(3, args.steps // 4)
+        ts, _, st = time_cut(g, n, 1, sweeps, bfs)
+        flow = g.flow
+        bits = g.labels_packed()
+        _, tot, pst = time_cut(g, 3, 0, sweeps, bfs, profile=True)
+        tab = kernel_table(tot, pst, t, W, H, 3)
+        out[str(t)] = {"ms": round(ts * 1e3, 4), "mpix_s": round(W * H / ts / 1e6, 1),
+                       "whole_cut_gbs": round(st["alg_bytes"] / ts / 1e9, 1),
+                       "push_first_gbs": tab.get("push_first", {}).get("achieved_gbs"),
+                       "parity": answer(flow, bits, exp)}
+        g.close()
+    return out
+
+
+# ------------------------------------------------------------------- warm
+
+
+def warm_leg(args, exp):
+    """2048x2048, 100 strokes: warm reflow vs from scratch, per stroke
+    paint + cut + packed mask readback; every stroke's flow and mask
+    checked (outside the timed region)."""
+    import gcmi_serving as S
+    import gcmi_workloads as G
+    from GraphCut import GraphCut
+    W = H = 2048
+    log("warm: generating the 2048x2048 image and 100 strokes")
+    work = G.centred(W, H, seed=3)
+    strokes = G.strokes(W, H, count=100, seed=3, raster=False)
+    res = {}
+    bad = {}
+    # warm reflows host-driven with full relabels, with local relabels, and
+    # device-driven with local relabels (2 or 1 rounds per relabel); scratch:
+    # every stroke from scratch, with the plain cut's settings
+    variants = {"warm_host": {}, "warm_local": dict(local=True),
+                "warm_device": dict(local=True, rounds=2),
+                "warm_device1": dict(local=True, rounds=1),
+                "scratch": dict(local=True, rounds=2)}
+    for mode, opt in variants.items():
+        gc = GraphCut(None, None, (W, H), tile=args.tile)
+        gc.set_local_relabel(opt.get("local", False))
+        gc.set_device_rounds(opt.get("rounds", 0))
+        gc.upload(*work.arrays())
+        gc.cut()
+        gc.labels_packed()
+        lat = []
+        nbad = 0
+        for k, st in enumerate(strokes):
+            s, t = st.capacities()
+            t0 = time.perf_counter()
+            gc.paint(st.points, st.width, s, t)
+            if mode == "scratch":
+                gc.forget()
+            flow = gc.cut()
+            bits = gc.labels_packed()
+            lat.append((time.perf_counter() - t0) * 1e3)
+            a = answer(flow, bits, exp[k])
+            nbad += not (a["flow_equal"] and a["mask_equal"])
+        bad[mode] = nbad
+        res[mode] = {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+                     "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                     "mean_ms": round(float(np.mean(lat)), 3)}
+        gc.close()
+    best = min((m for m in variants if m != "scratch"), key=lambda m: res[m]["mean_ms"])
+    out = {"workload": "2048x2048 centred ellipse (seed 3), 100 strokes (seed 3), "
+                       "device brush, packed mask to host", "tile": args.tile,
+           "warm": dict(res[best], variant=best), "scratch": res["scratch"],
+           "variants": res,
+           "speedup_mean": round(res["scratch"]["mean_ms"] / res[best]["mean_ms"], 2)}
+    parity = {"strokes_checked": len(strokes) * len(variants),
+              "mismatches": sum(bad.values()), "per_variant_mismatches": bad}
+    return out, parity
+
+
+# ---------------------------------------------------------------- serving
+
+
+def verify_served(srv, sids, mix, exp):
+    """The verification trace (gcmi_serving.verification_trace): every
+    session gets its base image again, then the trace is submitted from one
+    thread; every request's flow and every session's final mask against the
+    oracle's answers."""
+    import gcmi_serving as S
+    reset = [(0, sids[i]) + S.image_request(mix.base(i), True) for i in range(mix.nsess)]
+    for r in srv.replay(reset):
+        srv.wait(int(r))
+        srv.forget(int(r))
+    items = S.verification_trace(mix)
+    t0 = time.monotonic_ns() + int(0.1e9)
+    ids = srv.replay([(t0 + x[0], sids[x[1]], x[2], x[3]) for x in items])
+    flows_bad = 0
+    for j, rid in enumerate(ids):
+        info = srv.wait(int(rid))
+        srv.forget(int(rid))
+        flows_bad += info.flow != exp["request_flows"][j]
+    masks_bad = 0
+    for i in range(mix.nsess):
+        bits, _ = srv.mask_bits(sids[i])
+        e = exp["sessions"][i]
+        masks_bad += S.mask_sha(bits) != e["mask_sha"]
+    return {"requests_checked": len(ids), "request_flow_mismatches": int(flows_bad),
+            "sessions_checked": mix.nsess, "session_mask_mismatches": int(masks_bad)}
+
+
+def serve_leg(args, exp):
+    """The 256-session mix.  Modes: plain (one request at a time, each cut
+    from scratch: the baseline), cold (continuous batching, every request
+    cut from scratch) and batched (continuous batching, strokes reflowed
+    warm: device-driven runs with local relabels).  New images arrive as
+    8-bit colour images in every mode.  Each mode's sustained rate (p99 <=
+    20 ms on 2048^2 sessions, throughput within 10% of the offer) is found
+    by bisection to 2%; then the verification trace is checked."""
+    import gcmi_serving as S
+    from GraphCutServer import GraphCutServer
+    log("served: generating the 256-session mix")
+    mix = S.Mix(nsess=256, seed=4)
+    # every mode runs the plain cut's fastest settings (local relabels after
+    # the first, 2 device rounds per relabel)
+    fast = dict(local_relabel=True, device_rounds=2)
+    modes = {"plain": (dict(mode="plain", **fast), (2000, 4000)),
+             "cold": (dict(mode="cold", **fast), (2000, 6000)),
+             "batched": (dict(mode="batched", **fast), (16000, 36000))}
+    stroke_only = {"plain": (2000, 4000), "batched": (24000, 48000)}
+    out = {"workload": "256 sessions, sizes {512,1024,2048,4096}^2 weights 4:3:2:1 (seed 4), "
+                       "Poisson arrivals, %g%% new images (8-bit colour, capacities built on "
+                       "the device) / the rest polyline strokes drawn afresh per request, "
+                       "%.1f s per probed rate" % (100 * args.serve_image_frac,
+                                                   args.serve_seconds),
+           "resident_pixels": mix.pixels, "hbm_budget_bytes": 200 * 10**9, "modes": {},
+           "stroke_only": {}}
+    only = [m for m in args.serve_modes.split(",") if m]
+    modes = {m: v for m, v in modes.items() if m in only}
+    fixed = tuple(float(x) for x in args.serve_rates.split(",")) if args.serve_rates else None
+    parity = {}
+    seed = [11]
+
+    def search(srv, sids, bracket, image_frac, rgb):
+        def probe(rate):
+            seed[0] += 1
+            pt = S.run_trace(srv, sids, mix, rate, duration=args.serve_seconds, seed=seed[0],
+                             image_frac=image_frac, rgb=rgb)
+            log("served: %s rate %g: p99(2048) %s ms, throughput %s/s" % (
+                srv.mode, rate, pt.get("stroke_p99_ms_2048"), pt["stroke_throughput"]))
+            return pt
+        if fixed:
+            pts = [probe(r) for r in fixed]
+            return pts, S.sustained(pts)
+        return S.find_sustained(probe, *bracket)
+
+    for mode, (kw, bracket) in modes.items():
+        log("served: mode %s" % mode)
+        with GraphCutServer(tile=32, hbm_budget=200 * 10**9, max_sessions=256,
+                            max_batch=args.serve_max_batch, **kw) as srv:
+            sids = S.open_sessions(srv, mix, rgb=True)
+            # an unmeasured warm-up trace (first-use allocations, page-in)
+            S.run_trace(srv, sids, mix, bracket[0], duration=0.5, seed=7,
+                        image_frac=args.serve_image_frac, rgb=True)
+            pts, best = search(srv, sids, bracket, args.serve_image_frac, True)
+            out["modes"][mode] = {"points": pts, "sustained_2048_p99_20ms": best,
+                                  "device_bytes": srv.stats()["device_bytes"]}
+            if mode in stroke_only and not fixed:
+                log("served: mode %s, strokes only" % mode)
+                pts, best = search(srv, sids, stroke_only[mode], 0.0, False)
+                out["stroke_only"][mode] = {"points": pts, "sustained_2048_p99_20ms": best}
+            log("served: mode %s, verification trace" % mode)
+            parity[mode] = verify_served(srv, sids, mix, exp)
+        # the closed server's device memory is released (and cleared by the
+        # driver) in the background: let that finish before the next mode
+        time.sleep(3.0)
+
+    def rate(entry):
+        return (entry["sustained_2048_p99_20ms"] or {}).get("stroke_rate")
+
+    sus = {m: rate(v) for m, v in out["modes"].items()}
+    out["sustained_stroke_rate"] = sus
+    if sus.get("plain"):
+        best = max((v for m, v in sus.items() if m != "plain" and v), default=None)
+        out["served_vs_plain"] = round(best / sus["plain"], 2) if best else None
+    so = {m: rate(v) for m, v in out["stroke_only"].items()}
+    out["stroke_only_sustained_stroke_rate"] = so
+    if so.get("plain") and so.get("batched"):
+        out["stroke_only_served_vs_plain"] = round(so["batched"] / so["plain"], 2)
+    return out, parity
+
+
 # ---------------------------------------------------------------- batched
 
 
-def batched_leg(args):
+def batched_leg(args, exp):
     """64 ragged images: autotuned batched run vs 64 sequential plain cuts."""
+    import gcmi_serving as S
     import gcmi_workloads as G
     from GraphCut import GraphCut
     from GraphCutBatch import GraphCutBatch
@@ -204,6 +444,10 @@ def batched_leg(args):
     host = min((r for r in tune if r["graph"] == 0), key=lambda r: r["ms"])
     flows = b.cut(sweeps=best["sweeps"], bfs=best["bfs"])
     stats = b.stats()
+    nbad = 0
+    for i in range(len(works)):
+        a = answer(flows[i], S.pack_mask(b.labels(i)), exp[i])
+        nbad += not (a["flow_equal"] and a["mask_equal"])
     # sequential plain cuts with the same tile, sweeps and push rounds, and
     # the plain cut's own run settings (local relabels, 2 device rounds)
     log("batched: 64 sequential plain cuts")
@@ -221,186 +465,81 @@ def batched_leg(args):
         for g in cuts:
             g.cut(sweeps=best["sweeps"], bfs=best["bfs"])
     t_seq = (time.perf_counter() - t0) / reps
-    same = all(g.flow == f for g, f in zip(cuts, flows))
+    seq_bad = sum(g.flow != e["flow"] for g, e in zip(cuts, exp))
     for g in cuts:
         g.close()
     for x in batches.values():
         x.close()
-    return {"workload": "64 images, W,H ~ U[256,2048] (seed 2), centred ellipse, sigma 20",
-            "pixels": pixels, "autotune": best, "autotune_trials": len(tune),
-            "batched_ms": round(t_batch * 1e3, 3),
-            "batched_mpix_s": round(pixels / t_batch / 1e6, 1),
-            "sequential_ms": round(t_seq * 1e3, 3),
-            "sequential_mpix_s": round(pixels / t_seq / 1e6, 1),
-            "speedup": round(t_seq / t_batch, 2), "flows_equal_sequential": bool(same),
-            "best_host_driven": host, "host_syncs": stats["host_syncs"],
-            "rounds": stats["rounds"], "global_relabels": stats["global_relabels"]}
-
-
-# ------------------------------------------------------------------- warm
-
-
-def warm_leg(args):
-    """2048x2048, 100 strokes: warm reflow vs from scratch, per stroke
-    paint + cut + packed mask readback."""
-    import gcmi_workloads as G
-    from GraphCut import GraphCut
-    W = H = 2048
-    log("warm: generating the 2048x2048 image and 100 strokes")
-    work = G.centred(W, H, seed=3)
-    strokes = G.strokes(W, H, count=100, seed=3, raster=False)
-    res = {}
-    masks = {}
-    flows = {}
-    # warm reflows host-driven with full relabels, with local relabels, and
-    # device-driven with local relabels (2 or 1 rounds per relabel); scratch:
-    # every stroke from scratch, with the plain cut's settings
-    variants = {"warm_host": {}, "warm_local": dict(local=True),
-                "warm_device": dict(local=True, rounds=2),
-                "warm_device1": dict(local=True, rounds=1),
-                "scratch": dict(local=True, rounds=2)}
-    for mode, opt in variants.items():
-        gc = GraphCut(None, None, (W, H), tile=args.tile)
-        gc.set_local_relabel(opt.get("local", False))
-        gc.set_device_rounds(opt.get("rounds", 0))
-        gc.upload(*work.arrays())
-        gc.cut()
-        gc.labels_packed()
-        lat = []
-        fl = []
-        for st in strokes:
-            s, t = st.capacities()
-            t0 = time.perf_counter()
-            gc.paint(st.points, st.width, s, t)
-            if mode == "scratch":
-                gc.forget()
-            fl.append(gc.cut())
-            bits = gc.labels_packed()
-            lat.append((time.perf_counter() - t0) * 1e3)
-        masks[mode] = bits
-        flows[mode] = fl
-        res[mode] = {"p50_ms": round(float(np.percentile(lat, 50)), 3),
-                     "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                     "mean_ms": round(float(np.mean(lat)), 3)}
-        gc.close()
-    best = min((m for m in variants if m != "scratch"), key=lambda m: res[m]["mean_ms"])
-    out = {"workload": "2048x2048 centred ellipse (seed 3), 100 strokes (seed 3), "
-                       "device brush, packed mask to host", "tile": args.tile,
-           "warm": dict(res[best], variant=best), "scratch": res["scratch"],
-           "variants": res,
-           "speedup_mean": round(res["scratch"]["mean_ms"] / res[best]["mean_ms"], 2),
-           "flows_equal": all(flows[m] == flows["scratch"] for m in variants),
-           "final_mask_equal": all(bool((masks[m] == masks["scratch"]).all()) for m in variants)}
-    if not args.no_parity:
-        sys.path.insert(0, os.path.join(HERE, "oracle"))
-        import oracle as O
-        cur = work.copy()
-        for st in G.strokes(W, H, count=100, seed=3):
-            G.apply_stroke(cur, st)
-        fo, mo = O.bk_cut(*cur.arrays())
-        packed = np.packbits(mo.reshape(-1), bitorder="little")
-        packed = np.pad(packed, (0, (-packed.size) % 4)).view(np.uint32)
-        out["parity_last_stroke"] = {"flow_equal": fo == flows[best][-1],
-                                     "mask_equal": bool((packed == masks[best]).all())}
-    return out
-
-
-# ---------------------------------------------------------------- serving
-
-
-def serve_leg(args):
-    """The 256-session mix at swept Poisson rates.  Modes: plain (one request
-    at a time, each cut from scratch: the baseline), cold (continuous
-    batching, every request cut from scratch) and batched (continuous
-    batching, strokes reflowed warm: device-driven runs with local
-    relabels).  New images arrive as 8-bit colour images in every mode."""
-    import gcmi_serving as S
-    from GraphCutServer import GraphCutServer
-    log("served: generating the 256-session mix")
-    mix = S.Mix(nsess=256, seed=4)
-    # every mode runs the plain cut's fastest settings (local relabels after
-    # the first, 2 device rounds per relabel)
-    fast = dict(local_relabel=True, device_rounds=2)
-    modes = {"plain": (dict(mode="plain", **fast), (2000, 2500, 3000, 3500, 4000)),
-             "cold": (dict(mode="cold", **fast), (2000, 3000, 4000, 6000)),
-             "batched": (dict(mode="batched", **fast),
-                         (16000, 20000, 24000, 26000, 28000, 30000, 32000, 34000))}
-    stroke_only = {"plain": (1500, 2000, 2500, 3000, 3500),
-                   "batched": (30000, 35000, 40000, 45000, 50000)}
-    out = {"workload": "256 sessions, sizes {512,1024,2048,4096}^2 weights 4:3:2:1 (seed 4), "
-                       "Poisson arrivals, %g%% new images (8-bit colour, capacities built on "
-                       "the device) / the rest polyline strokes, %.1f s per rate"
-                       % (100 * args.serve_image_frac, args.serve_seconds),
-           "resident_pixels": mix.pixels, "hbm_budget_bytes": 200 * 10**9, "modes": {},
-           "stroke_only": {}}
-    only = [m for m in args.serve_modes.split(",") if m]
-    modes = {m: v for m, v in modes.items() if m in only}
-    if args.serve_rates:
-        rates = tuple(float(x) for x in args.serve_rates.split(","))
-        modes = {m: (kw, rates) for m, (kw, _) in modes.items()}
-        stroke_only = {}
-    for mode, (kw, rs) in modes.items():
-        log("served: mode %s" % mode)
-        with GraphCutServer(tile=32, hbm_budget=200 * 10**9, max_sessions=256,
-                            max_batch=args.serve_max_batch, **kw) as srv:
-            sids = S.open_sessions(srv, mix, rgb=True)
-            # an unmeasured warm-up trace (first-use allocations, page-in)
-            S.run_trace(srv, sids, mix, rs[0], duration=0.5, seed=7,
-                        image_frac=args.serve_image_frac, rgb=True)
-            pts = [S.run_trace(srv, sids, mix, r, duration=args.serve_seconds, seed=11 + k,
-                               image_frac=args.serve_image_frac, rgb=True)
-                   for k, r in enumerate(rs)]
-            out["modes"][mode] = {"points": pts, "sustained_2048_p99_20ms": S.sustained(pts),
-                                  "device_bytes": srv.stats()["device_bytes"]}
-            if mode in stroke_only:
-                # the same sessions, strokes only
-                log("served: mode %s, strokes only" % mode)
-                pts = [S.run_trace(srv, sids, mix, r, duration=args.serve_seconds,
-                                   seed=51 + k, image_frac=0.0)
-                       for k, r in enumerate(stroke_only[mode])]
-                out["stroke_only"][mode] = {"points": pts,
-                                            "sustained_2048_p99_20ms": S.sustained(pts)}
-        # the closed server's device memory is released (and cleared by the
-        # driver) in the background: let that finish before the next mode
-        time.sleep(3.0)
-
-    def rate(entry):
-        return (entry["sustained_2048_p99_20ms"] or {}).get("stroke_rate")
-
-    sus = {m: rate(v) for m, v in out["modes"].items()}
-    out["sustained_stroke_rate"] = sus
-    if sus.get("plain"):
-        best = max((v for m, v in sus.items() if m != "plain" and v), default=None)
-        out["served_vs_plain"] = round(best / sus["plain"], 2) if best else None
-    so = {m: rate(v) for m, v in out["stroke_only"].items()}
-    out["stroke_only_sustained_stroke_rate"] = so
-    if so.get("plain") and so.get("batched"):
-        out["stroke_only_served_vs_plain"] = round(so["batched"] / so["plain"], 2)
-    return out
+    out = {"workload": "64 images, W,H ~ U[256,2048] (seed 2), centred ellipse, sigma 20",
+           "pixels": pixels, "autotune": best, "autotune_trials": len(tune),
+           "batched_ms": round(t_batch * 1e3, 3),
+           "batched_mpix_s": round(pixels / t_batch / 1e6, 1),
+           "sequential_ms": round(t_seq * 1e3, 3),
+           "sequential_mpix_s": round(pixels / t_seq / 1e6, 1),
+           "speedup": round(t_seq / t_batch, 2),
+           "best_host_driven": host, "host_syncs": stats["host_syncs"],
+           "rounds": stats["rounds"], "global_relabels": stats["global_relabels"]}
+    parity = {"images_checked": len(works), "mismatches": int(nbad),
+              "sequential_flow_mismatches": int(seq_bad)}
+    return out, parity
 
 
 # ------------------------------------------------------------------- CPU
 
 
-def cpu_baseline(seconds):
+def host_cpu():
+    """nproc (the cores this process may use), the CPU model and whether an
+    OpenCL CPU runtime (an ICD entry or a pocl library) is installed, which
+    running the reference's own kernels on the host would need."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    icds = sorted(glob.glob("/etc/OpenCL/vendors/*.icd"))
+    pocl = sorted(set(sum((glob.glob(os.path.join(d, "libpocl*"))
+                           for d in ("/usr/lib", "/usr/lib/x86_64-linux-gnu", "/usr/local/lib",
+                                     "/opt/rocm/lib", "/usr/lib64")), [])))
+    return {"nproc": len(os.sched_getaffinity(0)), "cpu_model": model,
+            "opencl_icds": icds, "pocl_libraries": pocl}
+
+
+def cpu_baseline(seconds, threads):
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import gcmi_workloads as G
     import oracle as O
+    info = host_cpu()
+    if threads <= 0:
+        threads = max(1, min(16, info["nproc"]))
     w = G.centred(512, 512, seed=0)
     a = w.arrays()
     O.pr_cut(*a)  # warm (load, page in)
-    n = 0
+    stop = time.perf_counter() + seconds
+
+    def worker(_):
+        n = 0
+        while time.perf_counter() < stop:
+            O.pr_cut(*a, bfs=5)  # ctypes releases the GIL during the cut
+            n += 1
+        return n
+
     t0 = time.perf_counter()
-    while True:
-        O.pr_cut(*a, bfs=5)
-        n += 1
-        if time.perf_counter() - t0 >= seconds:
-            break
+    with cf.ThreadPoolExecutor(threads) as ex:
+        n = sum(ex.map(worker, range(threads)))
     dt = time.perf_counter() - t0
-    return {"value": round(n * w.pixels / dt / 1e6, 3), "unit": "Mpix/s", "cores": 1,
+    runtime = "absent" if not (info["opencl_icds"] or info["pocl_libraries"]) else "present"
+    return {"value": round(n * w.pixels / dt / 1e6, 3), "unit": "Mpix/s", "cores": threads,
             "kind": "port",
             "sample": "%d cuts of the 512x512 centred-ellipse workload (seed 0) by the "
-                      "oracle's C restatement of the reference push-relabel, %.1f s" % (n, dt)}
+                      "oracle's C restatement of the reference push-relabel, %d threads of "
+                      "independent cuts, %.1f s (oracle CPU, not the reference: an OpenCL "
+                      "CPU runtime for graphcut.cl is %s on this host)" % (n, threads, dt,
+                                                                            runtime),
+            "host": info}
 
 
 def main():
@@ -409,10 +548,12 @@ def main():
         print("bench.py measures one GPU: every box here has a single MI355X; "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         return 2
+    import gcmi_serving as S
     import gcmi_workloads as G
     from GraphCut import GraphCut
     from gcmi_lib import DeviceBuffer
 
+    exp = load_expected()
     W = H = args.size
     log("generating %dx%d workload" % (W, H))
     work = G.three_ellipses(W, H, seed=1)
@@ -424,10 +565,14 @@ def main():
     sweeps = args.sweeps or None
     bfs = args.bfs or None
     log("timing the plain cut")
-    t_step, totals, stats = time_cut(gc, args.steps, args.warmup, sweeps, bfs, profile=True)
-    t_plain, _, _ = time_cut(gc, args.steps, 1, sweeps, bfs, profile=False)
+    t_prof, totals, pstats = time_cut(gc, args.steps, args.warmup, sweeps, bfs, profile=True)
+    t_plain, _, stats = time_cut(gc, args.steps, 1, sweeps, bfs, profile=False)
     value = W * H / t_plain / 1e6
     flow = gc.flow
+    parity = {}
+    if W == 4096 and H == 4096:
+        parity["plain_4096"] = answer(flow, gc.labels_packed(), exp["plain_4096"])
+    table = kernel_table(totals, pstats, args.tile, W, H, args.steps)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_plain * 1e3, 4),
@@ -439,43 +584,49 @@ def main():
                    "pixels": W * H},
         "flow": flow,
         "run_stats": stats,
-        "roofline": roofline(totals, stats, args.tile, W, H, args.steps),
-        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in totals.items()},
-        "ms_per_step_profiled": round(t_step * 1e3, 4),
+        "roofline": roofline(table, W, args.tile),
+        "whole_cut": {"alg_bytes_per_cut": stats["alg_bytes"],
+                      "achieved_gbs": round(stats["alg_bytes"] / t_plain / 1e9, 1),
+                      "frac": round(stats["alg_bytes"] / t_plain / 1e9 / HBM_PEAK_GBS, 4)},
+        "kernels_host_driven": table,
+        "ms_per_step_profiled_host_driven": round(t_prof * 1e3, 4),
     }
-    if not args.no_parity:
-        sys.path.insert(0, os.path.join(HERE, "oracle"))
-        import oracle as O
-        fo, mo = O.bk_cut(*work.arrays())
-        out["parity"] = {"flow_equal": fo == flow,
-                         "mask_equal": bool((gc.labels() == mo).all()),
-                         "oracle": "Boykov-Kolmogorov (oracle/gc_oracle.c)"}
-    if not args.no_sweep:
-        sweep = {}
-        for t in (16, 32, 64):
-            g2 = GraphCut(None, None, (W, H), tile=t)
-            g2.set_local_relabel(bool(args.local))
-            g2.set_device_rounds(args.rounds)
-            g2.upload_device(*dev)
-            ts, _, _ = time_cut(g2, max(3, args.steps // 4), 1, sweeps, bfs)
-            sweep[str(t)] = round(W * H / ts / 1e6, 1)
-            g2.close()
-        out["tile_sweep_mpix_s"] = sweep
+    if not args.no_sweep and W == 4096 and H == 4096:
+        out["tiles"] = tile_leg(args, dev, W, H, exp["plain_4096"], sweeps, bfs)
+        parity["tiles"] = {t: v["parity"] for t, v in out["tiles"].items()}
     gc.close()
     del dev
     # (serving runs before the batched leg: after the autotuner's many short
     # runs, the served throughput came out lower in the same process)
     if not args.no_warm:
-        out["warm"] = warm_leg(args)
+        out["warm"], parity["warm_2048"] = warm_leg(args, exp["warm_2048"])
     if not args.no_serve:
-        out["served"] = serve_leg(args)
+        out["served"], parity["served"] = serve_leg(args, exp["served"])
     if not args.no_batch:
-        out["batched"] = batched_leg(args)
+        out["batched"], parity["batched_64"] = batched_leg(args, exp["batched_64"])
     if not args.no_cpu:
         log("CPU baseline")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, args.cpu_threads)
+    parity["all_equal"] = all_equal(parity)
+    parity["expected"] = os.path.relpath(EXPECTED, HERE)
+    out["parity"] = parity
     print(json.dumps(out), flush=True)
     return 0
+
+
+def all_equal(parity):
+    """True if every checked flow and mask equals the oracle's answer."""
+    def ok(x):
+        if isinstance(x, dict):
+            for k, v in x.items():
+                if k.endswith("_equal") and v is not True:
+                    return False
+                if "mismatches" in k and k != "per_variant_mismatches" and v:
+                    return False
+                if isinstance(v, dict) and not ok(v):
+                    return False
+        return True
+    return ok(parity)
 
 
 if __name__ == "__main__":

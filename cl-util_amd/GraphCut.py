@@ -28,7 +28,8 @@ BFS_DEFAULT = 0       # push rounds per global relabel (0: library default, 1)
 SWEEPS_DEFAULT = 0    # in-tile sweeps per tile residency (0: library default, 4)
 
 STAT_NAMES = ("rounds", "residencies", "global_relabels", "bfs_passes",
-              "flood_passes", "host_syncs", "alg_bytes", "tiles")
+              "flood_passes", "host_syncs", "alg_bytes", "tiles", "relabel_tiles",
+              "bfs_round_passes")
 
 
 def _i32(a, shape):

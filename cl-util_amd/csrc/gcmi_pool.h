@@ -46,7 +46,9 @@ enum {
   STAT_SYNCS = 5,
   STAT_ALG_BYTES = 6,
   STAT_TILES = 7,
-  STAT_N = 8
+  STAT_RELABEL_TILES = 8,  // tiles of the relabels' first passes
+  STAT_BFS_ROUND_PASSES = 9,  // tile passes of the later BFS rounds
+  STAT_N = 10
 };
 
 // Kernel classes timed by the optional HIP-event profiler.

@@ -2150,6 +2150,8 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
   p->stats[STAT_GLOBAL_RELABELS] = (int64_t)ht[TALLY_RELABELS];
   p->stats[STAT_BFS_PASSES] = (int64_t)(ht[TALLY_RELABEL_TILES] + ht[TALLY_BFS]);
   p->stats[STAT_FLOOD_PASSES] = (int64_t)ht[TALLY_FLOOD];
+  p->stats[STAT_RELABEL_TILES] = (int64_t)ht[TALLY_RELABEL_TILES];
+  p->stats[STAT_BFS_ROUND_PASSES] = (int64_t)ht[TALLY_BFS];
   // algorithmic bytes: first round 16 per image pixel read + 25 per padded
   // pixel written (state + mask); later residencies 49 (24 read, 25 written);
   // relabel first pass 5 (mask read, h written); later BFS passes 9 (mask +

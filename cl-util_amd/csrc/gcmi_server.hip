@@ -90,6 +90,7 @@ struct Session {
   int32_t max_pair = -1;     // largest pairwise weight of the submitted image
   uint32_t* bits = nullptr;  // pinned packed mask of the last completed request
   int64_t last_done = 0;     // id of the last completed request
+  int submits = 0;           // submissions past their session check, not yet queued
 };
 
 }  // namespace
@@ -351,17 +352,6 @@ void worker_main(gcmi_server* s) {
   }
 }
 
-// Queues r; the caller has set r->t_arrive.
-void enqueue(gcmi_server* s, Request* r, int64_t* id_out) {
-  std::lock_guard<std::mutex> lk(s->mu);
-  r->id = s->next_id++;
-  r->t_queued = now_ns();
-  s->reqs[r->id] = r;
-  s->queue.push_back(r);
-  if (id_out) *id_out = r->id;
-  s->cv_work.notify_one();
-}
-
 // Called with mu held.
 int check_session(gcmi_server* s, int session) {
   if (session < 0 || session >= (int)s->sessions.size() || !s->sessions[session].open ||
@@ -370,6 +360,49 @@ int check_session(gcmi_server* s, int session) {
     return GCMI_EINVAL;
   }
   return GCMI_OK;
+}
+
+// A submission between its session check and its enqueue.  begin() checks
+// the session and counts the submission in the session's `submits` (all
+// under mu); gcmi_server_close waits for that count to reach zero, so the
+// session cannot be closed, removed or reopened while a submission that
+// passed its check is still staging its data.  A submission that fails
+// after begin() drops its count in the destructor.
+struct Submit {
+  gcmi_server* s;
+  int session = -1;
+  explicit Submit(gcmi_server* srv) : s(srv) {}
+  // Called with mu held.
+  int begin(int sess) {
+    GCMI_TRY(check_session(s, sess));
+    session = sess;
+    s->sessions[sess].submits++;
+    return GCMI_OK;
+  }
+  // Called with mu held.
+  void end_locked() {
+    if (session < 0) return;
+    s->sessions[session].submits--;
+    session = -1;
+    s->cv_done.notify_all();
+  }
+  ~Submit() {
+    if (session < 0) return;
+    std::lock_guard<std::mutex> lk(s->mu);
+    end_locked();
+  }
+};
+
+// Queues r and ends its submission; the caller has set r->t_arrive.
+void enqueue(gcmi_server* s, Submit& sub, Request* r, int64_t* id_out) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  r->id = s->next_id++;
+  r->t_queued = now_ns();
+  s->reqs[r->id] = r;
+  s->queue.push_back(r);
+  if (id_out) *id_out = r->id;
+  sub.end_locked();
+  s->cv_work.notify_one();
 }
 
 }  // namespace
@@ -536,7 +569,7 @@ int gcmi_server_close(gcmi_server* s, int session) {
     GCMI_TRY(check_session(s, session));
     s->sessions[session].closing = true;
     s->cv_done.wait(lk, [&] {
-      if (s->sessions[session].busy) return false;
+      if (s->sessions[session].busy || s->sessions[session].submits) return false;
       for (Request* r : s->queue)
         if (r->session == session) return false;
       return true;
@@ -563,9 +596,10 @@ int submit_image_at(gcmi_server* s, int session, const int32_t* h_source, const 
     return GCMI_EINVAL;
   }
   size_t npx;
+  Submit sub(s);
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    GCMI_TRY(check_session(s, session));
+    GCMI_TRY(sub.begin(session));
     npx = (size_t)s->sessions[session].W * s->sessions[session].H;
   }
   int32_t maxw = 0;
@@ -612,7 +646,7 @@ int submit_image_at(gcmi_server* s, int session, const int32_t* h_source, const 
     std::lock_guard<std::mutex> lk(s->mu);
     s->sessions[session].max_pair = maxw;
   }
-  enqueue(s, r, req_out);
+  enqueue(s, sub, r, req_out);
   return GCMI_OK;
 }
 
@@ -628,9 +662,10 @@ int submit_rgb_at(gcmi_server* s, int session, const uint8_t* h_rgb, int channel
   }
   size_t npx;
   gcmi_rgb_model model;
+  Submit sub(s);
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    GCMI_TRY(check_session(s, session));
+    GCMI_TRY(sub.begin(session));
     npx = (size_t)s->sessions[session].W * s->sessions[session].H;
     model = s->rgb_model;
   }
@@ -677,7 +712,7 @@ int submit_rgb_at(gcmi_server* s, int session, const uint8_t* h_rgb, int channel
     std::lock_guard<std::mutex> lk(s->mu);
     s->sessions[session].max_pair = wmax;
   }
-  enqueue(s, r, req_out);
+  enqueue(s, sub, r, req_out);
   return GCMI_OK;
 }
 
@@ -690,9 +725,10 @@ int submit_stroke_at(gcmi_server* s, int session, int n, const int32_t* h_index,
   }
   int32_t wmax;
   long long npx;
+  Submit sub(s);
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    GCMI_TRY(check_session(s, session));
+    GCMI_TRY(sub.begin(session));
     wmax = s->sessions[session].max_pair;
     npx = (long long)s->sessions[session].W * s->sessions[session].H;
   }
@@ -719,7 +755,7 @@ int submit_stroke_at(gcmi_server* s, int session, int n, const int32_t* h_index,
     memcpy(r->stroke.data() + n, h_source, (size_t)n * 4);
     memcpy(r->stroke.data() + 2 * n, h_sink, (size_t)n * 4);
   }
-  enqueue(s, r, req_out);
+  enqueue(s, sub, r, req_out);
   return GCMI_OK;
 }
 
@@ -731,9 +767,10 @@ int submit_polyline_at(gcmi_server* s, int session, int npts, const double* h_xy
   }
   int32_t wmax;
   int W, H;
+  Submit sub(s);
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    GCMI_TRY(check_session(s, session));
+    GCMI_TRY(sub.begin(session));
     wmax = s->sessions[session].max_pair;
     W = s->sessions[session].W;
     H = s->sessions[session].H;
@@ -757,7 +794,7 @@ int submit_polyline_at(gcmi_server* s, int session, int npts, const double* h_xy
   r->width = width;
   r->pcs = source;
   r->pct = sink;
-  enqueue(s, r, req_out);
+  enqueue(s, sub, r, req_out);
   return GCMI_OK;
 }
 

@@ -8,9 +8,10 @@ The mix: `nsess` sessions with square sizes drawn from SIZES with weights
 process; each picks a session uniformly; 10% are new images (the session's
 next base image, cut from scratch; sent as its 8-bit colour image, whose
 capacities the server builds on the device, or as the four int32 capacity
-arrays) and 90% are brush strokes: polylines
-from a seeded bank per size (3-8 segments, width 8-32, alternating
-foreground/background hard constraints), rasterised on the device.
+arrays) and 90% are brush strokes: polylines drawn afresh for every request
+from the trace's seeded generator (3-8 segments with vertices uniform in the
+image, width 8-32, foreground and background hard constraints alternating
+per session), rasterised on the device.
 
 Latency of a request = completion (mask in host memory) - scheduled
 arrival, so any lag of the load generator counts against the server.
@@ -28,15 +29,15 @@ BASES = {512: 4, 1024: 3, 2048: 2, 4096: 2}
 
 
 class Mix:
-    def __init__(self, nsess=256, seed=4, bank=32, sizes=SIZES, weights=WEIGHTS, bases=BASES):
+    def __init__(self, nsess=256, seed=4, sizes=SIZES, weights=WEIGHTS, bases=BASES):
         rng = np.random.default_rng(seed)
         w = np.asarray(weights, float)
         self.sizes = [int(x) for x in rng.choice(sizes, size=nsess, p=w / w.sum())]
         self.bases = {n: [G.three_ellipses(n, n, seed=4000 + 10 * n + k) for k in range(bases[n])]
                       for n in sizes if n in self.sizes}
-        self.bank = {n: G.strokes(n, n, count=bank, seed=5000 + n, raster=False)
-                     for n in self.bases}
         self.nsess = nsess
+        # strokes painted per session so far: the next one is foreground if even
+        self.painted = np.zeros(nsess, np.int64)
 
     @property
     def pixels(self):
@@ -64,7 +65,10 @@ def open_sessions(srv, mix, rgb=False):
 
 
 def make_trace(mix, rate, duration, seed, t0_ns, image_frac=0.1, rgb=False):
-    """(strokes, images): lists of (t_ns, session index, kind, payload, size)."""
+    """(strokes, images): lists of (t_ns, session index, kind, payload, size,
+    source) where source is the Workload of a new image or the Stroke.
+    Every stroke is drawn afresh (gcmi_workloads.random_stroke) and
+    alternates foreground / background per session (mix.painted)."""
     rng = np.random.default_rng(seed)
     n = max(1, int(rate * duration))
     ts = t0_ns + np.cumsum(rng.exponential(1e9 / rate, size=n)).astype(np.int64)
@@ -75,27 +79,27 @@ def make_trace(mix, rate, duration, seed, t0_ns, image_frac=0.1, rgb=False):
         size = mix.sizes[i]
         if rng.random() < image_frac:
             nimg[i] += 1
-            kind, payload = image_request(mix.base(i, int(nimg[i])), rgb)
-            images.append((int(ts[k]), i, kind, payload, size))
+            work = mix.base(i, int(nimg[i]))
+            kind, payload = image_request(work, rgb)
+            images.append((int(ts[k]), i, kind, payload, size, work))
         else:
-            st = mix.bank[size][int(rng.integers(len(mix.bank[size])))]
+            st = G.random_stroke(rng, size, size, mix.painted[i] % 2 == 0)
+            mix.painted[i] += 1
             s, t = st.capacities()
-            strokes.append((int(ts[k]), i, "polyline", (st.points, st.width, s, t), size))
+            strokes.append((int(ts[k]), i, "polyline", (st.points, st.width, s, t), size, st))
     return strokes, images
 
 
-def run_trace(srv, sids, mix, rate, duration=2.0, seed=11, image_frac=0.1, lead=0.2, rgb=False):
-    """Replays a Poisson trace (strokes and new images from two generator
-    threads, so an image's host-side staging does not delay the strokes
-    behind it) and returns latency statistics in ms."""
-    t0 = time.monotonic_ns() + int(lead * 1e9)
-    strokes, images = make_trace(mix, rate, duration, seed, t0, image_frac, rgb)
+def replay_trace(srv, sids, strokes, images):
+    """Submits a trace's strokes and new images from two generator threads
+    (so an image's host-side staging does not delay the strokes behind it);
+    returns {"s": stroke request ids, "i": image request ids}."""
     ids = {}
     errors = []
 
     def go(name, tr):
         try:
-            ids[name] = srv.replay([(t, sids[i], k, p) for (t, i, k, p, _) in tr]) \
+            ids[name] = srv.replay([(x[0], sids[x[1]], x[2], x[3]) for x in tr]) \
                 if tr else np.zeros(0, np.int64)
         except Exception as e:  # reported by the caller
             errors.append(e)
@@ -108,6 +112,15 @@ def run_trace(srv, sids, mix, rate, duration=2.0, seed=11, image_frac=0.1, lead=
         x.join()
     if errors:
         raise errors[0]
+    return ids
+
+
+def run_trace(srv, sids, mix, rate, duration=2.0, seed=11, image_frac=0.1, lead=0.2, rgb=False):
+    """Replays a Poisson trace (replay_trace) and returns latency statistics
+    in ms."""
+    t0 = time.monotonic_ns() + int(lead * 1e9)
+    strokes, images = make_trace(mix, rate, duration, seed, t0, image_frac, rgb)
+    ids = replay_trace(srv, sids, strokes, images)
     lat = {"stroke": [], "image": []}
     by_size = {}
     batch = []
@@ -150,3 +163,72 @@ def sustained(points, p99_ms=20.0, size=2048):
             if best is None or p["rate"] > best["rate"]:
                 best = p
     return best
+
+
+def find_sustained(probe, lo, hi, tol=0.02, p99_ms=20.0, size=2048, expand=3):
+    """Bisects the offered rate for the highest one that passes `sustained`'s
+    rule: probe(rate) -> run_trace statistics.  Starts from [lo, hi]: lo is
+    halved while it fails and hi doubled while it passes (at most `expand`
+    times each), then the bracket is halved until (hi - lo) <= tol * lo.
+    Returns (points probed, the best passing point or None)."""
+    points = []
+
+    def ok(rate):
+        pt = probe(rate)
+        points.append(pt)
+        return sustained([pt], p99_ms, size) is not None
+
+    lo_ok = ok(lo)
+    tries = 0
+    while not lo_ok and tries < expand:
+        hi, lo = lo, lo / 2.0
+        lo_ok = ok(lo)
+        tries += 1
+    if not lo_ok:
+        return points, None
+    hi_fails = tries > 0  # hi was probed and failed
+    tries = 0
+    while not hi_fails and tries < expand:
+        if ok(hi):
+            lo, hi = hi, hi * 2.0
+            tries += 1
+        else:
+            hi_fails = True
+    if not hi_fails:
+        return points, sustained(points, p99_ms, size)
+    while hi - lo > tol * lo:
+        mid = round((lo + hi) / 2.0)
+        if ok(mid):
+            lo = mid
+        else:
+            hi = mid
+    return points, sustained(points, p99_ms, size)
+
+
+# The fixed trace whose every answer the benchmark checks: each session
+# first gets its base image again (a known state), then this Poisson trace is
+# submitted from one thread in time order, so each session's requests are
+# served in trace order.
+VERIFY = dict(rate=20000, duration=0.075, seed=99, image_frac=0.1)
+
+
+def verification_trace(mix, rgb=True):
+    """The verification trace's requests in time order (t_ns from 0), with
+    the per-session stroke alternation restarted."""
+    mix.painted[:] = 0
+    strokes, images = make_trace(mix, VERIFY["rate"], VERIFY["duration"], VERIFY["seed"], 0,
+                                 VERIFY["image_frac"], rgb)
+    return sorted(strokes + images, key=lambda x: x[0])
+
+
+def mask_sha(words):
+    """First 16 hex digits of the sha256 of a packed mask (uint32 words,
+    little-endian)."""
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(words, "<u4").tobytes()).hexdigest()[:16]
+
+
+def pack_mask(mask):
+    """uint8 [H][W] labels -> packed uint32 words (bit p % 32 of word p // 32)."""
+    b = np.packbits(np.asarray(mask, np.uint8).reshape(-1), bitorder="little")
+    return np.pad(b, (0, (-b.size) % 4)).view(np.uint32)

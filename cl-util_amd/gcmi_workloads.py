@@ -199,46 +199,109 @@ class Stroke:
         return (np.zeros(n, np.int32), np.full(n, HARD, np.int32))
 
 
-def rasterise_polyline(W, H, pts, width):
+def _capsule(xs, ys, x0, y0, dx, dy, L2, r):
+    """The capsule test of one segment at pixel centres (xs, ys), float64,
+    each operation rounded once in this order."""
+    if L2 > 0:
+        t = np.clip(((xs - x0) * dx + (ys - y0) * dy) / L2, 0.0, 1.0)
+    else:
+        t = np.zeros(np.broadcast(xs, ys).shape)
+    px, py = x0 + t * dx, y0 + t * dy
+    return (xs - px) ** 2 + (ys - py) ** 2 <= r * r
+
+
+def _segment_box(W, H, x0, y0, x1, y1, r):
+    xa = max(int(np.floor(min(x0, x1) - r)), 0)
+    xb = min(int(np.ceil(max(x0, x1) + r)), W - 1)
+    ya = max(int(np.floor(min(y0, y1) - r)), 0)
+    yb = min(int(np.ceil(max(y0, y1) + r)), H - 1)
+    return xa, xb, ya, yb
+
+
+def rasterise_polyline_box(W, H, pts, width):
     """Flat indices of pixels whose centre lies within width/2 of the
-    polyline (a union of capsules)."""
+    polyline (a union of capsules): every pixel of each segment's bounding
+    box tested.  The plain statement that rasterise_polyline must equal."""
     r = width / 2.0
     hit = np.zeros((H, W), dtype=bool)
     for (x0, y0), (x1, y1) in zip(pts[:-1], pts[1:]):
-        xa = max(int(np.floor(min(x0, x1) - r)), 0)
-        xb = min(int(np.ceil(max(x0, x1) + r)), W - 1)
-        ya = max(int(np.floor(min(y0, y1) - r)), 0)
-        yb = min(int(np.ceil(max(y0, y1) + r)), H - 1)
+        xa, xb, ya, yb = _segment_box(W, H, x0, y0, x1, y1, r)
         if xa > xb or ya > yb:
             continue
         ys = np.arange(ya, yb + 1, dtype=np.float64)[:, None]
         xs = np.arange(xa, xb + 1, dtype=np.float64)[None, :]
         dx, dy = x1 - x0, y1 - y0
-        L2 = dx * dx + dy * dy
-        if L2 > 0:
-            t = np.clip(((xs - x0) * dx + (ys - y0) * dy) / L2, 0.0, 1.0)
-        else:
-            t = np.zeros((ys.size, xs.size))
-        px, py = x0 + t * dx, y0 + t * dy
-        hit[ya:yb + 1, xa:xb + 1] |= (xs - px) ** 2 + (ys - py) ** 2 <= r * r
+        hit[ya:yb + 1, xa:xb + 1] |= _capsule(xs, ys, x0, y0, dx, dy, dx * dx + dy * dy, r)
     return np.flatnonzero(hit).astype(np.int32)
 
 
+def rasterise_polyline(W, H, pts, width):
+    """The pixel set of rasterise_polyline_box, testing only a band around
+    each segment: in row y the capsule lies within r of the segment points
+    whose y is within r of y, so its columns lie within r of those points'
+    x range (computed with one pixel of margin).  The same test on a
+    superset of the hit pixels gives the same set."""
+    r = width / 2.0
+    hit = np.zeros((H, W), dtype=bool)
+    for (x0, y0), (x1, y1) in zip(pts[:-1], pts[1:]):
+        xa, xb, ya, yb = _segment_box(W, H, x0, y0, x1, y1, r)
+        if xa > xb or ya > yb:
+            continue
+        dx, dy = x1 - x0, y1 - y0
+        rows = np.arange(ya, yb + 1, dtype=np.float64)
+        m = r + 1.0
+        if abs(dy) > 1e-9:
+            ta = (rows - m - y0) / dy
+            tb = (rows + m - y0) / dy
+            tlo = np.clip(np.minimum(ta, tb), 0.0, 1.0)
+            thi = np.clip(np.maximum(ta, tb), 0.0, 1.0)
+            xl = x0 + np.minimum(tlo * dx, thi * dx)
+            xh = x0 + np.maximum(tlo * dx, thi * dx)
+            lo = np.maximum(np.floor(xl - m), xa).astype(np.int64)
+            hi = np.minimum(np.ceil(xh + m), xb).astype(np.int64)
+        else:
+            lo = np.full(rows.size, xa, np.int64)
+            hi = np.full(rows.size, xb, np.int64)
+        keep = hi >= lo
+        if not keep.any():
+            continue
+        rows, lo, hi = rows[keep], lo[keep], hi[keep]
+        span = int((hi - lo).max()) + 1
+        cols = lo[:, None] + np.arange(span)[None, :]
+        inside = cols <= hi[:, None]
+        xs = cols.astype(np.float64)
+        ys = rows[:, None]
+        h = _capsule(xs, ys, x0, y0, dx, dy, dx * dx + dy * dy, r) & inside
+        yy, kk = np.nonzero(h)
+        hit[rows[yy].astype(np.int64), cols[yy, kk]] = True
+    return np.flatnonzero(hit).astype(np.int32)
+
+
+def random_stroke(rng, W, H, foreground, raster=False):
+    """One stroke drawn from `rng`: a polyline of 3-8 segments with vertices
+    uniform in the image and width 8-32 px.  With raster=False the pixel set
+    is not computed (index is None)."""
+    nseg = int(rng.integers(3, 9))
+    pts = list(zip(rng.uniform(0, W - 1, nseg + 1),
+                   rng.uniform(0, H - 1, nseg + 1)))
+    width = int(rng.integers(8, 33))
+    idx = rasterise_polyline(W, H, pts, width) if raster else None
+    return Stroke(idx, foreground, np.array(pts, dtype=np.float64), width)
+
+
 def strokes(W, H, count=100, seed=3, first_foreground=True, raster=True):
-    """`count` seeded strokes: polylines of 3-8 segments with vertices uniform
-    in the image, width 8-32 px, alternating foreground/background.  With
-    raster=False the pixel sets are not computed (index is None)."""
+    """`count` seeded strokes (random_stroke), alternating
+    foreground/background."""
     rng = np.random.default_rng(seed + 7919)
-    out = []
-    for i in range(count):
-        nseg = int(rng.integers(3, 9))
-        pts = list(zip(rng.uniform(0, W - 1, nseg + 1),
-                       rng.uniform(0, H - 1, nseg + 1)))
-        width = int(rng.integers(8, 33))
-        fg = (i % 2 == 0) == first_foreground
-        idx = rasterise_polyline(W, H, pts, width) if raster else None
-        out.append(Stroke(idx, fg, np.array(pts, dtype=np.float64), width))
-    return out
+    return [random_stroke(rng, W, H, (i % 2 == 0) == first_foreground, raster)
+            for i in range(count)]
+
+
+def rasterise(stroke, W, H):
+    """The stroke with its pixel set computed."""
+    if stroke.index is None:
+        stroke.index = rasterise_polyline(W, H, stroke.points, stroke.width)
+    return stroke
 
 
 def apply_stroke(work, stroke):

@@ -215,10 +215,14 @@ int gcmi_cut_labels_packed(gcmi_cut *cut, uint32_t *h_bits);
  * capacities (every edit included) instead of reflowing warm.  (The
  * reference's GraphCut.reset, GraphCut.pyx:420-421, does nothing.) */
 int gcmi_cut_forget(gcmi_cut *cut);
-/* Counters of the last run:
+/* Counters of the last run (the first n of them):
  *   [0] push rounds  [1] tile residencies  [2] global relabels
  *   [3] BFS tile passes  [4] flood tile passes  [5] host syncs
- *   [6] algorithmic bytes (see DESIGN)  [7] tiles  */
+ *   [6] algorithmic bytes: 16 per image pixel read + 25 per padded pixel
+ *       written by the first push round, 49 per pixel of a later residency,
+ *       5 per pixel of a relabel's first pass, 9 of a later BFS pass, 2 of
+ *       a flood pass, 2 per image pixel of the labels
+ *   [7] tiles  [8] relabel first-pass tiles  [9] later BFS tile passes  */
 int gcmi_cut_stats(gcmi_cut *cut, int64_t *out, int n);
 /* Device-resident tile-blocked height image (int32, padded) for inspection:
  * copies padded_w*padded_h ints in row-major order into h_out. */

@@ -90,6 +90,31 @@ def test_stroke_polyline_rasteriser_cases():
     assert ((xs - 4) ** 2 + (ys - 4) ** 2 <= 4).all() and idx.size == 13
     # outside the image
     assert G.rasterise_polyline(9, 9, [(-30.0, -30.0), (-10.0, -20.0)], 8).size == 0
+    # the same cases through the bounding-box statement
+    assert list(G.rasterise_polyline_box(6, 6, [(0.0, 0.0), (5.0, 5.0)], 0)) == \
+        [0, 7, 14, 21, 28, 35]
+    assert G.rasterise_polyline_box(9, 9, [(4.0, 4.0), (4.0, 4.0)], 4).size == 13
+
+
+def test_band_rasteriser_equals_box_rasteriser():
+    """The band rasteriser (tests only pixels near each segment) paints the
+    same pixels as the bounding-box statement, on random, axis-parallel,
+    integer-vertex and partly outside polylines."""
+    import gcmi_workloads as G
+    rng = np.random.default_rng(5)
+    for k in range(300):
+        W, H = int(rng.integers(1, 200)), int(rng.integers(1, 200))
+        n = int(rng.integers(1, 6))
+        pts = list(zip(rng.uniform(-40, W + 40, n), rng.uniform(-40, H + 40, n)))
+        if k % 7 == 0:
+            pts = [(x, pts[0][1]) for x, _ in pts]
+        if k % 11 == 0:
+            pts = [(pts[0][0], y) for _, y in pts]
+        if k % 13 == 0:
+            pts = [(float(int(x)), float(int(y))) for x, y in pts]
+        width = float(rng.choice([0, 1, 2, 3.5, 8, 17, 32]))
+        assert np.array_equal(G.rasterise_polyline(W, H, pts, width),
+                              G.rasterise_polyline_box(W, H, pts, width)), (k, W, H, pts, width)
 
 
 PRIM_BENCH = os.path.join(REPO, "cl-util_amd", "gcmi_prim_bench")

@@ -195,3 +195,52 @@ def test_request_errors():
         # an empty stroke is a valid request (re-solve)
         info = srv.wait(srv.submit_stroke(sid, [], 0, 0), timeout=60)
         assert info.flow == O.bk_cut(*w.arrays())[0]
+
+
+def test_submit_and_close_interleaved():
+    """Client threads keep submitting strokes to a session while another
+    thread closes it and reopens the slot at another size.  A submission
+    either fails cleanly (EINVAL: no open session) or is queued before the
+    close and served; close waits for every submission that passed its
+    session check.  The reopened session then answers as the oracle."""
+    W, H = 96, 80
+    with GraphCutServer(tile=32, hbm_budget=BUDGET, max_sessions=2, max_batch=8) as srv:
+        w = G.centred(W, H, seed=31)
+        sid = srv.open(W, H)
+        srv.wait(srv.submit_image(sid, *w.arrays()), timeout=60)
+        st = G.strokes(W, H, count=8, seed=32)
+        stop = threading.Event()
+        ok, refused, other = [], [], []
+
+        def client(k):
+            j = 0
+            while not stop.is_set():
+                s = st[(k + j) % len(st)]
+                a, b = s.capacities()
+                try:
+                    ok.append(srv.submit_polyline(sid, s.points, s.width, a, b))
+                except L.GcmiError as e:
+                    (refused if e.code == L.EINVAL else other).append(e)
+                    return
+                j += 1
+
+        threads = [threading.Thread(target=client, args=(k,)) for k in range(3)]
+        for t in threads:
+            t.start()
+        import time
+        time.sleep(0.05)
+        srv.close(sid)
+        stop.set()
+        for t in threads:
+            t.join(timeout=60)
+        assert not other, other
+        assert ok  # some were queued before the close
+        for rid in ok:
+            assert srv.wait(rid, timeout=60).status == DONE
+        # the slot reopens at another size and answers as the oracle
+        w2 = G.three_ellipses(130, 70, seed=33)
+        sid2 = srv.open(130, 70)
+        rid = srv.submit_image(sid2, *w2.arrays())
+        f, m = O.bk_cut(*w2.arrays())
+        assert srv.wait(rid, timeout=60).flow == f
+        assert (srv.mask(sid2)[0] == m).all()
