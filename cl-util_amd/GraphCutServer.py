@@ -29,7 +29,7 @@ from gcmi_lib import (EBUSY, OPT_DEVICE_ROUNDS, OPT_LOCAL_RELABEL, REQ_IMAGE, RE
 QUEUED, DONE, FAILED = 0, 1, 2
 MODES = {"batched": 0, "plain": 1, "cold": 2}
 STAT_NAMES = ("solves", "served", "queued", "sessions", "tiles_used", "tiles_capacity",
-              "device_bytes", "ns_apply", "ns_solve", "ns_out")
+              "device_bytes", "ns_apply", "ns_solve", "ns_out", "lanes")
 
 
 class RequestInfo:
@@ -52,14 +52,18 @@ class RequestInfo:
 
 class GraphCutServer:
     def __init__(self, tile=32, hbm_budget=200 * 10**9, max_sessions=256, max_batch=64,
-                 mode="batched", sweeps=0, bfs=0, local_relabel=False, device_rounds=0):
+                 mode="batched", sweeps=0, bfs=0, local_relabel=False, device_rounds=0,
+                 lanes=0):
+        """lanes: pools solved concurrently, each with its share of the
+        sessions (0: 3 for "batched" and "cold", 1 for "plain")."""
         self.tile = int(tile[0] if isinstance(tile, (tuple, list)) else tile)
         if mode not in MODES:
             raise ValueError("mode must be one of %s" % sorted(MODES))
         self.mode = mode
         h = ctypes.c_void_p()
-        check(lib().gcmi_server_create(self.tile, int(hbm_budget), int(max_sessions),
-                                       int(max_batch), MODES[mode], ctypes.byref(h)))
+        check(lib().gcmi_server_create_lanes(self.tile, int(hbm_budget), int(max_sessions),
+                                             int(max_batch), MODES[mode], int(lanes),
+                                             ctypes.byref(h)))
         self._h = h.value
         self.dims = {}
         if sweeps or bfs:

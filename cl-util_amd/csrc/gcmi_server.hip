@@ -91,6 +91,7 @@ struct Session {
   uint32_t* bits = nullptr;  // pinned packed mask of the last completed request
   int64_t last_done = 0;     // id of the last completed request
   int submits = 0;           // submissions past their session check, not yet queued
+  int lane = 0;              // the lane (pool) holding the session
 };
 
 }  // namespace
@@ -107,12 +108,25 @@ struct MaskCopy {
 struct Pending {
   std::vector<Request*> batch;
   std::vector<MaskCopy> copies;
-  hipEvent_t solved = nullptr;  // recorded on the pool's stream after the solve
+  hipEvent_t solved = nullptr;  // recorded on the lane's stream after the solve
   hipEvent_t done = nullptr;    // recorded on out_stream after the last copy
 };
 
-struct gcmi_server {
+// A lane: one pool of sessions with its own stream and worker thread.
+// Lanes solve their batches concurrently, so one batch's chain of small
+// dependent launches overlaps another's on the device.  A session lives in
+// one lane for its whole life.
+constexpr int kMaxLanes = 4;
+struct Lane {
   Pool pool;
+  std::mutex pool_mu;  // held by the lane's worker while it serves a batch
+  std::thread worker;
+  int64_t ns_apply = 0, ns_solve = 0, ns_out = 0;  // worker time per phase (under mu)
+};
+
+struct gcmi_server {
+  Lane lanes[kMaxLanes];
+  int nlanes = 1;
   int device = 0;
   int mode = 0;  // GCMI_SERVE_WARM, GCMI_SERVE_PLAIN or GCMI_SERVE_COLD
   int max_batch = 64;
@@ -121,41 +135,43 @@ struct gcmi_server {
   std::vector<Session> sessions;
   int32_t* d_stage = nullptr;  // device staging for new images
   RangeAlloc stage_alloc;
+  size_t stage_bytes = 0;
   hipStream_t copy_stream = nullptr;
   // Masks are copied out on out_stream, after the solve that produced them:
-  // the finisher thread issues a batch's copies and completes its requests
-  // when they are done, while the worker already runs the next batch.
+  // the finisher thread issues the copies of every solved batch and
+  // completes each batch's requests when its copies are done, while the
+  // lanes already run their next batches.
   hipStream_t out_stream = nullptr;
   std::deque<Pending> outq;
   std::vector<hipEvent_t> free_events;
   bool stop_out = false;
   std::thread finisher;
-  // Lock order: pool_mu, then copy_mu, then mu.  The worker holds pool_mu
-  // (not mu) while it serves a batch; open/close hold pool_mu while they
-  // change the pool; submitters hold copy_mu while they stage an image.
-  std::mutex pool_mu, copy_mu, mu;
+  // Lock order: the lanes' pool_mu (in lane order), then copy_mu, then mu.
+  // A lane's worker holds its pool_mu (not mu) while it serves a batch;
+  // open/close hold pool_mu while they change a pool; submitters hold
+  // copy_mu while they stage an image.
+  std::mutex copy_mu, mu;
   std::condition_variable cv_work, cv_done, cv_out;
   std::deque<Request*> queue;
   std::unordered_map<int64_t, Request*> reqs;
   int64_t next_id = 1;
   bool stop = false;
-  std::thread worker;
   int64_t iterations = 0, solved_requests = 0;
-  int64_t ns_apply = 0, ns_solve = 0, ns_out = 0;  // worker time per phase
   char err[256] = "";
 };
 
 namespace {
 
-// Applies the batch's requests to the pool, solves, records `solved` on the
-// pool's stream after the solve and lists the masks' copies for the
-// finisher.  Runs on the worker thread with pool_mu held.  The batch's
-// sessions stay busy until the finisher has completed them, so no later
-// solve rewrites their labels before the copies have read them.
-int serve_batch(gcmi_server* s, std::vector<Request*>& batch, hipEvent_t solved,
-                std::vector<MaskCopy>& copies) {
+// Applies the batch's requests to the lane's pool, solves, records `solved`
+// on the lane's stream after the solve and lists the masks' copies for the
+// finisher.  Runs on the lane's worker thread with its pool_mu held.  The
+// batch's sessions stay busy until the finisher has completed them, so no
+// later solve rewrites their labels before the copies have read them.
+// ns[0..2] receive the time spent applying, solving and listing.
+int serve_batch(gcmi_server* s, Lane& L, std::vector<Request*>& batch, hipEvent_t solved,
+                std::vector<MaskCopy>& copies, int64_t* ns) {
   const int64_t t0 = now_ns();
-  Pool* p = &s->pool;
+  Pool* p = &L.pool;
   std::vector<int> imgs;
   imgs.reserve(batch.size());
   std::vector<EditReq> edits;
@@ -212,9 +228,9 @@ int serve_batch(gcmi_server* s, std::vector<Request*>& batch, hipEvent_t solved,
     copies.push_back(MaskCopy{ss.bits, packed_mask(p, ss.img), (size_t)words * 4});
   }
   for (Request* r : batch) r->flow = p->imgs[s->sessions[r->session].img].flow;
-  s->ns_apply += t1 - t0;
-  s->ns_solve += t2 - t1;
-  s->ns_out += now_ns() - t2;
+  ns[0] = t1 - t0;
+  ns[1] = t2 - t1;
+  ns[2] = now_ns() - t2;
   return GCMI_OK;
 }
 
@@ -236,60 +252,82 @@ void complete(gcmi_server* s, std::vector<Request*>& batch, bool ok, int64_t t) 
   s->cv_work.notify_all();
 }
 
+// Takes every solved batch queued so far, issues all their mask copies on
+// out_stream (each batch behind its solve's event), then completes the
+// batches in order as their copies finish.
 void finisher_main(gcmi_server* s) {
   (void)hipSetDevice(s->device);
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
     s->cv_out.wait(lk, [&] { return s->stop_out || !s->outq.empty(); });
     if (s->outq.empty()) break;  // stopping, and nothing left to finish
-    Pending pd = std::move(s->outq.front());
-    s->outq.pop_front();
-    lk.unlock();
-    hipError_t e = hipStreamWaitEvent(s->out_stream, pd.solved, 0);
-    for (const MaskCopy& c : pd.copies)
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, s->out_stream);
-    if (e == hipSuccess) e = hipEventRecord(pd.done, s->out_stream);
-    if (e == hipSuccess) e = hipEventSynchronize(pd.done);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      (void)hipStreamSynchronize(s->out_stream);
+    std::vector<Pending> work;
+    while (!s->outq.empty()) {
+      work.push_back(std::move(s->outq.front()));
+      s->outq.pop_front();
     }
-    const int64_t t = now_ns();
+    lk.unlock();
+    std::vector<hipError_t> errs(work.size(), hipSuccess);
+    for (size_t k = 0; k < work.size(); ++k) {
+      const Pending& pd = work[k];
+      hipError_t e = hipStreamWaitEvent(s->out_stream, pd.solved, 0);
+      for (const MaskCopy& c : pd.copies)
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, s->out_stream);
+      if (e == hipSuccess) e = hipEventRecord(pd.done, s->out_stream);
+      errs[k] = e;
+    }
+    for (size_t k = 0; k < work.size(); ++k) {
+      Pending& pd = work[k];
+      hipError_t e = errs[k];
+      if (e == hipSuccess) e = hipEventSynchronize(pd.done);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(s->out_stream);
+      }
+      const int64_t t = now_ns();
+      lk.lock();
+      if (e != hipSuccess)
+        snprintf(s->err, sizeof s->err, "mask copy failed: %s", hipGetErrorString(e));
+      s->free_events.push_back(pd.solved);
+      s->free_events.push_back(pd.done);
+      complete(s, pd.batch, e == hipSuccess, t);
+      lk.unlock();
+    }
     lk.lock();
-    if (e != hipSuccess)
-      snprintf(s->err, sizeof s->err, "mask copy failed: %s", hipGetErrorString(e));
-    s->free_events.push_back(pd.solved);
-    s->free_events.push_back(pd.done);
-    complete(s, pd.batch, e == hipSuccess, t);
   }
 }
 
-// Whether some queued request's session is free (called with mu held).
-bool work_ready(const gcmi_server* s) {
-  for (const Request* r : s->queue)
-    if (!s->sessions[r->session].busy) return true;
+// Whether some queued request of the lane has a free session (called with
+// mu held).
+bool work_ready(const gcmi_server* s, int lane) {
+  for (const Request* r : s->queue) {
+    const Session& ss = s->sessions[r->session];
+    if (ss.lane == lane && !ss.busy) return true;
+  }
   return false;
 }
 
-void worker_main(gcmi_server* s) {
+void worker_main(gcmi_server* s, int lane) {
   (void)hipSetDevice(s->device);
+  Lane& L = s->lanes[lane];
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
-    s->cv_work.wait(lk, [&] { return s->stop || work_ready(s); });
+    s->cv_work.wait(lk, [&] { return s->stop || work_ready(s, lane); });
     if (s->stop) break;
-    // oldest first, one request per session, at most max_batch.  In the
-    // warm mode a batch holds either new images (cut from scratch) or
-    // strokes (reflowed warm), the class of the oldest request it can take,
-    // so strokes are not held to the longer solve of a new image; a session
-    // whose next request is of the other class waits for a later batch.
+    // oldest first, one request per session, at most max_batch, of this
+    // lane's sessions.  In the warm mode a batch holds either new images
+    // (cut from scratch) or strokes (reflowed warm), the class of the
+    // oldest request it can take, so strokes are not held to the longer
+    // solve of a new image; a session whose next request is of the other
+    // class waits for a later batch.
     const int cap = s->mode == GCMI_SERVE_PLAIN ? 1 : s->max_batch;
     std::vector<Request*> batch;
     int cls = -1;  // 1: new images, 0: strokes
     std::vector<char> held;
     for (auto it = s->queue.begin(); it != s->queue.end() && (int)batch.size() < cap;) {
       Session& ss = s->sessions[(*it)->session];
-      if (ss.busy || (!held.empty() && held[(*it)->session])) {
+      if (ss.lane != lane || ss.busy || (!held.empty() && held[(*it)->session])) {
         ++it;
         continue;
       }
@@ -327,12 +365,16 @@ void worker_main(gcmi_server* s) {
         snprintf(err, sizeof err, "event creation failed");
       }
     std::vector<MaskCopy> copies;
+    int64_t ns[3] = {0, 0, 0};
     if (rc == GCMI_OK) {
-      std::lock_guard<std::mutex> plk(s->pool_mu);
-      rc = serve_batch(s, batch, ev[0], copies);
+      std::lock_guard<std::mutex> plk(L.pool_mu);
+      rc = serve_batch(s, L, batch, ev[0], copies, ns);
       if (rc != GCMI_OK) snprintf(err, sizeof err, "%s", gcmi_last_error());
     }
     lk.lock();
+    L.ns_apply += ns[0];
+    L.ns_solve += ns[1];
+    L.ns_out += ns[2];
     // the images' staging ranges were read by the solve, which has finished
     for (Request* r : batch)
       if (r->stage >= 0) {
@@ -402,16 +444,24 @@ void enqueue(gcmi_server* s, Submit& sub, Request* r, int64_t* id_out) {
   s->queue.push_back(r);
   if (id_out) *id_out = r->id;
   sub.end_locked();
-  s->cv_work.notify_one();
+  s->cv_work.notify_all();  // (each lane's worker waits for its own sessions)
 }
+
+// Locks every lane's pool_mu, in lane order.
+struct AllPools {
+  std::unique_lock<std::mutex> lk[kMaxLanes];
+  explicit AllPools(gcmi_server* s) {
+    for (int i = 0; i < s->nlanes; ++i) lk[i] = std::unique_lock<std::mutex>(s->lanes[i].pool_mu);
+  }
+};
 
 }  // namespace
 
 extern "C" {
 
-int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_batch, int mode,
-                       gcmi_server** out) {
-  if (!out || max_sessions < 1 || max_batch < 1 ||
+int gcmi_server_create_lanes(int tile, size_t hbm_budget, int max_sessions, int max_batch,
+                             int mode, int lanes, gcmi_server** out) {
+  if (!out || max_sessions < 1 || max_batch < 1 || lanes < 0 || lanes > kMaxLanes ||
       (mode != GCMI_SERVE_WARM && mode != GCMI_SERVE_PLAIN && mode != GCMI_SERVE_COLD)) {
     set_error("bad server arguments");
     return GCMI_EINVAL;
@@ -421,9 +471,15 @@ int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_ba
     set_error("tile must be 16, 32 or 64 (got %d)", tile);
     return GCMI_EINVAL;
   }
+  // plain mode serves one request at a time: one lane.  Otherwise three by
+  // default: with the staging and mask-copy streams that is five streams,
+  // and more lanes measured slower (the device's hardware queues are shared
+  // between more streams).
+  if (lanes == 0) lanes = mode == GCMI_SERVE_PLAIN ? 1 : 3;
+  if (mode == GCMI_SERVE_PLAIN) lanes = 1;
   GCMI_TRY(require_device());
-  // a quarter of the budget (at most 1 GiB) stages new images; the pool
-  // (its packed masks included) gets the rest
+  // a quarter of the budget (at most 1 GiB) stages new images; the lanes'
+  // pools (their packed masks included) share the rest equally
   const size_t stage_bytes = std::min<size_t>(hbm_budget / 4, size_t(1) << 30) & ~size_t(255);
   const long long T2 = (long long)tile * tile;
   const int probe = 1 << 12;
@@ -431,7 +487,7 @@ int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_ba
   const size_t b2 = pool_bytes(tile, 2 * probe, 2 * probe * T2, max_sessions);
   const double per_tile = (double)(b2 - b1) / probe;
   const double fixed = (double)b1 - probe * per_tile + 4.0 * max_sessions;
-  const double rest = (double)hbm_budget - (double)stage_bytes - fixed;
+  const double rest = ((double)hbm_budget - (double)stage_bytes) / lanes - fixed;
   long long cap_tiles = rest > 0 ? (long long)(rest / per_tile) : 0;
   cap_tiles = std::min<long long>(cap_tiles, ((1LL << 30) - (1LL << 20)) / T2);
   if (cap_tiles < 1 || stage_bytes < 1024) {
@@ -445,10 +501,14 @@ int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_ba
   }
   s->mode = mode;
   s->max_batch = max_batch;
+  s->nlanes = lanes;
+  s->stage_bytes = stage_bytes;
   gcmi_rgb_model_default(&s->rgb_model);
   (void)hipGetDevice(&s->device);
   const long long cap_pix = cap_tiles * T2;
-  int rc = pool_init(&s->pool, tile, (int)cap_tiles, cap_pix, max_sessions, nullptr);
+  int rc = GCMI_OK;
+  for (int i = 0; i < lanes && rc == GCMI_OK; ++i)
+    rc = pool_init(&s->lanes[i].pool, tile, (int)cap_tiles, cap_pix, max_sessions, nullptr);
   if (rc == GCMI_OK) {
     if (hipMalloc((void**)&s->d_stage, stage_bytes) != hipSuccess ||
         hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess ||
@@ -462,16 +522,21 @@ int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_ba
     if (s->d_stage) (void)hipFree(s->d_stage);
     if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     if (s->out_stream) (void)hipStreamDestroy(s->out_stream);
-    pool_free(&s->pool);
+    for (int i = 0; i < lanes; ++i) pool_free(&s->lanes[i].pool);
     delete s;
     return rc;
   }
   s->stage_alloc.init((long long)(stage_bytes / 4));
   s->sessions.assign(max_sessions, Session());
-  s->worker = std::thread(worker_main, s);
+  for (int i = 0; i < lanes; ++i) s->lanes[i].worker = std::thread(worker_main, s, i);
   s->finisher = std::thread(finisher_main, s);
   *out = s;
   return GCMI_OK;
+}
+
+int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions, int max_batch, int mode,
+                       gcmi_server** out) {
+  return gcmi_server_create_lanes(tile, hbm_budget, max_sessions, max_batch, mode, 0, out);
 }
 
 int gcmi_server_destroy(gcmi_server* s) {
@@ -481,7 +546,8 @@ int gcmi_server_destroy(gcmi_server* s) {
     s->stop = true;
     s->cv_work.notify_all();
   }
-  if (s->worker.joinable()) s->worker.join();
+  for (int i = 0; i < s->nlanes; ++i)
+    if (s->lanes[i].worker.joinable()) s->lanes[i].worker.join();
   {
     // the finisher completes what is queued, then stops
     std::lock_guard<std::mutex> lk(s->mu);
@@ -496,7 +562,7 @@ int gcmi_server_destroy(gcmi_server* s) {
   if (s->out_stream) (void)hipStreamDestroy(s->out_stream);
   if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
   if (s->d_stage) (void)hipFree(s->d_stage);
-  pool_free(&s->pool);
+  for (int i = 0; i < s->nlanes; ++i) pool_free(&s->lanes[i].pool);
   delete s;
   return GCMI_OK;
 }
@@ -506,7 +572,7 @@ int gcmi_server_set_params(gcmi_server* s, int sweeps, int bfs) {
     set_error("null server");
     return GCMI_EINVAL;
   }
-  std::lock_guard<std::mutex> plk(s->pool_mu);
+  AllPools locks(s);
   s->sweeps = sweeps;
   s->bfs = bfs;
   return GCMI_OK;
@@ -517,8 +583,9 @@ int gcmi_server_set_option(gcmi_server* s, int option, int value) {
     set_error("null server");
     return GCMI_EINVAL;
   }
-  std::lock_guard<std::mutex> plk(s->pool_mu);
-  return pool_set_option(&s->pool, option, value);
+  AllPools locks(s);
+  for (int i = 0; i < s->nlanes; ++i) GCMI_TRY(pool_set_option(&s->lanes[i].pool, option, value));
+  return GCMI_OK;
 }
 
 int gcmi_server_open(gcmi_server* s, int width, int height, int* session_out) {
@@ -526,7 +593,7 @@ int gcmi_server_open(gcmi_server* s, int width, int height, int* session_out) {
     set_error("null argument");
     return GCMI_EINVAL;
   }
-  std::lock_guard<std::mutex> plk(s->pool_mu);
+  AllPools locks(s);
   std::lock_guard<std::mutex> lk(s->mu);
   int slot = -1;
   for (int i = 0; i < (int)s->sessions.size(); ++i)
@@ -538,21 +605,35 @@ int gcmi_server_open(gcmi_server* s, int width, int height, int* session_out) {
     set_error("all %zu session slots are open", s->sessions.size());
     return GCMI_EBUDGET;
   }
-  int img = -1;
-  GCMI_TRY(pool_add_image(&s->pool, width, height, &img));
+  // the lane with the most free room first
+  int order[kMaxLanes];
+  for (int i = 0; i < s->nlanes; ++i) order[i] = i;
+  std::sort(order, order + s->nlanes, [&](int a, int b) {
+    const long long fa = s->lanes[a].pool.tiles_alloc.largest();
+    const long long fb = s->lanes[b].pool.tiles_alloc.largest();
+    return fa != fb ? fa > fb : a < b;
+  });
+  int img = -1, lane = -1, rc = GCMI_EBUDGET;
+  for (int k = 0; k < s->nlanes && lane < 0; ++k) {
+    rc = pool_add_image(&s->lanes[order[k]].pool, width, height, &img);
+    if (rc == GCMI_OK) lane = order[k];
+    else if (rc != GCMI_EBUDGET) return rc;
+  }
+  if (lane < 0) return rc;
   Session& ss = s->sessions[slot];
   ss = Session();
   const size_t words = ((size_t)width * height + 31) / 32;
   if (hipHostMalloc((void**)&ss.bits, words * 4, hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
     ss.bits = nullptr;
-    pool_remove_image(&s->pool, img);
+    pool_remove_image(&s->lanes[lane].pool, img);
     set_error("pinned mask buffer allocation failed");
     return GCMI_ENOMEM;
   }
   memset(ss.bits, 0, words * 4);
   ss.open = true;
   ss.img = img;
+  ss.lane = lane;
   ss.W = width;
   ss.H = height;
   *session_out = slot;
@@ -564,6 +645,7 @@ int gcmi_server_close(gcmi_server* s, int session) {
     set_error("null server");
     return GCMI_EINVAL;
   }
+  int lane;
   {
     std::unique_lock<std::mutex> lk(s->mu);
     GCMI_TRY(check_session(s, session));
@@ -574,11 +656,12 @@ int gcmi_server_close(gcmi_server* s, int session) {
         if (r->session == session) return false;
       return true;
     });
+    lane = s->sessions[session].lane;
   }
-  std::lock_guard<std::mutex> plk(s->pool_mu);
+  std::lock_guard<std::mutex> plk(s->lanes[lane].pool_mu);
   std::lock_guard<std::mutex> lk(s->mu);
   Session& ss = s->sessions[session];
-  pool_remove_image(&s->pool, ss.img);
+  pool_remove_image(&s->lanes[lane].pool, ss.img);
   if (ss.bits) (void)hipHostFree(ss.bits);
   ss = Session();
   return GCMI_OK;
@@ -937,17 +1020,25 @@ int gcmi_server_stats(gcmi_server* s, int64_t* out, int n) {
     return GCMI_EINVAL;
   }
   std::lock_guard<std::mutex> lk(s->mu);
-  const int T = s->pool.T;
-  int64_t open = 0, used = 0;
+  const int T = s->lanes[0].pool.T;
+  int64_t open = 0, used = 0, cap = 0, bytes = (int64_t)s->stage_bytes;
+  int64_t ns_apply = 0, ns_solve = 0, ns_out = 0;
   for (const Session& ss : s->sessions)
     if (ss.open) {
       ++open;
       used += (int64_t)((ss.W + T - 1) / T) * ((ss.H + T - 1) / T);
     }
-  const int64_t v[10] = {s->iterations, s->solved_requests, (int64_t)s->queue.size(), open, used,
-                         s->pool.cap_tiles, (int64_t)s->pool.device_bytes, s->ns_apply,
-                         s->ns_solve, s->ns_out};
-  for (int k = 0; k < n && k < 10; ++k) out[k] = v[k];
+  for (int i = 0; i < s->nlanes; ++i) {
+    const Lane& L = s->lanes[i];
+    cap += L.pool.cap_tiles;
+    bytes += (int64_t)L.pool.device_bytes;
+    ns_apply += L.ns_apply;
+    ns_solve += L.ns_solve;
+    ns_out += L.ns_out;
+  }
+  const int64_t v[11] = {s->iterations, s->solved_requests, (int64_t)s->queue.size(), open, used,
+                         cap, bytes, ns_apply, ns_solve, ns_out, s->nlanes};
+  for (int k = 0; k < n && k < 11; ++k) out[k] = v[k];
   return GCMI_OK;
 }
 

@@ -105,6 +105,7 @@ SIGNATURES = {
     "gcmi_batch_profile": [_vp, _int],
     "gcmi_batch_kernel_times": [_vp, ctypes.POINTER(ctypes.c_double), _i64p, _int],
     "gcmi_server_create": [_int, _sz, _int, _int, _int, ctypes.POINTER(_vp)],
+    "gcmi_server_create_lanes": [_int, _sz, _int, _int, _int, _int, ctypes.POINTER(_vp)],
     "gcmi_server_destroy": [_vp],
     "gcmi_server_set_params": [_vp, _int, _int],
     "gcmi_server_open": [_vp, _int, _int, ctypes.POINTER(_int)],

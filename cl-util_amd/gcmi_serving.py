@@ -117,9 +117,12 @@ def replay_trace(srv, sids, strokes, images):
 
 def run_trace(srv, sids, mix, rate, duration=2.0, seed=11, image_frac=0.1, lead=0.2, rgb=False):
     """Replays a Poisson trace (replay_trace) and returns latency statistics
-    in ms."""
+    in ms.  The trace is generated first and starts `lead` seconds after
+    that, so the generator's own time is not counted as latency."""
+    strokes, images = make_trace(mix, rate, duration, seed, 0, image_frac, rgb)
     t0 = time.monotonic_ns() + int(lead * 1e9)
-    strokes, images = make_trace(mix, rate, duration, seed, t0, image_frac, rgb)
+    strokes = [(x[0] + t0,) + tuple(x[1:]) for x in strokes]
+    images = [(x[0] + t0,) + tuple(x[1:]) for x in images]
     ids = replay_trace(srv, sids, strokes, images)
     lat = {"stroke": [], "image": []}
     by_size = {}

@@ -321,11 +321,20 @@ int gcmi_batch_kernel_times(gcmi_batch *batch, double *ms_out,
 #define GCMI_SERVE_COLD 2  /* batched; every session cut from scratch     */
 typedef struct gcmi_server gcmi_server;
 
-/* hbm_budget: device bytes the session pool may use (GCMI_EBUDGET if too
+/* hbm_budget: device bytes the session pools may use (GCMI_EBUDGET if too
  * small for one tile); max_sessions: session slots; max_batch: sessions
- * per solve. */
+ * per solve.  The warm and cold modes serve from 3 lanes, the plain mode
+ * from 1 (gcmi_server_create_lanes). */
 int gcmi_server_create(int tile, size_t hbm_budget, int max_sessions,
                        int max_batch, int mode, gcmi_server **out);
+/* The same with `lanes` (1..4; 0: the mode's default) lanes: each lane is
+ * a pool with its own stream and worker, holding its share of the sessions
+ * (a session opens in the lane with the most free room) and of the budget;
+ * the lanes' solves run concurrently on the device.  The plain mode always
+ * has one lane. */
+int gcmi_server_create_lanes(int tile, size_t hbm_budget, int max_sessions,
+                             int max_batch, int mode, int lanes,
+                             gcmi_server **out);
 /* Stops the worker and frees everything; queued requests are dropped.  No
  * other call on this server may be in progress or follow. */
 int gcmi_server_destroy(gcmi_server *server);
@@ -402,7 +411,8 @@ int gcmi_server_mask(gcmi_server *server, int session, uint32_t *h_bits,
                      int64_t *req_out);
 /* [0] solves  [1] requests served  [2] queued  [3] open sessions
  * [4] pool tiles in use  [5] pool tile capacity  [6] device bytes
- * [7] [8] [9] worker ns applying requests, solving, reading masks out */
+ * [7] [8] [9] worker ns applying requests, solving, listing mask copies
+ * (summed over lanes)  [10] lanes */
 int gcmi_server_stats(gcmi_server *server, int64_t *out, int n);
 /* Drops the record of a completed request. */
 int gcmi_server_forget(gcmi_server *server, int64_t req);

@@ -85,11 +85,15 @@ def test_session_alone(tile, scripts, oracle_results):
             check(res, oracle_results[k], ("alone", tile, k))
 
 
-@pytest.mark.parametrize("mode", ["batched", "plain"])
-def test_mix_equals_alone(mode, scripts, oracle_results):
+@pytest.mark.parametrize("mode,lanes", [("batched", 0), ("plain", 0), ("batched", 1),
+                                        ("batched", 2), ("cold", 0)])
+def test_mix_equals_alone(mode, lanes, scripts, oracle_results):
     """All sessions submit concurrently from their own threads; each result
-    is the oracle's (and so the same as the session served alone)."""
-    with GraphCutServer(tile=32, hbm_budget=BUDGET, max_sessions=len(DIMS), mode=mode) as srv:
+    is the oracle's (and so the same as the session served alone), with one
+    lane or several solving concurrently."""
+    with GraphCutServer(tile=32, hbm_budget=BUDGET, max_sessions=len(DIMS), mode=mode,
+                        lanes=lanes) as srv:
+        assert srv.stats()["lanes"] == (lanes or (1 if mode == "plain" else 3))
         sids = [srv.open(W, H) for W, H in DIMS]
         results = [[] for _ in DIMS]
         errors = []
@@ -158,7 +162,10 @@ def test_admission_by_budget():
             for _ in range(64):
                 sids.append(srv.open(256, 256))  # 64 tiles each
         assert e.value.code == L.EBUDGET
-        assert len(sids) == cap // 64
+        # each lane holds whole sessions of its share of the tiles
+        lanes = srv.stats()["lanes"]
+        assert lanes == 3
+        assert cap // 64 - lanes < len(sids) <= cap // 64
         w = G.centred(256, 256, seed=1)
         rid = srv.submit_image(sids[0], *w.arrays())
         f, m = O.bk_cut(*w.arrays())
