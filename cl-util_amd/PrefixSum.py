@@ -3,8 +3,10 @@
 Drop-in for PrefixSum.pyx:25-129 (same constructor, factory and scan; the
 OpenCL context/devices arguments are accepted and ignored).  The reference's
 recursive Blelloch scan with one blocking wait per level is replaced by
-libgcmi's single-launch block scan (multi-block above 65,536 elements).
+libgcmi's single-launch block scan (multi-block above 16,384 elements).
 """
+import ctypes
+
 from gcmi_lib import DeviceBuffer, check, lib, ptr
 
 LEN_WORKGROUP = 256
@@ -27,9 +29,13 @@ def roundUp(size, multiple):
 
 
 class PrefixSum:
-    def __init__(self, context=None, devices=None, capacity=0):
+    def __init__(self, context=None, devices=None, capacity=0, profile=False):
+        """profile: each scan waits for its launches and adds their device
+        time (ns, HIP events) to `elapsed`, as the reference does when built
+        with PROFILE_GPU (PrefixSum.pyx:100-101)."""
         self.capacity = roundUp(int(capacity), ELEMENTS_PER_WORKGROUP)
         self.elapsed = 0
+        self.profile = bool(profile)
 
     def factory(self, length=None):
         if length is None:
@@ -44,4 +50,10 @@ class PrefixSum:
             length = dArray.size // szInt
         if length > ELEMENTS_PER_WORKGROUP and length > self.capacity:
             raise ValueError("length > self.capacity: {0}, {1}".format(length, self.capacity))
-        check(lib().gcmi_prefix_sum(ptr(dArray), ptr(dTotal), int(length), None))
+        if self.profile:
+            ns = ctypes.c_int64()
+            check(lib().gcmi_prefix_sum_timed(ptr(dArray), ptr(dTotal), int(length), None,
+                                              ctypes.byref(ns)))
+            self.elapsed += ns.value
+        else:
+            check(lib().gcmi_prefix_sum(ptr(dArray), ptr(dTotal), int(length), None))

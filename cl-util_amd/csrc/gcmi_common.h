@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <mutex>
 #include <string>
 
 #include "../../include/gcmi.h"
@@ -133,9 +134,19 @@ struct Pred {
   }
 };
 
-// Per-device scratch for the C-ABI primitives' multi-block passes; grown on
-// demand.  Callers synchronise their stream before returning.
-int scratch_i32(int count, int32_t** out);
+// Scratch for the C-ABI primitives' multi-block passes: one buffer per
+// (device, stream), grown on demand.  acquire() takes the device's scratch
+// lock and keeps it until the lease ends, so enqueue every kernel that uses
+// the scratch before then: a later call on the same stream is ordered after
+// them, one on another stream gets another buffer, and a buffer is grown
+// (freed and reallocated) only after its stream has drained.
+class ScratchLease {
+ public:
+  int acquire(hipStream_t stream, int count, int32_t** out);
+
+ private:
+  std::unique_lock<std::mutex> lk_;
+};
 
 // Ints of scratch that compact_async / scan_async need for n elements.
 int compact_scratch_ints(int n);

@@ -8,12 +8,16 @@
 // the compaction kernel (no flag buffer), a wave's share of the output comes
 // from one 64-bit ballot and a popcount, and sizes up to kSingleMax run as a
 // single work-group launch; larger inputs use count / scan / write passes.
+// The entry points only enqueue their kernels: a result's length stays in
+// device memory, and reading it back is what synchronises (as the
+// reference's lazy IncrementalTileList.length, IncrementalTileList.pyx:
+// 106-112).
 #include "gcmi_common.h"
 
 namespace gcmi {
 
 constexpr int kSingleNT = 1024;            // threads of the one-block kernels
-constexpr int kSingleMax = 1 << 16;        // one-block path up to this n
+constexpr int kSingleMax = 1 << 14;        // one-block path up to this n (one pass)
 constexpr int kMultiNT = 256;              // threads of the multi-block kernels
 constexpr int kMultiIT = 16;                    // elements per thread of a multi-block pass
 constexpr int kMultiChunk = kMultiNT * kMultiIT; // elements per block (4096): one step each
@@ -325,11 +329,10 @@ int grid1d(int n) { return (n + 255) / 256; }
 
 int run_compact(const Pred& p, int n, int32_t* list, int32_t* len, void* stream) {
   hipStream_t s = pick_stream(stream);
-  int32_t* scratch = nullptr;
-  GCMI_TRY(scratch_i32(compact_scratch_ints(n), &scratch));
-  GCMI_TRY(compact_async(p, n, list, len, nullptr, scratch, s));
-  GCMI_HIP(hipStreamSynchronize(s));
-  return GCMI_OK;
+  ScratchLease lease;
+  int32_t* scr = nullptr;
+  GCMI_TRY(lease.acquire(s, compact_scratch_ints(n), &scr));
+  return compact_async(p, n, list, len, nullptr, scr, s);
 }
 
 }  // namespace
@@ -343,7 +346,6 @@ int gcmi_memset_i32(int32_t* d_dst, int32_t value, size_t count, void* stream) {
   hipStream_t s = pick_stream(stream);
   hipLaunchKernelGGL(k_fill, dim3(grid1d((int)count)), dim3(256), 0, s, d_dst, (int)count, value);
   GCMI_HIP(hipGetLastError());
-  GCMI_HIP(hipStreamSynchronize(s));
   return GCMI_OK;
 }
 
@@ -352,11 +354,46 @@ int gcmi_prefix_sum(int32_t* d_array, int32_t* d_total, int n, void* stream) {
   GCMI_TRY(check_n(n));
   if (!d_total || (n > 0 && !d_array)) { set_error("null pointer"); return GCMI_EINVAL; }
   hipStream_t s = pick_stream(stream);
-  int32_t* scratch = nullptr;
-  GCMI_TRY(scratch_i32(compact_scratch_ints(n), &scratch));
-  GCMI_TRY(scan_async(d_array, d_total, n, scratch, s));
-  GCMI_HIP(hipStreamSynchronize(s));
-  return GCMI_OK;
+  ScratchLease lease;
+  int32_t* scr = nullptr;
+  GCMI_TRY(lease.acquire(s, compact_scratch_ints(n), &scr));
+  return scan_async(d_array, d_total, n, scr, s);
+}
+
+int gcmi_prefix_sum_timed(int32_t* d_array, int32_t* d_total, int n, void* stream,
+                          int64_t* ns_out) {
+  GCMI_TRY(require_device());
+  GCMI_TRY(check_n(n));
+  if (!d_total || !ns_out || (n > 0 && !d_array)) { set_error("null pointer"); return GCMI_EINVAL; }
+  hipStream_t s = pick_stream(stream);
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  for (hipEvent_t& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      (void)hipGetLastError();
+      if (ev[0]) (void)hipEventDestroy(ev[0]);
+      set_error("event creation failed");
+      return GCMI_EHIP;
+    }
+  int rc = GCMI_OK;
+  float ms = 0;
+  if (hipEventRecord(ev[0], s) != hipSuccess) rc = GCMI_EHIP;
+  if (rc == GCMI_OK) {
+    ScratchLease lease;
+    int32_t* scr = nullptr;
+    rc = lease.acquire(s, compact_scratch_ints(n), &scr);
+    if (rc == GCMI_OK) rc = scan_async(d_array, d_total, n, scr, s);
+  }
+  if (rc == GCMI_OK && (hipEventRecord(ev[1], s) != hipSuccess ||
+                        hipEventSynchronize(ev[1]) != hipSuccess ||
+                        hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess)) {
+    (void)hipGetLastError();
+    set_error("scan timing failed");
+    rc = GCMI_EHIP;
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (rc == GCMI_OK) *ns_out = (int64_t)((double)ms * 1e6);
+  return rc;
 }
 
 int gcmi_stream_compact(const int32_t* d_flags, int32_t* d_list, int32_t* d_length, int n,
@@ -378,7 +415,6 @@ int gcmi_tiles_fill(int32_t* d_tiles, int n, int32_t value, void* stream) {
   hipStream_t s = pick_stream(stream);
   hipLaunchKernelGGL(k_fill, dim3(grid1d(n)), dim3(256), 0, s, d_tiles, n, value);
   GCMI_HIP(hipGetLastError());
-  GCMI_HIP(hipStreamSynchronize(s));
   return GCMI_OK;
 }
 
@@ -393,7 +429,6 @@ int gcmi_tiles_flag(const int32_t* d_tiles, int32_t* d_flags, int n, int op, int
   hipLaunchKernelGGL(k_flag, dim3(grid1d(n)), dim3(256), 0, s, cmp_pred(d_tiles, op, operand), n,
                      d_flags);
   GCMI_HIP(hipGetLastError());
-  GCMI_HIP(hipStreamSynchronize(s));
   return GCMI_OK;
 }
 
@@ -410,7 +445,6 @@ int gcmi_tiles_flag_logical(const int32_t* d_t1, const int32_t* d_t2, int32_t* d
   hipLaunchKernelGGL(k_flag, dim3(grid1d(n)), dim3(256), 0, s,
                      logical_pred(d_t1, d_t2, op1, op2, a, b, logical), n, d_flags);
   GCMI_HIP(hipGetLastError());
-  GCMI_HIP(hipStreamSynchronize(s));
   return GCMI_OK;
 }
 
@@ -428,7 +462,6 @@ int gcmi_tiles_increment_logical(int32_t* d_t1, const int32_t* d_t2, int n, int 
   hipLaunchKernelGGL(k_increment, dim3(grid1d(n)), dim3(256), 0, s,
                      logical_pred(d_t1, d_t2, op1, op2, a, b, logical), n, d_t1, iteration);
   GCMI_HIP(hipGetLastError());
-  GCMI_HIP(hipStreamSynchronize(s));
   return GCMI_OK;
 }
 

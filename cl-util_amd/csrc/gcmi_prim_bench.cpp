@@ -7,8 +7,9 @@
 //   gcmi_prim_bench [iterations]      (default 100)
 //
 // Prints CSV: img dim, mp, tile dim, num tiles, then per primitive the mean
-// wall time per call in ms (each call returns after its work is done, as
-// the reference's .wait()-ed launches).  Exit status 0 if every check
+// wall time per call in ms, call and completion (the calls only enqueue;
+// each is followed by a wait for its stream inside the timed region, as the
+// reference's .wait()-ed launches).  Exit status 0 if every check
 // passed, 1 on a wrong result, 3 without a usable GPU.
 #include <stdio.h>
 #include <stdlib.h>
@@ -57,6 +58,7 @@ int run_case(Dim img, Dim tile, int iters) {
     CHECK(gcmi_memcpy_h2d(d_a, ones.data(), (size_t)n * 4));
     const double t0 = now_ms();
     CHECK(gcmi_prefix_sum(d_a, d_total, n, nullptr));
+    CHECK(gcmi_stream_sync(nullptr));
     t_scan += now_ms() - t0;
   }
   int32_t total = -1;
@@ -71,6 +73,7 @@ int run_case(Dim img, Dim tile, int iters) {
   for (int it = 0; it < iters; ++it) {
     const double t0 = now_ms();
     CHECK(gcmi_stream_compact(d_flags, d_list, d_total, n, nullptr));
+    CHECK(gcmi_stream_sync(nullptr));
     t_compact += now_ms() - t0;
   }
   CHECK(gcmi_memcpy_d2h(&len, d_total, 4));
@@ -85,6 +88,7 @@ int run_case(Dim img, Dim tile, int iters) {
   for (int it = 0; it < iters; ++it) {
     const double t0 = now_ms();
     CHECK(gcmi_tiles_build(d_flags, n, GCMI_OP_GTE, 10, d_list, d_total, nullptr));
+    CHECK(gcmi_stream_sync(nullptr));
     t_build += now_ms() - t0;
   }
   CHECK(gcmi_memcpy_d2h(&len, d_total, 4));

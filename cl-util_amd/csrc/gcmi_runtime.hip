@@ -22,19 +22,29 @@ void set_error(const char* fmt, ...) {
 void clear_error() { g_err[0] = 0; }
 
 namespace {
-struct DeviceState {
-  hipStream_t stream = nullptr;
-  int32_t* scratch = nullptr;
-  int scratch_count = 0;
+// Per device: the library's own stream and the primitives' scratch, one
+// buffer per stream the primitives were called on.  States live in a fixed
+// array, so a pointer to one stays valid while other devices are added.
+constexpr int kMaxDevices = 64;
+struct StreamScratch {
+  hipStream_t stream;
+  int32_t* buf;
+  int count;
 };
-std::mutex g_mu;
-std::vector<DeviceState> g_dev;
+struct DeviceState {
+  std::mutex mu;
+  hipStream_t stream = nullptr;
+  std::vector<StreamScratch> scratch;
+};
+DeviceState g_dev[kMaxDevices];
 
 int current_state(DeviceState** out) {
   int dev = 0;
   GCMI_HIP(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lock(g_mu);
-  if ((int)g_dev.size() <= dev) g_dev.resize(dev + 1);
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device %d beyond the %d the library tracks", dev, kMaxDevices);
+    return GCMI_ENODEVICE;
+  }
   *out = &g_dev[dev];
   return GCMI_OK;
 }
@@ -54,6 +64,7 @@ int require_device() {
 hipStream_t default_stream() {
   DeviceState* st = nullptr;
   if (current_state(&st) != GCMI_OK) return nullptr;
+  std::lock_guard<std::mutex> lk(st->mu);
   if (!st->stream) {
     hipStream_t s = nullptr;
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess)
@@ -62,24 +73,35 @@ hipStream_t default_stream() {
   return st->stream;
 }
 
-int scratch_i32(int count, int32_t** out) {
+int ScratchLease::acquire(hipStream_t stream, int count, int32_t** out) {
   DeviceState* st = nullptr;
   GCMI_TRY(current_state(&st));
-  if (st->scratch_count < count) {
-    if (st->scratch) {
-      GCMI_HIP(hipDeviceSynchronize());
-      GCMI_HIP(hipFree(st->scratch));
-      st->scratch = nullptr;
-      st->scratch_count = 0;
+  lk_ = std::unique_lock<std::mutex>(st->mu);
+  StreamScratch* e = nullptr;
+  for (StreamScratch& x : st->scratch)
+    if (x.stream == stream) e = &x;
+  if (!e) {
+    st->scratch.push_back(StreamScratch{stream, nullptr, 0});
+    e = &st->scratch.back();
+  }
+  if (e->count < count) {
+    if (e->buf) {
+      // kernels enqueued on this stream earlier may still read the buffer
+      GCMI_HIP(hipStreamSynchronize(stream));
+      GCMI_HIP(hipFree(e->buf));
+      e->buf = nullptr;
+      e->count = 0;
     }
-    int want = count < 4096 ? 4096 : count;
-    if (hipMalloc(&st->scratch, sizeof(int32_t) * (size_t)want) != hipSuccess) {
+    const int want = count < 4096 ? 4096 : count;
+    if (hipMalloc(&e->buf, sizeof(int32_t) * (size_t)want) != hipSuccess) {
+      (void)hipGetLastError();
+      e->buf = nullptr;
       set_error("scratch allocation of %d ints failed", want);
       return GCMI_ENOMEM;
     }
-    st->scratch_count = want;
+    e->count = want;
   }
-  *out = st->scratch;
+  *out = e->buf;
   return GCMI_OK;
 }
 
@@ -151,6 +173,12 @@ int gcmi_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes) {
   hipStream_t s = default_stream();
   GCMI_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, s));
   GCMI_HIP(hipStreamSynchronize(s));
+  return GCMI_OK;
+}
+
+int gcmi_device_sync(void) {
+  GCMI_TRY(require_device());
+  GCMI_HIP(hipDeviceSynchronize());
   return GCMI_OK;
 }
 

@@ -63,7 +63,9 @@ SIGNATURES = {
     "gcmi_memcpy_d2d": [_vp, _vp, _sz],
     "gcmi_memset_i32": [_vp, _i32, _sz, _vp],
     "gcmi_stream_sync": [_vp],
+    "gcmi_device_sync": [],
     "gcmi_prefix_sum": [_vp, _vp, _int, _vp],
+    "gcmi_prefix_sum_timed": [_vp, _vp, _int, _vp, _i64p],
     "gcmi_stream_compact": [_vp, _vp, _vp, _int, _vp],
     "gcmi_tiles_fill": [_vp, _int, _i32, _vp],
     "gcmi_tiles_flag": [_vp, _vp, _int, _int, _i32, _vp],

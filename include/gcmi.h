@@ -9,8 +9,11 @@
  *     allocation on the current device); "h_" pointers are host pointers,
  *     borrowed for the duration of the call only.
  *   - `stream` arguments are hipStream_t passed as void*; NULL means the
- *     library's own stream for the current device.  Calls that return a
- *     length or a value to the host synchronise that stream.
+ *     library's own stream for the current device.  The primitives
+ *     (prefix sum, compaction, tile lists, memset) only enqueue their work
+ *     on that stream and leave lengths and totals in device memory; the
+ *     gcmi_memcpy_* calls (on the library's stream) and calls that return a
+ *     value to the host synchronise.
  *   - Handles (gcmi_cut, gcmi_batch, gcmi_server) own their device memory and
  *     one HIP stream each.  A handle is not thread-safe; distinct handles are
  *     independent.
@@ -68,6 +71,8 @@ int gcmi_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes);
 int gcmi_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes);
 int gcmi_memset_i32(int32_t *d_dst, int32_t value, size_t count, void *stream);
 int gcmi_stream_sync(void *stream);
+/* Waits for all work on the current device (hipDeviceSynchronize). */
+int gcmi_device_sync(void);
 
 /* ------------------------------------------------------------- primitives */
 
@@ -76,6 +81,11 @@ int gcmi_stream_sync(void *stream);
  * written to d_total[0] (int32 arithmetic, wrapping as the reference's
  * OpenCL ints do).  n may be 0 (total = 0). */
 int gcmi_prefix_sum(int32_t *d_array, int32_t *d_total, int n, void *stream);
+/* The same, then waits for the scan: *ns_out = its device time in ns, from HIP events around its launches (the reference adds each
+ * launch's OpenCL profile time to PrefixSum.elapsed, PrefixSum.pyx:100-101,
+ * 116-117, 128-129). */
+int gcmi_prefix_sum_timed(int32_t *d_array, int32_t *d_total, int n,
+                          void *stream, int64_t *ns_out);
 
 /* StreamCompact.compact(dFlags, dList, dLength, length) --
  * StreamCompact.pyx:66-82.  d_list[0..k) = ascending indices i < n with
