@@ -284,6 +284,14 @@ int scan_async(int32_t* a, int32_t* d_total, int n, int32_t* scratch, hipStream_
 
 }  // namespace gcmi
 
+// Scratch of gcmi_tiles_build_noalloc: the per-block counts of its
+// multi-block compaction (one per 4096 tiles) and their scan.  Each device
+// holds its own copy until the process ends.  Host code never uses this
+// name as an address: it takes the device address from hipGetSymbolAddress
+// (static_scratch below).
+constexpr int kNoAllocScratchInts = GCMI_NOALLOC_MAX_TILES / gcmi::kMultiChunk + 64;
+static __device__ __align__(256) int32_t scratch[kNoAllocScratchInts];
+
 using namespace gcmi;
 
 namespace {
@@ -333,6 +341,34 @@ int run_compact(const Pred& p, int n, int32_t* list, int32_t* len, void* stream)
   int32_t* scr = nullptr;
   GCMI_TRY(lease.acquire(s, compact_scratch_ints(n), &scr));
   return compact_async(p, n, list, len, nullptr, scr, s);
+}
+
+// Device address of this file's `scratch` array, per device: taken once
+// per device (with that device current) by hipGetSymbolAddress.
+constexpr int kStaticDevices = 64;
+std::mutex g_static_mu;
+int32_t* g_static_ptr[kStaticDevices] = {};
+
+int static_scratch(int32_t** out) {
+  int dev = 0;
+  GCMI_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kStaticDevices) {
+    set_error("device %d beyond the %d the library tracks", dev, kStaticDevices);
+    return GCMI_ENODEVICE;
+  }
+  std::lock_guard<std::mutex> lk(g_static_mu);
+  if (!g_static_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      set_error("hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return GCMI_EHIP;
+    }
+    g_static_ptr[dev] = static_cast<int32_t*>(ptr);
+  }
+  *out = g_static_ptr[dev];
+  return GCMI_OK;
 }
 
 }  // namespace
@@ -472,6 +508,24 @@ int gcmi_tiles_build(const int32_t* d_tiles, int n, int op, int32_t operand, int
   GCMI_TRY(check_op(op));
   if (!d_length || (n > 0 && (!d_tiles || !d_list))) { set_error("null pointer"); return GCMI_EINVAL; }
   return run_compact(cmp_pred(d_tiles, op, operand), n, d_list, d_length, stream);
+}
+
+int gcmi_tiles_build_noalloc(const int32_t* d_tiles, int n, int op, int32_t operand,
+                             int32_t* d_list, int32_t* d_length, void* stream) {
+  GCMI_TRY(require_device());
+  GCMI_TRY(check_n(n));
+  GCMI_TRY(check_op(op));
+  if (!d_length || (n > 0 && (!d_tiles || !d_list))) { set_error("null pointer"); return GCMI_EINVAL; }
+  if (n > GCMI_NOALLOC_MAX_TILES) {
+    set_error("gcmi_tiles_build_noalloc: %d tiles exceed its maximum of %d", n,
+              GCMI_NOALLOC_MAX_TILES);
+    return GCMI_EINVAL;
+  }
+  int32_t* scr = nullptr;
+  GCMI_TRY(static_scratch(&scr));
+  // NULL: HIP's null stream (the library creates no stream here)
+  return compact_async(cmp_pred(d_tiles, op, operand), n, d_list, d_length, nullptr, scr,
+                       reinterpret_cast<hipStream_t>(stream));
 }
 
 int gcmi_tiles_build_logical(const int32_t* d_t1, const int32_t* d_t2, int n, int op1, int op2,

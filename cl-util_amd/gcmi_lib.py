@@ -72,6 +72,7 @@ SIGNATURES = {
     "gcmi_tiles_flag_logical": [_vp, _vp, _vp, _int, _int, _int, _i32, _i32, _int, _vp],
     "gcmi_tiles_increment_logical": [_vp, _vp, _int, _int, _int, _i32, _i32, _int, _i32, _vp],
     "gcmi_tiles_build": [_vp, _int, _int, _i32, _vp, _vp, _vp],
+    "gcmi_tiles_build_noalloc": [_vp, _int, _int, _i32, _vp, _vp, _vp],
     "gcmi_tiles_build_logical": [_vp, _vp, _int, _int, _int, _i32, _i32, _int, _vp, _vp, _vp],
     "gcmi_cut_create": [_int, _int, _int, ctypes.POINTER(_vp)],
     "gcmi_cut_destroy": [_vp],

@@ -121,6 +121,20 @@ int gcmi_tiles_increment_logical(int32_t *d_t1, const int32_t *d_t2, int n,
 int gcmi_tiles_build(const int32_t *d_tiles, int n, int op, int32_t operand,
                      int32_t *d_list, int32_t *d_length, void *stream);
 
+/* No reference counterpart.  gcmi_tiles_build for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the scratch of its multi-block
+ * pass is a fixed array of the library (about 64 KiB per device, held until
+ * the process ends) instead of memory the library allocates; it calls no
+ * hipMalloc/hipFree.  n <= GCMI_NOALLOC_MAX_TILES (GCMI_EINVAL above).
+ * NULL stream: HIP's null stream.  Every call on one device shares that
+ * device's array, so calls that overlap on one device (concurrent streams
+ * or threads) race; calls ordered on one stream do not. */
+#define GCMI_NOALLOC_MAX_TILES (1 << 26)
+int gcmi_tiles_build_noalloc(const int32_t *d_tiles, int n, int op,
+                             int32_t operand, int32_t *d_list,
+                             int32_t *d_length, void *stream);
+
 /* IncrementalTileList.buildLogical (IncrementalTileList.pyx:157-176), fused. */
 int gcmi_tiles_build_logical(const int32_t *d_t1, const int32_t *d_t2, int n,
                              int op1, int op2, int32_t a, int32_t b,
