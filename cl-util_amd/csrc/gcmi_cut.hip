@@ -174,6 +174,61 @@ int gcmi_cut_run(gcmi_cut* c, int sweeps, int bfs, int64_t max_rounds, int64_t* 
   return GCMI_OK;
 }
 
+int gcmi_cut_push(gcmi_cut* c, int sweeps) {
+  if (!c) {
+    set_error("null handle");
+    return GCMI_EINVAL;
+  }
+  return pool_step_push(&c->pool, c->img, sweeps);
+}
+
+int gcmi_cut_relabel(gcmi_cut* c, int32_t* active_out) {
+  if (!c) {
+    set_error("null handle");
+    return GCMI_EINVAL;
+  }
+  return pool_step_relabel(&c->pool, c->img, false, active_out);
+}
+
+int gcmi_cut_start_bfs(gcmi_cut* c, int32_t* active_out) {
+  if (!c) {
+    set_error("null handle");
+    return GCMI_EINVAL;
+  }
+  return pool_step_relabel(&c->pool, c->img, true, active_out);
+}
+
+int gcmi_cut_is_completed(gcmi_cut* c, int* done_out, int64_t* flow_out) {
+  if (!c || !done_out) {
+    set_error("null argument");
+    return GCMI_EINVAL;
+  }
+  return pool_step_completed(&c->pool, c->img, done_out, flow_out);
+}
+
+int gcmi_cut_tile_flags(gcmi_cut* c, int which, int32_t* d_out) {
+  if (!c || !d_out) {
+    set_error("null argument");
+    return GCMI_EINVAL;
+  }
+  Pool* p = &c->pool;
+  const int32_t* src = nullptr;
+  switch (which) {
+    case GCMI_TILES_ACTIVE: src = p->dev.act; break;
+    case GCMI_TILES_BORDER: src = p->dev.pflag; break;
+    case GCMI_TILES_CHANGED: src = p->dev.dirty; break;
+    case GCMI_TILES_EXCESS: src = p->dev.haspos; break;
+    default:
+      set_error("unknown tile set %d", which);
+      return GCMI_EINVAL;
+  }
+  const PoolImage& im = p->imgs[c->img];
+  GCMI_HIP(hipMemcpyAsync(d_out, src + im.tile0, (size_t)im.ntiles * 4, hipMemcpyDeviceToDevice,
+                          p->stream));
+  GCMI_HIP(hipStreamSynchronize(p->stream));
+  return GCMI_OK;
+}
+
 int gcmi_cut_labels(gcmi_cut* c, uint8_t* h_out) {
   if (!c || !h_out) {
     set_error("null argument");

@@ -212,7 +212,18 @@ struct Pool {
   int grid_cap = 2048;
   int64_t stats[STAT_N] = {0};
   Prof prof;
+  // a solve driven step by step (pool_step_*): its image (-1: none), whether
+  // it started from scratch, its in-tile sweeps, the last step, the active
+  // tile count of the last relabel and the relabels so far
+  int step_img = -1;
+  bool step_fresh = false;
+  int step_sweeps = kDefaultSweeps;
+  int step_last = 0;
+  int32_t step_nact = -1;
+  int step_relabels = 0;
 };
+
+enum { STEP_BEGIN = 0, STEP_PUSH = 1, STEP_RELABEL = 2 };
 
 // Bytes of device memory a pool of the given capacity allocates.
 size_t pool_bytes(int tile, int cap_tiles, long long cap_pix, int cap_imgs);
@@ -283,6 +294,19 @@ int pool_paint_many(Pool* p, int count, const PaintReq* reqs);
 // edited ones warm).  flows_out (may be NULL) receives one flow per image.
 int pool_solve(Pool* p, const int* img_ids, int n, int sweeps, int bfs, int64_t max_rounds,
                int64_t* flows_out);
+// One image's solve driven step by step, as the reference's GraphCut hooks
+// (GraphCut.pyx:187-346).  The first step sets the solve up (from scratch,
+// or warm after edits); uploads, edits, forget and pool_solve drop it.
+//   push: one push round over the tiles holding active pixels;
+//   relabel: a global relabel run until its queue drains -- over every
+//     tile (full), or over the tiles whose arcs or excess changed (valid
+//     once this solve has relabelled, or after a warm edit); *nact gets the
+//     tiles left active;
+//   completed: right after a relabel that left no active tile, finishes the
+//     solve (minimal source set, labels, flow) and sets *done = 1.
+int pool_step_push(Pool* p, int img, int sweeps);
+int pool_step_relabel(Pool* p, int img, bool full, int32_t* nact);
+int pool_step_completed(Pool* p, int img, int* done, int64_t* flow);
 int pool_labels(Pool* p, int img, uint8_t* out, bool device_dst);
 // Labels packed to bits (bit p % 32 of word p / 32 = label of pixel p).
 int pool_labels_packed(Pool* p, int img, uint32_t* out);

@@ -1008,6 +1008,7 @@ static int check_img(Pool* p, int img) {
 
 int pool_upload(Pool* p, int img, const int32_t* cs, const int32_t* ct, const int32_t* right,
                 const int32_t* down, bool device_src) {
+  p->step_img = -1;  // drops a stepped solve in progress
   GCMI_TRY(check_img(p, img));
   PoolImage& im = p->imgs[img];
   const size_t n = (size_t)im.W * im.H;
@@ -1070,6 +1071,7 @@ int check_rgb_model(const gcmi_rgb_model* m, int32_t* max_pair) {
 }
 
 int pool_upload_rgb_many(Pool* p, int count, const RgbReq* reqs) {
+  p->step_img = -1;  // drops a stepped solve in progress
   if (count <= 0) return GCMI_OK;
   static_assert(kPixAlign % 4 == 0, "colour capacities are stored four pixels at a time");
   std::vector<int32_t> wmax(count);
@@ -1228,6 +1230,7 @@ int check_edit(long long npx, int n, const int32_t* idx, const int32_t* cs, cons
 }
 
 int pool_edit_many(Pool* p, int count, const EditReq* e) {
+  p->step_img = -1;  // drops a stepped solve in progress
   long long total = 0;
   for (int k = 0; k < count; ++k) {
     GCMI_TRY(check_img(p, e[k].img));
@@ -1393,6 +1396,7 @@ int pool_paint(Pool* p, int img, int npts, const double* xy, double width, int32
 }
 
 int pool_paint_many(Pool* p, int count, const PaintReq* reqs) {
+  p->step_img = -1;  // drops a stepped solve in progress
   if (count <= 0) return GCMI_OK;
   if (!reqs) {
     set_error("null stroke list");
@@ -2019,15 +2023,56 @@ static int run_recorded(Pool* p, const Solve& S, int* flood_cur) {
   return GCMI_OK;
 }
 
+// A completed solve of ids[0..n): the images' flows and state flags, and
+// the run's counters (from the device tallies read with the flows).
+static void finish_images(Pool* p, const int* ids, int n, int64_t* flows_out) {
+  const int T = p->T;
+  for (int i = 0; i < n; ++i) {
+    PoolImage& im = p->imgs[ids[i]];
+    im.flow = p->h_flows[i];
+    im.fresh = false;
+    im.edited = false;
+    im.solved = true;
+    if (flows_out) flows_out[i] = im.flow;
+  }
+  const unsigned long long* ht = p->h_tally;
+  p->stats[STAT_ROUNDS] = (int64_t)ht[TALLY_PUSH_ROUNDS];
+  p->stats[STAT_RESIDENCIES] = (int64_t)ht[TALLY_PUSH_TILES];
+  p->stats[STAT_GLOBAL_RELABELS] = (int64_t)ht[TALLY_RELABELS];
+  p->stats[STAT_BFS_PASSES] = (int64_t)(ht[TALLY_RELABEL_TILES] + ht[TALLY_BFS]);
+  p->stats[STAT_FLOOD_PASSES] = (int64_t)ht[TALLY_FLOOD];
+  p->stats[STAT_RELABEL_TILES] = (int64_t)ht[TALLY_RELABEL_TILES];
+  p->stats[STAT_BFS_ROUND_PASSES] = (int64_t)ht[TALLY_BFS];
+  // algorithmic bytes: first round 16 per image pixel read + 25 per padded
+  // pixel written (state + mask); later residencies 49 (24 read, 25 written);
+  // relabel first pass 5 (mask read, h written); later BFS passes 9 (mask +
+  // h read, h written); flood passes 2 (mask read, reach written); labels 2.
+  int64_t img_px = 0, fresh_px = 0, nfresh = 0;
+  for (int i = 0; i < n; ++i) {
+    const PoolImage& im = p->imgs[ids[i]];
+    img_px += (int64_t)im.W * im.H;
+    if (p->h_prep[i].fresh_off >= 0) {
+      fresh_px += (int64_t)im.W * im.H;
+      nfresh += im.ntiles;
+    }
+  }
+  const int64_t tp = (int64_t)T * T;
+  p->stats[STAT_ALG_BYTES] = fresh_px * 16 + nfresh * tp * 25 +
+                             (p->stats[STAT_RESIDENCIES] - nfresh) * tp * 49 +
+                             (int64_t)ht[TALLY_RELABEL_TILES] * tp * 5 +
+                             (int64_t)ht[TALLY_BFS] * tp * 9 +
+                             (int64_t)ht[TALLY_FLOOD] * tp * 2 + img_px * 2;
+}
+
 int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_rounds,
                int64_t* flows_out) {
+  p->step_img = -1;  // a stepped solve in progress is dropped
   if (n <= 0 || n > p->cap_imgs || !ids) {
     set_error("bad image list");
     return GCMI_EINVAL;
   }
   if (sweeps <= 0) sweeps = kDefaultSweeps;
   if (bfs <= 0) bfs = kDefaultBfs;
-  const int T = p->T;
   int nsel = 0, nfresh = 0;
   for (int i = 0; i < n; ++i) {
     GCMI_TRY(check_img(p, ids[i]));
@@ -2136,38 +2181,118 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
     }
   }
   if (P) P->collect();
-  for (int i = 0; i < n; ++i) {
-    PoolImage& im = p->imgs[ids[i]];
-    im.flow = p->h_flows[i];
-    im.fresh = false;
-    im.edited = false;
-    im.solved = true;
-    if (flows_out) flows_out[i] = im.flow;
+  finish_images(p, ids, n, flows_out);
+  return GCMI_OK;
+}
+
+// Solving one image step by step, as the reference's GraphCut hooks do
+// (GraphCut.pyx:187-346): the solve's state lives in the pool between the
+// calls.  begin() sets it up: the image list and counts, and for an image
+// cut from scratch its tile state built from the capacities (round 1 of the
+// push kernel with no sweeps: e, caps, h = 0 on sinks and 1 elsewhere).
+static Solve step_solve(Pool* p) {
+  const PoolImage& im = p->imgs[p->step_img];
+  const int nfresh = p->step_fresh ? im.ntiles : 0;
+  return Solve{p, p->ops, 1, im.ntiles, nfresh, p->step_sweeps, kDefaultBfs,
+               std::min(im.ntiles, kRelabelGridCap), false, nullptr};
+}
+
+static int step_begin(Pool* p, int img) {
+  if (p->step_img == img) return GCMI_OK;
+  GCMI_TRY(check_img(p, img));
+  PoolImage& im = p->imgs[img];
+  if (!im.uploaded) {
+    set_error("image %d stepped before upload", img);
+    return GCMI_EINVAL;
   }
-  const unsigned long long* ht = p->h_tally;
-  p->stats[STAT_ROUNDS] = (int64_t)ht[TALLY_PUSH_ROUNDS];
-  p->stats[STAT_RESIDENCIES] = (int64_t)ht[TALLY_PUSH_TILES];
-  p->stats[STAT_GLOBAL_RELABELS] = (int64_t)ht[TALLY_RELABELS];
-  p->stats[STAT_BFS_PASSES] = (int64_t)(ht[TALLY_RELABEL_TILES] + ht[TALLY_BFS]);
-  p->stats[STAT_FLOOD_PASSES] = (int64_t)ht[TALLY_FLOOD];
-  p->stats[STAT_RELABEL_TILES] = (int64_t)ht[TALLY_RELABEL_TILES];
-  p->stats[STAT_BFS_ROUND_PASSES] = (int64_t)ht[TALLY_BFS];
-  // algorithmic bytes: first round 16 per image pixel read + 25 per padded
-  // pixel written (state + mask); later residencies 49 (24 read, 25 written);
-  // relabel first pass 5 (mask read, h written); later BFS passes 9 (mask +
-  // h read, h written); flood passes 2 (mask read, reach written); labels 2.
-  int64_t img_px = 0, fresh_px = 0;
-  for (int i = 0; i < n; ++i) {
-    const PoolImage& im = p->imgs[ids[i]];
-    img_px += (int64_t)im.W * im.H;
-    if (p->h_prep[i].fresh_off >= 0) fresh_px += (int64_t)im.W * im.H;
+  GCMI_TRY(stage_wait(p, STAGE_PREP));
+  const bool fresh = im.fresh || !im.edited;
+  p->h_prep[0] = Prep{img, 0, fresh ? 0 : -1, 0};
+  const bool ident = im.ntiles == p->cap_tiles && im.tile0 == 0;
+  p->lsel = ident ? nullptr : p->sel;
+  p->lfresh = (ident && fresh) ? nullptr : p->fresh;
+  for (int k = 0; k < STAT_N; ++k) p->stats[k] = 0;
+  p->stats[STAT_TILES] = im.ntiles;
+  if (p->ep_reset) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(std::min((p->cap_tiles + 255) / 256, 4096)), dim3(256), 0,
+                       p->stream, p->dev.stamp, (long long)p->cap_tiles, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(std::min((p->cap_tiles + 255) / 256, 4096)), dim3(256), 0,
+                       p->stream, p->dev.rflag, (long long)p->cap_tiles, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, p->stream, p->counts + CNT_EP, 2LL, 0);
+    GCMI_HIP(hipGetLastError());
+    p->ep_reset = false;
   }
-  const int64_t tp = (int64_t)T * T;
-  p->stats[STAT_ALG_BYTES] = fresh_px * 16 + (int64_t)nfresh * tp * 25 +
-                             (p->stats[STAT_RESIDENCIES] - nfresh) * tp * 49 +
-                             (int64_t)ht[TALLY_RELABEL_TILES] * tp * 5 +
-                             (int64_t)ht[TALLY_BFS] * tp * 9 +
-                             (int64_t)ht[TALLY_FLOOD] * tp * 2 + img_px * 2;
+  p->step_img = img;
+  p->step_fresh = fresh;
+  p->step_last = STEP_BEGIN;
+  p->step_nact = -1;
+  p->step_relabels = 0;
+  const Solve S = step_solve(p);
+  GCMI_TRY(phase_prepare(S));
+  if (fresh) {
+    S.ops->reset_tiles(p, std::min(S.nfresh, 4096));
+    S.ops->push(p, S.nfresh, 0, true);
+    GCMI_HIP(hipGetLastError());
+  }
+  return GCMI_OK;
+}
+
+int pool_step_push(Pool* p, int img, int sweeps) {
+  if (p->step_img != img) p->step_sweeps = sweeps > 0 ? sweeps : kDefaultSweeps;
+  GCMI_TRY(step_begin(p, img));
+  if (sweeps > 0) p->step_sweeps = sweeps;
+  const Solve S = step_solve(p);
+  // the tiles holding active pixels now (the flags are cleared as listed;
+  // the push sets them again where pixels stay active)
+  GCMI_TRY(compact_sel(p, p->dev.act, p->alist, CNT_ACT, true, S.nsel, nullptr, COUNT_FILL));
+  GCMI_TRY(read_counts(p));
+  const int32_t nact = p->h_counts[CNT_ACT];
+  if (nact > 0) {
+    S.ops->push(p, S.grid_for(nact), S.sweeps, false);
+    GCMI_HIP(hipGetLastError());
+  }
+  p->step_last = STEP_PUSH;
+  return GCMI_OK;
+}
+
+int pool_step_relabel(Pool* p, int img, bool full, int32_t* nact_out) {
+  if (p->step_img != img) p->step_sweeps = kDefaultSweeps;
+  GCMI_TRY(step_begin(p, img));
+  const Solve S = step_solve(p);
+  // a local relabel keeps the stored distances of clean tiles: valid once a
+  // relabel has run in this solve, or from the solve before a warm edit
+  const bool local = !full && (S.nfresh < S.nsel || p->step_relabels > 0);
+  int cur = 0;
+  int32_t nact = 0;
+  GCMI_TRY(phase_relabel(S, local, 0, &cur, &nact));
+  p->step_relabels++;
+  p->step_nact = nact;
+  p->step_last = STEP_RELABEL;
+  if (nact_out) *nact_out = nact;
+  return GCMI_OK;
+}
+
+int pool_step_completed(Pool* p, int img, int* done, int64_t* flow) {
+  GCMI_TRY(check_img(p, img));
+  *done = 0;
+  if (p->step_img != img) {
+    // not stepping: done if a cut of the current capacities is complete
+    const PoolImage& im = p->imgs[img];
+    *done = im.solved ? 1 : 0;
+    if (flow) *flow = im.flow;
+    return GCMI_OK;
+  }
+  // termination is decided right after a relabel that drained and left no
+  // active tile (the same rule as pool_solve)
+  if (p->step_last != STEP_RELABEL || p->step_nact != 0) return GCMI_OK;
+  const Solve S = step_solve(p);
+  int cur = 0;
+  GCMI_TRY(phase_flood_start(S, 0, &cur));
+  GCMI_TRY(flood_rounds_host(S, cur));
+  const int ids[1] = {img};
+  finish_images(p, ids, 1, flow);
+  p->step_img = -1;
+  *done = 1;
   return GCMI_OK;
 }
 
@@ -2226,6 +2351,7 @@ int pool_set_option(Pool* p, int option, int value) {
 }
 
 int pool_forget(Pool* p, int img) {
+  p->step_img = -1;  // drops a stepped solve in progress
   GCMI_TRY(check_img(p, img));
   PoolImage& im = p->imgs[img];
   im.fresh = true;

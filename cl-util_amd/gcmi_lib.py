@@ -80,6 +80,11 @@ SIGNATURES = {
     "gcmi_cut_upload_device": [_vp, _vp, _vp, _vp, _vp],
     "gcmi_cut_run": [_vp, _int, _int, _i64, _i64p],
     "gcmi_cut_labels": [_vp, _vp],
+    "gcmi_cut_push": [_vp, _int],
+    "gcmi_cut_relabel": [_vp, _i32p],
+    "gcmi_cut_start_bfs": [_vp, _i32p],
+    "gcmi_cut_is_completed": [_vp, ctypes.POINTER(_int), _i64p],
+    "gcmi_cut_tile_flags": [_vp, _int, _vp],
     "gcmi_cut_labels_device": [_vp, _vp],
     "gcmi_cut_stats": [_vp, _i64p, _int],
     "gcmi_cut_heights": [_vp, _vp],
@@ -132,6 +137,7 @@ PAIR_EXP, PAIR_INV = 0, 1
 OPT_LOCAL_RELABEL = 1
 OPT_DEVICE_ROUNDS = 2
 OPT_GRAPH = 3
+TILES_ACTIVE, TILES_BORDER, TILES_CHANGED, TILES_EXCESS = 0, 1, 2, 3
 
 
 class RgbModel(ctypes.Structure):

@@ -209,6 +209,46 @@ int gcmi_cut_capacities(gcmi_cut *cut, int32_t *h_source, int32_t *h_sink,
  * After it returns GCMI_OK, gcmi_cut_labels gives the minimal source set. */
 int gcmi_cut_run(gcmi_cut *cut, int sweeps, int bfs, int64_t max_rounds,
                  int64_t *flow_out);
+/* The reference's step hooks (GraphCut.pyx:187-346), for callers that
+ * drive the solve themselves.  They solve the cut's current capacities:
+ * the first hook after an upload, edit or run sets the solve up (from
+ * scratch, or warm after edits), and an upload, edit, paint, forget or run
+ * drops a stepped solve in progress.
+ *   gcmi_cut_push       GraphCut.push: one push round (sweeps in-tile
+ *                       sweeps, <= 0: default) over the tiles holding
+ *                       active pixels.
+ *   gcmi_cut_relabel    GraphCut.relabel: a relabel of the tiles whose arcs
+ *                       or excess changed since their distances were
+ *                       computed (every tile if this solve has not relabelled
+ *                       yet and started from scratch), run until its queue
+ *                       drains; *active_out (may be NULL) = tiles left
+ *                       holding active pixels.
+ *   gcmi_cut_start_bfs  GraphCut.startBfs: the same over every tile.
+ *   gcmi_cut_is_completed  GraphCut.isCompleted: *done_out = 1 right after
+ *                       a relabel that left no active tile -- the solve is
+ *                       then finished (minimal source set, labels, flow in
+ *                       *flow_out, may be NULL) -- or when no solve is in
+ *                       progress and the last one is complete; else 0.
+ * The result is the same exact cut as gcmi_cut_run's. */
+int gcmi_cut_push(gcmi_cut *cut, int sweeps);
+int gcmi_cut_relabel(gcmi_cut *cut, int32_t *active_out);
+int gcmi_cut_start_bfs(gcmi_cut *cut, int32_t *active_out);
+int gcmi_cut_is_completed(gcmi_cut *cut, int *done_out, int64_t *flow_out);
+/* Per-tile flags of the cut's current state, the nearest counterparts of
+ * the reference's four tile lists (GraphCut.pyx:142-149): d_out[ty * tiles
+ * across + tx] (int32, device memory, one per tile of the padded grid) is
+ * nonzero for a tile that
+ *   GCMI_TILES_ACTIVE   holds active pixels (tilelistLoad: tiles pushed next)
+ *   GCMI_TILES_BORDER   has inflow pending from a neighbour (tilelistBorder)
+ *   GCMI_TILES_CHANGED  changed since its distances were computed (the
+ *                       next relabel's tiles; tilelistBfs)
+ *   GCMI_TILES_EXCESS   holds positive excess (where the minimal source set
+ *                       grows from; tilelistEdges has no exact counterpart) */
+#define GCMI_TILES_ACTIVE 0
+#define GCMI_TILES_BORDER 1
+#define GCMI_TILES_CHANGED 2
+#define GCMI_TILES_EXCESS 3
+int gcmi_cut_tile_flags(gcmi_cut *cut, int which, int32_t *d_out);
 /* Warm-start edit (no reference counterpart; the brush tools of the
  * reference re-upload the whole excess image, test_GraphCut.py:74-75): new
  * source/sink capacities for n distinct pixels (h_index = y*width+x).  The
