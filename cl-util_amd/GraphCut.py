@@ -165,6 +165,29 @@ class GraphCut:
         self._device_pairwise = True
         self._stepping = False
 
+    def upload_gmm(self, image, fg, bg, scale=10.0, cap=1000.0, pairwise=None):
+        """Capacities built on the device from a colour image with unaries
+        from two colour GMMs (GMM objects or gcmi_lib.Gmm): source =
+        rint(scale * min(max(score_bg, 0), cap)), sink the same under fg --
+        the reference feeds GMM scores to the cut as its unaries
+        (test_GraphCut.py:44-47).  Pairwise weights as upload_rgb builds
+        them (pairwise: an RgbModel; default: the reference's weight with
+        this cut's lamda, or the workloads' model without one)."""
+        a, ch = rgb_image(image)
+        if a.shape[:2] != self.shape:
+            raise ValueError("expected an image of shape %s" % (self.shape,))
+        if pairwise is None:
+            pairwise = rgb_model() if self.lamda is None else \
+                rgb_model(pair_kind=PAIR_INV, pair_scale=self.lamda)
+        f = getattr(fg, "model", fg)
+        b = getattr(bg, "model", bg)
+        check(lib().gcmi_cut_upload_rgb_gmm(self._h, host_ptr(a), ch, ctypes.byref(pairwise),
+                                            ctypes.byref(f), ctypes.byref(b), float(scale),
+                                            float(cap)))
+        self._right = self._down = None
+        self._device_pairwise = True
+        self._stepping = False
+
     def capacities(self):
         """(source, sink, right, down) int32 [H][W] as stored on the device."""
         out = [np.empty(self.shape, np.int32) for _ in range(4)]

@@ -64,6 +64,19 @@ class GraphCutBatch:
         m = rgb_model() if model is None else model
         check(lib().gcmi_batch_upload_rgb(self._h, int(i), host_ptr(a), ch, ctypes.byref(m)))
 
+    def upload_gmm(self, i, image, fg, bg, scale=10.0, cap=1000.0, pairwise=None):
+        """Image i's capacities with GMM unaries (as GraphCut.upload_gmm;
+        pairwise default: the workloads' model)."""
+        a, ch = rgb_image(image)
+        if a.shape[:2] != self._shape(i):
+            raise ValueError("expected an image of shape %s" % (self._shape(i),))
+        m = rgb_model() if pairwise is None else pairwise
+        f = getattr(fg, "model", fg)
+        b = getattr(bg, "model", bg)
+        check(lib().gcmi_batch_upload_rgb_gmm(self._h, int(i), host_ptr(a), ch, ctypes.byref(m),
+                                              ctypes.byref(f), ctypes.byref(b), float(scale),
+                                              float(cap)))
+
     def upload_device(self, i, d_source, d_sink, d_right, d_down):
         check(lib().gcmi_batch_upload_device(self._h, int(i), ptr(d_source), ptr(d_sink),
                                              ptr(d_right), ptr(d_down)))

@@ -111,6 +111,17 @@ int gcmi_cut_upload_rgb(gcmi_cut* c, const uint8_t* h_rgb, int channels,
   return pool_upload_rgb_host(&c->pool, c->img, h_rgb, channels, model);
 }
 
+int gcmi_cut_upload_rgb_gmm(gcmi_cut* c, const uint8_t* h_rgb, int channels,
+                            const gcmi_rgb_model* pairwise, const gcmi_gmm* fg,
+                            const gcmi_gmm* bg, double unary_scale, double unary_cap) {
+  if (!c) {
+    set_error("null handle");
+    return GCMI_EINVAL;
+  }
+  return pool_upload_rgb_gmm_host(&c->pool, c->img, h_rgb, channels, pairwise, fg, bg,
+                                  unary_scale, unary_cap);
+}
+
 int gcmi_cut_capacities(gcmi_cut* c, int32_t* h_source, int32_t* h_sink, int32_t* h_right,
                         int32_t* h_down) {
   if (!c || !h_source || !h_sink || !h_right || !h_down) {
@@ -371,6 +382,14 @@ int gcmi_batch_upload_rgb(gcmi_batch* b, int i, const uint8_t* h_rgb, int channe
                           const gcmi_rgb_model* model) {
   GCMI_TRY(check_batch_img(b, i));
   return pool_upload_rgb_host(&b->pool, b->ids[i], h_rgb, channels, model);
+}
+
+int gcmi_batch_upload_rgb_gmm(gcmi_batch* b, int i, const uint8_t* h_rgb, int channels,
+                              const gcmi_rgb_model* pairwise, const gcmi_gmm* fg,
+                              const gcmi_gmm* bg, double unary_scale, double unary_cap) {
+  GCMI_TRY(check_batch_img(b, i));
+  return pool_upload_rgb_gmm_host(&b->pool, b->ids[i], h_rgb, channels, pairwise, fg, bg,
+                                  unary_scale, unary_cap);
 }
 
 int gcmi_batch_upload_device(gcmi_batch* b, int i, const int32_t* d_source,

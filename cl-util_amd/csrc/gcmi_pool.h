@@ -255,6 +255,16 @@ int pool_upload_rgb(Pool* p, int img, const uint8_t* d_rgb, int channels,
 // The same from host memory (copied; synchronises).
 int pool_upload_rgb_host(Pool* p, int img, const uint8_t* h_rgb, int channels,
                          const gcmi_rgb_model* m);
+// Copies a host colour image of image img into the pool's colour staging
+// (grown on demand) on the pool's stream; *d_out = its device address.
+int pool_stage_rgb(Pool* p, int img, const uint8_t* h_rgb, int channels, const uint8_t** d_out);
+// Capacities of image img from a host colour image: pairwise weights as
+// the colour model `pair` builds them, unaries from two colour GMMs --
+// source = the background model's quantised score, sink = the
+// foreground's (gcmi_gmm.hip; copied, synchronises).
+int pool_upload_rgb_gmm_host(Pool* p, int img, const uint8_t* h_rgb, int channels,
+                             const gcmi_rgb_model* pair, const gcmi_gmm* fg, const gcmi_gmm* bg,
+                             double scale, double cap);
 // Warm-start edit of n distinct pixels (row-major indices local to the
 // image) from host arrays.
 int pool_edit(Pool* p, int img, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct);

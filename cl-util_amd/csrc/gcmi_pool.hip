@@ -1152,8 +1152,8 @@ int pool_upload_rgb(Pool* p, int img, const uint8_t* d_rgb, int channels,
   return pool_upload_rgb_many(p, 1, &r);
 }
 
-int pool_upload_rgb_host(Pool* p, int img, const uint8_t* h_rgb, int channels,
-                         const gcmi_rgb_model* m) {
+int pool_stage_rgb(Pool* p, int img, const uint8_t* h_rgb, int channels,
+                   const uint8_t** d_out) {
   GCMI_TRY(check_img(p, img));
   if (!h_rgb) {
     set_error("null colour image");
@@ -1163,7 +1163,6 @@ int pool_upload_rgb_host(Pool* p, int img, const uint8_t* h_rgb, int channels,
     set_error("a colour image has 3 or 4 channels (got %d)", channels);
     return GCMI_EINVAL;
   }
-  GCMI_TRY(check_rgb_model(m, nullptr));
   const PoolImage& im = p->imgs[img];
   const long long bytes = (long long)im.W * im.H * channels;
   if (p->rgb_cap < bytes) {
@@ -1179,7 +1178,16 @@ int pool_upload_rgb_host(Pool* p, int img, const uint8_t* h_rgb, int channels,
     p->rgb_cap = bytes;
   }
   GCMI_HIP(hipMemcpyAsync(p->d_rgb, h_rgb, (size_t)bytes, hipMemcpyHostToDevice, p->stream));
-  GCMI_TRY(pool_upload_rgb(p, img, p->d_rgb, channels, m));
+  *d_out = p->d_rgb;
+  return GCMI_OK;
+}
+
+int pool_upload_rgb_host(Pool* p, int img, const uint8_t* h_rgb, int channels,
+                         const gcmi_rgb_model* m) {
+  GCMI_TRY(check_rgb_model(m, nullptr));
+  const uint8_t* d = nullptr;
+  GCMI_TRY(pool_stage_rgb(p, img, h_rgb, channels, &d));
+  GCMI_TRY(pool_upload_rgb(p, img, d, channels, m));
   GCMI_HIP(hipStreamSynchronize(p->stream));
   return GCMI_OK;
 }

@@ -130,6 +130,13 @@ SIGNATURES = {
     "gcmi_batch_upload_rgb": [_vp, _int, _vp, _int, _vp],
     "gcmi_server_set_rgb_model": [_vp, _vp],
     "gcmi_server_submit_rgb": [_vp, _int, _vp, _int, _i64p],
+    "gcmi_gmm_score": [_vp, _vp, _int, ctypes.c_longlong, _vp, _vp],
+    "gcmi_gmm_fit": [_vp, _int, ctypes.c_longlong, _int, _int, ctypes.c_double, ctypes.c_double,
+                     ctypes.c_double, _int, _vp, ctypes.POINTER(ctypes.c_double),
+                     ctypes.POINTER(_int)],
+    "gcmi_cut_upload_rgb_gmm": [_vp, _vp, _int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double],
+    "gcmi_batch_upload_rgb_gmm": [_vp, _int, _vp, _int, _vp, _vp, _vp, ctypes.c_double,
+                                  ctypes.c_double],
 }
 
 REQ_IMAGE, REQ_STROKE, REQ_POLYLINE, REQ_RGB = 0, 1, 2, 3
@@ -171,6 +178,40 @@ def rgb_image(image):
         raise ValueError("expected a [H][W][3] or [H][W][4] uint8 image, got shape %s"
                          % (a.shape,))
     return a, a.shape[2]
+
+
+GMM_MAX_COMPONENTS = 8
+
+
+class Gmm(ctypes.Structure):
+    """struct gcmi_gmm of gcmi.h: a colour mixture with diagonal covariances."""
+    _fields_ = [("k", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("weight", ctypes.c_double * GMM_MAX_COMPONENTS),
+                ("mean", (ctypes.c_double * 3) * GMM_MAX_COMPONENTS),
+                ("var", (ctypes.c_double * 3) * GMM_MAX_COMPONENTS)]
+
+    def params(self):
+        """(weights [k], means [k][3], variances [k][3]) as float64 arrays."""
+        k = self.k
+        return (np.array(self.weight[:k]), np.array([list(m) for m in self.mean[:k]]),
+                np.array([list(v) for v in self.var[:k]]))
+
+
+def gmm(weights, means, var):
+    """A Gmm from k weights and [k][3] means and diagonal variances."""
+    w = np.asarray(weights, np.float64).reshape(-1)
+    mu = np.asarray(means, np.float64).reshape(-1, 3)
+    v = np.asarray(var, np.float64).reshape(-1, 3)
+    if not (1 <= w.size <= GMM_MAX_COMPONENTS) or mu.shape[0] != w.size or v.shape[0] != w.size:
+        raise ValueError("a GMM has 1..%d components with a mean and variance each"
+                         % GMM_MAX_COMPONENTS)
+    g = Gmm()
+    g.k = w.size
+    for i in range(w.size):
+        g.weight[i] = w[i]
+        g.mean[i][:] = [float(x) for x in mu[i]]
+        g.var[i][:] = [float(x) for x in v[i]]
+    return g
 
 
 class Request(ctypes.Structure):

@@ -197,6 +197,54 @@ void gcmi_rgb_model_default(gcmi_rgb_model *model);
  * uint8 image (channels 3 or 4) in host memory (copied). */
 int gcmi_cut_upload_rgb(gcmi_cut *cut, const uint8_t *h_rgb, int channels,
                         const gcmi_rgb_model *model);
+/* ------------------------------------------------------------ colour GMMs
+ * GMM (GMM.pyx:18-420, gmm.cl:33-432): a mixture of k <= 8 Gaussians with
+ * diagonal covariances over a pixel's colour channels 0..2 (values 0..255),
+ * fitted by EM on the device and scored per pixel.  The score of colour x
+ * is its negative log-likelihood, computed in float64 as
+ *   e_i = c_i - ((d_0^2 h_i0 + d_1^2 h_i1) + d_2^2 h_i2),  d_j = x_j - mean_ij,
+ *   c_i = log(weight_i) - 0.5 ((log(2 pi var_i0) + log(2 pi var_i1))
+ *                              + log(2 pi var_i2)),  h_ij = 0.5 / var_ij,
+ *   score = -(m + log(sum_i exp(e_i - m))),  m = max_i e_i,
+ * each operation rounded once, in this order. */
+#define GCMI_GMM_MAX_COMPONENTS 8
+typedef struct gcmi_gmm {
+  int32_t k; /* components, 1..GCMI_GMM_MAX_COMPONENTS */
+  int32_t reserved;
+  double weight[GCMI_GMM_MAX_COMPONENTS];  /* >= 0, sum > 0 */
+  double mean[GCMI_GMM_MAX_COMPONENTS][3];
+  double var[GCMI_GMM_MAX_COMPONENTS][3]; /* diagonal covariance, > 0 */
+} gcmi_gmm;
+/* GMM.score (GMM.pyx:130-163, score_buf / score_img2d gmm.cl:373-432):
+ * d_out[i] = the score of pixel i of n pixels of channels (3 or 4) bytes
+ * each at d_rgb, rounded to float32.  Enqueued on `stream`. */
+int gcmi_gmm_score(const gcmi_gmm *model, const uint8_t *d_rgb, int channels,
+                   long long n, float *d_out, void *stream);
+/* GMM.fit (GMM.pyx:165-366; initA, em1, em2, eval gmm.cl:33-340): EM over
+ * n >= k samples (channels bytes each at d_rgb), in float64.  Start: the
+ * model in *model_io if init_from_model, else weights 1/k, variances
+ * init_covar and means the samples at i * (n - 1) / (k - 1).  Each
+ * iteration: responsibilities r_ij of every sample (E), then weight_i =
+ * sum_j r_ij / n, mean_i = sum_j r_ij x_j / sum_j r_ij, var_i = sum_j
+ * r_ij x_j^2 / sum_j r_ij - mean_i^2 + min_covar (M; a component with no
+ * responsibility keeps its mean and variance).  Stops after `iters`
+ * iterations or when the mean log-likelihood of the E step changed by less
+ * than tol (tol <= 0: never).  *model_io gets the model, *loglik_out (may
+ * be NULL) the last E step's mean log-likelihood, *iters_out (may be NULL)
+ * the iterations run.  Synchronises. */
+int gcmi_gmm_fit(const uint8_t *d_rgb, int channels, long long n, int k, int iters,
+                 double tol, double init_covar, double min_covar, int init_from_model,
+                 gcmi_gmm *model_io, double *loglik_out, int *iters_out);
+/* Capacities built on the device from an 8-bit colour image with GMM
+ * unaries (the reference's tools feed GMM scores to the cut as unaries,
+ * test_GraphCut.py:44-47): source = q(score under bg), sink = q(score under
+ * fg), q(s) = rint(unary_scale * min(max(s, 0), unary_cap)) (half to even);
+ * pairwise weights as gcmi_cut_upload_rgb builds them with `pairwise` (its
+ * unary fields are ignored).  unary_scale * unary_cap <= 2^26. */
+int gcmi_cut_upload_rgb_gmm(gcmi_cut *cut, const uint8_t *h_rgb, int channels,
+                            const gcmi_rgb_model *pairwise, const gcmi_gmm *fg,
+                            const gcmi_gmm *bg, double unary_scale,
+                            double unary_cap);
 /* Copies the stored capacities (as uploaded, built or edited; each array
  * [height][width] int32) to host memory. */
 int gcmi_cut_capacities(gcmi_cut *cut, int32_t *h_source, int32_t *h_sink,
@@ -347,6 +395,11 @@ int gcmi_batch_upload_device(gcmi_batch *batch, int i,
 /* Capacities of image i built from a colour image (as gcmi_cut_upload_rgb). */
 int gcmi_batch_upload_rgb(gcmi_batch *batch, int i, const uint8_t *h_rgb,
                           int channels, const gcmi_rgb_model *model);
+/* Capacities of image i with GMM unaries (as gcmi_cut_upload_rgb_gmm). */
+int gcmi_batch_upload_rgb_gmm(gcmi_batch *batch, int i, const uint8_t *h_rgb,
+                              int channels, const gcmi_rgb_model *pairwise,
+                              const gcmi_gmm *fg, const gcmi_gmm *bg,
+                              double unary_scale, double unary_cap);
 /* Warm-start edit of image i (as gcmi_cut_edit). */
 int gcmi_batch_edit(gcmi_batch *batch, int i, int n, const int32_t *h_index,
                     const int32_t *h_source, const int32_t *h_sink);
