@@ -423,6 +423,20 @@ class GraphCut:
         check(lib().gcmi_cut_geometry(self._h, g))
         return g[0] * g[1]
 
+    def execute(self, queue=None, input="height"):
+        """A state field as an image, the counterpart of the reference's
+        display filter (GraphCut.execute / filter_transpose,
+        GraphCut.pyx:423-437): int32 [PH][PW] row-major over the padded grid
+        of "height", "excess", "up", "down", "left" or "right" (residual
+        capacities toward that neighbour)."""
+        from gcmi_lib import FIELDS
+        if input not in FIELDS:
+            raise ValueError("input: one of %s" % sorted(FIELDS))
+        g = self._geometry()
+        out = np.empty((g[1], g[0]), np.int32)
+        check(lib().gcmi_cut_field(self._h, FIELDS[input], host_ptr(out)))
+        return out
+
     def reset(self):
         """Kept for API parity (GraphCut.pyx:420-421): every cut() starts
         from the uploaded capacities, so there is nothing to reset."""

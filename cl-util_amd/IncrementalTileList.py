@@ -86,6 +86,19 @@ class IncrementalTileList:
         self.iteration = self.init_iteration + 1
         check(lib().gcmi_tiles_fill(ptr(self.d_tiles), self._n, int(self.init_iteration), None))
 
+    def execute(self, queue=None, input=None):
+        """The visualisation filter (IncrementalTileList.pyx:191-208): a
+        Buffer2D of uint32 [H][W] pixels, opaque red (0xFF0000FF) where the
+        pixel's tile in `input` (default: this list) holds input's
+        iteration, opaque black elsewhere."""
+        src = self if input is None else input
+        out = Buffer2D(None, None, src.global_dim, np.uint32)
+        check(lib().gcmi_tiles_filter(ptr(src.d_tiles), src.dim[0], src.dim[1], src.tiles_dim[0],
+                                      src.tiles_dim[1], src.global_dim[0], src.global_dim[1],
+                                      int(src.iteration), ptr(out), None))
+        check(lib().gcmi_stream_sync(None))
+        return out
+
     # host conveniences (the reference reads these with cl.enqueue_copy)
     def tiles(self):
         """Stamps as a [tilesH][tilesW] int32 array."""

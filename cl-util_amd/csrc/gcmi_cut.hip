@@ -273,6 +273,14 @@ int gcmi_cut_heights(gcmi_cut* c, int32_t* h_out) {
   return pool_heights(&c->pool, c->img, h_out);
 }
 
+int gcmi_cut_field(gcmi_cut* c, int which, int32_t* h_out) {
+  if (!c || !h_out) {
+    set_error("null argument");
+    return GCMI_EINVAL;
+  }
+  return pool_field(&c->pool, c->img, which, h_out);
+}
+
 int gcmi_cut_geometry(gcmi_cut* c, int* out3) {
   if (!c || !out3) {
     set_error("null argument");

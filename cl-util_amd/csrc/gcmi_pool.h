@@ -328,6 +328,9 @@ int pool_forget(Pool* p, int img);
 // GCMI_OPT_* options of the C ABI.
 int pool_set_option(Pool* p, int option, int value);
 int pool_heights(Pool* p, int img, int32_t* out);
+// A tile-blocked int32 state field (GCMI_FIELD_*) of an image, row-major
+// over its padded grid, to host memory.
+int pool_field(Pool* p, int img, int which, int32_t* out);
 int validate_caps(const int32_t* cs, const int32_t* ct, const int32_t* right,
                   const int32_t* down, size_t n, int32_t* max_pair);
 int tiles_for(int W, int H, int tile, int* tw, int* th);

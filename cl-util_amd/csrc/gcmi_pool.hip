@@ -2368,8 +2368,22 @@ int pool_forget(Pool* p, int img) {
   return GCMI_OK;
 }
 
-int pool_heights(Pool* p, int img, int32_t* out) {
+int pool_heights(Pool* p, int img, int32_t* out) { return pool_field(p, img, GCMI_FIELD_HEIGHT, out); }
+
+int pool_field(Pool* p, int img, int which, int32_t* out) {
   GCMI_TRY(check_img(p, img));
+  const int32_t* src = nullptr;
+  switch (which) {
+    case GCMI_FIELD_HEIGHT: src = p->dev.h; break;
+    case GCMI_FIELD_EXCESS: src = p->dev.e; break;
+    case GCMI_FIELD_CAP_UP: src = p->dev.cu; break;
+    case GCMI_FIELD_CAP_DOWN: src = p->dev.cd; break;
+    case GCMI_FIELD_CAP_LEFT: src = p->dev.cl; break;
+    case GCMI_FIELD_CAP_RIGHT: src = p->dev.cr; break;
+    default:
+      set_error("unknown state field %d", which);
+      return GCMI_EINVAL;
+  }
   const PoolImage& im = p->imgs[img];
   const size_t np = (size_t)im.ntiles * p->T * p->T;
   int32_t* tmp = nullptr;
@@ -2378,7 +2392,7 @@ int pool_heights(Pool* p, int img, int32_t* out) {
     set_error("allocation failed");
     return GCMI_ENOMEM;
   }
-  p->ops->untile(p, img, p->dev.h, tmp);
+  p->ops->untile(p, img, src, tmp);
   hipError_t e1 = hipMemcpyAsync(out, tmp, np * 4, hipMemcpyDeviceToHost, p->stream);
   hipError_t e2 = hipStreamSynchronize(p->stream);
   (void)hipFree(tmp);

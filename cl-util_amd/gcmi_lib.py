@@ -89,6 +89,7 @@ SIGNATURES = {
     "gcmi_cut_stats": [_vp, _i64p, _int],
     "gcmi_cut_heights": [_vp, _vp],
     "gcmi_cut_geometry": [_vp, ctypes.POINTER(_int)],
+    "gcmi_cut_field": [_vp, _int, _vp],
     "gcmi_cut_profile": [_vp, _int],
     "gcmi_cut_kernel_times": [_vp, ctypes.POINTER(ctypes.c_double), _i64p, _int],
     "gcmi_cut_edit": [_vp, _int, _vp, _vp, _vp],
@@ -130,6 +131,12 @@ SIGNATURES = {
     "gcmi_batch_upload_rgb": [_vp, _int, _vp, _int, _vp],
     "gcmi_server_set_rgb_model": [_vp, _vp],
     "gcmi_server_submit_rgb": [_vp, _int, _vp, _int, _i64p],
+    "gcmi_tiles_filter": [_vp, _int, _int, _int, _int, _int, _int, _i32, _vp, _vp],
+    "gcmi_growcut_create": [_int, _int, _vp, _int, _int, ctypes.POINTER(_vp)],
+    "gcmi_growcut_destroy": [_vp],
+    "gcmi_growcut_label": [_vp, _int, _vp, _int],
+    "gcmi_growcut_evolve": [_vp, _int, ctypes.POINTER(_int), ctypes.POINTER(_int)],
+    "gcmi_growcut_read": [_vp, _vp, _vp, _vp, ctypes.POINTER(_i32)],
     "gcmi_gmm_score": [_vp, _vp, _int, ctypes.c_longlong, _vp, _vp],
     "gcmi_gmm_fit": [_vp, _int, ctypes.c_longlong, _int, _int, ctypes.c_double, ctypes.c_double,
                      ctypes.c_double, _int, _vp, ctypes.POINTER(ctypes.c_double),
@@ -145,6 +152,7 @@ OPT_LOCAL_RELABEL = 1
 OPT_DEVICE_ROUNDS = 2
 OPT_GRAPH = 3
 TILES_ACTIVE, TILES_BORDER, TILES_CHANGED, TILES_EXCESS = 0, 1, 2, 3
+FIELDS = {"height": 0, "excess": 1, "up": 2, "down": 3, "left": 4, "right": 5}
 
 
 class RgbModel(ctypes.Structure):

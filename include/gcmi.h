@@ -141,6 +141,54 @@ int gcmi_tiles_build_logical(const int32_t *d_t1, const int32_t *d_t2, int n,
                              int logical, int32_t *d_list, int32_t *d_length,
                              void *stream);
 
+/* IncrementalTileList's visualisation (IncrementalTileList.pyx:191-208,
+ * filter incremental_tile_list.cl:156-175): d_out[y * width + x] =
+ * 0xFF0000FF if the stamp of the pixel's tile, d_tiles[(y / tile_h) *
+ * tiles_w + x / tile_w] of a tiles_w x tiles_h grid, equals `iteration`,
+ * else 0xFF000000 (also for pixels beyond the grid's last whole tile). */
+int gcmi_tiles_filter(const int32_t *d_tiles, int tiles_w, int tiles_h,
+                      int tile_w, int tile_h, int width, int height,
+                      int32_t iteration, uint32_t *d_out, void *stream);
+
+/* ---------------------------------------------------------------- GrowCut
+ * GrowCut (GrowCut.py:26-153, growcut.cl:28-179): a cellular automaton
+ * over a colour image on 16x16 tiles with an IncrementalTileList.  Each
+ * pixel has a label (0..255) and a strength (float32); an evolve step
+ * updates the tiles stamped with the current iteration synchronously: a
+ * pixel's neighbours up, down, left, right attack in turn with g(|c_p -
+ * c_q|) * strength_q, and an attack above the defence takes over the
+ * neighbour's label and strength (colours c / 255 per channel, alpha
+ * included for 4-channel images; |.| = sqrt(((dr^2 + dg^2) + db^2) + da^2);
+ * g(X) with x = X / sqrt(3): LINEAR 1 - x, POW2 1 - x^2, POW3 1 - x^3,
+ * POW1_5 1 - x sqrt(x), SQRT 1 - sqrt(x); float32, each operation rounded
+ * once, in this order).  A tile that changes, and a tile across a changed
+ * edge pixel, is stamped with the step's iteration. */
+#define GCMI_GROWCUT_WEIGHT_LINEAR 0
+#define GCMI_GROWCUT_WEIGHT_POW2 1
+#define GCMI_GROWCUT_WEIGHT_POW3 2
+#define GCMI_GROWCUT_WEIGHT_POW1_5 3
+#define GCMI_GROWCUT_WEIGHT_SQRT 4
+typedef struct gcmi_growcut gcmi_growcut;
+/* An automaton over a [height][width][channels] uint8 image (copied);
+ * labels and strengths start at 0 and every tile's stamp at -1. */
+int gcmi_growcut_create(int width, int height, const uint8_t *h_rgb,
+                        int channels, int weight, gcmi_growcut **out);
+int gcmi_growcut_destroy(gcmi_growcut *gc);
+/* GrowCut.label (GrowCut.py:110-123): pixels h_points[0..n) (y * width + x)
+ * get `label` and strength 1; their tiles, and the tiles across an edge a
+ * point lies on, the current iteration. */
+int gcmi_growcut_label(gcmi_growcut *gc, int n, const int32_t *h_points,
+                       int label);
+/* GrowCut.evolve (GrowCut.py:126-153): up to `iterations` steps, the list's
+ * length read every few steps; *complete_out = 1 once a step had no tile
+ * to update (nothing changes any more); *steps_out = steps run. */
+int gcmi_growcut_evolve(gcmi_growcut *gc, int iterations, int *complete_out,
+                        int *steps_out);
+/* Current labels (width*height bytes), strengths (float32), tile stamps
+ * (tiles across x down int32) and iteration; any pointer may be NULL. */
+int gcmi_growcut_read(gcmi_growcut *gc, uint8_t *h_labels, float *h_strength,
+                      int32_t *h_tiles, int32_t *iteration_out);
+
 /* ------------------------------------------------------------- grid cut
  * GraphCut (GraphCut.pyx:80-418).  The reference builds its pairwise weights
  * from an RGBA image on the device and takes one float "excess" image; this
@@ -339,6 +387,17 @@ int gcmi_cut_stats(gcmi_cut *cut, int64_t *out, int n);
 /* Device-resident tile-blocked height image (int32, padded) for inspection:
  * copies padded_w*padded_h ints in row-major order into h_out. */
 int gcmi_cut_heights(gcmi_cut *cut, int32_t *h_out);
+/* Any int32 state field of the tile-blocked solver state, row-major over the
+ * padded grid (padded_w*padded_h ints): the counterpart of GraphCut.execute
+ * / filter_transpose (GraphCut.pyx:423-437, graphcut.cl:1134-1169), which
+ * turn the reference's tile-transposed buffers into images for display. */
+#define GCMI_FIELD_HEIGHT 0
+#define GCMI_FIELD_EXCESS 1
+#define GCMI_FIELD_CAP_UP 2
+#define GCMI_FIELD_CAP_DOWN 3
+#define GCMI_FIELD_CAP_LEFT 4
+#define GCMI_FIELD_CAP_RIGHT 5
+int gcmi_cut_field(gcmi_cut *cut, int which, int32_t *h_out);
 /* Padded geometry: out[0]=padded width, out[1]=padded height, out[2]=tile. */
 int gcmi_cut_geometry(gcmi_cut *cut, int *out3);
 /* Per-kernel HIP-event timing of later runs (no reference counterpart; the
