@@ -286,10 +286,7 @@ def verify_served(srv, sids, mix, exp):
     thread; every request's flow and every session's final mask against the
     oracle's answers."""
     import gcmi_serving as S
-    reset = [(0, sids[i]) + S.image_request(mix.base(i), True) for i in range(mix.nsess)]
-    for r in srv.replay(reset):
-        srv.wait(int(r))
-        srv.forget(int(r))
+    S.reset_sessions(srv, sids, mix, rgb=True)
     items = S.verification_trace(mix)
     t0 = time.monotonic_ns() + int(0.1e9)
     ids = srv.replay([(t0 + x[0], sids[x[1]], x[2], x[3]) for x in items])
@@ -341,7 +338,13 @@ def serve_leg(args, exp):
 
     def search(srv, sids, bracket, image_frac, rgb):
         def probe(rate):
+            # every probe starts from the sessions' base images, so that the
+            # strokes earlier probes painted (many, at overloaded rates) do
+            # not carry over, whatever order the bisection probes in
             seed[0] += 1
+            S.reset_sessions(srv, sids, mix, rgb=True)
+            S.run_trace(srv, sids, mix, rate, duration=0.3, seed=1000 + seed[0],
+                        image_frac=image_frac, rgb=rgb)
             pt = S.run_trace(srv, sids, mix, rate, duration=args.serve_seconds, seed=seed[0],
                              image_frac=image_frac, rgb=rgb)
             log("served: %s rate %g: p99(2048) %s ms, throughput %s/s" % (

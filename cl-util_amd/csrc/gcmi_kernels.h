@@ -1315,14 +1315,18 @@ __device__ __forceinline__ void put_bits(uint32_t* packed, long long bitpos,
 
 // Row-major labels of the listed tiles' images from the reach bytes (a
 // tile writes its own T x T block of its image), as bytes and as bits of
-// the bit-packed masks (a tile row's bits gathered by one ballot).
+// the bit-packed masks.  A thread moves four pixels of a tile row: one
+// 4-byte reach load and, where the image's rows are 4-byte aligned, one
+// 4-byte label store; a row's bits are gathered from its T / 4 threads by
+// shuffles and merged into the packed words by the row's first thread.
 template <int T, int NT>
 __global__ void __launch_bounds__(NT)
 k_labels(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count) {
-  constexpr int R = T * T / NT;
-  constexpr unsigned long long seg = T == 64 ? ~0ull : ((1ull << T) - 1);
-  const int lx = threadIdx.x % T, chain = threadIdx.x / T;
-  const int sh = lane_id() & ~(T - 1) & 63;  // this row segment's first lane
+  constexpr int TPR = T / 4;  // threads per tile row (an aligned group of lanes)
+  constexpr int ROWS = NT / TPR;  // rows per pass
+  constexpr int PASSES = (T + ROWS - 1) / ROWS;
+  static_assert(kWave % TPR == 0, "a row's threads lie in one wave");
+  const int j = threadIdx.x % TPR, r0 = threadIdx.x / TPR;
   const int n = *count;
   const int p0 = xcd_pos(blockIdx.x, gridDim.x);
   int tnext = p0 < n ? list_tile(list, p0) : 0;
@@ -1330,17 +1334,44 @@ k_labels(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __res
     const TG q = tile_geo(g, tnext);
     if (pos + (int)gridDim.x < n) tnext = list_tile(list, pos + gridDim.x);
     const size_t base = tbase<T>(q.t);
+    const bool aligned = (q.W & 3) == 0;  // (pix0 is a multiple of 32)
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int ly = chain * R + k;
-      const int x = q.tx * T + lx, y = q.ty * T + ly;
-      const bool in = x < q.W && y < q.H;
-      const uint8_t v = s.reach[base + ly * T + lx];
-      if (in) s.labels[q.pix0 + (size_t)y * q.W + x] = v;
-      const unsigned long long bits = (__ballot(in && v) >> sh) & seg;
-      const unsigned long long mask = (__ballot(in) >> sh) & seg;
-      if (lx == 0 && mask)
-        put_bits(s.packed, q.pix0 + (long long)y * q.W + q.tx * T, bits, mask);
+    for (int ps = 0; ps < PASSES; ++ps) {
+      const int ly = r0 + ps * ROWS;
+      const bool row = ly < T;
+      const int x = q.tx * T + 4 * j, y = q.ty * T + ly;
+      const uint32_t v4 = row ? *reinterpret_cast<const uint32_t*>(s.reach + base + ly * T + 4 * j) : 0u;
+      const int nin = (row && y < q.H) ? min(max(q.W - x, 0), 4) : 0;
+      uint8_t* dst = s.labels + q.pix0 + (size_t)y * q.W + x;
+      if (nin == 4 && aligned) {
+        *reinterpret_cast<uint32_t*>(dst) = v4;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (k < nin) dst[k] = (uint8_t)(v4 >> (8 * k));
+      }
+      uint32_t nib = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < nin && ((v4 >> (8 * k)) & 0xffu)) nib |= 1u << k;
+      const uint32_t nm = (1u << nin) - 1u;
+      // the row's bits: lane j's nibble at bit 4j (T = 64: two 32-bit halves)
+      const int sh = (4 * j) & 31;
+      const bool hi = 4 * j >= 32;
+      uint32_t blo = hi ? 0u : nib << sh, bhi = hi ? nib << sh : 0u;
+      uint32_t mlo = hi ? 0u : nm << sh, mhi = hi ? nm << sh : 0u;
+#pragma unroll
+      for (int off = TPR / 2; off > 0; off >>= 1) {
+        blo |= __shfl_xor(blo, off, kWave);
+        mlo |= __shfl_xor(mlo, off, kWave);
+        if constexpr (T == 64) {
+          bhi |= __shfl_xor(bhi, off, kWave);
+          mhi |= __shfl_xor(mhi, off, kWave);
+        }
+      }
+      const unsigned long long b = ((unsigned long long)bhi << 32) | blo;
+      const unsigned long long m = ((unsigned long long)mhi << 32) | mlo;
+      if (j == 0 && m) put_bits(s.packed, q.pix0 + (long long)y * q.W + q.tx * T, b, m);
     }
   }
 }

@@ -57,11 +57,18 @@ def image_request(work, rgb):
 def open_sessions(srv, mix, rgb=False):
     """Opens every session and cuts its first image; returns the ids."""
     sids = [srv.open(n, n) for n in mix.sizes]
+    reset_sessions(srv, sids, mix, rgb)
+    return sids
+
+
+def reset_sessions(srv, sids, mix, rgb=False):
+    """Gives every session its first base image again (a known state: the
+    strokes of earlier traces are gone) and waits for the cuts."""
     trace = [(0, sids[i]) + image_request(mix.base(i), rgb) for i in range(mix.nsess)]
     for r in srv.replay(trace):
         srv.wait(int(r))
         srv.forget(int(r))
-    return sids
+    mix.painted[:] = 0
 
 
 def make_trace(mix, rate, duration, seed, t0_ns, image_frac=0.1, rgb=False):
