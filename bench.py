@@ -93,6 +93,9 @@ def parse():
                     help="CPU baseline threads (0: the usable cores, at most 16)")
     ap.add_argument("--serve-max-batch", type=int, default=64,
                     help="most requests per served solve")
+    ap.add_argument("--serve-inline", action="store_true",
+                    help="run the serving leg in this process (default: a child process)")
+    ap.add_argument("--serve-child", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
 
@@ -374,8 +377,16 @@ def serve_leg(args, exp):
             parity[mode] = verify_served(srv, sids, mix, exp)
         # the closed server's device memory is released (and cleared by the
         # driver) in the background: let that finish before the next mode
-        time.sleep(3.0)
+        if mode != list(modes)[-1]:
+            time.sleep(3.0)
 
+    summarise(out)
+    return out, parity
+
+
+def summarise(out):
+    """The sustained stroke rates of the served modes and their ratios to
+    the plain mode's."""
     def rate(entry):
         return (entry["sustained_2048_p99_20ms"] or {}).get("stroke_rate")
 
@@ -388,7 +399,6 @@ def serve_leg(args, exp):
     out["stroke_only_sustained_stroke_rate"] = so
     if so.get("plain") and so.get("batched"):
         out["stroke_only_served_vs_plain"] = round(so["batched"] / so["plain"], 2)
-    return out, parity
 
 
 # ---------------------------------------------------------------- batched
@@ -545,12 +555,44 @@ def cpu_baseline(seconds, threads):
             "host": info}
 
 
+def serve_in_child(args):
+    """Each serving mode in a child process of its own, as a server runs:
+    its HIP context holds only that server's streams.  (A process's streams
+    share the device's four hardware queues.  With another stream in the
+    process -- the null stream of earlier copies, or a server created and
+    closed before -- the served lanes were measured to lose about a third
+    of their throughput.)"""
+    import subprocess
+    out, parity = None, {}
+    for mode in [m for m in args.serve_modes.split(",") if m]:
+        cmd = [sys.executable, os.path.abspath(__file__), "--serve-child",
+               "--serve-seconds", str(args.serve_seconds), "--serve-modes", mode,
+               "--serve-image-frac", str(args.serve_image_frac),
+               "--serve-max-batch", str(args.serve_max_batch)]
+        if args.serve_rates:
+            cmd += ["--serve-rates", args.serve_rates]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, check=True)
+        d = json.loads(r.stdout.strip().splitlines()[-1])
+        if out is None:
+            out = d["served"]
+        else:
+            out["modes"].update(d["served"]["modes"])
+            out["stroke_only"].update(d["served"]["stroke_only"])
+        parity.update(d["parity"])
+    summarise(out)
+    return out, parity
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X; "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         return 2
+    if args.serve_child:
+        served, parity = serve_leg(args, load_expected()["served"])
+        print(json.dumps({"served": served, "parity": parity}), flush=True)
+        return 0
     import gcmi_serving as S
     import gcmi_workloads as G
     from GraphCut import GraphCut
@@ -604,7 +646,10 @@ def main():
     if not args.no_warm:
         out["warm"], parity["warm_2048"] = warm_leg(args, exp["warm_2048"])
     if not args.no_serve:
-        out["served"], parity["served"] = serve_leg(args, exp["served"])
+        if args.serve_inline:
+            out["served"], parity["served"] = serve_leg(args, exp["served"])
+        else:
+            out["served"], parity["served"] = serve_in_child(args)
     if not args.no_batch:
         out["batched"], parity["batched_64"] = batched_leg(args, exp["batched_64"])
     if not args.no_cpu:

@@ -29,7 +29,7 @@ from gcmi_lib import (EBUSY, OPT_DEVICE_ROUNDS, OPT_LOCAL_RELABEL, REQ_IMAGE, RE
 QUEUED, DONE, FAILED = 0, 1, 2
 MODES = {"batched": 0, "plain": 1, "cold": 2}
 STAT_NAMES = ("solves", "served", "queued", "sessions", "tiles_used", "tiles_capacity",
-              "device_bytes", "ns_apply", "ns_solve", "ns_out", "lanes")
+              "device_bytes", "ns_apply", "ns_solve", "ns_out", "lanes", "ns_wait")
 
 
 class RequestInfo:

@@ -211,6 +211,7 @@ struct Pool {
   size_t device_bytes = 0;
   int grid_cap = 2048;
   int64_t stats[STAT_N] = {0};
+  int64_t ns_wait = 0;  // host time spent waiting for the stream (all solves so far)
   Prof prof;
   // a solve driven step by step (pool_step_*): its image (-1: none), whether
   // it started from scratch, its in-tile sweeps, the last step, the active

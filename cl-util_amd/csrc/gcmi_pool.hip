@@ -22,6 +22,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstddef>
 
@@ -749,7 +750,10 @@ size_t pool_bytes(int tile, int cap_tiles, long long cap_pix, int cap_imgs) {
 
 // Synchronises the pool's stream: every staging copy queued on it is done.
 static int stream_sync(Pool* p) {
+  const auto t0 = std::chrono::steady_clock::now();
   GCMI_HIP(hipStreamSynchronize(p->stream));
+  p->ns_wait += std::chrono::duration_cast<std::chrono::nanoseconds>(
+                    std::chrono::steady_clock::now() - t0).count();
   for (bool& b : p->stage_pending) b = false;
   return GCMI_OK;
 }

@@ -61,6 +61,16 @@ int require_device() {
   return GCMI_OK;
 }
 
+// The library's stream if it was created already, else null (no stream is
+// created: every extra stream takes a share of the device's few hardware
+// queues, which concurrent solvers in the same process need).
+static hipStream_t existing_default_stream() {
+  DeviceState* st = nullptr;
+  if (current_state(&st) != GCMI_OK) return nullptr;
+  std::lock_guard<std::mutex> lk(st->mu);
+  return st->stream;
+}
+
 hipStream_t default_stream() {
   DeviceState* st = nullptr;
   if (current_state(&st) != GCMI_OK) return nullptr;
@@ -150,9 +160,15 @@ int gcmi_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes) {
   GCMI_TRY(require_device());
   if (!bytes) return GCMI_OK;
   if (!d_dst || !h_src) { set_error("null pointer"); return GCMI_EINVAL; }
-  hipStream_t s = default_stream();
-  GCMI_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, s));
-  GCMI_HIP(hipStreamSynchronize(s));
+  // ordered after the primitives enqueued on the library's stream, if any;
+  // a synchronous copy otherwise
+  hipStream_t s = existing_default_stream();
+  if (s) {
+    GCMI_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, s));
+    GCMI_HIP(hipStreamSynchronize(s));
+  } else {
+    GCMI_HIP(hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice));
+  }
   return GCMI_OK;
 }
 
@@ -160,9 +176,15 @@ int gcmi_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes) {
   GCMI_TRY(require_device());
   if (!bytes) return GCMI_OK;
   if (!h_dst || !d_src) { set_error("null pointer"); return GCMI_EINVAL; }
-  hipStream_t s = default_stream();
-  GCMI_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, s));
-  GCMI_HIP(hipStreamSynchronize(s));
+  // ordered after the primitives enqueued on the library's stream, if any;
+  // a synchronous copy otherwise
+  hipStream_t s = existing_default_stream();
+  if (s) {
+    GCMI_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, s));
+    GCMI_HIP(hipStreamSynchronize(s));
+  } else {
+    GCMI_HIP(hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost));
+  }
   return GCMI_OK;
 }
 
@@ -170,9 +192,15 @@ int gcmi_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes) {
   GCMI_TRY(require_device());
   if (!bytes) return GCMI_OK;
   if (!d_dst || !d_src) { set_error("null pointer"); return GCMI_EINVAL; }
-  hipStream_t s = default_stream();
-  GCMI_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, s));
-  GCMI_HIP(hipStreamSynchronize(s));
+  // ordered after the primitives enqueued on the library's stream, if any;
+  // a synchronous copy otherwise
+  hipStream_t s = existing_default_stream();
+  if (s) {
+    GCMI_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, s));
+    GCMI_HIP(hipStreamSynchronize(s));
+  } else {
+    GCMI_HIP(hipMemcpy(d_dst, d_src, bytes, hipMemcpyDeviceToDevice));
+  }
   return GCMI_OK;
 }
 
@@ -184,7 +212,10 @@ int gcmi_device_sync(void) {
 
 int gcmi_stream_sync(void* stream) {
   GCMI_TRY(require_device());
-  GCMI_HIP(hipStreamSynchronize(pick_stream(stream)));
+  // NULL: the library's stream, if it exists (nothing was enqueued on it
+  // otherwise)
+  hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : existing_default_stream();
+  if (s) GCMI_HIP(hipStreamSynchronize(s));
   return GCMI_OK;
 }
 

@@ -1036,9 +1036,11 @@ int gcmi_server_stats(gcmi_server* s, int64_t* out, int n) {
     ns_solve += L.ns_solve;
     ns_out += L.ns_out;
   }
-  const int64_t v[11] = {s->iterations, s->solved_requests, (int64_t)s->queue.size(), open, used,
-                         cap, bytes, ns_apply, ns_solve, ns_out, s->nlanes};
-  for (int k = 0; k < n && k < 11; ++k) out[k] = v[k];
+  int64_t ns_wait = 0;
+  for (int i = 0; i < s->nlanes; ++i) ns_wait += s->lanes[i].pool.ns_wait;
+  const int64_t v[12] = {s->iterations, s->solved_requests, (int64_t)s->queue.size(), open, used,
+                         cap, bytes, ns_apply, ns_solve, ns_out, s->nlanes, ns_wait};
+  for (int k = 0; k < n && k < 12; ++k) out[k] = v[k];
   return GCMI_OK;
 }
 

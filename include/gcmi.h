@@ -588,7 +588,8 @@ int gcmi_server_mask(gcmi_server *server, int session, uint32_t *h_bits,
 /* [0] solves  [1] requests served  [2] queued  [3] open sessions
  * [4] pool tiles in use  [5] pool tile capacity  [6] device bytes
  * [7] [8] [9] worker ns applying requests, solving, listing mask copies
- * (summed over lanes)  [10] lanes */
+ * (summed over lanes)  [10] lanes  [11] worker ns waiting for the device
+ * (inside the solves' synchronisations; the rest of [8] is host work) */
 int gcmi_server_stats(gcmi_server *server, int64_t *out, int n);
 /* Drops the record of a completed request. */
 int gcmi_server_forget(gcmi_server *server, int64_t req);
