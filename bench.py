@@ -501,9 +501,9 @@ def batched_leg(args, exp):
 
 
 def host_cpu():
-    """nproc (the cores this process may use), the CPU model and whether an
-    OpenCL CPU runtime (an ICD entry or a pocl library) is installed, which
-    running the reference's own kernels on the host would need."""
+    """nproc (the cores this process may use), the CPU model, the OpenCL
+    ICDs installed and any pocl library (the OpenCL CPU runtime that running
+    the reference's own kernels on the host would need)."""
     model = None
     try:
         with open("/proc/cpuinfo") as f:
@@ -544,14 +544,20 @@ def cpu_baseline(seconds, threads):
     with cf.ThreadPoolExecutor(threads) as ex:
         n = sum(ex.map(worker, range(threads)))
     dt = time.perf_counter() - t0
-    runtime = "absent" if not (info["opencl_icds"] or info["pocl_libraries"]) else "present"
+    # The reference's kernels would need an OpenCL CPU device (pocl); an
+    # installed ICD of AMD's OpenCL (amdocl64) exposes GPUs only.
+    if info["pocl_libraries"]:
+        runtime = "pocl is installed on this host but not used"
+    else:
+        runtime = "no OpenCL CPU runtime (pocl) on this host"
+        if info["opencl_icds"]:
+            runtime += "; the OpenCL ICDs present are GPU runtimes"
     return {"value": round(n * w.pixels / dt / 1e6, 3), "unit": "Mpix/s", "cores": threads,
             "kind": "port",
             "sample": "%d cuts of the 512x512 centred-ellipse workload (seed 0) by the "
                       "oracle's C restatement of the reference push-relabel, %d threads of "
-                      "independent cuts, %.1f s (oracle CPU, not the reference: an OpenCL "
-                      "CPU runtime for graphcut.cl is %s on this host)" % (n, threads, dt,
-                                                                            runtime),
+                      "independent cuts, %.1f s (oracle CPU, not the reference: %s)"
+                      % (n, threads, dt, runtime),
             "host": info}
 
 

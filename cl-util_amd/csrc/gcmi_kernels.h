@@ -622,62 +622,84 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
 }
 
 // Folds the pending inflow of every flagged tile into its state (between
-// launches, when no tile is resident): e, the reverse capacity, the mask
-// byte and the tile's positive-excess sum.  One block per listed tile
-// (unflagged tiles exit at once); one owner thread per edge pixel (threads
-// < 4T - 4 walk the perimeter), so corners are not raced.  The block is the
-// perimeter rounded up to whole waves: 64, 128 and 256 threads at T = 16,
-// 32 and 64.
+// launches, when no tile is resident): e, the reverse capacity of each side
+// that received inflow, the mask byte and the tile's positive-excess sum.
+// A block walks listed tiles; thread j < 4T - 4 owns perimeter pixel j (the
+// top and bottom rows, then the left and right columns), so corners are not
+// raced.  Inflow only raises a residual, so the mask's residual bits are set
+// for the sides that received some and kept for the others, and only the
+// one residual of each receiving side is read and written.  Thread 0 alone
+// reads and clears the tiles' flags, the next tile's flag loading while the
+// block folds the current one.  The block is the perimeter rounded up to
+// whole waves: 64, 128 and 256 threads at T = 16, 32 and 64.  (One wave per
+// tile, with four tiles per block, was measured 2x slower on the 4096^2
+// cut, and a grid of twice the blocks 40% slower: the scattered column
+// accesses, not latency, bound the kernel.)
 template <int T>
 constexpr int kFoldNT = (4 * T - 4 + 63) / 64 * 64;
 
 template <int T>
-__device__ __forceinline__ long long fold_edge_pixel(CutDev& s, int t, int j) {
-  int x, y;
+__global__ void __launch_bounds__(kFoldNT<T>)
+k_fold(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count) {
+  constexpr int P = 4 * T - 4;
+  __shared__ int sflag[2];
+  const int n = *count;
+  const int j = threadIdx.x;
+  // this thread's pixel, and the pending slot of its vertical side (top /
+  // bottom row) and of its horizontal side (left / right column), -1 where
+  // it has none; each slot belongs to one pixel
+  int x = 0, y = 0;
   if (j < T) { x = j; y = 0; }
   else if (j < 2 * T) { x = j - T; y = T - 1; }
   else if (j < 3 * T - 2) { x = 0; y = j - 2 * T + 1; }
-  else { x = T - 1; y = j - (3 * T - 2) + 1; }
-  const size_t i = tbase<T>(t) + y * T + x;
-  int32_t* pe = s.pend + (size_t)t * 4 * T;
-  const int vu = y == 0 ? pe[0 * T + x] : 0;
-  const int vd = y == T - 1 ? pe[1 * T + x] : 0;
-  const int vl = x == 0 ? pe[2 * T + y] : 0;
-  const int vr = x == T - 1 ? pe[3 * T + y] : 0;
-  if ((vu | vd | vl | vr) == 0) return 0;
-  const int e0 = s.e[i];
-  const int e1 = e0 + vu + vd + vl + vr;
-  const int cu = s.cu[i] + vu, cd = s.cd[i] + vd, cl = s.cl[i] + vl, cr = s.cr[i] + vr;
-  s.e[i] = e1;
-  s.cu[i] = cu;
-  s.cd[i] = cd;
-  s.cl[i] = cl;
-  s.cr[i] = cr;
-  s.mask[i] = (uint8_t)pixel_mask(e1, cu, cd, cl, cr);
-  if (vu) pe[0 * T + x] = 0;
-  if (vd) pe[1 * T + x] = 0;
-  if (vl) pe[2 * T + y] = 0;
-  if (vr) pe[3 * T + y] = 0;
-  return (long long)(e1 > 0 ? e1 : 0) - (e0 > 0 ? e0 : 0);
-}
-
-template <int T>
-__global__ void __launch_bounds__(kFoldNT<T>)
-k_fold(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count) {
-  static_assert(4 * T - 4 <= kFoldNT<T>, "perimeter owners");
-  const int n = *count;
-  const int j = threadIdx.x;
+  else if (j < P) { x = T - 1; y = j - (3 * T - 2) + 1; }
+  const int off = y * T + x;
+  const int vs = j >= P ? -1 : (y == 0 ? x : (y == T - 1 ? T + x : -1));
+  const int hs = j >= P ? -1 : (x == 0 ? 2 * T + y : (x == T - 1 ? 3 * T + y : -1));
+  const bool up = y == 0, lf = x == 0;
   const int p0 = xcd_pos(blockIdx.x, gridDim.x);
   int tnext = p0 < n ? list_tile(list, p0) : 0;
-  for (int pos = p0; pos < n; pos += gridDim.x) {
+  if (j == 0) sflag[0] = p0 < n ? s.pflag[tnext] : 0;
+  int par = 0;
+  for (int pos = p0; pos < n; pos += gridDim.x, par ^= 1) {
     const int t = tnext;
-    if (pos + (int)gridDim.x < n) tnext = list_tile(list, pos + gridDim.x);
-    const int flagged = s.pflag[t];
-    __syncthreads();  // every thread has read the flag before it is cleared
+    const bool more = pos + (int)gridDim.x < n;
+    if (more) tnext = list_tile(list, pos + gridDim.x);
+    __syncthreads();  // sflag[par] is written; every thread has read sflag[par ^ 1]
+    const int flagged = sflag[par];
+    if (j == 0) {
+      if (flagged) s.pflag[t] = 0;
+      sflag[par ^ 1] = more ? s.pflag[tnext] : 0;
+    }
     if (!flagged) continue;
-    if (j == 0) s.pflag[t] = 0;
+    int32_t* pe = s.pend + (size_t)t * 4 * T;
+    const size_t b = tbase<T>(t);
+    const int sv = vs >= 0 ? pe[vs] : 0;
+    const int sh = hs >= 0 ? pe[hs] : 0;
+    // unconditional state loads (a thread with nothing to fold reads the
+    // tile's first pixel, whose line other threads read anyway), so that
+    // no select waits for a load
+    const size_t i = (sv | sh) ? b + off : b;
+    const size_t iv = sv ? i : b, ih = sh ? i : b;
+    const int e0 = s.e[i];
+    const int m0 = s.mask[i];
+    const int cv = (up ? s.cu : s.cd)[iv];
+    const int ch = (lf ? s.cl : s.cr)[ih];
+    long long delta = 0;
+    if (sv | sh) {
+      const int e1 = e0 + sv + sh;
+      s.e[i] = e1;
+      if (sv) (up ? s.cu : s.cd)[i] = cv + sv;
+      if (sh) (lf ? s.cl : s.cr)[i] = ch + sh;
+      const int m = (m0 & (M_UP | M_DOWN | M_LEFT | M_RIGHT)) |
+                    (sv ? (up ? M_UP : M_DOWN) : 0) | (sh ? (lf ? M_LEFT : M_RIGHT) : 0) |
+                    (e1 < 0 ? M_SINK : 0) | (e1 > 0 ? M_EXCESS : 0);
+      s.mask[i] = (uint8_t)m;
+      if (sv) pe[vs] = 0;
+      if (sh) pe[hs] = 0;
+      delta = (long long)(e1 > 0 ? e1 : 0) - (e0 > 0 ? e0 : 0);
+    }
     // the change of the tile's positive-excess sum, one atomic per wave
-    long long delta = j < 4 * T - 4 ? fold_edge_pixel<T>(s, t, j) : 0;
     delta = wave_sum64(delta);
     if (lane_id() == 0 && delta)
       atomicAdd(reinterpret_cast<unsigned long long*>(&s.tpos[t]), (unsigned long long)delta);
@@ -768,20 +790,6 @@ __device__ __forceinline__ int bfs_news(const TG& q, int lx, int ly, int d, bool
     const int h = hin[halo_j<T>(side, lx, ly)];
     if ((h & 1) && !((h & 2) && sink) && d + 1 < D[halo_idx<T>(side, lx, ly)]) dirs |= 1 << side;
   }
-  return dirs;
-}
-
-// Flood news for the neighbours of newly reached edge pixel (lx, ly) with
-// arc bits m: a side counts if the arc across it is residual and the halo
-// pixel there is not known to be reached (RM bit 0; reach only grows, so a
-// stale 0 can only make a side count needlessly).
-template <int T>
-__device__ __forceinline__ int flood_news(const TG& q, int lx, int ly, int m, const int32_t* RM) {
-  const int e = edge_dirs<T>(q, lx, ly) & m;
-  int dirs = 0;
-#pragma unroll
-  for (int side = 0; side < 4; ++side)
-    if (((e >> side) & 1) && !(RM[halo_idx<T>(side, lx, ly)] & 1)) dirs |= 1 << side;
   return dirs;
 }
 
@@ -1081,91 +1089,6 @@ k_bfs_round(Geo g, CutDev s, Queue Q) {
 // source set once no pixel with e > 0 can reach a sink.  Same tile scheme
 // as the BFS: in-tile fixed point in LDS, flags across edges.
 
-// Reach along one row (T consecutive lanes): a reached pixel reaches its
-// neighbours along straight runs of residual arcs.  With the row's reach and
-// arc bits as masks (bit x = column x) this is a carry-style fill: per
-// doubling step s, G |= (G << s) & P and P &= P << s, where P marks the
-// columns whose arc from the left (for the rightward fill) is residual.
-template <int T>
-__device__ __forceinline__ int row_reach(int r, int m, int lx) {
-  constexpr unsigned long long seg = T == 64 ? ~0ull : ((1ull << T) - 1);
-  const int base = lane_id() & ~(T - 1) & 63;
-  const unsigned long long g0 = (__ballot(r != 0) >> base) & seg;
-  unsigned long long pr = ((__ballot((m & M_RIGHT) != 0) >> base) << 1) & seg;  // arc x-1 -> x
-  unsigned long long pl = ((__ballot((m & M_LEFT) != 0) >> base) & seg) >> 1;   // arc x+1 -> x
-  unsigned long long gr = g0, gl = g0;
-#pragma unroll
-  for (int s = 1; s < T; s <<= 1) {
-    gr |= (gr << s) & pr;
-    pr &= pr << s;
-    gl |= (gl >> s) & pl;
-    pl &= pl >> s;
-  }
-  return (int)(((gr | gl) & seg) >> lx) & 1;
-}
-
-// Relaxes r[] to the in-tile fixed point.  RM holds (reach | mask << 1) of
-// every tile pixel and halo pixel (halo: the neighbour's reach and mask).
-// Each iteration: one step from the LDS copy (across chains and from the
-// halo), then sweeps along the thread's column segment and fills along
-// every row.
-template <int T, int NT>
-__device__ __forceinline__ int flood_relax(int (&r)[T * T / NT], const int (&m)[T * T / NT],
-                                          int32_t* RM) {
-  constexpr int R = T * T / NT;
-  constexpr int HP = T + 2;
-  const int lx = threadIdx.x % T, chain = threadIdx.x / T;
-  for (int it = 0;; ++it) {
-    int changed = 0;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      if (r[k]) continue;
-      const int ly = chain * R + k;
-      const int c = (ly + 1) * HP + lx + 1;
-      // a neighbour q reaches p if q is reached and q's arc toward p is residual
-      const int up = RM[c - HP], dn = RM[c + HP], lf = RM[c - 1], rt = RM[c + 1];
-      if (((up & 1) && (up & (M_DOWN << 1))) || ((dn & 1) && (dn & (M_UP << 1))) ||
-          ((lf & 1) && (lf & (M_RIGHT << 1))) || ((rt & 1) && (rt & (M_LEFT << 1)))) {
-        r[k] = 1;
-        changed = 1;
-      }
-    }
-    // column segment, both ways
-#pragma unroll
-    for (int k = 1; k < R; ++k)
-      if (!r[k] && r[k - 1] && (m[k - 1] & M_DOWN)) {
-        r[k] = 1;
-        changed = 1;
-      }
-#pragma unroll
-    for (int k = R - 2; k >= 0; --k)
-      if (!r[k] && r[k + 1] && (m[k + 1] & M_UP)) {
-        r[k] = 1;
-        changed = 1;
-      }
-    // rows (as in bfs_relax, after the first iterations)
-    if (it >= kScanAfter) {
-#pragma unroll
-      for (int k = 0; k < R; ++k) {
-        const int v = row_reach<T>(r[k], m[k], lx);
-        if (v && !r[k]) {
-          r[k] = 1;
-          changed = 1;
-        }
-      }
-    }
-    // publish new reach bits (only ever set, so a reader racing the update
-    // sees 0 or 1, both consistent with the monotone fixed point)
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int ly = chain * R + k;
-      const int c = (ly + 1) * HP + lx + 1;
-      if (r[k]) RM[c] |= 1;
-    }
-    if (!__syncthreads_or(changed)) return it + 1;
-  }
-}
-
 // mode FLOOD_FIRST: over the tiles holding excess, reach bytes all cleared:
 // seeds from the masks' excess bit, no halo reach, no import.
 // FLOOD_FIRST_INC: over listed tiles whose reach bytes were cleared (other
@@ -1174,87 +1097,246 @@ __device__ __forceinline__ int flood_relax(int (&r)[T * T / NT], const int (&m)[
 // own excess reaches through its own arcs (no halo).  FLOOD_ROUND: over
 // queued tiles, from their stored reach bytes, halo reach; a tile that
 // gains reach has imported it.  A pixel newly reached on an edge with a
-// residual arc across it queues that neighbour.  lmark (may be null): every
-// processed tile is flagged there (its labels are rewritten after).
+// residual arc across it queues that neighbour, unless the halo pixel there
+// is reached already.  lmark (may be null): every processed tile is flagged
+// there (its labels are rewritten after).
 enum { FLOOD_ROUND = 0, FLOOD_FIRST = 1, FLOOD_FIRST_INC = 2 };
 
+// The flood works on bit rows: lane y of a T-lane segment holds row y of
+// one tile as a T-bit word (reach, and each arc kind of the mask), so a
+// wave holds 64 / T tiles.  Reach spreads along rows by Kogge-Stone steps
+// on the words (shifts) and down / up the columns by the same steps across
+// the segment's lanes (shuffles); the fills alternate until no row changes.
+template <int T>
+struct FloodRow {
+  using type = unsigned;
+};
+template <>
+struct FloodRow<64> {
+  using type = unsigned long long;
+};
+
+// Bit k of each of w's four bytes, as four bits.
+__device__ __forceinline__ unsigned byte_bits(unsigned w, int k) {
+  return (((w >> k) & 0x01010101u) * 0x01020408u) >> 24;
+}
+// Four bits as four bytes of 0 / 1.
+__device__ __forceinline__ unsigned bits_bytes(unsigned n) {
+  return (n * 0x00204081u) & 0x01010101u;
+}
+
+// Row y of a tile's byte array (T bytes, 16-byte aligned) as bit rows of
+// bits k0 .. k0 + NK - 1 of each byte.
+template <int T, int NK>
+__device__ __forceinline__ void load_bit_rows(const uint8_t* row, int k0,
+                                              typename FloodRow<T>::type (&out)[NK]) {
+  using Row = typename FloodRow<T>::type;
+#pragma unroll
+  for (int k = 0; k < NK; ++k) out[k] = 0;
+#pragma unroll
+  for (int c = 0; c < T / 16; ++c) {
+    const uint4 v = reinterpret_cast<const uint4*>(row)[c];
+    const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int k = 0; k < NK; ++k) out[k] |= (Row)byte_bits(w[i], k0 + k) << (16 * c + 4 * i);
+  }
+}
+
+template <int T>
+__device__ __forceinline__ void store_bit_row(uint8_t* row, typename FloodRow<T>::type r) {
+#pragma unroll
+  for (int c = 0; c < T / 16; ++c) {
+    const unsigned n = (unsigned)(r >> (16 * c));
+    reinterpret_cast<uint4*>(row)[c] = make_uint4(bits_bytes(n & 15), bits_bytes((n >> 4) & 15),
+                                                  bits_bytes((n >> 8) & 15),
+                                                  bits_bytes((n >> 12) & 15));
+  }
+}
+
+template <typename Row>
+__device__ __forceinline__ Row seg_shfl_up(Row v, int d, int T) {
+  return __shfl_up(v, d, T);
+}
+template <typename Row>
+__device__ __forceinline__ Row seg_shfl_down(Row v, int d, int T) {
+  return __shfl_down(v, d, T);
+}
+
+// Reach of the segment's rows closed under the arcs: g seeds; rt, lf, dn,
+// up the rows' arc bits (bit x of rt: arc x -> x+1 residual, ...); y the
+// lane's row.  Every lane of the wave calls it (the loop is wave-uniform).
+template <int T>
+__device__ __forceinline__ typename FloodRow<T>::type flood_fill(typename FloodRow<T>::type g,
+                                                                 typename FloodRow<T>::type rt,
+                                                                 typename FloodRow<T>::type lf,
+                                                                 typename FloodRow<T>::type dn,
+                                                                 typename FloodRow<T>::type up,
+                                                                 int y) {
+  using Row = typename FloodRow<T>::type;
+  constexpr Row SEG = T == 64 ? ~0ull : (Row)((1ull << T) - 1);
+  // into x from x - 1 / x + 1; into row y from row y - 1 / y + 1
+  const Row pr0 = (rt << 1) & SEG, pl0 = lf >> 1;
+  Row qd0 = seg_shfl_up(dn, 1, T), qu0 = seg_shfl_down(up, 1, T);
+  if (y == 0) qd0 = 0;
+  if (y == T - 1) qu0 = 0;
+  for (;;) {
+    const Row before = g;
+    Row gr = g, gl = g, pr = pr0, pl = pl0;
+#pragma unroll
+    for (int s = 1; s < T; s <<= 1) {
+      gr |= (gr << s) & pr;
+      pr &= pr << s;
+      gl |= (gl >> s) & pl;
+      pl &= pl >> s;
+    }
+    g = (gr | gl) & SEG;
+    Row gd = g, gu = g, qd = qd0, qu = qu0;
+#pragma unroll
+    for (int s = 1; s < T; s <<= 1) {
+      Row a = seg_shfl_up(gd, s, T), b = seg_shfl_up(qd, s, T);
+      Row c = seg_shfl_down(gu, s, T), d = seg_shfl_down(qu, s, T);
+      if (y < s) a = b = 0;
+      if (y + s >= T) c = d = 0;
+      gd |= a & qd;
+      qd &= b;
+      gu |= c & qu;
+      qu &= d;
+    }
+    g = gd | gu;
+    if (!__any(g != before)) return g;
+  }
+}
+
+constexpr int kFloodNT = 256;
+
 template <int T, int NT>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(kFloodNT)
 k_flood(Geo g, CutDev s, Queue Q, int mode, int32_t* lmark) {
+  using Row = typename FloodRow<T>::type;
+  constexpr int TPW = kWave / T;                 // tiles per wave
+  constexpr int TPB = TPW * (kFloodNT / kWave);  // tiles per block
+  constexpr Row SEG = T == 64 ? ~0ull : (Row)((1ull << T) - 1);
+  constexpr Row TOP = (Row)1 << (T - 1);
   const bool first = mode != FLOOD_ROUND;
-  constexpr int R = T * T / NT;
-  constexpr int HP = T + 2;
-  __shared__ int32_t RM[HP * HP];
   __shared__ BlockQ bq;
-  const int lx = threadIdx.x % T, chain = threadIdx.x / T;
   const int n = *Q.in_count;
   bq_init(bq, Q);
   if (!first && blockIdx.x == 0 && threadIdx.x == 0 && n > 0)
     atomicAdd(&s.tally[TALLY_FLOOD], (unsigned long long)n);
-  const int p0 = xcd_pos(blockIdx.x, gridDim.x);
-  int tnext = p0 < n ? list_tile(Q.in, p0) : 0;
-  for (int pos = p0; pos < n; pos += gridDim.x) {
-    const TG q = tile_geo(g, tnext);
-    if (pos + (int)gridDim.x < n) tnext = list_tile(Q.in, pos + gridDim.x);
-    const size_t base = tbase<T>(q.t);
-    int r[R], r0[R], m[R], rl[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int ly = chain * R + k;
-      const size_t i = base + ly * T + lx;
-      m[k] = s.mask[i];
-      r0[k] = first ? 0 : (s.reach[i] ? 1 : 0);
-      r[k] = r0[k] | ((m[k] & M_EXCESS) ? 1 : 0);
-      RM[(ly + 1) * HP + lx + 1] = (m[k] << 1) | r[k];
-    }
-    int hv = 0;
-    Halo hs{};
-    if (threadIdx.x < 4 * T) {
-      hs = halo_slot<T>(q, threadIdx.x);
-      if (hs.nb >= 0) {
-        const size_t p = tbase<T>(hs.nb) + hs.ny * T + hs.nx;
-        hv = (int)s.mask[p] << 1;
-        if (mode != FLOOD_FIRST) hv |= s.reach[p] ? 1 : 0;
+  const int lane = lane_id();
+  const int seg = lane / T, y = lane % T;
+  const int slot = (threadIdx.x / kWave) * TPW + seg;  // this segment's tile of the block's TPB
+  const int stride = gridDim.x * TPB;
+  int pos = xcd_pos(blockIdx.x, gridDim.x) * TPB + slot;
+  int tnext = pos < n ? list_tile(Q.in, pos) : 0;
+  for (int base = xcd_pos(blockIdx.x, gridDim.x) * TPB; base < n; base += stride, pos += stride) {
+    const bool valid = pos < n;
+    const int t = tnext;
+    if (pos + stride < n) tnext = list_tile(Q.in, pos + stride);
+    Row mr[6] = {0, 0, 0, 0, 0, 0}, r0 = 0;
+    Row hin_top = 0, hin_bot = 0, hr_top = 0, hr_bot = 0;
+    int hin_l = 0, hin_r = 0, hr_l = 0, hr_r = 0, any_halo_reach = 0;
+    TG q{};
+    if (valid) {
+      q = tile_geo(g, t);
+      const size_t rowp = tbase<T>(t) + (size_t)y * T;
+      load_bit_rows<T, 6>(s.mask + rowp, 0, mr);
+      if (!first) {
+        Row rr[1];
+        load_bit_rows<T, 1>(s.reach + rowp, 0, rr);
+        r0 = rr[0];
       }
-      // a first pass relaxes without halo reach first: the tile's own reach
-      RM[hs.idx] = first ? (hv & ~1) : hv;
-    }
-    // (the barrier publishes RM; with no reached halo pixel the tile's own
-    // reach is all it gets)
-    const bool two =
-        __syncthreads_or(mode == FLOOD_FIRST_INC && threadIdx.x < 4 * T && (hv & 1)) != 0;
-    if (two) {
-      flood_relax<T, NT>(r, m, RM);
-#pragma unroll
-      for (int k = 0; k < R; ++k) rl[k] = r[k];
-      if (threadIdx.x < 4 * T) RM[hs.idx] = hv;
-      __syncthreads();
-    }
-    flood_relax<T, NT>(r, m, RM);
-    if (first && !two) {
-#pragma unroll
-      for (int k = 0; k < R; ++k) rl[k] = r[k];
-    }
-    int dirs = 0, imported = 0;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int ly = chain * R + k;
-      if (r[k] && !r0[k]) {
-        s.reach[base + ly * T + lx] = 1;
-        dirs |= flood_news<T>(q, lx, ly, m[k], RM);
-        imported |= first ? !rl[k] : 1;
+      // the halo pixels of this lane: (y, -1) above, (y, T) below, (-1, y)
+      // left, (T, y) right; mask byte and reach byte of each
+      int mt = 0, mb = 0, ml = 0, mrr = 0, rt_ = 0, rb_ = 0, rl_ = 0, rr_ = 0;
+      const bool hreach = mode != FLOOD_FIRST;
+      if (q.ty > 0) {
+        const size_t p = tbase<T>(t - q.tw) + (size_t)(T - 1) * T + y;
+        mt = s.mask[p];
+        if (hreach) rt_ = s.reach[p];
       }
+      if (q.ty < q.th - 1) {
+        const size_t p = tbase<T>(t + q.tw) + y;
+        mb = s.mask[p];
+        if (hreach) rb_ = s.reach[p];
+      }
+      if (q.tx > 0) {
+        const size_t p = tbase<T>(t - 1) + (size_t)y * T + (T - 1);
+        ml = s.mask[p];
+        if (hreach) rl_ = s.reach[p];
+      }
+      if (q.tx < q.tw - 1) {
+        const size_t p = tbase<T>(t + 1) + (size_t)y * T;
+        mrr = s.mask[p];
+        if (hreach) rr_ = s.reach[p];
+      }
+      // reach coming in over an arc into this tile
+      hin_l = (rl_ && (ml & M_RIGHT)) ? 1 : 0;
+      hin_r = (rr_ && (mrr & M_LEFT)) ? 1 : 0;
+      hr_l = rl_ ? 1 : 0;
+      hr_r = rr_ ? 1 : 0;
+      any_halo_reach = (rt_ | rb_ | rl_ | rr_) ? 1 : 0;
+      // (the top / bottom rows' words gather below, by ballot)
+      hin_top = (rt_ && (mt & M_DOWN)) ? 1 : 0;
+      hin_bot = (rb_ && (mb & M_UP)) ? 1 : 0;
+      hr_top = rt_ ? 1 : 0;
+      hr_bot = rb_ ? 1 : 0;
     }
-    if (dirs) atomicOr(&bq.dirs, dirs);
-    imported = __syncthreads_or(imported);
-    if (threadIdx.x == 0) {
+    // segment words of the top and bottom halo rows (bit x from lane x)
+    const int sh = seg * T;
+    const Row w_in_top = (Row)((__ballot(hin_top != 0) >> sh) & SEG);
+    const Row w_in_bot = (Row)((__ballot(hin_bot != 0) >> sh) & SEG);
+    const Row w_r_top = (Row)((__ballot(hr_top != 0) >> sh) & SEG);
+    const Row w_r_bot = (Row)((__ballot(hr_bot != 0) >> sh) & SEG);
+    const bool seg_halo = ((__ballot(any_halo_reach != 0) >> sh) & SEG) != 0;
+    const Row U = mr[0], D = mr[1], L = mr[2], Rt = mr[3], X = mr[5];
+    // halo reach entering this lane's row
+    Row hin = (y == 0 ? w_in_top : 0) | (y == T - 1 ? w_in_bot : 0) | (Row)hin_l |
+              (hin_r ? TOP : 0);
+    Row r, rl;
+    if (mode == FLOOD_FIRST) {
+      r = rl = flood_fill<T>(X, Rt, L, D, U, y);
+    } else if (mode == FLOOD_FIRST_INC) {
+      rl = flood_fill<T>(X, Rt, L, D, U, y);
+      // (a second fill only where some halo pixel is reached; the fill is
+      // wave-uniform, so segments without halo reach add nothing)
+      r = __any(seg_halo) ? flood_fill<T>(rl | (seg_halo ? hin : 0), Rt, L, D, U, y) : rl;
+    } else {
+      r = rl = flood_fill<T>(r0 | X | hin, Rt, L, D, U, y);
+    }
+    if (!valid) r = rl = r0 = 0;
+    const Row nw = r & ~r0;
+    if (nw) store_bit_row<T>(s.reach + tbase<T>(t) + (size_t)y * T, r);
+    // news: a newly reached edge pixel with a residual arc across the edge,
+    // the halo pixel there not reached
+    int dirs = 0;
+    if (valid) {
+      if (y == 0 && q.ty > 0 && (nw & U & ~w_r_top)) dirs |= 1 << D_UP;
+      if (y == T - 1 && q.ty < q.th - 1 && (nw & D & ~w_r_bot)) dirs |= 1 << D_DOWN;
+      if (q.tx > 0 && (nw & L & 1) && !hr_l) dirs |= 1 << D_LEFT;
+      if (q.tx < q.tw - 1 && (nw & Rt & TOP) && !hr_r) dirs |= 1 << D_RIGHT;
+    }
+    const int imported = first ? ((r & ~rl) != 0) : (nw != 0);
+    // per segment: the OR of its lanes' news, and whether it imported
+    unsigned long long bd[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) bd[d] = (__ballot((dirs >> d) & 1) >> sh) & SEG;
+    const bool seg_imp = ((__ballot(imported) >> sh) & SEG) != 0;
+    if (valid && y < 4 && bd[y]) {
+      const int nb = y == D_UP ? t - q.tw : y == D_DOWN ? t + q.tw : y == D_LEFT ? t - 1 : t + 1;
+      if (atomicExch(&s.stamp[nb], bq.epoch) != bq.epoch) bq.buf[atomicAdd(&bq.n, 1)] = nb;
+    }
+    if (valid && y == 0) {
       if (first)
-        s.imp[q.t] = imported;
-      else if (imported)
-        s.imp[q.t] = 1;
-      if (lmark) lmark[q.t] = 1;
+        s.imp[t] = seg_imp ? 1 : 0;
+      else if (seg_imp)
+        s.imp[t] = 1;
+      if (lmark) lmark[t] = 1;
     }
-    bq_take(bq, q, s.stamp, Q);
+    __syncthreads();
+    if (bq.n > kQCap - 4 * TPB) bq_flush(bq, Q);
   }
   bq_flush(bq, Q);
 }

@@ -608,7 +608,7 @@ struct PoolKernels {
                        q);
   }
   static void flood(Pool* p, int grid, const Queue& q, int mode, int32_t* lmark) {
-    hipLaunchKernelGGL((k_flood<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev, q,
+    hipLaunchKernelGGL((k_flood<T, NT>), dim3(grid), dim3(kFloodNT), 0, p->stream, p->geo, p->dev, q,
                        mode, lmark);
   }
   static void labels(Pool* p, int grid, const int32_t* list, int cnt) {
