@@ -747,67 +747,6 @@ __device__ __forceinline__ void bq_init(BlockQ& b, const Queue& Q) {
   __syncthreads();
 }
 
-// Neighbours of tile q across the edges pixel (lx, ly) lies on, as bits
-// 1 << D_UP ... 1 << D_RIGHT (the same bits as M_UP ... M_RIGHT).
-template <int T>
-__device__ __forceinline__ int edge_dirs(const TG& q, int lx, int ly) {
-  return (ly == 0 && q.ty > 0 ? 1 << D_UP : 0) | (ly == T - 1 && q.ty < q.th - 1 ? 1 << D_DOWN : 0) |
-         (lx == 0 && q.tx > 0 ? 1 << D_LEFT : 0) | (lx == T - 1 && q.tx < q.tw - 1 ? 1 << D_RIGHT : 0);
-}
-
-// LDS index of the halo pixel across side `side` from edge pixel (lx, ly),
-// in a (T+2)^2 halo-padded array, and its halo slot (as halo_slot's j).
-template <int T>
-__device__ __forceinline__ int halo_idx(int side, int lx, int ly) {
-  constexpr int HP = T + 2;
-  return side == D_UP ? lx + 1
-         : side == D_DOWN ? (T + 1) * HP + lx + 1
-         : side == D_LEFT ? (ly + 1) * HP
-                          : (ly + 1) * HP + T + 1;
-}
-template <int T>
-__device__ __forceinline__ int halo_j(int side, int lx, int ly) {
-  return side * T + (side <= D_DOWN ? lx : ly);
-}
-
-// Arc bit of a halo pixel on side `side` that points into this tile.
-__device__ __forceinline__ int arc_into(int side) { return 1 << (side ^ 1); }
-
-// BFS news for the neighbours of edge pixel (lx, ly) at distance d: a side
-// counts if the neighbour pixel across it has a residual arc into this pixel
-// (hin bit 0) and d + 1 beats the value the halo holds for it (D).  hin bit
-// 1: the neighbour is being recomputed in the same pass from a halo of 0 at
-// sinks and MAX elsewhere, so a sink here is no news to it.  Distances only
-// fall, so a stale halo value can only make a side count needlessly.
-template <int T>
-__device__ __forceinline__ int bfs_news(const TG& q, int lx, int ly, int d, bool sink,
-                                        const int32_t* D, const uint8_t* hin) {
-  const int e = edge_dirs<T>(q, lx, ly);
-  int dirs = 0;
-#pragma unroll
-  for (int side = 0; side < 4; ++side) {
-    if (!((e >> side) & 1)) continue;
-    const int h = hin[halo_j<T>(side, lx, ly)];
-    if ((h & 1) && !((h & 2) && sink) && d + 1 < D[halo_idx<T>(side, lx, ly)]) dirs |= 1 << side;
-  }
-  return dirs;
-}
-
-// Appends the neighbours in b.dirs (complete: a barrier has passed since the
-// block's atomicOr into it) to the block's buffer; flushes a nearly full
-// buffer.  Called by all threads.
-__device__ __forceinline__ void bq_flush(BlockQ& b, const Queue& Q);
-__device__ __forceinline__ void bq_take(BlockQ& b, const TG& q, int32_t* stamp, const Queue& Q) {
-  if (threadIdx.x < 4 && ((b.dirs >> threadIdx.x) & 1)) {
-    const int k = threadIdx.x;
-    const int nb = k == D_UP ? q.t - q.tw : k == D_DOWN ? q.t + q.tw : k == D_LEFT ? q.t - 1 : q.t + 1;
-    if (atomicExch(&stamp[nb], b.epoch) != b.epoch) b.buf[atomicAdd(&b.n, 1)] = nb;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) b.dirs = 0;
-  if (b.n > kQCap - 4) bq_flush(b, Q);
-}
-
 // Copies the block's buffer to the output list.  Called by all threads, a
 // barrier after the last change of b.n.
 __device__ __forceinline__ void bq_flush(BlockQ& b, const Queue& Q) {
@@ -822,135 +761,332 @@ __device__ __forceinline__ void bq_flush(BlockQ& b, const Queue& Q) {
   __syncthreads();
 }
 
+// ------------------------------------------------------------ bit rows
+// The relabel and flood kernels work on bit rows: lane y of a T-lane
+// segment holds row y of one tile as a T-bit word (a frontier, a visited or
+// reached set, and each arc kind of the mask), so a wave holds 64 / T tiles.
+template <int T>
+struct BitRow {
+  using type = unsigned;
+};
+template <>
+struct BitRow<64> {
+  using type = unsigned long long;
+};
+
+// Bit k of each of w's four bytes, as four bits.
+__device__ __forceinline__ unsigned byte_bits(unsigned w, int k) {
+  return (((w >> k) & 0x01010101u) * 0x01020408u) >> 24;
+}
+// Four bits as four bytes of 0 / 1.
+__device__ __forceinline__ unsigned bits_bytes(unsigned n) {
+  return (n * 0x00204081u) & 0x01010101u;
+}
+
+// Row y of a tile's byte array (T bytes, 16-byte aligned) as bit rows of
+// bits k0 .. k0 + NK - 1 of each byte.
+template <int T, int NK>
+__device__ __forceinline__ void load_bit_rows(const uint8_t* row, int k0,
+                                              typename BitRow<T>::type (&out)[NK]) {
+  using Row = typename BitRow<T>::type;
+#pragma unroll
+  for (int k = 0; k < NK; ++k) out[k] = 0;
+#pragma unroll
+  for (int c = 0; c < T / 16; ++c) {
+    const uint4 v = reinterpret_cast<const uint4*>(row)[c];
+    const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int k = 0; k < NK; ++k) out[k] |= (Row)byte_bits(w[i], k0 + k) << (16 * c + 4 * i);
+  }
+}
+
+template <int T>
+__device__ __forceinline__ void store_bit_row(uint8_t* row, typename BitRow<T>::type r) {
+#pragma unroll
+  for (int c = 0; c < T / 16; ++c) {
+    const unsigned n = (unsigned)(r >> (16 * c));
+    reinterpret_cast<uint4*>(row)[c] = make_uint4(bits_bytes(n & 15), bits_bytes((n >> 4) & 15),
+                                                  bits_bytes((n >> 8) & 15),
+                                                  bits_bytes((n >> 12) & 15));
+  }
+}
+
+template <typename Row>
+__device__ __forceinline__ Row seg_shfl_up(Row v, int d, int T) {
+  return __shfl_up(v, d, T);
+}
+template <typename Row>
+__device__ __forceinline__ Row seg_shfl_down(Row v, int d, int T) {
+  return __shfl_down(v, d, T);
+}
+
+template <typename Row>
+__device__ __forceinline__ int row_ctz(Row v) {
+  return sizeof(Row) == 8 ? __builtin_ctzll((unsigned long long)v) : __builtin_ctz((unsigned)v);
+}
+
+// Word of a predicate over the T lanes of this lane's segment (bit j: lane j
+// of the segment).  Every lane of the wave calls it.
+template <int T>
+__device__ __forceinline__ typename BitRow<T>::type seg_word(bool pred) {
+  using Row = typename BitRow<T>::type;
+  constexpr unsigned long long SEG = T == 64 ? ~0ull : ((1ull << T) - 1);
+  return (Row)((__ballot(pred) >> (lane_id() / T * T)) & SEG);
+}
+
+// Minimum over the T lanes of this lane's segment (every lane of the
+// segment calls it).
+template <int T>
+__device__ __forceinline__ int seg_min(int v) {
+#pragma unroll
+  for (int o = T / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, T));
+  return v;
+}
+
 // ------------------------------------------------------ global relabel
 // Distance of every pixel to the sink set {e < 0} over residual arcs
 // (MAX if unreachable): what init_bfs / bfs_intratile / bfs_intertile
 // (graphcut.cl:582-1016) compute, without their 30-round cap.  Inside a
-// tile the Bellman-Ford relaxation runs in LDS to its fixed point; across
-// tiles, a tile whose edge distances fell flags its neighbours for another
-// round, until no tile is flagged.
+// tile, a level-synchronous BFS on bit rows: a level moves the frontier one
+// step against the arcs (shifts along the rows, a shuffle across them) and
+// takes in the edge pixels whose halo neighbour sits one level below; an
+// empty frontier jumps to the next level some halo neighbour seeds.  A
+// pixel's distance is the level at which the frontier first reaches it,
+// written then into the block's LDS copy of the tile.  Across tiles, a tile
+// whose edge distances fell queues its neighbours for another round, until
+// no tile is queued.
 
-// Iterations of an in-tile relaxation before its row scans switch on.
-constexpr int kScanAfter = 2;
-
-// One row of a tile spans T consecutive lanes of a wave (lane % T = column).
-// row_min_scan relaxes each pixel of the row along straight runs of
-// residual arcs: d(x) = min over j of d(x -/+ j) + j where the j arcs from x
-// toward x -/+ j are all residual (the pixel's own M_LEFT / M_RIGHT bits
-// first).  Doubling steps: after the step of size s every lane holds the
-// best value over runs shorter than 2s and whether its run of 2s arcs is
-// open.
+constexpr int kBfsNT = 256;
+constexpr int kInf = 0x7fffffff;
 template <int T>
-__device__ __forceinline__ int row_min_scan(int d, int m, int lx) {
-  int a = d, oa = (m & M_LEFT) ? 1 : 0;
-  int b = d, ob = (m & M_RIGHT) ? 1 : 0;
-#pragma unroll
-  for (int s = 1; s < T; s <<= 1) {
-    const int au = __shfl_up(a, s, T), ou = __shfl_up(oa, s, T);
-    const int bd = __shfl_down(b, s, T), od = __shfl_down(ob, s, T);
-    const bool lo = lx >= s, hi = lx + s < T;
-    if (lo && oa) a = min(a, au + s);
-    oa = lo ? (oa & ou) : 0;
-    if (hi && ob) b = min(b, bd + s);
-    ob = hi ? (ob & od) : 0;
-  }
-  return min(a, b);
-}
+constexpr int kBfsTPB = (kWave / T) * (kBfsNT / kWave);  // tiles per block
 
-// Relaxes d[] (this thread's R pixels, rows chain*R .. chain*R+R-1 of
-// column lx) to the in-tile fixed point given halo values already in
-// D0/D1.  Each iteration: one Jacobi step from the LDS copy (across chains
-// and into the halo), then sweeps along the thread's column segment in
-// registers and straight-run scans along every row -- the fixed point is
-// the same, with far fewer barriers.  Returns after the last barrier, with
-// the number of iterations.
-template <int T, int NT>
-__device__ __forceinline__ int bfs_relax(int (&d)[T * T / NT], const int (&m)[T * T / NT],
-                                          int32_t* D0, int32_t* D1) {
-  constexpr int R = T * T / NT;
-  constexpr int HP = T + 2;
-  const int lx = threadIdx.x % T, chain = threadIdx.x / T;
-  int32_t* Dc = D0;
-  int32_t* Dn = D1;
-  for (int it = 0;; ++it) {
-    int changed = 0;
+// One pass over listed tiles.  INIT (the relabel's first pass): seeds are
+// the tile's sinks (distance 0) and the halo: a neighbour relabelled in the
+// same pass (all, or its rflag holds the pass's epoch) counts as 0 at its
+// sinks and MAX elsewhere, a clean one keeps its stored distances (valid:
+// its arcs and excess have not changed since they were computed, and the
+// rounds lower any of them a recomputed tile now undercuts).  Writes every
+// distance and, for the completion test (check_completion,
+// graphcut.cl:20-60),
+//   act[t]    = some pixel has e > 0 and a finite distance
+//   haspos[t] = some pixel has e > 0
+// and moves a dirty flag to the flood's changed flag.  Later rounds (!INIT):
+// the stored distances are the tile's in-tile fixed point for the halo it
+// had, and halo distances only fall, so the new distances are the minimum
+// of the stored ones and a BFS from the halo alone (seeded only where a
+// halo distance + 1 beats the edge pixel's stored one); a pass writes the
+// rows that fell and flags the tile active if an excess pixel's fell.
+// A neighbour is queued when an edge pixel's distance (new, for a round) +
+// 1 beats the value the halo holds for the pixel across the edge and that
+// pixel's arc into this tile is residual; in the first pass a sink is no
+// news to a neighbour relabelled in the same pass (it starts from 0 at
+// sinks).  Distances only fall, so a stale halo value can only queue a
+// neighbour needlessly.
+template <int T, bool INIT>
+__device__ __forceinline__ void bfs_pass(const Geo& g, CutDev& s, const int32_t* __restrict__ list,
+                                         int n, int all, const Queue& Q, BlockQ& bq,
+                                         int32_t* lds) {
+  using Row = typename BitRow<T>::type;
+  constexpr int TPB = kBfsTPB<T>;
+  constexpr Row TOP = (Row)1 << (T - 1);
+  const int MAXH = g.maxh;
+  const int lane = lane_id(), y = lane % T;
+  const int slot = (threadIdx.x / kWave) * (kWave / T) + lane / T;
+  int32_t* buf = lds + slot * T * T;  // this segment's tile, row-major
+  const int stride = gridDim.x * TPB;
+  const int b0 = xcd_pos(blockIdx.x, gridDim.x) * TPB;
+  int pos = b0 + slot;
+  int tnext = pos < n ? list_tile(list, pos) : 0;
+  for (int base = b0; base < n; base += stride, pos += stride) {
+    const bool valid = pos < n;
+    const int t = tnext;
+    if (pos + stride < n) tnext = list_tile(list, pos + stride);
+    Row mr[6] = {0, 0, 0, 0, 0, 0};
+    TG q{};
+    // per side (D_UP, D_DOWN, D_LEFT, D_RIGHT): this lane's edge pixel on
+    // that side -- (y, 0), (y, T-1), (0, y), (T-1, y) -- and the halo pixel
+    // across: its distance, whether its arc into this tile is residual, and
+    // (INIT) whether it is relabelled in this pass; (!INIT) the edge
+    // pixel's stored distance
+    int hv[4] = {MAXH, MAXH, MAXH, MAXH}, hin[4] = {0, 0, 0, 0}, hls[4] = {0, 0, 0, 0};
+    int e0[4] = {kInf, kInf, kInf, kInf};
+    bool ex[4] = {false, false, false, false};
+    int nbt[4] = {0, 0, 0, 0};
+    const int epx[4] = {y, (T - 1) * T + y, y * T, y * T + T - 1};
+    if (valid) {
+      q = tile_geo(g, t);
+      const size_t tb = tbase<T>(t);
+      load_bit_rows<T, 6>(s.mask + tb + (size_t)y * T, 0, mr);
+      ex[D_UP] = q.ty > 0;
+      ex[D_DOWN] = q.ty < q.th - 1;
+      ex[D_LEFT] = q.tx > 0;
+      ex[D_RIGHT] = q.tx < q.tw - 1;
+      nbt[D_UP] = t - q.tw;
+      nbt[D_DOWN] = t + q.tw;
+      nbt[D_LEFT] = t - 1;
+      nbt[D_RIGHT] = t + 1;
+      const int npx[4] = {(T - 1) * T + y, y, y * T + T - 1, y * T};
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int ly = chain * R + k;
-      const int c = (ly + 1) * HP + lx + 1;
-      int nd = d[k];
-      if (nd > 0) {
-        if (m[k] & M_UP) nd = min(nd, Dc[c - HP] + 1);
-        if (m[k] & M_DOWN) nd = min(nd, Dc[c + HP] + 1);
-        if (m[k] & M_LEFT) nd = min(nd, Dc[c - 1] + 1);
-        if (m[k] & M_RIGHT) nd = min(nd, Dc[c + 1] + 1);
-        if (nd < d[k]) {
-          d[k] = nd;
-          changed = 1;
+      for (int side = 0; side < 4; ++side) {
+        if (!ex[side]) continue;
+        const size_t p = tbase<T>(nbt[side]) + npx[side];
+        const int mq = s.mask[p], hq = s.h[p];
+        if (INIT) {
+          const bool listed = all || s.rflag[nbt[side]] == bq.epoch;
+          hv[side] = listed ? ((mq & M_SINK) ? 0 : MAXH) : hq;
+          hls[side] = listed ? 1 : 0;
+        } else {
+          hv[side] = hq;
+          e0[side] = s.h[tb + epx[side]];
         }
+        // (the arc of the halo pixel that points into this tile)
+        hin[side] = (mq & (1 << (side ^ 1))) ? 1 : 0;
       }
     }
-    // column segment, both ways
+    const Row U = mr[0], D = mr[1], Lf = mr[2], Rt = mr[3], SNK = mr[4], X = mr[5];
+    // the first and last rows' arc and sink words (lanes hold columns there)
+    const Row u0 = __shfl(U, 0, T), dl = __shfl(D, T - 1, T);
+    const Row s0 = __shfl(SNK, 0, T), sl = __shfl(SNK, T - 1, T);
+    // the level at which each edge pixel of this lane is seeded from the
+    // halo (kInf: not), if its arc toward the halo pixel is residual
+    const int arc[4] = {(int)((u0 >> y) & 1), (int)((dl >> y) & 1), (int)(Lf & 1),
+                        (int)((Rt >> (T - 1)) & 1)};
+    int inj[4];
 #pragma unroll
-    for (int k = 1; k < R; ++k)
-      if ((m[k] & M_UP) && d[k - 1] + 1 < d[k]) {
-        d[k] = d[k - 1] + 1;
-        changed = 1;
+    for (int side = 0; side < 4; ++side)
+      inj[side] = (arc[side] && hv[side] < MAXH && hv[side] + 1 < e0[side]) ? hv[side] + 1 : kInf;
+    // the tile's distances start as 0 at sinks and MAX elsewhere (INIT) or
+    // as the stored ones (a round)
+#pragma unroll
+    for (int c = 0; c < T / 4; ++c) {
+      int4 v;
+      if (INIT) {
+        int w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w[k] = ((SNK >> (4 * c + k)) & 1) ? 0 : MAXH;
+        v = make_int4(w[0], w[1], w[2], w[3]);
+      } else {
+        v = valid ? reinterpret_cast<const int4*>(s.h + tbase<T>(t) + (size_t)y * T)[c]
+                  : make_int4(MAXH, MAXH, MAXH, MAXH);
       }
+      reinterpret_cast<int4*>(buf + y * T)[c] = v;
+    }
+    // V: pixels the frontier has reached; a round's frontier keeps only the
+    // pixels whose distance fell (a pixel at or below the level cannot
+    // lower its neighbours: the stored distances are a fixed point)
+    Row V = INIT ? SNK : 0, F = V, fell = 0;
+    int lc = INIT ? 0 : -1;  // the segment's current level
+    bool done = !valid;
+    for (;;) {
+      if (seg_word<T>(F != 0) == 0 && !done) {
+        // frontier empty: jump to the next seeded level, or finish
+        int m = kInf;
 #pragma unroll
-    for (int k = R - 2; k >= 0; --k)
-      if ((m[k] & M_DOWN) && d[k + 1] + 1 < d[k]) {
-        d[k] = d[k + 1] + 1;
-        changed = 1;
+        for (int side = 0; side < 4; ++side)
+          if (inj[side] > lc) m = min(m, inj[side]);
+        m = seg_min<T>(m);
+        if (m == kInf)
+          done = true;
+        else
+          lc = m - 1;
       }
-    // rows, once the tile has shown paths longer than a few steps (short
-    // ones settle in a couple of iterations without the scans)
-    if (it >= kScanAfter) {
-#pragma unroll
-      for (int k = 0; k < R; ++k) {
-        const int v = row_min_scan<T>(d[k], m[k], lx);
-        if (v < d[k]) {
-          d[k] = v;
-          changed = 1;
+      if (!__any(!done)) break;
+      const int L = lc + 1;
+      const Row fu = seg_shfl_up(F, 1, T), fd = seg_shfl_down(F, 1, T);
+      Row nw = ((F >> 1) & Rt) | ((F << 1) & Lf) | ((y > 0 ? fu : 0) & U) |
+               ((y < T - 1 ? fd : 0) & D);
+      if (inj[D_LEFT] == L) nw |= 1;
+      if (inj[D_RIGHT] == L) nw |= TOP;
+      const Row wt = seg_word<T>(inj[D_UP] == L), wb = seg_word<T>(inj[D_DOWN] == L);
+      if (y == 0) nw |= wt;
+      if (y == T - 1) nw |= wb;
+      nw &= ~V;
+      if (done) nw = 0;
+      V |= nw;
+      if (INIT) {
+        for (Row m = nw; m; m &= m - 1) buf[y * T + row_ctz(m)] = L;
+      } else {
+        for (Row m = nw; m; m &= m - 1) {
+          const int x = row_ctz(m);
+          if (atomicMin(&buf[y * T + x], L) <= L) nw &= ~((Row)1 << x);
         }
+        fell |= nw;
+      }
+      F = nw;
+      lc = L;
+    }
+    __syncthreads();  // the block's distances are in LDS
+    // this lane's row: INIT writes it, a round the 16-byte pieces that fell
+    if (valid) {
+      const size_t rowg = tbase<T>(t) + (size_t)y * T;
+#pragma unroll
+      for (int c = 0; c < T / 4; ++c)
+        if (INIT || ((fell >> (4 * c)) & 15))
+          reinterpret_cast<int4*>(s.h + rowg)[c] = reinterpret_cast<const int4*>(buf + y * T)[c];
+    }
+    // news for the neighbours
+    int dirs = 0;
+    const Row f0 = __shfl(fell, 0, T), fl = __shfl(fell, T - 1, T);
+    if (valid) {
+      const int sink[4] = {(int)((s0 >> y) & 1), (int)((sl >> y) & 1), (int)(SNK & 1),
+                           (int)((SNK >> (T - 1)) & 1)};
+      const int fel[4] = {(int)((f0 >> y) & 1), (int)((fl >> y) & 1), (int)(fell & 1),
+                          (int)((fell >> (T - 1)) & 1)};
+#pragma unroll
+      for (int side = 0; side < 4; ++side) {
+        if (!ex[side] || !hin[side]) continue;
+        const int d = buf[epx[side]];
+        const bool cand = INIT ? (d < MAXH && !(hls[side] && sink[side])) : fel[side] != 0;
+        if (cand && d + 1 < hv[side]) dirs |= 1 << side;
       }
     }
+    // lane y < 4 of the segment queues the neighbour across side y
+    bool mine = false;
 #pragma unroll
-    for (int k = 0; k < R; ++k) Dn[(chain * R + k + 1) * HP + lx + 1] = d[k];
-    if (!__syncthreads_or(changed)) return it + 1;
-    int32_t* tmp = Dc;
-    Dc = Dn;
-    Dn = tmp;
+    for (int d = 0; d < 4; ++d) {
+      const bool any = seg_word<T>((dirs >> d) & 1) != 0;
+      if (y == d) mine = any;
+    }
+    if (valid && mine) {
+      const int nb = y == D_UP ? t - q.tw : y == D_DOWN ? t + q.tw : y == D_LEFT ? t - 1 : t + 1;
+      if (atomicExch(&s.stamp[nb], bq.epoch) != bq.epoch) bq.buf[atomicAdd(&bq.n, 1)] = nb;
+    }
+    if (INIT) {
+      const bool any_pos = seg_word<T>(X != 0) != 0;
+      const bool any_act = seg_word<T>((X & V) != 0) != 0;
+      if (valid && y == 0) {
+        s.haspos[t] = any_pos;
+        s.act[t] = any_act;
+        // (no other block reads this tile's dirty flag during the pass)
+        if (s.dirty[t]) {
+          s.fchg[t] = 1;
+          s.dirty[t] = 0;
+        }
+      }
+    } else {
+      const bool any_act = seg_word<T>((fell & X) != 0) != 0;
+      if (valid && y == 0 && any_act) s.act[t] = 1;
+    }
+    __syncthreads();
+    if (bq.n > kQCap - 4 * TPB) bq_flush(bq, Q);
   }
 }
 
 // First pass over the listed (dirty) tiles, from the residual masks
-// (pending inflow has been folded): seeds, in-tile distances, and the
-// completion test (check_completion, graphcut.cl:20-60) for the tile:
-//   act[t]    = some pixel has e > 0 and a finite distance
-//   haspos[t] = some pixel has e > 0
-// all != 0: every tile of the images is listed (a full relabel); else the
-// listed tiles are those whose rflag holds this pass's epoch (*Q.ep_in).
-// A listed tile that was dirty is marked changed for the next flood and
-// its dirty flag cleared; *zero (the count the relabel's active list is
-// compacted into) is zeroed.
-// Halo pixels of listed neighbours start as 0 (sink) or MAX; those of clean
-// neighbours keep their stored distances, which stay valid: a clean tile's
-// arcs and excess have not changed since its distances were computed, and
-// the rounds lower any of them that a recomputed tile now undercuts.  Later
-// rounds only lower distances, so they can only switch act on.  The
-// neighbours this tile's distances may improve start the rounds' queue
-// (Q.out; Q.in is not read).
-template <int T, int NT>
-__global__ void __launch_bounds__(NT)
+// (pending inflow has been folded).  all != 0: every tile of the images is
+// listed (a full relabel); else the listed tiles are those whose rflag holds
+// this pass's epoch (*Q.ep_in).  *zero (the count the relabel's active list
+// is compacted into) is zeroed.  The neighbours this pass's distances may
+// improve start the rounds' queue (Q.out; Q.in is not read).
+template <int T>
+__global__ void __launch_bounds__(kBfsNT)
 k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
            int all, Queue Q, int32_t* zero) {
-  constexpr int R = T * T / NT;
-  constexpr int HP = T + 2;
-  __shared__ int32_t D0[HP * HP], D1[HP * HP];
-  __shared__ uint8_t hin[4 * T];
+  __shared__ __align__(16) int32_t lds[kBfsTPB<T> * T * T];
   __shared__ BlockQ bq;
-  const int lx = threadIdx.x % T, chain = threadIdx.x / T;
   bq_init(bq, Q);
   const int n = *count;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -960,127 +1096,21 @@ k_bfs_init(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __r
       atomicAdd(&s.tally[TALLY_RELABEL_TILES], (unsigned long long)n);
     }
   }
-  const int epoch = bq.epoch;
-  const int p0 = xcd_pos(blockIdx.x, gridDim.x);
-  int tnext = p0 < n ? list_tile(list, p0) : 0;
-  for (int pos = p0; pos < n; pos += gridDim.x) {
-    const TG q = tile_geo(g, tnext);
-    if (pos + (int)gridDim.x < n) tnext = list_tile(list, pos + gridDim.x);
-    const size_t base = tbase<T>(q.t);
-    int d[R], m[R];
-    int any_pos = 0;
-#pragma unroll
-    for (int k = 0; k < R; ++k) m[k] = s.mask[base + (chain * R + k) * T + lx];
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int ly = chain * R + k;
-      d[k] = (m[k] & M_SINK) ? 0 : g.maxh;
-      D0[(ly + 1) * HP + lx + 1] = d[k];
-      any_pos |= (m[k] & M_EXCESS) != 0;
-    }
-    if (threadIdx.x < 4 * T) {
-      // a neighbour being relabelled too starts at 0 (sink) or MAX; a clean
-      // one keeps its stored (valid) distance
-      const Halo hs = halo_slot<T>(q, threadIdx.x);
-      int v = g.maxh, in = 0;
-      if (hs.nb >= 0) {
-        const size_t p = tbase<T>(hs.nb) + hs.ny * T + hs.nx;
-        // (all three loads issued together)
-        const int mq = s.mask[p], hq = s.h[p];
-        const bool listed = all || s.rflag[hs.nb] == epoch;
-        v = listed ? ((mq & M_SINK) ? 0 : g.maxh) : hq;
-        in = ((mq & arc_into(threadIdx.x / T)) ? 1 : 0) | (listed ? 2 : 0);
-      }
-      D0[hs.idx] = v;
-      D1[hs.idx] = v;
-      hin[threadIdx.x] = (uint8_t)in;
-    }
-    __syncthreads();
-    bfs_relax<T, NT>(d, m, D0, D1);
-    int any_act = 0, dirs = 0;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int ly = chain * R + k;
-      s.h[base + ly * T + lx] = d[k];
-      any_act |= (m[k] & M_EXCESS) && d[k] < g.maxh;
-      // news for a neighbour: it may now reach a sink through this pixel
-      if (d[k] < g.maxh) dirs |= bfs_news<T>(q, lx, ly, d[k], (m[k] & M_SINK) != 0, D0, hin);
-    }
-    if (dirs) atomicOr(&bq.dirs, dirs);
-    any_pos = __syncthreads_or(any_pos);
-    any_act = __syncthreads_or(any_act);
-    if (threadIdx.x == 0) {
-      s.haspos[q.t] = any_pos;
-      s.act[q.t] = any_act;
-      // (no other block reads this tile's dirty flag during the pass)
-      if (s.dirty[q.t]) {
-        s.fchg[q.t] = 1;
-        s.dirty[q.t] = 0;
-      }
-    }
-    bq_take(bq, q, s.stamp, Q);
-  }
+  bfs_pass<T, true>(g, s, list, n, all, Q, bq, lds);
   bq_flush(bq, Q);
 }
 
 // Later passes over queued tiles, from the stored distances.
-template <int T, int NT>
-__global__ void __launch_bounds__(NT)
+template <int T>
+__global__ void __launch_bounds__(kBfsNT)
 k_bfs_round(Geo g, CutDev s, Queue Q) {
-  constexpr int R = T * T / NT;
-  constexpr int HP = T + 2;
-  __shared__ int32_t D0[HP * HP], D1[HP * HP];
-  __shared__ uint8_t hin[4 * T];
+  __shared__ __align__(16) int32_t lds[kBfsTPB<T> * T * T];
   __shared__ BlockQ bq;
-  const int lx = threadIdx.x % T, chain = threadIdx.x / T;
-  const int n = *Q.in_count;
   bq_init(bq, Q);
+  const int n = *Q.in_count;
   if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0)
     atomicAdd(&s.tally[TALLY_BFS], (unsigned long long)n);
-  const int p0 = xcd_pos(blockIdx.x, gridDim.x);
-  int tnext = p0 < n ? Q.in[p0] : 0;
-  for (int pos = p0; pos < n; pos += gridDim.x) {
-    const TG q = tile_geo(g, tnext);
-    if (pos + (int)gridDim.x < n) tnext = Q.in[pos + gridDim.x];
-    const size_t base = tbase<T>(q.t);
-    int d[R], d0[R], m[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int ly = chain * R + k;
-      const size_t i = base + ly * T + lx;
-      d[k] = d0[k] = s.h[i];
-      m[k] = s.mask[i];
-      D0[(ly + 1) * HP + lx + 1] = d[k];
-    }
-    if (threadIdx.x < 4 * T) {
-      const Halo hs = halo_slot<T>(q, threadIdx.x);
-      int v = g.maxh, in = 0;
-      if (hs.nb >= 0) {
-        const size_t p = tbase<T>(hs.nb) + hs.ny * T + hs.nx;
-        v = s.h[p];
-        in = (s.mask[p] & arc_into(threadIdx.x / T)) ? 1 : 0;
-      }
-      D0[hs.idx] = v;
-      D1[hs.idx] = v;
-      hin[threadIdx.x] = (uint8_t)in;
-    }
-    __syncthreads();
-    bfs_relax<T, NT>(d, m, D0, D1);
-    int any_act = 0, dirs = 0;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int ly = chain * R + k;
-      if (d[k] < d0[k]) {
-        s.h[base + ly * T + lx] = d[k];
-        dirs |= bfs_news<T>(q, lx, ly, d[k], false, D0, hin);
-        any_act |= (m[k] & M_EXCESS) != 0;
-      }
-    }
-    if (dirs) atomicOr(&bq.dirs, dirs);
-    any_act = __syncthreads_or(any_act);
-    if (threadIdx.x == 0 && any_act) s.act[q.t] = 1;
-    bq_take(bq, q, s.stamp, Q);
-  }
+  bfs_pass<T, false>(g, s, Q.in, n, 0, Q, bq, lds);
   bq_flush(bq, Q);
 }
 
@@ -1102,79 +1132,17 @@ k_bfs_round(Geo g, CutDev s, Queue Q) {
 // there (its labels are rewritten after).
 enum { FLOOD_ROUND = 0, FLOOD_FIRST = 1, FLOOD_FIRST_INC = 2 };
 
-// The flood works on bit rows: lane y of a T-lane segment holds row y of
-// one tile as a T-bit word (reach, and each arc kind of the mask), so a
-// wave holds 64 / T tiles.  Reach spreads along rows by Kogge-Stone steps
-// on the words (shifts) and down / up the columns by the same steps across
-// the segment's lanes (shuffles); the fills alternate until no row changes.
-template <int T>
-struct FloodRow {
-  using type = unsigned;
-};
-template <>
-struct FloodRow<64> {
-  using type = unsigned long long;
-};
-
-// Bit k of each of w's four bytes, as four bits.
-__device__ __forceinline__ unsigned byte_bits(unsigned w, int k) {
-  return (((w >> k) & 0x01010101u) * 0x01020408u) >> 24;
-}
-// Four bits as four bytes of 0 / 1.
-__device__ __forceinline__ unsigned bits_bytes(unsigned n) {
-  return (n * 0x00204081u) & 0x01010101u;
-}
-
-// Row y of a tile's byte array (T bytes, 16-byte aligned) as bit rows of
-// bits k0 .. k0 + NK - 1 of each byte.
-template <int T, int NK>
-__device__ __forceinline__ void load_bit_rows(const uint8_t* row, int k0,
-                                              typename FloodRow<T>::type (&out)[NK]) {
-  using Row = typename FloodRow<T>::type;
-#pragma unroll
-  for (int k = 0; k < NK; ++k) out[k] = 0;
-#pragma unroll
-  for (int c = 0; c < T / 16; ++c) {
-    const uint4 v = reinterpret_cast<const uint4*>(row)[c];
-    const unsigned w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int k = 0; k < NK; ++k) out[k] |= (Row)byte_bits(w[i], k0 + k) << (16 * c + 4 * i);
-  }
-}
-
-template <int T>
-__device__ __forceinline__ void store_bit_row(uint8_t* row, typename FloodRow<T>::type r) {
-#pragma unroll
-  for (int c = 0; c < T / 16; ++c) {
-    const unsigned n = (unsigned)(r >> (16 * c));
-    reinterpret_cast<uint4*>(row)[c] = make_uint4(bits_bytes(n & 15), bits_bytes((n >> 4) & 15),
-                                                  bits_bytes((n >> 8) & 15),
-                                                  bits_bytes((n >> 12) & 15));
-  }
-}
-
-template <typename Row>
-__device__ __forceinline__ Row seg_shfl_up(Row v, int d, int T) {
-  return __shfl_up(v, d, T);
-}
-template <typename Row>
-__device__ __forceinline__ Row seg_shfl_down(Row v, int d, int T) {
-  return __shfl_down(v, d, T);
-}
-
 // Reach of the segment's rows closed under the arcs: g seeds; rt, lf, dn,
 // up the rows' arc bits (bit x of rt: arc x -> x+1 residual, ...); y the
 // lane's row.  Every lane of the wave calls it (the loop is wave-uniform).
 template <int T>
-__device__ __forceinline__ typename FloodRow<T>::type flood_fill(typename FloodRow<T>::type g,
-                                                                 typename FloodRow<T>::type rt,
-                                                                 typename FloodRow<T>::type lf,
-                                                                 typename FloodRow<T>::type dn,
-                                                                 typename FloodRow<T>::type up,
+__device__ __forceinline__ typename BitRow<T>::type flood_fill(typename BitRow<T>::type g,
+                                                                 typename BitRow<T>::type rt,
+                                                                 typename BitRow<T>::type lf,
+                                                                 typename BitRow<T>::type dn,
+                                                                 typename BitRow<T>::type up,
                                                                  int y) {
-  using Row = typename FloodRow<T>::type;
+  using Row = typename BitRow<T>::type;
   constexpr Row SEG = T == 64 ? ~0ull : (Row)((1ull << T) - 1);
   // into x from x - 1 / x + 1; into row y from row y - 1 / y + 1
   const Row pr0 = (rt << 1) & SEG, pl0 = lf >> 1;
@@ -1214,7 +1182,7 @@ constexpr int kFloodNT = 256;
 template <int T, int NT>
 __global__ void __launch_bounds__(kFloodNT)
 k_flood(Geo g, CutDev s, Queue Q, int mode, int32_t* lmark) {
-  using Row = typename FloodRow<T>::type;
+  using Row = typename BitRow<T>::type;
   constexpr int TPW = kWave / T;                 // tiles per wave
   constexpr int TPB = TPW * (kFloodNT / kWave);  // tiles per block
   constexpr Row SEG = T == 64 ? ~0ull : (Row)((1ull << T) - 1);

@@ -575,17 +575,8 @@ __global__ void k_flows(const unsigned long long* __restrict__ sums, const Prep*
   if (threadIdx.x == 0) out[blockIdx.x] = v;
 }
 
-// Threads per tile of the relabel kernels: at T = 32, two waves (8 pixels
-// per thread) instead of the push's four, so a CU keeps more tiles in
-// flight (their time per tile is the latency of a few dependent loads).
-// The flood and labels kernels lose at two waves (more registers per
-// thread for the flood, and its occupancy halves), so they keep the push's.
-template <int T, int NT>
-constexpr int kTileNT = T == 32 ? 128 : NT;
-
 template <int T, int NT>
 struct PoolKernels {
-  static constexpr int TN = kTileNT<T, NT>;
   static void push(Pool* p, int grid, int sweeps, bool first) {
     if (first)
       hipLaunchKernelGGL((k_push<T, NT, true>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
@@ -598,13 +589,19 @@ struct PoolKernels {
     hipLaunchKernelGGL((k_fold<T>), dim3(grid), dim3(kFoldNT<T>), 0, p->stream, p->geo, p->dev,
                        list, p->counts + cnt);
   }
+  // (grid: the most tiles the pass can list; a block takes kBfsTPB tiles
+  // at a time, and at most grid_cap blocks run)
+  static int bfs_grid(Pool* p, int tiles) {
+    return std::max(1, std::min((tiles + kBfsTPB<T> - 1) / kBfsTPB<T>, p->grid_cap));
+  }
   static void bfs_init(Pool* p, int grid, const int32_t* list, int cnt, bool all,
                        const Queue& q) {
-    hipLaunchKernelGGL((k_bfs_init<T, TN>), dim3(grid), dim3(TN), 0, p->stream, p->geo, p->dev,
+    grid = bfs_grid(p, grid);
+    hipLaunchKernelGGL((k_bfs_init<T>), dim3(grid), dim3(kBfsNT), 0, p->stream, p->geo, p->dev,
                        list, p->counts + cnt, all ? 1 : 0, q, p->counts + CNT_ACT);
   }
   static void bfs_round(Pool* p, int grid, const Queue& q) {
-    hipLaunchKernelGGL((k_bfs_round<T, TN>), dim3(grid), dim3(TN), 0, p->stream, p->geo, p->dev,
+    hipLaunchKernelGGL((k_bfs_round<T>), dim3(grid), dim3(kBfsNT), 0, p->stream, p->geo, p->dev,
                        q);
   }
   static void flood(Pool* p, int grid, const Queue& q, int mode, int32_t* lmark) {
@@ -1734,7 +1731,7 @@ int phase_relabel(const Solve& S, bool local, int rounds_dd, int* cur_out, int32
                     p->counts + CNT_EP + 1 - p->ep_par};
   p->ep_par ^= 1;
   S.B(K_BFS_INIT);
-  S.ops->bfs_init(p, S.grid_relabel, glist, gcnt, !local, first);
+  S.ops->bfs_init(p, S.nsel, glist, gcnt, !local, first);
   S.E();
   GCMI_HIP(hipGetLastError());
   int cur = 0;
