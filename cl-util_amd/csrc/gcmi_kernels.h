@@ -1365,64 +1365,98 @@ __device__ __forceinline__ void put_bits(uint32_t* packed, long long bitpos,
 
 // Row-major labels of the listed tiles' images from the reach bytes (a
 // tile writes its own T x T block of its image), as bytes and as bits of
-// the bit-packed masks.  A thread moves four pixels of a tile row: one
-// 4-byte reach load and, where the image's rows are 4-byte aligned, one
-// 4-byte label store; a row's bits are gathered from its T / 4 threads by
-// shuffles and merged into the packed words by the row's first thread.
-template <int T, int NT>
-__global__ void __launch_bounds__(NT)
+// the bit-packed masks.  A thread moves 16 pixels of a tile row: one
+// 16-byte reach load and, where the image's rows are 16-byte aligned, one
+// 16-byte label store; the T / 16 threads of a row gather its bits by
+// shuffles and its first thread merges them into the packed words.  A
+// block of kLabelsNT threads takes kLabelsTPB<T> tiles at a time, and the
+// loop is software-pipelined: the next tiles' geometry and reach bytes load
+// while this step's labels are stored (the memory a CU keeps in flight, not
+// the bytes, bounds this kernel).
+constexpr int kLabelsNT = 256;
+template <int T>
+constexpr int kLabelsTPB = kLabelsNT / (T * T / 16);
+
+template <int T>
+__global__ void __launch_bounds__(kLabelsNT)
 k_labels(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count) {
-  constexpr int TPR = T / 4;  // threads per tile row (an aligned group of lanes)
-  constexpr int ROWS = NT / TPR;  // rows per pass
-  constexpr int PASSES = (T + ROWS - 1) / ROWS;
+  constexpr int TPR = T / 16;          // threads per tile row
+  constexpr int TPT = T * T / 16;      // threads per tile
+  constexpr int TPB = kLabelsTPB<T>;
   static_assert(kWave % TPR == 0, "a row's threads lie in one wave");
-  const int j = threadIdx.x % TPR, r0 = threadIdx.x / TPR;
+  const int slot = threadIdx.x / TPT, j = threadIdx.x % TPR, ly = (threadIdx.x % TPT) / TPR;
   const int n = *count;
-  const int p0 = xcd_pos(blockIdx.x, gridDim.x);
-  int tnext = p0 < n ? list_tile(list, p0) : 0;
-  for (int pos = p0; pos < n; pos += gridDim.x) {
-    const TG q = tile_geo(g, tnext);
-    if (pos + (int)gridDim.x < n) tnext = list_tile(list, pos + gridDim.x);
-    const size_t base = tbase<T>(q.t);
-    const bool aligned = (q.W & 3) == 0;  // (pix0 is a multiple of 32)
+  const int stride = gridDim.x * TPB;
+  int pos = xcd_pos(blockIdx.x, gridDim.x) * TPB + slot;
+  // the tile after next (its list entry), and the next tile (its geometry
+  // and reach bytes)
+  int t2 = pos + stride < n ? list_tile(list, pos + stride) : 0;
+  TG qn{};
+  uint4 vn = make_uint4(0, 0, 0, 0);
+  if (pos < n) {
+    const int t1 = list_tile(list, pos);
+    qn = tile_geo(g, t1);
+    vn = *reinterpret_cast<const uint4*>(s.reach + tbase<T>(t1) + ly * T + 16 * j);
+  }
+  // (the loop runs while any tile of the block's step is listed: every
+  // thread takes part in the shuffles)
+  for (int base = xcd_pos(blockIdx.x, gridDim.x) * TPB; base < n; base += stride, pos += stride) {
+    const bool valid = pos < n;
+    const TG q = qn;
+    const uint4 v = vn;
+    if (pos + stride < n) {
+      const int t1 = t2;
+      if (pos + 2 * stride < n) t2 = list_tile(list, pos + 2 * stride);
+      qn = tile_geo(g, t1);
+      vn = *reinterpret_cast<const uint4*>(s.reach + tbase<T>(t1) + ly * T + 16 * j);
+    }
+    const int x = q.tx * T + 16 * j, y = q.ty * T + ly;
+    const int nin = (valid && y < q.H) ? min(max(q.W - x, 0), 16) : 0;
+    const unsigned w[4] = {v.x, v.y, v.z, v.w};
+    unsigned bits = 0;
 #pragma unroll
-    for (int ps = 0; ps < PASSES; ++ps) {
-      const int ly = r0 + ps * ROWS;
-      const bool row = ly < T;
-      const int x = q.tx * T + 4 * j, y = q.ty * T + ly;
-      const uint32_t v4 = row ? *reinterpret_cast<const uint32_t*>(s.reach + base + ly * T + 4 * j) : 0u;
-      const int nin = (row && y < q.H) ? min(max(q.W - x, 0), 4) : 0;
+    for (int i = 0; i < 4; ++i) bits |= byte_bits(w[i], 0) << (4 * i);
+    const unsigned m16 = nin >= 16 ? 0xffffu : ((1u << nin) - 1u);
+    bits &= m16;
+    if (nin > 0) {
       uint8_t* dst = s.labels + q.pix0 + (size_t)y * q.W + x;
-      if (nin == 4 && aligned) {
-        *reinterpret_cast<uint32_t*>(dst) = v4;
+      if (nin == 16 && (q.W & 15) == 0) {
+        *reinterpret_cast<uint4*>(dst) = v;
+      } else if ((q.W & 3) == 0) {
+        // (pix0 is a multiple of 32, x of 16)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (4 * i + 4 <= nin) {
+            reinterpret_cast<uint32_t*>(dst)[i] = w[i];
+          } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+              if (4 * i + k < nin) dst[4 * i + k] = (uint8_t)(w[i] >> (8 * k));
+          }
       } else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-          if (k < nin) dst[k] = (uint8_t)(v4 >> (8 * k));
+        for (int k = 0; k < 16; ++k)
+          if (k < nin) dst[k] = (uint8_t)(w[k / 4] >> (8 * (k % 4)));
       }
-      uint32_t nib = 0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-        if (k < nin && ((v4 >> (8 * k)) & 0xffu)) nib |= 1u << k;
-      const uint32_t nm = (1u << nin) - 1u;
-      // the row's bits: lane j's nibble at bit 4j (T = 64: two 32-bit halves)
-      const int sh = (4 * j) & 31;
-      const bool hi = 4 * j >= 32;
-      uint32_t blo = hi ? 0u : nib << sh, bhi = hi ? nib << sh : 0u;
-      uint32_t mlo = hi ? 0u : nm << sh, mhi = hi ? nm << sh : 0u;
-#pragma unroll
-      for (int off = TPR / 2; off > 0; off >>= 1) {
-        blo |= __shfl_xor(blo, off, kWave);
-        mlo |= __shfl_xor(mlo, off, kWave);
-        if constexpr (T == 64) {
-          bhi |= __shfl_xor(bhi, off, kWave);
-          mhi |= __shfl_xor(mhi, off, kWave);
-        }
-      }
-      const unsigned long long b = ((unsigned long long)bhi << 32) | blo;
-      const unsigned long long m = ((unsigned long long)mhi << 32) | mlo;
-      if (j == 0 && m) put_bits(s.packed, q.pix0 + (long long)y * q.W + q.tx * T, b, m);
     }
+    // the row's bits and mask: thread j's 16 at bit 16j (T = 64: two
+    // 32-bit halves)
+    const int sh = (16 * j) & 31;
+    const bool hi = 16 * j >= 32;
+    uint32_t blo = hi ? 0u : bits << sh, bhi = hi ? bits << sh : 0u;
+    uint32_t mlo = hi ? 0u : m16 << sh, mhi = hi ? m16 << sh : 0u;
+#pragma unroll
+    for (int off = TPR / 2; off > 0; off >>= 1) {
+      blo |= __shfl_xor(blo, off, kWave);
+      mlo |= __shfl_xor(mlo, off, kWave);
+      if constexpr (T == 64) {
+        bhi |= __shfl_xor(bhi, off, kWave);
+        mhi |= __shfl_xor(mhi, off, kWave);
+      }
+    }
+    const unsigned long long b = ((unsigned long long)bhi << 32) | blo;
+    const unsigned long long m = ((unsigned long long)mhi << 32) | mlo;
+    if (j == 0 && m) put_bits(s.packed, q.pix0 + (long long)y * q.W + q.tx * T, b, m);
   }
 }
 

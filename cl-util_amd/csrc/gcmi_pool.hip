@@ -608,8 +608,11 @@ struct PoolKernels {
     hipLaunchKernelGGL((k_flood<T, NT>), dim3(grid), dim3(kFloodNT), 0, p->stream, p->geo, p->dev, q,
                        mode, lmark);
   }
-  static void labels(Pool* p, int grid, const int32_t* list, int cnt) {
-    hipLaunchKernelGGL((k_labels<T, NT>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
+  // (tiles: the most tiles the list can hold)
+  static void labels(Pool* p, int tiles, const int32_t* list, int cnt) {
+    const int grid =
+        std::max(1, std::min((tiles + kLabelsTPB<T> - 1) / kLabelsTPB<T>, p->grid_cap));
+    hipLaunchKernelGGL((k_labels<T>), dim3(grid), dim3(kLabelsNT), 0, p->stream, p->geo, p->dev,
                        list, p->counts + cnt);
   }
   static void reset_tiles(Pool* p, int grid) {
@@ -1900,10 +1903,10 @@ int phase_results(const Solve& S, int zeroed) {
     // so their labels, did not change); the active list is free by now
     GCMI_TRY(compact_sel(p, p->dev.bflag, p->alist, CNT_ACT, true, S.nsel, nullptr, zeroed));
     S.B(K_LABELS);
-    S.ops->labels(p, S.grid_for(S.nsel), p->alist, CNT_ACT);
+    S.ops->labels(p, S.nsel, p->alist, CNT_ACT);
   } else {
     S.B(K_LABELS);
-    S.ops->labels(p, S.grid_for(S.nsel), p->lsel, CNT_SEL);
+    S.ops->labels(p, S.nsel, p->lsel, CNT_SEL);
   }
   S.E();
   GCMI_HIP(hipGetLastError());
