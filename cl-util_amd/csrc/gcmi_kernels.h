@@ -154,6 +154,26 @@ __device__ __forceinline__ long long block_sum64(long long v, long long* red) {
   return s;
 }
 
+// Block sums of N 64-bit values at once (two barriers for all of them);
+// thread 0 receives the sums, red holds N * NT / 64 entries.
+template <int NT, int N>
+__device__ __forceinline__ void block_sums64(long long (&v)[N], long long* red) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    v[i] = wave_sum64(v[i]);
+    if (lane_id() == 0) red[i * (NT / kWave) + threadIdx.x / kWave] = v[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      long long s = 0;
+      for (int w = 0; w < NT / kWave; ++w) s += red[i * (NT / kWave) + w];
+      v[i] = s;
+    }
+  __syncthreads();
+}
+
 // Zeroes counts[i] for every bit i of `mask` (the solver's list counts a
 // later launch appends to), from thread 0 of block 0.
 __device__ __forceinline__ void zero_counts(int32_t* counts, unsigned mask) {
@@ -304,7 +324,7 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
   __shared__ __align__(16) int32_t H0[HN], H1[HN];
   __shared__ __align__(16) int32_t F[4][T * T];
   __shared__ int32_t BO[4][T];
-  __shared__ long long red[NT / kWave];
+  __shared__ long long red[3 * (NT / kWave)];
   __shared__ int sflag;
   const int ly = threadIdx.x / CPR, x0 = (threadIdx.x % CPR) * R;
   const int32_t MAXH = g.maxh;
@@ -600,10 +620,14 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
         s.dirty[t + 1] = 1;
       }
     }
-    tp = block_sum64<NT>(tp, red);
     if (FIRST) {
-      smin = block_sum64<NT>(smin, red);
-      spos = block_sum64<NT>(spos, red);
+      long long v3[3] = {tp, smin, spos};
+      block_sums64<NT, 3>(v3, red);
+      tp = v3[0];
+      smin = v3[1];
+      spos = v3[2];
+    } else {
+      tp = block_sum64<NT>(tp, red);
     }
     if (threadIdx.x == 0) {
       s.tpos[t] = tp;

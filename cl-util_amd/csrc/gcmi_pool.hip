@@ -1532,17 +1532,20 @@ __device__ __forceinline__ void gate_last_block(const Gate& G) {
 // set, appended to `out` in any order with one atomic per block of
 // kFlagNT * kFlagPer positions; *count must be zero before.  clear: each
 // listed flag is zeroed.  imgskip (may be null): tiles of images flagged
-// there are left out (and not cleared).
+// there are left out (and not cleared).  done (may be null): unless *done
+// is set, nothing is listed or cleared (a device-driven solve's flood runs
+// only once the solve is done).
 constexpr int kFlagNT = 256;
 constexpr int kFlagPer = 4;
 
 __global__ void __launch_bounds__(kFlagNT)
 k_flag_list(const int32_t* __restrict__ sel, int n, int32_t* flags, int clear, int32_t* out,
             int32_t* count, const int32_t* __restrict__ tile_img,
-            const int32_t* __restrict__ imgskip, Gate G) {
+            const int32_t* __restrict__ imgskip, const int32_t* __restrict__ done, Gate G) {
   __shared__ int s_off[kFlagNT / kWave];
   __shared__ int s_base;
   const int lane = lane_id(), wave = threadIdx.x / kWave;
+  if (done && *done == 0) n = 0;
   const unsigned long long below = (1ull << lane) - 1;
   for (int base = blockIdx.x * kFlagNT * kFlagPer; base < n;
        base += gridDim.x * kFlagNT * kFlagPer) {
@@ -1598,7 +1601,8 @@ k_flag_list(const int32_t* __restrict__ sel, int n, int32_t* flags, int clear, i
 // the solve already zeroed the count (no fill launch).
 enum { COUNT_FILL = 0, COUNT_ZEROED = 1 };
 static int compact_sel(Pool* p, int32_t* flags, int32_t* out, int cnt, bool clear, int nsel,
-                       const int32_t* imgskip, int zeroed, const Gate& gate = Gate{}) {
+                       const int32_t* imgskip, int zeroed, const Gate& gate = Gate{},
+                       const int32_t* done = nullptr) {
   Prof* P = p->prof.on ? &p->prof : nullptr;
   if (P) P->begin(p->stream, K_COMPACT);
   if (zeroed != COUNT_ZEROED)
@@ -1607,7 +1611,7 @@ static int compact_sel(Pool* p, int32_t* flags, int32_t* out, int cnt, bool clea
   hipLaunchKernelGGL(k_flag_list,
                      dim3(std::max(1, std::min((nsel + per_block - 1) / per_block, 2048))),
                      dim3(kFlagNT), 0, p->stream, p->lsel, nsel, flags, clear ? 1 : 0, out,
-                     p->counts + cnt, p->tile_img, imgskip, gate);
+                     p->counts + cnt, p->tile_img, imgskip, done, gate);
   if (P) P->end(p->stream);
   GCMI_HIP(hipGetLastError());
   return GCMI_OK;
@@ -1842,6 +1846,13 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
   const size_t cap = (size_t)p->cap_tiles;
   const bool inc = S.nfresh < S.nsel;
   p->flood_inc = inc;
+  // A device-driven solve's flood is gated on its done flag: its lists stay
+  // empty, and the changed flags and reach bytes untouched, unless the
+  // solve finished.  (A flood over an unfinished solve's state would leave
+  // reach computed for that state -- and, with its rounds cut short,
+  // incomplete -- in tiles whose changed flags it consumed, which the
+  // incremental flood after the host finishes the solve would then trust.)
+  const int32_t* done = S.dd ? p->counts + CNT_DONE : nullptr;
   // (an incremental flood flags the tiles it processes in bflag, all zero
   // since the last relabel's compaction, for their labels)
   int32_t* lmark = inc ? p->dev.bflag : nullptr;
@@ -1852,10 +1863,12 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
     // (a fresh image's tiles are all in the set already, and so are their
     // neighbours: the closure starts from the warm images' changed tiles)
     // (k_zero_posf zeroed the counts of these compactions)
-    GCMI_TRY(compact_sel(p, p->dev.fchg, p->q, CNT_Q, false, S.nsel, p->d_imgfresh, COUNT_ZEROED));
+    GCMI_TRY(compact_sel(p, p->dev.fchg, p->q, CNT_Q, false, S.nsel, p->d_imgfresh, COUNT_ZEROED,
+                         Gate{}, done));
     hipLaunchKernelGGL(k_imp_closure, dim3(1), dim3(1024), 0, p->stream, p->geo, p->dev, p->q,
                        p->counts + CNT_Q, p->q + cap, p->q + 2 * cap);
-    GCMI_TRY(compact_sel(p, p->dev.fchg, p->plist, CNT_POS, true, S.nsel, nullptr, COUNT_ZEROED));
+    GCMI_TRY(compact_sel(p, p->dev.fchg, p->plist, CNT_POS, true, S.nsel, nullptr, COUNT_ZEROED,
+                         Gate{}, done));
     S.ops->clear_reach(p, std::min(S.nsel, 4096), p->plist, CNT_POS, lmark);
     hipLaunchKernelGGL(k_first_pass_list, dim3(std::min((S.nsel + 255) / 256, 2048)), dim3(256),
                        0, p->stream, p->geo, p->dev, p->plist, p->counts + CNT_POS,
@@ -1864,7 +1877,8 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
     // from scratch: the tiles holding excess seed the first pass (haspos
     // persists: a clean tile keeps the flag its last relabel wrote)
     GCMI_TRY(
-        compact_sel(p, p->dev.haspos, p->plist, CNT_POS, false, S.nsel, nullptr, COUNT_ZEROED));
+        compact_sel(p, p->dev.haspos, p->plist, CNT_POS, false, S.nsel, nullptr, COUNT_ZEROED,
+                    Gate{}, done));
     S.ops->clear_reach(p, std::min(S.nsel, 4096), p->lsel, CNT_SEL, nullptr);
   }
   GCMI_HIP(hipGetLastError());
@@ -1901,7 +1915,8 @@ int phase_results(const Solve& S, int zeroed) {
   if (p->flood_inc) {
     // the labels of the tiles the flood processed (the others' reach, and
     // so their labels, did not change); the active list is free by now
-    GCMI_TRY(compact_sel(p, p->dev.bflag, p->alist, CNT_ACT, true, S.nsel, nullptr, zeroed));
+    GCMI_TRY(compact_sel(p, p->dev.bflag, p->alist, CNT_ACT, true, S.nsel, nullptr, zeroed, Gate{},
+                         S.dd ? p->counts + CNT_DONE : nullptr));
     S.B(K_LABELS);
     S.ops->labels(p, S.nsel, p->alist, CNT_ACT);
   } else {

@@ -113,10 +113,13 @@ def test_local_relabel_batch_and_server():
             assert (mask == O.bk_cut(*cur[i].arrays())[1]).all(), i
 
 
-@pytest.mark.parametrize("device_rounds", [0, 4])
+@pytest.mark.parametrize("device_rounds", [0, 1, 2, 4])
 def test_local_relabel_mixed_fresh_and_warm(device_rounds):
     """A run that cuts some images from new capacities and reflows strokes on
-    the others, with local relabels (host- or device-driven)."""
+    the others, with local relabels (host- or device-driven).  With few
+    device rounds the device-driven part of a solve often does not finish,
+    and the host goes on from its state: the flood must then run once, over
+    the finished solve."""
     dims = [(300, 220), (128, 300), (257, 129), (200, 200), (64, 500)]
     works = [G.three_ellipses(w, h, seed=60 + i) for i, (w, h) in enumerate(dims)]
     strokes = [G.strokes(w, h, count=4, seed=70 + i) for i, (w, h) in enumerate(dims)]
@@ -179,3 +182,42 @@ def test_far_excess_behind_zero_excess_corridor(tile, device_rounds):
         assert fo > 0
         assert gc.cut() == fo, (tile, device_rounds, k)
         assert (gc.labels() == mo).all(), (tile, device_rounds, k)
+
+
+@pytest.mark.parametrize("seed", [21, 28, 31, 33, 35, 37])
+def test_unfinished_device_run_floods_once(seed):
+    """Batches mixing images cut from new capacities with reflowed strokes,
+    with 1-3 device BFS rounds per relabel, so that the device-driven part of
+    a solve often ends unfinished and the host goes on from its state.  The
+    minimal source set must come from one flood over the finished solve: a
+    flood over the unfinished state would leave reach the later incremental
+    flood trusts.  (These seeds gave wrong masks, with the right flows, when
+    the device-driven flood ran regardless of the done flag.)"""
+    rng0 = np.random.default_rng(seed)
+    n = 5
+    dims = [tuple(int(v) for v in rng0.integers(64, 320, 2)) for _ in range(n)]
+    dr = int(rng0.integers(1, 4))
+    works = [G.three_ellipses(w, h, seed=1000 * seed + i) for i, (w, h) in enumerate(dims)]
+    strokes = [G.strokes(w, h, count=3, seed=2000 * seed + i) for i, (w, h) in enumerate(dims)]
+    b = GraphCutBatch(dims, tile=32)
+    b.set_local_relabel(True)
+    b.set_device_rounds(dr)
+    for i, w in enumerate(works):
+        b.upload(i, *w.arrays())
+    b.cut()
+    cur = [w.copy() for w in works]
+    for r in range(3):
+        for i in range(n):
+            if (i + r + seed) % 3 == 0:
+                cur[i] = G.tiny(dims[i][0], dims[i][1], 500 + 10 * r + i + 100 * seed, cap_hi=400,
+                                w_hi=50)
+                b.upload(i, *cur[i].arrays())
+            else:
+                st = strokes[i][r]
+                s, t = st.capacities()
+                b.paint(i, st.points, st.width, s, t)
+                G.apply_stroke(cur[i], st)
+        flows = b.cut()
+        for i in range(n):
+            fo, mo = O.bk_cut(*cur[i].arrays())
+            assert flows[i] == fo and (b.labels(i) == mo).all(), (r, i)
