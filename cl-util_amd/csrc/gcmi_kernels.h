@@ -957,13 +957,16 @@ __device__ __forceinline__ void bfs_pass(const Geo& g, CutDev& s, const int32_t*
       for (int side = 0; side < 4; ++side) {
         if (!ex[side]) continue;
         const size_t p = tbase<T>(nbt[side]) + npx[side];
-        const int mq = s.mask[p], hq = s.h[p];
+        const int mq = s.mask[p];
         if (INIT) {
+          // (a neighbour relabelled in the same pass needs only its mask;
+          // the halo columns cost a line per pixel, so its heights are not
+          // loaded)
           const bool listed = all || s.rflag[nbt[side]] == bq.epoch;
-          hv[side] = listed ? ((mq & M_SINK) ? 0 : MAXH) : hq;
+          hv[side] = listed ? ((mq & M_SINK) ? 0 : MAXH) : s.h[p];
           hls[side] = listed ? 1 : 0;
         } else {
-          hv[side] = hq;
+          hv[side] = s.h[p];
           e0[side] = s.h[tb + epx[side]];
         }
         // (the arc of the halo pixel that points into this tile)
