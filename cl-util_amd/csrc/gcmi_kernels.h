@@ -121,6 +121,30 @@ __device__ __forceinline__ TG tile_geo(const Geo& g, int t) {
 }
 
 
+// The same two reads through the scalar cache (the geometry lines and tile
+// lists do not change while a kernel runs), for a block-uniform tile: they
+// count against the scalar-load counter, so waiting for them never waits
+// for the wave's vector loads and stores in flight.
+using ConstI32 = const __attribute__((address_space(4))) int32_t;
+__device__ __forceinline__ TG tile_geo_s(const Geo& g, int t) {
+  t = __builtin_amdgcn_readfirstlane(t);
+  ConstI32* c = (ConstI32*)g.tgeo + 8 * (size_t)t;
+  TG r;
+  r.t = t;
+  r.img = c[0];
+  r.tx = c[1] & 0xffff;
+  r.ty = (int)((unsigned)c[1] >> 16);
+  r.tw = c[2] & 0xffff;
+  r.th = (int)((unsigned)c[2] >> 16);
+  r.W = c[3];
+  r.H = c[4];
+  r.pix0 = (long long)(((unsigned long long)(unsigned)c[6] << 32) | (unsigned)c[5]);
+  return r;
+}
+__device__ __forceinline__ int list_tile_s(const int32_t* list, int pos) {
+  return list ? __builtin_amdgcn_readfirstlane(((ConstI32*)list)[pos]) : pos;
+}
+
 template <int T>
 __device__ __forceinline__ size_t tbase(int t) {
   return (size_t)t * (T * T);
@@ -139,6 +163,21 @@ __device__ __forceinline__ int xcd_pos(int b, int grid) {
   const int x = b % kXcds, k = b / kXcds;
   const int per = grid / kXcds, rem = grid % kXcds;
   return x * per + min(x, rem) + k;
+}
+
+// The list positions block b of `grid` takes, XCD by XCD: XCD x (the blocks
+// b % 8 == x) owns a contiguous eighth of the n positions and its blocks
+// take them in turn, so the tiles an XCD holds at once are neighbours and
+// the row above them was its own recent work (in its L2).  With one block
+// per position this is xcd_pos.
+struct XcdRun {
+  int pos, step, end;
+};
+__device__ __forceinline__ XcdRun xcd_run(int b, int grid, int n) {
+  const int x = b % kXcds, k = b / kXcds;
+  const int per = n / kXcds, rem = n % kXcds;
+  const int lo = x * per + min(x, rem);
+  return XcdRun{lo + k, grid / kXcds + (x < grid % kXcds ? 1 : 0), lo + per + (x < rem ? 1 : 0)};
 }
 
 // Block-wide sum of a 64-bit value; result valid in thread 0.
@@ -301,11 +340,79 @@ __device__ __forceinline__ void store_row(int32_t* p, const int (&v)[R]) {
   }
 }
 
-// Waves per SIMD the push kernel is compiled for: at T = 32 the compiler's
-// own choice (97 VGPRs) misses 5 waves by one register, and 5 waves run
-// round 1 about 15% faster despite an 8-byte spill.
+// Waves per SIMD the push kernel is compiled for: at T = 32 four blocks of
+// 256 threads per CU (at five the sweep loop spills registers).
 template <int T>
-constexpr int kPushWaves = T == 32 ? 5 : 1;
+constexpr int kPushWaves = T == 32 ? 4 : 1;
+
+// The value `v` of the lane one to the left (right) in this lane's 16-lane
+// row, by DPP (0 where that lane lies outside the row).  A tile row's
+// threads are consecutive lanes aligned to their count (8 or 16), so for
+// every thread but the first (last) of a tile row this is the thread
+// holding the pixels to its left (right).  Every lane of the wave must be
+// active.
+__device__ __forceinline__ int from_left_lane(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+}
+__device__ __forceinline__ int from_right_lane(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x101, 0xf, 0xf, false);  // row_shl:1
+}
+
+// Round 1's inputs of one thread's segment (row-major capacities): unaries,
+// right and down weights, the down weight of the row above and the right
+// weight left of the segment (0 outside the image), and for the halo
+// loaders the unaries of their halo pixel.
+template <int R>
+struct PushIn {
+  int a[R], b[R], rr[R], dd[R], du[R];
+  int rl;
+  int ha, hb;
+  bool hin;
+};
+
+template <int T, int R>
+__device__ __forceinline__ void push_in_load(const CutDev& s, const TG& q, int ly, int x0,
+                                             PushIn<R>& in) {
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    in.a[k] = 0; in.b[k] = 0; in.rr[k] = 0; in.dd[k] = 0; in.du[k] = 0;
+  }
+  in.rl = 0;
+  const int y = q.ty * T + ly, xg = q.tx * T + x0;
+  if (y < q.H && xg < q.W) {
+    const size_t p = q.pix0 + (size_t)y * q.W + xg;
+    if (R > 1 && q.W % R == 0) {
+      // whole aligned segment (pixel ranges start on a multiple of 4)
+      load_row<R>(s.cs + p, in.a);
+      load_row<R>(s.ct + p, in.b);
+      load_row<R>(s.right + p, in.rr);
+      load_row<R>(s.down + p, in.dd);
+      if (y > 0) load_row<R>(s.down + p - q.W, in.du);
+    } else {
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+        if (xg + k < q.W) {
+          in.a[k] = s.cs[p + k];
+          in.b[k] = s.ct[p + k];
+          in.rr[k] = s.right[p + k];
+          in.dd[k] = s.down[p + k];
+          if (y > 0) in.du[k] = s.down[p + k - q.W];
+        }
+    }
+    if (xg > 0) in.rl = s.right[p - 1];
+  }
+  in.hin = false;
+  in.ha = in.hb = 0;
+  if (threadIdx.x < 4 * T) {
+    const Halo hs = halo_slot<T>(q, threadIdx.x);
+    if (hs.nb >= 0 && hs.gx < q.W && hs.gy < q.H) {
+      const size_t p = q.pix0 + (size_t)hs.gy * q.W + hs.gx;
+      in.hin = true;
+      in.ha = s.cs[p];
+      in.hb = s.ct[p];
+    }
+  }
+}
 
 template <int T, int NT, bool FIRST>
 __global__ void __launch_bounds__(NT, kPushWaves<T>)
@@ -320,80 +427,66 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
   constexpr int HS = T + 8;
   constexpr int HN = (T + 2) * HS;
   static_assert(4 * T <= NT, "halo loaders");
-  static_assert(T % R == 0, "row segments");
+  static_assert(T % R == 0 && 16 % CPR == 0, "a tile row's threads share a 16-lane row");
   __shared__ __align__(16) int32_t H0[HN], H1[HN];
-  __shared__ __align__(16) int32_t F[4][T * T];
-  __shared__ int32_t BO[4][T];
+  // a sweep's vertical flows (the horizontal ones pass between lanes)
+  __shared__ __align__(16) int32_t FU[T * T], FD[T * T];
+  // the residual of each edge pixel's arc out of the tile at the start of
+  // the sweeps (up, down, left, right side); written and read back by the
+  // thread owning the pixel: the flow it sent out is the start value minus
+  // the final one (nothing flows in over such an arc during a residency)
+  __shared__ __align__(16) int32_t EC[4][T];
   __shared__ long long red[3 * (NT / kWave)];
   __shared__ int sflag;
-  const int ly = threadIdx.x / CPR, x0 = (threadIdx.x % CPR) * R;
+  const int seg = threadIdx.x % CPR;
+  const int ly = threadIdx.x / CPR, x0 = seg * R;
+  const bool row0 = ly == 0, rowL = ly == T - 1, seg0 = seg == 0, segL = seg == CPR - 1;
   const int32_t MAXH = g.maxh;
   const int n = *count;
   if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) {
     atomicAdd(&s.tally[TALLY_PUSH_TILES], (unsigned long long)n);
     atomicAdd(&s.tally[TALLY_PUSH_ROUNDS], 1ull);
   }
-  const int rpos = xcd_pos(blockIdx.x, gridDim.x);
+  const XcdRun run = xcd_run(blockIdx.x, gridDim.x, n);
+  const int G = run.step, rpos = run.pos, rend = run.end;
   ImgAcc amin, apos;
-  int tnext = rpos < n ? list_tile(list, rpos) : 0;
-  for (int pos = rpos; pos < n; pos += gridDim.x) {
-    const TG q = tile_geo(g, tnext);
-    if (pos + (int)gridDim.x < n) tnext = list_tile(list, pos + gridDim.x);
+  int tcur = rpos < rend ? list_tile_s(list, rpos) : 0;
+  int tnext = rpos + G < rend ? list_tile_s(list, rpos + G) : 0;
+  const int hrow = (ly + 1) * HS + 4 + x0;  // this thread's segment in H0 / H1
+  const int i0 = ly * T + x0;               // ... in the flow planes
+  const int iu = (row0 ? 0 : ly - 1) * T + x0, id = (rowL ? ly : ly + 1) * T + x0;
+  for (int pos = rpos; pos < rend; pos += G) {
+    const TG q = tile_geo_s(g, tcur);
     const int t = q.t;
     const size_t row = tbase<T>(t) + ly * T + x0;  // this thread's segment, tile-blocked
     int e[R], h[R], cu[R], cd[R], cl[R], cr[R];
     long long smin = 0, spos = 0;
-    if (FIRST) {
+    int hv0 = MAXH;  // the halo loaders' halo height
+    if constexpr (FIRST) {
+      PushIn<R> in;
+      push_in_load<T, R>(s, q, ly, x0, in);
+      const int y = q.ty * T + ly, xg = q.tx * T + x0;
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        e[k] = 0; cu[k] = 0; cd[k] = 0; cl[k] = 0; cr[k] = 0;
+        const bool in_x = xg + k < q.W;
+        cr[k] = xg + k < q.W - 1 ? in.rr[k] : 0;
+        cl[k] = in_x ? (k == 0 ? in.rl : in.rr[k - 1]) : 0;
+        cd[k] = y < q.H - 1 ? in.dd[k] : 0;
+        cu[k] = in.du[k];
+        e[k] = in.a[k] - in.b[k];
+        smin += min(in.a[k], in.b[k]);
+        spos += e[k] > 0 ? e[k] : 0;
+        h[k] = e[k] < 0 ? 0 : 1;
       }
-      const int y = q.ty * T + ly, xg = q.tx * T + x0;
-      if (y < q.H && xg < q.W) {
-        const size_t p = q.pix0 + (size_t)y * q.W + xg;
-        if (R > 1 && q.W % R == 0) {
-          // whole aligned segment (pixel ranges start on a multiple of 4)
-          int a[R], b[R], rr[R], dd[R], du[R];
-          load_row<R>(s.cs + p, a);
-          load_row<R>(s.ct + p, b);
-          load_row<R>(s.right + p, rr);
-          load_row<R>(s.down + p, dd);
-          if (y > 0) {
-            load_row<R>(s.down + p - q.W, du);
-          } else {
-#pragma unroll
-            for (int k = 0; k < R; ++k) du[k] = 0;
-          }
-          const int rl = xg > 0 ? s.right[p - 1] : 0;
-#pragma unroll
-          for (int k = 0; k < R; ++k) {
-            cr[k] = xg + k < q.W - 1 ? rr[k] : 0;
-            cl[k] = k == 0 ? rl : rr[k - 1];
-            cd[k] = y < q.H - 1 ? dd[k] : 0;
-            cu[k] = du[k];
-            e[k] = a[k] - b[k];
-            smin += min(a[k], b[k]);
-            spos += e[k] > 0 ? e[k] : 0;
-          }
-        } else {
-#pragma unroll
-          for (int k = 0; k < R; ++k) {
-            const int x = xg + k;
-            if (x < q.W) {
-              const int a = s.cs[p + k], b = s.ct[p + k];
-              cr[k] = x < q.W - 1 ? s.right[p + k] : 0;
-              cl[k] = x > 0 ? s.right[p + k - 1] : 0;
-              cd[k] = y < q.H - 1 ? s.down[p + k] : 0;
-              cu[k] = y > 0 ? s.down[p + k - q.W] : 0;
-              e[k] = a - b;
-              smin += min(a, b);
-              spos += e[k] > 0 ? e[k] : 0;
-            }
-          }
-        }
+      if (in.hin) hv0 = in.ha - in.hb < 0 ? 0 : 1;
+      // the image sums' wave totals go to LDS now (their registers are
+      // free for the sweeps)
+      smin = wave_sum64(smin);
+      spos = wave_sum64(spos);
+      if (lane_id() == 0) {
+        red[1 * (NT / kWave) + threadIdx.x / kWave] = smin;
+        red[2 * (NT / kWave) + threadIdx.x / kWave] = spos;
       }
-#pragma unroll
-      for (int k = 0; k < R; ++k) h[k] = e[k] < 0 ? 0 : 1;
     } else {
       load_row<R>(s.e + row, e);
       load_row<R>(s.h + row, h);
@@ -401,43 +494,37 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
       load_row<R>(s.cd + row, cd);
       load_row<R>(s.cl + row, cl);
       load_row<R>(s.cr + row, cr);
+      if (threadIdx.x < 4 * T) {
+        const Halo hs = halo_slot<T>(q, threadIdx.x);
+        if (hs.nb >= 0) hv0 = s.h[tbase<T>(hs.nb) + hs.ny * T + hs.nx];
+      }
     }
-    const int hrow = (ly + 1) * HS + 4 + x0;  // this thread's segment in H0 / H1
+    tcur = tnext;
+    if (pos + 2 * G < rend) tnext = list_tile_s(list, pos + 2 * G);
     store_row<R>(H0 + hrow, h);
     store_row<R>(H1 + hrow, h);
     if (threadIdx.x < 4 * T) {
-      const Halo hs = halo_slot<T>(q, threadIdx.x);
       const int side = threadIdx.x / T, j = threadIdx.x % T;
       const int hidx = side == 0   ? 4 + j
                        : side == 1 ? (T + 1) * HS + 4 + j
                        : side == 2 ? (j + 1) * HS + 3
                                    : (j + 1) * HS + 4 + T;
-      int v = MAXH;
-      if (hs.nb >= 0) {
-        if (FIRST) {
-          if (hs.gx < q.W && hs.gy < q.H) {
-            const size_t p = q.pix0 + (size_t)hs.gy * q.W + hs.gx;
-            v = s.cs[p] - s.ct[p] < 0 ? 0 : 1;
-          }
-        } else {
-          v = s.h[tbase<T>(hs.nb) + hs.ny * T + hs.nx];
-        }
-      }
-      H0[hidx] = v;
-      H1[hidx] = v;
-      BO[side][j] = 0;
+      H0[hidx] = hv0;
+      H1[hidx] = hv0;
     }
     // round 1 starts from reset tiles: nothing pending
     if (threadIdx.x == 0) sflag = FIRST ? 0 : s.pflag[t];
     __syncthreads();
+    // the halo heights at this row's ends (frozen for the residency)
+    const int hL = H0[(ly + 1) * HS + 3], hR = H0[(ly + 1) * HS + 4 + T];
     if (sflag) {
       // issue every exchange first, then use the results (one wait)
       int vu[R], vd[R], vl[R], vr[R];
 #pragma unroll
       for (int k = 0; k < R; ++k) {
         const int x = x0 + k;
-        vu[k] = ly == 0 ? atomicExch(&s.pend[((size_t)t * 4 + 0) * T + x], 0) : 0;
-        vd[k] = ly == T - 1 ? atomicExch(&s.pend[((size_t)t * 4 + 1) * T + x], 0) : 0;
+        vu[k] = row0 ? atomicExch(&s.pend[((size_t)t * 4 + 0) * T + x], 0) : 0;
+        vd[k] = rowL ? atomicExch(&s.pend[((size_t)t * 4 + 1) * T + x], 0) : 0;
         vl[k] = x == 0 ? atomicExch(&s.pend[((size_t)t * 4 + 2) * T + ly], 0) : 0;
         vr[k] = x == T - 1 ? atomicExch(&s.pend[((size_t)t * 4 + 3) * T + ly], 0) : 0;
       }
@@ -450,6 +537,9 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
         cr[k] += vr[k];
       }
     }
+    if (row0 || rowL) store_row<R>(&EC[row0 ? 0 : 1][x0], row0 ? cu : cd);
+    if (seg0) EC[2][ly] = cl[0];
+    if (segL) EC[3][ly] = cr[R - 1];
     int any0 = 0;
 #pragma unroll
     for (int k = 0; k < R; ++k) any0 |= e[k] > 0 && h[k] < MAXH;
@@ -458,113 +548,90 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
     for (int sw = 0; sw < sweeps && active; ++sw) {
       const int32_t* Hc = (sw & 1) ? H1 : H0;
       int32_t* Hn = (sw & 1) ? H0 : H1;
-      // push along admissible arcs, down / up / right / left, from the
-      // heights at the start of the sweep (rows above / below as vectors,
-      // left / right from this thread except at the segment's ends)
+      // heights at the start of the sweep: rows above / below from LDS,
+      // left / right from this thread or the neighbouring lanes (the halo
+      // at the tile's edges)
+      int hu[R], hd[R];
+      load_row<R>(Hc + hrow - HS, hu);
+      load_row<R>(Hc + hrow + HS, hd);
+      const int hlx = from_left_lane(h[R - 1]), hrx = from_right_lane(h[0]);
+      const int hl0 = seg0 ? hL : hlx, hrR = segL ? hR : hrx;
+      // push along admissible arcs, down / up / right / left; an inactive
+      // pixel pushes min(0, cap) = 0 (capacities are never negative)
       int fu[R], fd[R], fl[R], fr[R];
-      {
-        int hu[R], hd[R];
-        load_row<R>(Hc + hrow - HS, hu);
-        load_row<R>(Hc + hrow + HS, hd);
-        const int hl0 = Hc[hrow - 1], hrR = Hc[hrow + R];
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-          const int hl = k > 0 ? h[k - 1] : hl0;
-          const int hr = k < R - 1 ? h[k + 1] : hrR;
-          fu[k] = fd[k] = fl[k] = fr[k] = 0;
-          const int hv = h[k];
-          if (e[k] > 0 && hv < MAXH) {
-            int rem = e[k];
-            if (cd[k] > 0 && hv == hd[k] + 1) {
-              fd[k] = min(rem, cd[k]);
-              rem -= fd[k];
-              cd[k] -= fd[k];
-            }
-            if (rem > 0 && cu[k] > 0 && hv == hu[k] + 1) {
-              fu[k] = min(rem, cu[k]);
-              rem -= fu[k];
-              cu[k] -= fu[k];
-            }
-            if (rem > 0 && cr[k] > 0 && hv == hr + 1) {
-              fr[k] = min(rem, cr[k]);
-              rem -= fr[k];
-              cr[k] -= fr[k];
-            }
-            if (rem > 0 && cl[k] > 0 && hv == hl + 1) {
-              fl[k] = min(rem, cl[k]);
-              rem -= fl[k];
-              cl[k] -= fl[k];
-            }
-            e[k] = rem;
-            const int x = x0 + k;
-            if (ly == 0) BO[0][x] += fu[k];
-            if (ly == T - 1) BO[1][x] += fd[k];
-            if (x == 0) BO[2][ly] += fl[k];
-            if (x == T - 1) BO[3][ly] += fr[k];
-          }
-        }
+      for (int k = 0; k < R; ++k) {
+        const int hv = h[k];
+        const int hl = k > 0 ? h[k - 1] : hl0;
+        const int hr = k < R - 1 ? h[k + 1] : hrR;
+        int rem = (e[k] > 0 && hv < MAXH) ? e[k] : 0;
+        fd[k] = hv == hd[k] + 1 ? min(rem, cd[k]) : 0;
+        rem -= fd[k];
+        fu[k] = hv == hu[k] + 1 ? min(rem, cu[k]) : 0;
+        rem -= fu[k];
+        fr[k] = hv == hr + 1 ? min(rem, cr[k]) : 0;
+        rem -= fr[k];
+        fl[k] = hv == hl + 1 ? min(rem, cl[k]) : 0;
+        cd[k] -= fd[k];
+        cu[k] -= fu[k];
+        cr[k] -= fr[k];
+        cl[k] -= fl[k];
+        e[k] -= fd[k] + fu[k] + fr[k] + fl[k];
       }
-      const int i0 = ly * T + x0;  // this thread's segment in the flow planes
-      store_row<R>(F[D_UP] + i0, fu);
-      store_row<R>(F[D_DOWN] + i0, fd);
-      store_row<R>(F[D_LEFT] + i0, fl);
-      store_row<R>(F[D_RIGHT] + i0, fr);
+      store_row<R>(FU + i0, fu);
+      store_row<R>(FD + i0, fd);
+      // inflow from the left / right neighbours inside the tile: from the
+      // neighbouring lanes at the segment's ends, and inside the segment
+      // now (every pixel has pushed), so only two values outlive the barrier
+      const int vlx = from_left_lane(fr[R - 1]), vrx = from_right_lane(fl[0]);
+      const int vl0 = seg0 ? 0 : vlx, vrR = segL ? 0 : vrx;
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int vl = k > 0 ? fr[k - 1] : 0;
+        const int vr = k < R - 1 ? fl[k + 1] : 0;
+        e[k] += vl + vr;
+        cl[k] += vl;
+        cr[k] += vr;
+      }
       __syncthreads();
       // gather inflow: the sender's arc toward this pixel is spent, so the
       // reverse arc (back to the sender) gains the same capacity
-      int any = 0;
-      {
-        int vu[R], vd[R];
-        if (ly > 0) {
-          load_row<R>(F[D_DOWN] + i0 - T, vu);
-        } else {
+      int vu[R], vd[R];
+      load_row<R>(FD + iu, vu);
+      load_row<R>(FU + id, vd);
+      load_row<R>(Hc + hrow - HS, hu);
+      load_row<R>(Hc + hrow + HS, hd);
+      int any = 0, hn[R];
 #pragma unroll
-          for (int k = 0; k < R; ++k) vu[k] = 0;
-        }
-        if (ly < T - 1) {
-          load_row<R>(F[D_UP] + i0 + T, vd);
-        } else {
-#pragma unroll
-          for (int k = 0; k < R; ++k) vd[k] = 0;
-        }
-        const int vl0 = x0 > 0 ? F[D_RIGHT][i0 - 1] : 0;
-        const int vrR = x0 + R < T ? F[D_LEFT][i0 + R] : 0;
-        int hu[R], hd[R];
-        load_row<R>(Hc + hrow - HS, hu);
-        load_row<R>(Hc + hrow + HS, hd);
-        const int hl0 = Hc[hrow - 1], hrR = Hc[hrow + R];
-        int hn[R];
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-          const int vl = k > 0 ? fr[k - 1] : vl0;
-          const int vr = k < R - 1 ? fl[k + 1] : vrR;
-          e[k] += vl + vr + vu[k] + vd[k];
-          cl[k] += vl;
-          cr[k] += vr;
-          cu[k] += vu[k];
-          cd[k] += vd[k];
-          // relabel if still active with no admissible arc
-          const int hl = k > 0 ? h[k - 1] : hl0;
-          const int hr = k < R - 1 ? h[k + 1] : hrR;
-          const int hv = h[k];
-          hn[k] = hv;
-          if (e[k] > 0 && hv < MAXH) {
-            const bool adm = (cd[k] > 0 && hv == hd[k] + 1) || (cu[k] > 0 && hv == hu[k] + 1) ||
-                             (cr[k] > 0 && hv == hr + 1) || (cl[k] > 0 && hv == hl + 1);
-            if (!adm) {
-              int nh = MAXH;
-              if (cd[k] > 0) nh = min(nh, hd[k] + 1);
-              if (cu[k] > 0) nh = min(nh, hu[k] + 1);
-              if (cr[k] > 0) nh = min(nh, hr + 1);
-              if (cl[k] > 0) nh = min(nh, hl + 1);
-              hn[k] = nh;
-            }
-            any |= hn[k] < MAXH;
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < R; ++k) h[k] = hn[k];
+      for (int k = 0; k < R; ++k) {
+        const int inu = row0 ? 0 : vu[k], ind = rowL ? 0 : vd[k];
+        const int vl = k == 0 ? vl0 : 0;
+        const int vr = k == R - 1 ? vrR : 0;
+        e[k] += vl + vr + inu + ind;
+        cl[k] += vl;
+        cr[k] += vr;
+        cu[k] += inu;
+        cd[k] += ind;
+        // relabel if still active with no admissible arc, to the lowest
+        // neighbour over a residual arc + 1 (MAX if none)
+        const int hv = h[k];
+        const int hl = k > 0 ? h[k - 1] : hl0;
+        const int hr = k < R - 1 ? h[k + 1] : hrR;
+        const int nd = cd[k] > 0 ? hd[k] + 1 : MAXH;
+        const int nu = cu[k] > 0 ? hu[k] + 1 : MAXH;
+        const int nr = cr[k] > 0 ? hr + 1 : MAXH;
+        const int nl = cl[k] > 0 ? hl + 1 : MAXH;
+        const bool act = e[k] > 0 && hv < MAXH;
+        // (hv < MAX for an active pixel, so nX == hv means arc X is
+        // residual and admissible)
+        const bool adm = nd == hv || nu == hv || nr == hv || nl == hv;
+        const int nh = min(min(min(nd, nu), min(nr, nl)), MAXH);
+        const int nh2 = act && !adm ? nh : hv;
+        any |= act && nh2 < MAXH;
+        hn[k] = nh2;
       }
+#pragma unroll
+      for (int k = 0; k < R; ++k) h[k] = hn[k];
       store_row<R>(Hn + hrow, h);
       active = __syncthreads_or(any);
     }
@@ -592,40 +659,49 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
     }
     // edge outflow -> neighbour's pending slot (the receiving side is the
     // opposite one); a nonzero outflow implies the arc and so the neighbour
+    if ((row0 && q.ty > 0) || (rowL && q.ty < q.th - 1)) {
+      const int nb = row0 ? t - q.tw : t + q.tw;
+      int c0[R];
+      load_row<R>(&EC[row0 ? 0 : 1][x0], c0);
+      bool sent = false;
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const int x = x0 + k;
-      if (ly == 0 && q.ty > 0 && BO[0][x]) {
-        atomicAdd(&s.pend[((size_t)(t - q.tw) * 4 + 1) * T + x], BO[0][x]);
-        s.pflag[t - q.tw] = 1;
-        s.act[t - q.tw] = 1;
-        s.dirty[t - q.tw] = 1;
+      for (int k = 0; k < R; ++k) {
+        const int out = c0[k] - (row0 ? cu[k] : cd[k]);
+        if (out) {
+          atomicAdd(&s.pend[((size_t)nb * 4 + (row0 ? 1 : 0)) * T + x0 + k], out);
+          sent = true;
+        }
       }
-      if (ly == T - 1 && q.ty < q.th - 1 && BO[1][x]) {
-        atomicAdd(&s.pend[((size_t)(t + q.tw) * 4 + 0) * T + x], BO[1][x]);
-        s.pflag[t + q.tw] = 1;
-        s.act[t + q.tw] = 1;
-        s.dirty[t + q.tw] = 1;
+      if (sent) {
+        s.pflag[nb] = 1;
+        s.act[nb] = 1;
+        s.dirty[nb] = 1;
       }
-      if (x == 0 && q.tx > 0 && BO[2][ly]) {
-        atomicAdd(&s.pend[((size_t)(t - 1) * 4 + 3) * T + ly], BO[2][ly]);
-        s.pflag[t - 1] = 1;
-        s.act[t - 1] = 1;
-        s.dirty[t - 1] = 1;
-      }
-      if (x == T - 1 && q.tx < q.tw - 1 && BO[3][ly]) {
-        atomicAdd(&s.pend[((size_t)(t + 1) * 4 + 2) * T + ly], BO[3][ly]);
-        s.pflag[t + 1] = 1;
-        s.act[t + 1] = 1;
-        s.dirty[t + 1] = 1;
+    }
+    if ((seg0 && q.tx > 0) || (segL && q.tx < q.tw - 1)) {
+      const int nb = seg0 ? t - 1 : t + 1;
+      const int out = seg0 ? EC[2][ly] - cl[0] : EC[3][ly] - cr[R - 1];
+      if (out) {
+        atomicAdd(&s.pend[((size_t)nb * 4 + (seg0 ? 3 : 2)) * T + ly], out);
+        s.pflag[nb] = 1;
+        s.act[nb] = 1;
+        s.dirty[nb] = 1;
       }
     }
     if (FIRST) {
-      long long v3[3] = {tp, smin, spos};
-      block_sums64<NT, 3>(v3, red);
-      tp = v3[0];
-      smin = v3[1];
-      spos = v3[2];
+      // (rows 1 and 2 of red hold the image sums' wave totals)
+      tp = wave_sum64(tp);
+      if (lane_id() == 0) red[threadIdx.x / kWave] = tp;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        tp = smin = spos = 0;
+        for (int w = 0; w < NT / kWave; ++w) {
+          tp += red[w];
+          smin += red[NT / kWave + w];
+          spos += red[2 * (NT / kWave) + w];
+        }
+      }
+      __syncthreads();
     } else {
       tp = block_sum64<NT>(tp, red);
     }
