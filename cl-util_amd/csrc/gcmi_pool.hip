@@ -425,12 +425,27 @@ __device__ __forceinline__ bool rint_settled(double v, double rel) {
   return 0.5 - fabs(v - k) > rel * (fabs(v) + 1.0);
 }
 
-// Weight of a pair with squared colour difference d2; inv_tb = 1 / two_beta.
+// Weight of a pair with squared colour difference d2; inv_tb = 1 / two_beta,
+// inv_tb_f and scale_f: 1 / two_beta and pair_scale as float.  The
+// exponential weight is first tried in float32: with a = d2 * inv_tb_f
+// (d2 < 2^24 is exact), the float value scale_f * exp(-a) is within (a + 1)
+// 2^-22 of itself of the float64 one (the roundings of inv_tb_f, a, the
+// hardware exp's argument and result, scale_f and the product); where it
+// lies further than 4 (2a + 16) 2^-22 (w + 1) from every half-integer, rint
+// of either is the same integer; otherwise the float64 steps decide.  For
+// a > 80 the weight is below 2^24 e^-79.9 < 0.5 (the model caps pair_scale
+// at 2^24): 0.
 __device__ __forceinline__ int32_t rgb_pair_weight(const gcmi_rgb_model& m, int d2,
-                                                   double two_beta, double inv_tb) {
+                                                   double two_beta, double inv_tb,
+                                                   float inv_tb_f, float scale_f) {
 #pragma clang fp contract(off)
   const double d = (double)d2;
   if (m.pair_kind == GCMI_PAIR_EXP) {
+    const float a = (float)d2 * inv_tb_f;
+    if (a > 80.0f) return 0;
+    const float wf = scale_f * __expf(-a);
+    const float k = rintf(wf);
+    if (0.5f - fabsf(wf - k) > (2.0f * a + 16.0f) * 0x1p-20f * (wf + 1.0f)) return (int32_t)k;
     const double w = m.pair_scale * exp(-(d * inv_tb));
     if (rint_settled(w, 0x1p-38)) return (int32_t)rint(w);
     return (int32_t)rint(m.pair_scale * exp(-d / two_beta));
@@ -469,6 +484,7 @@ k_rgb_caps(const RgbJobs* __restrict__ J, int njobs, const unsigned long long* _
   const double two_beta = m.pair_kind == GCMI_PAIR_EXP
       ? rgb_two_beta(W, H, sums + (size_t)j * kSumSlots, &s_two_beta) : 0.0;
   const double inv_tb = m.pair_kind == GCMI_PAIR_EXP ? 1.0 / two_beta : 0.0;
+  const float inv_tb_f = (float)inv_tb, scale_f = (float)m.pair_scale;
   const double inv_div = 1.0 / m.unary_div;
   const long long pix0 = jobs[j].pix0;
   cs += pix0;
@@ -494,10 +510,11 @@ k_rgb_caps(const RgbJobs* __restrict__ J, int njobs, const unsigned long long* _
                    a[k] = rgb_unary(m, m.mu_bg, r, g, bl, inv_div);
                    b[k] = rgb_unary(m, m.mu_fg, r, g, bl, inv_div);
                    if (x + 1 < W)
-                     rr[k] = rgb_pair_weight(m, rgb_d2(q, q + ch), two_beta, inv_tb);
+                     rr[k] = rgb_pair_weight(m, rgb_d2(q, q + ch), two_beta, inv_tb, inv_tb_f,
+                                             scale_f);
                    if (i + W < n)
                      dd[k] = rgb_pair_weight(m, rgb_d2(q, D + ((i + W) * ch - dn0)), two_beta,
-                                             inv_tb);
+                                             inv_tb, inv_tb_f, scale_f);
                  }
                  if (++x == W) x = 0;
                }

@@ -156,6 +156,26 @@ int gcmi_free(void* d_ptr) {
   return GCMI_OK;
 }
 
+int gcmi_host_alloc(void** h_out, size_t bytes) {
+  if (!h_out) { set_error("null out"); return GCMI_EINVAL; }
+  GCMI_TRY(require_device());
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("hipHostMalloc(%zu) failed", bytes);
+    return GCMI_ENOMEM;
+  }
+  *h_out = p;
+  return GCMI_OK;
+}
+
+int gcmi_host_free(void* h_ptr) {
+  if (!h_ptr) return GCMI_OK;
+  GCMI_TRY(require_device());
+  GCMI_HIP(hipHostFree(h_ptr));
+  return GCMI_OK;
+}
+
 int gcmi_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes) {
   GCMI_TRY(require_device());
   if (!bytes) return GCMI_OK;

@@ -58,6 +58,8 @@ SIGNATURES = {
     "gcmi_device_count": [ctypes.POINTER(_int)],
     "gcmi_malloc": [ctypes.POINTER(_vp), _sz],
     "gcmi_free": [_vp],
+    "gcmi_host_alloc": [ctypes.POINTER(_vp), _sz],
+    "gcmi_host_free": [_vp],
     "gcmi_memcpy_h2d": [_vp, _vp, _sz],
     "gcmi_memcpy_d2h": [_vp, _vp, _sz],
     "gcmi_memcpy_d2d": [_vp, _vp, _sz],
@@ -291,6 +293,36 @@ def ptr(obj):
 
 def host_ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class _PinnedBlock:
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        check(lib().gcmi_host_alloc(ctypes.byref(p), max(1, int(nbytes))))
+        self.ptr = p.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().gcmi_host_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+
+def pinned_empty(shape, dtype):
+    """A numpy array in page-locked host memory (freed with the array)."""
+    dt = np.dtype(dtype)
+    n = int(np.prod(shape)) * dt.itemsize
+    blk = _PinnedBlock(n)
+    buf = (ctypes.c_uint8 * max(1, n)).from_address(blk.ptr)
+    buf._gcmi_block = blk  # the memory lives as long as the buffer
+    return np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
+
+
+def pinned_copy(a):
+    """A C-contiguous copy of `a` in page-locked host memory."""
+    a = np.asarray(a)
+    out = pinned_empty(a.shape, a.dtype)
+    out[...] = a
+    return out
 
 
 class DeviceBuffer:

@@ -49,6 +49,25 @@ class Mix:
         return b[(i + k) % len(b)]
 
 
+# Page-locked copies of the colour images the load generator replays (a few
+# base images, each sent many times), so that their uploads run at the
+# link's rate; made on first use, on the GPU side of a run.
+_PINNED = {}
+
+
+def pinned(a):
+    hit = _PINNED.get(id(a))
+    if hit is None or hit[0] is not a:
+        import gcmi_lib as L
+        hit = (a, L.pinned_copy(a))
+        _PINNED[id(a)] = hit
+    return hit[1]
+
+
+def _pin_payload(kind, payload):
+    return pinned(payload) if kind == "rgb" else payload
+
+
 def image_request(work, rgb):
     """(kind, payload) of a new-image request for a workload."""
     return ("rgb", work.image) if rgb else ("image", work.arrays())
@@ -64,7 +83,10 @@ def open_sessions(srv, mix, rgb=False):
 def reset_sessions(srv, sids, mix, rgb=False):
     """Gives every session its first base image again (a known state: the
     strokes of earlier traces are gone) and waits for the cuts."""
-    trace = [(0, sids[i]) + image_request(mix.base(i), rgb) for i in range(mix.nsess)]
+    trace = []
+    for i in range(mix.nsess):
+        kind, payload = image_request(mix.base(i), rgb)
+        trace.append((0, sids[i], kind, _pin_payload(kind, payload)))
     for r in srv.replay(trace):
         srv.wait(int(r))
         srv.forget(int(r))
@@ -97,28 +119,40 @@ def make_trace(mix, rate, duration, seed, t0_ns, image_frac=0.1, rgb=False):
     return strokes, images
 
 
+IMAGE_THREADS = 2
+
+
 def replay_trace(srv, sids, strokes, images):
-    """Submits a trace's strokes and new images from two generator threads
-    (so an image's host-side staging does not delay the strokes behind it);
-    returns {"s": stroke request ids, "i": image request ids}."""
+    """Submits a trace's strokes from one generator thread and its new
+    images from IMAGE_THREADS more (sessions split between them, so each
+    session's images stay in order), so that an image's upload does not
+    delay the strokes behind it; colour images go out from page-locked
+    copies.  Returns {"s": stroke request ids, "i": image request ids}."""
     ids = {}
     errors = []
+    parts = [[k for k, x in enumerate(images) if x[1] % IMAGE_THREADS == j]
+             for j in range(IMAGE_THREADS)]
 
     def go(name, tr):
         try:
-            ids[name] = srv.replay([(x[0], sids[x[1]], x[2], x[3]) for x in tr]) \
-                if tr else np.zeros(0, np.int64)
+            ids[name] = srv.replay([(x[0], sids[x[1]], x[2], _pin_payload(x[2], x[3]))
+                                    for x in tr]) if tr else np.zeros(0, np.int64)
         except Exception as e:  # reported by the caller
             errors.append(e)
 
-    th = [threading.Thread(target=go, args=("s", strokes)),
-          threading.Thread(target=go, args=("i", images))]
+    th = [threading.Thread(target=go, args=("s", strokes))]
+    th += [threading.Thread(target=go, args=(j, [images[k] for k in part]))
+           for j, part in enumerate(parts)]
     for x in th:
         x.start()
     for x in th:
         x.join()
     if errors:
         raise errors[0]
+    img = np.zeros(len(images), np.int64)
+    for j, part in enumerate(parts):
+        img[part] = ids.pop(j)
+    ids["i"] = img
     return ids
 
 
@@ -153,6 +187,8 @@ def run_trace(srv, sids, mix, rate, duration=2.0, seed=11, image_frac=0.1, lead=
            "strokes": len(strokes), "images": len(images),
            "stroke_rate": round(rate * len(strokes) / max(1, len(strokes) + len(images)), 1),
            "stroke_throughput": round(len(strokes) / max((t_end["s"] - t0) / 1e9, 1e-9), 1),
+           "image_rate": round(rate * len(images) / max(1, len(strokes) + len(images)), 1),
+           "image_throughput": round(len(images) / max((t_end["i"] - t0) / 1e9, 1e-9), 1),
            "stroke_p50_ms": pct(lat["stroke"], 50), "stroke_p99_ms": pct(lat["stroke"], 99),
            "image_p50_ms": pct(lat["image"], 50), "image_p99_ms": pct(lat["image"], 99),
            "mean_batch": round(float(np.mean(batch)), 2)}
@@ -164,12 +200,14 @@ def run_trace(srv, sids, mix, rate, duration=2.0, seed=11, image_frac=0.1, lead=
 
 def sustained(points, p99_ms=20.0, size=2048):
     """Highest offered rate whose strokes on `size` sessions kept p99 within
-    p99_ms and whose stroke throughput kept up with the offer (within 10%)."""
+    p99_ms and whose stroke and new-image throughputs kept up with the offer
+    (within 10%)."""
     best = None
     for p in points:
         p99 = p.get("stroke_p99_ms_%d" % size)
         if p99 is not None and p99 <= p99_ms and \
-                p["stroke_throughput"] >= 0.9 * p["stroke_rate"]:
+                p["stroke_throughput"] >= 0.9 * p["stroke_rate"] and \
+                p.get("image_throughput", 0) >= 0.9 * p.get("image_rate", 0):
             if best is None or p["rate"] > best["rate"]:
                 best = p
     return best

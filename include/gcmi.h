@@ -66,6 +66,12 @@ int gcmi_device_count(int *out);
  * these are the equivalents the Python layer uses. */
 int gcmi_malloc(void **d_out, size_t bytes);
 int gcmi_free(void *d_ptr);
+/* Page-locked host memory (hipHostMalloc): copies from it to the device run
+ * at the link's rate and need no staging.  No counterpart in the reference
+ * (pyopencl copies from ordinary host arrays); the load generator of the
+ * serving benchmark keeps its colour images in it. */
+int gcmi_host_alloc(void **h_out, size_t bytes);
+int gcmi_host_free(void *h_ptr);
 int gcmi_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes);
 int gcmi_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes);
 int gcmi_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes);

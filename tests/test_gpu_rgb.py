@@ -42,6 +42,14 @@ def images():
         out.append(rng.integers(0, 256, size=(h, w, 3), dtype=np.uint8))
     flat = np.full((40, 30, 3), 77, np.uint8)  # every d2 = 0: beta falls back to 1
     out.append(flat)
+    # every sum of two squares up to 2 * 255^2 as a horizontal pair (black |
+    # (a, b, 0)), and the vertical pairs between rows: the float32 first try
+    # of the exponential weight and its float64 fallback over the whole range
+    a, b = np.meshgrid(np.arange(256), np.arange(256), indexing="ij")
+    pairs = np.zeros((65536, 2, 3), np.uint8)
+    pairs[:, 1, 0] = a.reshape(-1)
+    pairs[:, 1, 1] = b.reshape(-1)
+    out.append(pairs)
     rgba = np.concatenate([out[1], rng.integers(0, 256, size=out[1].shape[:2] + (1,),
                                                 dtype=np.uint8)], axis=2)
     out.append(rgba)
