@@ -113,20 +113,17 @@ __global__ void k_clear_listed(int32_t* flags, const int32_t* __restrict__ list,
 
 
 // The tiles whose reach a flood must recompute: the changed tiles (fchg, the
-// `count` tiles of `init`) and, transitively, every tile that imported
-// reach (imp) next to one -- its reach may have come through the change.
-// One block, a worklist of the tiles added last (in buf0 / buf1, each room
-// for every tile); a tile enters once (the fchg swap).  Ends when a pass
-// adds nothing, after at most as many passes as there are tiles.
-__global__ void __launch_bounds__(1024)
-k_imp_closure(Geo g, CutDev s, const int32_t* __restrict__ init, const int32_t* __restrict__ count,
-              int32_t* buf0, int32_t* buf1) {
-  __shared__ int nnext;
-  const int32_t* cur = init;
-  int n = *count;
+// `n` tiles of `cur`) and, transitively, every tile that imported reach
+// (imp) next to one -- its reach may have come through the change.  One
+// block, a worklist of the tiles added last (in buf0 / buf1, each room for
+// every tile); a tile enters once (the fchg swap).  Ends when a pass adds
+// nothing, after at most as many passes as there are tiles.  All threads of
+// the block call it.
+__device__ void imp_closure(const Geo& g, CutDev& s, const int32_t* cur, int n, int32_t* buf0,
+                            int32_t* buf1, int* nnext) {
   int32_t* next = buf0;
   while (n > 0) {
-    if (threadIdx.x == 0) nnext = 0;
+    if (threadIdx.x == 0) *nnext = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
       const TG q = tile_geo(g, cur[i]);
@@ -143,14 +140,68 @@ k_imp_closure(Geo g, CutDev s, const int32_t* __restrict__ init, const int32_t* 
         was[k] = atomicOr(&s.fchg[nb[k] >= 0 ? nb[k] : q.t], im[k] ? 1 : 0);
 #pragma unroll
       for (int k = 0; k < 4; ++k)
-        if (im[k] && was[k] == 0) next[atomicAdd(&nnext, 1)] = nb[k];
+        if (im[k] && was[k] == 0) next[atomicAdd(nnext, 1)] = nb[k];
     }
     __syncthreads();
-    n = nnext;
+    n = *nnext;
     cur = next;
     next = next == buf0 ? buf1 : buf0;
     __syncthreads();
   }
+}
+
+__global__ void __launch_bounds__(1024)
+k_imp_closure(Geo g, CutDev s, const int32_t* __restrict__ init, const int32_t* __restrict__ count,
+              int32_t* buf0, int32_t* buf1) {
+  __shared__ int nnext;
+  imp_closure(g, s, init, *count, buf0, buf1, &nnext);
+}
+
+// A small incremental flood's start (at most kSmallSel tiles) in one block:
+// the images' final positive-excess sums (k_posf), the changed tiles of the
+// images reflowed warm (none unless *done, when given) and their import
+// closure -- one launch instead of three on the path of a single image's
+// reflow.  zmask: the counts zeroed, as k_posf's.  buf0 receives the
+// changed tiles; buf1 / buf2 hold the closure's worklists.
+constexpr int kSmallSel = 16384;
+__global__ void __launch_bounds__(1024)
+k_flood_start_small(Geo g, CutDev s, const int32_t* __restrict__ sel, int n, int32_t* counts,
+                    unsigned zmask, const int32_t* __restrict__ imgfresh,
+                    const int32_t* __restrict__ done, int32_t* buf0, int32_t* buf1, int32_t* buf2) {
+  __shared__ int nseed, nnext;
+  zero_counts(counts, zmask);
+  if (threadIdx.x == 0) nseed = 0;
+  __syncthreads();
+  const bool seed = !done || *done != 0;
+  const unsigned long long below = (1ull << lane_id()) - 1;
+  for (int base = 0; base < n; base += blockDim.x) {
+    const int i = base + threadIdx.x;
+    int img = -1, t = 0;
+    long long v = 0;
+    bool f = false;
+    if (i < n) {
+      t = sel ? sel[i] : i;
+      img = g.tile_img[t];
+      v = s.tpos[t];
+      f = seed && s.fchg[t] != 0 && !imgfresh[img];
+    }
+    const int img0 = __shfl(img, 0);
+    const unsigned long long row = SUM_POSF * kSumSlots + (base / blockDim.x) % kSumSlots;
+    if (__all(img == img0 || img < 0)) {
+      v = wave_sum64(v);
+      if (lane_id() == 0 && img0 >= 0 && v)
+        atomicAdd(&s.sums[(size_t)img0 * SUM_ROWS * kSumSlots + row], (unsigned long long)v);
+    } else if (img >= 0 && v) {
+      atomicAdd(&s.sums[(size_t)img * SUM_ROWS * kSumSlots + row], (unsigned long long)v);
+    }
+    const unsigned long long m = __ballot(f);
+    int at = 0;
+    if (lane_id() == 0 && m) at = atomicAdd(&nseed, __popcll(m));
+    at = __shfl(at, 0);
+    if (f) buf0[at + __popcll(m & below)] = t;
+  }
+  __syncthreads();
+  imp_closure(g, s, buf0, nseed, buf1, buf2, &nnext);
 }
 
 // Local relabel set: the dirty tiles (arcs or excess changed; a push dirties
@@ -549,34 +600,59 @@ __global__ void k_zero_posf(unsigned long long* sums, const Prep* __restrict__ p
   }
 }
 
-// The first pass of an incremental flood: of the recompute set `list`, the
-// tiles of images reflowed warm (their neighbours may hold reach to import)
-// and the tiles holding excess of images cut from scratch (the others get
-// reach only from their own image's tiles, through the rounds).  Appended
-// to `out` in any order, count in counts[CNT_FIRST].  Zeroes the queue
-// counts for the flood's rounds.
-__global__ void k_first_pass_list(Geo g, CutDev s, const int32_t* __restrict__ list,
-                                  const int32_t* __restrict__ count,
-                                  const int32_t* __restrict__ imgfresh, int32_t* out,
-                                  int32_t* counts) {
-  const int n = *count;
+// The incremental flood's recompute set in one pass over the solve's tiles
+// (sel, n of them): every tile flagged in fchg (the changed tiles and their
+// import closure) gets its changed and import flags cleared, its reach
+// bytes zeroed and lmark set (its labels are rewritten), and joins `out`,
+// the first pass's list (count counts[CNT_FIRST], zero before), if it holds
+// excess or belongs to an image reflowed warm (a fresh image's other tiles
+// get reach from its own tiles, through the rounds).  Gated on *done when
+// given (a device-driven solve floods only once it is done).  Block 0
+// zeroes the queue counts of the flood's rounds.
+constexpr int kFloodSetNT = 256;
+template <int T>
+__global__ void __launch_bounds__(kFloodSetNT)
+k_flood_set(Geo g, CutDev s, const int32_t* __restrict__ sel, int n, int32_t* lmark,
+            const int32_t* __restrict__ imgfresh, int32_t* out, int32_t* counts,
+            const int32_t* __restrict__ done) {
+  __shared__ int32_t tl[kFloodSetNT];
+  __shared__ int ntl;
   if (blockIdx.x == 0 && threadIdx.x == 0)
     counts[CNT_Q] = counts[CNT_Q + 1] = counts[CNT_Q + 2] = 0;
-  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+  if (done && *done == 0) n = 0;
+  const unsigned long long below = (1ull << lane_id()) - 1;
+  for (int base = blockIdx.x * kFloodSetNT; base < n; base += gridDim.x * kFloodSetNT) {
+    if (threadIdx.x == 0) ntl = 0;
+    __syncthreads();
     const int i = base + threadIdx.x;
-    bool keep = false;
     int t = 0;
+    bool f = false, first = false;
     if (i < n) {
-      t = list[i];
-      keep = s.haspos[t] || !imgfresh[g.tile_img[t]];
+      t = sel ? sel[i] : i;
+      f = s.fchg[t] != 0;
     }
-    const unsigned long long m = __ballot(keep);
+    if (f) {
+      s.fchg[t] = 0;
+      s.imp[t] = 0;
+      if (lmark) lmark[t] = 1;
+      first = s.haspos[t] || !imgfresh[g.tile_img[t]];
+      tl[atomicAdd(&ntl, 1)] = t;
+    }
+    const unsigned long long m = __ballot(first);
     int at = 0;
     if (lane_id() == 0 && m) at = atomicAdd(&counts[CNT_FIRST], __popcll(m));
     at = __shfl(at, 0);
-    if (keep) out[at + __popcll(m & ((1ull << lane_id()) - 1))] = t;
+    if (first) out[at + __popcll(m & below)] = t;
+    __syncthreads();
+    // the flagged tiles' reach bytes, 16 per thread
+    constexpr int V = T * T / 16;
+    const int nv = ntl * V;
+    for (int k = threadIdx.x; k < nv; k += kFloodSetNT)
+      reinterpret_cast<uint4*>(s.reach + tbase<T>(tl[k / V]))[k % V] = make_uint4(0, 0, 0, 0);
+    __syncthreads();  // tl and ntl are rewritten by the next step
   }
 }
+
 
 // Flow value of each selected image: sum min(cs, ct) + sum max(e0, 0)
 // - sum max(e, 0) (one wave per image adds the slots).
@@ -644,6 +720,11 @@ struct PoolKernels {
     hipLaunchKernelGGL((k_posf<T>), dim3(grid), dim3(256), 0, p->stream, p->geo, p->dev, p->lsel,
                        p->counts + CNT_SEL, p->counts, zmask);
   }
+  static void flood_set(Pool* p, int nsel, int32_t* lmark, const int32_t* done) {
+    const int grid = std::max(1, std::min((nsel + kFloodSetNT - 1) / kFloodSetNT, 2048));
+    hipLaunchKernelGGL((k_flood_set<T>), dim3(grid), dim3(kFloodSetNT), 0, p->stream, p->geo,
+                       p->dev, p->lsel, nsel, lmark, p->d_imgfresh, p->alist, p->counts, done);
+  }
   static void edit(Pool* p, int img, int n, const int32_t* idx, const int32_t* cs,
                    const int32_t* ct) {
     const int grid = std::min((n + 255) / 256, 1024);
@@ -678,6 +759,7 @@ static const PoolOps* ops_for() {
                               &PoolKernels<T, NT>::reset_tiles,
                               &PoolKernels<T, NT>::clear_reach,
                               &PoolKernels<T, NT>::posf,
+                              &PoolKernels<T, NT>::flood_set,
                               &PoolKernels<T, NT>::edit,
                               &PoolKernels<T, NT>::paint,
                               &PoolKernels<T, NT>::untile};
@@ -1857,9 +1939,8 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
     hipLaunchKernelGGL(k_zero_posf, dim3(S.n), dim3(64), 0, p->stream, p->dev.sums, p->d_prep,
                        p->counts);
   p->flood_ran = true;
-  S.ops->posf(p, std::min((S.nsel + 255) / 256, 1024),
-              (1u << CNT_FIRST) | (1u << CNT_POS) | (1u << CNT_ACT) | (1u << CNT_Q) |
-                  (1u << (CNT_Q + 1)) | (1u << (CNT_Q + 2)));
+  const unsigned zmask = (1u << CNT_FIRST) | (1u << CNT_POS) | (1u << CNT_ACT) | (1u << CNT_Q) |
+                         (1u << (CNT_Q + 1)) | (1u << (CNT_Q + 2));
   const size_t cap = (size_t)p->cap_tiles;
   const bool inc = S.nfresh < S.nsel;
   p->flood_inc = inc;
@@ -1873,6 +1954,15 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
   // (an incremental flood flags the tiles it processes in bflag, all zero
   // since the last relabel's compaction, for their labels)
   int32_t* lmark = inc ? p->dev.bflag : nullptr;
+  // a small incremental flood starts in one block (sums, changed tiles and
+  // their closure); otherwise the sums, a compaction and the closure
+  const bool small = inc && S.nsel <= kSmallSel;
+  if (small)
+    hipLaunchKernelGGL(k_flood_start_small, dim3(1), dim3(1024), 0, p->stream, p->geo, p->dev,
+                       p->lsel, S.nsel, p->counts, zmask, p->d_imgfresh, done, p->q, p->q + cap,
+                       p->q + 2 * cap);
+  else
+    S.ops->posf(p, std::min((S.nsel + 255) / 256, 1024), zmask);
   if (inc) {
     // incremental: recompute the changed tiles and the tiles whose reach
     // may have come through them; every other tile keeps its reach bytes
@@ -1880,16 +1970,15 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
     // (a fresh image's tiles are all in the set already, and so are their
     // neighbours: the closure starts from the warm images' changed tiles)
     // (k_zero_posf zeroed the counts of these compactions)
-    GCMI_TRY(compact_sel(p, p->dev.fchg, p->q, CNT_Q, false, S.nsel, p->d_imgfresh, COUNT_ZEROED,
-                         Gate{}, done));
-    hipLaunchKernelGGL(k_imp_closure, dim3(1), dim3(1024), 0, p->stream, p->geo, p->dev, p->q,
-                       p->counts + CNT_Q, p->q + cap, p->q + 2 * cap);
-    GCMI_TRY(compact_sel(p, p->dev.fchg, p->plist, CNT_POS, true, S.nsel, nullptr, COUNT_ZEROED,
-                         Gate{}, done));
-    S.ops->clear_reach(p, std::min(S.nsel, 4096), p->plist, CNT_POS, lmark);
-    hipLaunchKernelGGL(k_first_pass_list, dim3(std::min((S.nsel + 255) / 256, 2048)), dim3(256),
-                       0, p->stream, p->geo, p->dev, p->plist, p->counts + CNT_POS,
-                       p->d_imgfresh, p->alist, p->counts);
+    if (!small) {
+      GCMI_TRY(compact_sel(p, p->dev.fchg, p->q, CNT_Q, false, S.nsel, p->d_imgfresh,
+                           COUNT_ZEROED, Gate{}, done));
+      hipLaunchKernelGGL(k_imp_closure, dim3(1), dim3(1024), 0, p->stream, p->geo, p->dev, p->q,
+                         p->counts + CNT_Q, p->q + cap, p->q + 2 * cap);
+    }
+    // the set's tiles: flags cleared, reach zeroed, marked for labels, and
+    // the first pass's list
+    S.ops->flood_set(p, S.nsel, lmark, done);
   } else {
     // from scratch: the tiles holding excess seed the first pass (haspos
     // persists: a clean tile keeps the flag its last relabel wrote)
@@ -1899,7 +1988,7 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
     S.ops->clear_reach(p, std::min(S.nsel, 4096), p->lsel, CNT_SEL, nullptr);
   }
   GCMI_HIP(hipGetLastError());
-  // (the queue counts are zero: k_zero_posf, and k_first_pass_list after the
+  // (the queue counts are zero: k_zero_posf, and k_flood_set after the
   // incremental path's use of the queue as a compaction target)
   S.B(K_FLOOD);
   const Queue first{inc ? p->alist : p->plist,
