@@ -340,10 +340,10 @@ __device__ __forceinline__ void store_row(int32_t* p, const int (&v)[R]) {
   }
 }
 
-// Waves per SIMD the push kernel is compiled for: at T = 32 four blocks of
-// 256 threads per CU (at five the sweep loop spills registers).
+// Waves per SIMD the push kernel is compiled for: at T = 32 five blocks of
+// 256 threads per CU (96 registers).
 template <int T>
-constexpr int kPushWaves = T == 32 ? 4 : 1;
+constexpr int kPushWaves = T == 32 ? 5 : 1;
 
 // The value `v` of the lane one to the left (right) in this lane's 16-lane
 // row, by DPP (0 where that lane lies outside the row).  A tile row's
@@ -552,31 +552,40 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
       // left / right from this thread or the neighbouring lanes (the halo
       // at the tile's edges)
       int hu[R], hd[R];
-      load_row<R>(Hc + hrow - HS, hu);
-      load_row<R>(Hc + hrow + HS, hd);
       const int hlx = from_left_lane(h[R - 1]), hrx = from_right_lane(h[0]);
       const int hl0 = seg0 ? hL : hlx, hrR = segL ? hR : hrx;
       // push along admissible arcs, down / up / right / left; an inactive
-      // pixel pushes min(0, cap) = 0 (capacities are never negative)
+      // pixel pushes min(0, cap) = 0 (capacities are never negative); a
+      // wave with no active pixel pushes nothing
       int fu[R], fd[R], fl[R], fr[R];
+      int wact = 0;
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        const int hv = h[k];
-        const int hl = k > 0 ? h[k - 1] : hl0;
-        const int hr = k < R - 1 ? h[k + 1] : hrR;
-        int rem = (e[k] > 0 && hv < MAXH) ? e[k] : 0;
-        fd[k] = hv == hd[k] + 1 ? min(rem, cd[k]) : 0;
-        rem -= fd[k];
-        fu[k] = hv == hu[k] + 1 ? min(rem, cu[k]) : 0;
-        rem -= fu[k];
-        fr[k] = hv == hr + 1 ? min(rem, cr[k]) : 0;
-        rem -= fr[k];
-        fl[k] = hv == hl + 1 ? min(rem, cl[k]) : 0;
-        cd[k] -= fd[k];
-        cu[k] -= fu[k];
-        cr[k] -= fr[k];
-        cl[k] -= fl[k];
-        e[k] -= fd[k] + fu[k] + fr[k] + fl[k];
+        fu[k] = fd[k] = fl[k] = fr[k] = 0;
+        wact |= e[k] > 0 && h[k] < MAXH;
+      }
+      if (__any(wact)) {
+        load_row<R>(Hc + hrow - HS, hu);
+        load_row<R>(Hc + hrow + HS, hd);
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          const int hv = h[k];
+          const int hl = k > 0 ? h[k - 1] : hl0;
+          const int hr = k < R - 1 ? h[k + 1] : hrR;
+          int rem = (e[k] > 0 && hv < MAXH) ? e[k] : 0;
+          fd[k] = hv == hd[k] + 1 ? min(rem, cd[k]) : 0;
+          rem -= fd[k];
+          fu[k] = hv == hu[k] + 1 ? min(rem, cu[k]) : 0;
+          rem -= fu[k];
+          fr[k] = hv == hr + 1 ? min(rem, cr[k]) : 0;
+          rem -= fr[k];
+          fl[k] = hv == hl + 1 ? min(rem, cl[k]) : 0;
+          cd[k] -= fd[k];
+          cu[k] -= fu[k];
+          cr[k] -= fr[k];
+          cl[k] -= fl[k];
+          e[k] -= fd[k] + fu[k] + fr[k] + fl[k];
+        }
       }
       store_row<R>(FU + i0, fu);
       store_row<R>(FD + i0, fd);
@@ -599,9 +608,8 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
       int vu[R], vd[R];
       load_row<R>(FD + iu, vu);
       load_row<R>(FU + id, vd);
-      load_row<R>(Hc + hrow - HS, hu);
-      load_row<R>(Hc + hrow + HS, hd);
-      int any = 0, hn[R];
+      int any = 0;
+      wact = 0;
 #pragma unroll
       for (int k = 0; k < R; ++k) {
         const int inu = row0 ? 0 : vu[k], ind = rowL ? 0 : vd[k];
@@ -612,26 +620,36 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
         cr[k] += vr;
         cu[k] += inu;
         cd[k] += ind;
-        // relabel if still active with no admissible arc, to the lowest
-        // neighbour over a residual arc + 1 (MAX if none)
-        const int hv = h[k];
-        const int hl = k > 0 ? h[k - 1] : hl0;
-        const int hr = k < R - 1 ? h[k + 1] : hrR;
-        const int nd = cd[k] > 0 ? hd[k] + 1 : MAXH;
-        const int nu = cu[k] > 0 ? hu[k] + 1 : MAXH;
-        const int nr = cr[k] > 0 ? hr + 1 : MAXH;
-        const int nl = cl[k] > 0 ? hl + 1 : MAXH;
-        const bool act = e[k] > 0 && hv < MAXH;
-        // (hv < MAX for an active pixel, so nX == hv means arc X is
-        // residual and admissible)
-        const bool adm = nd == hv || nu == hv || nr == hv || nl == hv;
-        const int nh = min(min(min(nd, nu), min(nr, nl)), MAXH);
-        const int nh2 = act && !adm ? nh : hv;
-        any |= act && nh2 < MAXH;
-        hn[k] = nh2;
+        wact |= e[k] > 0 && h[k] < MAXH;
       }
+      // relabel each pixel still active with no admissible arc, to the
+      // lowest neighbour over a residual arc + 1 (MAX if none); a wave with
+      // no active pixel keeps its heights
+      if (__any(wact)) {
+        load_row<R>(Hc + hrow - HS, hu);
+        load_row<R>(Hc + hrow + HS, hd);
+        int hn[R];
 #pragma unroll
-      for (int k = 0; k < R; ++k) h[k] = hn[k];
+        for (int k = 0; k < R; ++k) {
+          const int hv = h[k];
+          const int hl = k > 0 ? h[k - 1] : hl0;
+          const int hr = k < R - 1 ? h[k + 1] : hrR;
+          const int nd = cd[k] > 0 ? hd[k] + 1 : MAXH;
+          const int nu = cu[k] > 0 ? hu[k] + 1 : MAXH;
+          const int nr = cr[k] > 0 ? hr + 1 : MAXH;
+          const int nl = cl[k] > 0 ? hl + 1 : MAXH;
+          const bool act = e[k] > 0 && hv < MAXH;
+          // (hv < MAX for an active pixel, so nX == hv means arc X is
+          // residual and admissible)
+          const bool adm = nd == hv || nu == hv || nr == hv || nl == hv;
+          const int nh = min(min(min(nd, nu), min(nr, nl)), MAXH);
+          const int nh2 = act && !adm ? nh : hv;
+          any |= act && nh2 < MAXH;
+          hn[k] = nh2;
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) h[k] = hn[k];
+      }
       store_row<R>(Hn + hrow, h);
       active = __syncthreads_or(any);
     }
