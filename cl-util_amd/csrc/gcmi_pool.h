@@ -112,6 +112,11 @@ inline long long pix_len(int W, int H) {
   return ((long long)W * H + kPixAlign - 1) / kPixAlign * kPixAlign;
 }
 
+constexpr size_t kStatusTally = 64;
+constexpr size_t kStatusFlows = 128;
+static_assert(CNT_N * 4 <= kStatusTally && kStatusTally + TALLY_N * 8 <= kStatusFlows,
+              "status block layout");
+
 struct PoolImage {
   bool live = false;
   int W = 0, H = 0, tw = 0, th = 0;
@@ -183,6 +188,15 @@ struct Pool {
   // the lists the kernels of the current solve read: null when the
   // selection is the whole pool in tile order (positions are tiles)
   int32_t *lsel = nullptr, *lfresh = nullptr;
+  // the solve's status block: counts, then tallies (dev.tally) and the
+  // images' flows, at byte offsets 0, kStatusTally and kStatusFlows, so
+  // one copy brings all three to the host at a solve's end
+  uint8_t* d_status = nullptr;
+  uint8_t* h_status = nullptr;
+  // the table d_prep holds (the last one copied), and whether a solve is
+  // being recorded (its copies become graph nodes)
+  std::vector<uint8_t> last_prep;
+  bool capturing = false;
   int32_t* counts = nullptr;
   int32_t* scratch = nullptr;
   Prep* d_prep = nullptr;

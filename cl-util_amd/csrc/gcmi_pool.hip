@@ -935,7 +935,6 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   ALLOC(d.imp, (size_t)cap_tiles * 4);
   ALLOC(d.tpos, (size_t)cap_tiles * 8);
   ALLOC(d.sums, (size_t)cap_imgs * SUM_ROWS * kSumSlots * 8);
-  ALLOC(d.tally, TALLY_N * 8);
   ALLOC(d.cs, (size_t)cap_pix * 4);
   ALLOC(d.ct, (size_t)cap_pix * 4);
   int32_t* right = nullptr;
@@ -954,23 +953,30 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   ALLOC(p->dlist, (size_t)cap_tiles * 4);
   ALLOC(p->plist, (size_t)cap_tiles * 4);
   ALLOC(p->alist, (size_t)cap_tiles * 4);
-  ALLOC(p->counts, CNT_N * 4);
+  ALLOC(p->d_status, kStatusFlows + (size_t)cap_imgs * 8);
   ALLOC(p->d_rgbsum, 8 * kSumSlots * kRgbJobsMax);
   ALLOC(p->d_imgfresh, (size_t)cap_imgs * 4);
   ALLOC(p->scratch, (size_t)compact_scratch_ints(cap_tiles) * 4);
   ALLOC(p->d_prep, (size_t)cap_imgs * sizeof(Prep));
-  ALLOC(p->d_flows, (size_t)cap_imgs * 8);
 #undef ALLOC
   if (rc == GCMI_OK &&
-      (hipHostMalloc((void**)&p->h_counts, CNT_N * 4, hipHostMallocDefault) != hipSuccess ||
+      (hipHostMalloc((void**)&p->h_status, kStatusFlows + (size_t)cap_imgs * 8,
+                     hipHostMallocDefault) != hipSuccess ||
        hipHostMalloc((void**)&p->h_prep, (size_t)cap_imgs * sizeof(Prep), hipHostMallocDefault) !=
-           hipSuccess ||
-       hipHostMalloc((void**)&p->h_flows, (size_t)cap_imgs * 8, hipHostMallocDefault) !=
-           hipSuccess ||
-       hipHostMalloc((void**)&p->h_tally, TALLY_N * 8, hipHostMallocDefault) != hipSuccess)) {
+           hipSuccess)) {
     (void)hipGetLastError();
     set_error("pinned host allocation failed");
     rc = GCMI_ENOMEM;
+  }
+  if (p->d_status) {
+    p->counts = reinterpret_cast<int32_t*>(p->d_status);
+    d.tally = reinterpret_cast<unsigned long long*>(p->d_status + kStatusTally);
+    p->d_flows = reinterpret_cast<long long*>(p->d_status + kStatusFlows);
+  }
+  if (p->h_status) {
+    p->h_counts = reinterpret_cast<int32_t*>(p->h_status);
+    p->h_tally = reinterpret_cast<unsigned long long*>(p->h_status + kStatusTally);
+    p->h_flows = reinterpret_cast<long long*>(p->h_status + kStatusFlows);
   }
   p->device_bytes = tot;
   if (rc != GCMI_OK) {
@@ -990,7 +996,7 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   int32_t* flags[] = {d.pflag, d.act, d.stamp, d.bflag, d.dirty, d.rflag, d.haspos, d.fchg, d.imp};
   for (int32_t* f : flags) GCMI_HIP(hipMemsetAsync(f, 0, (size_t)cap_tiles * 4, p->stream));
   GCMI_HIP(hipMemsetAsync(d.tpos, 0, (size_t)cap_tiles * 8, p->stream));
-  GCMI_HIP(hipMemsetAsync(p->counts, 0, CNT_N * 4, p->stream));
+  GCMI_HIP(hipMemsetAsync(p->d_status, 0, kStatusFlows, p->stream));
   GCMI_HIP(hipStreamSynchronize(p->stream));
   return GCMI_OK;
 }
@@ -1000,17 +1006,17 @@ void pool_free(Pool* p) {
   void* ptrs[] = {d.e,     d.h,       d.cu,     d.cd,       d.cl,         d.cr,       d.mask,
                   d.reach, d.pend,    d.pflag,  d.act,      d.stamp,      d.bflag,    d.dirty,
                   d.rflag, d.haspos,  d.fchg,   d.imp,      d.packed,
-                  d.tpos,  d.sums,    d.tally,  d.cs,       d.ct,         (void*)d.right,
+                  d.tpos,  d.sums,    d.cs,     d.ct,       (void*)d.right,
                   (void*)d.down,      d.labels, p->d_img,   p->tile_img,  p->sel,     p->fresh,
-                  p->q,    p->dlist,  p->plist, p->alist,   p->counts,    p->scratch, p->d_prep,
-                  p->d_flows, p->tgeo,
+                  p->q,    p->dlist,  p->plist, p->alist,   p->d_status,  p->scratch, p->d_prep,
+                  p->tgeo,
                   p->d_edit, p->d_rgbsum, p->d_rgbjobs, p->d_rgb, p->d_jobs, p->d_imgfresh};
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   p->graph_exec = nullptr;
   for (void* x : ptrs)
     if (x) (void)hipFree(x);
-  void* hp[] = {p->h_counts, p->h_prep, p->h_flows, p->h_tally, p->h_edit, p->h_jobs, p->h_rgbjobs};
+  void* hp[] = {p->h_status, p->h_prep, p->h_edit, p->h_jobs, p->h_rgbjobs};
   for (void* x : hp)
     if (x) (void)hipHostFree(x);
   p->prof.release();
@@ -1729,8 +1735,11 @@ static Queue round_queue(Pool* p, int cur) {
   return q;
 }
 
-static int read_counts(Pool* p) {
-  GCMI_HIP(hipMemcpyAsync(p->h_counts, p->counts, CNT_N * 4, hipMemcpyDeviceToHost, p->stream));
+// The counts to the host (and with nres > 0 the tallies and the first nres
+// images' flows, the rest of the status block, in the same copy); waits.
+static int read_counts(Pool* p, int nres = 0) {
+  const size_t bytes = nres > 0 ? kStatusFlows + (size_t)nres * 8 : CNT_N * 4;
+  GCMI_HIP(hipMemcpyAsync(p->h_status, p->d_status, bytes, hipMemcpyDeviceToHost, p->stream));
   GCMI_TRY(stream_sync(p));
   p->stats[STAT_SYNCS]++;
   // stamps hold epochs: restart both before the counter nears overflow
@@ -1763,9 +1772,17 @@ struct Solve {
 // Start of every solve: the image lists, sums, tallies and counts.
 int phase_prepare(const Solve& S) {
   Pool* p = S.p;
-  GCMI_HIP(hipMemcpyAsync(p->d_prep, p->h_prep, (size_t)S.n * sizeof(Prep),
-                          hipMemcpyHostToDevice, p->stream));
-  p->stage_pending[STAGE_PREP] = true;
+  // (the same image list as the last solve's -- a run of warm strokes on
+  // one image -- finds its table on the device already; a recording copies
+  // it on every replay)
+  const size_t pb = (size_t)S.n * sizeof(Prep);
+  if (p->capturing || p->last_prep.size() != pb ||
+      memcmp(p->last_prep.data(), p->h_prep, pb) != 0) {
+    GCMI_HIP(hipMemcpyAsync(p->d_prep, p->h_prep, pb, hipMemcpyHostToDevice, p->stream));
+    p->stage_pending[STAGE_PREP] = true;
+    p->last_prep.assign(reinterpret_cast<const uint8_t*>(p->h_prep),
+                        reinterpret_cast<const uint8_t*>(p->h_prep) + pb);
+  }
   hipLaunchKernelGGL(k_prepare, dim3(S.n), dim3(256), 0, p->stream, p->geo, p->dev, p->d_prep,
                      S.nsel, S.nfresh, p->sel, p->fresh, p->counts, p->d_imgfresh);
   GCMI_HIP(hipGetLastError());
@@ -2014,7 +2031,8 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
   return GCMI_OK;
 }
 
-// Labels, flows and tallies to the host buffers (no synchronisation).
+// Labels and flows (no synchronisation; read_counts with the solve's image
+// count then brings flows and tallies to the host in the counts' copy).
 // zeroed: the active count is zero (right after the flood's start).
 int phase_results(const Solve& S, int zeroed) {
   Pool* p = S.p;
@@ -2034,10 +2052,6 @@ int phase_results(const Solve& S, int zeroed) {
   hipLaunchKernelGGL(k_flows, dim3(S.n), dim3(64), 0, p->stream, p->dev.sums, p->d_prep,
                      p->d_flows);
   GCMI_HIP(hipGetLastError());
-  GCMI_HIP(hipMemcpyAsync(p->h_flows, p->d_flows, (size_t)S.n * 8, hipMemcpyDeviceToHost,
-                          p->stream));
-  GCMI_HIP(hipMemcpyAsync(p->h_tally, p->dev.tally, TALLY_N * 8, hipMemcpyDeviceToHost,
-                          p->stream));
   return GCMI_OK;
 }
 
@@ -2057,7 +2071,7 @@ int flood_rounds_host(const Solve& S, int cur) {
     }
     // labels and flows ride along with the check
     GCMI_TRY(phase_results(S, COUNT_FILL));
-    GCMI_TRY(read_counts(p));
+    GCMI_TRY(read_counts(p, S.n));
     if (p->trace)
       fprintf(stderr, "gcmi:   flood round -> queue %d (tiles so far %llu)\n", hc[CNT_Q + cur],
               p->h_tally[TALLY_FLOOD]);
@@ -2127,7 +2141,10 @@ static int run_recorded(Pool* p, const Solve& S, int* flood_cur) {
     }
     GCMI_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeRelaxed));
     int cur = 0;
+    p->capturing = true;
     const int rc = emit_device_driven(S, p->dd_rounds, &cur);
+    p->capturing = false;
+    p->last_prep.clear();
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(p->stream, &graph);
     if (rc != GCMI_OK) {
@@ -2265,7 +2282,7 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
       GCMI_TRY(run_recorded(p, R, &cur));
     else
       GCMI_TRY(emit_device_driven(R, p->dd_rounds, &cur));
-    GCMI_TRY(read_counts(p));
+    GCMI_TRY(read_counts(p, n));
     p->flood_ran = true;  // (a replayed recording ran its flood too)
     solved = hc[CNT_DONE] != 0;
     flooded = solved && hc[CNT_Q + cur] == 0;
