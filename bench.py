@@ -74,7 +74,7 @@ def parse():
     ap.add_argument("--bfs", type=int, default=0)
     ap.add_argument("--local", type=int, default=1,
                     help="plain cut: local relabels after the first (1) or full relabels (0)")
-    ap.add_argument("--rounds", type=int, default=2,
+    ap.add_argument("--rounds", type=int, default=1,
                     help="plain cut: device-driven BFS/flood rounds per relabel (0: host-driven)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-sweep", action="store_true", help="skip the tile sweep")
@@ -243,7 +243,7 @@ def warm_leg(args, exp):
     variants = {"warm_host": {}, "warm_local": dict(local=True),
                 "warm_device": dict(local=True, rounds=2),
                 "warm_device1": dict(local=True, rounds=1),
-                "scratch": dict(local=True, rounds=2)}
+                "scratch": dict(local=True, rounds=1)}
     for mode, opt in variants.items():
         gc = GraphCut(None, None, (W, H), tile=args.tile)
         gc.set_local_relabel(opt.get("local", False))
@@ -464,7 +464,7 @@ def batched_leg(args, exp):
         a = answer(flows[i], S.pack_mask(b.labels(i)), exp[i])
         nbad += not (a["flow_equal"] and a["mask_equal"])
     # sequential plain cuts with the same tile, sweeps and push rounds, and
-    # the plain cut's own run settings (local relabels, 2 device rounds)
+    # the plain cut's own run settings (local relabels, 1 device round)
     log("batched: 64 sequential plain cuts")
     cuts = []
     for i, d in enumerate(dev):

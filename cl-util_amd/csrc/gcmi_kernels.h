@@ -1581,41 +1581,6 @@ k_labels(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __res
   }
 }
 
-// Per listed tile: clear pending slots, pending flag, flags and reach (the
-// tile range may have held another image before).
-template <int T>
-__global__ void k_reset_tiles(CutDev s, const int32_t* __restrict__ list,
-                              const int32_t* __restrict__ count) {
-  const int n = *count;
-  for (int pos = blockIdx.x; pos < n; pos += gridDim.x) {
-    const int t = list ? list[pos] : pos;
-    for (int j = threadIdx.x; j < 4 * T; j += blockDim.x) s.pend[(size_t)t * 4 * T + j] = 0;
-    if (threadIdx.x == 0) {
-      s.pflag[t] = 0;
-      s.act[t] = 0;
-      s.haspos[t] = 0;
-    }
-  }
-}
-
-// Per listed tile: clear the reach bytes, the import flag and the changed
-// flag (before a flood recomputes the tile); flag it in lmark (may be null)
-// for its labels.
-template <int T>
-__global__ void k_clear_reach(CutDev s, const int32_t* __restrict__ list,
-                              const int32_t* __restrict__ count, int32_t* lmark) {
-  const int n = *count;
-  for (int pos = blockIdx.x; pos < n; pos += gridDim.x) {
-    const int t = list ? list[pos] : pos;
-    uint32_t* r = reinterpret_cast<uint32_t*>(s.reach + tbase<T>(t));
-    for (int j = threadIdx.x; j < T * T / 4; j += blockDim.x) r[j] = 0;
-    if (threadIdx.x == 0) {
-      s.imp[t] = 0;
-      s.fchg[t] = 0;
-      if (lmark) lmark[t] = 1;
-    }
-  }
-}
 
 // Tile-blocked int32 field of one image -> row-major padded image.
 template <int T>

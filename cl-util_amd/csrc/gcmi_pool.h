@@ -93,10 +93,9 @@ struct PoolOps {
   void (*bfs_round)(Pool*, int grid, const Queue& q);
   void (*flood)(Pool*, int grid, const Queue& q, int mode, int32_t* lmark);
   void (*labels)(Pool*, int grid, const int32_t* list, int cnt);
-  void (*reset_tiles)(Pool*, int grid);
-  void (*clear_reach)(Pool*, int grid, const int32_t* list, int cnt, int32_t* lmark);
   void (*posf)(Pool*, int grid, unsigned zmask);
   void (*flood_set)(Pool*, int nsel, int32_t* lmark, const int32_t* done);
+  void (*flood_start_full)(Pool*, int nsel, unsigned zmask, const int32_t* done);
   void (*edit)(Pool*, int img, int n, const int32_t* idx, const int32_t* cs, const int32_t* ct);
   void (*paint)(Pool*, const PaintJob* d_jobs, int njobs, int nrows);
   void (*untile)(Pool*, int img, const int32_t* src, int32_t* out);

@@ -236,28 +236,47 @@ __global__ void k_dilate(Geo g, CutDev s, const int32_t* __restrict__ list,
   }
 }
 
-// Builds the solve's tile lists, zeroes the sums it recomputes (all rows of
-// a fresh image, the final-excess row of every image) and the tallies, sets
-// the counts and the first queue epoch.
+// Builds the solve's tile lists, resets the fresh images' tiles (pending
+// slots and flags: the tile range may have held another image before),
+// zeroes the sums it recomputes (all rows of a fresh image, the
+// final-excess row of every image) and the tallies, sets the counts and the
+// first queue epoch.  Grid: (images, blocks per image); an image's tiles
+// are a contiguous range, so its lists and pending slots are filled as flat
+// arrays.  pend_per_tile: the pending slots of a tile (4T).
 __global__ void k_prepare(Geo g, CutDev s, const Prep* __restrict__ prep, int nsel_tiles,
                           int nfresh_tiles, int32_t* sel, int32_t* fresh, int32_t* counts,
-                          int32_t* imgfresh) {
+                          int32_t* imgfresh, int pend_per_tile) {
   const Prep p = prep[blockIdx.x];
-  if (threadIdx.x == 0) imgfresh[p.img] = p.fresh_off >= 0;
+  const bool fr = p.fresh_off >= 0;
+  if (blockIdx.y == 0 && threadIdx.x == 0) imgfresh[p.img] = fr;
   const ImgDesc d = g.img[p.img];
-  for (int j = threadIdx.x; j < d.ntiles; j += blockDim.x) {
-    sel[p.sel_off + j] = d.tile0 + j;
-    if (p.fresh_off >= 0) fresh[p.fresh_off + j] = d.tile0 + j;
+  const int stride = gridDim.y * blockDim.x;
+  for (int j = blockIdx.y * blockDim.x + threadIdx.x; j < d.ntiles; j += stride) {
+    const int t = d.tile0 + j;
+    sel[p.sel_off + j] = t;
+    if (fr) {
+      fresh[p.fresh_off + j] = t;
+      s.pflag[t] = 0;
+      s.act[t] = 0;
+      s.haspos[t] = 0;
+    }
   }
-  for (int j = threadIdx.x; j < SUM_ROWS * kSumSlots; j += blockDim.x)
-    if (p.fresh_off >= 0 || j / kSumSlots == SUM_POSF)
-      s.sums[(size_t)p.img * SUM_ROWS * kSumSlots + j] = 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (fr) {
+    int4* pe = reinterpret_cast<int4*>(s.pend + (size_t)d.tile0 * pend_per_tile);
+    const long long nv = (long long)d.ntiles * pend_per_tile / 4;
+    for (long long k = blockIdx.y * blockDim.x + threadIdx.x; k < nv; k += stride)
+      pe[k] = make_int4(0, 0, 0, 0);
+  }
+  if (blockIdx.y == 0)
+    for (int j = threadIdx.x; j < SUM_ROWS * kSumSlots; j += blockDim.x)
+      if (fr || j / kSumSlots == SUM_POSF) s.sums[(size_t)p.img * SUM_ROWS * kSumSlots + j] = 0;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
     counts[CNT_SEL] = nsel_tiles;
     counts[CNT_FRESH] = nfresh_tiles;
     counts[CNT_GSEL] = nsel_tiles;
     counts[CNT_DONE] = 0;
     counts[CNT_ACT] = 0;
+    counts[CNT_POS] = 0;
     // the first relabel's queue and local-set counts (a device-driven run
     // appends to them without a fill)
     counts[CNT_Q] = counts[CNT_Q + 1] = counts[CNT_Q + 2] = counts[CNT_DIRTY] = 0;
@@ -600,6 +619,59 @@ __global__ void k_zero_posf(unsigned long long* sums, const Prep* __restrict__ p
   }
 }
 
+// The start of a flood from scratch in one pass over the solve's tiles (sel,
+// n of them): each image's final positive-excess sum (of its tiles' sums,
+// as k_posf), the tiles holding excess (the first pass's list `out`, count
+// counts[CNT_POS], zero before; none unless *done, when given), and every
+// tile's reach bytes, import and changed flags cleared.  zmask: counts
+// block 0 zeroes (none of them is appended to here).
+constexpr int kFloodStartNT = 256;
+template <int T>
+__global__ void __launch_bounds__(kFloodStartNT)
+k_flood_start_full(Geo g, CutDev s, const int32_t* __restrict__ sel, int n, int32_t* counts,
+                   unsigned zmask, int32_t* out, const int32_t* __restrict__ done) {
+  __shared__ int32_t tl[kFloodStartNT];
+  zero_counts(counts, zmask);
+  const bool gate = !done || *done != 0;
+  const unsigned long long below = (1ull << lane_id()) - 1;
+  const unsigned long long row = SUM_POSF * kSumSlots + blockIdx.x % kSumSlots;
+  for (int base = blockIdx.x * kFloodStartNT; base < n; base += gridDim.x * kFloodStartNT) {
+    const int i = base + threadIdx.x;
+    int img = -1, t = 0;
+    long long v = 0;
+    bool f = false;
+    if (i < n) {
+      t = sel ? sel[i] : i;
+      img = g.tile_img[t];
+      v = s.tpos[t];
+      f = gate && s.haspos[t] != 0;
+      s.imp[t] = 0;
+      s.fchg[t] = 0;
+    }
+    tl[threadIdx.x] = t;
+    const int img0 = __shfl(img, 0);
+    if (__all(img == img0 || img < 0)) {
+      v = wave_sum64(v);
+      if (lane_id() == 0 && img0 >= 0 && v)
+        atomicAdd(&s.sums[(size_t)img0 * SUM_ROWS * kSumSlots + row], (unsigned long long)v);
+    } else if (img >= 0 && v) {
+      atomicAdd(&s.sums[(size_t)img * SUM_ROWS * kSumSlots + row], (unsigned long long)v);
+    }
+    const unsigned long long m = __ballot(f);
+    int at = 0;
+    if (lane_id() == 0 && m) at = atomicAdd(&counts[CNT_POS], __popcll(m));
+    at = __shfl(at, 0);
+    if (f) out[at + __popcll(m & below)] = t;
+    __syncthreads();
+    // the step's tiles' reach bytes, 16 per thread
+    constexpr int V = T * T / 16;
+    const int nv = min(kFloodStartNT, n - base) * V;
+    for (int k = threadIdx.x; k < nv; k += kFloodStartNT)
+      reinterpret_cast<uint4*>(s.reach + tbase<T>(tl[k / V]))[k % V] = make_uint4(0, 0, 0, 0);
+    __syncthreads();  // tl is rewritten by the next step
+  }
+}
+
 // The incremental flood's recompute set in one pass over the solve's tiles
 // (sel, n of them): every tile flagged in fchg (the changed tiles and their
 // import closure) gets its changed and import flags cleared, its reach
@@ -708,17 +780,14 @@ struct PoolKernels {
     hipLaunchKernelGGL((k_labels<T>), dim3(grid), dim3(kLabelsNT), 0, p->stream, p->geo, p->dev,
                        list, p->counts + cnt);
   }
-  static void reset_tiles(Pool* p, int grid) {
-    hipLaunchKernelGGL((k_reset_tiles<T>), dim3(grid), dim3(256), 0, p->stream, p->dev, p->lfresh,
-                       p->counts + CNT_FRESH);
-  }
-  static void clear_reach(Pool* p, int grid, const int32_t* list, int cnt, int32_t* lmark) {
-    hipLaunchKernelGGL((k_clear_reach<T>), dim3(grid), dim3(256), 0, p->stream, p->dev, list,
-                       p->counts + cnt, lmark);
-  }
   static void posf(Pool* p, int grid, unsigned zmask) {
     hipLaunchKernelGGL((k_posf<T>), dim3(grid), dim3(256), 0, p->stream, p->geo, p->dev, p->lsel,
                        p->counts + CNT_SEL, p->counts, zmask);
+  }
+  static void flood_start_full(Pool* p, int nsel, unsigned zmask, const int32_t* done) {
+    const int grid = std::max(1, std::min((nsel + kFloodStartNT - 1) / kFloodStartNT, 2048));
+    hipLaunchKernelGGL((k_flood_start_full<T>), dim3(grid), dim3(kFloodStartNT), 0, p->stream,
+                       p->geo, p->dev, p->lsel, nsel, p->counts, zmask, p->plist, done);
   }
   static void flood_set(Pool* p, int nsel, int32_t* lmark, const int32_t* done) {
     const int grid = std::max(1, std::min((nsel + kFloodSetNT - 1) / kFloodSetNT, 2048));
@@ -756,10 +825,9 @@ static const PoolOps* ops_for() {
                               &PoolKernels<T, NT>::bfs_round,
                               &PoolKernels<T, NT>::flood,
                               &PoolKernels<T, NT>::labels,
-                              &PoolKernels<T, NT>::reset_tiles,
-                              &PoolKernels<T, NT>::clear_reach,
                               &PoolKernels<T, NT>::posf,
                               &PoolKernels<T, NT>::flood_set,
+                              &PoolKernels<T, NT>::flood_start_full,
                               &PoolKernels<T, NT>::edit,
                               &PoolKernels<T, NT>::paint,
                               &PoolKernels<T, NT>::untile};
@@ -1783,8 +1851,11 @@ int phase_prepare(const Solve& S) {
     p->last_prep.assign(reinterpret_cast<const uint8_t*>(p->h_prep),
                         reinterpret_cast<const uint8_t*>(p->h_prep) + pb);
   }
-  hipLaunchKernelGGL(k_prepare, dim3(S.n), dim3(256), 0, p->stream, p->geo, p->dev, p->d_prep,
-                     S.nsel, S.nfresh, p->sel, p->fresh, p->counts, p->d_imgfresh);
+  int maxnt = 1;
+  for (int i = 0; i < S.n; ++i) maxnt = std::max(maxnt, p->imgs[p->h_prep[i].img].ntiles);
+  const int gy = std::min(1024, (maxnt + 63) / 64);
+  hipLaunchKernelGGL(k_prepare, dim3(S.n, gy), dim3(256), 0, p->stream, p->geo, p->dev, p->d_prep,
+                     S.nsel, S.nfresh, p->sel, p->fresh, p->counts, p->d_imgfresh, 4 * p->T);
   GCMI_HIP(hipGetLastError());
   p->ep_par = 0;
   p->flood_ran = false;
@@ -1796,7 +1867,7 @@ int phase_prepare(const Solve& S) {
 int phase_first(const Solve& S, int32_t* nact) {
   Pool* p = S.p;
   if (S.nfresh > 0) {
-    S.ops->reset_tiles(p, std::min(S.nfresh, 4096));
+    // (k_prepare reset the fresh images' tiles)
     S.B(K_PUSH_FIRST);
     S.ops->push(p, S.nfresh, S.sweeps, true);
     S.E();
@@ -1973,13 +2044,16 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
   int32_t* lmark = inc ? p->dev.bflag : nullptr;
   // a small incremental flood starts in one block (sums, changed tiles and
   // their closure); otherwise the sums, a compaction and the closure
+  // (a flood from scratch starts in one pass: sums, first list, reach)
   const bool small = inc && S.nsel <= kSmallSel;
   if (small)
     hipLaunchKernelGGL(k_flood_start_small, dim3(1), dim3(1024), 0, p->stream, p->geo, p->dev,
                        p->lsel, S.nsel, p->counts, zmask, p->d_imgfresh, done, p->q, p->q + cap,
                        p->q + 2 * cap);
-  else
+  else if (inc)
     S.ops->posf(p, std::min((S.nsel + 255) / 256, 1024), zmask);
+  else
+    S.ops->flood_start_full(p, S.nsel, zmask & ~(1u << CNT_POS), done);
   if (inc) {
     // incremental: recompute the changed tiles and the tiles whose reach
     // may have come through them; every other tile keeps its reach bytes
@@ -1996,14 +2070,9 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
     // the set's tiles: flags cleared, reach zeroed, marked for labels, and
     // the first pass's list
     S.ops->flood_set(p, S.nsel, lmark, done);
-  } else {
-    // from scratch: the tiles holding excess seed the first pass (haspos
-    // persists: a clean tile keeps the flag its last relabel wrote)
-    GCMI_TRY(
-        compact_sel(p, p->dev.haspos, p->plist, CNT_POS, false, S.nsel, nullptr, COUNT_ZEROED,
-                    Gate{}, done));
-    S.ops->clear_reach(p, std::min(S.nsel, 4096), p->lsel, CNT_SEL, nullptr);
   }
+  // (from scratch: the tiles holding excess seed the first pass; haspos
+  // persists, a clean tile keeping the flag its last relabel wrote)
   GCMI_HIP(hipGetLastError());
   // (the queue counts are zero: k_zero_posf, and k_flood_set after the
   // incremental path's use of the queue as a compaction target)
@@ -2380,7 +2449,6 @@ static int step_begin(Pool* p, int img) {
   const Solve S = step_solve(p);
   GCMI_TRY(phase_prepare(S));
   if (fresh) {
-    S.ops->reset_tiles(p, std::min(S.nfresh, 4096));
     S.ops->push(p, S.nfresh, 0, true);
     GCMI_HIP(hipGetLastError());
   }

@@ -62,7 +62,7 @@ def test_config1_4096_tiles(ellipses_4096, tile, bench_settings):
     gc = GraphCut(None, None, (4096, 4096), tile=tile)
     if bench_settings:
         gc.set_local_relabel(True)
-        gc.set_device_rounds(2)
+        gc.set_device_rounds(1)
     gc.upload(*w.arrays())
     for rep in range(2):  # a repeat cuts from scratch again
         assert gc.cut() == f, (tile, rep)
@@ -151,7 +151,7 @@ def test_config3_100_strokes(strokes_2048, variant):
 def test_config4_served_mix():
     mix = S.Mix(nsess=256, seed=4)
     with GraphCutServer(tile=32, hbm_budget=200 * 10**9, max_sessions=256, max_batch=64,
-                        mode="batched", local_relabel=True, device_rounds=2) as srv:
+                        mode="batched", local_relabel=True, device_rounds=1) as srv:
         sids = S.open_sessions(srv, mix, rgb=True)
         t0 = time.monotonic_ns() + int(0.2e9)
         strokes, images = S.make_trace(mix, 12000, 0.1, seed=11, t0_ns=t0, image_frac=0.1,
