@@ -1494,13 +1494,38 @@ __device__ __forceinline__ void put_bits(uint32_t* packed, long long bitpos,
 // loop is software-pipelined: the next tiles' geometry and reach bytes load
 // while this step's labels are stored (the memory a CU keeps in flight, not
 // the bytes, bounds this kernel).
+// The flows of a solve's images, which the last block of the labels launch
+// adds up besides its tiles (the sums are final by then): image i of the
+// solve is img[i * stride]; its flow is sum min(cs, ct) + sum max(e0, 0) -
+// sum max(e, 0) over the slotted sums, one wave per image.
+struct FlowOut {
+  const int32_t* img;
+  int stride, n;
+  long long* out;
+};
+__device__ __forceinline__ void image_flows(const unsigned long long* __restrict__ sums,
+                                            const FlowOut& F) {
+  const int nw = blockDim.x / kWave;
+  for (int i = threadIdx.x / kWave; i < F.n; i += nw) {
+    const unsigned long long* r = sums + (size_t)F.img[(size_t)i * F.stride] * SUM_ROWS * kSumSlots;
+    long long v = 0;
+    for (int j = lane_id(); j < kSumSlots; j += kWave)
+      v += (long long)r[SUM_MIN * kSumSlots + j] + (long long)r[SUM_POS0 * kSumSlots + j] -
+           (long long)r[SUM_POSF * kSumSlots + j];
+    v = wave_sum64(v);
+    if (lane_id() == 0) F.out[i] = v;
+  }
+}
+
 constexpr int kLabelsNT = 256;
 template <int T>
 constexpr int kLabelsTPB = kLabelsNT / (T * T / 16);
 
 template <int T>
 __global__ void __launch_bounds__(kLabelsNT)
-k_labels(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count) {
+k_labels(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
+         FlowOut F) {
+  if (blockIdx.x == gridDim.x - 1) image_flows(s.sums, F);
   constexpr int TPR = T / 16;          // threads per tile row
   constexpr int TPT = T * T / 16;      // threads per tile
   constexpr int TPB = kLabelsTPB<T>;
@@ -1704,7 +1729,7 @@ k_edit(Geo g, CutDev s, int img, int n, const int32_t* __restrict__ idx,
 // A brush stroke: the pixels whose centre (x, y) lies within width/2 of a
 // polyline -- a union of capsules, one per segment -- get unaries (cs, ct).
 constexpr int kMaxStrokePoints = 16;
-constexpr int kPaintCols = 256;  // columns of one row of one segment painted by one wave
+constexpr int kPaintCols = 64;  // columns of one row of one segment painted by one wave
 constexpr int kPaintRows = 4;    // rows of one segment painted by one wave
 struct Polyline {
   double x[kMaxStrokePoints], y[kMaxStrokePoints];

@@ -92,7 +92,7 @@ struct PoolOps {
   void (*bfs_init)(Pool*, int grid, const int32_t* list, int cnt, bool all, const Queue& q);
   void (*bfs_round)(Pool*, int grid, const Queue& q);
   void (*flood)(Pool*, int grid, const Queue& q, int mode, int32_t* lmark);
-  void (*labels)(Pool*, int grid, const int32_t* list, int cnt);
+  void (*labels)(Pool*, int grid, const int32_t* list, int cnt, int nflows);
   void (*posf)(Pool*, int grid, unsigned zmask);
   void (*flood_set)(Pool*, int nsel, int32_t* lmark, const int32_t* done);
   void (*flood_start_full)(Pool*, int nsel, unsigned zmask, const int32_t* done);

@@ -726,20 +726,6 @@ k_flood_set(Geo g, CutDev s, const int32_t* __restrict__ sel, int n, int32_t* lm
 }
 
 
-// Flow value of each selected image: sum min(cs, ct) + sum max(e0, 0)
-// - sum max(e, 0) (one wave per image adds the slots).
-__global__ void k_flows(const unsigned long long* __restrict__ sums, const Prep* __restrict__ prep,
-                        long long* out) {
-  const int img = prep[blockIdx.x].img;
-  const unsigned long long* r = sums + (size_t)img * SUM_ROWS * kSumSlots;
-  long long v = 0;
-  for (int j = threadIdx.x; j < kSumSlots; j += blockDim.x)
-    v += (long long)r[SUM_MIN * kSumSlots + j] + (long long)r[SUM_POS0 * kSumSlots + j] -
-         (long long)r[SUM_POSF * kSumSlots + j];
-  v = wave_sum64(v);
-  if (threadIdx.x == 0) out[blockIdx.x] = v;
-}
-
 template <int T, int NT>
 struct PoolKernels {
   static void push(Pool* p, int grid, int sweeps, bool first) {
@@ -774,11 +760,13 @@ struct PoolKernels {
                        mode, lmark);
   }
   // (tiles: the most tiles the list can hold)
-  static void labels(Pool* p, int tiles, const int32_t* list, int cnt) {
+  static void labels(Pool* p, int tiles, const int32_t* list, int cnt, int nflows) {
     const int grid =
         std::max(1, std::min((tiles + kLabelsTPB<T> - 1) / kLabelsTPB<T>, p->grid_cap));
+    const FlowOut F{reinterpret_cast<const int32_t*>(p->d_prep), (int)(sizeof(Prep) / 4), nflows,
+                    p->d_flows};
     hipLaunchKernelGGL((k_labels<T>), dim3(grid), dim3(kLabelsNT), 0, p->stream, p->geo, p->dev,
-                       list, p->counts + cnt);
+                       list, p->counts + cnt, F);
   }
   static void posf(Pool* p, int grid, unsigned zmask) {
     hipLaunchKernelGGL((k_posf<T>), dim3(grid), dim3(256), 0, p->stream, p->geo, p->dev, p->lsel,
@@ -2100,8 +2088,9 @@ int phase_flood_start(const Solve& S, int rounds_dd, int* cur_out) {
   return GCMI_OK;
 }
 
-// Labels and flows (no synchronisation; read_counts with the solve's image
-// count then brings flows and tallies to the host in the counts' copy).
+// Labels and flows in one launch (no synchronisation; read_counts with the
+// solve's image count then brings flows and tallies to the host in the
+// counts' copy).
 // zeroed: the active count is zero (right after the flood's start).
 int phase_results(const Solve& S, int zeroed) {
   Pool* p = S.p;
@@ -2111,15 +2100,12 @@ int phase_results(const Solve& S, int zeroed) {
     GCMI_TRY(compact_sel(p, p->dev.bflag, p->alist, CNT_ACT, true, S.nsel, nullptr, zeroed, Gate{},
                          S.dd ? p->counts + CNT_DONE : nullptr));
     S.B(K_LABELS);
-    S.ops->labels(p, S.nsel, p->alist, CNT_ACT);
+    S.ops->labels(p, S.nsel, p->alist, CNT_ACT, S.n);
   } else {
     S.B(K_LABELS);
-    S.ops->labels(p, S.nsel, p->lsel, CNT_SEL);
+    S.ops->labels(p, S.nsel, p->lsel, CNT_SEL, S.n);
   }
   S.E();
-  GCMI_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_flows, dim3(S.n), dim3(64), 0, p->stream, p->dev.sums, p->d_prep,
-                     p->d_flows);
   GCMI_HIP(hipGetLastError());
   return GCMI_OK;
 }
