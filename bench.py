@@ -228,6 +228,7 @@ def warm_leg(args, exp):
     """2048x2048, 100 strokes: warm reflow vs from scratch, per stroke
     paint + cut + packed mask readback; every stroke's flow and mask
     checked (outside the timed region)."""
+    import gcmi_lib as L
     import gcmi_serving as S
     import gcmi_workloads as G
     from GraphCut import GraphCut
@@ -250,7 +251,9 @@ def warm_leg(args, exp):
         gc.set_device_rounds(opt.get("rounds", 0))
         gc.upload(*work.arrays())
         gc.cut()
-        gc.labels_packed()
+        # the mask lands in a page-locked buffer, as a latency-bound client's would
+        mbuf = L.pinned_empty(((W * H + 31) // 32,), np.uint32)
+        gc.labels_packed(out=mbuf)
         lat = []
         nbad = 0
         for k, st in enumerate(strokes):
@@ -260,7 +263,7 @@ def warm_leg(args, exp):
             if mode == "scratch":
                 gc.forget()
             flow = gc.cut()
-            bits = gc.labels_packed()
+            bits = gc.labels_packed(out=mbuf)
             lat.append((time.perf_counter() - t0) * 1e3)
             a = answer(flow, bits, exp[k])
             nbad += not (a["flow_equal"] and a["mask_equal"])

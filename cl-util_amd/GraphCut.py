@@ -345,9 +345,16 @@ class GraphCut:
         check(lib().gcmi_cut_labels(self._h, host_ptr(out)))
         return out
 
-    def labels_packed(self):
-        """uint32 words, bit p % 32 of word p // 32 = label of pixel p."""
-        out = np.empty((self.shape[0] * self.shape[1] + 31) // 32, np.uint32)
+    def labels_packed(self, out=None):
+        """uint32 words, bit p % 32 of word p // 32 = label of pixel p.  `out`
+        (optional): a C-contiguous uint32 array of that many words to copy
+        into, e.g. page-locked (gcmi_lib.pinned_empty), which the copy
+        reaches faster than ordinary host memory."""
+        n = (self.shape[0] * self.shape[1] + 31) // 32
+        if out is None:
+            out = np.empty(n, np.uint32)
+        elif out.dtype != np.uint32 or out.size != n or not out.flags.c_contiguous:
+            raise ValueError("out: %d C-contiguous uint32 words" % n)
         check(lib().gcmi_cut_labels_packed(self._h, host_ptr(out)))
         return out
 
