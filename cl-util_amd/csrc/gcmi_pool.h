@@ -202,7 +202,7 @@ struct Pool {
   long long* d_flows = nullptr;
   int32_t* d_edit = nullptr;  // staging for edits: idx, cs, ct per edit
   int32_t* d_imgfresh = nullptr;           // per image: cut from scratch in this solve
-  unsigned long long* d_rgbsum = nullptr;  // pair-difference sum of a colour upload (kSumSlots slots)
+  unsigned long long* d_rgbsum = nullptr;  // pair-difference sums of a colour upload (padded slots)
   // colour uploads: the job table of a launch's images (allocated on first
   // use)
   void* h_rgbjobs = nullptr;

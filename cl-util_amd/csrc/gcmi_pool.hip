@@ -305,14 +305,19 @@ __device__ __forceinline__ int rgb_col(long long i, int W, bool small) {
 }
 
 // A chunk of the colour kernels is kRgbNT * kRgbPx consecutive pixels; a
-// block covers kRgbChunks consecutive chunks of one image and stages the
-// bytes of each in LDS: its pixels and the next one (right neighbours), and
-// the pixels one row below them.  The bytes of the next chunk are loaded
-// into registers while the block works on the current one.  Words per
-// staged range (up to 4 channels, plus the range's offset inside its first
-// word).
+// block covers a few consecutive chunks of one image and stages the bytes of
+// each in LDS: its pixels and the next one (right neighbours), and the
+// pixels one row below them.  The bytes of the next chunk are loaded into
+// registers while the block works on the current one.  Chunks per block:
+// the pair sums (light work per pixel) take 4, which hides each block's
+// set-up behind its later chunks; the capacities (float64 work, fewer waves
+// per SIMD) take 1, so that a 1024^2 image has four blocks per CU rather
+// than one working its chunks in turn (k_rgb_caps on 4096^2 / 1024^2: 97.6
+// / 19.7 us with 4 chunks, 82.2 / 11.0 us with 1; k_rgb_pair_sum: 42.2 /
+// 11.8 us with 4, 81.4 / 10.2 us with 1).  Words per staged range (up to 4
+// channels, plus the range's offset inside its first word).
 constexpr int kRgbBlockPx = kRgbNT * kRgbPx;
-constexpr int kRgbChunks = 4;
+constexpr int kRgbSumChunks = 4, kRgbCapsChunks = 1;
 constexpr int kRgbStageWords = (kRgbBlockPx + 1) + 1;
 constexpr int kRgbStageIters = (kRgbStageWords + kRgbNT - 1) / kRgbNT;
 
@@ -325,8 +330,14 @@ struct RgbWords {
 // Loads bytes [b0, b1) of a colour image of nb bytes (4-byte aligned),
 // clipped to the image: whole words where the image holds them, a last
 // partial word byte by byte.  Byte i lands at word (i - (b0 & ~3)) / 4.
-__device__ __forceinline__ void rgb_load(const uint8_t* __restrict__ rgb, long long nb,
+// (the image pointer comes from the job table, so the compiler cannot tell
+// that it is global memory: flat loads count against the LDS counter too,
+// and every wait for the staged LDS bytes would also wait for them)
+using GlobalU8 = const __attribute__((address_space(1))) uint8_t;
+using GlobalU32 = const __attribute__((address_space(1))) uint32_t;
+__device__ __forceinline__ void rgb_load(const uint8_t* __restrict__ rgb_flat, long long nb,
                                          long long b0, long long b1, RgbWords& w) {
+  GlobalU8* rgb = (GlobalU8*)rgb_flat;
   const long long a0 = b0 & ~3LL;
   b1 = b1 < nb ? b1 : nb;
   w.nw = b1 > a0 ? (int)((b1 - a0 + 3) >> 2) : 0;
@@ -337,7 +348,7 @@ __device__ __forceinline__ void rgb_load(const uint8_t* __restrict__ rgb, long l
     w.v[j] = 0;
     if (k < w.nw) {
       if (o + 4 <= nb) {
-        w.v[j] = *reinterpret_cast<const uint32_t*>(rgb + o);
+        w.v[j] = *(GlobalU32*)(rgb + o);
       } else {
         for (int i = 0; i < 4 && o + i < nb; ++i) w.v[j] |= (uint32_t)rgb[o + i] << (8 * i);
       }
@@ -367,7 +378,7 @@ __device__ __forceinline__ void rgb_load_chunk(const uint8_t* __restrict__ rgb, 
 // for each chunk at pixel p0 that starts inside the image, with its bytes
 // in LDS (pixel i at O + (i * ch - own0) for p0 <= i <= p0 + kRgbBlockPx,
 // the pixel a row below at D + ((i + W) * ch - dn0)).  All threads call it.
-template <class F>
+template <int CHUNKS, class F>
 __device__ __forceinline__ void rgb_chunks(const uint8_t* __restrict__ rgb, long long n, int W,
                                            int ch, long long first, F&& f) {
   __shared__ uint32_t own[kRgbStageWords], dn[kRgbStageWords];
@@ -375,44 +386,52 @@ __device__ __forceinline__ void rgb_chunks(const uint8_t* __restrict__ rgb, long
   const uint8_t* D = reinterpret_cast<const uint8_t*>(dn);
   RgbWords wo, wd;
   rgb_load_chunk(rgb, n, W, ch, first, wo, wd);
-  for (int c = 0; c < kRgbChunks; ++c) {
+  for (int c = 0; c < CHUNKS; ++c) {
     const long long p0 = first + (long long)c * kRgbBlockPx;
     if (p0 >= n) break;  // uniform
     rgb_store(wo, own);
     rgb_store(wd, dn);
     __syncthreads();
     const long long p1 = p0 + kRgbBlockPx;
-    if (c + 1 < kRgbChunks && p1 < n) rgb_load_chunk(rgb, n, W, ch, p1, wo, wd);
+    if (c + 1 < CHUNKS && p1 < n) rgb_load_chunk(rgb, n, W, ch, p1, wo, wd);
     f(p0, O, (p0 * ch) & ~3LL, D, ((p0 + W) * ch) & ~3LL);
     __syncthreads();  // the chunk's bytes are read before the next store
   }
 }
 
 // One colour image of a launch: its bytes, pool pixel offset, size and
-// model, and its first block in the launch (blocks of kRgbChunks *
-// kRgbBlockPx pixels, images in turn).
+// model, and its first block in each of the launch's two kernels (blocks of
+// kRgbSumChunks / kRgbCapsChunks chunks, images in turn).
+enum { RGB_SUM = 0, RGB_CAPS = 1 };
 struct RgbJob {
   const uint8_t* rgb;
   long long pix0;
   int W, H, ch;
-  int block0;
+  int block0[2];
   gcmi_rgb_model m;
 };
 constexpr int kRgbJobsMax = 64;  // images per launch
+// u64 words per pair-sum slot: each slot of a colour upload's sums on a
+// 128-byte line of its own.  Atomics on slots that share a line are done one
+// after the other (16,384 blocks adding to 64 packed slots take ~40 us, to
+// 64 padded ones ~1 us).
+constexpr int kRgbSumStride = 16;
 
-// A launch's job table: the first blocks of its jobs (ascending, from 0),
-// then the jobs.
+// A launch's job table: the first blocks of its jobs in each kernel
+// (ascending, from 0), then the jobs.
 struct RgbJobs {
-  int block0[kRgbJobsMax];
+  int block0[2][kRgbJobsMax];
   RgbJob job[kRgbJobsMax];
 };
 
-// The job this block belongs to: one ballot of wave 0 over the first blocks
-// (njobs <= kWave); the result is wave-uniform.  All threads call it.
+// The job this block of kernel K (RGB_SUM, RGB_CAPS) belongs to: one ballot
+// of wave 0 over the first blocks (njobs <= kWave); the result is
+// wave-uniform.  All threads call it.
+template <int K>
 __device__ __forceinline__ int rgb_job(const RgbJobs* __restrict__ J, int njobs, int* s_j) {
   static_assert(kRgbJobsMax <= kWave, "one lane per job");
   if (threadIdx.x < kWave) {
-    const int b0 = threadIdx.x < njobs ? J->block0[threadIdx.x] : 0x7fffffff;
+    const int b0 = threadIdx.x < njobs ? J->block0[K][threadIdx.x] : 0x7fffffff;
     const unsigned long long m = __ballot(b0 <= (int)blockIdx.x);
     if (threadIdx.x == 0) *s_j = __popcll(m) - 1;
   }
@@ -422,21 +441,23 @@ __device__ __forceinline__ int rgb_job(const RgbJobs* __restrict__ J, int njobs,
 
 // Per image of the launch, the sum of d2 over its horizontal and vertical
 // neighbour pairs (beta's numerator; an exact integer), spread over
-// kSumSlots slots at sums[job * kSumSlots].  Every thread takes kRgbPx
+// kSumSlots slots at sums[(job * kSumSlots + slot) * kRgbSumStride].  Every
+// thread takes kRgbPx
 // pixels of each chunk; the grid covers every image once.
 __global__ void __launch_bounds__(kRgbNT)
 k_rgb_pair_sum(const RgbJobs* __restrict__ J, int njobs, unsigned long long* sums) {
   __shared__ long long red[kRgbNT / kWave];
   __shared__ int s_j;
-  const int j = rgb_job(J, njobs, &s_j);
+  const int j = rgb_job<RGB_SUM>(J, njobs, &s_j);
   const RgbJob* jobs = J->job;
   const uint8_t* rgb = jobs[j].rgb;
   const int W = jobs[j].W, H = jobs[j].H, ch = jobs[j].ch;
   const long long n = (long long)W * H;
   const bool small = n <= 0x7fffffffLL;
-  const long long first = (long long)(blockIdx.x - jobs[j].block0) * kRgbChunks * kRgbBlockPx;
+  const long long first =
+      (long long)(blockIdx.x - jobs[j].block0[RGB_SUM]) * kRgbSumChunks * kRgbBlockPx;
   long long acc = 0;
-  rgb_chunks(rgb, n, W, ch, first,
+  rgb_chunks<kRgbSumChunks>(rgb, n, W, ch, first,
              [&](long long p0, const uint8_t* O, long long own0, const uint8_t* D, long long dn0) {
                const long long i0 = p0 + (long long)threadIdx.x * kRgbPx;
                int x = i0 < n ? rgb_col(i0, W, small) : 0;
@@ -453,7 +474,8 @@ k_rgb_pair_sum(const RgbJobs* __restrict__ J, int njobs, unsigned long long* sum
              });
   acc = block_sum64<kRgbNT>(acc, red);
   if (threadIdx.x == 0 && acc)
-    atomicAdd(&sums[(size_t)j * kSumSlots + blockIdx.x % kSumSlots], (unsigned long long)acc);
+    atomicAdd(&sums[((size_t)j * kSumSlots + blockIdx.x % kSumSlots) * kRgbSumStride],
+              (unsigned long long)acc);
 }
 
 // 2 beta of the exponential pairwise model from an image's pair-sum slots
@@ -464,7 +486,7 @@ __device__ double rgb_two_beta(int W, int H, const unsigned long long* __restric
 #pragma clang fp contract(off)
   if (threadIdx.x < kWave) {
     unsigned long long v = 0;
-    for (int j = threadIdx.x; j < kSumSlots; j += kWave) v += pair_sum[j];
+    for (int j = threadIdx.x; j < kSumSlots; j += kWave) v += pair_sum[(size_t)j * kRgbSumStride];
     v = (unsigned long long)wave_sum64((long long)v);
     if (threadIdx.x == 0) {
       double beta = 1.0;
@@ -545,14 +567,14 @@ k_rgb_caps(const RgbJobs* __restrict__ J, int njobs, const unsigned long long* _
            int32_t* __restrict__ down) {
 #pragma clang fp contract(off)
   __shared__ int s_j;
-  const int j = rgb_job(J, njobs, &s_j);
+  const int j = rgb_job<RGB_CAPS>(J, njobs, &s_j);
   const RgbJob* jobs = J->job;
   const uint8_t* rgb = jobs[j].rgb;
   const int W = jobs[j].W, H = jobs[j].H, ch = jobs[j].ch;
   const gcmi_rgb_model& m = jobs[j].m;
   __shared__ double s_two_beta;
   const double two_beta = m.pair_kind == GCMI_PAIR_EXP
-      ? rgb_two_beta(W, H, sums + (size_t)j * kSumSlots, &s_two_beta) : 0.0;
+      ? rgb_two_beta(W, H, sums + (size_t)j * kSumSlots * kRgbSumStride, &s_two_beta) : 0.0;
   const double inv_tb = m.pair_kind == GCMI_PAIR_EXP ? 1.0 / two_beta : 0.0;
   const float inv_tb_f = (float)inv_tb, scale_f = (float)m.pair_scale;
   const double inv_div = 1.0 / m.unary_div;
@@ -563,8 +585,9 @@ k_rgb_caps(const RgbJobs* __restrict__ J, int njobs, const unsigned long long* _
   down += pix0;
   const long long n = (long long)W * H;
   const bool small = n <= 0x7fffffffLL;
-  const long long first = (long long)(blockIdx.x - jobs[j].block0) * kRgbChunks * kRgbBlockPx;
-  rgb_chunks(rgb, n, W, ch, first,
+  const long long first =
+      (long long)(blockIdx.x - jobs[j].block0[RGB_CAPS]) * kRgbCapsChunks * kRgbBlockPx;
+  rgb_chunks<kRgbCapsChunks>(rgb, n, W, ch, first,
              [&](long long p0, const uint8_t* O, long long own0, const uint8_t* D, long long dn0) {
 #pragma clang fp contract(off)
                const long long i0 = p0 + (long long)threadIdx.x * kRgbPx;
@@ -900,7 +923,8 @@ size_t pool_bytes(int tile, int cap_tiles, long long cap_pix, int cap_imgs) {
          (size_t)cap_pix * (4 * 4 + 1) + (size_t)cap_pix / 8 +
          (size_t)cap_imgs *
              (sizeof(ImgDesc) + SUM_ROWS * kSumSlots * 8 + sizeof(Prep) + 8 + kPixAlign * 17) +
-         (size_t)compact_scratch_ints(cap_tiles) * 4 + 8 * kSumSlots * kRgbJobsMax + 4096;
+         (size_t)compact_scratch_ints(cap_tiles) * 4 + 8 * kSumSlots * kRgbSumStride * kRgbJobsMax +
+         4096;
 }
 
 // Synchronises the pool's stream: every staging copy queued on it is done.
@@ -1010,7 +1034,7 @@ int pool_init(Pool* p, int tile, int cap_tiles, long long cap_pix, int cap_imgs,
   ALLOC(p->plist, (size_t)cap_tiles * 4);
   ALLOC(p->alist, (size_t)cap_tiles * 4);
   ALLOC(p->d_status, kStatusFlows + (size_t)cap_imgs * 8);
-  ALLOC(p->d_rgbsum, 8 * kSumSlots * kRgbJobsMax);
+  ALLOC(p->d_rgbsum, 8 * kSumSlots * kRgbSumStride * kRgbJobsMax);
   ALLOC(p->d_imgfresh, (size_t)cap_imgs * 4);
   ALLOC(p->scratch, (size_t)compact_scratch_ints(cap_tiles) * 4);
   ALLOC(p->d_prep, (size_t)cap_imgs * sizeof(Prep));
@@ -1274,15 +1298,18 @@ int pool_upload_rgb_many(Pool* p, int count, const RgbReq* reqs) {
     const int m = std::min(count - base, kRgbJobsMax);
     // the job table may still be read by the previous launch's copy
     GCMI_TRY(stage_wait(p, STAGE_RGBJOBS));
-    long long blocks = 0;
+    long long blocks[2] = {0, 0};
     for (int i = 0; i < m; ++i) {
       const RgbReq& r = reqs[base + i];
       const PoolImage& im = p->imgs[r.img];
-      hj->block0[i] = (int)blocks;
-      hj->job[i] = RgbJob{r.d_rgb, im.pix0, im.W, im.H, r.channels, (int)blocks, *r.model};
-      constexpr long long px_per_block = (long long)kRgbChunks * kRgbBlockPx;
-      blocks += ((long long)im.W * im.H + px_per_block - 1) / px_per_block;
-      if (blocks > 0x7fffffffLL) {
+      hj->job[i] = RgbJob{r.d_rgb, im.pix0, im.W, im.H, r.channels, {0, 0}, *r.model};
+      for (int k = 0; k < 2; ++k) {
+        const long long px_per_block =
+            (long long)(k == RGB_SUM ? kRgbSumChunks : kRgbCapsChunks) * kRgbBlockPx;
+        hj->block0[k][i] = hj->job[i].block0[k] = (int)blocks[k];
+        blocks[k] += ((long long)im.W * im.H + px_per_block - 1) / px_per_block;
+      }
+      if (blocks[RGB_CAPS] > 0x7fffffffLL) {
         set_error("colour images too large for one launch");
         return GCMI_EINVAL;
       }
@@ -1290,12 +1317,14 @@ int pool_upload_rgb_many(Pool* p, int count, const RgbReq* reqs) {
     GCMI_HIP(hipMemcpyAsync(p->d_rgbjobs, hj, offsetof(RgbJobs, job) + (size_t)m * sizeof(RgbJob),
                             hipMemcpyHostToDevice, p->stream));
     p->stage_pending[STAGE_RGBJOBS] = true;
-    hipLaunchKernelGGL(k_fill_i32, dim3(std::max(1, m * kSumSlots * 2 / 256)), dim3(256), 0,
-                       p->stream, (int32_t*)p->d_rgbsum, 2LL * kSumSlots * m, 0);
-    if (blocks > 0) {
-      hipLaunchKernelGGL(k_rgb_pair_sum, dim3((unsigned)blocks), dim3(kRgbNT), 0, p->stream, dj,
-                         m, p->d_rgbsum);
-      hipLaunchKernelGGL(k_rgb_caps, dim3((unsigned)blocks), dim3(kRgbNT), 0, p->stream, dj, m,
+    hipLaunchKernelGGL(k_fill_i32, dim3(std::max(1, m * kSumSlots * kRgbSumStride * 2 / 256)),
+                       dim3(256), 0, p->stream, (int32_t*)p->d_rgbsum,
+                       2LL * kSumSlots * kRgbSumStride * m, 0);
+    if (blocks[RGB_CAPS] > 0) {
+      hipLaunchKernelGGL(k_rgb_pair_sum, dim3((unsigned)blocks[RGB_SUM]), dim3(kRgbNT), 0,
+                         p->stream, dj, m, p->d_rgbsum);
+      hipLaunchKernelGGL(k_rgb_caps, dim3((unsigned)blocks[RGB_CAPS]), dim3(kRgbNT), 0, p->stream,
+                         dj, m,
                          (const unsigned long long*)p->d_rgbsum, p->dev.cs, p->dev.ct,
                          (int32_t*)p->dev.right, (int32_t*)p->dev.down);
     }
