@@ -308,16 +308,16 @@ __device__ __forceinline__ int rgb_col(long long i, int W, bool small) {
 // block covers a few consecutive chunks of one image and stages the bytes of
 // each in LDS: its pixels and the next one (right neighbours), and the
 // pixels one row below them.  The bytes of the next chunk are loaded into
-// registers while the block works on the current one.  Chunks per block:
-// the pair sums (light work per pixel) take 4, which hides each block's
-// set-up behind its later chunks; the capacities (float64 work, fewer waves
-// per SIMD) take 1, so that a 1024^2 image has four blocks per CU rather
-// than one working its chunks in turn (k_rgb_caps on 4096^2 / 1024^2: 97.6
-// / 19.7 us with 4 chunks, 82.2 / 11.0 us with 1; k_rgb_pair_sum: 42.2 /
-// 11.8 us with 4, 81.4 / 10.2 us with 1).  Words per staged range (up to 4
-// channels, plus the range's offset inside its first word).
+// registers while the block works on the current one.  The capacities
+// (float64 work, few waves per SIMD) take one chunk per block, so that a
+// 1024^2 image has four blocks per CU rather than one working its chunks in
+// turn (k_rgb_caps on 4096^2 / 1024^2: 97.6 / 19.7 us with 4 chunks, 82.2 /
+// 11.0 us with 1).  The pair sums read their words straight from the image
+// and take 2 chunks' pixels per block (4096^2 / 1024^2: 27.0 / 6.0 us; 27.1
+// / 9.2 us with 4).  Words per staged range (up to 4 channels, plus the
+// range's offset inside its first word).
 constexpr int kRgbBlockPx = kRgbNT * kRgbPx;
-constexpr int kRgbSumChunks = 4, kRgbCapsChunks = 1;
+constexpr int kRgbSumChunks = 2, kRgbCapsChunks = 1;
 constexpr int kRgbStageWords = (kRgbBlockPx + 1) + 1;
 constexpr int kRgbStageIters = (kRgbStageWords + kRgbNT - 1) / kRgbNT;
 
@@ -335,6 +335,9 @@ struct RgbWords {
 // and every wait for the staged LDS bytes would also wait for them)
 using GlobalU8 = const __attribute__((address_space(1))) uint8_t;
 using GlobalU32 = const __attribute__((address_space(1))) uint32_t;
+// four words at a word-aligned address (one 16-byte load)
+typedef uint32_t RgbW4 __attribute__((ext_vector_type(4), aligned(4)));
+using GlobalW4 = const __attribute__((address_space(1))) RgbW4;
 __device__ __forceinline__ void rgb_load(const uint8_t* __restrict__ rgb_flat, long long nb,
                                          long long b0, long long b1, RgbWords& w) {
   GlobalU8* rgb = (GlobalU8*)rgb_flat;
@@ -439,43 +442,109 @@ __device__ __forceinline__ int rgb_job(const RgbJobs* __restrict__ J, int njobs,
   return __builtin_amdgcn_readfirstlane(*s_j);
 }
 
+// d2 of the pairs of four pixels whose bytes are whole in registers: the
+// pixels' CH-byte colours from byte 0 of own (and the next pixel's after
+// them: their right neighbours), the pixels a row below from byte 0 of dn.
+template <int CH>
+__device__ __forceinline__ uint32_t rgb_d2_group(const uint32_t* own, const uint32_t* dn) {
+  uint32_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int dr = 0, dd = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int b = CH * k + c;
+      const int v = (own[b >> 2] >> (8 * (b & 3))) & 0xff;
+      const int br = b + CH;
+      const int r = (own[br >> 2] >> (8 * (br & 3))) & 0xff;
+      const int d = (dn[b >> 2] >> (8 * (b & 3))) & 0xff;
+      dr += (v - r) * (v - r);
+      dd += (v - d) * (v - d);
+    }
+    acc += (uint32_t)(dr + dd);
+  }
+  return acc;
+}
+
 // Per image of the launch, the sum of d2 over its horizontal and vertical
 // neighbour pairs (beta's numerator; an exact integer), spread over
-// kSumSlots slots at sums[(job * kSumSlots + slot) * kRgbSumStride].  Every
-// thread takes kRgbPx
-// pixels of each chunk; the grid covers every image once.
+// kSumSlots slots at sums[(job * kSumSlots + slot) * kRgbSumStride].  A
+// block takes kRgbSumChunks * kRgbBlockPx consecutive pixels, a thread
+// groups of four pixels kRgbNT groups apart; the grid covers every image
+// once.  A group inside the image whose pixels all have a right neighbour
+// and a row below reads its bytes as words straight from the image (the
+// group starts on a word: 4 * ch * group; its row below is realigned with
+// alignbyte); the groups at row and image ends read byte by byte.
+constexpr int kRgbSumGroups = kRgbSumChunks * kRgbBlockPx / (4 * kRgbNT);
+static_assert(kRgbPx == 4 && kRgbSumGroups * 4 * kRgbNT == kRgbSumChunks * kRgbBlockPx,
+              "four pixels per group");
 __global__ void __launch_bounds__(kRgbNT)
 k_rgb_pair_sum(const RgbJobs* __restrict__ J, int njobs, unsigned long long* sums) {
   __shared__ long long red[kRgbNT / kWave];
   __shared__ int s_j;
   const int j = rgb_job<RGB_SUM>(J, njobs, &s_j);
   const RgbJob* jobs = J->job;
-  const uint8_t* rgb = jobs[j].rgb;
+  GlobalU8* rgb = (GlobalU8*)jobs[j].rgb;
   const int W = jobs[j].W, H = jobs[j].H, ch = jobs[j].ch;
-  const long long n = (long long)W * H;
+  const long long n = (long long)W * H, nb = n * ch;
   const bool small = n <= 0x7fffffffLL;
   const long long first =
       (long long)(blockIdx.x - jobs[j].block0[RGB_SUM]) * kRgbSumChunks * kRgbBlockPx;
-  long long acc = 0;
-  rgb_chunks<kRgbSumChunks>(rgb, n, W, ch, first,
-             [&](long long p0, const uint8_t* O, long long own0, const uint8_t* D, long long dn0) {
-               const long long i0 = p0 + (long long)threadIdx.x * kRgbPx;
-               int x = i0 < n ? rgb_col(i0, W, small) : 0;
+  // the words of every group first (all loads in flight), then the sums
+  uint32_t own[kRgbSumGroups][5], dn[kRgbSumGroups][5];
+  bool fast[kRgbSumGroups];
 #pragma unroll
-               for (int k = 0; k < kRgbPx; ++k) {
-                 const long long i = i0 + k;
-                 if (i < n) {
-                   const uint8_t* q = O + (i * ch - own0);
-                   if (x + 1 < W) acc += rgb_d2(q, q + ch);
-                   if (i + W < n) acc += rgb_d2(q, D + ((i + W) * ch - dn0));
-                 }
-                 if (++x == W) x = 0;
-               }
-             });
-  acc = block_sum64<kRgbNT>(acc, red);
-  if (threadIdx.x == 0 && acc)
+  for (int g = 0; g < kRgbSumGroups; ++g) {
+    const long long i0 = first + 4LL * (g * kRgbNT + threadIdx.x);
+    const int x0 = i0 < n ? rgb_col(i0, W, small) : 0;
+    const long long ob = i0 * ch, db = (i0 + W) * ch, da = db & ~3LL;
+    // own: 4 ch + 3 bytes (the group and its right neighbour's colour) in
+    // 4 (ch 3) or 5 (ch 4) words; below: 4 ch bytes from da + (db - da)
+    const int nwo = ch == 3 ? 4 : 5;
+    fast[g] = i0 + 4 <= n && x0 + 4 < W && i0 + 4 + W <= n && ob + 4 * nwo <= nb &&
+              da + 20 <= nb;
+    if (fast[g]) {
+      const RgbW4 a = *(GlobalW4*)(rgb + ob), b = *(GlobalW4*)(rgb + da);
+      own[g][0] = a.x, own[g][1] = a.y, own[g][2] = a.z, own[g][3] = a.w;
+      own[g][4] = nwo == 5 ? *(GlobalU32*)(rgb + ob + 16) : 0u;
+      dn[g][0] = b.x, dn[g][1] = b.y, dn[g][2] = b.z, dn[g][3] = b.w;
+      dn[g][4] = *(GlobalU32*)(rgb + da + 16);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) own[g][k] = dn[g][k] = 0u;
+    }
+  }
+  uint32_t acc = 0;  // at most 4 kRgbSumGroups 2 3 255^2 < 2^32
+#pragma unroll
+  for (int g = 0; g < kRgbSumGroups; ++g) {
+    const long long i0 = first + 4LL * (g * kRgbNT + threadIdx.x);
+    if (fast[g]) {
+      const int sh = (int)(((i0 + W) * ch) & 3);
+      uint32_t d[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) d[k] = __builtin_amdgcn_alignbyte(dn[g][k + 1], dn[g][k], sh);
+      acc += ch == 3 ? rgb_d2_group<3>(own[g], d) : rgb_d2_group<4>(own[g], d);
+    } else if (i0 < n) {
+      int x = rgb_col(i0, W, small);
+      for (int k = 0; k < 4 && i0 + k < n; ++k) {
+        const long long i = i0 + k;
+        GlobalU8* q = rgb + i * ch;
+        int v = 0;
+        if (x + 1 < W)
+          for (int c = 0; c < 3; ++c) v += ((int)q[c] - (int)q[ch + c]) * ((int)q[c] - (int)q[ch + c]);
+        if (i + W < n) {
+          GlobalU8* r = q + (long long)W * ch;
+          for (int c = 0; c < 3; ++c) v += ((int)q[c] - (int)r[c]) * ((int)q[c] - (int)r[c]);
+        }
+        acc += (uint32_t)v;
+        if (++x == W) x = 0;
+      }
+    }
+  }
+  const long long tot = block_sum64<kRgbNT>((long long)acc, red);
+  if (threadIdx.x == 0 && tot)
     atomicAdd(&sums[((size_t)j * kSumSlots + blockIdx.x % kSumSlots) * kRgbSumStride],
-              (unsigned long long)acc);
+              (unsigned long long)tot);
 }
 
 // 2 beta of the exponential pairwise model from an image's pair-sum slots
