@@ -978,6 +978,18 @@ __device__ __forceinline__ int seg_min(int v) {
 
 constexpr int kBfsNT = 256;
 constexpr int kInf = 0x7fffffff;
+
+// A block's LDS copy of a tile holds row y's 16-byte pieces in a rotated
+// order: piece c of the row at piece c ^ (y % kBfsSwz<T>).  The level loop's
+// lanes (one per row) often store to the same column of their rows at once;
+// row-major rows T words apart would put those stores in one LDS bank, the
+// rotation spreads them over kBfsSwz<T> pieces.
+template <int T>
+constexpr int kBfsSwz = T / 4 < 8 ? T / 4 : 8;
+template <int T>
+__device__ __forceinline__ int bfs_px(int y, int x) {
+  return y * T + (x ^ ((y & (kBfsSwz<T> - 1)) << 2));
+}
 template <int T>
 constexpr int kBfsTPB = (kWave / T) * (kBfsNT / kWave);  // tiles per block
 
@@ -1093,7 +1105,7 @@ __device__ __forceinline__ void bfs_pass(const Geo& g, CutDev& s, const int32_t*
         v = valid ? reinterpret_cast<const int4*>(s.h + tbase<T>(t) + (size_t)y * T)[c]
                   : make_int4(MAXH, MAXH, MAXH, MAXH);
       }
-      reinterpret_cast<int4*>(buf + y * T)[c] = v;
+      reinterpret_cast<int4*>(buf + y * T)[c ^ (y & (kBfsSwz<T> - 1))] = v;
     }
     // V: pixels the frontier has reached; a round's frontier keeps only the
     // pixels whose distance fell (a pixel at or below the level cannot
@@ -1128,11 +1140,11 @@ __device__ __forceinline__ void bfs_pass(const Geo& g, CutDev& s, const int32_t*
       if (done) nw = 0;
       V |= nw;
       if (INIT) {
-        for (Row m = nw; m; m &= m - 1) buf[y * T + row_ctz(m)] = L;
+        for (Row m = nw; m; m &= m - 1) buf[bfs_px<T>(y, row_ctz(m))] = L;
       } else {
         for (Row m = nw; m; m &= m - 1) {
           const int x = row_ctz(m);
-          if (atomicMin(&buf[y * T + x], L) <= L) nw &= ~((Row)1 << x);
+          if (atomicMin(&buf[bfs_px<T>(y, x)], L) <= L) nw &= ~((Row)1 << x);
         }
         fell |= nw;
       }
@@ -1146,7 +1158,8 @@ __device__ __forceinline__ void bfs_pass(const Geo& g, CutDev& s, const int32_t*
 #pragma unroll
       for (int c = 0; c < T / 4; ++c)
         if (INIT || ((fell >> (4 * c)) & 15))
-          reinterpret_cast<int4*>(s.h + rowg)[c] = reinterpret_cast<const int4*>(buf + y * T)[c];
+          reinterpret_cast<int4*>(s.h + rowg)[c] =
+              reinterpret_cast<const int4*>(buf + y * T)[c ^ (y & (kBfsSwz<T> - 1))];
     }
     // news for the neighbours
     int dirs = 0;
@@ -1159,7 +1172,10 @@ __device__ __forceinline__ void bfs_pass(const Geo& g, CutDev& s, const int32_t*
 #pragma unroll
       for (int side = 0; side < 4; ++side) {
         if (!ex[side] || !hin[side]) continue;
-        const int d = buf[epx[side]];
+        const int d = buf[side == D_UP     ? bfs_px<T>(0, y)
+                          : side == D_DOWN ? bfs_px<T>(T - 1, y)
+                          : side == D_LEFT ? bfs_px<T>(y, 0)
+                                           : bfs_px<T>(y, T - 1)];
         const bool cand = INIT ? (d < MAXH && !(hls[side] && sink[side])) : fel[side] != 0;
         if (cand && d + 1 < hv[side]) dirs |= 1 << side;
       }
