@@ -515,6 +515,9 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
     // round 1 starts from reset tiles: nothing pending
     if (threadIdx.x == 0) sflag = FIRST ? 0 : s.pflag[t];
     __syncthreads();
+    // (later rounds) nonzero once a flow or a height of this thread's
+    // pixels changed: a thread whose pixels kept their state stores nothing
+    int chg = 0;
     // the halo heights at this row's ends (frozen for the residency)
     const int hL = H0[(ly + 1) * HS + 3], hR = H0[(ly + 1) * HS + 4 + T];
     if (sflag) {
@@ -530,6 +533,7 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
       }
 #pragma unroll
       for (int k = 0; k < R; ++k) {
+        if constexpr (!FIRST) chg |= vu[k] | vd[k] | vl[k] | vr[k];
         e[k] += vu[k] + vd[k] + vl[k] + vr[k];
         cu[k] += vu[k];
         cd[k] += vd[k];
@@ -580,6 +584,7 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
           fr[k] = hv == hr + 1 ? min(rem, cr[k]) : 0;
           rem -= fr[k];
           fl[k] = hv == hl + 1 ? min(rem, cl[k]) : 0;
+          if constexpr (!FIRST) chg |= fd[k] | fu[k] | fr[k] | fl[k];
           cd[k] -= fd[k];
           cu[k] -= fu[k];
           cr[k] -= fr[k];
@@ -615,6 +620,7 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
         const int inu = row0 ? 0 : vu[k], ind = rowL ? 0 : vd[k];
         const int vl = k == 0 ? vl0 : 0;
         const int vr = k == R - 1 ? vrR : 0;
+        if constexpr (!FIRST) chg |= vl | vr | inu | ind;
         e[k] += vl + vr + inu + ind;
         cl[k] += vl;
         cr[k] += vr;
@@ -648,7 +654,10 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
           hn[k] = nh2;
         }
 #pragma unroll
-        for (int k = 0; k < R; ++k) h[k] = hn[k];
+        for (int k = 0; k < R; ++k) {
+          if constexpr (!FIRST) chg |= hn[k] ^ h[k];
+          h[k] = hn[k];
+        }
       }
       store_row<R>(Hn + hrow, h);
       active = __syncthreads_or(any);
@@ -661,19 +670,21 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
       mk |= (uint32_t)pixel_mask(e[k], cu[k], cd[k], cl[k], cr[k]) << (8 * k);
       tp += e[k] > 0 ? e[k] : 0;
     }
-    store_row<R>(s.e + row, e);
-    store_row<R>(s.h + row, h);
-    store_row<R>(s.cu + row, cu);
-    store_row<R>(s.cd + row, cd);
-    store_row<R>(s.cl + row, cl);
-    store_row<R>(s.cr + row, cr);
-    if constexpr (R == 4) {
-      *reinterpret_cast<uint32_t*>(s.mask + row) = mk;
-    } else if constexpr (R == 2) {
-      *reinterpret_cast<uint16_t*>(s.mask + row) = (uint16_t)mk;
-    } else {
+    if (FIRST || chg) {
+      store_row<R>(s.e + row, e);
+      store_row<R>(s.h + row, h);
+      store_row<R>(s.cu + row, cu);
+      store_row<R>(s.cd + row, cd);
+      store_row<R>(s.cl + row, cl);
+      store_row<R>(s.cr + row, cr);
+      if constexpr (R == 4) {
+        *reinterpret_cast<uint32_t*>(s.mask + row) = mk;
+      } else if constexpr (R == 2) {
+        *reinterpret_cast<uint16_t*>(s.mask + row) = (uint16_t)mk;
+      } else {
 #pragma unroll
-      for (int k = 0; k < R; ++k) s.mask[row + k] = (uint8_t)(mk >> (8 * k));
+        for (int k = 0; k < R; ++k) s.mask[row + k] = (uint8_t)(mk >> (8 * k));
+      }
     }
     // edge outflow -> neighbour's pending slot (the receiving side is the
     // opposite one); a nonzero outflow implies the arc and so the neighbour
