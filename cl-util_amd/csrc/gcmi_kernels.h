@@ -1683,6 +1683,7 @@ __device__ __forceinline__ void edit_load(const CutDev& s, const ImgDesc& d, int
 __device__ __forceinline__ void edit_apply(const CutDev& s, const PixEdit& E, int a1, int b1,
                                            int warm, long long& dmin, long long& dpos) {
   const int a0 = E.a0, b0 = E.b0;
+  if (a1 == a0 && b1 == b0) return;  // (a pixel painted again with the same values)
   s.cs[E.r] = a1;
   s.ct[E.r] = b1;
   if (!warm) return;  // no state yet: the first round reads the unaries
