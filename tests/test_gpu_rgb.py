@@ -53,6 +53,11 @@ def images():
     rgba = np.concatenate([out[1], rng.integers(0, 256, size=out[1].shape[:2] + (1,),
                                                 dtype=np.uint8)], axis=2)
     out.append(rgba)
+    # odd widths with many whole groups of four pixels: the pair sums' word
+    # reads of the row below start at every byte offset of a word (3 * W is
+    # odd), and an RGBA image whose rows end inside a group
+    out.append(rng.integers(0, 256, size=(257, 1001, 3), dtype=np.uint8))
+    out.append(rng.integers(0, 256, size=(111, 333, 4), dtype=np.uint8))
     return out
 
 
