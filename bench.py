@@ -436,30 +436,38 @@ def batched_leg(args, exp):
             b.cut(sweeps=sweeps, bfs=bfs)
         return (time.perf_counter() - t0) / reps
 
-    # autotuner: tile x sweeps x push rounds per relabel x recording (0:
-    # host-driven; r > 0: one replayed hipGraph with r BFS rounds per
-    # relabel) x relabels after the first (full or local), two timed runs each
+    # autotuner: tile x sweeps x push rounds per relabel x run mode (host-
+    # driven; device-driven with r BFS / flood rounds per relabel and one
+    # synchronisation; the same recorded once as a hipGraph and replayed) x
+    # relabels after the first (full or local), two timed runs each
     log("batched: autotuning")
+    modes = [(0, 0), (1, 0), (2, 0), (0, 1), (0, 2), (0, 4)]  # (device rounds, graph rounds)
+
+    def set_mode(b, device, graph):
+        b.set_graph(graph)  # (0: host-driven, no recording)
+        if device:
+            b.set_device_rounds(device)
+
     tune = []
     batches = {}
     for tile in (16, 32, 64):
         batches[tile] = make_batch(tile)
         for sweeps in (2, 4, 8):
             for bfs in (1, 2):
-                for graph in (0, 2, 4, 8):
+                for device, graph in modes:
                     for local in (0, 1):
-                        batches[tile].set_graph(graph)
+                        set_mode(batches[tile], device, graph)
                         batches[tile].set_local_relabel(bool(local))
                         dt = time_batch(batches[tile], sweeps, bfs, 2)
-                        tune.append({"tile": tile, "sweeps": sweeps, "bfs": bfs, "graph": graph,
-                                     "local": local, "ms": round(dt * 1e3, 3)})
+                        tune.append({"tile": tile, "sweeps": sweeps, "bfs": bfs, "device": device,
+                                     "graph": graph, "local": local, "ms": round(dt * 1e3, 3)})
     best = min(tune, key=lambda r: r["ms"])
     b = batches[best["tile"]]
-    b.set_graph(best["graph"])
+    set_mode(b, best["device"], best["graph"])
     b.set_local_relabel(bool(best["local"]))
     reps = max(3, args.steps // 2)
     t_batch = time_batch(b, best["sweeps"], best["bfs"], reps)
-    host = min((r for r in tune if r["graph"] == 0), key=lambda r: r["ms"])
+    host = min((r for r in tune if r["graph"] == 0 and r["device"] == 0), key=lambda r: r["ms"])
     flows = b.cut(sweeps=best["sweeps"], bfs=best["bfs"])
     stats = b.stats()
     nbad = 0

@@ -80,7 +80,8 @@ def ragged_64():
 
 
 @pytest.mark.timeout(300)
-@pytest.mark.parametrize("setting", ["autotuned", "host", "tile16_graph4", "tile64_device8"])
+@pytest.mark.parametrize("setting", ["autotuned", "host", "device1_local", "tile16_graph4",
+                                     "tile64_device8"])
 def test_config2_batch_64(ragged_64, setting):
     works, expect = ragged_64
     dims = [(w.shape[1], w.shape[0]) for w in works]
@@ -93,6 +94,11 @@ def test_config2_batch_64(ragged_64, setting):
         b.set_graph(2)
         b.set_local_relabel(True)
         sweeps, reps = 2, 3
+    elif setting == "device1_local":
+        # (the bench's autotuner picks this or the host-driven run)
+        b.set_device_rounds(1)
+        b.set_local_relabel(True)
+        bfs, reps = 2, 2
     elif setting == "tile16_graph4":
         b.set_graph(4)
         sweeps, bfs, reps = 8, 2, 2
