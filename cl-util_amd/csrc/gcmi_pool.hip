@@ -491,7 +491,7 @@ k_rgb_pair_sum(const RgbJobs* __restrict__ J, int njobs, unsigned long long* sum
   const long long first =
       (long long)(blockIdx.x - jobs[j].block0[RGB_SUM]) * kRgbSumChunks * kRgbBlockPx;
   // the words of every group first (all loads in flight), then the sums
-  uint32_t own[kRgbSumGroups][5], dn[kRgbSumGroups][5];
+  uint32_t own[kRgbSumGroups][5], dn[kRgbSumGroups][4];
   bool fast[kRgbSumGroups];
 #pragma unroll
   for (int g = 0; g < kRgbSumGroups; ++g) {
@@ -499,19 +499,20 @@ k_rgb_pair_sum(const RgbJobs* __restrict__ J, int njobs, unsigned long long* sum
     const int x0 = i0 < n ? rgb_col(i0, W, small) : 0;
     const long long ob = i0 * ch, db = (i0 + W) * ch, da = db & ~3LL;
     // own: 4 ch + 3 bytes (the group and its right neighbour's colour) in
-    // 4 (ch 3) or 5 (ch 4) words; below: 4 ch bytes from da + (db - da)
+    // 4 (ch 3) or 5 (ch 4) words; below: the 4 ch bytes from db, inside the
+    // 4 words from da (ch 3: 12 bytes at db - da <= 3; ch 4: db = da)
     const int nwo = ch == 3 ? 4 : 5;
     fast[g] = i0 + 4 <= n && x0 + 4 < W && i0 + 4 + W <= n && ob + 4 * nwo <= nb &&
-              da + 20 <= nb;
+              da + 16 <= nb;
     if (fast[g]) {
       const RgbW4 a = *(GlobalW4*)(rgb + ob), b = *(GlobalW4*)(rgb + da);
       own[g][0] = a.x, own[g][1] = a.y, own[g][2] = a.z, own[g][3] = a.w;
       own[g][4] = nwo == 5 ? *(GlobalU32*)(rgb + ob + 16) : 0u;
       dn[g][0] = b.x, dn[g][1] = b.y, dn[g][2] = b.z, dn[g][3] = b.w;
-      dn[g][4] = *(GlobalU32*)(rgb + da + 16);
     } else {
 #pragma unroll
-      for (int k = 0; k < 5; ++k) own[g][k] = dn[g][k] = 0u;
+      for (int k = 0; k < 4; ++k) own[g][k] = dn[g][k] = 0u;
+      own[g][4] = 0u;
     }
   }
   uint32_t acc = 0;  // at most 4 kRgbSumGroups 2 3 255^2 < 2^32
@@ -520,9 +521,11 @@ k_rgb_pair_sum(const RgbJobs* __restrict__ J, int njobs, unsigned long long* sum
     const long long i0 = first + 4LL * (g * kRgbNT + threadIdx.x);
     if (fast[g]) {
       const int sh = (int)(((i0 + W) * ch) & 3);
+      // (d[3] is read only for ch 4, where sh = 0)
       uint32_t d[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) d[k] = __builtin_amdgcn_alignbyte(dn[g][k + 1], dn[g][k], sh);
+      for (int k = 0; k < 4; ++k)
+        d[k] = __builtin_amdgcn_alignbyte(k < 3 ? dn[g][k + 1] : 0u, dn[g][k], sh);
       acc += ch == 3 ? rgb_d2_group<3>(own[g], d) : rgb_d2_group<4>(own[g], d);
     } else if (i0 < n) {
       int x = rgb_col(i0, W, small);
