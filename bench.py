@@ -49,8 +49,10 @@ HBM_PEAK_GBS = 8000.0
 EXPECTED = os.path.join(HERE, "tests", "golden", "bench_expected.json")
 
 # Algorithmic HBM bytes per pixel touched, per kernel class:
-#   push_first 16 read per image pixel (source, sink, right, down) + 25 written
-#              per padded pixel (e, h, 4 caps, residual mask)
+#   push_first 16 read per image pixel (source, sink, right, down) + 21 written
+#              per padded pixel (e, 4 caps, residual mask; with one push round
+#              per relabel its heights are not stored: the full relabel that
+#              follows rewrites them)
 #   push       49 per pixel per later residency: 24 read + 25 written
 #   bfs_init    5 per pixel of a relabel's first pass: residual mask read, h written
 #   bfs_round   9 per pixel of a later BFS pass: mask + h read, h written
@@ -140,7 +142,7 @@ def class_bytes(stats, tile, W, H):
     T2 = tile * tile
     ntiles = stats["tiles"]
     return {
-        "push_first": 16 * W * H + 25 * ntiles * T2,
+        "push_first": 16 * W * H + 21 * ntiles * T2,
         "push": ALG_BYTES_PER_PX["push"] * (stats["residencies"] - ntiles) * T2,
         "bfs_init": ALG_BYTES_PER_PX["bfs_init"] * stats["relabel_tiles"] * T2,
         "bfs_round": ALG_BYTES_PER_PX["bfs_round"] * stats["bfs_round_passes"] * T2,

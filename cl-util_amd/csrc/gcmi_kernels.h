@@ -295,7 +295,9 @@ __device__ __forceinline__ Halo halo_slot(const TG& q, int j) {
 // FIRST (an image's first round, over all its tiles): e = cs - ct, caps from
 // the pairwise weights, h = 0 on sink pixels (e < 0) and 1 elsewhere -- a
 // valid labelling, since every residual arc then satisfies h(p) <= h(q) + 1
-// -- and the image's sums get sum min(cs, ct) and sum max(e0, 0).
+// -- and the image's sums get sum min(cs, ct) and sum max(e0, 0).  Its
+// heights are stored only if store_h (a full relabel that comes next
+// rewrites them unread; later rounds always store).
 //
 // A sweep: every active pixel (e > 0, h < MAX) pushes min(rem, cap) along
 // each admissible arc (h(p) == h(q) + 1) in the order down, up, right, left
@@ -417,7 +419,7 @@ __device__ __forceinline__ void push_in_load(const CutDev& s, const TG& q, int l
 template <int T, int NT, bool FIRST>
 __global__ void __launch_bounds__(NT, kPushWaves<T>)
 k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
-       int sweeps) {
+       int sweeps, int store_h) {
   // thread -> R consecutive pixels of one tile row: row ly, columns x0 .. x0+R-1
   constexpr int R = T * T / NT;
   constexpr int CPR = T / R;  // threads per row
@@ -672,7 +674,7 @@ k_push(Geo g, CutDev s, const int32_t* __restrict__ list, const int32_t* __restr
     }
     if (FIRST || chg) {
       store_row<R>(s.e + row, e);
-      store_row<R>(s.h + row, h);
+      if (!FIRST || store_h) store_row<R>(s.h + row, h);
       store_row<R>(s.cu + row, cu);
       store_row<R>(s.cd + row, cd);
       store_row<R>(s.cl + row, cl);

@@ -83,11 +83,17 @@ struct Prof {
 
 struct Pool;
 
+// A push launch: a later round, or round 1 of the fresh images, storing
+// its heights or not.  Round 1 may leave them unstored when a full relabel
+// of those tiles comes next: it rewrites every distance of each tile it
+// lists and, for a listed neighbour, reads its mask only.
+enum PushKind { PUSH_LATER = 0, PUSH_FIRST = 1, PUSH_FIRST_NO_H = 2 };
+
 // Kernel launchers for one tile size.
 struct PoolOps {
   int tile;
   int threads;
-  void (*push)(Pool*, int grid, int sweeps, bool first);
+  void (*push)(Pool*, int grid, int sweeps, int kind);
   void (*fold)(Pool*, int grid, const int32_t* list, int cnt);
   void (*bfs_init)(Pool*, int grid, const int32_t* list, int cnt, bool all, const Queue& q);
   void (*bfs_round)(Pool*, int grid, const Queue& q);

@@ -823,13 +823,13 @@ k_flood_set(Geo g, CutDev s, const int32_t* __restrict__ sel, int n, int32_t* lm
 
 template <int T, int NT>
 struct PoolKernels {
-  static void push(Pool* p, int grid, int sweeps, bool first) {
-    if (first)
+  static void push(Pool* p, int grid, int sweeps, int kind) {
+    if (kind != PUSH_LATER)
       hipLaunchKernelGGL((k_push<T, NT, true>), dim3(grid), dim3(NT), 0, p->stream, p->geo, p->dev,
-                         p->lfresh, p->counts + CNT_FRESH, sweeps);
+                         p->lfresh, p->counts + CNT_FRESH, sweeps, kind == PUSH_FIRST ? 1 : 0);
     else
       hipLaunchKernelGGL((k_push<T, NT, false>), dim3(grid), dim3(NT), 0, p->stream, p->geo,
-                         p->dev, p->alist, p->counts + CNT_ACT, sweeps);
+                         p->dev, p->alist, p->counts + CNT_ACT, sweeps, 1);
   }
   static void fold(Pool* p, int grid, const int32_t* list, int cnt) {
     hipLaunchKernelGGL((k_fold<T>), dim3(grid), dim3(kFoldNT<T>), 0, p->stream, p->geo, p->dev,
@@ -1958,7 +1958,9 @@ int phase_first(const Solve& S, int32_t* nact) {
   if (S.nfresh > 0) {
     // (k_prepare reset the fresh images' tiles)
     S.B(K_PUSH_FIRST);
-    S.ops->push(p, S.nfresh, S.sweeps, true);
+    // (with one push round per relabel the full relabel of these tiles
+    // comes next, in the host loop and in a device-driven run alike)
+    S.ops->push(p, S.nfresh, S.sweeps, S.bfs <= 1 ? PUSH_FIRST_NO_H : PUSH_FIRST);
     S.E();
     GCMI_HIP(hipGetLastError());
   }
@@ -1972,7 +1974,7 @@ int phase_first(const Solve& S, int32_t* nact) {
     }
     if (S.dd || *nact > 0) {
       S.B(K_PUSH);
-      S.ops->push(p, S.grid_for(*nact), S.sweeps, false);
+      S.ops->push(p, S.grid_for(*nact), S.sweeps, PUSH_LATER);
       S.E();
       GCMI_HIP(hipGetLastError());
     }
@@ -2074,7 +2076,7 @@ int phase_push(const Solve& S, int32_t nact) {
     hipLaunchKernelGGL(k_clear_listed, dim3((nact + 255) / 256), dim3(256), 0, p->stream,
                        p->dev.act, p->alist, p->counts + CNT_ACT);
   S.B(K_PUSH);
-  S.ops->push(p, S.grid_for(nact), S.sweeps, false);
+  S.ops->push(p, S.grid_for(nact), S.sweeps, PUSH_LATER);
   S.E();
   GCMI_HIP(hipGetLastError());
   return GCMI_OK;
@@ -2097,7 +2099,7 @@ int phase_extra_pushes(const Solve& S, bool* any) {
       }
     }
     S.B(K_PUSH);
-    S.ops->push(p, S.grid_for(nact), S.sweeps, false);
+    S.ops->push(p, S.grid_for(nact), S.sweeps, PUSH_LATER);
     S.E();
     GCMI_HIP(hipGetLastError());
   }
@@ -2536,7 +2538,8 @@ static int step_begin(Pool* p, int img) {
   const Solve S = step_solve(p);
   GCMI_TRY(phase_prepare(S));
   if (fresh) {
-    S.ops->push(p, S.nfresh, 0, true);
+    // (a stepped solve may push again before it relabels)
+    S.ops->push(p, S.nfresh, 0, PUSH_FIRST);
     GCMI_HIP(hipGetLastError());
   }
   return GCMI_OK;
@@ -2553,7 +2556,7 @@ int pool_step_push(Pool* p, int img, int sweeps) {
   GCMI_TRY(read_counts(p));
   const int32_t nact = p->h_counts[CNT_ACT];
   if (nact > 0) {
-    S.ops->push(p, S.grid_for(nact), S.sweeps, false);
+    S.ops->push(p, S.grid_for(nact), S.sweeps, PUSH_LATER);
     GCMI_HIP(hipGetLastError());
   }
   p->step_last = STEP_PUSH;
