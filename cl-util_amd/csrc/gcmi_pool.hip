@@ -1958,9 +1958,12 @@ int phase_first(const Solve& S, int32_t* nact) {
   if (S.nfresh > 0) {
     // (k_prepare reset the fresh images' tiles)
     S.B(K_PUSH_FIRST);
-    // (with one push round per relabel the full relabel of these tiles
-    // comes next, in the host loop and in a device-driven run alike)
-    S.ops->push(p, S.nfresh, S.sweeps, S.bfs <= 1 ? PUSH_FIRST_NO_H : PUSH_FIRST);
+    // (with one push round per relabel and no warm image, whose pre-push
+    // below lists every active tile, fresh ones too, the full relabel of
+    // these tiles comes next, in the host loop and in a device-driven run
+    // alike)
+    const bool relabel_next = S.bfs <= 1 && S.nsel == S.nfresh;
+    S.ops->push(p, S.nfresh, S.sweeps, relabel_next ? PUSH_FIRST_NO_H : PUSH_FIRST);
     S.E();
     GCMI_HIP(hipGetLastError());
   }
