@@ -175,6 +175,7 @@ struct Pool {
   int ep_par = 0;         // epoch slot the next queue round reads
   bool flood_inc = false;  // the current flood recomputes changed tiles only
   bool flood_ran = false;  // a flood has added to the final-excess sums in this solve
+  bool first_no_h = false;  // this solve's round 1 left its heights unstored
   // pinned staging buffers with a host-to-device copy queued on the stream
   // since its last synchronisation (a buffer is rewritten only after it)
   bool stage_pending[STAGE_N] = {};

@@ -1951,6 +1951,12 @@ int phase_prepare(const Solve& S) {
   return GCMI_OK;
 }
 
+// Whether round 1 of the fresh images may leave its heights unstored: with
+// one push round per relabel and no warm image (whose pre-push lists every
+// active tile, fresh ones too), the full relabel of those tiles comes next,
+// in the host loop and in a device-driven run alike.
+bool first_relabel_next(const Solve& S) { return S.bfs <= 1 && S.nsel == S.nfresh; }
+
 // The first push round: every tile of the fresh images, then the tiles the
 // edits of the warm images made active.
 int phase_first(const Solve& S, int32_t* nact) {
@@ -1958,12 +1964,7 @@ int phase_first(const Solve& S, int32_t* nact) {
   if (S.nfresh > 0) {
     // (k_prepare reset the fresh images' tiles)
     S.B(K_PUSH_FIRST);
-    // (with one push round per relabel and no warm image, whose pre-push
-    // below lists every active tile, fresh ones too, the full relabel of
-    // these tiles comes next, in the host loop and in a device-driven run
-    // alike)
-    const bool relabel_next = S.bfs <= 1 && S.nsel == S.nfresh;
-    S.ops->push(p, S.nfresh, S.sweeps, relabel_next ? PUSH_FIRST_NO_H : PUSH_FIRST);
+    S.ops->push(p, S.nfresh, S.sweeps, first_relabel_next(S) ? PUSH_FIRST_NO_H : PUSH_FIRST);
     S.E();
     GCMI_HIP(hipGetLastError());
   }
@@ -2355,7 +2356,7 @@ static void finish_images(Pool* p, const int* ids, int n, int64_t* flows_out) {
   p->stats[STAT_RELABEL_TILES] = (int64_t)ht[TALLY_RELABEL_TILES];
   p->stats[STAT_BFS_ROUND_PASSES] = (int64_t)ht[TALLY_BFS];
   // algorithmic bytes: first round 16 per image pixel read + 25 per padded
-  // pixel written (state + mask); later residencies 49 (24 read, 25 written);
+  // pixel written (state + mask; 21 without its heights); later residencies 49 (24 read, 25 written);
   // relabel first pass 5 (mask read, h written); later BFS passes 9 (mask +
   // h read, h written); flood passes 2 (mask read, reach written); labels 2.
   int64_t img_px = 0, fresh_px = 0, nfresh = 0;
@@ -2368,7 +2369,7 @@ static void finish_images(Pool* p, const int* ids, int n, int64_t* flows_out) {
     }
   }
   const int64_t tp = (int64_t)T * T;
-  p->stats[STAT_ALG_BYTES] = fresh_px * 16 + nfresh * tp * 25 +
+  p->stats[STAT_ALG_BYTES] = fresh_px * 16 + nfresh * tp * (p->first_no_h ? 21 : 25) +
                              (p->stats[STAT_RESIDENCIES] - nfresh) * tp * 49 +
                              (int64_t)ht[TALLY_RELABEL_TILES] * tp * 5 +
                              (int64_t)ht[TALLY_BFS] * tp * 9 +
@@ -2428,6 +2429,8 @@ int pool_solve(Pool* p, const int* ids, int n, int sweeps, int bfs, int64_t max_
   // the relabel's first pass runs one block per listed tile (their number
   // is known on the device only; surplus blocks exit at once)
   Solve S{p, p->ops, n, nsel, nfresh, sweeps, bfs, std::min(nsel, kRelabelGridCap), false, P};
+  // (set here too for a replayed recording, whose host code does not run)
+  p->first_no_h = nfresh > 0 && first_relabel_next(S);
   const int32_t* hc = p->h_counts;
   int32_t nact = 0;
   int64_t rounds = 0;
@@ -2542,6 +2545,7 @@ static int step_begin(Pool* p, int img) {
   GCMI_TRY(phase_prepare(S));
   if (fresh) {
     // (a stepped solve may push again before it relabels)
+    p->first_no_h = false;
     S.ops->push(p, S.nfresh, 0, PUSH_FIRST);
     GCMI_HIP(hipGetLastError());
   }
