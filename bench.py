@@ -136,13 +136,15 @@ def time_cut(gc, steps, warmup, sweeps, bfs, profile=False):
     return dt / steps, totals, stats
 
 
-def class_bytes(stats, tile, W, H):
+def class_bytes(stats, tile, W, H, bfs=None):
     """Algorithmic bytes per run of each kernel class (from the run's
-    counters)."""
+    counters; round 1 stores its heights with more than one push round per
+    relabel)."""
     T2 = tile * tile
     ntiles = stats["tiles"]
+    first_written = 25 if (bfs or 1) > 1 else 21
     return {
-        "push_first": 16 * W * H + 21 * ntiles * T2,
+        "push_first": 16 * W * H + first_written * ntiles * T2,
         "push": ALG_BYTES_PER_PX["push"] * (stats["residencies"] - ntiles) * T2,
         "bfs_init": ALG_BYTES_PER_PX["bfs_init"] * stats["relabel_tiles"] * T2,
         "bfs_round": ALG_BYTES_PER_PX["bfs_round"] * stats["bfs_round_passes"] * T2,
@@ -151,10 +153,10 @@ def class_bytes(stats, tile, W, H):
     }
 
 
-def kernel_table(totals, stats, tile, W, H, steps):
+def kernel_table(totals, stats, tile, W, H, steps, bfs=None):
     """Per kernel class of a profiled (host-driven) run: time per step, mean
     launch time, algorithmic bytes per launch, GB/s and fraction of peak."""
-    per_run = class_bytes(stats, tile, W, H)
+    per_run = class_bytes(stats, tile, W, H, bfs)
     out = {}
     for k, (ms, launches) in totals.items():
         if not launches:
@@ -214,7 +216,7 @@ def tile_leg(args, dev, W, H, exp, sweeps, bfs):
         flow = g.flow
         bits = g.labels_packed()
         _, tot, pst = time_cut(g, 3, 0, sweeps, bfs, profile=True)
-        tab = kernel_table(tot, pst, t, W, H, 3)
+        tab = kernel_table(tot, pst, t, W, H, 3, bfs)
         out[str(t)] = {"ms": round(ts * 1e3, 4), "mpix_s": round(W * H / ts / 1e6, 1),
                        "whole_cut_gbs": round(st["alg_bytes"] / ts / 1e9, 1),
                        "push_first_gbs": tab.get("push_first", {}).get("achieved_gbs"),
@@ -638,7 +640,7 @@ def main():
     parity = {}
     if W == 4096 and H == 4096:
         parity["plain_4096"] = answer(flow, gc.labels_packed(), exp["plain_4096"])
-    table = kernel_table(totals, pstats, args.tile, W, H, args.steps)
+    table = kernel_table(totals, pstats, args.tile, W, H, args.steps, bfs)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_plain * 1e3, 4),
